@@ -1,0 +1,111 @@
+// eq_internal.h -- host-side batch state shared by the translation units of
+// libeq_field.so.  Not part of the ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/eq_field.h"
+#include "eq_common.h"
+
+namespace eqk {
+
+// tile table entry: problem (index into the level table) and tile inside it
+struct TileRef {
+    int32_t pi, local;
+};
+
+struct LevelView {
+    const LevelProb* probs;
+    const TileRef* tiles;
+    int32_t ntiles;
+};
+
+// per-problem Krylov scalars, device resident
+struct CgState {
+    double rz, pq, rr, bb, alpha, beta, rr_true, rzold;
+    double tol2;            // rtol^2 (times bb when compared)
+    int32_t iter, done, max_iter, flexible;
+    int32_t restart, pad_;  // restart: next beta is 0 (after a residual replacement)
+    uint32_t ticket[4];
+};
+
+struct Level {
+    std::vector<LevelProb> probs;     // host copy
+    std::vector<TileRef> tiles_all;   // every problem at this level
+    std::vector<TileRef> tiles_smooth;// problems that are not coarsest here
+    std::vector<int32_t> coarse;      // problems whose coarsest level is this one
+    LevelProb* d_probs = nullptr;
+    TileRef* d_tiles_all = nullptr;
+    TileRef* d_tiles_smooth = nullptr;
+    int32_t* d_coarse = nullptr;
+    int64_t pool = 0;                 // elements per vector on this level
+    double nodes_all = 0.0;           // grid nodes summed over the level's problems
+    double nodes_smooth = 0.0;        // ... over the problems that smooth here
+    int64_t max_dense = 0;            // largest coarse problem (nodes)
+    // multigrid vectors (element type = smoother precision)
+    void* b = nullptr;  // level >= 1, or level 0 FP32 copy of r
+    void* xa = nullptr;
+    void* xb = nullptr;
+    void* r = nullptr;
+    LevelView all() const { return LevelView{d_probs, d_tiles_all, (int32_t)tiles_all.size()}; }
+    LevelView smooth() const { return LevelView{d_probs, d_tiles_smooth, (int32_t)tiles_smooth.size()}; }
+};
+
+struct ProfileSlot {
+    bool enabled = false;
+    int64_t launches = 0;
+    double total_ms = 0.0;
+    double bytes = 0.0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    std::vector<double> pending_bytes;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;
+};
+
+} // namespace eqk
+
+struct eq_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int32_t nprob = 0, nspecies = 1, ntraps = 0;
+    std::vector<eq_field_desc> descs;
+    eq_solver_desc solver{};
+    int32_t mixed = 0;                 // 1: FP32 V-cycle
+    std::vector<eqk::Level> levels;
+    // level 0 FP64 Krylov vectors (pooled over problems)
+    double* u = nullptr;   // field / iterate x
+    double* rhs = nullptr;
+    double* r = nullptr;
+    double* p = nullptr;
+    double* q = nullptr;
+    double* z = nullptr;
+    double* partial = nullptr;  // 4 slots per level-0 tile
+    void* dense = nullptr;      // dense inverses of coarsest levels (smoother precision)
+    eqk::CgState* d_state = nullptr;
+    eqk::CgState* h_state = nullptr; // pinned mirror
+    int32_t last_iters = 0;
+    std::vector<int32_t> iters;
+    std::vector<double> relres;
+    // cell coupling scratch (grown on demand)
+    void* cell_scratch = nullptr;
+    size_t cell_scratch_bytes = 0;
+    double* flux_out = nullptr;      // device scalar
+    double* h_flux = nullptr;        // pinned
+    eqk::ProfileSlot prof[EQ_KERNEL_COUNT_];
+};
+
+namespace eqk {
+int set_error(int code, const std::string& msg);
+int hip_check(hipError_t e, const char* what);
+// event bracket for optional per-launch timing
+void prof_begin(eq_batch* b, int kernel);
+void prof_end(eq_batch* b, int kernel, double bytes);
+} // namespace eqk
+
+#define EQ_HIP(call)                                                  \
+    do {                                                              \
+        hipError_t e_ = (call);                                       \
+        if (e_ != hipSuccess) return eqk::hip_check(e_, #call);       \
+    } while (0)

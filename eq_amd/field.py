@@ -1,0 +1,290 @@
+"""Python binding of the C ABI in include/eq_field.h (libeq_field.so).
+
+This is the host-side mirror of eQ's ``eQ::diffusionSolver`` /
+``fenicsInterface`` surface (src/eQ.h:302-330, src/fHSL.h:333-449) for Python
+callers: ``FieldBatch`` holds one problem per (trap, species) and forwards
+every call to the HIP library.  There is no CPU fallback: constructing a
+batch without the built library or without a HIP device raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libeq_field.so")
+
+NEUMANN, DIRICHLET, ROBIN = 0, 1, 2
+LEFT, RIGHT, TOP, BOTTOM = 0, 1, 2, 3
+JACOBI_CG, GMG_CG, MIXED_GMG_CG = 0, 1, 2
+SAMPLE, DEPOSIT = 1, 2
+KERNEL_STENCIL64, KERNEL_SMOOTH, KERNEL_CG_UPDATE, KERNEL_COUPLE = 0, 1, 2, 3
+
+STATUS = {0: "EQ_OK", 1: "EQ_ERR_INVALID", 2: "EQ_ERR_HIP", 3: "EQ_ERR_NOMEM",
+          4: "EQ_ERR_NOT_CONVERGED", 5: "EQ_ERR_NO_DEVICE"}
+
+# every symbol include/eq_field.h declares
+EXPORTED = [
+    "eq_last_error", "eq_solver_default",
+    "eq_batch_create", "eq_batch_destroy", "eq_batch_num_problems", "eq_batch_num_nodes",
+    "eq_batch_stream", "eq_batch_sync", "eq_batch_upload", "eq_batch_download",
+    "eq_batch_upload_dev", "eq_batch_download_dev", "eq_batch_step", "eq_batch_problem_stats",
+    "eq_batch_couple", "eq_batch_couple_dev", "eq_batch_boundary_flux", "eq_batch_vertex_map",
+    "eq_field_create", "eq_field_destroy", "eq_field_upload", "eq_field_download",
+    "eq_field_step", "eq_field_vertex_map", "eq_field_boundary_flux", "eq_cells_couple",
+    "eq_batch_profile", "eq_batch_profile_read", "eq_batch_apply", "eq_batch_rhs",
+]
+
+
+class Bc(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("pad_", C.c_int32), ("value", C.c_double),
+                ("r", C.c_double), ("s", C.c_double)]
+
+
+class FieldDesc(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("width", C.c_double),
+                ("height", C.c_double), ("dt", C.c_double), ("D", C.c_double),
+                ("gamma", C.c_double), ("f", C.c_double), ("wall", Bc * 4)]
+
+
+class SolverDesc(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("max_iter", C.c_int32), ("rtol", C.c_double),
+                ("mg_pre", C.c_int32), ("mg_post", C.c_int32), ("mg_omega", C.c_double),
+                ("mg_coarse_max_nodes", C.c_int32), ("mg_max_levels", C.c_int32),
+                ("check_every", C.c_int32), ("use_graphs", C.c_int32)]
+
+
+class StepStats(C.Structure):
+    _fields_ = [("iterations_max", C.c_int32), ("nconverged", C.c_int32),
+                ("relres_max", C.c_double), ("seconds", C.c_double)]
+
+
+class Cell(C.Structure):
+    _fields_ = [("x", C.c_double), ("y", C.c_double), ("angle", C.c_double),
+                ("length", C.c_double), ("width", C.c_double)]
+
+
+class EqError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load_library():
+    """Load libeq_field.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: run `python -m eq_amd.build` (the HIP path "
+                           "has no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    vp, dp, ip32, ip64 = C.c_void_p, P(C.c_double), P(C.c_int32), P(C.c_int64)
+    L.eq_last_error.restype = C.c_char_p
+    L.eq_solver_default.argtypes = [P(SolverDesc)]
+    L.eq_batch_create.argtypes = [P(FieldDesc), C.c_int32, C.c_int32, P(SolverDesc), P(vp)]
+    L.eq_batch_destroy.argtypes = [vp]
+    L.eq_batch_num_problems.argtypes = [vp]
+    L.eq_batch_num_problems.restype = C.c_int32
+    L.eq_batch_num_nodes.argtypes = [vp, C.c_int32]
+    L.eq_batch_num_nodes.restype = C.c_int64
+    L.eq_batch_stream.argtypes = [vp]
+    L.eq_batch_stream.restype = vp
+    L.eq_batch_sync.argtypes = [vp]
+    for f in ("eq_batch_upload", "eq_batch_download", "eq_batch_upload_dev", "eq_batch_download_dev"):
+        getattr(L, f).argtypes = [vp, C.c_int32, vp, C.c_size_t]
+    L.eq_batch_step.argtypes = [vp, P(StepStats)]
+    L.eq_batch_problem_stats.argtypes = [vp, ip32, dp]
+    L.eq_batch_couple.argtypes = [vp, C.c_int32, vp, C.c_size_t, C.c_double, vp, vp, vp, C.c_uint32]
+    L.eq_batch_couple_dev.argtypes = [vp, C.c_int32, vp, C.c_size_t, C.c_double, vp, vp, vp,
+                                      C.c_uint32]
+    L.eq_batch_boundary_flux.argtypes = [vp, C.c_int32, dp]
+    L.eq_batch_vertex_map.argtypes = [vp, C.c_int32, dp, ip32]
+    L.eq_batch_profile.argtypes = [vp, C.c_int32, C.c_int32]
+    L.eq_batch_profile_read.argtypes = [vp, C.c_int32, ip64, dp, dp]
+    L.eq_batch_apply.argtypes = [vp, C.c_int32, vp, vp, C.c_size_t]
+    L.eq_batch_rhs.argtypes = [vp, C.c_int32, vp, vp, C.c_size_t]
+    L.eq_field_create.argtypes = [P(FieldDesc), P(SolverDesc), P(vp)]
+    L.eq_field_destroy.argtypes = [vp]
+    for f in ("eq_field_upload", "eq_field_download"):
+        getattr(L, f).argtypes = [vp, vp, C.c_size_t]
+    L.eq_field_step.argtypes = [vp, P(StepStats)]
+    L.eq_field_vertex_map.argtypes = [vp, dp, ip32]
+    L.eq_field_boundary_flux.argtypes = [vp, dp]
+    L.eq_cells_couple.argtypes = [vp, vp, C.c_size_t, C.c_double, vp, vp, vp, C.c_uint32]
+    _lib = L
+    return L
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise EqError(rc, load_library().eq_last_error().decode())
+
+
+def field_desc(nx, ny, width, height, dt, D, gamma=0.0, f=0.0, walls=None) -> FieldDesc:
+    """walls: {wall: (kind, value, r, s)}; missing walls are Neumann."""
+    d = FieldDesc()
+    d.nx, d.ny, d.width, d.height = nx, ny, width, height
+    d.dt, d.D, d.gamma, d.f = dt, D, gamma, f
+    for w in range(4):
+        kind, value, r, s = (walls or {}).get(w, (NEUMANN, 0.0, 0.0, 0.0))
+        d.wall[w].kind, d.wall[w].value, d.wall[w].r, d.wall[w].s = kind, value, r, s
+    return d
+
+
+def solver_desc(kind=MIXED_GMG_CG, **kw) -> SolverDesc:
+    s = SolverDesc()
+    load_library().eq_solver_default(C.byref(s))
+    s.kind = kind
+    for k, v in kw.items():
+        setattr(s, k, v)
+    return s
+
+
+@dataclass
+class Stats:
+    iterations_max: int
+    nconverged: int
+    relres_max: float
+    seconds: float
+
+
+def _ptr(a: np.ndarray):
+    return C.c_void_p(a.ctypes.data)
+
+
+class FieldBatch:
+    """Problems (trap x species) advanced together; problem p = trap*nspecies + species."""
+
+    def __init__(self, descs, nspecies: int = 1, solver: SolverDesc | None = None):
+        L = load_library()
+        arr = (FieldDesc * len(descs))(*descs)
+        h = C.c_void_p()
+        _check(L.eq_batch_create(arr, len(descs), nspecies, C.byref(solver) if solver else None,
+                                 C.byref(h)))
+        self._h = h
+        self.nprob = len(descs)
+        self.nspecies = nspecies
+        self.descs = list(descs)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().eq_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def num_nodes(self, prob: int) -> int:
+        return int(load_library().eq_batch_num_nodes(self._h, prob))
+
+    def stream(self) -> int:
+        return int(load_library().eq_batch_stream(self._h) or 0)
+
+    def sync(self):
+        _check(load_library().eq_batch_sync(self._h))
+
+    def upload(self, prob: int, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        _check(load_library().eq_batch_upload(self._h, prob, _ptr(u), u.size))
+
+    def download(self, prob: int) -> np.ndarray:
+        u = np.empty(self.num_nodes(prob))
+        _check(load_library().eq_batch_download(self._h, prob, _ptr(u), u.size))
+        return u
+
+    def upload_dev(self, prob: int, dev_ptr: int, n: int):
+        _check(load_library().eq_batch_upload_dev(self._h, prob, C.c_void_p(dev_ptr), n))
+
+    def download_dev(self, prob: int, dev_ptr: int, n: int):
+        _check(load_library().eq_batch_download_dev(self._h, prob, C.c_void_p(dev_ptr), n))
+
+    def step(self, check: bool = True) -> Stats:
+        st = StepStats()
+        rc = load_library().eq_batch_step(self._h, C.byref(st))
+        if check:
+            _check(rc)
+        return Stats(st.iterations_max, st.nconverged, st.relres_max, st.seconds)
+
+    def problem_stats(self):
+        it = np.empty(self.nprob, np.int32)
+        rr = np.empty(self.nprob)
+        _check(load_library().eq_batch_problem_stats(
+            self._h, it.ctypes.data_as(C.POINTER(C.c_int32)), rr.ctypes.data_as(C.POINTER(C.c_double))))
+        return it, rr
+
+    def couple(self, trap: int, cells: np.ndarray, npm: float, q: np.ndarray,
+               flags: int = SAMPLE | DEPOSIT):
+        """cells: (nc,5) x,y,angle,length,width; q: (nc, nspecies). Returns (sampled, counts)."""
+        cells = np.ascontiguousarray(cells, dtype=np.float64).reshape(-1, 5)
+        nc = cells.shape[0]
+        q = np.ascontiguousarray(q, dtype=np.float64).reshape(nc, self.nspecies)
+        sampled = np.empty((nc, self.nspecies))
+        counts = np.empty(nc, np.int32)
+        _check(load_library().eq_batch_couple(self._h, trap, _ptr(cells), nc, npm, _ptr(q),
+                                              _ptr(sampled), _ptr(counts), flags))
+        return sampled, counts
+
+    def couple_dev(self, trap: int, cells_ptr: int, nc: int, npm: float, q_ptr: int,
+                   sampled_ptr: int = 0, counts_ptr: int = 0, flags: int = SAMPLE | DEPOSIT):
+        _check(load_library().eq_batch_couple_dev(
+            self._h, trap, C.c_void_p(cells_ptr), nc, npm, C.c_void_p(q_ptr),
+            C.c_void_p(sampled_ptr) if sampled_ptr else None,
+            C.c_void_p(counts_ptr) if counts_ptr else None, flags))
+
+    def boundary_flux(self, prob: int) -> float:
+        v = C.c_double()
+        _check(load_library().eq_batch_boundary_flux(self._h, prob, C.byref(v)))
+        return v.value
+
+    def vertex_map(self, prob: int):
+        n = self.num_nodes(prob)
+        xy = np.empty(2 * n)
+        dof = np.empty(n, np.int32)
+        _check(load_library().eq_batch_vertex_map(self._h, prob, xy.ctypes.data_as(C.POINTER(C.c_double)),
+                                                  dof.ctypes.data_as(C.POINTER(C.c_int32))))
+        return xy.reshape(-1, 2), dof
+
+    def apply(self, prob: int, x) -> np.ndarray:
+        """y = A x of the Dirichlet-row-replaced operator (natural order)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(x)
+        _check(load_library().eq_batch_apply(self._h, prob, _ptr(x), _ptr(y), x.size))
+        return y
+
+    def rhs(self, prob: int, u0) -> np.ndarray:
+        """b = M u0 + sources with Dirichlet rows set to the wall value."""
+        u0 = np.ascontiguousarray(u0, dtype=np.float64)
+        y = np.empty_like(u0)
+        _check(load_library().eq_batch_rhs(self._h, prob, _ptr(u0), _ptr(y), u0.size))
+        return y
+
+    def profile(self, kernel: int, enable: bool = True):
+        _check(load_library().eq_batch_profile(self._h, kernel, 1 if enable else 0))
+
+    def profile_read(self, kernel: int):
+        n = C.c_int64()
+        ms = C.c_double()
+        by = C.c_double()
+        _check(load_library().eq_batch_profile_read(self._h, kernel, C.byref(n), C.byref(ms), C.byref(by)))
+        return n.value, ms.value, by.value

@@ -1,0 +1,197 @@
+/*
+ * eq_field.h -- C ABI of the MI355X quorum-sensing field step.
+ *
+ * One "problem" is one HSL species on one trap: the unit that eQ's
+ * fenicsInterface solves on its own MPI rank (src/simulation.cpp:38-41,
+ * src/fHSL.cpp:98-161).  A batch holds many problems grouped by trap
+ * (problem p = trap * nspecies + species) and advances all of them with one
+ * launch per kernel.  A single field (eq_field_*) is a batch of one.
+ *
+ * Conventions
+ *   - Status: every entry point returns 0 (EQ_OK) or a positive eq_status;
+ *     nothing throws across the ABI; eq_last_error() describes the last
+ *     failure of the calling thread.
+ *   - Host pointers are caller-owned and are read or written before return.
+ *     *_dev entry points take device pointers and are ordered on the batch's
+ *     HIP stream (eq_batch_stream); eq_batch_sync waits for it.
+ *   - Node order is natural: node (i,j), row i = y, column j = x, index
+ *     j + i*(nx+1) (src/eQ.h:131-134).  eq_*_vertex_map returns the identity
+ *     dof_from_vertex, so the host's (i,j)->dof lookup tables stay valid.
+ *   - A handle is not thread-safe; distinct handles may be used concurrently.
+ */
+#ifndef EQ_FIELD_H
+#define EQ_FIELD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    EQ_OK = 0,
+    EQ_ERR_INVALID = 1,       /* bad argument or descriptor            */
+    EQ_ERR_HIP = 2,           /* HIP runtime failure                   */
+    EQ_ERR_NOMEM = 3,         /* device or host allocation failed      */
+    EQ_ERR_NOT_CONVERGED = 4, /* a solve hit max_iter above rtol       */
+    EQ_ERR_NO_DEVICE = 5      /* no HIP device visible                 */
+} eq_status;
+
+/* wall order of eQ::boundaryCondition::wall (src/eQ.h:335-340) */
+typedef enum { EQ_WALL_LEFT = 0, EQ_WALL_RIGHT = 1, EQ_WALL_TOP = 2, EQ_WALL_BOTTOM = 3 } eq_wall;
+
+/* Per-wall condition, decoded as fenicsInterface::createHSL does
+ * (src/fHSL.cpp:467-539): Dirichlet u = value; Neumann adds nothing;
+ * Robin adds dt*r*(u - s) on the wall. */
+typedef enum { EQ_BC_NEUMANN = 0, EQ_BC_DIRICHLET = 1, EQ_BC_ROBIN = 2 } eq_bc_kind;
+
+typedef struct {
+    int32_t kind;
+    int32_t pad_;
+    double value; /* Dirichlet value                 */
+    double r;     /* Robin rate                      */
+    double s;     /* Robin exterior concentration    */
+} eq_bc;
+
+/* One species on one trap: the trap mesh of fenicsInterface::createMesh
+ * (RectangleMesh (0,0)-(width,height), nx x ny cells, "right" diagonal,
+ * src/fHSL.cpp:164-173) and the hslD form (fenics/hslD.ufl:31-49) with the
+ * reaction term added:  (M + dt*D*K + dt*gamma*M) u+ = M u + dt*f*int(phi). */
+typedef struct {
+    int32_t nx, ny;
+    double width, height; /* microns */
+    double dt, D, gamma, f;
+    eq_bc wall[4];
+} eq_field_desc;
+
+typedef enum {
+    EQ_SOLVER_JACOBI_CG = 0,    /* FP64 CG, Jacobi preconditioner                    */
+    EQ_SOLVER_GMG_CG = 1,       /* FP64 CG, FP64 geometric-multigrid V-cycle         */
+    EQ_SOLVER_MIXED_GMG_CG = 2  /* FP64 CG, FP32 V-cycle (FP64 residual/refinement)  */
+} eq_solver_kind;
+
+typedef struct {
+    int32_t kind;                /* eq_solver_kind                                  */
+    int32_t max_iter;            /* per solve                                       */
+    double rtol;                 /* stop at ||b - A u+||_2 <= rtol * ||b||_2 (FP64) */
+    int32_t mg_pre, mg_post;     /* damped-Jacobi sweeps per level                  */
+    double mg_omega;             /* Jacobi damping                                  */
+    int32_t mg_coarse_max_nodes; /* coarsen until a level has at most this many    */
+    int32_t mg_max_levels;
+    int32_t check_every;         /* iterations between host convergence checks      */
+    int32_t use_graphs;          /* capture iteration chunks in hipGraphs (0/1)     */
+} eq_solver_desc;
+
+typedef struct {
+    int32_t iterations_max;   /* over problems, last step                        */
+    int32_t nconverged;       /* problems at or below rtol                       */
+    double relres_max;        /* max over problems of ||b-Au||/||b||             */
+    double seconds;           /* host wall time of the call                      */
+} eq_step_stats;
+
+/* A rod cell as the ABM sees it: centre, angle, current length and width in
+ * microns (Ecoli::getCenter_x/y, getAngle, getLengthMicrons, getWidthMicrons,
+ * src/abm/Ecoli.h:39-43).  Array order = the host's list iteration order. */
+typedef struct {
+    double x, y, angle, length, width;
+} eq_cell;
+
+/* coupling flags */
+#define EQ_COUPLE_SAMPLE 1u  /* write per-cell means into `sampled`  */
+#define EQ_COUPLE_DEPOSIT 2u /* add q_c/|S_c| to every covered node   */
+
+typedef struct eq_batch eq_batch;
+typedef struct eq_batch eq_field;
+
+const char* eq_last_error(void);
+/* fills the defaults: mixed GMG-CG, rtol 1e-10, V(2,2), omega 0.8, coarse <= 256 nodes */
+void eq_solver_default(eq_solver_desc* s);
+
+/* ---------------- batch API ---------------- */
+/* Replaces one fenicsInterface per species and rank (fenicsInterface::initDiffusion,
+ * src/fHSL.cpp:37-53 -> fenicsClassInit 195-328 -> createHSL 367-599).
+ * descs has nprob entries, problems t*nspecies .. t*nspecies+nspecies-1 share
+ * trap t (same nx, ny, width, height). */
+int eq_batch_create(const eq_field_desc* descs, int32_t nprob, int32_t nspecies,
+                    const eq_solver_desc* solver, eq_batch** out);
+void eq_batch_destroy(eq_batch* b);
+int32_t eq_batch_num_problems(const eq_batch* b);
+int64_t eq_batch_num_nodes(const eq_batch* b, int32_t prob);
+void* eq_batch_stream(const eq_batch* b); /* hipStream_t */
+int eq_batch_sync(eq_batch* b);
+
+/* u0->vector()->set_local(solution_vector)  (src/fHSL.cpp:104) */
+int eq_batch_upload(eq_batch* b, int32_t prob, const double* u, size_t n);
+/* u->vector()->get_local(solution_vector)   (src/fHSL.cpp:108) */
+int eq_batch_download(eq_batch* b, int32_t prob, double* u, size_t n);
+/* device-pointer variants (natural order, stream ordered) */
+int eq_batch_upload_dev(eq_batch* b, int32_t prob, const double* d_u, size_t n);
+int eq_batch_download_dev(eq_batch* b, int32_t prob, double* d_u, size_t n);
+
+/* LVS->solve() for every problem  (src/fHSL.cpp:105-106): rebuild b = M u
+ * (+ sources, Robin), apply the Dirichlet walls, solve to rtol. */
+int eq_batch_step(eq_batch* b, eq_step_stats* stats);
+/* per-problem iterations and final relative residual of the last step */
+int eq_batch_problem_stats(const eq_batch* b, int32_t* iterations, double* relres);
+
+/* updateCells' coverage/read/write loop (src/abm/eQabm.cpp:234-449) for one
+ * trap and all of its species.  npm = nodesPerMicronSignaling.  q is
+ * nc x nspecies, sampled (may be NULL) nc x nspecies, counts (may be NULL) nc.
+ * Per cell in array order: sample every species, then deposit q/|S_c|. */
+int eq_batch_couple(eq_batch* b, int32_t trap, const eq_cell* cells, size_t nc, double npm,
+                    const double* q, double* sampled, int32_t* counts, uint32_t flags);
+/* device-pointer variant: d_cells/d_q/d_sampled as above, on the batch stream */
+int eq_batch_couple_dev(eq_batch* b, int32_t trap, const eq_cell* d_cells, size_t nc, double npm,
+                        const double* d_q, double* d_sampled, int32_t* d_counts, uint32_t flags);
+
+/* totalBoundaryFlux = D*dt * (-int grad(u).n ds)  (src/fHSL.cpp:156-160,
+ * fenics/boundary.ufl:9-12) of the current field */
+int eq_batch_boundary_flux(eq_batch* b, int32_t prob, double* total);
+/* shell->mesh_coords (2N, vertex order) and shell->dof_from_vertex (N)
+ * (src/fHSL.h:82-88); vertex order = natural order, dof_from_vertex = identity */
+int eq_batch_vertex_map(const eq_batch* b, int32_t prob, double* coords, int32_t* dof_from_vertex);
+
+/* Matrix-free counterparts of the reference's assembled system, for checking:
+ *   eq_batch_apply: y = A x with A = (1+dt*gamma)M + dt*D*K + Robin after
+ *     DirichletBC::apply (Dirichlet rows are identity rows), i.e. the matrix
+ *     LinearVariationalSolver factorises (assemble(A,a) + bc.apply(A),
+ *     src/fHSL.h:156-160 [ext]);
+ *   eq_batch_rhs: b = M u0 + dt*f*int(phi) + Robin, Dirichlet rows set to the
+ *     wall value (assemble(b,L) + bc.apply(b)).
+ * Natural order, host pointers, length (nx+1)*(ny+1).  Uses the batch's CG
+ * scratch vectors: call between steps only. */
+int eq_batch_apply(eq_batch* b, int32_t prob, const double* x, double* y, size_t n);
+int eq_batch_rhs(eq_batch* b, int32_t prob, const double* u0, double* rhs, size_t n);
+
+/* ---------------- single-field API (batch of one) ---------------- */
+int eq_field_create(const eq_field_desc* desc, const eq_solver_desc* solver, eq_field** out);
+void eq_field_destroy(eq_field* f);
+int eq_field_upload(eq_field* f, const double* u, size_t n);
+int eq_field_download(eq_field* f, double* u, size_t n);
+int eq_field_step(eq_field* f, eq_step_stats* stats);
+int eq_field_vertex_map(const eq_field* f, double* coords, int32_t* dof_from_vertex);
+int eq_field_boundary_flux(eq_field* f, double* total);
+int eq_cells_couple(eq_field* f, const eq_cell* cells, size_t nc, double npm, const double* q,
+                    double* sampled, int32_t* counts, uint32_t flags);
+
+/* ---------------- measurement ---------------- */
+/* Kernel ids for per-launch HIP-event timing on the batch stream. */
+typedef enum {
+    EQ_KERNEL_STENCIL64 = 0,  /* FP64 CG operator apply q = A p          */
+    EQ_KERNEL_SMOOTH = 1,     /* fine-level Jacobi sweep of the V-cycle   */
+    EQ_KERNEL_CG_UPDATE = 2,  /* FP64 x,r update + dot products           */
+    EQ_KERNEL_COUPLE = 3,     /* cell coverage/sample/deposit row pass    */
+    EQ_KERNEL_COUNT_ = 4
+} eq_kernel_id;
+/* enable (1) / disable (0) event timing around every launch of `kernel`;
+ * timing forces eager launches (no graph replay) while enabled */
+int eq_batch_profile(eq_batch* b, int32_t kernel, int32_t enable);
+/* launches, summed event time (ms) and algorithmic bytes summed over launches */
+int eq_batch_profile_read(eq_batch* b, int32_t kernel, int64_t* launches, double* total_ms,
+                          double* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EQ_FIELD_H */

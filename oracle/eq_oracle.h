@@ -1,0 +1,139 @@
+/*
+ * eq_oracle.h -- CPU FP64 restatement of eQ's quorum-sensing field step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the checker
+ * (or as the timed CPU baseline).  The product path (eq_amd/, libeq_field.so)
+ * never links, imports or calls it.
+ *
+ * What it restates (reference paths are relative to the eQ source tree):
+ *   - mesh:        DOLFIN RectangleMesh(p0,p1,nx,ny,"right") as built by
+ *                  fenicsInterface::createMesh            src/fHSL.cpp:164-173
+ *   - operator:    FFC element tensors of the hslD form    fenics/hslD.ufl:31-49
+ *                  bilinear cell kernel                     fenics/hslD.h:3123-3259
+ *                  bilinear Robin facets                    fenics/hslD.h:3284-3441
+ *                  linear cell kernel                       fenics/hslD.h:3466-3529
+ *                  linear Robin facets                      fenics/hslD.h:3554-3689
+ *                  coefficient order                        fenics/hslD.h:4675-4730, 4968-5020
+ *                  plus the reaction term dt*gamma*M (not in hslD.ufl; gamma=0
+ *                  reproduces the reference form exactly)
+ *   - boundaries:  per-wall decode of MICROFLUIDIC_TRAP     src/fHSL.cpp:467-539
+ *                  DirichletBC::apply row replacement (DOLFIN 2019.1, [ext])
+ *   - solve:       LinearVariationalSolver default LU       src/fHSL.h:156-160 ([ext])
+ *   - cells:       eQabm::updateCells                       src/abm/eQabm.cpp:234-449
+ *                  Ecoli::updatePoleCenters                 src/abm/Ecoli.cpp:36-63
+ *                  cpmEcoli::pointIsInCell                  src/abm/cpmEcoli.cpp:313-328
+ *                  box vertices / edges                     src/abm/cpmEcoli.cpp:71-81,102-124
+ *                  index helpers                            src/eQ.h:115-134
+ *
+ * Natural node order everywhere: node (i,j), row i = y, column j = x,
+ * index j + i*(nx+1)  (src/eQ.h:131-134).
+ */
+#ifndef EQ_ORACLE_H
+#define EQ_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* wall order follows eQ::boundaryCondition::wall (src/eQ.h:335-340) */
+enum { ORC_LEFT = 0, ORC_RIGHT = 1, ORC_TOP = 2, ORC_BOTTOM = 3 };
+/* kinds: Neumann (natural, r = 0), Dirichlet (u = value), Robin (rate r, exterior value s) */
+enum { ORC_NEUMANN = 0, ORC_DIRICHLET = 1, ORC_ROBIN = 2 };
+
+typedef struct {
+    int32_t kind;
+    int32_t pad_;
+    double value; /* Dirichlet value */
+    double r;     /* Robin rate      */
+    double s;     /* Robin exterior concentration */
+} orc_bc;
+
+typedef struct {
+    int32_t nx, ny;         /* cells in x and y                      */
+    double width, height;   /* trap size in microns                  */
+    double dt, D, gamma, f; /* step, diffusivity, degradation, source */
+    orc_bc wall[4];
+} orc_problem;
+
+typedef struct {
+    double x, y, angle, length, width;
+} orc_cell;
+
+typedef struct {
+    int32_t nodes_w, nodes_h; /* grid nodes (nx+1, ny+1)                 */
+    double npm;               /* nodes per micron used by the ABM maps    */
+    double trap_w, trap_h;    /* trap size in microns (pole clamping)     */
+} orc_grid;
+
+typedef struct {
+    int32_t iterations;
+    int32_t pad_;
+    double relres;      /* ||b - A x|| / ||b|| on the row-replaced system */
+    double bnorm;
+} orc_solve_info;
+
+/* ---- mesh ---- */
+int64_t orc_num_nodes(const orc_problem* p);
+void orc_mesh_coords(const orc_problem* p, double* xy);
+
+/* ---- assembly ---- */
+/* Assemble A = (1+dt*gamma)*M + dt*D*K(Dtensor) + Robin, no Dirichlet rows yet.
+ * D11/D22/D12 are per-node tensor entries in natural order; NULL means the
+ * isotropic default (1,1,0) of fenicsClassInit (src/fHSL.cpp:313-323).
+ * CSR with sorted columns; rowptr has N+1 entries; returns nnz.  col/val must
+ * hold 7*N entries. */
+int64_t orc_assemble(const orc_problem* p, const double* D11, const double* D22,
+                     const double* D12, int64_t* rowptr, int32_t* col, double* val);
+/* b = M*u0 + dt*f*int(phi) + Robin dt*r*s*int(phi) on the marked walls */
+void orc_assemble_rhs(const orc_problem* p, const double* u0, double* b);
+/* mask[n] = 1 for Dirichlet nodes, g[n] their value */
+int64_t orc_dirichlet_nodes(const orc_problem* p, uint8_t* mask, double* g);
+/* DirichletBC::apply: zero the row, 1 on the diagonal, b_i = g_i */
+void orc_apply_dirichlet(const orc_problem* p, const int64_t* rowptr, const int32_t* col,
+                         double* val, double* b);
+/* y = A x */
+void orc_spmv(int64_t n, const int64_t* rowptr, const int32_t* col, const double* val,
+              const double* x, double* y);
+
+/* ---- solves ---- */
+/* Direct banded LU (no pivoting; leading minors are non-singular because
+ * Dirichlet rows are unit rows and the free block is SPD) of the
+ * row-replaced system.  Returns 0 on success. */
+int orc_solve_banded(const orc_problem* p, const int64_t* rowptr, const int32_t* col,
+                     const double* val, const double* b, double* x);
+/* Jacobi-PCG on the symmetrically eliminated system (same solution).  x holds
+ * the initial guess.  Stops at ||b-Ax||/||b|| <= rtol. */
+int orc_solve_pcg(const orc_problem* p, const int64_t* rowptr, const int32_t* col,
+                  const double* val, const double* b, double* x, double rtol, int32_t maxit,
+                  orc_solve_info* info);
+/* One reference step: reassemble A and b from u (in/out), apply Dirichlet,
+ * solve (method 0 = banded LU, 1 = PCG to rtol or maxit iterations), write u+ into u. */
+int orc_step(const orc_problem* p, const double* D11, const double* D22, const double* D12,
+             double* u, int32_t method, double rtol, int32_t maxit, orc_solve_info* info);
+
+/* ---- cells ---- */
+/* Covered-node set S_c of one cell, in the reference scan order (row-major
+ * over the search window).  Writes up to cap (i,j) pairs; returns |S_c|
+ * (including the centre-node fallback when the scan finds nothing). */
+int64_t orc_cell_coverage(const orc_grid* g, const orc_cell* c, int32_t* ij, int64_t cap);
+/* Search window [i1,i2]x[j1,j2] of one cell (eQabm.cpp:274-287). */
+void orc_cell_window(const orc_grid* g, const orc_cell* c, int32_t* win4);
+/* Sequential read-then-write loop of updateCells over cells in list order for
+ * nspecies fields (natural order, length nodes_w*nodes_h each).  q is
+ * nc x nspecies (amount per cell, divided over |S_c|), sampled (may be NULL)
+ * receives the per-cell mean read before the cell's own write.  counts (may
+ * be NULL) receives |S_c|. */
+void orc_couple(const orc_grid* g, const orc_cell* cells, int64_t nc, int32_t nspecies,
+                double* const* fields, const double* q, double* sampled, int32_t* counts);
+
+/* boundary-flux functional of stepDiffusion (fenics/boundary.ufl:9-12):
+ * -int_{boundary} grad(u).n ds over all walls, P1 gradient per boundary triangle */
+double orc_boundary_flux(const orc_problem* p, const double* u);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,224 @@
+"""GPU parity of the HIP field step against the CPU oracle and the textbook
+golden fixtures.  Every call goes through the C ABI (libeq_field.so).
+
+Tolerances:
+  operator and right-hand side vs the oracle's assembled CSR: 1e-14 relative
+    to the row scale (sum |A_ij x_j|);
+  solves vs the oracle's direct LU and vs the fixtures: 1e-9 relative L2 per
+    species per step (the solver stops at ||b-Ax||/||b|| <= 1e-10);
+  coverage counts: exact; deposited fields: bit-identical (same per-node
+    addition order as the sequential loop); sampled means: 1e-14 relative.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "p1_tiny.npz")
+SOLVERS = ["jacobi", "gmg", "mixed"]
+
+
+def _kind(field, name):
+    return {"jacobi": field.JACOBI_CG, "gmg": field.GMG_CG, "mixed": field.MIXED_GMG_CG}[name]
+
+
+def _walls_from(arr):
+    return {w: (int(arr[w][0]), arr[w][1], arr[w][2], arr[w][3]) for w in range(4) if arr[w][0] != 0}
+
+
+def _pair(eqf, oracle, nx, ny, W, H, dt, D, gamma=0.0, f=0.0, walls=None):
+    return (eqf.field_desc(nx, ny, W, H, dt, D, gamma, f, walls),
+            oracle.problem(nx, ny, W, H, dt, D, gamma, f, walls=walls))
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def _iso_cases():
+    z = np.load(GOLD)
+    return [str(n) for n in z["names"] if not np.any(z[f"{n}/tensor"][:, 2])]
+
+
+@pytest.mark.parametrize("name", _iso_cases())
+def test_operator_entries_match_fixture(eqf, oracle, name):
+    """Columns of the matrix-free operator = the textbook-assembled matrix."""
+    z = np.load(GOLD)
+    nx, ny, W, H, dt, D, gamma, f = z[f"{name}/params"]
+    d = eqf.field_desc(int(nx), int(ny), W, H, dt, D, gamma, f, _walls_from(z[f"{name}/walls"]))
+    A_ref = z[f"{name}/A_bc"]
+    n = A_ref.shape[0]
+    with eqf.FieldBatch([d], solver=eqf.solver_desc(eqf.JACOBI_CG)) as fb:
+        A = np.stack([fb.apply(0, np.eye(n)[k]) for k in range(n)], axis=1)
+        assert np.abs(A - A_ref).max() <= 1e-14 * np.abs(A_ref).max()
+        b = fb.rhs(0, z[f"{name}/u0"])
+        assert np.abs(b - z[f"{name}/b_bc"]).max() <= 1e-14 * max(1.0, np.abs(b).max())
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (3, 2), (64, 4), (130, 67), (160, 80)])
+def test_operator_matches_oracle_csr(eqf, oracle, shape):
+    nx, ny = shape
+    walls = {oracle.TOP: (oracle.DIRICHLET, 0.3, 0, 0), oracle.LEFT: (oracle.ROBIN, 0, 1.7, 0.4)}
+    d, p = _pair(eqf, oracle, nx, ny, nx * 0.25, ny * 0.3, 0.1, 400.0, 0.01, 0.2, walls)
+    rp, c, v = oracle.assemble(p)
+    x = np.random.default_rng(nx * 1000 + ny).random(oracle.num_nodes(p))
+    u0 = np.random.default_rng(ny).random(oracle.num_nodes(p))
+    b_ref = oracle.assemble_rhs(p, u0)
+    v2, b2 = oracle.apply_dirichlet(p, rp, c, v, b_ref)
+    y_ref = oracle.spmv(rp, c, v2, x)
+    scale = oracle.spmv(rp, c, np.abs(v2), np.abs(x))
+    with eqf.FieldBatch([d], solver=eqf.solver_desc(eqf.JACOBI_CG)) as fb:
+        y = fb.apply(0, x)
+        assert np.all(np.abs(y - y_ref) <= 1e-14 * scale + 1e-300)
+        b = fb.rhs(0, u0)
+        bscale = oracle.assemble_rhs(p, np.abs(u0)) + 1.0
+        assert np.all(np.abs(b - b2) <= 1e-14 * bscale)
+
+
+@pytest.mark.parametrize("solver", SOLVERS)
+@pytest.mark.parametrize("name", _iso_cases())
+def test_solve_matches_fixture(eqf, oracle, solver, name):
+    z = np.load(GOLD)
+    nx, ny, W, H, dt, D, gamma, f = z[f"{name}/params"]
+    d = eqf.field_desc(int(nx), int(ny), W, H, dt, D, gamma, f, _walls_from(z[f"{name}/walls"]))
+    with eqf.FieldBatch([d], solver=eqf.solver_desc(_kind(eqf, solver))) as fb:
+        fb.upload(0, z[f"{name}/u0"])
+        st = fb.step()
+        x = fb.download(0)
+    assert st.relres_max <= 1e-10
+    assert _rel(x, z[f"{name}/x"]) <= 1e-9
+
+
+@pytest.mark.parametrize("solver", SOLVERS)
+def test_config0_steps_match_oracle_lu(eqf, oracle, solver):
+    """The reference CPU configuration: 160x80 trap, 2 species, 200 rods;
+    couple + solve for 4 steps, each step against the oracle's direct LU."""
+    from eq_amd import workloads as Wl
+    t = Wl.config0()
+    walls = Wl.TOP_DIRICHLET
+    descs = [eqf.field_desc(t.nx, t.ny, t.width, t.height, 0.1, D, g, 0.0, walls) for D, g in zip(t.D, t.gamma)]
+    probs = [oracle.problem(t.nx, t.ny, t.width, t.height, 0.1, D, g, 0.0, walls=walls) for D, g in zip(t.D, t.gamma)]
+    grid = oracle.grid(t.nx + 1, t.ny + 1, t.npm, t.width, t.height)
+    ref = [u.copy() for u in t.u0]
+    q = t.rates * 0.1
+    with eqf.FieldBatch(descs, nspecies=2, solver=eqf.solver_desc(_kind(eqf, solver))) as fb:
+        for s in range(2):
+            fb.upload(s, ref[s])
+        for step in range(4):
+            sampled, counts = fb.couple(0, t.cells, t.npm, q)
+            ref, s_ref, c_ref = oracle.couple(grid, t.cells, ref, q)
+            assert np.array_equal(counts, c_ref)
+            assert np.allclose(sampled, s_ref, rtol=1e-13, atol=1e-300)
+            st = fb.step()
+            assert st.relres_max <= 1e-10
+            for s in range(2):
+                ref[s], info = oracle.step(probs[s], ref[s], "lu")
+                got = fb.download(s)
+                assert _rel(got, ref[s]) <= 1e-9, (step, s, _rel(got, ref[s]))
+                fb.upload(s, ref[s])  # continue both from the same state
+
+
+def test_couple_bit_exact_against_sequential_loop(eqf, oracle):
+    from eq_amd import workloads as Wl
+    rng = Wl.SplitMix64(11)
+    nx, ny, npm, W, H = 400, 200, 20.0, 20.0, 10.0
+    cells, rates = Wl.seeded_cells(rng, 700, W, H, 2)
+    cells[::7, 4] = 1.6            # some wide rods (window clipping)
+    cells[::11, 3] = cells[::11, 4]  # some zero-length boxes (centre fallback)
+    u = [np.random.default_rng(1).random((nx + 1) * (ny + 1)), np.zeros((nx + 1) * (ny + 1))]
+    d = [eqf.field_desc(nx, ny, W, H, 0.1, 100.0) for _ in range(2)]
+    grid = oracle.grid(nx + 1, ny + 1, npm, W, H)
+    with eqf.FieldBatch(d, nspecies=2, solver=eqf.solver_desc(eqf.JACOBI_CG)) as fb:
+        for s in range(2):
+            fb.upload(s, u[s])
+        sampled, counts = fb.couple(0, cells, npm, rates)
+        got = [fb.download(s) for s in range(2)]
+    ref, s_ref, c_ref = oracle.couple(grid, cells, u, rates)
+    assert np.array_equal(counts, c_ref)
+    for s in range(2):
+        assert np.array_equal(got[s], ref[s])
+    assert np.allclose(sampled, s_ref, rtol=1e-14, atol=1e-300)
+
+
+def test_uneven_batch_matches_oracle(eqf, oracle):
+    shapes = [(37, 23, 3.7, 2.3), (64, 33, 6.4, 3.3), (101, 50, 10.1, 5.0)]
+    walls = {oracle.TOP: (oracle.DIRICHLET, 0.0, 0, 0), oracle.RIGHT: (oracle.ROBIN, 0, 3.0, 0.1)}
+    descs, probs, u0 = [], [], []
+    rng = np.random.default_rng(9)
+    for (nx, ny, W, H) in shapes:
+        for D, g in ((300.0, 0.01), (80.0, 0.05)):
+            descs.append(eqf.field_desc(nx, ny, W, H, 0.1, D, g, 0.0, walls))
+            probs.append(oracle.problem(nx, ny, W, H, 0.1, D, g, 0.0, walls=walls))
+            u0.append(rng.random((nx + 1) * (ny + 1)))
+    for solver in SOLVERS:
+        with eqf.FieldBatch(descs, nspecies=2, solver=eqf.solver_desc(_kind(eqf, solver))) as fb:
+            for k, u in enumerate(u0):
+                fb.upload(k, u)
+            st = fb.step()
+            assert st.nconverged == len(descs)
+            for k in range(len(descs)):
+                ref, _ = oracle.step(probs[k], u0[k], "lu")
+                assert _rel(fb.download(k), ref) <= 1e-9, (solver, k)
+
+
+def test_boundary_flux_and_vertex_map(eqf, oracle):
+    d, p = _pair(eqf, oracle, 30, 20, 3.0, 2.0, 0.1, 50.0)
+    u = np.random.default_rng(4).random(oracle.num_nodes(p))
+    with eqf.FieldBatch([d], solver=eqf.solver_desc(eqf.JACOBI_CG)) as fb:
+        fb.upload(0, u)
+        flux = fb.boundary_flux(0)
+        xy, dof = fb.vertex_map(0)
+    assert math.isclose(flux, 0.1 * 50.0 * oracle.boundary_flux(p, u), rel_tol=1e-12)
+    assert np.array_equal(xy, oracle.mesh_coords(p))
+    assert np.array_equal(dof, np.arange(oracle.num_nodes(p)))
+
+
+def test_full_size_trap_residual_and_solver_agreement(eqf, oracle):
+    """2000x1000 trap (config 1 size): the relative residual of every solver,
+    recomputed in FP64 with the oracle's assembled CSR, is <= 1e-10, and the
+    three solvers agree to 1e-9."""
+    from eq_amd import workloads as Wl
+    t = Wl.config1(u0_seed=2)
+    walls = Wl.TOP_DIRICHLET
+    p = oracle.problem(t.nx, t.ny, t.width, t.height, 0.1, t.D[0], t.gamma[0], 0.0, walls=walls)
+    rp, c, v = oracle.assemble(p)
+    b = oracle.assemble_rhs(p, t.u0[0])
+    v2, b2 = oracle.apply_dirichlet(p, rp, c, v, b)
+    sols = {}
+    for solver in SOLVERS:
+        d = eqf.field_desc(t.nx, t.ny, t.width, t.height, 0.1, t.D[0], t.gamma[0], 0.0, walls)
+        with eqf.FieldBatch([d], solver=eqf.solver_desc(_kind(eqf, solver), max_iter=20000)) as fb:
+            fb.upload(0, t.u0[0])
+            st = fb.step()
+            x = fb.download(0)
+        res = np.linalg.norm(b2 - oracle.spmv(rp, c, v2, x)) / np.linalg.norm(b2)
+        print(f"{solver}: iterations={st.iterations_max} relres(gpu)={st.relres_max:.3e} relres(csr)={res:.3e}",
+              flush=True)
+        assert st.relres_max <= 1e-10
+        assert res <= 1.2e-10  # CSR evaluation adds its own rounding floor
+        sols[solver] = x
+    assert _rel(sols["gmg"], sols["jacobi"]) <= 1e-9
+    assert _rel(sols["mixed"], sols["jacobi"]) <= 1e-9
+
+
+def test_degenerate_inputs(eqf, oracle):
+    # 1x1 trap, all walls Dirichlet with distinct values; zero field; pure Neumann
+    walls = {w: (oracle.DIRICHLET, 0.1 * (w + 1), 0, 0) for w in range(4)}
+    d, p = _pair(eqf, oracle, 1, 1, 1.0, 1.0, 0.1, 1.0, 0.0, 0.0, walls)
+    for solver in SOLVERS:
+        with eqf.FieldBatch([d], solver=eqf.solver_desc(_kind(eqf, solver))) as fb:
+            fb.upload(0, np.zeros(4))
+            fb.step()
+            ref, _ = oracle.step(p, np.zeros(4), "lu")
+            assert np.array_equal(fb.download(0), ref)
+    d, p = _pair(eqf, oracle, 17, 9, 1.7, 0.9, 0.1, 10.0)
+    with eqf.FieldBatch([d], solver=eqf.solver_desc(eqf.MIXED_GMG_CG)) as fb:
+        fb.upload(0, np.zeros(180))
+        st = fb.step()
+        assert np.array_equal(fb.download(0), np.zeros(180)) and st.nconverged == 1
+        fb.upload(0, np.full(180, 0.25))
+        fb.step()
+        assert np.abs(fb.download(0) - 0.25).max() <= 1e-12

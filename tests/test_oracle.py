@@ -1,0 +1,204 @@
+"""Pins the CPU oracle (FFC-quadrature restatement of eQ's field step) against
+independent facts: the textbook-assembly golden fixtures, closed-form stencil
+entries, conservation/invariance laws, hand-countable cell coverages and a
+brute-force coverage scan.  CPU only."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "p1_tiny.npz")
+
+
+def _problem(O, params, walls):
+    nx, ny, W, H, dt, D, gamma, f = params
+    wd = {w: (int(walls[w][0]), walls[w][1], walls[w][2], walls[w][3]) for w in range(4) if walls[w][0] != 0}
+    return O.problem(int(nx), int(ny), W, H, dt, D, gamma, f, walls=wd)
+
+
+def _cases():
+    z = np.load(GOLD)
+    return [str(n) for n in z["names"]]
+
+
+@pytest.mark.parametrize("name", _cases())
+def test_oracle_matches_textbook_fixture(oracle, name):
+    O = oracle
+    z = np.load(GOLD)
+    p = _problem(O, z[f"{name}/params"], z[f"{name}/walls"])
+    T = z[f"{name}/tensor"]
+    n = O.num_nodes(p)
+    rp, c, v = O.assemble(p, T[:, 0], T[:, 1], T[:, 2])
+    A = O.to_scipy(rp, c, v, n).toarray()
+    Aref = z[f"{name}/A"]
+    assert np.abs(A - Aref).max() <= 1e-13 * np.abs(Aref).max()
+    b = O.assemble_rhs(p, z[f"{name}/u0"])
+    v2, b2 = O.apply_dirichlet(p, rp, c, v, b)
+    A2 = O.to_scipy(rp, c, v2, n).toarray()
+    assert np.abs(A2 - z[f"{name}/A_bc"]).max() <= 1e-13 * np.abs(Aref).max()
+    assert np.abs(b2 - z[f"{name}/b_bc"]).max() <= 1e-13 * max(1.0, np.abs(b2).max())
+    x = O.solve_banded(p, rp, c, v2, b2)
+    xr = z[f"{name}/x"]
+    assert np.linalg.norm(x - xr) <= 1e-12 * np.linalg.norm(xr)
+
+
+def test_interior_stencil_closed_form(oracle):
+    O = oracle
+    h, dt, D, g = 0.25, 0.1, 400.0, 0.01
+    p = O.problem(8, 6, 2.0, 1.5, dt, D, g, walls={O.TOP: (O.DIRICHLET, 0.0, 0, 0)})
+    rp, c, v = O.assemble(p)
+    A = O.to_scipy(rp, c, v, O.num_nodes(p)).toarray()
+    nw = 9
+    k = 3 + 2 * nw
+    m = 1.0 + dt * g
+    assert math.isclose(A[k, k], m * h * h / 2 + 4 * dt * D, rel_tol=1e-14)
+    for off in (1, -1, nw, -nw):
+        assert math.isclose(A[k, k + off], m * h * h / 12 - dt * D, rel_tol=1e-14)
+    for off in (nw + 1, -nw - 1):  # NE, SW: mass only (stiffness coupling is zero)
+        assert math.isclose(A[k, k + off], m * h * h / 12, rel_tol=1e-12)
+    for off in (nw - 1, -nw + 1):  # NW, SE: not mesh edges of the "right" diagonal
+        assert A[k, k + off] == 0.0
+    # corner (0,0) belongs to two triangles, edge node (0,3) to three
+    assert math.isclose(A[0, 0], m * h * h / 6 + dt * D, rel_tol=1e-13)
+    assert math.isclose(A[3, 3], m * h * h / 4 + 2 * dt * D, rel_tol=1e-13)
+
+
+def test_row_sums_are_mass(oracle):
+    """K has zero row sums, so A.1 = (1+dt*gamma) M.1 and sum(M) = W*H."""
+    O = oracle
+    p = O.problem(10, 7, 3.0, 2.1, 0.1, 250.0, 0.05)
+    rp, c, v = O.assemble(p)
+    A = O.to_scipy(rp, c, v, O.num_nodes(p))
+    ones = np.ones(O.num_nodes(p))
+    Aone = A @ ones
+    b = O.assemble_rhs(p, ones)  # = M.1
+    assert np.allclose(Aone, (1.0 + 0.1 * 0.05) * b, rtol=1e-11, atol=1e-13)
+    assert math.isclose(b.sum(), 3.0 * 2.1, rel_tol=1e-13)
+
+
+def test_constant_field_invariant_and_decay(oracle):
+    O = oracle
+    p = O.problem(12, 6, 3.0, 1.5, 0.1, 400.0, 0.0)
+    u = np.full(O.num_nodes(p), 0.7)
+    u1, info = O.step(p, u, "lu")
+    assert np.abs(u1 - 0.7).max() < 1e-12
+    p = O.problem(12, 6, 3.0, 1.5, 0.1, 400.0, 0.2)
+    u1, info = O.step(p, u, "lu")
+    assert np.abs(u1 - 0.7 / 1.02).max() < 1e-12
+
+
+def test_mass_conservation_neumann(oracle):
+    O = oracle
+    p = O.problem(16, 8, 4.0, 2.0, 0.1, 100.0, 0.0)
+    u = np.random.default_rng(3).random(O.num_nodes(p))
+    u1, _ = O.step(p, u, "lu")
+    ones = np.ones_like(u)
+    M1 = O.assemble_rhs(p, ones)
+    assert math.isclose(M1 @ u1, M1 @ u, rel_tol=1e-12)
+
+
+def test_lu_matches_pcg(oracle):
+    O = oracle
+    p = O.problem(40, 20, 10.0, 5.0, 0.1, 400.0, 0.01, walls={O.TOP: (O.DIRICHLET, 0.0, 0, 0)})
+    u = np.random.default_rng(5).random(O.num_nodes(p))
+    x1, i1 = O.step(p, u, "lu")
+    x2, i2 = O.step(p, u, "pcg", 1e-13)
+    assert np.linalg.norm(x1 - x2) <= 1e-10 * np.linalg.norm(x1)
+
+
+def test_boundary_flux_closed_forms(oracle):
+    O = oracle
+    nx, ny, W, H = 10, 5, 2.0, 1.0
+    p = O.problem(nx, ny, W, H, 0.1, 1.0)
+    xy = O.mesh_coords(p)
+    lin = 0.3 * xy[:, 0] - 1.7 * xy[:, 1] + 0.2
+    assert abs(O.boundary_flux(p, lin)) < 1e-12  # divergence theorem for a linear field
+    quad = xy[:, 0] ** 2
+    h = W / nx
+    assert math.isclose(O.boundary_flux(p, quad), H * (2 * h - 2 * W), rel_tol=1e-12)
+
+
+# ---------------------------------------------------------------- cells
+
+def test_coverage_axis_aligned_hand_count(oracle):
+    O = oracle
+    g = O.grid(41, 21, 4.0, 10.0, 5.0)
+    # rod of length 3, width 1 at a node: |dx| < 1, |dy| < 0.5 strictly -> 7 x 3 nodes
+    cov = O.cell_coverage(g, [5.0, 2.5, 0.0, 3.0, 1.0])
+    assert len(cov) == 21
+    assert set(map(tuple, cov)) == {(i, j) for i in (9, 10, 11) for j in range(17, 24)}
+    # same rod turned by pi/2: |dy| < 1, |dx| < 0.5
+    cov = O.cell_coverage(g, [5.0, 2.5, math.pi / 2, 3.0, 1.0])
+    assert set(map(tuple, cov)) == {(i, j) for i in range(7, 14) for j in (19, 20, 21)}
+    # scan order is row-major over the window
+    assert [tuple(r) for r in cov] == sorted(map(tuple, cov))
+
+
+def test_coverage_fallback_centre_node(oracle):
+    O = oracle
+    g = O.grid(21, 11, 2.0, 10.0, 5.0)
+    # length == width: zero-length box -> nothing strictly inside -> centre node
+    cov = O.cell_coverage(g, [3.3, 1.2, 0.4, 1.0, 1.0])
+    assert cov.tolist() == [[2, 7]]
+
+
+def _brute_coverage(cell, nodes_w, nodes_h, npm):
+    x, y, a, L, w = cell
+    j = np.arange(nodes_w)
+    i = np.arange(nodes_h)
+    X, Y = np.meshgrid(j / npm, i / npm)
+    lx = np.cos(a) * (X - x) + np.sin(a) * (Y - y)
+    ly = -np.sin(a) * (X - x) + np.cos(a) * (Y - y)
+    inside = (np.abs(lx) < (L - w) / 2) & (np.abs(ly) < w / 2)
+    ii, jj = np.nonzero(inside)
+    return set(zip(ii.tolist(), jj.tolist())), inside
+
+
+def test_coverage_matches_brute_force(oracle):
+    """Away from rounding ties the window never clips a unit-width rod, so the
+    reference scan equals a brute-force scan of the whole grid."""
+    O = oracle
+    rng = np.random.default_rng(7)
+    npm, W, H = 10.0, 20.0, 10.0
+    g = O.grid(201, 101, npm, W, H)
+    for _ in range(60):
+        cell = [rng.uniform(2, 18), rng.uniform(2, 8), rng.uniform(0, math.pi), rng.uniform(2, 4), 1.0]
+        ref, _ = _brute_coverage(cell, 201, 101, npm)
+        got = set(map(tuple, O.cell_coverage(g, cell).tolist()))
+        if not ref:
+            continue
+        assert got == ref
+
+
+def test_wide_rod_is_clipped_by_default_width_window(oracle):
+    """nodesToEdge uses the default 1 um width (eQabm.cpp:75): a 2 um wide rod
+    keeps only the nodes inside the narrower window."""
+    O = oracle
+    npm = 10.0
+    g = O.grid(101, 101, npm, 10.0, 10.0)
+    cell = [5.0, 5.0, 0.0, 4.0, 2.0]
+    got = set(map(tuple, O.cell_coverage(g, cell).tolist()))
+    ref, _ = _brute_coverage(cell, 101, 101, npm)
+    i1, i2, j1, j2 = O.cell_window(g, cell)
+    assert (i1, i2) == (45, 55)  # round(5*10) -/+ round(10/2)
+    assert got == {(i, j) for (i, j) in ref if i1 <= i <= i2 and j1 <= j <= j2}
+    assert len(got) < len(ref)
+
+
+def test_couple_sequential_semantics(oracle):
+    O = oracle
+    g = O.grid(41, 21, 4.0, 10.0, 5.0)
+    cells = np.array([[5.0, 2.5, 0.0, 3.0, 1.0],    # 21 nodes
+                      [5.5, 2.5, 0.0, 3.0, 1.0]])   # overlaps the first on 15 nodes
+    u = np.zeros(41 * 21)
+    q = np.array([[2.1], [4.2]])
+    (f,), sampled, counts = O.couple(g, cells, [u], q)
+    assert counts.tolist() == [21, 21]
+    assert sampled[0, 0] == 0.0
+    # the second cell reads the first cell's deposit on the shared nodes
+    assert math.isclose(sampled[1, 0], (15 * 2.1 / 21) / 21, rel_tol=1e-14)
+    assert math.isclose(f.sum(), 2.1 + 4.2, rel_tol=1e-14)
+    # constant field is sampled exactly
+    (f2,), s2, _ = O.couple(g, cells, [np.full(41 * 21, 0.3)], np.zeros((2, 1)))
+    assert np.allclose(s2, 0.3, rtol=1e-15, atol=0)
