@@ -66,11 +66,11 @@ class Run:
         self.iters = []
         self.relres = []
 
-    def step(self, k):
+    def step(self, k, check=True):
         stride = self.nc * 5 * 8
         self.fb.couple_dev(0, self.cells.data_ptr() + k * stride, self.nc, self.trap.npm, self.q.data_ptr(),
                            self.sampled.data_ptr(), 0, self.F.SAMPLE | self.F.DEPOSIT)
-        st = self.fb.step()
+        st = self.fb.step(check=check)
         self.iters.append(st.iterations_max)
         self.relres.append(st.relres_max)
         return st
@@ -132,6 +132,9 @@ def main():
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run a 256 MiB read + 256 MiB write elementwise kernel (torch.mul) three times first, "
+                         "as the known-size reference for HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this field step runs on one GPU; multi-GPU runs are not part of this build",
@@ -143,6 +146,13 @@ def main():
     from eq_amd import workloads as W
 
     torch.cuda.set_device(0)
+    if args.copy_probe:
+        a = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            torch.mul(a, 1.0, out=b)
+        torch.cuda.synchronize()
+        del a, b
     kinds = {"mixed": F.MIXED_GMG_CG, "gmg": F.GMG_CG, "jacobi": F.JACOBI_CG}
     trap = W.config1()
     nsteps = args.warmup + args.steps
@@ -244,17 +254,22 @@ def main():
             log(f"bench: variant {name}: {variants[name]['value']:.4e} DOF-timesteps/s")
         out["variants"] = variants
         # accuracy of the mixed path against an FP64 solve of the same step
-        ref = Run(F, trap, F.GMG_CG, cells_dev, q_dev, sampled_dev, rtol=1e-12, max_iter=1000)
+        ref = Run(F, trap, F.GMG_CG, cells_dev, q_dev, sampled_dev, rtol=1e-13, max_iter=200)
         mix = Run(F, trap, F.MIXED_GMG_CG, cells_dev, q_dev, sampled_dev)
         for k in range(3):
-            ref.step(k)
+            ref.step(k, check=False)  # may stop at its rounding floor above 1e-13
             mix.step(k)
+            # continue both from the FP64 reference state
+            for s in range(2):
+                mix.fb.upload(s, ref.fb.download(s)) if k < 2 else None
         errs = []
         for s in range(2):
             a, b = mix.fb.download(s), ref.fb.download(s)
             errs.append({"species": s, "rel_l2": float(np.linalg.norm(a - b) / np.linalg.norm(b)),
                          "max_abs": float(np.abs(a - b).max())})
-        out["mixed_vs_fp64"] = {"reference": "FP64 GMG-CG at rtol 1e-12, same 3 steps", "errors": errs}
+        out["mixed_vs_fp64"] = {"reference": "FP64 GMG-CG to rtol 1e-13 (or its rounding floor), same step "
+                                              "from the same state", "reference_relres": ref.relres,
+                                 "errors": errs}
         ref.fb.close()
         mix.fb.close()
 

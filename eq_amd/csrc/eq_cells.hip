@@ -6,17 +6,19 @@
 // q_c/|S_c| to every covered node of every species (writeHSL, 338-359).  A
 // cell therefore samples the deposits of all earlier cells.
 //
-// GPU plan (no atomics, deterministic):
+// GPU plan (deterministic, no atomics anywhere):
 //   1. k_cell_geo    one wavefront per cell: pole window, |S_c| (ballot counts)
 //   2. k_scan        exclusive scan of (window rows x row segments) per cell
-//   3. k_cell_rows   one wavefront per (row, 4096-column segment, species):
-//                    the row segment lives in LDS; the wave walks the cells
-//                    in list order (64 at a time by ballot), for each cell
-//                    covering the row it sums the covered values (the read)
-//                    and then adds q_c/|S_c| (the write).  Per node the
-//                    additions happen in list order, so the deposited field
-//                    is bit-identical to the sequential loop.
-//   4. k_cell_sample one thread per cell: mean over its row partials.
+//   3. k_band_masks  one wavefront per band of kBand rows: bit c of word c/64
+//                    says cell c's window meets the band (ballots)
+//   4. k_cell_rows   one wavefront per (row, column segment, species): the
+//                    segment lives in LDS; the wave visits the band's cells
+//                    in list order, and for each cell covering the row sums
+//                    the covered values (the read) and then adds q_c/|S_c|
+//                    (the write).  Per node the additions happen in list
+//                    order, so the deposited field is bit-identical to the
+//                    sequential loop.
+//   5. k_cell_sample one thread per cell: mean over its row partials.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,13 +31,13 @@
 
 namespace eqk {
 
-constexpr int kSeg = 4096; // columns per LDS row segment (32 KB of FP64)
+constexpr int kSegMax = 512;  // LDS row segment capacity (4 KB of FP64)
+constexpr int kBand = 32;     // rows per band of the cell bitmasks
 
 struct CellGeo {
     double cx, cy, cs, sn, o, rad; // centre, cos/sin of the angle, box half-length/half-width
-    int32_t i1, i2, j1, j2;        // search window (inclusive)
+    int32_t i1, i2, j1, j2;        // search window (inclusive); the centre node for fallback cells
     int32_t cnt;                   // nodes found by the scan (0: centre-node fallback)
-    int32_t ci, cj;                // centre node (fallback)
     int32_t nseg0;                 // first row segment touched
     int64_t poff;                  // offset of this cell's row partials
 };
@@ -43,6 +45,7 @@ struct CellGeo {
 struct GridDesc {
     int32_t nodes_w, nodes_h;
     double npm, trap_w, trap_h;
+    int32_t segw, nseg;            // column segment width and count
 };
 
 // size_t(round(v)) of eQ::data::ij_from_xy (src/eQ.h:119-124); negative values clamp to 0
@@ -60,13 +63,14 @@ __device__ __forceinline__ int32_t to_index(double v, double npm)
 // cpvcross(edge, p - vertex) < 0 tests are exactly -o < lx < o and
 // -r < ly < r (products with a zero edge component are +-0, and the sign of
 // a rounded difference is the sign of the exact difference).
-__device__ __forceinline__ bool inside_cell(const CellGeo& g, double X, double Y)
+__device__ __forceinline__ bool inside_cell(double cx, double cy, double cs, double sn, double o, double rad,
+                                            double X, double Y)
 {
-    const double itx = (-g.sn) * g.cy - g.cs * g.cx;
-    const double ity = g.sn * g.cx - g.cs * g.cy;
-    const double lx = g.cs * X + g.sn * Y + itx;
-    const double ly = (-g.sn) * X + g.cs * Y + ity;
-    return (lx - g.o < 0.0) && (ly - g.rad < 0.0) && (ly + g.rad > 0.0) && (lx + g.o > 0.0);
+    const double itx = (-sn) * cy - cs * cx;
+    const double ity = sn * cx - cs * cy;
+    const double lx = cs * X + sn * Y + itx;
+    const double ly = (-sn) * X + cs * Y + ity;
+    return (lx - o < 0.0) && (ly - rad < 0.0) && (ly + rad > 0.0) && (lx + o > 0.0);
 }
 
 // Ecoli::updatePoleCenters (src/abm/Ecoli.cpp:36-63)
@@ -110,7 +114,6 @@ __global__ __launch_bounds__(256) void k_cell_geo(const eq_cell* __restrict__ ce
     j1 = (j1 >= nte) ? j1 - nte : 0;
     i2 = (i2 + nte >= G.nodes_h - 1) ? G.nodes_h - 1 : i2 + nte;
     j2 = (j2 + nte >= G.nodes_w - 1) ? G.nodes_w - 1 : j2 + nte;
-    g.i1 = i1; g.i2 = i2; g.j1 = j1; g.j2 = j2;
     const int32_t ww = j2 - j1 + 1;
     const int64_t area = (i2 >= i1 && j2 >= j1) ? (int64_t)(i2 - i1 + 1) * ww : 0;
     int32_t cnt = 0;
@@ -119,34 +122,36 @@ __global__ __launch_bounds__(256) void k_cell_geo(const eq_cell* __restrict__ ce
         bool in = false;
         if (t < area) {
             const int32_t i = i1 + (int32_t)(t / ww), j = j1 + (int32_t)(t % ww);
-            in = inside_cell(g, (double)j / G.npm, (double)i / G.npm);
+            in = inside_cell(g.cx, g.cy, g.cs, g.sn, g.o, g.rad, (double)j / G.npm, (double)i / G.npm);
         }
         cnt += __popcll(__ballot(in));
     }
     g.cnt = cnt;
-    // fallback: the centre node (src/abm/eQabm.cpp:299-303)
-    g.ci = min(to_index(g.cy, G.npm), G.nodes_h - 1);
-    g.cj = min(to_index(g.cx, G.npm), G.nodes_w - 1);
-    g.nseg0 = (cnt > 0) ? j1 / kSeg : g.cj / kSeg;
+    if (cnt == 0) {
+        // fallback: the centre node (src/abm/eQabm.cpp:299-303)
+        i1 = i2 = min(to_index(g.cy, G.npm), G.nodes_h - 1);
+        j1 = j2 = min(to_index(g.cx, G.npm), G.nodes_w - 1);
+    }
+    g.i1 = i1; g.i2 = i2; g.j1 = j1; g.j2 = j2;
+    g.nseg0 = j1 / G.segw;
     g.poff = 0;
     if (lane == 0) geo[c] = g;
 }
 
-__device__ __forceinline__ int64_t cell_slots(const CellGeo& g)
+__device__ __forceinline__ int64_t cell_slots(const CellGeo& g, int segw)
 {
-    if (g.cnt == 0) return 1;
-    return (int64_t)(g.i2 - g.i1 + 1) * (g.j2 / kSeg - g.j1 / kSeg + 1);
+    return (int64_t)(g.i2 - g.i1 + 1) * (g.j2 / segw - g.j1 / segw + 1);
 }
 
 // exclusive scan of the per-cell partial slots (single block, deterministic)
-__global__ __launch_bounds__(1024) void k_scan(CellGeo* geo, int nc, int64_t* total)
+__global__ __launch_bounds__(1024) void k_scan(CellGeo* geo, int nc, int segw, int64_t* total)
 {
     __shared__ int64_t part[1024];
     const int t = threadIdx.x, nt = blockDim.x;
     const int per = (nc + nt - 1) / nt;
     const int lo = min(nc, t * per), hi = min(nc, lo + per);
     int64_t s = 0;
-    for (int c = lo; c < hi; ++c) s += cell_slots(geo[c]);
+    for (int c = lo; c < hi; ++c) s += cell_slots(geo[c], segw);
     part[t] = s;
     __syncthreads();
     if (t == 0) {
@@ -162,7 +167,22 @@ __global__ __launch_bounds__(1024) void k_scan(CellGeo* geo, int nc, int64_t* to
     int64_t acc = part[t];
     for (int c = lo; c < hi; ++c) {
         geo[c].poff = acc;
-        acc += cell_slots(geo[c]);
+        acc += cell_slots(geo[c], segw);
+    }
+}
+
+// one wavefront per band: bit (c % 64) of word c/64 = cell c's window meets the band
+__global__ __launch_bounds__(64) void k_band_masks(const CellGeo* __restrict__ geo, int nc, int nwords,
+                                                   uint64_t* __restrict__ masks)
+{
+    const int band = blockIdx.x, lane = threadIdx.x;
+    const int r0 = band * kBand, r1 = r0 + kBand - 1;
+    for (int w = 0; w < nwords; ++w) {
+        const int c = w * 64 + lane;
+        bool hit = false;
+        if (c < nc) hit = (geo[c].i1 <= r1 && geo[c].i2 >= r0);
+        const uint64_t m = __ballot(hit);
+        if (lane == 0) masks[(int64_t)band * nwords + w] = m;
     }
 }
 
@@ -173,62 +193,82 @@ __device__ __forceinline__ double wave_sum64(double v)
     return v;
 }
 
+__device__ __forceinline__ double bcast(double v, int k)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), k);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__device__ __forceinline__ int bcasti(int v, int k) { return __builtin_amdgcn_readlane(v, k); }
+
+__device__ __forceinline__ int64_t bcastl(int64_t v, int k)
+{
+    const int lo = __builtin_amdgcn_readlane((int)(v & 0xffffffffll), k);
+    const int hi = __builtin_amdgcn_readlane((int)(v >> 32), k);
+    return ((int64_t)hi << 32) | (unsigned int)lo;
+}
+
 // one wavefront per (row, segment); blockIdx.y = species
-__global__ __launch_bounds__(64) void k_cell_rows(const CellGeo* __restrict__ geo, int nc, GridDesc G, int nseg,
+__global__ __launch_bounds__(64) void k_cell_rows(const CellGeo* __restrict__ geo, int nc, GridDesc G,
+                                                  const uint64_t* __restrict__ masks, int nwords,
                                                   double* const* __restrict__ fields, const double* __restrict__ q,
                                                   int ns, double* __restrict__ partial, int64_t total,
                                                   unsigned flags, int pitch)
 {
-    __shared__ double row[kSeg];
+    __shared__ double row[kSegMax];
     const int lane = threadIdx.x;
-    const int i = blockIdx.x / nseg, seg = blockIdx.x - (blockIdx.x / nseg) * nseg;
+    const int i = blockIdx.x / G.nseg, seg = blockIdx.x - i * G.nseg;
     const int s = blockIdx.y;
-    const int jb = seg * kSeg, je = min(G.nodes_w, jb + kSeg) - 1;
+    const int jb = seg * G.segw, je = min(G.nodes_w, jb + G.segw) - 1;
     double* U = fields[s] + (int64_t)i * pitch;
     for (int j = jb + lane; j <= je; j += 64) row[j - jb] = U[j];
     __syncthreads();
     const bool dep = (flags & EQ_COUPLE_DEPOSIT) != 0;
     const double Y = (double)i / G.npm;
     double* part = partial + (int64_t)s * total;
-    for (int cb = 0; cb < nc; cb += 64) {
-        bool hit = false;
-        const int c = cb + lane;
-        if (c < nc) {
-            const CellGeo& g = geo[c];
-            if (g.cnt > 0)
+    const uint64_t* bm = masks + (int64_t)(i / kBand) * nwords;
+    for (int wb = 0; wb < nwords; wb += 64) {
+        const uint64_t myword = (wb + lane < nwords) ? bm[wb + lane] : 0ull;
+        uint64_t nz = __ballot(myword != 0ull);
+        while (nz) {
+            const int wl = __ffsll((unsigned long long)nz) - 1;
+            nz &= nz - 1;
+            const uint64_t word = (uint64_t)bcastl((int64_t)myword, wl);
+            const int c = (wb + wl) * 64 + lane;
+            bool hit = false;
+            CellGeo g{};
+            double qc = 0.0;
+            if ((word >> lane) & 1ull) {
+                g = geo[c];
                 hit = (i >= g.i1 && i <= g.i2 && g.j1 <= je && g.j2 >= jb);
-            else
-                hit = (i == g.ci && g.cj >= jb && g.cj <= je);
-        }
-        uint64_t mask = __ballot(hit);
-        while (mask) {
-            const int k = __ffsll((unsigned long long)mask) - 1;
-            mask &= mask - 1;
-            const CellGeo g = geo[cb + k];
-            const double d = q[(int64_t)(cb + k) * ns + s] / (double)(g.cnt > 0 ? g.cnt : 1);
-            double sum = 0.0;
-            int64_t slot;
-            if (g.cnt == 0) {
-                if (lane == 0) {
-                    const double v = row[g.cj - jb];
-                    sum = v;
-                    if (dep) row[g.cj - jb] = v + d;
-                }
-                slot = g.poff;
-            } else {
-                const int lo = max(g.j1, jb), hi = min(g.j2, je);
+                if (hit) qc = q[(int64_t)c * ns + s];
+            }
+            uint64_t mask = __ballot(hit);
+            while (mask) {
+                const int k = __ffsll((unsigned long long)mask) - 1;
+                mask &= mask - 1;
+                const double cx = bcast(g.cx, k), cy = bcast(g.cy, k), cs = bcast(g.cs, k), sn = bcast(g.sn, k);
+                const double o = bcast(g.o, k), rad = bcast(g.rad, k);
+                const int i1 = bcasti(g.i1, k), j1 = bcasti(g.j1, k), j2 = bcasti(g.j2, k);
+                const int cnt = bcasti(g.cnt, k), nseg0 = bcasti(g.nseg0, k);
+                const int64_t poff = bcastl(g.poff, k);
+                const double d = bcast(qc, k) / (double)(cnt > 0 ? cnt : 1);
+                const int lo = max(j1, jb), hi = min(j2, je);
+                double sum = 0.0;
                 for (int j = lo + lane; j <= hi; j += 64) {
-                    if (inside_cell(g, (double)j / G.npm, Y)) {
+                    if (cnt == 0 || inside_cell(cx, cy, cs, sn, o, rad, (double)j / G.npm, Y)) {
                         const double v = row[j - jb];
                         sum += v;
                         if (dep) row[j - jb] = v + d;
                     }
                 }
-                const int nsegc = g.j2 / kSeg - g.j1 / kSeg + 1;
-                slot = g.poff + (int64_t)(i - g.i1) * nsegc + (seg - g.nseg0);
+                const int nsegc = j2 / G.segw - j1 / G.segw + 1;
+                const int64_t slot = poff + (int64_t)(i - i1) * nsegc + (seg - nseg0);
+                sum = wave_sum64(sum);
+                if (lane == 0) part[slot] = sum;
             }
-            sum = wave_sum64(sum);
-            if (lane == 0) part[slot] = sum;
         }
     }
     __syncthreads();
@@ -237,14 +277,14 @@ __global__ __launch_bounds__(64) void k_cell_rows(const CellGeo* __restrict__ ge
 }
 
 // sampled[c,s] = (sum of the cell's row partials in row order) / |S_c|
-__global__ __launch_bounds__(256) void k_cell_sample(const CellGeo* __restrict__ geo, int nc, int ns,
+__global__ __launch_bounds__(256) void k_cell_sample(const CellGeo* __restrict__ geo, int nc, int ns, int segw,
                                                      const double* __restrict__ partial, int64_t total,
                                                      double* __restrict__ sampled, int32_t* __restrict__ counts)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
     const CellGeo g = geo[c];
-    const int64_t n = cell_slots(g);
+    const int64_t n = cell_slots(g, segw);
     const int cnt = g.cnt > 0 ? g.cnt : 1;
     if (counts) counts[c] = cnt;
     if (!sampled) return;
@@ -292,17 +332,23 @@ static int couple_impl(eq_batch* b, int32_t trap, const eq_cell* d_cells, size_t
     G.npm = npm;
     G.trap_w = d.width;
     G.trap_h = d.height;
+    // column segments of at most kSegMax nodes (more wavefronts in flight)
+    G.nseg = (G.nodes_w + kSegMax - 1) / kSegMax;
+    G.segw = (G.nodes_w + G.nseg - 1) / G.nseg;
+    const int nwords = (int)((nc + 63) / 64);
+    const int nbands = (G.nodes_h + kBand - 1) / kBand;
     hipStream_t s = b->stream;
-    // scratch: geo | total | field pointers | partials
+    // scratch: geo | total | band masks | partials
     const size_t geo_bytes = align256(sizeof(CellGeo) * nc);
-    const size_t head = geo_bytes + 256 + align256(sizeof(double*) * ns);
+    const size_t mask_bytes = align256(sizeof(uint64_t) * (size_t)nwords * nbands);
+    const size_t head = geo_bytes + 256 + mask_bytes;
     int rc = ensure_scratch(b, head + 256);
     if (rc) return rc;
     char* base = (char*)b->cell_scratch;
     CellGeo* geo = (CellGeo*)base;
     int64_t* d_total = (int64_t*)(base + geo_bytes);
     k_cell_geo<<<(unsigned)((nc + 3) / 4), 256, 0, s>>>(d_cells, (int)nc, G, geo);
-    k_scan<<<1, 1024, 0, s>>>(geo, (int)nc, d_total);
+    k_scan<<<1, 1024, 0, s>>>(geo, (int)nc, G.segw, d_total);
     int64_t total = 0;
     EQ_HIP(hipMemcpyAsync(&total, d_total, sizeof(int64_t), hipMemcpyDeviceToHost, s));
     EQ_HIP(hipStreamSynchronize(s));
@@ -314,23 +360,21 @@ static int couple_impl(eq_batch* b, int32_t trap, const eq_cell* d_cells, size_t
         geo = (CellGeo*)base;
         d_total = (int64_t*)(base + geo_bytes);
         k_cell_geo<<<(unsigned)((nc + 3) / 4), 256, 0, s>>>(d_cells, (int)nc, G, geo);
-        k_scan<<<1, 1024, 0, s>>>(geo, (int)nc, d_total);
+        k_scan<<<1, 1024, 0, s>>>(geo, (int)nc, G.segw, d_total);
     }
-    double** d_fields = (double**)(base + geo_bytes + 256);
+    uint64_t* masks = (uint64_t*)(base + geo_bytes + 256);
+    double* const* d_fields = b->d_field_ptrs + p0; // the trap's species, in order
     double* partial = (double*)(base + head);
-    std::vector<double*> hf(ns);
-    for (int k = 0; k < ns; ++k) hf[k] = b->u + b->levels[0].probs[p0 + k].origin;
-    EQ_HIP(hipMemcpyAsync(d_fields, hf.data(), sizeof(double*) * ns, hipMemcpyHostToDevice, s));
-    const int nseg = (G.nodes_w + kSeg - 1) / kSeg;
+    k_band_masks<<<nbands, 64, 0, s>>>(geo, (int)nc, nwords, masks);
     prof_begin(b, EQ_KERNEL_COUPLE);
-    k_cell_rows<<<dim3((unsigned)(G.nodes_h * nseg), (unsigned)ns), 64, 0, s>>>(
-        geo, (int)nc, G, nseg, d_fields, d_q, ns, partial, total, flags, P.pitch);
-    // algorithmic bytes: every row read (+written when depositing) once per species
+    k_cell_rows<<<dim3((unsigned)(G.nodes_h * G.nseg), (unsigned)ns), 64, 0, s>>>(
+        geo, (int)nc, G, masks, nwords, d_fields, d_q, ns, partial, total, flags, P.pitch);
+    // algorithmic bytes: every row read (and written when depositing) once per species
     prof_end(b, EQ_KERNEL_COUPLE,
              (double)G.nodes_w * G.nodes_h * ns * sizeof(double) * ((flags & EQ_COUPLE_DEPOSIT) ? 2 : 1));
     if (d_sampled || d_counts)
         k_cell_sample<<<(unsigned)((nc + 255) / 256), 256, 0, s>>>(
-            geo, (int)nc, ns, partial, total, (flags & EQ_COUPLE_SAMPLE) ? d_sampled : nullptr, d_counts);
+            geo, (int)nc, ns, G.segw, partial, total, (flags & EQ_COUPLE_SAMPLE) ? d_sampled : nullptr, d_counts);
     EQ_HIP(hipGetLastError());
     return EQ_OK;
 }
@@ -375,7 +419,7 @@ int eq_batch_couple(eq_batch* b, int32_t trap, const eq_cell* cells, size_t nc, 
         hipMemsetAsync(dq, 0, qb, s);
     }
     if (rc == EQ_OK)
-        rc = couple_impl(b, 0 + trap, dc, nc, npm, dq, sampled ? ds : nullptr, counts ? dk : nullptr, flags);
+        rc = couple_impl(b, trap, dc, nc, npm, dq, sampled ? ds : nullptr, counts ? dk : nullptr, flags);
     if (rc == EQ_OK && sampled && hipMemcpyAsync(sampled, ds, sb, hipMemcpyDeviceToHost, s) != hipSuccess)
         rc = set_error(EQ_ERR_HIP, "couple download");
     if (rc == EQ_OK && counts && hipMemcpyAsync(counts, dk, kb, hipMemcpyDeviceToHost, s) != hipSuccess)

@@ -29,7 +29,6 @@ struct CgState {
     double tol2;            // rtol^2 (times bb when compared)
     int32_t iter, done, max_iter, flexible;
     int32_t restart, pad_;  // restart: next beta is 0 (after a residual replacement)
-    uint32_t ticket[4];
 };
 
 struct Level {
@@ -91,6 +90,7 @@ struct eq_batch {
     // cell coupling scratch (grown on demand)
     void* cell_scratch = nullptr;
     size_t cell_scratch_bytes = 0;
+    double** d_field_ptrs = nullptr; // device table: node (0,0) of every problem's field
     double* flux_out = nullptr;      // device scalar
     double* h_flux = nullptr;        // pinned
     eqk::ProfileSlot prof[EQ_KERNEL_COUNT_];

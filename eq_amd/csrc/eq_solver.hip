@@ -108,39 +108,20 @@ __device__ __forceinline__ void block_sum(double (&v)[NV])
     __syncthreads();
 }
 
-// Write this tile's partial sums; the last tile of the problem to arrive
-// reduces all partials of the problem in tile order (deterministic, no
-// floating-point atomics) and returns true in every thread of its block.
+// Write this tile's partial sums (plain stores).  The per-problem totals are
+// formed by k_finish in the next launch: the kernel boundary makes the
+// partials visible across XCDs, where an in-kernel device-scope fence per
+// block would write back the XCD's whole L2 every time.
 template <int NV>
-__device__ __forceinline__ bool tile_partial_then_last(double (&v)[NV], double* partial, int t, uint32_t* ticket,
-                                                       const LevelProb& P, double (&tot)[NV])
+__device__ __forceinline__ void tile_partial(double (&v)[NV], double* partial, int t)
 {
     block_sum<NV>(v);
-    __shared__ int last;
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0)
 #pragma unroll
         for (int k = 0; k < NV; ++k) partial[(int64_t)t * 4 + k] = v[k];
-        __threadfence();
-        const uint32_t prev = atomicAdd(ticket, 1u);
-        last = (prev == (uint32_t)(P.ntiles - 1));
-    }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
-    double s[NV];
-#pragma unroll
-    for (int k = 0; k < NV; ++k) s[k] = 0.0;
-    for (int q = threadIdx.x; q < P.ntiles; q += blockDim.x)
-#pragma unroll
-        for (int k = 0; k < NV; ++k) s[k] += __builtin_nontemporal_load(&partial[(int64_t)(P.tile_begin + q) * 4 + k]);
-    block_sum<NV>(s);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; ++k) tot[k] = s[k];
-        *ticket = 0u;
-    }
-    return true;
 }
+
+enum FinishOp { kFinRhs = 0, kFinInit = 1, kFinAp = 2, kFinUpdate = 3, kFinRz = 4, kFinTrue = 5 };
 
 template <typename T>
 __device__ __forceinline__ void load7(const T* __restrict__ v, int64_t idx, int pitch, T& c, T& w, T& e, T& s, T& n,
@@ -177,10 +158,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs(LevelView L, const double* __res
             v[0] = bv * bv;
         }
     }
-    double tot[1];
-    if (tile_partial_then_last<1>(v, partial, c.t, &st[P.prob].ticket[0], P, tot) && threadIdx.x == 0) {
-        st[P.prob].bb = tot[0];
-    }
+    tile_partial<1>(v, partial, c.t);
 }
 
 // x_D = g; r = b - A x; Jacobi: z = p = r / diag(A)
@@ -220,15 +198,7 @@ __global__ __launch_bounds__(kBlock) void k_init(LevelView L, double* x, const d
             }
         }
     }
-    double tot[2];
-    if (tile_partial_then_last<2>(v, partial, c.t, &st[P.prob].ticket[1], P, tot) && threadIdx.x == 0) {
-        CgState& s = st[P.prob];
-        s.rr = tot[0];
-        s.rz = tot[1];
-        s.iter = 0;
-        s.beta = 0.0;
-        s.done = (tot[0] <= s.tol2 * s.bb) ? 1 : 0;
-    }
+    tile_partial<2>(v, partial, c.t);
 }
 
 // q = A p; alpha = rz / (p.q)
@@ -251,17 +221,7 @@ __global__ __launch_bounds__(kBlock) void k_ap(LevelView L, const double* __rest
         q[c.idx] = qv;
         v[0] = pc * qv;
     }
-    double tot[1];
-    if (tile_partial_then_last<1>(v, partial, c.t, &st[P.prob].ticket[2], P, tot) && threadIdx.x == 0) {
-        CgState& s = st[P.prob];
-        s.pq = tot[0];
-        if (tot[0] > 0.0 && s.rz > 0.0) {
-            s.alpha = s.rz / tot[0];
-        } else {
-            s.alpha = 0.0;
-            s.done = 1; // exact solution reached (or breakdown): stop this problem
-        }
-    }
+    tile_partial<1>(v, partial, c.t);
 }
 
 // x += alpha p; r -= alpha q; Jacobi: z = r/diag, beta; flexible: r.z_old
@@ -293,18 +253,7 @@ __global__ __launch_bounds__(kBlock) void k_update(LevelView L, double* __restri
             v[2] = rv * z[k];
         }
     }
-    double tot[3];
-    if (tile_partial_then_last<3>(v, partial, c.t, &st[P.prob].ticket[3], P, tot) && threadIdx.x == 0) {
-        CgState& s = st[P.prob];
-        s.iter += 1;
-        s.rr = tot[0];
-        s.rzold = tot[2];
-        if (jacobi) {
-            s.beta = tot[1] / s.rz;
-            s.rz = tot[1];
-        }
-        s.done = (tot[0] <= s.tol2 * s.bb || s.iter >= s.max_iter) ? 1 : 0;
-    }
+    tile_partial<3>(v, partial, c.t);
 }
 
 // multigrid path: rz = r.z (after the V-cycle), beta (Fletcher-Reeves or
@@ -318,18 +267,7 @@ __global__ __launch_bounds__(kBlock) void k_rz(LevelView L, const double* __rest
     if (st[P.prob].done) return;
     double v[1] = {0.0};
     if (c.in) v[0] = r[c.idx] * z[c.idx];
-    double tot[1];
-    if (tile_partial_then_last<1>(v, partial, c.t, &st[P.prob].ticket[0], P, tot) && threadIdx.x == 0) {
-        CgState& s = st[P.prob];
-        if (s.iter == 0 || s.restart) {
-            s.beta = 0.0;
-            s.restart = 0;
-        } else {
-            const double num = s.flexible ? (tot[0] - s.rzold) : tot[0];
-            s.beta = num / s.rz;
-        }
-        s.rz = tot[0];
-    }
+    tile_partial<1>(v, partial, c.t);
 }
 
 // p = z + beta p
@@ -361,13 +299,75 @@ __global__ __launch_bounds__(kBlock) void k_true_residual(LevelView L, const dou
         r[c.idx] = rv;
         v[0] = rv * rv;
     }
-    double tot[1];
-    if (tile_partial_then_last<1>(v, partial, c.t, &st[P.prob].ticket[1], P, tot) && threadIdx.x == 0) {
-        CgState& s = st[P.prob];
-        s.rr_true = tot[0];
-        s.rr = tot[0];
-        s.done = (tot[0] <= s.tol2 * s.bb || s.iter >= s.max_iter) ? 1 : 0;
+    tile_partial<1>(v, partial, c.t);
+}
+
+// Per-problem totals of the tile partials, summed in tile order
+// (deterministic), and the Krylov scalar updates that follow from them.
+// One block per problem; problems the producer skipped (done) are skipped.
+__global__ __launch_bounds__(kBlock) void k_finish(const LevelProb* __restrict__ probs,
+                                                   const double* __restrict__ partial, CgState* st, int op,
+                                                   int jacobi)
+{
+    const LevelProb& P = probs[blockIdx.x];
+    CgState& s = st[P.prob];
+    if ((op == kFinAp || op == kFinUpdate || op == kFinRz) && s.done) return;
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int q = threadIdx.x; q < P.ntiles; q += blockDim.x) {
+        const double* pp = partial + (int64_t)(P.tile_begin + q) * 4;
+        v[0] += pp[0];
+        v[1] += pp[1];
+        v[2] += pp[2];
+    }
+    block_sum<3>(v);
+    if (threadIdx.x != 0) return;
+    switch (op) {
+    case kFinRhs:
+        s.bb = v[0];
+        break;
+    case kFinInit:
+        s.rr = v[0];
+        s.rz = v[1];
+        s.iter = 0;
+        s.beta = 0.0;
+        s.restart = 0;
+        s.done = (v[0] <= s.tol2 * s.bb) ? 1 : 0;
+        break;
+    case kFinAp:
+        s.pq = v[0];
+        if (v[0] > 0.0 && s.rz > 0.0) {
+            s.alpha = s.rz / v[0];
+        } else {
+            s.alpha = 0.0;
+            s.done = 1; // exact solution reached (or breakdown): stop this problem
+        }
+        break;
+    case kFinUpdate:
+        s.iter += 1;
+        s.rr = v[0];
+        s.rzold = v[2];
+        if (jacobi) {
+            s.beta = v[1] / s.rz;
+            s.rz = v[1];
+        }
+        s.done = (v[0] <= s.tol2 * s.bb || s.iter >= s.max_iter) ? 1 : 0;
+        break;
+    case kFinRz:
+        if (s.iter == 0 || s.restart) {
+            s.beta = 0.0;
+            s.restart = 0;
+        } else {
+            const double num = s.flexible ? (v[0] - s.rzold) : v[0];
+            s.beta = num / s.rz;
+        }
+        s.rz = v[0];
+        break;
+    case kFinTrue:
+        s.rr_true = v[0];
+        s.rr = v[0];
+        s.done = (v[0] <= s.tol2 * s.bb || s.iter >= s.max_iter) ? 1 : 0;
         s.restart = s.done ? 0 : 1;
+        break;
     }
 }
 
@@ -972,23 +972,31 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
     const bool jacobi = (S.kind == EQ_SOLVER_JACOBI_CG);
     const double fine_nodes = L0.nodes_all;
     VCycle<T> vc{b};
+    auto finish = [&](int op) {
+        k_finish<<<b->nprob, kBlock, 0, s>>>(L0.d_probs, b->partial, b->d_state, op, jacobi ? 1 : 0);
+    };
     auto precondition = [&]() {
         vc.run(0, b->r);
         k_rz<<<g0, kBlock, 0, s>>>(A, b->r, b->z, b->partial, b->d_state);
+        finish(kFinRz);
     };
     auto iteration = [&]() {
         prof_begin(b, EQ_KERNEL_STENCIL64);
         k_ap<<<g0, kBlock, 0, s>>>(A, b->p, b->q, b->partial, b->d_state);
         prof_end(b, EQ_KERNEL_STENCIL64, fine_nodes * 2 * sizeof(double));
+        finish(kFinAp);
         prof_begin(b, EQ_KERNEL_CG_UPDATE);
         k_update<<<g0, kBlock, 0, s>>>(A, b->u, b->r, b->z, b->p, b->q, b->partial, b->d_state, jacobi ? 1 : 0);
         prof_end(b, EQ_KERNEL_CG_UPDATE, fine_nodes * sizeof(double) * (jacobi ? 7 : (b->mixed ? 7 : 6)));
+        finish(kFinUpdate);
         if (!jacobi) precondition();
         k_dir<<<g0, kBlock, 0, s>>>(A, b->z, b->p, b->d_state);
     };
 
     k_rhs<<<g0, kBlock, 0, s>>>(A, b->u, b->rhs, b->partial, b->d_state);
+    finish(kFinRhs);
     k_init<<<g0, kBlock, 0, s>>>(A, b->u, b->rhs, b->r, b->z, b->p, b->partial, b->d_state, jacobi ? 1 : 0);
+    finish(kFinInit);
     if (!jacobi) {
         precondition();
         k_dir<<<g0, kBlock, 0, s>>>(A, b->z, b->p, b->d_state);
@@ -1008,6 +1016,7 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
         if (all) {
             // confirm on the true residual; problems above rtol resume
             k_true_residual<<<g0, kBlock, 0, s>>>(A, b->u, b->rhs, b->r, b->partial, b->d_state);
+            finish(kFinTrue);
             if ((rc = read_states(b))) return rc;
             all = true;
             for (int p = 0; p < b->nprob; ++p) all = all && b->h_state[p].done;
@@ -1016,6 +1025,7 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
             if (jacobi) {
                 k_jacobi_z<<<g0, kBlock, 0, s>>>(A, b->r, b->z, b->d_state);
                 k_rz<<<g0, kBlock, 0, s>>>(A, b->r, b->z, b->partial, b->d_state);
+                finish(kFinRz);
             } else {
                 precondition();
             }
@@ -1044,7 +1054,11 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
         stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
     if (nconv < b->nprob)
-        return set_error(EQ_ERR_NOT_CONVERGED, "max_iter reached above rtol (relres " + std::to_string(relmax) + ")");
+        {
+        char msg[128];
+        std::snprintf(msg, sizeof(msg), "max_iter reached above rtol (relres %.3e)", relmax);
+        return set_error(EQ_ERR_NOT_CONVERGED, msg);
+    }
     return EQ_OK;
 }
 
@@ -1063,7 +1077,7 @@ void eq_solver_default(eq_solver_desc* s)
     if (!s) return;
     std::memset(s, 0, sizeof(*s));
     s->kind = EQ_SOLVER_MIXED_GMG_CG;
-    s->max_iter = 500;
+    s->max_iter = 20000;
     s->rtol = 1e-10;
     s->mg_pre = 2;
     s->mg_post = 2;
@@ -1130,6 +1144,12 @@ int eq_batch_create(const eq_field_desc* descs, int32_t nprob, int32_t nspecies,
             rc = set_error(EQ_ERR_NOMEM, "state allocation");
         } else if (hipMemcpy(b->d_state, st.data(), sizeof(CgState) * nprob, hipMemcpyHostToDevice) != hipSuccess) {
             rc = set_error(EQ_ERR_HIP, "state upload");
+        } else {
+            std::vector<double*> fp(nprob);
+            for (int p = 0; p < nprob; ++p) fp[p] = b->u + b->levels[0].probs[p].origin;
+            if (hipMalloc((void**)&b->d_field_ptrs, sizeof(double*) * nprob) != hipSuccess ||
+                hipMemcpy(b->d_field_ptrs, fp.data(), sizeof(double*) * nprob, hipMemcpyHostToDevice) != hipSuccess)
+                rc = set_error(EQ_ERR_NOMEM, "field table");
         }
     }
     if (rc != EQ_OK) {
@@ -1167,6 +1187,7 @@ void eq_batch_destroy(eq_batch* b)
     hipFree(b->dense);
     hipFree(b->d_state);
     hipFree(b->flux_out);
+    hipFree(b->d_field_ptrs);
     hipFree(b->cell_scratch);
     if (b->h_state) hipHostFree(b->h_state);
     if (b->h_flux) hipHostFree(b->h_flux);

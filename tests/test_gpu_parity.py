@@ -7,7 +7,9 @@ Tolerances:
   solves vs the oracle's direct LU and vs the fixtures: 1e-9 relative L2 per
     species per step (the solver stops at ||b-Ax||/||b|| <= 1e-10);
   coverage counts: exact; deposited fields: bit-identical (same per-node
-    addition order as the sequential loop); sampled means: 1e-14 relative.
+    addition order as the sequential loop); sampled means: n*eps relative
+    (the per-cell sum runs in another order), 1e-13 at ~30 nodes per cell,
+    2e-13 at ~800.
 """
 import math
 import os
@@ -140,7 +142,8 @@ def test_couple_bit_exact_against_sequential_loop(eqf, oracle):
     assert np.array_equal(counts, c_ref)
     for s in range(2):
         assert np.array_equal(got[s], ref[s])
-    assert np.allclose(sampled, s_ref, rtol=1e-14, atol=1e-300)
+    # sums of up to ~1000 terms in another order: |err| <= n*eps*sum|x|
+    assert np.allclose(sampled, s_ref, rtol=2e-13, atol=1e-300)
 
 
 def test_uneven_batch_matches_oracle(eqf, oracle):
