@@ -1,0 +1,208 @@
+// fhsl_gpu.hpp -- C++ host mirror of eQ's diffusion-solver surface over the
+// C ABI of libeq_field.so (include/eq_field.h).
+//
+// The reference host (src/simulation.cpp) creates one fenicsInterface per
+// HSL species (src/simulation.cpp:181-394) and touches these members:
+//   initDiffusion(params&)                       src/fHSL.cpp:37-53
+//   stepDiffusion()                              src/fHSL.cpp:98-161
+//   shell->mesh->num_vertices(), shell->mesh_coords,
+//   shell->dof_from_vertex                       src/simulation.cpp:298-308, src/fHSL.h:82-88
+//   D11 / D22 / D12                              src/simulation.cpp:311-313, 503-505
+//   solution_vector                              src/simulation.cpp:492, 501
+//   totalBoundaryFlux, writeDiffusionFiles, finalize
+// eqgpu::fenicsInterface keeps those names and meanings.  All species of a
+// trap share one device batch (eqgpu::trapField), so the solve of every
+// species is one set of launches, and the controller's cell loop
+// (eQabm::updateCells, src/abm/eQabm.cpp:234-449) becomes one call,
+// trapField::coupleCells.
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/eq_field.h"
+
+namespace eqgpu {
+
+inline void check(int rc, const char* what)
+{
+    if (rc != EQ_OK) throw std::runtime_error(std::string(what) + ": " + eq_last_error());
+}
+
+// eQ::diffusionSolver::params (src/eQ.h:305-321) without the MPI and JSON
+// handles, which a single-GPU process does not need
+struct solverParams {
+    size_t uniqueID = 0;
+    double dt = 0.1;
+    double D_HSL = 0.0;
+    std::string filePath;
+    double trapHeightMicrons = 0.0;
+    double trapWidthMicrons = 0.0;
+    double nodesPerMicron = 1.0;
+    double trapChannelVelocity = 0.0;
+};
+
+// One trap, all of its species, on the device.
+class trapField {
+public:
+    // walls in eQ::boundaryCondition::wall order: left, right, top, bottom
+    trapField(const solverParams& base, const std::vector<double>& D, const std::vector<double>& gamma,
+              const eq_bc walls[4], const eq_solver_desc* solver = nullptr)
+    {
+        // nodesW/H = ceil(W*npm) cells, +1 vertices (src/fHSL.cpp:240-243, 285-287)
+        nx_ = (int)std::ceil(base.trapWidthMicrons * base.nodesPerMicron);
+        ny_ = (int)std::ceil(base.trapHeightMicrons * base.nodesPerMicron);
+        npm_ = base.nodesPerMicron;
+        std::vector<eq_field_desc> descs(D.size());
+        for (size_t s = 0; s < D.size(); ++s) {
+            eq_field_desc& d = descs[s];
+            d.nx = nx_;
+            d.ny = ny_;
+            d.width = base.trapWidthMicrons;
+            d.height = base.trapHeightMicrons;
+            d.dt = base.dt;
+            d.D = D[s];
+            d.gamma = gamma.empty() ? 0.0 : gamma[s];
+            d.f = 0.0; // f is bound to Constant(0) in the reference (src/fHSL.h:201)
+            for (int w = 0; w < 4; ++w) d.wall[w] = walls[w];
+        }
+        eq_batch* b = nullptr;
+        check(eq_batch_create(descs.data(), (int32_t)descs.size(), (int32_t)descs.size(), solver, &b),
+              "eq_batch_create");
+        batch_.reset(b);
+    }
+
+    int32_t species() const { return eq_batch_num_problems(batch_.get()); }
+    size_t nodes() const { return (size_t)(nx_ + 1) * (size_t)(ny_ + 1); }
+    eq_batch* handle() const { return batch_.get(); }
+
+    void upload(int s, const std::vector<double>& u) { check(eq_batch_upload(batch_.get(), s, u.data(), u.size()), "upload"); }
+    void download(int s, std::vector<double>& u)
+    {
+        u.resize(nodes());
+        check(eq_batch_download(batch_.get(), s, u.data(), u.size()), "download");
+    }
+    eq_step_stats step()
+    {
+        eq_step_stats st{};
+        check(eq_batch_step(batch_.get(), &st), "eq_batch_step");
+        last_ = st;
+        return st;
+    }
+    // sampled and counts may be null; q holds nc x species amounts
+    void coupleCells(const std::vector<eq_cell>& cells, const std::vector<double>& q, std::vector<double>* sampled,
+                     std::vector<int32_t>* counts = nullptr)
+    {
+        const size_t nc = cells.size();
+        if (sampled) sampled->resize(nc * species());
+        if (counts) counts->resize(nc);
+        check(eq_batch_couple(batch_.get(), 0, cells.data(), nc, npm_, q.data(), sampled ? sampled->data() : nullptr,
+                              counts ? counts->data() : nullptr, EQ_COUPLE_SAMPLE | EQ_COUPLE_DEPOSIT),
+              "eq_batch_couple");
+    }
+    double boundaryFlux(int s)
+    {
+        double v = 0.0;
+        check(eq_batch_boundary_flux(batch_.get(), s, &v), "boundary flux");
+        return v;
+    }
+
+    // Per-species host vectors registered by the fenicsInterface views.  The
+    // first species to step in a time step uploads every registered vector,
+    // solves all species in one batch and downloads every result; the other
+    // species' calls of the same time step find their vector already solved.
+    void attach(int s, std::vector<double>* v)
+    {
+        if ((int)host_.size() <= s) {
+            host_.resize(s + 1, nullptr);
+            solved_.resize(s + 1, 0);
+        }
+        host_[s] = v;
+    }
+    void stepSpecies(int s)
+    {
+        if (!solved_[s]) {
+            for (size_t k = 0; k < host_.size(); ++k)
+                if (host_[k]) upload((int)k, *host_[k]);
+            step();
+            for (size_t k = 0; k < host_.size(); ++k)
+                if (host_[k]) {
+                    download((int)k, *host_[k]);
+                    solved_[k] = 1;
+                }
+        }
+        solved_[s] = 0;
+    }
+    const eq_step_stats& lastStats() const { return last_; }
+
+private:
+    std::vector<std::vector<double>*> host_;
+    std::vector<char> solved_;
+    eq_step_stats last_{};
+    struct deleter {
+        void operator()(eq_batch* b) const { eq_batch_destroy(b); }
+    };
+    std::unique_ptr<eq_batch, deleter> batch_;
+    int nx_ = 0, ny_ = 0;
+    double npm_ = 1.0;
+};
+
+// The per-species surface the reference host uses.
+class fenicsInterface {
+public:
+    struct meshView {
+        size_t n = 0;
+        size_t num_vertices() const { return n; }
+    };
+    struct shellView {
+        std::shared_ptr<meshView> mesh = std::make_shared<meshView>();
+        std::vector<double> mesh_coords;  // 2N, vertex order
+        std::vector<int> dof_from_vertex; // N (identity)
+    };
+
+    fenicsInterface(std::shared_ptr<trapField> field, int species) : field_(std::move(field)), s_(species) {}
+
+    void initDiffusion(const solverParams& p)
+    {
+        params_ = p;
+        const size_t n = field_->nodes();
+        shell = std::make_shared<shellView>();
+        shell->mesh->n = n;
+        shell->mesh_coords.resize(2 * n);
+        shell->dof_from_vertex.resize(n);
+        check(eq_batch_vertex_map(field_->handle(), s_, shell->mesh_coords.data(), shell->dof_from_vertex.data()),
+              "vertex map");
+        solution_vector.assign(n, 0.0);
+        field_->attach(s_, &solution_vector);
+        // isotropic default tensor (src/fHSL.cpp:313-323)
+        D11 = std::make_shared<std::vector<double>>(n, 1.0);
+        D22 = std::make_shared<std::vector<double>>(n, 1.0);
+        D12 = std::make_shared<std::vector<double>>(n, 0.0);
+    }
+
+    // u0 <- solution_vector; solve; solution_vector <- u; boundary flux
+    // (src/fHSL.cpp:104-108, 156-160).  Call once per species per time step.
+    void stepDiffusion()
+    {
+        field_->stepSpecies(s_);
+        totalBoundaryFlux = field_->boundaryFlux(s_);
+    }
+    void writeDiffusionFiles(double) {}
+    void finalize() {}
+
+    std::shared_ptr<shellView> shell;
+    std::vector<double> solution_vector;
+    std::shared_ptr<std::vector<double>> D11, D22, D12;
+    double totalBoundaryFlux = 0.0;
+
+private:
+    std::shared_ptr<trapField> field_;
+    int s_ = 0;
+    solverParams params_;
+};
+
+} // namespace eqgpu

@@ -1,0 +1,112 @@
+// fake_host.cpp -- a stand-in for eQ's C++ host driving the GPU field step
+// through eqgpu::fenicsInterface / eqgpu::trapField (eq_amd/host/fhsl_gpu.hpp)
+// in the order of Simulation::stepSimulation (src/simulation.cpp:414-510).
+//
+//   usage: fake_host <input.bin> <output.bin>
+//
+// input (little-endian): int32 mode, nx, ny, nspecies, ncells, nsteps;
+//   double W, H, npm, dt; double D[ns], gamma[ns]; 4 x {int32 kind, int32 pad,
+//   double value, r, s}; eq_cell cells[nc]; double q[nc*ns]; double u0[ns*N].
+// mode 0 (host vectors, the reference's MPI flow): each step the host adds
+//   q_c to node (c mod N) of species s's solution_vector, then calls
+//   stepDiffusion() on every species.
+// mode 1 (device resident): each step trapField::coupleCells, then one batch
+//   step for all species; the fields are downloaded at the end.
+// output: double fields[ns*N], double sampled[nc*ns] (mode 1, last step),
+//   double flux[ns], int32 iterations_max.
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <vector>
+
+#include "../../eq_amd/host/fhsl_gpu.hpp"
+
+template <typename T>
+static void rd(FILE* f, T* p, size_t n)
+{
+    if (fread(p, sizeof(T), n, f) != n) throw std::runtime_error("short input");
+}
+
+int main(int argc, char** argv)
+{
+    if (argc != 3) {
+        std::fprintf(stderr, "usage: fake_host <input.bin> <output.bin>\n");
+        return 2;
+    }
+    try {
+        FILE* in = std::fopen(argv[1], "rb");
+        if (!in) throw std::runtime_error("cannot open input");
+        int32_t hdr[6];
+        rd(in, hdr, 6);
+        const int mode = hdr[0], nx = hdr[1], ny = hdr[2], ns = hdr[3], nc = hdr[4], nsteps = hdr[5];
+        double g4[4];
+        rd(in, g4, 4);
+        std::vector<double> D(ns), gamma(ns);
+        rd(in, D.data(), ns);
+        rd(in, gamma.data(), ns);
+        eq_bc walls[4];
+        rd(in, walls, 4);
+        std::vector<eq_cell> cells(nc);
+        rd(in, cells.data(), nc);
+        std::vector<double> q((size_t)nc * ns);
+        rd(in, q.data(), q.size());
+        const size_t N = (size_t)(nx + 1) * (ny + 1);
+        std::vector<std::vector<double>> u0(ns, std::vector<double>(N));
+        for (int s = 0; s < ns; ++s) rd(in, u0[s].data(), N);
+        std::fclose(in);
+
+        eqgpu::solverParams p;
+        p.trapWidthMicrons = g4[0];
+        p.trapHeightMicrons = g4[1];
+        p.nodesPerMicron = g4[2];
+        p.dt = g4[3];
+        auto field = std::make_shared<eqgpu::trapField>(p, D, gamma, walls);
+        if ((size_t)field->nodes() != N) throw std::runtime_error("grid size mismatch");
+        std::vector<eqgpu::fenicsInterface> species;
+        species.reserve(ns);
+        for (int s = 0; s < ns; ++s) {
+            species.emplace_back(field, s);
+            species.back().initDiffusion(p);
+            if (species.back().shell->mesh->num_vertices() != N) throw std::runtime_error("num_vertices");
+            species.back().solution_vector = u0[s];
+        }
+        std::vector<double> sampled;
+        if (mode == 1)
+            for (int s = 0; s < ns; ++s) field->upload(s, u0[s]);
+        for (int k = 0; k < nsteps; ++k) {
+            if (mode == 0) {
+                // controller: updateCells writes into its copy of each field,
+                // then the copies travel to the solvers (src/simulation.cpp:447-463)
+                for (int c = 0; c < nc; ++c)
+                    for (int s = 0; s < ns; ++s) species[s].solution_vector[(size_t)c % N] += q[(size_t)c * ns + s];
+                for (int s = 0; s < ns; ++s) species[s].stepDiffusion();
+            } else {
+                field->coupleCells(cells, q, &sampled);
+                field->step();
+            }
+        }
+        std::vector<double> out;
+        for (int s = 0; s < ns; ++s) {
+            std::vector<double> u;
+            if (mode == 1) field->download(s, u);
+            else u = species[s].solution_vector;
+            out.insert(out.end(), u.begin(), u.end());
+        }
+        FILE* fo = std::fopen(argv[2], "wb");
+        if (!fo) throw std::runtime_error("cannot open output");
+        std::fwrite(out.data(), sizeof(double), out.size(), fo);
+        sampled.resize((size_t)nc * ns, 0.0);
+        std::fwrite(sampled.data(), sizeof(double), sampled.size(), fo);
+        for (int s = 0; s < ns; ++s) {
+            const double fl = field->boundaryFlux(s);
+            std::fwrite(&fl, sizeof(double), 1, fo);
+        }
+        const int32_t it = field->lastStats().iterations_max;
+        std::fwrite(&it, sizeof(int32_t), 1, fo);
+        std::fclose(fo);
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "fake_host: %s\n", e.what());
+        return 1;
+    }
+    return 0;
+}

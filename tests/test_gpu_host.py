@@ -1,0 +1,74 @@
+"""The C++ fake-host harness (tests/host/fake_host.cpp) drives the GPU field
+step through the C++ fenicsInterface mirror (eq_amd/host/fhsl_gpu.hpp) in the
+order of Simulation::stepSimulation; its results must match the oracle."""
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "tests", "host", "bin", "fake_host")
+
+pytestmark = pytest.mark.gpu
+
+
+def _write_input(path, mode, t, nsteps, q, walls):
+    ns = len(t.D)
+    N = t.nodes
+    with open(path, "wb") as f:
+        f.write(struct.pack("<6i", mode, t.nx, t.ny, ns, t.cells.shape[0], nsteps))
+        f.write(struct.pack("<4d", t.width, t.height, t.npm, 0.1))
+        f.write(np.asarray(t.D, np.float64).tobytes())
+        f.write(np.asarray(t.gamma, np.float64).tobytes())
+        for w in range(4):
+            kind, value, r, s = walls.get(w, (0, 0.0, 0.0, 0.0))
+            f.write(struct.pack("<ii3d", kind, 0, value, r, s))
+        f.write(np.ascontiguousarray(t.cells, np.float64).tobytes())
+        f.write(np.ascontiguousarray(q, np.float64).tobytes())
+        for u in t.u0:
+            f.write(np.ascontiguousarray(u, np.float64).tobytes())
+    return N
+
+
+def _read_output(path, ns, N, nc):
+    raw = open(path, "rb").read()
+    fields = np.frombuffer(raw, np.float64, ns * N).reshape(ns, N)
+    off = ns * N * 8
+    sampled = np.frombuffer(raw, np.float64, nc * ns, offset=off).reshape(nc, ns)
+    off += nc * ns * 8
+    flux = np.frombuffer(raw, np.float64, ns, offset=off)
+    return fields, sampled, flux
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_fake_host_matches_oracle(oracle, tmp_path, mode):
+    from eq_amd import workloads as W
+    assert os.path.exists(EXE), "build with __graft_entry__.build()"
+    t = W.config0()
+    walls = W.TOP_DIRICHLET
+    q = t.rates * 0.1
+    nsteps = 3
+    inp, outp = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    N = _write_input(inp, mode, t, nsteps, q, walls)
+    print(f"fake_host mode={mode} nsteps={nsteps}", flush=True)
+    subprocess.run([EXE, inp, outp], check=True, timeout=120)
+    fields, sampled, flux = _read_output(outp, 2, N, t.cells.shape[0])
+    probs = [oracle.problem(t.nx, t.ny, t.width, t.height, 0.1, D, g, 0.0, walls=walls) for D, g in zip(t.D, t.gamma)]
+    grid = oracle.grid(t.nx + 1, t.ny + 1, t.npm, t.width, t.height)
+    ref = [u.copy() for u in t.u0]
+    for _ in range(nsteps):
+        if mode == 0:
+            for c in range(t.cells.shape[0]):
+                for s in range(2):
+                    ref[s][c % N] += q[c, s]
+        else:
+            ref, s_ref, _ = oracle.couple(grid, t.cells, ref, q)
+        ref = [oracle.step(probs[s], ref[s], "lu")[0] for s in range(2)]
+    for s in range(2):
+        err = np.linalg.norm(fields[s] - ref[s]) / np.linalg.norm(ref[s])
+        assert err <= 1e-9, (mode, s, err)
+        assert abs(flux[s] - 0.1 * t.D[s] * oracle.boundary_flux(probs[s], fields[s])) <= 1e-10 * (1 + abs(flux[s]))
+    if mode == 1:
+        assert np.allclose(sampled, s_ref, rtol=1e-9)
