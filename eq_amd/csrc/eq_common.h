@@ -54,6 +54,10 @@ struct LevelProb {
     int32_t cidx;          // index of the problem in the next level's table (-1 if none)
     int32_t nxc, nyc;      // next level's cells (0 if none)
     int32_t fidx;          // index of the problem in the previous (finer) level's table (-1 on level 0)
+    int32_t bottom;        // 1: from this level down, one workgroup runs the V-cycle
+    int32_t below;         // 1: level lies under the problem's bottom level (no grid launches)
+    int32_t lds_off;       // bottom levels: element offset of this level's LDS arrays
+    int32_t lds_total;     // bottom level: LDS elements the problem's bottom V-cycle needs
     // left-hand side (all levels)
     double mh, mo;         // (1+dt*gamma)*hx*hy/12 and /24: mass per triangle
     double sa, sb;         // dt*D*hy/(2*hx), dt*D*hx/(2*hy): stiffness per triangle
@@ -175,31 +179,26 @@ EQ_HD inline double rhs_row(const LevelProb& P, int i, int j, double uc, double 
     return b;
 }
 
-// Coarse-to-fine P1 interpolation on the "right" triangulation of the coarse
-// level, evaluated at a fine node.  Coordinates are exact rationals:
-// s = j*nxc/nxf  (so nested even levels are the exact finite-element
-// embedding and odd levels fall back to non-nested P1 interpolation).
-struct Interp {
-    int32_t I0, J0;        // coarse cell
-    double wbl, wbr, wtl, wtr;
-};
-
-EQ_HD inline Interp interp_weights(int i, int j, int nxf, int nyf, int nxc, int nyc)
+// Coarse-to-fine transfer: P1 interpolation on the coarse level's "right"
+// triangulation at fine node coordinates s = j*nxc/nxf, t = i*nyc/nyf (in
+// coarse cells).  The hat function of coarse node (I,J) on that
+// triangulation is
+//     phi(u, v) = max(0, 1 - max(|u|, |v|, |u - v|)),   u = s - J, v = t - I,
+// whose support is the hexagon with corners (+-1,0), (0,+-1), (1,1), (-1,-1).
+// Nested (even) levels give the exact finite-element embedding; odd levels
+// are non-nested.  Restriction gathers the same weights, so R = P^T exactly.
+template <typename T>
+EQ_HD inline T hat(int du_num, int dv_num, T inx, T iny)
 {
-    int64_t sj = (int64_t)j * nxc, si = (int64_t)i * nyc;
-    int32_t J0 = (int32_t)(sj / nxf), I0 = (int32_t)(si / nyf);
-    int64_t ra = sj - (int64_t)J0 * nxf, rb = si - (int64_t)I0 * nyf;
-    if (J0 >= nxc) { J0 = nxc - 1; ra = nxf; }
-    if (I0 >= nyc) { I0 = nyc - 1; rb = nyf; }
-    const double a = (double)ra / (double)nxf, b = (double)rb / (double)nyf;
-    Interp w;
-    w.I0 = I0; w.J0 = J0;
-    if (ra * (int64_t)nyf >= rb * (int64_t)nxf) { // a >= b: lower-right triangle (BL,BR,TR)
-        w.wbl = 1.0 - a; w.wbr = a - b; w.wtl = 0.0; w.wtr = b;
-    } else {                                     // upper-left triangle (BL,TL,TR)
-        w.wbl = 1.0 - b; w.wtl = b - a; w.wbr = 0.0; w.wtr = a;
-    }
-    return w;
+    const T u = (T)du_num * inx, v = (T)dv_num * iny;
+    const T au = u < (T)0 ? -u : u, av = v < (T)0 ? -v : v, d = u - v, ad = d < (T)0 ? -d : d;
+    T m = au > av ? au : av;
+    m = m > ad ? m : ad;
+    const T w = (T)1 - m;
+    return w > (T)0 ? w : (T)0;
 }
+
+// floor(a / b) for b > 0 (a may be negative)
+EQ_HD inline int floor_div(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
 
 } // namespace eqk

@@ -28,22 +28,27 @@ struct CgState {
     double rz, pq, rr, bb, alpha, beta, rr_true, rzold;
     double tol2;            // rtol^2 (times bb when compared)
     int32_t iter, done, max_iter, flexible;
-    int32_t restart, pad_;  // restart: next beta is 0 (after a residual replacement)
+    int32_t restart;        // next beta is 0 (after a residual replacement)
+    int32_t have_prev;      // the previous solution is kept: start CG from it
 };
 
 struct Level {
     std::vector<LevelProb> probs;     // host copy
     std::vector<TileRef> tiles_all;   // every problem at this level
-    std::vector<TileRef> tiles_smooth;// problems that are not coarsest here
-    std::vector<int32_t> coarse;      // problems whose coarsest level is this one
+    std::vector<TileRef> tiles_smooth;// problems smoothed here by grid launches (above their bottom level)
+    std::vector<TileRef> tiles_bottom;// problems whose bottom level is this one
+    std::vector<TileRef> tiles_active;// problems at or above their bottom level (restriction targets)
+    std::vector<int32_t> bottom;      // problems whose bottom level is this one
     LevelProb* d_probs = nullptr;
     TileRef* d_tiles_all = nullptr;
     TileRef* d_tiles_smooth = nullptr;
-    int32_t* d_coarse = nullptr;
+    TileRef* d_tiles_bottom = nullptr;
+    TileRef* d_tiles_active = nullptr;
+    int32_t* d_bottom = nullptr;
     int64_t pool = 0;                 // elements per vector on this level
     double nodes_all = 0.0;           // grid nodes summed over the level's problems
     double nodes_smooth = 0.0;        // ... over the problems that smooth here
-    int64_t max_dense = 0;            // largest coarse problem (nodes)
+    size_t bottom_lds_bytes = 0;      // dynamic LDS of the bottom V-cycle launched from this level
     // multigrid vectors (element type = smoother precision)
     void* b = nullptr;  // level >= 1, or level 0 FP32 copy of r
     void* xa = nullptr;
@@ -51,6 +56,8 @@ struct Level {
     void* r = nullptr;
     LevelView all() const { return LevelView{d_probs, d_tiles_all, (int32_t)tiles_all.size()}; }
     LevelView smooth() const { return LevelView{d_probs, d_tiles_smooth, (int32_t)tiles_smooth.size()}; }
+    LevelView bottom_tiles() const { return LevelView{d_probs, d_tiles_bottom, (int32_t)tiles_bottom.size()}; }
+    LevelView active() const { return LevelView{d_probs, d_tiles_active, (int32_t)tiles_active.size()}; }
 };
 
 struct ProfileSlot {
@@ -72,19 +79,26 @@ struct eq_batch {
     std::vector<eq_field_desc> descs;
     eq_solver_desc solver{};
     int32_t mixed = 0;                 // 1: FP32 V-cycle
+    int64_t lds_budget = 64 * 1024;    // LDS bytes the bottom V-cycle may use per workgroup
     std::vector<eqk::Level> levels;
     // level 0 FP64 Krylov vectors (pooled over problems)
     double* u = nullptr;   // field / iterate x
     double* rhs = nullptr;
     double* r = nullptr;
-    double* p = nullptr;
+    double* p = nullptr;   // search direction of the previous iteration
+    double* p2 = nullptr;  // search direction being formed (swapped with p every iteration)
     double* q = nullptr;
     double* z = nullptr;
+    double* xprev = nullptr;    // last solution, the next solve's starting iterate
     double* partial = nullptr;  // 4 slots per level-0 tile
     void* dense = nullptr;      // dense inverses of coarsest levels (smoother precision)
     eqk::CgState* d_state = nullptr;
     eqk::CgState* h_state = nullptr; // pinned mirror
     int32_t last_iters = 0;
+    // captured launch sequences (created on first use): two CG iterations
+    // (the search-direction buffers swap back after two), and the set-up
+    hipGraphExec_t g_iter2 = nullptr;
+    hipGraphExec_t g_init = nullptr;
     std::vector<int32_t> iters;
     std::vector<double> relres;
     // cell coupling scratch (grown on demand)

@@ -29,6 +29,11 @@
 
 namespace eqk {
 
+constexpr int kMaxLev = 16;          // multigrid levels per problem
+constexpr int64_t kBottomLdsMax = 150 * 1024; // cap on the LDS bytes of the levels one workgroup runs
+constexpr int kBottomThreads = 1024;
+constexpr int kFinishThreads = 1024;
+
 // ------------------------------------------------------------ errors
 
 static thread_local std::string g_last_error;
@@ -59,7 +64,9 @@ inline int grid_for(int ntiles) { return 8 * ((ntiles + 7) / 8); }
 
 struct NodeCtx {
     const LevelProb* P;
-    int t, pi, i, j;
+    int t;      // index in the launched tile table
+    int gt;     // tile index in the level's full table (partial sums)
+    int pi, i, j;
     int64_t idx;
     bool in;
 };
@@ -71,6 +78,7 @@ __device__ __forceinline__ bool decode(const LevelView& L, NodeCtx& c)
     const TileRef tr = L.tiles[c.t];
     c.pi = tr.pi;
     c.P = &L.probs[tr.pi];
+    c.gt = c.P->tile_begin + tr.local;
     const int ty = tr.local / c.P->tiles_x, tx = tr.local - ty * c.P->tiles_x;
     c.j = tx * kTileW + (int)(threadIdx.x & (kTileW - 1));
     c.i = ty * kTileH + (int)(threadIdx.x / kTileW);
@@ -86,11 +94,11 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-// deterministic block sum; result valid in thread 0
-template <int NV>
+// deterministic block sum over NT threads; result valid in thread 0
+template <int NV, int NT>
 __device__ __forceinline__ void block_sum(double (&v)[NV])
 {
-    __shared__ double red[kBlock / 64][NV];
+    __shared__ double red[NT / 64][NV];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
@@ -102,7 +110,7 @@ __device__ __forceinline__ void block_sum(double (&v)[NV])
 #pragma unroll
         for (int k = 0; k < NV; ++k) {
             double s = 0.0;
-            for (int q = 0; q < (int)(blockDim.x / 64); ++q) s += red[q][k];
+            for (int q = 0; q < NT / 64; ++q) s += red[q][k];
             v[k] = s;
         }
     __syncthreads();
@@ -113,12 +121,12 @@ __device__ __forceinline__ void block_sum(double (&v)[NV])
 // partials visible across XCDs, where an in-kernel device-scope fence per
 // block would write back the XCD's whole L2 every time.
 template <int NV>
-__device__ __forceinline__ void tile_partial(double (&v)[NV], double* partial, int t)
+__device__ __forceinline__ void tile_partial(double (&v)[NV], double* partial, int gt)
 {
-    block_sum<NV>(v);
+    block_sum<NV, kBlock>(v);
     if (threadIdx.x == 0)
 #pragma unroll
-        for (int k = 0; k < NV; ++k) partial[(int64_t)t * 4 + k] = v[k];
+        for (int k = 0; k < NV; ++k) partial[(int64_t)gt * 4 + k] = v[k];
 }
 
 enum FinishOp { kFinRhs = 0, kFinInit = 1, kFinAp = 2, kFinUpdate = 3, kFinRz = 4, kFinTrue = 5 };
@@ -136,11 +144,11 @@ __device__ __forceinline__ void load7(const T* __restrict__ v, int64_t idx, int 
     ne = v[idx + pitch + 1];
 }
 
-// ------------------------------------------------------------ level-0 kernels (FP64)
+// ------------------------------------------------------------ level-0 Krylov kernels (FP64)
 
 // b = M u0 + sources; ||b||^2 of the row-replaced system (b_D = g)
 __global__ __launch_bounds__(kBlock) void k_rhs(LevelView L, const double* __restrict__ u, double* __restrict__ b,
-                                                double* partial, CgState* st)
+                                                double* partial)
 {
     NodeCtx c;
     if (!decode(L, c)) return;
@@ -158,34 +166,41 @@ __global__ __launch_bounds__(kBlock) void k_rhs(LevelView L, const double* __res
             v[0] = bv * bv;
         }
     }
-    tile_partial<1>(v, partial, c.t);
+    tile_partial<1>(v, partial, c.gt);
 }
 
-// x_D = g; r = b - A x; Jacobi: z = p = r / diag(A)
-__global__ __launch_bounds__(kBlock) void k_init(LevelView L, double* x, const double* __restrict__ b,
-                                                 double* __restrict__ r, double* __restrict__ z,
-                                                 double* __restrict__ p, double* partial, CgState* st, int jacobi)
+// x_D = g; r = b - A x; Jacobi: z = r / diag(A), r.z.  The iterate starts
+// from the previous step's solution when it is kept (xprev), else from the
+// field itself: the field carries this step's cell deposits, whose sharp
+// edges would otherwise make the first residual ~1e4 times larger.
+__global__ __launch_bounds__(kBlock) void k_init(LevelView L, double* x, const double* __restrict__ xprev,
+                                                 const double* __restrict__ b, double* __restrict__ r,
+                                                 double* __restrict__ z, double* partial, const CgState* st,
+                                                 int jacobi)
 {
     NodeCtx c;
     if (!decode(L, c)) return;
     const LevelProb& P = *c.P;
+    const bool prev = st[P.prob].have_prev != 0;
+    const double* __restrict__ src = prev ? xprev : x;
     double v[2] = {0.0, 0.0};
     if (c.in) {
         if (is_dirichlet(P, c.i, c.j)) {
             x[c.idx] = dirichlet_value(P, c.i, c.j);
             r[c.idx] = 0.0;
-            z[c.idx] = 0.0;
-            p[c.idx] = 0.0;
+            if (jacobi) z[c.idx] = 0.0;
         } else {
             const int i = c.i, j = c.j, pt = P.pitch;
             const int64_t k = c.idx;
+            // Dirichlet neighbours take their wall value (x is being set to g there)
             auto val = [&](int ii, int jj, int64_t kk) -> double {
-                return is_dirichlet(P, ii, jj) ? dirichlet_value(P, ii, jj) : x[kk];
+                return is_dirichlet(P, ii, jj) ? dirichlet_value(P, ii, jj) : src[kk];
             };
-            const double xc = x[k];
+            const double xc = src[k];
             const double xw = val(i, j - 1, k - 1), xe = val(i, j + 1, k + 1);
             const double xs = val(i - 1, j, k - pt), xn = val(i + 1, j, k + pt);
             const double xsw = val(i - 1, j - 1, k - pt - 1), xne = val(i + 1, j + 1, k + pt + 1);
+            if (prev) x[k] = xc;
             const Row<double> rc = row_coefs<double>(P, i, j);
             const double rv = b[k] - row_apply(rc, xc, xw, xe, xs, xn, xsw, xne);
             r[k] = rv;
@@ -193,41 +208,62 @@ __global__ __launch_bounds__(kBlock) void k_init(LevelView L, double* x, const d
             if (jacobi) {
                 const double zv = rv / row_diag(rc);
                 z[k] = zv;
-                p[k] = zv;
                 v[1] = rv * zv;
             }
         }
     }
-    tile_partial<2>(v, partial, c.t);
+    tile_partial<2>(v, partial, c.gt);
 }
 
-// q = A p; alpha = rz / (p.q)
-__global__ __launch_bounds__(kBlock) void k_ap(LevelView L, const double* __restrict__ p, double* __restrict__ q,
-                                               double* partial, CgState* st)
+// keep the solution as the next step's starting iterate
+__global__ __launch_bounds__(kBlock) void k_keep(LevelView L, const double* __restrict__ x,
+                                                 double* __restrict__ xprev, CgState* st)
+{
+    NodeCtx c;
+    if (!decode(L, c)) return;
+    if (c.in) xprev[c.idx] = x[c.idx];
+    if (c.gt == c.P->tile_begin && threadIdx.x == 0) st[c.P->prob].have_prev = 1;
+}
+
+// p_new = z + beta p_old (formed at the seven stencil points and rounded to
+// the storage type TP), q = A p_new in FP64 from the stored values, p_new.q.
+// q is exactly A times the direction x will move along, so the recurrence
+// residual tracks b - A x whatever TP is.  p_old and p_new are different buffers.
+template <typename TP>
+__global__ __launch_bounds__(kBlock) void k_ap_dir(LevelView L, const TP* __restrict__ z, const TP* __restrict__ pold,
+                                                   TP* __restrict__ pnew, double* __restrict__ q, double* partial,
+                                                   const CgState* st)
 {
     NodeCtx c;
     if (!decode(L, c)) return;
     const LevelProb& P = *c.P;
-    if (st[P.prob].done) return;
+    const CgState& s = st[P.prob];
+    if (s.done) return;
+    const double beta = s.beta;
     double v[1] = {0.0};
     if (c.in) {
-        double pc, pw, pe, ps, pn, psw, pne;
-        load7(p, c.idx, P.pitch, pc, pw, pe, ps, pn, psw, pne);
+        TP zc, zw, ze, zs, zn, zsw, zne, pc, pw, pe, ps, pn, psw, pne;
+        load7(z, c.idx, P.pitch, zc, zw, ze, zs, zn, zsw, zne);
+        load7(pold, c.idx, P.pitch, pc, pw, pe, ps, pn, psw, pne);
+        auto dir = [beta](TP zz, TP pp) -> double { return (double)(TP)((double)zz + beta * (double)pp); };
+        const double nc = dir(zc, pc);
         double qv = 0.0;
         if (!is_dirichlet(P, c.i, c.j)) {
             const Row<double> rc = row_coefs<double>(P, c.i, c.j);
-            qv = row_apply(rc, pc, pw, pe, ps, pn, psw, pne);
+            qv = row_apply(rc, nc, dir(zw, pw), dir(ze, pe), dir(zs, ps), dir(zn, pn), dir(zsw, psw), dir(zne, pne));
         }
+        pnew[c.idx] = (TP)nc;
         q[c.idx] = qv;
-        v[0] = pc * qv;
+        v[0] = nc * qv;
     }
-    tile_partial<1>(v, partial, c.t);
+    tile_partial<1>(v, partial, c.gt);
 }
 
-// x += alpha p; r -= alpha q; Jacobi: z = r/diag, beta; flexible: r.z_old
+// x += alpha p; r -= alpha q; Jacobi: z = r/diag, r.z; flexible: r.z_old
+template <typename TP>
 __global__ __launch_bounds__(kBlock) void k_update(LevelView L, double* __restrict__ x, double* __restrict__ r,
-                                                   double* __restrict__ z, const double* __restrict__ p,
-                                                   const double* __restrict__ q, double* partial, CgState* st,
+                                                   TP* __restrict__ z, const TP* __restrict__ p,
+                                                   const double* __restrict__ q, double* partial, const CgState* st,
                                                    int jacobi)
 {
     NodeCtx c;
@@ -240,52 +276,26 @@ __global__ __launch_bounds__(kBlock) void k_update(LevelView L, double* __restri
     double v[3] = {0.0, 0.0, 0.0};
     if (c.in && !is_dirichlet(P, c.i, c.j)) {
         const int64_t k = c.idx;
-        x[k] += alpha * p[k];
+        x[k] += alpha * (double)p[k];
         const double rv = r[k] - alpha * q[k];
         r[k] = rv;
         v[0] = rv * rv;
         if (jacobi) {
             const Row<double> rc = row_coefs<double>(P, c.i, c.j);
-            const double zv = rv / row_diag(rc);
+            const TP zv = (TP)(rv / row_diag(rc));
             z[k] = zv;
-            v[1] = rv * zv;
+            v[1] = rv * (double)zv;
         } else if (flexible) {
-            v[2] = rv * z[k];
+            v[2] = rv * (double)z[k];
         }
     }
-    tile_partial<3>(v, partial, c.t);
-}
-
-// multigrid path: rz = r.z (after the V-cycle), beta (Fletcher-Reeves or
-// Polak-Ribiere for a variable preconditioner)
-__global__ __launch_bounds__(kBlock) void k_rz(LevelView L, const double* __restrict__ r,
-                                               const double* __restrict__ z, double* partial, CgState* st)
-{
-    NodeCtx c;
-    if (!decode(L, c)) return;
-    const LevelProb& P = *c.P;
-    if (st[P.prob].done) return;
-    double v[1] = {0.0};
-    if (c.in) v[0] = r[c.idx] * z[c.idx];
-    tile_partial<1>(v, partial, c.t);
-}
-
-// p = z + beta p
-__global__ __launch_bounds__(kBlock) void k_dir(LevelView L, const double* __restrict__ z, double* __restrict__ p,
-                                                const CgState* st)
-{
-    NodeCtx c;
-    if (!decode(L, c)) return;
-    const LevelProb& P = *c.P;
-    const CgState& s = st[P.prob];
-    if (s.done) return;
-    if (c.in) p[c.idx] = z[c.idx] + s.beta * p[c.idx];
+    tile_partial<3>(v, partial, c.gt);
 }
 
 // true residual r = b - A x (replaces the recurrence residual); final stop test
 __global__ __launch_bounds__(kBlock) void k_true_residual(LevelView L, const double* __restrict__ x,
                                                           const double* __restrict__ b, double* __restrict__ r,
-                                                          double* partial, CgState* st)
+                                                          double* partial)
 {
     NodeCtx c;
     if (!decode(L, c)) return;
@@ -299,27 +309,59 @@ __global__ __launch_bounds__(kBlock) void k_true_residual(LevelView L, const dou
         r[c.idx] = rv;
         v[0] = rv * rv;
     }
-    tile_partial<1>(v, partial, c.t);
+    tile_partial<1>(v, partial, c.gt);
+}
+
+// Jacobi preconditioner on its own (restart after a residual replacement): z = r/diag, r.z
+__global__ __launch_bounds__(kBlock) void k_jacobi_z(LevelView L, const double* __restrict__ r,
+                                                     double* __restrict__ z, double* partial, const CgState* st)
+{
+    NodeCtx c;
+    if (!decode(L, c)) return;
+    const LevelProb& P = *c.P;
+    if (st[P.prob].done) return;
+    double v[1] = {0.0};
+    if (c.in) {
+        double zv = 0.0;
+        if (!is_dirichlet(P, c.i, c.j)) {
+            const Row<double> rc = row_coefs<double>(P, c.i, c.j);
+            zv = r[c.idx] / row_diag(rc);
+        }
+        z[c.idx] = zv;
+        v[0] = r[c.idx] * zv;
+    }
+    tile_partial<1>(v, partial, c.gt);
 }
 
 // Per-problem totals of the tile partials, summed in tile order
 // (deterministic), and the Krylov scalar updates that follow from them.
 // One block per problem; problems the producer skipped (done) are skipped.
-__global__ __launch_bounds__(kBlock) void k_finish(const LevelProb* __restrict__ probs,
-                                                   const double* __restrict__ partial, CgState* st, int op,
-                                                   int jacobi)
+__global__ __launch_bounds__(kFinishThreads) void k_finish(const LevelProb* __restrict__ probs,
+                                                           const double* __restrict__ partial, CgState* st, int op,
+                                                           int jacobi)
 {
     const LevelProb& P = probs[blockIdx.x];
     CgState& s = st[P.prob];
     if ((op == kFinAp || op == kFinUpdate || op == kFinRz) && s.done) return;
     double v[3] = {0.0, 0.0, 0.0};
-    for (int q = threadIdx.x; q < P.ntiles; q += blockDim.x) {
-        const double* pp = partial + (int64_t)(P.tile_begin + q) * 4;
-        v[0] += pp[0];
-        v[1] += pp[1];
-        v[2] += pp[2];
+    const double* base = partial + (int64_t)P.tile_begin * 4;
+    int q = threadIdx.x;
+    for (; q + 3 * kFinishThreads < P.ntiles; q += 4 * kFinishThreads) {
+        const double* p0 = base + (int64_t)q * 4;
+        const double* p1 = p0 + 4 * kFinishThreads;
+        const double* p2 = p1 + 4 * kFinishThreads;
+        const double* p3 = p2 + 4 * kFinishThreads;
+        v[0] += (p0[0] + p1[0]) + (p2[0] + p3[0]);
+        v[1] += (p0[1] + p1[1]) + (p2[1] + p3[1]);
+        v[2] += (p0[2] + p1[2]) + (p2[2] + p3[2]);
     }
-    block_sum<3>(v);
+    for (; q < P.ntiles; q += kFinishThreads) {
+        const double* p0 = base + (int64_t)q * 4;
+        v[0] += p0[0];
+        v[1] += p0[1];
+        v[2] += p0[2];
+    }
+    block_sum<3, kFinishThreads>(v);
     if (threadIdx.x != 0) return;
     switch (op) {
     case kFinRhs:
@@ -371,55 +413,102 @@ __global__ __launch_bounds__(kBlock) void k_finish(const LevelProb* __restrict__
     }
 }
 
-// Jacobi preconditioner on its own (restart after a residual replacement)
-__global__ __launch_bounds__(kBlock) void k_jacobi_z(LevelView L, const double* __restrict__ r,
-                                                     double* __restrict__ z, const CgState* st)
+// ------------------------------------------------------------ multigrid node operations
+// Each takes the array base, the element index k of node (i,j) and the row
+// pitch, so the same code runs on the pitched HBM pools and on the compact
+// LDS copies of the bottom levels.
+
+// damped Jacobi value at a free node: x + omega (b - A x) / diag  (x = 0 when zero_init)
+template <typename T, typename TB>
+__device__ __forceinline__ T smooth_val(const LevelProb& P, int i, int j, int64_t k, int pitch,
+                                        const TB* __restrict__ b, const T* __restrict__ xin, T omega,
+                                        bool zero_init, T& bv)
 {
-    NodeCtx c;
-    if (!decode(L, c)) return;
-    const LevelProb& P = *c.P;
-    if (st[P.prob].done) return;
-    if (!c.in) return;
-    if (is_dirichlet(P, c.i, c.j)) {
-        z[c.idx] = 0.0;
-        return;
-    }
-    const Row<double> rc = row_coefs<double>(P, c.i, c.j);
-    z[c.idx] = r[c.idx] / row_diag(rc);
+    const Row<T> rc = row_coefs<T>(P, i, j);
+    const T d = row_diag(rc);
+    bv = (T)b[k];
+    if (zero_init) return omega * bv / d;
+    T xc, xw, xe, xs, xn, xsw, xne;
+    load7(xin, k, pitch, xc, xw, xe, xs, xn, xsw, xne);
+    return xc + omega * (bv - row_apply(rc, xc, xw, xe, xs, xn, xsw, xne)) / d;
 }
 
-// ------------------------------------------------------------ multigrid kernels
+template <typename T>
+__device__ __forceinline__ T residual_val(const LevelProb& P, int i, int j, int64_t k, int pitch,
+                                          const T* __restrict__ b, const T* __restrict__ x)
+{
+    T xc, xw, xe, xs, xn, xsw, xne;
+    load7(x, k, pitch, xc, xw, xe, xs, xn, xsw, xne);
+    const Row<T> rc = row_coefs<T>(P, i, j);
+    return b[k] - row_apply(rc, xc, xw, xe, xs, xn, xsw, xne);
+}
 
-// Damped Jacobi sweep  x_out = x_in + omega (b - A x_in) / diag  (x_in = 0
-// when zero_init).  bcopy (optional) keeps b in the smoother precision.
+// b_c(I,J) = sum_f phi_IJ(f) r_f over the fine nodes within one coarse cell of (I,J);
+// fine node (i,j) lives at rf[forg + i*fpitch + j]
+template <typename T>
+__device__ __forceinline__ T restrict_val(const LevelProb& Pc, const LevelProb& Pf, int I, int J,
+                                          const T* __restrict__ rf, int64_t forg, int fpitch)
+{
+    const int nxf = Pf.nx, nyf = Pf.ny, nxc = Pc.nx, nyc = Pc.ny;
+    const int jlo = max(0, floor_div((J - 1) * nxf, nxc) + 1);
+    const int jhi = min(nxf, floor_div((J + 1) * nxf - 1, nxc));
+    const int ilo = max(0, floor_div((I - 1) * nyf, nyc) + 1);
+    const int ihi = min(nyf, floor_div((I + 1) * nyf - 1, nyc));
+    const T inx = (T)1 / (T)nxf, iny = (T)1 / (T)nyf;
+    T sum = (T)0;
+    for (int i = ilo; i <= ihi; ++i) {
+        const T* row = rf + forg + (int64_t)i * fpitch;
+        const int dv = i * nyc - I * nyf;
+        for (int j = jlo; j <= jhi; ++j) sum += hat<T>(j * nxc - J * nxf, dv, inx, iny) * row[j];
+    }
+    return sum;
+}
+
+// (P x_c) at fine node (i,j): the four corners of the coarse cell holding it;
+// coarse node (I,J) lives at xc[corg + I*cpitch + J]
+template <typename T>
+__device__ __forceinline__ T prolong_val(const LevelProb& Pf, const LevelProb& Pc, int i, int j,
+                                         const T* __restrict__ xc, int64_t corg, int cpitch)
+{
+    const int nxf = Pf.nx, nyf = Pf.ny, nxc = Pc.nx, nyc = Pc.ny;
+    const int J0 = min(nxc - 1, (j * nxc) / nxf), I0 = min(nyc - 1, (i * nyc) / nyf);
+    const T inx = (T)1 / (T)nxf, iny = (T)1 / (T)nyf;
+    const int du = j * nxc - J0 * nxf, dv = i * nyc - I0 * nyf;
+    const int64_t k = corg + (int64_t)I0 * cpitch + J0;
+    return hat<T>(du, dv, inx, iny) * xc[k] + hat<T>(du - nxf, dv, inx, iny) * xc[k + 1] +
+           hat<T>(du, dv - nyf, inx, iny) * xc[k + cpitch] +
+           hat<T>(du - nxf, dv - nyf, inx, iny) * xc[k + cpitch + 1];
+}
+
+// ------------------------------------------------------------ multigrid grid kernels
+
+// Damped Jacobi sweep over the launched tiles.  bcopy (optional) keeps b in
+// the smoother precision; dot (last level-0 sweep) adds the partial r.z with
+// b the FP64 residual and x_out the preconditioned residual.
 template <typename T, typename TB, typename TO>
 __global__ __launch_bounds__(kBlock) void k_smooth(LevelView L, const TB* __restrict__ b, const T* __restrict__ xin,
                                                    TO* __restrict__ xout, T* __restrict__ bcopy, T omega,
-                                                   int zero_init, const CgState* st)
+                                                   int zero_init, int dot, double* partial, const CgState* st)
 {
     NodeCtx c;
     if (!decode(L, c)) return;
     const LevelProb& P = *c.P;
     if (st[P.prob].done) return;
-    if (!c.in) return;
-    if (is_dirichlet(P, c.i, c.j)) {
-        xout[c.idx] = (TO)0;
-        if (bcopy) bcopy[c.idx] = (T)0;
-        return;
+    double v[1] = {0.0};
+    if (c.in) {
+        if (is_dirichlet(P, c.i, c.j)) {
+            xout[c.idx] = (TO)0;
+            if (bcopy) bcopy[c.idx] = (T)0;
+        } else {
+            T bv;
+            const T out = smooth_val<T, TB>(P, c.i, c.j, c.idx, P.pitch, b, xin, omega, zero_init != 0, bv);
+            if (bcopy) bcopy[c.idx] = bv;
+            const TO o = (TO)out;
+            xout[c.idx] = o;
+            if (dot) v[0] = (double)b[c.idx] * (double)o;
+        }
     }
-    const Row<T> rc = row_coefs<T>(P, c.i, c.j);
-    const T d = row_diag(rc);
-    const T bv = (T)b[c.idx];
-    if (bcopy) bcopy[c.idx] = bv;
-    T out;
-    if (zero_init) {
-        out = omega * bv / d;
-    } else {
-        T xc, xw, xe, xs, xn, xsw, xne;
-        load7(xin, c.idx, P.pitch, xc, xw, xe, xs, xn, xsw, xne);
-        out = xc + omega * (bv - row_apply(rc, xc, xw, xe, xs, xn, xsw, xne)) / d;
-    }
-    xout[c.idx] = (TO)out;
+    if (dot) tile_partial<1>(v, partial, c.gt);
 }
 
 template <typename T>
@@ -431,18 +520,9 @@ __global__ __launch_bounds__(kBlock) void k_residual(LevelView L, const T* __res
     const LevelProb& P = *c.P;
     if (st[P.prob].done) return;
     if (!c.in) return;
-    if (is_dirichlet(P, c.i, c.j)) {
-        r[c.idx] = (T)0;
-        return;
-    }
-    T xc, xw, xe, xs, xn, xsw, xne;
-    load7(x, c.idx, P.pitch, xc, xw, xe, xs, xn, xsw, xne);
-    const Row<T> rc = row_coefs<T>(P, c.i, c.j);
-    r[c.idx] = b[c.idx] - row_apply(rc, xc, xw, xe, xs, xn, xsw, xne);
+    r[c.idx] = is_dirichlet(P, c.i, c.j) ? (T)0 : residual_val<T>(P, c.i, c.j, c.idx, P.pitch, b, x);
 }
 
-// b_c = P^T r_f, gathered per coarse node over the fine nodes whose P1
-// interpolation uses it (no atomics).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_restrict(LevelView Lc, const LevelProb* __restrict__ fine,
                                                      const T* __restrict__ rf, T* __restrict__ bc, const CgState* st)
@@ -452,37 +532,10 @@ __global__ __launch_bounds__(kBlock) void k_restrict(LevelView Lc, const LevelPr
     const LevelProb& Pc = *c.P;
     if (st[Pc.prob].done) return;
     if (!c.in) return;
-    if (is_dirichlet(Pc, c.i, c.j)) {
-        bc[c.idx] = (T)0;
-        return;
-    }
     const LevelProb& Pf = fine[Pc.fidx];
-    const int nxf = Pf.nx, nyf = Pf.ny, nxc = Pc.nx, nyc = Pc.ny;
-    const int I = c.i, J = c.j;
-    const int jlo = (J >= 1) ? (int)(((int64_t)(J - 1) * nxf + nxc - 1) / nxc) : 0;
-    const int jhi = (int)std::min<int64_t>((int64_t)nxf, ((int64_t)(J + 1) * nxf) / nxc);
-    const int ilo = (I >= 1) ? (int)(((int64_t)(I - 1) * nyf + nyc - 1) / nyc) : 0;
-    const int ihi = (int)std::min<int64_t>((int64_t)nyf, ((int64_t)(I + 1) * nyf) / nyc);
-    T sum = (T)0;
-    for (int i = ilo; i <= ihi; ++i) {
-        const T* row = rf + Pf.origin + (int64_t)i * Pf.pitch;
-        for (int j = jlo; j <= jhi; ++j) {
-            const Interp w = interp_weights(i, j, nxf, nyf, nxc, nyc);
-            double wt = 0.0;
-            if (w.I0 == I) {
-                if (w.J0 == J) wt += w.wbl;
-                else if (w.J0 + 1 == J) wt += w.wbr;
-            } else if (w.I0 + 1 == I) {
-                if (w.J0 == J) wt += w.wtl;
-                else if (w.J0 + 1 == J) wt += w.wtr;
-            }
-            if (wt != 0.0) sum += (T)wt * row[j];
-        }
-    }
-    bc[c.idx] = sum;
+    bc[c.idx] = is_dirichlet(Pc, c.i, c.j) ? (T)0 : restrict_val<T>(Pc, Pf, c.i, c.j, rf, Pf.origin, Pf.pitch);
 }
 
-// x_f += P x_c  (fine Dirichlet nodes stay 0)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_prolong(LevelView Lf, const LevelProb* __restrict__ coarse,
                                                     const T* __restrict__ xc, T* __restrict__ xf, const CgState* st)
@@ -493,35 +546,184 @@ __global__ __launch_bounds__(kBlock) void k_prolong(LevelView Lf, const LevelPro
     if (st[Pf.prob].done) return;
     if (!c.in || is_dirichlet(Pf, c.i, c.j)) return;
     const LevelProb& Pc = coarse[Pf.cidx];
-    const Interp w = interp_weights(c.i, c.j, Pf.nx, Pf.ny, Pc.nx, Pc.ny);
-    const int64_t k = Pc.origin + (int64_t)w.I0 * Pc.pitch + w.J0;
-    const T v = (T)w.wbl * xc[k] + (T)w.wbr * xc[k + 1] + (T)w.wtl * xc[k + Pc.pitch] +
-                (T)w.wtr * xc[k + Pc.pitch + 1];
-    xf[c.idx] += v;
+    xf[c.idx] += prolong_val<T>(Pf, Pc, c.i, c.j, xc, Pc.origin, Pc.pitch);
 }
 
-// coarsest level: x = A^{-1} b with a precomputed dense inverse (one block per problem)
-template <typename T, typename TO>
-__global__ __launch_bounds__(256) void k_coarse_dense(const LevelProb* __restrict__ probs,
-                                                      const int32_t* __restrict__ list, const T* __restrict__ dense,
-                                                      const T* __restrict__ b, TO* __restrict__ x, const CgState* st)
+// level-0 problems whose whole V-cycle runs in one workgroup: cast the FP64
+// residual in, and the result out (with the partial r.z)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cast_in(LevelView L, const double* __restrict__ r, T* __restrict__ b,
+                                                    const CgState* st)
 {
-    __shared__ T sb[1024];
-    const LevelProb& P = probs[list[blockIdx.x]];
-    if (st[P.prob].done) return;
-    const int nw = P.nx + 1, n = nw * (P.ny + 1);
-    for (int m = threadIdx.x; m < n; m += blockDim.x) {
-        const int i = m / nw, j = m - i * nw;
-        sb[m] = b[P.origin + (int64_t)i * P.pitch + j];
+    NodeCtx c;
+    if (!decode(L, c)) return;
+    if (st[c.P->prob].done || !c.in) return;
+    b[c.idx] = (T)r[c.idx];
+}
+
+template <typename T, typename TZ>
+__global__ __launch_bounds__(kBlock) void k_cast_out(LevelView L, const T* __restrict__ x, const double* __restrict__ r,
+                                                     TZ* __restrict__ z, double* partial, const CgState* st)
+{
+    NodeCtx c;
+    if (!decode(L, c)) return;
+    if (st[c.P->prob].done) return;
+    double v[1] = {0.0};
+    if (c.in) {
+        const TZ zv = (TZ)x[c.idx];
+        z[c.idx] = zv;
+        v[0] = r[c.idx] * (double)zv;
+    }
+    tile_partial<1>(v, partial, c.gt);
+}
+
+// ------------------------------------------------------------ bottom V-cycle (one workgroup per problem, in LDS)
+
+template <typename T>
+struct BottomArgs {
+    const LevelProb* probs[kMaxLev];
+    T* b[kMaxLev];
+};
+
+#define EQ_FOR_NODES(P)                                                              \
+    for (int i = (int)(threadIdx.x >> 6); i <= (P).ny; i += kBottomThreads / 64)      \
+        for (int j = (int)(threadIdx.x & 63); j <= (P).nx; j += 64)
+
+// LDS arrays of a bottom level: x0, x1, b, each (ny+3) x (nx+3) with a zero frame
+template <typename T>
+struct LdsLevel {
+    T* x0;
+    T* x1;
+    T* b;
+    int pitch;
+    __device__ int64_t at(int i, int j) const { return (int64_t)(i + 1) * pitch + (j + 1); }
+    __device__ T* x(int k) const { return k ? x1 : x0; } // select, not an indexed array (no scratch)
+};
+
+template <typename T>
+__device__ __forceinline__ LdsLevel<T> lds_level(T* sm, const LevelProb& P)
+{
+    LdsLevel<T> L;
+    L.pitch = P.nx + 3;
+    const int64_t sz = (int64_t)(P.ny + 3) * L.pitch;
+    L.x0 = sm + P.lds_off;
+    L.x1 = L.x0 + sz;
+    L.b = L.x1 + sz;
+    return L;
+}
+
+template <typename T>
+__device__ __forceinline__ void lds_sweep(const LevelProb& P, const LdsLevel<T>& S, int from, int to, T omega,
+                                          bool zero_init)
+{
+    EQ_FOR_NODES(P)
+    {
+        if (!is_dirichlet(P, i, j)) {
+            T bv;
+            const int64_t k = S.at(i, j);
+            S.x(to)[k] = smooth_val<T, T>(P, i, j, k, S.pitch, S.b, S.x(from), omega, zero_init, bv);
+        }
     }
     __syncthreads();
-    const T* A = dense + P.dense_off;
-    for (int k = threadIdx.x; k < n; k += blockDim.x) {
-        T s = (T)0;
-        const T* row = A + (int64_t)k * n;
-        for (int m = 0; m < n; ++m) s += row[m] * sb[m];
-        const int i = k / nw, j = k - i * nw;
-        x[P.origin + (int64_t)i * P.pitch + j] = (TO)s;
+}
+
+// The V-cycle from the problem's bottom level down, entirely in LDS: the
+// bottom level's right-hand side is read from HBM, its correction is written
+// to `out` (the buffer its finer level's prolongation reads).  Dirichlet
+// entries stay at the zero the LDS is cleared to.
+template <typename T>
+__global__ __launch_bounds__(kBottomThreads) void k_bottom(BottomArgs<T> A, int lb, const int32_t* __restrict__ list,
+                                                           T* __restrict__ out, T omega, int pre, int post,
+                                                           const T* __restrict__ dense, const CgState* st)
+{
+    extern __shared__ unsigned char eq_smem[];
+    __shared__ int idx[kMaxLev];
+    __shared__ int fin[kMaxLev];
+    __shared__ int s_nl;
+    T* sm = reinterpret_cast<T*>(eq_smem);
+    const LevelProb& P0 = A.probs[lb][list[blockIdx.x]];
+    if (st[P0.prob].done) return;
+    if (threadIdx.x == 0) {
+        int n = 0;
+        idx[0] = list[blockIdx.x];
+        while (!A.probs[lb + n][idx[n]].coarsest && lb + n + 1 < kMaxLev) {
+            idx[n + 1] = A.probs[lb + n][idx[n]].cidx;
+            ++n;
+        }
+        s_nl = n;
+    }
+    __syncthreads();
+    const int nl = s_nl;
+    for (int m = threadIdx.x; m < P0.lds_total; m += kBottomThreads) sm[m] = (T)0;
+    __syncthreads();
+    {
+        const LdsLevel<T> S = lds_level(sm, P0);
+        EQ_FOR_NODES(P0) S.b[S.at(i, j)] = A.b[lb][P0.origin + (int64_t)i * P0.pitch + j];
+        __syncthreads();
+    }
+    for (int k = 0; k < nl; ++k) {
+        const LevelProb& P = A.probs[lb + k][idx[k]];
+        const LevelProb& Pc = A.probs[lb + k + 1][idx[k + 1]];
+        const LdsLevel<T> S = lds_level(sm, P), C = lds_level(sm, Pc);
+        lds_sweep<T>(P, S, 0, 0, omega, true);
+        int cur = 0;
+        for (int s = 1; s < pre; ++s) {
+            lds_sweep<T>(P, S, cur, cur ^ 1, omega, false);
+            cur ^= 1;
+        }
+        if (threadIdx.x == 0) fin[k] = cur;
+        // residual into the free buffer, then restrict it into the coarser b
+        T* r = S.x(cur ^ 1);
+        EQ_FOR_NODES(P)
+        {
+            const int64_t kk = S.at(i, j);
+            r[kk] = is_dirichlet(P, i, j) ? (T)0 : residual_val<T>(P, i, j, kk, S.pitch, S.b, S.x(cur));
+        }
+        __syncthreads();
+        EQ_FOR_NODES(Pc)
+        {
+            if (!is_dirichlet(Pc, i, j)) C.b[C.at(i, j)] = restrict_val<T>(Pc, P, i, j, r, S.at(0, 0), S.pitch);
+        }
+        __syncthreads();
+    }
+    // coarsest level: x = A^{-1} b with the dense inverse
+    {
+        const LevelProb& Pc = A.probs[lb + nl][idx[nl]];
+        const LdsLevel<T> C = lds_level(sm, Pc);
+        const int nw = Pc.nx + 1, n = nw * (Pc.ny + 1);
+        const T* D = dense + Pc.dense_off;
+        for (int k = threadIdx.x; k < n; k += kBottomThreads) {
+            T s = (T)0;
+            const T* row = D + (int64_t)k * n;
+            for (int m = 0; m < n; ++m) {
+                const int mi = m / nw, mj = m - mi * nw;
+                s += row[m] * C.b[C.at(mi, mj)];
+            }
+            const int i = k / nw, j = k - i * nw;
+            C.x(0)[C.at(i, j)] = s;
+        }
+        __syncthreads();
+    }
+    int from = 0; // buffer of the coarser level's result
+    for (int k = nl - 1; k >= 0; --k) {
+        const LevelProb& P = A.probs[lb + k][idx[k]];
+        const LevelProb& Pc = A.probs[lb + k + 1][idx[k + 1]];
+        const LdsLevel<T> S = lds_level(sm, P), C = lds_level(sm, Pc);
+        int cur = fin[k];
+        EQ_FOR_NODES(P)
+        {
+            if (!is_dirichlet(P, i, j)) S.x(cur)[S.at(i, j)] += prolong_val<T>(P, Pc, i, j, C.x(from), C.at(0, 0), C.pitch);
+        }
+        __syncthreads();
+        for (int s = 0; s < post; ++s) {
+            lds_sweep<T>(P, S, cur, cur ^ 1, omega, false);
+            cur ^= 1;
+        }
+        from = cur;
+    }
+    {
+        const LdsLevel<T> S = lds_level(sm, P0);
+        EQ_FOR_NODES(P0) out[P0.origin + (int64_t)i * P0.pitch + j] = S.x(from)[S.at(i, j)];
     }
 }
 
@@ -573,7 +775,7 @@ __global__ __launch_bounds__(256) void k_flux(const LevelProb* __restrict__ prob
         // right edge (i,nx)-(i+1,nx) in the lower-right triangle of cell (i,nx-1)
         v[0] += P.hy * (U[(int64_t)i * pt + nx - 1] - U[(int64_t)i * pt + nx]) / P.hx;
     }
-    block_sum<1>(v);
+    block_sum<1, 256>(v);
     if (threadIdx.x == 0) *out = P.Ddt * v[0];
 }
 
@@ -620,6 +822,8 @@ static void prof_collect(eq_batch* b)
         s.pending_bytes.clear();
     }
 }
+
+// ------------------------------------------------------------ hierarchy set-up
 
 // ------------------------------------------------------------ hierarchy set-up
 
@@ -727,10 +931,21 @@ static int alloc_zero(T** p, int64_t n, hipStream_t s)
     return EQ_OK;
 }
 
+template <typename V>
+static int upload_vec(V** d, const std::vector<V>& h, hipStream_t s)
+{
+    EQ_HIP(hipMalloc((void**)d, sizeof(V) * std::max<size_t>(1, h.size())));
+    if (!h.empty()) EQ_HIP(hipMemcpyAsync(*d, h.data(), sizeof(V) * h.size(), hipMemcpyHostToDevice, s));
+    return EQ_OK;
+}
+
 static int validate(const eq_field_desc& d, std::string& why)
 {
     if (d.nx < 1 || d.ny < 1) { why = "nx and ny must be >= 1"; return 1; }
-    if ((int64_t)(d.nx + 1) * (d.ny + 1) > (int64_t)1 << 30) { why = "grid too large"; return 1; }
+    if (d.nx > 65536 || d.ny > 65536 || (int64_t)(d.nx + 1) * (d.ny + 1) > (int64_t)1 << 30) {
+        why = "grid too large (nx, ny <= 65536, nodes <= 2^30)";
+        return 1;
+    }
     if (!(d.width > 0) || !(d.height > 0)) { why = "width and height must be > 0"; return 1; }
     if (!(d.dt > 0) || !(d.D >= 0) || !(d.gamma >= 0) || !std::isfinite(d.f)) {
         why = "need dt > 0, D >= 0, gamma >= 0, finite f";
@@ -749,9 +964,10 @@ static int build_hierarchy(eq_batch* b)
 {
     const eq_solver_desc& S = b->solver;
     const bool mg = S.kind != EQ_SOLVER_JACOBI_CG;
-    const int maxlev = mg ? std::max(1, S.mg_max_levels) : 1;
-    // sizes per problem per level
+    const int maxlev = mg ? std::min(kMaxLev, std::max(1, S.mg_max_levels)) : 1;
+    // sizes per problem per level; bottom level = first level at or below kBottomNodes
     std::vector<std::vector<std::pair<int, int>>> dims(b->nprob);
+    std::vector<int> lbot(b->nprob, 0);
     int nlev = 1;
     for (int p = 0; p < b->nprob; ++p) {
         int nx = b->descs[p].nx, ny = b->descs[p].ny;
@@ -766,6 +982,21 @@ static int build_hierarchy(eq_batch* b)
         }
         if (mg && (int64_t)(nx + 1) * (ny + 1) > 1024)
             return set_error(EQ_ERR_INVALID, "coarsest multigrid level exceeds 1024 nodes; raise mg_max_levels");
+        if (mg) {
+            // bottom level: the first level from which every coarser level's
+            // three LDS arrays (x0, x1, b with a zero frame) fit in kBottomLds
+            int64_t foot = 0;
+            int l = (int)dims[p].size();
+            while (l > 0) {
+                const int64_t f = 3 * (int64_t)(dims[p][l - 1].first + 3) * (dims[p][l - 1].second + 3) * (int64_t)sizeof(T);
+                if (foot + f > b->lds_budget) break;
+                foot += f;
+                --l;
+            }
+            if (l == (int)dims[p].size())
+                return set_error(EQ_ERR_INVALID, "coarsest multigrid level does not fit the bottom solver's LDS");
+            lbot[p] = l;
+        }
         nlev = std::max(nlev, (int)dims[p].size());
     }
     b->levels.assign(nlev, Level());
@@ -780,6 +1011,18 @@ static int build_hierarchy(eq_batch* b)
             fill_level_prob(P, b->descs[p], dims[p][l].first, dims[p][l].second, l == 0);
             P.prob = p;
             P.coarsest = ((int)dims[p].size() == l + 1) ? 1 : 0;
+            P.bottom = (mg && l == lbot[p]) ? 1 : 0;
+            P.below = (mg && l > lbot[p]) ? 1 : 0;
+            if (mg && l >= lbot[p]) {
+                int64_t off = 0, total = 0;
+                for (int k = lbot[p]; k < (int)dims[p].size(); ++k) {
+                    const int64_t sz = 3 * (int64_t)(dims[p][k].first + 3) * (dims[p][k].second + 3);
+                    if (k < l) off += sz;
+                    total += sz;
+                }
+                P.lds_off = (int32_t)off;
+                P.lds_total = (int32_t)total;
+            }
             if (!P.coarsest) {
                 P.nxc = dims[p][l + 1].first;
                 P.nyc = dims[p][l + 1].second;
@@ -805,59 +1048,57 @@ static int build_hierarchy(eq_batch* b)
         for (size_t k = 0; k < L.probs.size(); ++k) {
             LevelProb& P = L.probs[k];
             if (!P.coarsest) P.cidx = where[P.prob][l + 1];
+            const bool smooth = mg && !P.bottom && !P.below;
+            const bool active = !P.below;
             for (int t = 0; t < P.ntiles; ++t) {
                 L.tiles_all.push_back(TileRef{(int32_t)k, t});
-                if (!P.coarsest) L.tiles_smooth.push_back(TileRef{(int32_t)k, t});
+                if (smooth) L.tiles_smooth.push_back(TileRef{(int32_t)k, t});
+                if (active && P.bottom) L.tiles_bottom.push_back(TileRef{(int32_t)k, t});
+                if (active) L.tiles_active.push_back(TileRef{(int32_t)k, t});
             }
             const double nodes = (double)(P.nx + 1) * (double)(P.ny + 1);
             L.nodes_all += nodes;
-            if (!P.coarsest) L.nodes_smooth += nodes;
+            if (smooth) L.nodes_smooth += nodes;
+            if (P.bottom) {
+                L.bottom.push_back((int32_t)k);
+                L.bottom_lds_bytes = std::max<size_t>(L.bottom_lds_bytes, (size_t)P.lds_total * sizeof(T));
+            }
             if (P.coarsest && mg) {
                 std::vector<double> inv;
                 if (!dense_inverse(P, inv)) return set_error(EQ_ERR_INVALID, "singular coarse operator");
                 P.dense_off = (int64_t)dense_all.size();
                 for (double v : inv) dense_all.push_back((T)v);
-                L.coarse.push_back((int32_t)k);
-                L.max_dense = std::max<int64_t>(L.max_dense, (int64_t)(P.nx + 1) * (P.ny + 1));
             }
         }
     }
     // device tables and vectors
     hipStream_t s = b->stream;
+    int rc;
     for (int l = 0; l < nlev; ++l) {
         Level& L = b->levels[l];
-        EQ_HIP(hipMalloc((void**)&L.d_probs, sizeof(LevelProb) * L.probs.size()));
-        EQ_HIP(hipMemcpyAsync(L.d_probs, L.probs.data(), sizeof(LevelProb) * L.probs.size(), hipMemcpyHostToDevice, s));
-        EQ_HIP(hipMalloc((void**)&L.d_tiles_all, sizeof(TileRef) * std::max<size_t>(1, L.tiles_all.size())));
-        EQ_HIP(hipMemcpyAsync(L.d_tiles_all, L.tiles_all.data(), sizeof(TileRef) * L.tiles_all.size(),
-                              hipMemcpyHostToDevice, s));
-        EQ_HIP(hipMalloc((void**)&L.d_tiles_smooth, sizeof(TileRef) * std::max<size_t>(1, L.tiles_smooth.size())));
-        if (!L.tiles_smooth.empty())
-            EQ_HIP(hipMemcpyAsync(L.d_tiles_smooth, L.tiles_smooth.data(), sizeof(TileRef) * L.tiles_smooth.size(),
-                                  hipMemcpyHostToDevice, s));
-        EQ_HIP(hipMalloc((void**)&L.d_coarse, sizeof(int32_t) * std::max<size_t>(1, L.coarse.size())));
-        if (!L.coarse.empty())
-            EQ_HIP(hipMemcpyAsync(L.d_coarse, L.coarse.data(), sizeof(int32_t) * L.coarse.size(),
-                                  hipMemcpyHostToDevice, s));
+        if ((rc = upload_vec(&L.d_probs, L.probs, s)) || (rc = upload_vec(&L.d_tiles_all, L.tiles_all, s)) ||
+            (rc = upload_vec(&L.d_tiles_smooth, L.tiles_smooth, s)) ||
+            (rc = upload_vec(&L.d_tiles_bottom, L.tiles_bottom, s)) ||
+            (rc = upload_vec(&L.d_tiles_active, L.tiles_active, s)) || (rc = upload_vec(&L.d_bottom, L.bottom, s)))
+            return rc;
         if (mg) {
-            int rc;
-            if (l > 0 || b->mixed) {
-                if ((rc = alloc_zero((T**)&L.b, L.pool, s))) return rc;
-            }
-            if ((rc = alloc_zero((T**)&L.xa, L.pool, s))) return rc;
-            if ((rc = alloc_zero((T**)&L.xb, L.pool, s))) return rc;
-            if ((rc = alloc_zero((T**)&L.r, L.pool, s))) return rc;
+            if ((rc = alloc_zero((T**)&L.b, L.pool, s)) || (rc = alloc_zero((T**)&L.xa, L.pool, s)) ||
+                (rc = alloc_zero((T**)&L.xb, L.pool, s)) || (rc = alloc_zero((T**)&L.r, L.pool, s)))
+                return rc;
         }
     }
     if (!dense_all.empty()) {
-        int rc;
         if ((rc = alloc_zero((T**)&b->dense, (int64_t)dense_all.size(), s))) return rc;
         EQ_HIP(hipMemcpyAsync(b->dense, dense_all.data(), sizeof(T) * dense_all.size(), hipMemcpyHostToDevice, s));
     }
+    size_t lds_max = 0;
+    for (const Level& L : b->levels) lds_max = std::max(lds_max, L.bottom_lds_bytes);
+    if (lds_max > 64 * 1024)
+        EQ_HIP(hipFuncSetAttribute((const void*)k_bottom<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max));
     const int64_t n0 = b->levels[0].pool;
-    int rc;
     if ((rc = alloc_zero(&b->u, n0, s)) || (rc = alloc_zero(&b->rhs, n0, s)) || (rc = alloc_zero(&b->r, n0, s)) ||
-        (rc = alloc_zero(&b->p, n0, s)) || (rc = alloc_zero(&b->q, n0, s)) || (rc = alloc_zero(&b->z, n0, s)) ||
+        (rc = alloc_zero(&b->p, n0, s)) || (rc = alloc_zero(&b->p2, n0, s)) || (rc = alloc_zero(&b->q, n0, s)) ||
+        (rc = alloc_zero(&b->z, n0, s)) || (rc = alloc_zero(&b->xprev, n0, s)) ||
         (rc = alloc_zero(&b->partial, 4 * (int64_t)b->levels[0].tiles_all.size(), s)))
         return rc;
     EQ_HIP(hipStreamSynchronize(s));
@@ -869,8 +1110,33 @@ static int build_hierarchy(eq_batch* b)
 template <typename T>
 struct VCycle {
     eq_batch* b;
-    // returns the buffer holding the level's result (level 0: z)
-    void* run(int l, const void* bl_in)
+    BottomArgs<T> args;
+
+    explicit VCycle(eq_batch* bb) : b(bb)
+    {
+        std::memset(&args, 0, sizeof(args));
+        for (size_t l = 0; l < b->levels.size() && l < (size_t)kMaxLev; ++l) {
+            const Level& L = b->levels[l];
+            args.probs[l] = L.d_probs;
+            args.b[l] = (T*)L.b;
+        }
+    }
+
+    // z = V(r) on level 0 (and the partial r.z for the next k_finish)
+    void run0() { run(0); }
+
+    // Level l: its right-hand side is in L.b (level 0: the FP64 residual
+    // b->r); on return the level's correction is in final_buffer(l)
+    // (level 0: b->z).
+    T* final_buffer(int l) const
+    {
+        const eq_solver_desc& S = b->solver;
+        const Level& L = b->levels[l];
+        const int f = ((S.mg_pre - 1) + S.mg_post) & 1;
+        return f ? (T*)L.xb : (T*)L.xa;
+    }
+
+    void run(int l)
     {
         const eq_solver_desc& S = b->solver;
         Level& L = b->levels[l];
@@ -880,74 +1146,66 @@ struct VCycle {
         const bool top = (l == 0);
         const int ns = (int)L.tiles_smooth.size();
         const int gs = grid_for(ns);
-        // final buffer of this level after post-smoothing (z on level 0)
         T* X[2] = {(T*)L.xa, (T*)L.xb};
         int cur = 0;
-        const T* bl = (const T*)bl_in;
         if (ns > 0) {
-            // pre-smoothing (first sweep from x = 0); level 0 reads the FP64
-            // residual and, in mixed precision, keeps an FP32 copy of it
-            const double fine_bytes = L.nodes_smooth; // nodes; times bytes per node below
+            const double nodes = L.nodes_smooth;
+            // pre-smoothing; the first sweep starts from x = 0; level 0 reads
+            // the FP64 residual and (mixed precision) keeps an FP32 copy of it
+            const T* bl = top ? (b->mixed ? (const T*)L.b : (const T*)b->r) : (const T*)L.b;
             if (top) {
                 prof_begin(b, EQ_KERNEL_SMOOTH);
-                if (b->mixed)
-                    k_smooth<T, double, T><<<gs, kBlock, 0, s>>>(L.smooth(), (const double*)bl_in, nullptr, X[0],
-                                                                 (T*)L.b, omega, 1, st);
-                else
-                    k_smooth<T, double, T><<<gs, kBlock, 0, s>>>(L.smooth(), (const double*)bl_in, nullptr, X[0],
-                                                                 nullptr, omega, 1, st);
-                prof_end(b, EQ_KERNEL_SMOOTH, fine_bytes * (sizeof(double) + sizeof(T) * (b->mixed ? 2 : 1)));
-                bl = b->mixed ? (const T*)L.b : (const T*)bl_in;
+                k_smooth<T, double, T><<<gs, kBlock, 0, s>>>(L.smooth(), b->r, nullptr, X[0],
+                                                             b->mixed ? (T*)L.b : nullptr, omega, 1, 0, nullptr, st);
+                prof_end(b, EQ_KERNEL_SMOOTH, nodes * (sizeof(double) + sizeof(T) * (b->mixed ? 2 : 1)));
             } else {
-                k_smooth<T, T, T><<<gs, kBlock, 0, s>>>(L.smooth(), bl, nullptr, X[0], nullptr, omega, 1, st);
+                k_smooth<T, T, T><<<gs, kBlock, 0, s>>>(L.smooth(), bl, nullptr, X[0], nullptr, omega, 1, 0, nullptr,
+                                                        st);
             }
             for (int k = 1; k < S.mg_pre; ++k) {
                 if (top) prof_begin(b, EQ_KERNEL_SMOOTH);
-                k_smooth<T, T, T><<<gs, kBlock, 0, s>>>(L.smooth(), bl, X[cur], X[cur ^ 1], nullptr, omega, 0, st);
-                if (top) prof_end(b, EQ_KERNEL_SMOOTH, fine_bytes * 3 * sizeof(T));
+                k_smooth<T, T, T><<<gs, kBlock, 0, s>>>(L.smooth(), bl, X[cur], X[cur ^ 1], nullptr, omega, 0, 0,
+                                                        nullptr, st);
+                if (top) prof_end(b, EQ_KERNEL_SMOOTH, nodes * 3 * sizeof(T));
                 cur ^= 1;
             }
             k_residual<T><<<gs, kBlock, 0, s>>>(L.smooth(), bl, X[cur], (T*)L.r, st);
             Level& C = b->levels[l + 1];
-            const int nc = (int)C.tiles_all.size();
-            k_restrict<T><<<grid_for(nc), kBlock, 0, s>>>(C.all(), L.d_probs, (const T*)L.r, (T*)C.b, st);
-            const T* xc = (const T*)run(l + 1, C.b);
+            const int na = (int)C.tiles_active.size();
+            k_restrict<T><<<grid_for(na), kBlock, 0, s>>>(C.active(), L.d_probs, (const T*)L.r, (T*)C.b, st);
+            run(l + 1);
+            const T* xc = final_buffer(l + 1);
             k_prolong<T><<<gs, kBlock, 0, s>>>(L.smooth(), C.d_probs, xc, X[cur], st);
             for (int k = 0; k < S.mg_post; ++k) {
                 const bool last = (k == S.mg_post - 1);
                 if (top && last) {
+                    // final sweep: FP64 residual in, FP64 z out, partial r.z
                     prof_begin(b, EQ_KERNEL_SMOOTH);
-                    k_smooth<T, T, double><<<gs, kBlock, 0, s>>>(L.smooth(), bl, X[cur], b->z, nullptr, omega, 0, st);
-                    prof_end(b, EQ_KERNEL_SMOOTH, fine_bytes * (2 * sizeof(T) + sizeof(double)));
+                    k_smooth<T, double, T><<<gs, kBlock, 0, s>>>(L.smooth(), b->r, X[cur], (T*)b->z, nullptr,
+                                                                      omega, 0, 1, b->partial, st);
+                    prof_end(b, EQ_KERNEL_SMOOTH, nodes * (sizeof(T) + 2 * sizeof(double)));
                 } else {
                     if (top) prof_begin(b, EQ_KERNEL_SMOOTH);
                     k_smooth<T, T, T><<<gs, kBlock, 0, s>>>(L.smooth(), bl, X[cur], X[cur ^ 1], nullptr, omega, 0,
-                                                            st);
-                    if (top) prof_end(b, EQ_KERNEL_SMOOTH, fine_bytes * 3 * sizeof(T));
+                                                            0, nullptr, st);
+                    if (top) prof_end(b, EQ_KERNEL_SMOOTH, nodes * 3 * sizeof(T));
                     cur ^= 1;
                 }
             }
         }
-        void* out = top ? (void*)b->z : (void*)X[cur];
-        if (!L.coarse.empty()) {
+        if (!L.bottom.empty()) {
+            T* out = final_buffer(l);
             if (top) {
-                // single-level problems: the dense solve reads the residual directly
-                if (b->mixed) {
-                    // FP32 copy of the residual for the coarse problems
-                    k_smooth<T, double, T><<<grid_for((int)L.tiles_all.size()), kBlock, 0, s>>>(
-                        L.all(), (const double*)bl_in, nullptr, X[1], (T*)L.b, (T)0, 1, st);
-                    k_coarse_dense<T, double><<<(int)L.coarse.size(), 256, 0, s>>>(
-                        L.d_probs, L.d_coarse, (const T*)b->dense, (const T*)L.b, b->z, st);
-                } else {
-                    k_coarse_dense<T, double><<<(int)L.coarse.size(), 256, 0, s>>>(
-                        L.d_probs, L.d_coarse, (const T*)b->dense, (const T*)bl_in, b->z, st);
-                }
-            } else {
-                k_coarse_dense<T, T><<<(int)L.coarse.size(), 256, 0, s>>>(L.d_probs, L.d_coarse, (const T*)b->dense,
-                                                                          (const T*)bl_in, (T*)out, st);
+                const int nb = (int)L.tiles_bottom.size();
+                k_cast_in<T><<<grid_for(nb), kBlock, 0, s>>>(L.bottom_tiles(), b->r, (T*)L.b, st);
+            }
+            k_bottom<T><<<(int)L.bottom.size(), kBottomThreads, L.bottom_lds_bytes, s>>>(args, l, L.d_bottom, out, omega, S.mg_pre,
+                                                                        S.mg_post, (const T*)b->dense, st);
+            if (top) {
+                const int nb = (int)L.tiles_bottom.size();
+                k_cast_out<T, T><<<grid_for(nb), kBlock, 0, s>>>(L.bottom_tiles(), out, b->r, (T*)b->z, b->partial, st);
             }
         }
-        return out;
     }
 };
 
@@ -971,68 +1229,96 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
     hipStream_t s = b->stream;
     const bool jacobi = (S.kind == EQ_SOLVER_JACOBI_CG);
     const double fine_nodes = L0.nodes_all;
-    VCycle<T> vc{b};
+    VCycle<T> vc(b);
     auto finish = [&](int op) {
-        k_finish<<<b->nprob, kBlock, 0, s>>>(L0.d_probs, b->partial, b->d_state, op, jacobi ? 1 : 0);
+        k_finish<<<b->nprob, kFinishThreads, 0, s>>>(L0.d_probs, b->partial, b->d_state, op, jacobi ? 1 : 0);
     };
-    auto precondition = [&]() {
-        vc.run(0, b->r);
-        k_rz<<<g0, kBlock, 0, s>>>(A, b->r, b->z, b->partial, b->d_state);
-        finish(kFinRz);
-    };
-    auto iteration = [&]() {
+    // one CG iteration from search direction pold to pnew
+    auto iteration = [&](double* pold, double* pnew) {
+        if (!jacobi) {
+            vc.run0();
+            finish(kFinRz);
+        }
         prof_begin(b, EQ_KERNEL_STENCIL64);
-        k_ap<<<g0, kBlock, 0, s>>>(A, b->p, b->q, b->partial, b->d_state);
-        prof_end(b, EQ_KERNEL_STENCIL64, fine_nodes * 2 * sizeof(double));
+        k_ap_dir<T><<<g0, kBlock, 0, s>>>(A, (const T*)b->z, (const T*)pold, (T*)pnew, b->q, b->partial, b->d_state);
+        prof_end(b, EQ_KERNEL_STENCIL64, fine_nodes * 4 * sizeof(double));
         finish(kFinAp);
         prof_begin(b, EQ_KERNEL_CG_UPDATE);
-        k_update<<<g0, kBlock, 0, s>>>(A, b->u, b->r, b->z, b->p, b->q, b->partial, b->d_state, jacobi ? 1 : 0);
-        prof_end(b, EQ_KERNEL_CG_UPDATE, fine_nodes * sizeof(double) * (jacobi ? 7 : (b->mixed ? 7 : 6)));
+        k_update<T><<<g0, kBlock, 0, s>>>(A, b->u, b->r, (T*)b->z, (const T*)pnew, b->q, b->partial, b->d_state,
+                                          jacobi ? 1 : 0);
+        prof_end(b, EQ_KERNEL_CG_UPDATE, fine_nodes * sizeof(double) * ((jacobi || b->mixed) ? 7 : 6));
         finish(kFinUpdate);
-        if (!jacobi) precondition();
-        k_dir<<<g0, kBlock, 0, s>>>(A, b->z, b->p, b->d_state);
     };
-
-    k_rhs<<<g0, kBlock, 0, s>>>(A, b->u, b->rhs, b->partial, b->d_state);
-    finish(kFinRhs);
-    k_init<<<g0, kBlock, 0, s>>>(A, b->u, b->rhs, b->r, b->z, b->p, b->partial, b->d_state, jacobi ? 1 : 0);
-    finish(kFinInit);
-    if (!jacobi) {
-        precondition();
-        k_dir<<<g0, kBlock, 0, s>>>(A, b->z, b->p, b->d_state);
+    auto two_iterations = [&]() {
+        iteration(b->p, b->p2);
+        iteration(b->p2, b->p);
+    };
+    auto setup = [&]() {
+        k_rhs<<<g0, kBlock, 0, s>>>(A, b->u, b->rhs, b->partial);
+        finish(kFinRhs);
+        k_init<<<g0, kBlock, 0, s>>>(A, b->u, b->xprev, b->rhs, b->r, b->z, b->partial, b->d_state, jacobi ? 1 : 0);
+        finish(kFinInit);
+    };
+    // Launch sequences are replayed from hipGraphs unless per-launch event
+    // timing is on (events need eager launches).
+    bool profiling = false;
+    for (int k = 0; k < EQ_KERNEL_COUNT_; ++k) profiling = profiling || b->prof[k].enabled;
+    const bool graphs = S.use_graphs && !profiling;
+    auto capture = [&](hipGraphExec_t* exec, auto&& body) -> int {
+        if (*exec) return EQ_OK;
+        hipGraph_t g = nullptr;
+        EQ_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        body();
+        const hipError_t e = hipStreamEndCapture(s, &g);
+        if (e != hipSuccess) return hip_check(e, "hipStreamEndCapture");
+        const hipError_t e2 = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (e2 != hipSuccess) {
+            *exec = nullptr;
+            return hip_check(e2, "hipGraphInstantiate");
+        }
+        return EQ_OK;
+    };
+    int rc;
+    if (graphs) {
+        if ((rc = capture(&b->g_init, setup)) || (rc = capture(&b->g_iter2, two_iterations))) return rc;
+        EQ_HIP(hipGraphLaunch(b->g_init, s));
+    } else {
+        setup();
     }
     EQ_HIP(hipGetLastError());
 
-    int done_iters = 0;
     int chunk = std::max(1, b->last_iters > 0 ? b->last_iters : S.check_every);
-    int rc;
     for (int guard = 0; guard < 1000000; ++guard) {
-        for (int k = 0; k < chunk; ++k) iteration();
-        done_iters += chunk;
+        // iterations go in pairs, so p holds the current direction between chunks
+        for (int k = 0; k < chunk; k += 2) {
+            if (graphs) {
+                EQ_HIP(hipGraphLaunch(b->g_iter2, s));
+            } else {
+                two_iterations();
+            }
+        }
         EQ_HIP(hipGetLastError());
         if ((rc = read_states(b))) return rc;
         bool all = true;
         for (int p = 0; p < b->nprob; ++p) all = all && b->h_state[p].done;
         if (all) {
-            // confirm on the true residual; problems above rtol resume
-            k_true_residual<<<g0, kBlock, 0, s>>>(A, b->u, b->rhs, b->r, b->partial, b->d_state);
+            // confirm on the true residual; problems above rtol restart CG from it
+            k_true_residual<<<g0, kBlock, 0, s>>>(A, b->u, b->rhs, b->r, b->partial);
             finish(kFinTrue);
             if ((rc = read_states(b))) return rc;
             all = true;
             for (int p = 0; p < b->nprob; ++p) all = all && b->h_state[p].done;
             if (all) break;
-            // restart CG from the replaced residual: p = M^-1 r
             if (jacobi) {
-                k_jacobi_z<<<g0, kBlock, 0, s>>>(A, b->r, b->z, b->d_state);
-                k_rz<<<g0, kBlock, 0, s>>>(A, b->r, b->z, b->partial, b->d_state);
+                k_jacobi_z<<<g0, kBlock, 0, s>>>(A, b->r, b->z, b->partial, b->d_state);
                 finish(kFinRz);
-            } else {
-                precondition();
             }
-            k_dir<<<g0, kBlock, 0, s>>>(A, b->z, b->p, b->d_state);
         }
         chunk = std::max(1, S.check_every);
     }
+    k_keep<<<g0, kBlock, 0, s>>>(A, b->u, b->xprev, b->d_state);
+    EQ_HIP(hipGetLastError());
     prof_collect(b);
     int32_t itmax = 0, nconv = 0;
     double relmax = 0.0;
@@ -1046,15 +1332,13 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
         if (c.rr_true <= c.tol2 * c.bb) ++nconv;
     }
     b->last_iters = itmax;
-    (void)done_iters;
     if (stats) {
         stats->iterations_max = itmax;
         stats->nconverged = nconv;
         stats->relres_max = relmax;
         stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
-    if (nconv < b->nprob)
-        {
+    if (nconv < b->nprob) {
         char msg[128];
         std::snprintf(msg, sizeof(msg), "max_iter reached above rtol (relres %.3e)", relmax);
         return set_error(EQ_ERR_NOT_CONVERGED, msg);
@@ -1085,7 +1369,7 @@ void eq_solver_default(eq_solver_desc* s)
     s->mg_coarse_max_nodes = 256;
     s->mg_max_levels = 16;
     s->check_every = 4;
-    s->use_graphs = 0;
+    s->use_graphs = 1;
 }
 
 int eq_batch_create(const eq_field_desc* descs, int32_t nprob, int32_t nspecies, const eq_solver_desc* solver,
@@ -1115,6 +1399,13 @@ int eq_batch_create(const eq_field_desc* descs, int32_t nprob, int32_t nspecies,
     eq_batch* b = new (std::nothrow) eq_batch();
     if (!b) return set_error(EQ_ERR_NOMEM, "host allocation");
     hipGetDevice(&b->device);
+    {
+        int lds = 0;
+        if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, b->device) != hipSuccess || lds <= 0)
+            lds = 64 * 1024;
+        // leave room for the kernel's static shared memory
+        b->lds_budget = std::min<int64_t>(kBottomLdsMax, (int64_t)lds - 1024);
+    }
     b->nprob = nprob;
     b->nspecies = nspecies;
     b->ntraps = nprob / nspecies;
@@ -1167,11 +1458,15 @@ void eq_batch_destroy(eq_batch* b)
     if (!b) return;
     hipSetDevice(b->device);
     if (b->stream) hipStreamSynchronize(b->stream);
+    if (b->g_iter2) hipGraphExecDestroy(b->g_iter2);
+    if (b->g_init) hipGraphExecDestroy(b->g_init);
     for (Level& L : b->levels) {
         hipFree(L.d_probs);
         hipFree(L.d_tiles_all);
         hipFree(L.d_tiles_smooth);
-        hipFree(L.d_coarse);
+        hipFree(L.d_bottom);
+        hipFree(L.d_tiles_bottom);
+        hipFree(L.d_tiles_active);
         hipFree(L.b);
         hipFree(L.xa);
         hipFree(L.xb);
@@ -1181,6 +1476,8 @@ void eq_batch_destroy(eq_batch* b)
     hipFree(b->rhs);
     hipFree(b->r);
     hipFree(b->p);
+    hipFree(b->p2);
+    hipFree(b->xprev);
     hipFree(b->q);
     hipFree(b->z);
     hipFree(b->partial);
@@ -1228,6 +1525,8 @@ static int copy_field(eq_batch* b, int32_t prob, const double* src, double* dst,
     double* dev = b->u + P.origin;
     const bool to_dev = (kind == hipMemcpyHostToDevice) || (dst == nullptr);
     if (to_dev) {
+        // a new state from the host: the kept solution is no starting point for it
+        EQ_HIP(hipMemsetAsync(&b->d_state[prob].have_prev, 0, sizeof(int32_t), b->stream));
         EQ_HIP(hipMemcpy2DAsync(dev, sizeof(double) * P.pitch, src, sizeof(double) * nw, sizeof(double) * nw, nh,
                                 kind, b->stream));
     } else {

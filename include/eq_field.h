@@ -105,7 +105,8 @@ typedef struct eq_batch eq_batch;
 typedef struct eq_batch eq_field;
 
 const char* eq_last_error(void);
-/* fills the defaults: mixed GMG-CG, rtol 1e-10, V(2,2), omega 0.8, coarse <= 256 nodes */
+/* fills the defaults: mixed GMG-CG, rtol 1e-10, V(2,2), omega 0.8, coarse <= 256 nodes,
+ * graphs on */
 void eq_solver_default(eq_solver_desc* s);
 
 /* ---------------- batch API ---------------- */

@@ -123,6 +123,33 @@ def test_config0_steps_match_oracle_lu(eqf, oracle, solver):
                 fb.upload(s, ref[s])  # continue both from the same state
 
 
+@pytest.mark.parametrize("solver", SOLVERS)
+def test_graph_replays_match_eager_launches(eqf, oracle, solver):
+    """The captured iteration graph is replayed many times per step; with
+    host work (downloads, uploads, coupling) between the replays every step
+    must be bit-identical to eager launches."""
+    from eq_amd import workloads as Wl
+    t = Wl.config0()
+    descs = [eqf.field_desc(t.nx, t.ny, t.width, t.height, 0.1, D, g, 0.0, Wl.TOP_DIRICHLET)
+             for D, g in zip(t.D, t.gamma)]
+    q = t.rates * 0.1
+    out = {}
+    for graphs in (0, 1):
+        with eqf.FieldBatch(descs, nspecies=2, solver=eqf.solver_desc(_kind(eqf, solver), use_graphs=graphs)) as fb:
+            for s in range(2):
+                fb.upload(s, t.u0[s])
+            hist = []
+            for step in range(5):
+                fb.couple(0, t.cells, t.npm, q)
+                fb.step()
+                hist.append([fb.download(s) for s in range(2)])
+                fb.upload(1, hist[-1][1])  # eager host work between replays
+            out[graphs] = hist
+    for step in range(5):
+        for s in range(2):
+            assert np.array_equal(out[0][step][s], out[1][step][s]), (step, s)
+
+
 def test_couple_bit_exact_against_sequential_loop(eqf, oracle):
     from eq_amd import workloads as Wl
     rng = Wl.SplitMix64(11)
