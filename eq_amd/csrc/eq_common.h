@@ -31,14 +31,26 @@
 
 namespace eqk {
 
-constexpr int kTileW = 64;  // one wavefront per tile row
-constexpr int kTileH = 4;   // rows per tile (4 waves per 256-thread block)
-constexpr int kBlock = kTileW * kTileH;
+constexpr int kTileW = 64;  // one wavefront spans the tile's columns
+constexpr int kWaves = 4;   // waves per 256-thread block
+constexpr int kBlock = kTileW * kWaves;
+// consecutive rows per thread: four on the finest level (amortises each
+// block's set-up over more nodes and keeps more loads in flight), one on the
+// coarse levels (too few nodes there to give up parallelism)
+constexpr int kRows0 = 4;
+constexpr int kRowsC = 1;
+constexpr int rows_for_level(int l) { return l == 0 ? kRows0 : kRowsC; }
 constexpr int kRowPad = 64; // pitch granularity in elements
 
 // walls in eQ::boundaryCondition::wall order (src/eQ.h:335-340)
 enum { kLeft = 0, kRight = 1, kTop = 2, kBottom = 3 };
 enum { kNeumann = 0, kDirichlet = 1, kRobin = 2 };
+
+template <typename T>
+struct Row {
+    T mc, mw, me, ms, mn, msw, mne; // mass-like couplings (mass, reaction, Robin)
+    T kw, ke, ks, kn;               // stiffness difference weights
+};
 
 // One problem (species x trap) on one multigrid level.
 struct LevelProb {
@@ -69,6 +81,15 @@ struct LevelProb {
     double g[4];           // Dirichlet value per wall
     double hx, hy;         // spacing (boundary flux)
     double Ddt;            // dt*D (boundary flux scaling)
+    // the row every interior node shares (0 < i < ny, 0 < j < nx), in both
+    // precisions, and the reciprocal of its diagonal
+    Row<double> rin;
+    Row<float> rinf;
+    double rin_dinv;
+    float rinf_dinv;
+    // 1/nx, 1/ny (grid transfers: hat arguments and index division)
+    double rnx, rny;
+    float rnxf, rnyf;
 };
 
 EQ_HD inline bool is_dirichlet(const LevelProb& P, int i, int j)
@@ -90,55 +111,68 @@ EQ_HD inline double dirichlet_value(const LevelProb& P, int i, int j)
     return v;
 }
 
-template <typename T>
-struct Row {
-    T mc, mw, me, ms, mn, msw, mne; // mass-like couplings (mass, reaction, Robin)
-    T kw, ke, ks, kn;               // stiffness difference weights
-};
 
-// Row of A = (1+dt*gamma)M + dt*D*K + Robin at node (i,j) (Dirichlet rows are
-// handled by the callers).
+EQ_HD inline bool interior(const LevelProb& P, int i, int j) { return i > 0 && j > 0 && i < P.ny && j < P.nx; }
+
+template <typename T> EQ_HD inline const Row<T>& interior_row(const LevelProb& P);
+template <> EQ_HD inline const Row<double>& interior_row<double>(const LevelProb& P) { return P.rin; }
+template <> EQ_HD inline const Row<float>& interior_row<float>(const LevelProb& P) { return P.rinf; }
+template <typename T> EQ_HD inline T interior_dinv(const LevelProb& P);
+template <> EQ_HD inline double interior_dinv<double>(const LevelProb& P) { return P.rin_dinv; }
+template <> EQ_HD inline float interior_dinv<float>(const LevelProb& P) { return P.rinf_dinv; }
+
+// Row of A = (1+dt*gamma)M + dt*D*K + Robin at node (i,j) from the element
+// counts around it (Dirichlet rows are handled by the callers), evaluated in
+// the precision T of the caller (the FP32 smoother builds FP32 rows).
 template <typename T>
-EQ_HD inline Row<T> row_coefs(const LevelProb& P, int i, int j)
+EQ_HD inline Row<T> row_coefs_at(const LevelProb& P, int i, int j)
 {
-    const double sw = (i > 0 && j > 0) ? 1.0 : 0.0;
-    const double se = (i > 0 && j < P.nx) ? 1.0 : 0.0;
-    const double nw = (i < P.ny && j > 0) ? 1.0 : 0.0;
-    const double ne = (i < P.ny && j < P.nx) ? 1.0 : 0.0;
-    double mc = (2.0 * sw + se + nw + 2.0 * ne) * P.mh;
-    double mw = (sw + nw) * P.mo, me = (se + ne) * P.mo;
-    double ms = (sw + se) * P.mo, mn = (nw + ne) * P.mo;
-    const double msw = 2.0 * sw * P.mo, mne = 2.0 * ne * P.mo;
+    const T one = (T)1, zero = (T)0, two = (T)2;
+    const T mh = (T)P.mh, mo = (T)P.mo;
+    const T sw = (i > 0 && j > 0) ? one : zero;
+    const T se = (i > 0 && j < P.nx) ? one : zero;
+    const T nw = (i < P.ny && j > 0) ? one : zero;
+    const T ne = (i < P.ny && j < P.nx) ? one : zero;
+    Row<T> r;
+    r.mc = (two * sw + se + nw + two * ne) * mh;
+    r.mw = (sw + nw) * mo;
+    r.me = (se + ne) * mo;
+    r.ms = (sw + se) * mo;
+    r.mn = (nw + ne) * mo;
+    r.msw = two * sw * mo;
+    r.mne = two * ne * mo;
+    r.kw = (sw + nw) * (T)P.sa;
+    r.ke = (se + ne) * (T)P.sa;
+    r.ks = (sw + se) * (T)P.sb;
+    r.kn = (nw + ne) * (T)P.sb;
     // Robin wall edges: dt*r*|e|*[[1/3,1/6],[1/6,1/3]]  (fenics/hslD.h:3284-3441)
-    const double dn = (i > 0) ? 1.0 : 0.0, up = (i < P.ny) ? 1.0 : 0.0;
-    const double lf = (j > 0) ? 1.0 : 0.0, rt = (j < P.nx) ? 1.0 : 0.0;
+    const T third = (T)(1.0 / 3.0), sixth = (T)(1.0 / 6.0);
+    const T dn = (i > 0) ? one : zero, up = (i < P.ny) ? one : zero;
+    const T lf = (j > 0) ? one : zero, rt = (j < P.nx) ? one : zero;
     if (j == 0 && P.rob[kLeft] != 0.0) {
-        mc += (dn + up) * P.rob[kLeft] * (1.0 / 3.0);
-        ms += dn * P.rob[kLeft] * (1.0 / 6.0);
-        mn += up * P.rob[kLeft] * (1.0 / 6.0);
+        const T q = (T)P.rob[kLeft];
+        r.mc += (dn + up) * q * third;
+        r.ms += dn * q * sixth;
+        r.mn += up * q * sixth;
     }
     if (j == P.nx && P.rob[kRight] != 0.0) {
-        mc += (dn + up) * P.rob[kRight] * (1.0 / 3.0);
-        ms += dn * P.rob[kRight] * (1.0 / 6.0);
-        mn += up * P.rob[kRight] * (1.0 / 6.0);
+        const T q = (T)P.rob[kRight];
+        r.mc += (dn + up) * q * third;
+        r.ms += dn * q * sixth;
+        r.mn += up * q * sixth;
     }
     if (i == 0 && P.rob[kBottom] != 0.0) {
-        mc += (lf + rt) * P.rob[kBottom] * (1.0 / 3.0);
-        mw += lf * P.rob[kBottom] * (1.0 / 6.0);
-        me += rt * P.rob[kBottom] * (1.0 / 6.0);
+        const T q = (T)P.rob[kBottom];
+        r.mc += (lf + rt) * q * third;
+        r.mw += lf * q * sixth;
+        r.me += rt * q * sixth;
     }
     if (i == P.ny && P.rob[kTop] != 0.0) {
-        mc += (lf + rt) * P.rob[kTop] * (1.0 / 3.0);
-        mw += lf * P.rob[kTop] * (1.0 / 6.0);
-        me += rt * P.rob[kTop] * (1.0 / 6.0);
+        const T q = (T)P.rob[kTop];
+        r.mc += (lf + rt) * q * third;
+        r.mw += lf * q * sixth;
+        r.me += rt * q * sixth;
     }
-    Row<T> r;
-    r.mc = (T)mc; r.mw = (T)mw; r.me = (T)me; r.ms = (T)ms; r.mn = (T)mn;
-    r.msw = (T)msw; r.mne = (T)mne;
-    r.kw = (T)((sw + nw) * P.sa);
-    r.ke = (T)((se + ne) * P.sa);
-    r.ks = (T)((sw + se) * P.sb);
-    r.kn = (T)((nw + ne) * P.sb);
     return r;
 }
 
@@ -146,6 +180,56 @@ template <typename T>
 EQ_HD inline T row_diag(const Row<T>& r)
 {
     return r.mc + ((r.kw + r.ke) + (r.ks + r.kn));
+}
+
+// interior rows come from the per-level table, the rest are built in place
+template <typename T>
+EQ_HD inline Row<T> row_coefs(const LevelProb& P, int i, int j)
+{
+    if (interior(P, i, j)) return interior_row<T>(P);
+    return row_coefs_at<T>(P, i, j);
+}
+
+// 1 / diag(A) at node (i,j) whose row is rc
+template <typename T>
+EQ_HD inline T row_dinv(const LevelProb& P, int i, int j, const Row<T>& rc)
+{
+    if (interior(P, i, j)) return interior_dinv<T>(P);
+    return (T)1 / row_diag(rc);
+}
+
+// fills the interior row of a level (host)
+inline void set_interior_row(LevelProb& P)
+{
+    LevelProb Q = P;
+    Q.nx = 2;
+    Q.ny = 2;
+    P.rin = row_coefs_at<double>(Q, 1, 1);
+    P.rinf = row_coefs_at<float>(Q, 1, 1);
+    P.rin_dinv = 1.0 / row_diag(P.rin);
+    P.rinf_dinv = 1.0f / row_diag(P.rinf);
+    P.rnx = 1.0 / P.nx;
+    P.rny = 1.0 / P.ny;
+    P.rnxf = 1.0f / (float)P.nx;
+    P.rnyf = 1.0f / (float)P.ny;
+}
+
+template <typename T> EQ_HD inline T recip_nx(const LevelProb& P);
+template <> EQ_HD inline double recip_nx<double>(const LevelProb& P) { return P.rnx; }
+template <> EQ_HD inline float recip_nx<float>(const LevelProb& P) { return P.rnxf; }
+template <typename T> EQ_HD inline T recip_ny(const LevelProb& P);
+template <> EQ_HD inline double recip_ny<double>(const LevelProb& P) { return P.rny; }
+template <> EQ_HD inline float recip_ny<float>(const LevelProb& P) { return P.rnyf; }
+
+// floor(a / b) for a >= 0, b > 0, a / b < 2^22 (and a + b < 2^31), from the
+// FP32 reciprocal rb ~ 1/b and one exact correction step each way (the FP32
+// quotient is off by less than one)
+EQ_HD inline int div_floor_pos(int a, int b, float rb)
+{
+    int q = (int)((float)a * rb);
+    if (q * b > a) --q;
+    if ((q + 1) * b <= a) ++q;
+    return q;
 }
 
 // y_c = (A v)_c from the seven values around the node
@@ -198,7 +282,5 @@ EQ_HD inline T hat(int du_num, int dv_num, T inx, T iny)
     return w > (T)0 ? w : (T)0;
 }
 
-// floor(a / b) for b > 0 (a may be negative)
-EQ_HD inline int floor_div(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
 
 } // namespace eqk

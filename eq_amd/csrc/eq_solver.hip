@@ -71,6 +71,12 @@ struct NodeCtx {
     bool in;
 };
 
+// A thread owns R consecutive rows of one column of its tile (the level's
+// tile height is kWaves * R): decode gives the first, each_row visits them
+// all (unrolled, so the loads of all rows are in flight together and the
+// block's set-up is paid once per R nodes).
+
+template <int R>
 __device__ __forceinline__ bool decode(const LevelView& L, NodeCtx& c)
 {
     c.t = tile_index();
@@ -81,10 +87,24 @@ __device__ __forceinline__ bool decode(const LevelView& L, NodeCtx& c)
     c.gt = c.P->tile_begin + tr.local;
     const int ty = tr.local / c.P->tiles_x, tx = tr.local - ty * c.P->tiles_x;
     c.j = tx * kTileW + (int)(threadIdx.x & (kTileW - 1));
-    c.i = ty * kTileH + (int)(threadIdx.x / kTileW);
+    c.i = ty * (kWaves * R) + (int)(threadIdx.x / kTileW) * R;
     c.in = (c.i <= c.P->ny) && (c.j <= c.P->nx);
     c.idx = c.P->origin + (int64_t)c.i * c.P->pitch + c.j;
     return true;
+}
+
+template <int R, typename F>
+__device__ __forceinline__ void each_row(const NodeCtx& c, F&& f)
+{
+    const LevelProb& P = *c.P;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        NodeCtx n = c;
+        n.i = c.i + r;
+        n.idx = c.idx + (int64_t)r * P.pitch;
+        n.in = (n.i <= P.ny) && (c.j <= P.nx);
+        f(n);
+    }
 }
 
 __device__ __forceinline__ double wave_sum(double v)
@@ -151,21 +171,22 @@ __global__ __launch_bounds__(kBlock) void k_rhs(LevelView L, const double* __res
                                                 double* partial)
 {
     NodeCtx c;
-    if (!decode(L, c)) return;
+    if (!decode<kRows0>(L, c)) return;
     const LevelProb& P = *c.P;
     double v[1] = {0.0};
-    if (c.in) {
+    each_row<kRows0>(c, [&](const NodeCtx& n) {
+        if (!n.in) return;
         double uc, uw, ue, us, un, usw, une;
-        load7(u, c.idx, P.pitch, uc, uw, ue, us, un, usw, une);
-        const double bv = rhs_row(P, c.i, c.j, uc, uw, ue, us, un, usw, une);
-        b[c.idx] = bv;
-        if (is_dirichlet(P, c.i, c.j)) {
-            const double g = dirichlet_value(P, c.i, c.j);
-            v[0] = g * g;
+        load7(u, n.idx, P.pitch, uc, uw, ue, us, un, usw, une);
+        const double bv = rhs_row(P, n.i, n.j, uc, uw, ue, us, un, usw, une);
+        b[n.idx] = bv;
+        if (is_dirichlet(P, n.i, n.j)) {
+            const double g = dirichlet_value(P, n.i, n.j);
+            v[0] += g * g;
         } else {
-            v[0] = bv * bv;
+            v[0] += bv * bv;
         }
-    }
+    });
     tile_partial<1>(v, partial, c.gt);
 }
 
@@ -179,19 +200,20 @@ __global__ __launch_bounds__(kBlock) void k_init(LevelView L, double* x, const d
                                                  int jacobi)
 {
     NodeCtx c;
-    if (!decode(L, c)) return;
+    if (!decode<kRows0>(L, c)) return;
     const LevelProb& P = *c.P;
     const bool prev = st[P.prob].have_prev != 0;
     const double* __restrict__ src = prev ? xprev : x;
     double v[2] = {0.0, 0.0};
-    if (c.in) {
-        if (is_dirichlet(P, c.i, c.j)) {
-            x[c.idx] = dirichlet_value(P, c.i, c.j);
-            r[c.idx] = 0.0;
-            if (jacobi) z[c.idx] = 0.0;
+    each_row<kRows0>(c, [&](const NodeCtx& n) {
+        if (!n.in) return;
+        if (is_dirichlet(P, n.i, n.j)) {
+            x[n.idx] = dirichlet_value(P, n.i, n.j);
+            r[n.idx] = 0.0;
+            if (jacobi) z[n.idx] = 0.0;
         } else {
-            const int i = c.i, j = c.j, pt = P.pitch;
-            const int64_t k = c.idx;
+            const int i = n.i, j = n.j, pt = P.pitch;
+            const int64_t k = n.idx;
             // Dirichlet neighbours take their wall value (x is being set to g there)
             auto val = [&](int ii, int jj, int64_t kk) -> double {
                 return is_dirichlet(P, ii, jj) ? dirichlet_value(P, ii, jj) : src[kk];
@@ -204,14 +226,14 @@ __global__ __launch_bounds__(kBlock) void k_init(LevelView L, double* x, const d
             const Row<double> rc = row_coefs<double>(P, i, j);
             const double rv = b[k] - row_apply(rc, xc, xw, xe, xs, xn, xsw, xne);
             r[k] = rv;
-            v[0] = rv * rv;
+            v[0] += rv * rv;
             if (jacobi) {
-                const double zv = rv / row_diag(rc);
+                const double zv = rv * row_dinv(P, i, j, rc);
                 z[k] = zv;
-                v[1] = rv * zv;
+                v[1] += rv * zv;
             }
         }
-    }
+    });
     tile_partial<2>(v, partial, c.gt);
 }
 
@@ -220,8 +242,10 @@ __global__ __launch_bounds__(kBlock) void k_keep(LevelView L, const double* __re
                                                  double* __restrict__ xprev, CgState* st)
 {
     NodeCtx c;
-    if (!decode(L, c)) return;
-    if (c.in) xprev[c.idx] = x[c.idx];
+    if (!decode<kRows0>(L, c)) return;
+    each_row<kRows0>(c, [&](const NodeCtx& n) {
+        if (n.in) xprev[n.idx] = x[n.idx];
+    });
     if (c.gt == c.P->tile_begin && threadIdx.x == 0) st[c.P->prob].have_prev = 1;
 }
 
@@ -235,27 +259,28 @@ __global__ __launch_bounds__(kBlock) void k_ap_dir(LevelView L, const TP* __rest
                                                    const CgState* st)
 {
     NodeCtx c;
-    if (!decode(L, c)) return;
+    if (!decode<kRows0>(L, c)) return;
     const LevelProb& P = *c.P;
     const CgState& s = st[P.prob];
     if (s.done) return;
     const double beta = s.beta;
     double v[1] = {0.0};
-    if (c.in) {
+    auto dir = [beta](TP zz, TP pp) -> double { return (double)(TP)((double)zz + beta * (double)pp); };
+    each_row<kRows0>(c, [&](const NodeCtx& n) {
+        if (!n.in) return;
         TP zc, zw, ze, zs, zn, zsw, zne, pc, pw, pe, ps, pn, psw, pne;
-        load7(z, c.idx, P.pitch, zc, zw, ze, zs, zn, zsw, zne);
-        load7(pold, c.idx, P.pitch, pc, pw, pe, ps, pn, psw, pne);
-        auto dir = [beta](TP zz, TP pp) -> double { return (double)(TP)((double)zz + beta * (double)pp); };
+        load7(z, n.idx, P.pitch, zc, zw, ze, zs, zn, zsw, zne);
+        load7(pold, n.idx, P.pitch, pc, pw, pe, ps, pn, psw, pne);
         const double nc = dir(zc, pc);
         double qv = 0.0;
-        if (!is_dirichlet(P, c.i, c.j)) {
-            const Row<double> rc = row_coefs<double>(P, c.i, c.j);
+        if (!is_dirichlet(P, n.i, n.j)) {
+            const Row<double> rc = row_coefs<double>(P, n.i, n.j);
             qv = row_apply(rc, nc, dir(zw, pw), dir(ze, pe), dir(zs, ps), dir(zn, pn), dir(zsw, psw), dir(zne, pne));
         }
-        pnew[c.idx] = (TP)nc;
-        q[c.idx] = qv;
-        v[0] = nc * qv;
-    }
+        pnew[n.idx] = (TP)nc;
+        q[n.idx] = qv;
+        v[0] += nc * qv;
+    });
     tile_partial<1>(v, partial, c.gt);
 }
 
@@ -267,28 +292,35 @@ __global__ __launch_bounds__(kBlock) void k_update(LevelView L, double* __restri
                                                    int jacobi)
 {
     NodeCtx c;
-    if (!decode(L, c)) return;
+    if (!decode<kRows0>(L, c)) return;
     const LevelProb& P = *c.P;
     const CgState& s0 = st[P.prob];
     if (s0.done) return;
     const double alpha = s0.alpha;
     const int flexible = s0.flexible;
-    double v[3] = {0.0, 0.0, 0.0};
-    if (c.in && !is_dirichlet(P, c.i, c.j)) {
-        const int64_t k = c.idx;
+    // one accumulator for whichever r.z this mode forms (two accumulators
+    // picked by the branch end up addressed through a select, in scratch)
+    double rr = 0.0, rzx = 0.0;
+    each_row<kRows0>(c, [&](const NodeCtx& n) {
+        if (!n.in || is_dirichlet(P, n.i, n.j)) return;
+        const int64_t k = n.idx;
         x[k] += alpha * (double)p[k];
         const double rv = r[k] - alpha * q[k];
         r[k] = rv;
-        v[0] = rv * rv;
+        rr += rv * rv;
+        double t = 0.0;
         if (jacobi) {
-            const Row<double> rc = row_coefs<double>(P, c.i, c.j);
-            const TP zv = (TP)(rv / row_diag(rc));
+            const Row<double> rc = row_coefs<double>(P, n.i, n.j);
+            const TP zv = (TP)(rv * row_dinv(P, n.i, n.j, rc));
             z[k] = zv;
-            v[1] = rv * (double)zv;
+            t = rv * (double)zv;
         } else if (flexible) {
-            v[2] = rv * (double)z[k];
+            t = rv * (double)z[k];
         }
-    }
+        rzx += t;
+    });
+    // slot 1: r.z (Jacobi), slot 2: r_new.z_old (flexible)
+    double v[3] = {rr, jacobi ? rzx : 0.0, jacobi ? 0.0 : rzx};
     tile_partial<3>(v, partial, c.gt);
 }
 
@@ -298,17 +330,18 @@ __global__ __launch_bounds__(kBlock) void k_true_residual(LevelView L, const dou
                                                           double* partial)
 {
     NodeCtx c;
-    if (!decode(L, c)) return;
+    if (!decode<kRows0>(L, c)) return;
     const LevelProb& P = *c.P;
     double v[1] = {0.0};
-    if (c.in && !is_dirichlet(P, c.i, c.j)) {
+    each_row<kRows0>(c, [&](const NodeCtx& n) {
+        if (!n.in || is_dirichlet(P, n.i, n.j)) return;
         double xc, xw, xe, xs, xn, xsw, xne;
-        load7(x, c.idx, P.pitch, xc, xw, xe, xs, xn, xsw, xne);
-        const Row<double> rc = row_coefs<double>(P, c.i, c.j);
-        const double rv = b[c.idx] - row_apply(rc, xc, xw, xe, xs, xn, xsw, xne);
-        r[c.idx] = rv;
-        v[0] = rv * rv;
-    }
+        load7(x, n.idx, P.pitch, xc, xw, xe, xs, xn, xsw, xne);
+        const Row<double> rc = row_coefs<double>(P, n.i, n.j);
+        const double rv = b[n.idx] - row_apply(rc, xc, xw, xe, xs, xn, xsw, xne);
+        r[n.idx] = rv;
+        v[0] += rv * rv;
+    });
     tile_partial<1>(v, partial, c.gt);
 }
 
@@ -317,19 +350,20 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_z(LevelView L, const double* 
                                                      double* __restrict__ z, double* partial, const CgState* st)
 {
     NodeCtx c;
-    if (!decode(L, c)) return;
+    if (!decode<kRows0>(L, c)) return;
     const LevelProb& P = *c.P;
     if (st[P.prob].done) return;
     double v[1] = {0.0};
-    if (c.in) {
+    each_row<kRows0>(c, [&](const NodeCtx& n) {
+        if (!n.in) return;
         double zv = 0.0;
-        if (!is_dirichlet(P, c.i, c.j)) {
-            const Row<double> rc = row_coefs<double>(P, c.i, c.j);
-            zv = r[c.idx] / row_diag(rc);
+        if (!is_dirichlet(P, n.i, n.j)) {
+            const Row<double> rc = row_coefs<double>(P, n.i, n.j);
+            zv = r[n.idx] * row_dinv(P, n.i, n.j, rc);
         }
-        z[c.idx] = zv;
-        v[0] = r[c.idx] * zv;
-    }
+        z[n.idx] = zv;
+        v[0] += r[n.idx] * zv;
+    });
     tile_partial<1>(v, partial, c.gt);
 }
 
@@ -425,12 +459,12 @@ __device__ __forceinline__ T smooth_val(const LevelProb& P, int i, int j, int64_
                                         bool zero_init, T& bv)
 {
     const Row<T> rc = row_coefs<T>(P, i, j);
-    const T d = row_diag(rc);
+    const T wd = omega * row_dinv(P, i, j, rc);
     bv = (T)b[k];
-    if (zero_init) return omega * bv / d;
+    if (zero_init) return wd * bv;
     T xc, xw, xe, xs, xn, xsw, xne;
     load7(xin, k, pitch, xc, xw, xe, xs, xn, xsw, xne);
-    return xc + omega * (bv - row_apply(rc, xc, xw, xe, xs, xn, xsw, xne)) / d;
+    return xc + wd * (bv - row_apply(rc, xc, xw, xe, xs, xn, xsw, xne));
 }
 
 template <typename T>
@@ -450,11 +484,18 @@ __device__ __forceinline__ T restrict_val(const LevelProb& Pc, const LevelProb& 
                                           const T* __restrict__ rf, int64_t forg, int fpitch)
 {
     const int nxf = Pf.nx, nyf = Pf.ny, nxc = Pc.nx, nyc = Pc.ny;
-    const int jlo = max(0, floor_div((J - 1) * nxf, nxc) + 1);
-    const int jhi = min(nxf, floor_div((J + 1) * nxf - 1, nxc));
-    const int ilo = max(0, floor_div((I - 1) * nyf, nyc) + 1);
-    const int ihi = min(nyf, floor_div((I + 1) * nyf - 1, nyc));
-    const T inx = (T)1 / (T)nxf, iny = (T)1 / (T)nyf;
+    if (nxf == 2 * nxc && nyf == 2 * nyc) {
+        // nested: weight 1 at the coincident node, 1/2 at its six stencil
+        // neighbours, 0 at NW/SE (outside nodes read the zero frame)
+        const T* c0 = rf + forg + (int64_t)(2 * I) * fpitch + 2 * J;
+        return c0[0] + (T)0.5 * (((c0[-1] + c0[1]) + (c0[-fpitch] + c0[fpitch])) + (c0[-fpitch - 1] + c0[fpitch + 1]));
+    }
+    // fine nodes strictly inside the hat's support: (J-1) nxf/nxc < j < (J+1) nxf/nxc
+    const int jlo = J == 0 ? 0 : div_floor_pos((J - 1) * nxf, nxc, Pc.rnxf) + 1;
+    const int jhi = min(nxf, div_floor_pos((J + 1) * nxf - 1, nxc, Pc.rnxf));
+    const int ilo = I == 0 ? 0 : div_floor_pos((I - 1) * nyf, nyc, Pc.rnyf) + 1;
+    const int ihi = min(nyf, div_floor_pos((I + 1) * nyf - 1, nyc, Pc.rnyf));
+    const T inx = recip_nx<T>(Pf), iny = recip_ny<T>(Pf);
     T sum = (T)0;
     for (int i = ilo; i <= ihi; ++i) {
         const T* row = rf + forg + (int64_t)i * fpitch;
@@ -471,8 +512,18 @@ __device__ __forceinline__ T prolong_val(const LevelProb& Pf, const LevelProb& P
                                          const T* __restrict__ xc, int64_t corg, int cpitch)
 {
     const int nxf = Pf.nx, nyf = Pf.ny, nxc = Pc.nx, nyc = Pc.ny;
-    const int J0 = min(nxc - 1, (j * nxc) / nxf), I0 = min(nyc - 1, (i * nyc) / nyf);
-    const T inx = (T)1 / (T)nxf, iny = (T)1 / (T)nyf;
+    if (nxf == 2 * nxc && nyf == 2 * nyc) {
+        // nested: coincident node, or the mean of the two ends of the coarse
+        // edge (horizontal, vertical or the SW-NE diagonal) the node halves
+        const int a = i & 1, bb = j & 1;
+        const T* c0 = xc + corg + (int64_t)(i >> 1) * cpitch + (j >> 1);
+        const T x00 = c0[0];
+        const T xo = a ? (bb ? c0[cpitch + 1] : c0[cpitch]) : (bb ? c0[1] : x00);
+        return (a | bb) ? (T)0.5 * (x00 + xo) : x00;
+    }
+    const int J0 = min(nxc - 1, div_floor_pos(j * nxc, nxf, Pf.rnxf));
+    const int I0 = min(nyc - 1, div_floor_pos(i * nyc, nyf, Pf.rnyf));
+    const T inx = recip_nx<T>(Pf), iny = recip_ny<T>(Pf);
     const int du = j * nxc - J0 * nxf, dv = i * nyc - I0 * nyf;
     const int64_t k = corg + (int64_t)I0 * cpitch + J0;
     return hat<T>(du, dv, inx, iny) * xc[k] + hat<T>(du - nxf, dv, inx, iny) * xc[k + 1] +
@@ -485,42 +536,45 @@ __device__ __forceinline__ T prolong_val(const LevelProb& Pf, const LevelProb& P
 // Damped Jacobi sweep over the launched tiles.  bcopy (optional) keeps b in
 // the smoother precision; dot (last level-0 sweep) adds the partial r.z with
 // b the FP64 residual and x_out the preconditioned residual.
-template <typename T, typename TB, typename TO>
+template <int R, typename T, typename TB, typename TO>
 __global__ __launch_bounds__(kBlock) void k_smooth(LevelView L, const TB* __restrict__ b, const T* __restrict__ xin,
                                                    TO* __restrict__ xout, T* __restrict__ bcopy, T omega,
                                                    int zero_init, int dot, double* partial, const CgState* st)
 {
     NodeCtx c;
-    if (!decode(L, c)) return;
+    if (!decode<R>(L, c)) return;
     const LevelProb& P = *c.P;
     if (st[P.prob].done) return;
     double v[1] = {0.0};
-    if (c.in) {
-        if (is_dirichlet(P, c.i, c.j)) {
-            xout[c.idx] = (TO)0;
-            if (bcopy) bcopy[c.idx] = (T)0;
+    each_row<R>(c, [&](const NodeCtx& n) {
+        if (!n.in) return;
+        if (is_dirichlet(P, n.i, n.j)) {
+            xout[n.idx] = (TO)0;
+            if (bcopy) bcopy[n.idx] = (T)0;
         } else {
             T bv;
-            const T out = smooth_val<T, TB>(P, c.i, c.j, c.idx, P.pitch, b, xin, omega, zero_init != 0, bv);
-            if (bcopy) bcopy[c.idx] = bv;
+            const T out = smooth_val<T, TB>(P, n.i, n.j, n.idx, P.pitch, b, xin, omega, zero_init != 0, bv);
+            if (bcopy) bcopy[n.idx] = bv;
             const TO o = (TO)out;
-            xout[c.idx] = o;
-            if (dot) v[0] = (double)b[c.idx] * (double)o;
+            xout[n.idx] = o;
+            if (dot) v[0] += (double)b[n.idx] * (double)o;
         }
-    }
+    });
     if (dot) tile_partial<1>(v, partial, c.gt);
 }
 
-template <typename T>
+template <int R, typename T>
 __global__ __launch_bounds__(kBlock) void k_residual(LevelView L, const T* __restrict__ b, const T* __restrict__ x,
                                                      T* __restrict__ r, const CgState* st)
 {
     NodeCtx c;
-    if (!decode(L, c)) return;
+    if (!decode<R>(L, c)) return;
     const LevelProb& P = *c.P;
     if (st[P.prob].done) return;
-    if (!c.in) return;
-    r[c.idx] = is_dirichlet(P, c.i, c.j) ? (T)0 : residual_val<T>(P, c.i, c.j, c.idx, P.pitch, b, x);
+    each_row<R>(c, [&](const NodeCtx& n) {
+        if (!n.in) return;
+        r[n.idx] = is_dirichlet(P, n.i, n.j) ? (T)0 : residual_val<T>(P, n.i, n.j, n.idx, P.pitch, b, x);
+    });
 }
 
 template <typename T>
@@ -528,25 +582,29 @@ __global__ __launch_bounds__(kBlock) void k_restrict(LevelView Lc, const LevelPr
                                                      const T* __restrict__ rf, T* __restrict__ bc, const CgState* st)
 {
     NodeCtx c;
-    if (!decode(Lc, c)) return;
+    if (!decode<kRowsC>(Lc, c)) return;
     const LevelProb& Pc = *c.P;
     if (st[Pc.prob].done) return;
-    if (!c.in) return;
     const LevelProb& Pf = fine[Pc.fidx];
-    bc[c.idx] = is_dirichlet(Pc, c.i, c.j) ? (T)0 : restrict_val<T>(Pc, Pf, c.i, c.j, rf, Pf.origin, Pf.pitch);
+    each_row<kRowsC>(c, [&](const NodeCtx& n) {
+        if (!n.in) return;
+        bc[n.idx] = is_dirichlet(Pc, n.i, n.j) ? (T)0 : restrict_val<T>(Pc, Pf, n.i, n.j, rf, Pf.origin, Pf.pitch);
+    });
 }
 
-template <typename T>
+template <int R, typename T>
 __global__ __launch_bounds__(kBlock) void k_prolong(LevelView Lf, const LevelProb* __restrict__ coarse,
                                                     const T* __restrict__ xc, T* __restrict__ xf, const CgState* st)
 {
     NodeCtx c;
-    if (!decode(Lf, c)) return;
+    if (!decode<R>(Lf, c)) return;
     const LevelProb& Pf = *c.P;
     if (st[Pf.prob].done) return;
-    if (!c.in || is_dirichlet(Pf, c.i, c.j)) return;
     const LevelProb& Pc = coarse[Pf.cidx];
-    xf[c.idx] += prolong_val<T>(Pf, Pc, c.i, c.j, xc, Pc.origin, Pc.pitch);
+    each_row<R>(c, [&](const NodeCtx& n) {
+        if (!n.in || is_dirichlet(Pf, n.i, n.j)) return;
+        xf[n.idx] += prolong_val<T>(Pf, Pc, n.i, n.j, xc, Pc.origin, Pc.pitch);
+    });
 }
 
 // level-0 problems whose whole V-cycle runs in one workgroup: cast the FP64
@@ -556,9 +614,11 @@ __global__ __launch_bounds__(kBlock) void k_cast_in(LevelView L, const double* _
                                                     const CgState* st)
 {
     NodeCtx c;
-    if (!decode(L, c)) return;
-    if (st[c.P->prob].done || !c.in) return;
-    b[c.idx] = (T)r[c.idx];
+    if (!decode<kRows0>(L, c)) return;
+    if (st[c.P->prob].done) return;
+    each_row<kRows0>(c, [&](const NodeCtx& n) {
+        if (n.in) b[n.idx] = (T)r[n.idx];
+    });
 }
 
 template <typename T, typename TZ>
@@ -566,14 +626,15 @@ __global__ __launch_bounds__(kBlock) void k_cast_out(LevelView L, const T* __res
                                                      TZ* __restrict__ z, double* partial, const CgState* st)
 {
     NodeCtx c;
-    if (!decode(L, c)) return;
+    if (!decode<kRows0>(L, c)) return;
     if (st[c.P->prob].done) return;
     double v[1] = {0.0};
-    if (c.in) {
-        const TZ zv = (TZ)x[c.idx];
-        z[c.idx] = zv;
-        v[0] = r[c.idx] * (double)zv;
-    }
+    each_row<kRows0>(c, [&](const NodeCtx& n) {
+        if (!n.in) return;
+        const TZ zv = (TZ)x[n.idx];
+        z[n.idx] = zv;
+        v[0] += r[n.idx] * (double)zv;
+    });
     tile_partial<1>(v, partial, c.gt);
 }
 
@@ -612,18 +673,62 @@ __device__ __forceinline__ LdsLevel<T> lds_level(T* sm, const LevelProb& P)
     return L;
 }
 
+// Visit every node of a bottom level once: interior nodes by rows (whole
+// waves take the interior row, no boundary lanes mixed in), then the
+// boundary ring, one node per thread.
+template <typename FI, typename FB>
+__device__ __forceinline__ void for_nodes_split(const LevelProb& P, FI&& fi, FB&& fb)
+{
+    for (int i = 1 + (int)(threadIdx.x >> 6); i < P.ny; i += kBottomThreads / 64)
+        for (int j = 1 + (int)(threadIdx.x & 63); j < P.nx; j += 64) fi(i, j);
+    const int nw = P.nx + 1, nr = 2 * nw + 2 * (P.ny - 1);
+    for (int t = threadIdx.x; t < nr; t += kBottomThreads) {
+        int i, j;
+        if (t < nw) {
+            i = 0;
+            j = t;
+        } else if (t < 2 * nw) {
+            i = P.ny;
+            j = t - nw;
+        } else if (t < 2 * nw + P.ny - 1) {
+            i = 1 + t - 2 * nw;
+            j = 0;
+        } else {
+            i = 1 + t - 2 * nw - (P.ny - 1);
+            j = P.nx;
+        }
+        fb(i, j);
+    }
+}
+
 template <typename T>
 __device__ __forceinline__ void lds_sweep(const LevelProb& P, const LdsLevel<T>& S, int from, int to, T omega,
                                           bool zero_init)
 {
-    EQ_FOR_NODES(P)
-    {
-        if (!is_dirichlet(P, i, j)) {
-            T bv;
+    const Row<T> ri = interior_row<T>(P);
+    const T wdi = omega * interior_dinv<T>(P);
+    T* __restrict__ xo = S.x(to);
+    const T* __restrict__ xi = S.x(from);
+    for_nodes_split(
+        P,
+        [&](int i, int j) {
             const int64_t k = S.at(i, j);
-            S.x(to)[k] = smooth_val<T, T>(P, i, j, k, S.pitch, S.b, S.x(from), omega, zero_init, bv);
-        }
-    }
+            const T bv = S.b[k];
+            if (zero_init) {
+                xo[k] = wdi * bv;
+            } else {
+                T xc, xw, xe, xs, xn, xsw, xne;
+                load7(xi, k, S.pitch, xc, xw, xe, xs, xn, xsw, xne);
+                xo[k] = xc + wdi * (bv - row_apply(ri, xc, xw, xe, xs, xn, xsw, xne));
+            }
+        },
+        [&](int i, int j) {
+            if (!is_dirichlet(P, i, j)) {
+                T bv;
+                const int64_t k = S.at(i, j);
+                xo[k] = smooth_val<T, T>(P, i, j, k, S.pitch, S.b, xi, omega, zero_init, bv);
+            }
+        });
     __syncthreads();
 }
 
@@ -640,8 +745,11 @@ __global__ __launch_bounds__(kBottomThreads) void k_bottom(BottomArgs<T> A, int 
     __shared__ int idx[kMaxLev];
     __shared__ int fin[kMaxLev];
     __shared__ int s_nl;
+    __shared__ T red[kBottomThreads]; // coarsest-level partial products
     T* sm = reinterpret_cast<T*>(eq_smem);
-    const LevelProb& P0 = A.probs[lb][list[blockIdx.x]];
+    // level records are copied into (uniform) registers: through a reference
+    // the compiler would reload them after every LDS store it cannot tell apart
+    const LevelProb P0 = A.probs[lb][list[blockIdx.x]];
     if (st[P0.prob].done) return;
     if (threadIdx.x == 0) {
         int n = 0;
@@ -662,8 +770,8 @@ __global__ __launch_bounds__(kBottomThreads) void k_bottom(BottomArgs<T> A, int 
         __syncthreads();
     }
     for (int k = 0; k < nl; ++k) {
-        const LevelProb& P = A.probs[lb + k][idx[k]];
-        const LevelProb& Pc = A.probs[lb + k + 1][idx[k + 1]];
+        const LevelProb P = A.probs[lb + k][__builtin_amdgcn_readfirstlane(idx[k])];
+        const LevelProb Pc = A.probs[lb + k + 1][__builtin_amdgcn_readfirstlane(idx[k + 1])];
         const LdsLevel<T> S = lds_level(sm, P), C = lds_level(sm, Pc);
         lds_sweep<T>(P, S, 0, 0, omega, true);
         int cur = 0;
@@ -674,10 +782,21 @@ __global__ __launch_bounds__(kBottomThreads) void k_bottom(BottomArgs<T> A, int 
         if (threadIdx.x == 0) fin[k] = cur;
         // residual into the free buffer, then restrict it into the coarser b
         T* r = S.x(cur ^ 1);
-        EQ_FOR_NODES(P)
         {
-            const int64_t kk = S.at(i, j);
-            r[kk] = is_dirichlet(P, i, j) ? (T)0 : residual_val<T>(P, i, j, kk, S.pitch, S.b, S.x(cur));
+            const Row<T> ri = interior_row<T>(P);
+            const T* __restrict__ xc_ = S.x(cur);
+            for_nodes_split(
+                P,
+                [&](int i, int j) {
+                    const int64_t kk = S.at(i, j);
+                    T xc, xw, xe, xs, xn, xsw, xne;
+                    load7(xc_, kk, S.pitch, xc, xw, xe, xs, xn, xsw, xne);
+                    r[kk] = S.b[kk] - row_apply(ri, xc, xw, xe, xs, xn, xsw, xne);
+                },
+                [&](int i, int j) {
+                    const int64_t kk = S.at(i, j);
+                    r[kk] = is_dirichlet(P, i, j) ? (T)0 : residual_val<T>(P, i, j, kk, S.pitch, S.b, xc_);
+                });
         }
         __syncthreads();
         EQ_FOR_NODES(Pc)
@@ -686,19 +805,33 @@ __global__ __launch_bounds__(kBottomThreads) void k_bottom(BottomArgs<T> A, int 
         }
         __syncthreads();
     }
-    // coarsest level: x = A^{-1} b with the dense inverse
+    // coarsest level: x = A^{-1} b with the dense inverse, stored column by
+    // column.  The columns are dealt round-robin to nch groups of n threads,
+    // so a group reads whole columns (coalesced) and every load is
+    // independent; the group partials are summed in a fixed order.
     {
-        const LevelProb& Pc = A.probs[lb + nl][idx[nl]];
+        const LevelProb Pc = A.probs[lb + nl][__builtin_amdgcn_readfirstlane(idx[nl])];
         const LdsLevel<T> C = lds_level(sm, Pc);
         const int nw = Pc.nx + 1, n = nw * (Pc.ny + 1);
         const T* D = dense + Pc.dense_off;
+        T* cb = C.x1; // b in natural order (x1 is free on the coarsest level)
+        for (int m = threadIdx.x; m < n; m += kBottomThreads) {
+            const int mi = m / nw, mj = m - mi * nw;
+            cb[m] = C.b[C.at(mi, mj)];
+        }
+        __syncthreads();
+        const int nch = max(1, min(kBottomThreads / n, kBottomThreads));
+        for (int t = threadIdx.x; t < nch * n; t += kBottomThreads) {
+            const int ch = t / n, k = t - ch * n;
+            T s = (T)0;
+#pragma unroll 4
+            for (int m = ch; m < n; m += nch) s += D[(int64_t)m * n + k] * cb[m];
+            red[t] = s;
+        }
+        __syncthreads();
         for (int k = threadIdx.x; k < n; k += kBottomThreads) {
             T s = (T)0;
-            const T* row = D + (int64_t)k * n;
-            for (int m = 0; m < n; ++m) {
-                const int mi = m / nw, mj = m - mi * nw;
-                s += row[m] * C.b[C.at(mi, mj)];
-            }
+            for (int ch = 0; ch < nch; ++ch) s += red[ch * n + k];
             const int i = k / nw, j = k - i * nw;
             C.x(0)[C.at(i, j)] = s;
         }
@@ -706,8 +839,8 @@ __global__ __launch_bounds__(kBottomThreads) void k_bottom(BottomArgs<T> A, int 
     }
     int from = 0; // buffer of the coarser level's result
     for (int k = nl - 1; k >= 0; --k) {
-        const LevelProb& P = A.probs[lb + k][idx[k]];
-        const LevelProb& Pc = A.probs[lb + k + 1][idx[k + 1]];
+        const LevelProb P = A.probs[lb + k][__builtin_amdgcn_readfirstlane(idx[k])];
+        const LevelProb Pc = A.probs[lb + k + 1][__builtin_amdgcn_readfirstlane(idx[k + 1])];
         const LdsLevel<T> S = lds_level(sm, P), C = lds_level(sm, Pc);
         int cur = fin[k];
         EQ_FOR_NODES(P)
@@ -734,21 +867,22 @@ __global__ __launch_bounds__(kBlock) void k_apply_full(LevelView L, int pi_only,
                                                        double* __restrict__ y, int mode)
 {
     NodeCtx c;
-    if (!decode(L, c)) return;
-    if (c.pi != pi_only || !c.in) return;
+    if (!decode<kRows0>(L, c)) return;
+    if (c.pi != pi_only) return;
     const LevelProb& P = *c.P;
-    double vc, vw, ve, vs, vn, vsw, vne;
-    load7(x, c.idx, P.pitch, vc, vw, ve, vs, vn, vsw, vne);
-    if (is_dirichlet(P, c.i, c.j)) {
-        y[c.idx] = (mode == 0) ? vc : dirichlet_value(P, c.i, c.j);
-        return;
-    }
-    if (mode == 0) {
-        const Row<double> rc = row_coefs<double>(P, c.i, c.j);
-        y[c.idx] = row_apply(rc, vc, vw, ve, vs, vn, vsw, vne);
-    } else {
-        y[c.idx] = rhs_row(P, c.i, c.j, vc, vw, ve, vs, vn, vsw, vne);
-    }
+    each_row<kRows0>(c, [&](const NodeCtx& n) {
+        if (!n.in) return;
+        double vc, vw, ve, vs, vn, vsw, vne;
+        load7(x, n.idx, P.pitch, vc, vw, ve, vs, vn, vsw, vne);
+        if (is_dirichlet(P, n.i, n.j)) {
+            y[n.idx] = (mode == 0) ? vc : dirichlet_value(P, n.i, n.j);
+        } else if (mode == 0) {
+            const Row<double> rc = row_coefs<double>(P, n.i, n.j);
+            y[n.idx] = row_apply(rc, vc, vw, ve, vs, vn, vsw, vne);
+        } else {
+            y[n.idx] = rhs_row(P, n.i, n.j, vc, vw, ve, vs, vn, vsw, vne);
+        }
+    });
 }
 
 // ------------------------------------------------------------ boundary flux
@@ -861,6 +995,7 @@ static void fill_level_prob(LevelProb& P, const eq_field_desc& d, int nx, int ny
     }
     P.cidx = -1;
     P.fidx = -1;
+    set_interior_row(P);
 }
 
 // Dense inverse of the eliminated operator on a coarsest level (FP64 Gauss-Jordan
@@ -1031,7 +1166,8 @@ static int build_hierarchy(eq_batch* b)
             P.origin = off + kRowPad + P.pitch;
             off += round_up(kRowPad + rows * P.pitch + kRowPad, kRowPad);
             P.tiles_x = (P.nx + 1 + kTileW - 1) / kTileW;
-            P.tiles_y = (P.ny + 1 + kTileH - 1) / kTileH;
+            const int th = kWaves * rows_for_level(l); // tile height on this level
+            P.tiles_y = (P.ny + 1 + th - 1) / th;
             P.ntiles = P.tiles_x * P.tiles_y;
             P.tile_begin = tile;
             tile += P.ntiles;
@@ -1067,7 +1203,9 @@ static int build_hierarchy(eq_batch* b)
                 std::vector<double> inv;
                 if (!dense_inverse(P, inv)) return set_error(EQ_ERR_INVALID, "singular coarse operator");
                 P.dense_off = (int64_t)dense_all.size();
-                for (double v : inv) dense_all.push_back((T)v);
+                const int n = (P.nx + 1) * (P.ny + 1);
+                for (int m = 0; m < n; ++m) // column-major
+                    for (int k = 0; k < n; ++k) dense_all.push_back((T)inv[(size_t)k * n + m]);
             }
         }
     }
@@ -1138,6 +1276,15 @@ struct VCycle {
 
     void run(int l)
     {
+        if (l == 0)
+            run_level<kRows0>(l);
+        else
+            run_level<kRowsC>(l);
+    }
+
+    template <int R>
+    void run_level(int l)
+    {
         const eq_solver_desc& S = b->solver;
         Level& L = b->levels[l];
         const T omega = (T)S.mg_omega;
@@ -1155,38 +1302,38 @@ struct VCycle {
             const T* bl = top ? (b->mixed ? (const T*)L.b : (const T*)b->r) : (const T*)L.b;
             if (top) {
                 prof_begin(b, EQ_KERNEL_SMOOTH);
-                k_smooth<T, double, T><<<gs, kBlock, 0, s>>>(L.smooth(), b->r, nullptr, X[0],
+                k_smooth<R, T, double, T><<<gs, kBlock, 0, s>>>(L.smooth(), b->r, nullptr, X[0],
                                                              b->mixed ? (T*)L.b : nullptr, omega, 1, 0, nullptr, st);
                 prof_end(b, EQ_KERNEL_SMOOTH, nodes * (sizeof(double) + sizeof(T) * (b->mixed ? 2 : 1)));
             } else {
-                k_smooth<T, T, T><<<gs, kBlock, 0, s>>>(L.smooth(), bl, nullptr, X[0], nullptr, omega, 1, 0, nullptr,
+                k_smooth<R, T, T, T><<<gs, kBlock, 0, s>>>(L.smooth(), bl, nullptr, X[0], nullptr, omega, 1, 0, nullptr,
                                                         st);
             }
             for (int k = 1; k < S.mg_pre; ++k) {
                 if (top) prof_begin(b, EQ_KERNEL_SMOOTH);
-                k_smooth<T, T, T><<<gs, kBlock, 0, s>>>(L.smooth(), bl, X[cur], X[cur ^ 1], nullptr, omega, 0, 0,
+                k_smooth<R, T, T, T><<<gs, kBlock, 0, s>>>(L.smooth(), bl, X[cur], X[cur ^ 1], nullptr, omega, 0, 0,
                                                         nullptr, st);
                 if (top) prof_end(b, EQ_KERNEL_SMOOTH, nodes * 3 * sizeof(T));
                 cur ^= 1;
             }
-            k_residual<T><<<gs, kBlock, 0, s>>>(L.smooth(), bl, X[cur], (T*)L.r, st);
+            k_residual<R, T><<<gs, kBlock, 0, s>>>(L.smooth(), bl, X[cur], (T*)L.r, st);
             Level& C = b->levels[l + 1];
             const int na = (int)C.tiles_active.size();
             k_restrict<T><<<grid_for(na), kBlock, 0, s>>>(C.active(), L.d_probs, (const T*)L.r, (T*)C.b, st);
             run(l + 1);
             const T* xc = final_buffer(l + 1);
-            k_prolong<T><<<gs, kBlock, 0, s>>>(L.smooth(), C.d_probs, xc, X[cur], st);
+            k_prolong<R, T><<<gs, kBlock, 0, s>>>(L.smooth(), C.d_probs, xc, X[cur], st);
             for (int k = 0; k < S.mg_post; ++k) {
                 const bool last = (k == S.mg_post - 1);
                 if (top && last) {
                     // final sweep: FP64 residual in, FP64 z out, partial r.z
                     prof_begin(b, EQ_KERNEL_SMOOTH);
-                    k_smooth<T, double, T><<<gs, kBlock, 0, s>>>(L.smooth(), b->r, X[cur], (T*)b->z, nullptr,
+                    k_smooth<R, T, double, T><<<gs, kBlock, 0, s>>>(L.smooth(), b->r, X[cur], (T*)b->z, nullptr,
                                                                       omega, 0, 1, b->partial, st);
                     prof_end(b, EQ_KERNEL_SMOOTH, nodes * (sizeof(T) + 2 * sizeof(double)));
                 } else {
                     if (top) prof_begin(b, EQ_KERNEL_SMOOTH);
-                    k_smooth<T, T, T><<<gs, kBlock, 0, s>>>(L.smooth(), bl, X[cur], X[cur ^ 1], nullptr, omega, 0,
+                    k_smooth<R, T, T, T><<<gs, kBlock, 0, s>>>(L.smooth(), bl, X[cur], X[cur ^ 1], nullptr, omega, 0,
                                                             0, nullptr, st);
                     if (top) prof_end(b, EQ_KERNEL_SMOOTH, nodes * 3 * sizeof(T));
                     cur ^= 1;
@@ -1403,8 +1550,9 @@ int eq_batch_create(const eq_field_desc* descs, int32_t nprob, int32_t nspecies,
         int lds = 0;
         if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, b->device) != hipSuccess || lds <= 0)
             lds = 64 * 1024;
-        // leave room for the kernel's static shared memory
-        b->lds_budget = std::min<int64_t>(kBottomLdsMax, (int64_t)lds - 1024);
+        // leave room for the kernel's static shared memory (tables and the
+        // coarsest-level partials)
+        b->lds_budget = std::min<int64_t>(kBottomLdsMax, (int64_t)lds - 1024 - kBottomThreads * (int64_t)sizeof(double));
     }
     b->nprob = nprob;
     b->nspecies = nspecies;
