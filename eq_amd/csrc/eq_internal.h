@@ -53,7 +53,7 @@ struct Level {
     void* b = nullptr;  // level >= 1, or level 0 FP32 copy of r
     void* xa = nullptr;
     void* xb = nullptr;
-    void* r = nullptr;
+    void* r = nullptr;  // residual (levels that restrict it in a separate launch)
     LevelView all() const { return LevelView{d_probs, d_tiles_all, (int32_t)tiles_all.size()}; }
     LevelView smooth() const { return LevelView{d_probs, d_tiles_smooth, (int32_t)tiles_smooth.size()}; }
     LevelView bottom_tiles() const { return LevelView{d_probs, d_tiles_bottom, (int32_t)tiles_bottom.size()}; }

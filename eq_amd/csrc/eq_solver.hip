@@ -33,6 +33,7 @@ constexpr int kMaxLev = 16;          // multigrid levels per problem
 constexpr int64_t kBottomLdsMax = 150 * 1024; // cap on the LDS bytes of the levels one workgroup runs
 constexpr int kBottomThreads = 1024;
 constexpr int kFinishThreads = 1024;
+constexpr double kFuseMaxNodes = 512.0 * 1024.0; // levels below this many nodes use the fused multigrid kernels
 
 // ------------------------------------------------------------ errors
 
@@ -477,18 +478,21 @@ __device__ __forceinline__ T residual_val(const LevelProb& P, int i, int j, int6
     return b[k] - row_apply(rc, xc, xw, xe, xs, xn, xsw, xne);
 }
 
-// b_c(I,J) = sum_f phi_IJ(f) r_f over the fine nodes within one coarse cell of (I,J);
-// fine node (i,j) lives at rf[forg + i*fpitch + j]
-template <typename T>
-__device__ __forceinline__ T restrict_val(const LevelProb& Pc, const LevelProb& Pf, int I, int J,
-                                          const T* __restrict__ rf, int64_t forg, int fpitch)
+// b_c(I,J) = sum_f phi_IJ(f) r_f over the fine nodes within one coarse cell of
+// (I,J); rf(i, j, k) gives r at fine node (i,j), k = i*fpitch + j, and must
+// give 0 one node outside the grid (the nested form reads there).
+template <typename T, typename RF>
+__device__ __forceinline__ T restrict_gen(const LevelProb& Pc, const LevelProb& Pf, int I, int J, int fpitch, RF&& rf)
 {
     const int nxf = Pf.nx, nyf = Pf.ny, nxc = Pc.nx, nyc = Pc.ny;
     if (nxf == 2 * nxc && nyf == 2 * nyc) {
         // nested: weight 1 at the coincident node, 1/2 at its six stencil
-        // neighbours, 0 at NW/SE (outside nodes read the zero frame)
-        const T* c0 = rf + forg + (int64_t)(2 * I) * fpitch + 2 * J;
-        return c0[0] + (T)0.5 * (((c0[-1] + c0[1]) + (c0[-fpitch] + c0[fpitch])) + (c0[-fpitch - 1] + c0[fpitch + 1]));
+        // neighbours, 0 at NW/SE
+        const int i0 = 2 * I, j0 = 2 * J;
+        const int64_t k0 = (int64_t)i0 * fpitch + j0;
+        return rf(i0, j0, k0) + (T)0.5 * (((rf(i0, j0 - 1, k0 - 1) + rf(i0, j0 + 1, k0 + 1)) +
+                                           (rf(i0 - 1, j0, k0 - fpitch) + rf(i0 + 1, j0, k0 + fpitch))) +
+                                          (rf(i0 - 1, j0 - 1, k0 - fpitch - 1) + rf(i0 + 1, j0 + 1, k0 + fpitch + 1)));
     }
     // fine nodes strictly inside the hat's support: (J-1) nxf/nxc < j < (J+1) nxf/nxc
     const int jlo = J == 0 ? 0 : div_floor_pos((J - 1) * nxf, nxc, Pc.rnxf) + 1;
@@ -498,11 +502,19 @@ __device__ __forceinline__ T restrict_val(const LevelProb& Pc, const LevelProb& 
     const T inx = recip_nx<T>(Pf), iny = recip_ny<T>(Pf);
     T sum = (T)0;
     for (int i = ilo; i <= ihi; ++i) {
-        const T* row = rf + forg + (int64_t)i * fpitch;
         const int dv = i * nyc - I * nyf;
-        for (int j = jlo; j <= jhi; ++j) sum += hat<T>(j * nxc - J * nxf, dv, inx, iny) * row[j];
+        for (int j = jlo; j <= jhi; ++j) sum += hat<T>(j * nxc - J * nxf, dv, inx, iny) * rf(i, j, (int64_t)i * fpitch + j);
     }
     return sum;
+}
+
+// the same from a stored residual: fine node (i,j) at rf[forg + i*fpitch + j],
+// zero frame around the grid
+template <typename T>
+__device__ __forceinline__ T restrict_val(const LevelProb& Pc, const LevelProb& Pf, int I, int J,
+                                          const T* __restrict__ rf, int64_t forg, int fpitch)
+{
+    return restrict_gen<T>(Pc, Pf, I, J, fpitch, [&](int, int, int64_t k) { return rf[forg + k]; });
 }
 
 // (P x_c) at fine node (i,j): the four corners of the coarse cell holding it;
@@ -605,6 +617,116 @@ __global__ __launch_bounds__(kBlock) void k_prolong(LevelView Lf, const LevelPro
         if (!n.in || is_dirichlet(Pf, n.i, n.j)) return;
         xf[n.idx] += prolong_val<T>(Pf, Pc, n.i, n.j, xc, Pc.origin, Pc.pitch);
     });
+}
+
+// ------------------------------------------------------------ fused multigrid kernels
+// Each gives bit for bit what the two launches it replaces give: the values
+// the first launch would have stored are recomputed, in the same precision
+// and order, at the stencil points the second one reads.
+
+// x after the first sweep from x = 0: omega b / diag (0 on Dirichlet nodes
+// and outside the grid)
+template <typename T, typename TB>
+__device__ __forceinline__ T first_sweep_at(const LevelProb& P, int i, int j, int64_t k, const TB* __restrict__ b,
+                                            T omega)
+{
+    if (i < 0 || j < 0 || i > P.ny || j > P.nx || is_dirichlet(P, i, j)) return (T)0;
+    const Row<T> rc = row_coefs<T>(P, i, j);
+    const T wd = omega * row_dinv(P, i, j, rc);
+    return wd * (T)b[k];
+}
+
+// pre-smoothing sweeps 1 and 2 (sweep 1 from x = 0); bcopy as in k_smooth
+template <int R, typename T, typename TB>
+__global__ __launch_bounds__(kBlock) void k_presmooth2(LevelView L, const TB* __restrict__ b, T* __restrict__ xout,
+                                                       T* __restrict__ bcopy, T omega, const CgState* st)
+{
+    NodeCtx c;
+    if (!decode<R>(L, c)) return;
+    const LevelProb& P = *c.P;
+    if (st[P.prob].done) return;
+    each_row<R>(c, [&](const NodeCtx& n) {
+        if (!n.in) return;
+        const int i = n.i, j = n.j, pt = P.pitch;
+        const int64_t k = n.idx;
+        if (is_dirichlet(P, i, j)) {
+            xout[k] = (T)0;
+            if (bcopy) bcopy[k] = (T)0;
+            return;
+        }
+        const T xc = first_sweep_at<T, TB>(P, i, j, k, b, omega);
+        const T xw = first_sweep_at<T, TB>(P, i, j - 1, k - 1, b, omega);
+        const T xe = first_sweep_at<T, TB>(P, i, j + 1, k + 1, b, omega);
+        const T xs = first_sweep_at<T, TB>(P, i - 1, j, k - pt, b, omega);
+        const T xn = first_sweep_at<T, TB>(P, i + 1, j, k + pt, b, omega);
+        const T xsw = first_sweep_at<T, TB>(P, i - 1, j - 1, k - pt - 1, b, omega);
+        const T xne = first_sweep_at<T, TB>(P, i + 1, j + 1, k + pt + 1, b, omega);
+        const Row<T> rc = row_coefs<T>(P, i, j);
+        const T wd = omega * row_dinv(P, i, j, rc);
+        const T bv = (T)b[k];
+        xout[k] = xc + wd * (bv - row_apply(rc, xc, xw, xe, xs, xn, xsw, xne));
+        if (bcopy) bcopy[k] = bv;
+    });
+}
+
+// residual of the fine level and its restriction, one coarse node per thread
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_residual_restrict(LevelView Lc, const LevelProb* __restrict__ fine,
+                                                              const T* __restrict__ bf, const T* __restrict__ xf,
+                                                              T* __restrict__ bc, const CgState* st)
+{
+    NodeCtx c;
+    if (!decode<kRowsC>(Lc, c)) return;
+    const LevelProb& Pc = *c.P;
+    if (st[Pc.prob].done) return;
+    const LevelProb& Pf = fine[Pc.fidx];
+    auto racc = [&](int i, int j, int64_t k) -> T {
+        if (i < 0 || j < 0 || i > Pf.ny || j > Pf.nx || is_dirichlet(Pf, i, j)) return (T)0;
+        return residual_val<T>(Pf, i, j, Pf.origin + k, Pf.pitch, bf, xf);
+    };
+    each_row<kRowsC>(c, [&](const NodeCtx& n) {
+        if (!n.in) return;
+        bc[n.idx] = is_dirichlet(Pc, n.i, n.j) ? (T)0 : restrict_gen<T>(Pc, Pf, n.i, n.j, Pf.pitch, racc);
+    });
+}
+
+// coarse-grid correction x += P x_c followed by the first post-smoothing
+// sweep (xin holds x, xout receives the sweep; dot as in k_smooth)
+template <int R, typename T, typename TB, typename TO>
+__global__ __launch_bounds__(kBlock) void k_prolong_smooth(LevelView Lf, const LevelProb* __restrict__ coarse,
+                                                           const T* __restrict__ xc, const TB* __restrict__ b,
+                                                           const T* __restrict__ xin, TO* __restrict__ xout, T omega,
+                                                           int dot, double* partial, const CgState* st)
+{
+    NodeCtx c;
+    if (!decode<R>(Lf, c)) return;
+    const LevelProb& P = *c.P;
+    if (st[P.prob].done) return;
+    const LevelProb& Pc = coarse[P.cidx];
+    auto y_at = [&](int i, int j, int64_t k) -> T {
+        if (i < 0 || j < 0 || i > P.ny || j > P.nx || is_dirichlet(P, i, j)) return xin[k];
+        return xin[k] + prolong_val<T>(P, Pc, i, j, xc, Pc.origin, Pc.pitch);
+    };
+    double v[1] = {0.0};
+    each_row<R>(c, [&](const NodeCtx& n) {
+        if (!n.in) return;
+        const int i = n.i, j = n.j, pt = P.pitch;
+        const int64_t k = n.idx;
+        if (is_dirichlet(P, i, j)) {
+            xout[k] = (TO)0;
+            return;
+        }
+        const T yc = y_at(i, j, k), yw = y_at(i, j - 1, k - 1), ye = y_at(i, j + 1, k + 1);
+        const T ys = y_at(i - 1, j, k - pt), yn = y_at(i + 1, j, k + pt);
+        const T ysw = y_at(i - 1, j - 1, k - pt - 1), yne = y_at(i + 1, j + 1, k + pt + 1);
+        const Row<T> rc = row_coefs<T>(P, i, j);
+        const T wd = omega * row_dinv(P, i, j, rc);
+        const T bv = (T)b[k];
+        const TO o = (TO)(yc + wd * (bv - row_apply(rc, yc, yw, ye, ys, yn, ysw, yne)));
+        xout[k] = o;
+        if (dot) v[0] += (double)b[k] * (double)o;
+    });
+    if (dot) tile_partial<1>(v, partial, c.gt);
 }
 
 // level-0 problems whose whole V-cycle runs in one workgroup: cast the FP64
@@ -1300,44 +1422,73 @@ struct VCycle {
             // pre-smoothing; the first sweep starts from x = 0; level 0 reads
             // the FP64 residual and (mixed precision) keeps an FP32 copy of it
             const T* bl = top ? (b->mixed ? (const T*)L.b : (const T*)b->r) : (const T*)L.b;
-            if (top) {
-                prof_begin(b, EQ_KERNEL_SMOOTH);
-                k_smooth<R, T, double, T><<<gs, kBlock, 0, s>>>(L.smooth(), b->r, nullptr, X[0],
-                                                             b->mixed ? (T*)L.b : nullptr, omega, 1, 0, nullptr, st);
-                prof_end(b, EQ_KERNEL_SMOOTH, nodes * (sizeof(double) + sizeof(T) * (b->mixed ? 2 : 1)));
+            T* bcopy = (top && b->mixed) ? (T*)L.b : nullptr;
+            const double bytes_b = top ? sizeof(double) : sizeof(T);
+            // Small levels run at the launch-latency floor: there the fused
+            // kernels (two launches' work in one, the first launch's values
+            // recomputed at the stencil points) win; large levels are
+            // bandwidth- and issue-bound and keep the separate launches.
+            const bool fuse = nodes < kFuseMaxNodes;
+            if (top) prof_begin(b, EQ_KERNEL_SMOOTH);
+            if (S.mg_pre >= 2 && fuse) {
+                // sweeps 1 and 2 in one launch
+                if (top)
+                    k_presmooth2<R, T, double><<<gs, kBlock, 0, s>>>(L.smooth(), b->r, X[1], bcopy, omega, st);
+                else
+                    k_presmooth2<R, T, T><<<gs, kBlock, 0, s>>>(L.smooth(), bl, X[1], nullptr, omega, st);
+                cur = 1;
+            } else if (top) {
+                k_smooth<R, T, double, T><<<gs, kBlock, 0, s>>>(L.smooth(), b->r, nullptr, X[0], bcopy, omega, 1, 0,
+                                                                nullptr, st);
             } else {
                 k_smooth<R, T, T, T><<<gs, kBlock, 0, s>>>(L.smooth(), bl, nullptr, X[0], nullptr, omega, 1, 0, nullptr,
-                                                        st);
+                                                           st);
             }
-            for (int k = 1; k < S.mg_pre; ++k) {
+            if (top) prof_end(b, EQ_KERNEL_SMOOTH, nodes * (bytes_b + sizeof(T) * (bcopy ? 2 : 1)));
+            for (int k = (S.mg_pre >= 2 && fuse) ? 2 : 1; k < S.mg_pre; ++k) {
                 if (top) prof_begin(b, EQ_KERNEL_SMOOTH);
                 k_smooth<R, T, T, T><<<gs, kBlock, 0, s>>>(L.smooth(), bl, X[cur], X[cur ^ 1], nullptr, omega, 0, 0,
-                                                        nullptr, st);
+                                                           nullptr, st);
                 if (top) prof_end(b, EQ_KERNEL_SMOOTH, nodes * 3 * sizeof(T));
                 cur ^= 1;
             }
-            k_residual<R, T><<<gs, kBlock, 0, s>>>(L.smooth(), bl, X[cur], (T*)L.r, st);
+            // residual and its restriction over the coarse nodes
             Level& C = b->levels[l + 1];
             const int na = (int)C.tiles_active.size();
-            k_restrict<T><<<grid_for(na), kBlock, 0, s>>>(C.active(), L.d_probs, (const T*)L.r, (T*)C.b, st);
+            if (fuse) {
+                k_residual_restrict<T><<<grid_for(na), kBlock, 0, s>>>(C.active(), L.d_probs, bl, X[cur], (T*)C.b, st);
+            } else {
+                k_residual<R, T><<<gs, kBlock, 0, s>>>(L.smooth(), bl, X[cur], (T*)L.r, st);
+                k_restrict<T><<<grid_for(na), kBlock, 0, s>>>(C.active(), L.d_probs, (const T*)L.r, (T*)C.b, st);
+            }
             run(l + 1);
             const T* xc = final_buffer(l + 1);
-            k_prolong<R, T><<<gs, kBlock, 0, s>>>(L.smooth(), C.d_probs, xc, X[cur], st);
+            // coarse-grid correction fused with the first post-smoothing sweep;
+            // the last level-0 sweep reads the FP64 residual, writes z and the
+            // partial r.z
             for (int k = 0; k < S.mg_post; ++k) {
                 const bool last = (k == S.mg_post - 1);
+                const bool first = (k == 0) && fuse;
+                if (k == 0 && !fuse)
+                    k_prolong<R, T><<<gs, kBlock, 0, s>>>(L.smooth(), C.d_probs, xc, X[cur], st);
+                if (top) prof_begin(b, EQ_KERNEL_SMOOTH);
                 if (top && last) {
-                    // final sweep: FP64 residual in, FP64 z out, partial r.z
-                    prof_begin(b, EQ_KERNEL_SMOOTH);
-                    k_smooth<R, T, double, T><<<gs, kBlock, 0, s>>>(L.smooth(), b->r, X[cur], (T*)b->z, nullptr,
-                                                                      omega, 0, 1, b->partial, st);
-                    prof_end(b, EQ_KERNEL_SMOOTH, nodes * (sizeof(T) + 2 * sizeof(double)));
+                    if (first)
+                        k_prolong_smooth<R, T, double, T><<<gs, kBlock, 0, s>>>(L.smooth(), C.d_probs, xc, b->r, X[cur],
+                                                                                (T*)b->z, omega, 1, b->partial, st);
+                    else
+                        k_smooth<R, T, double, T><<<gs, kBlock, 0, s>>>(L.smooth(), b->r, X[cur], (T*)b->z, nullptr,
+                                                                        omega, 0, 1, b->partial, st);
                 } else {
-                    if (top) prof_begin(b, EQ_KERNEL_SMOOTH);
-                    k_smooth<R, T, T, T><<<gs, kBlock, 0, s>>>(L.smooth(), bl, X[cur], X[cur ^ 1], nullptr, omega, 0,
-                                                            0, nullptr, st);
-                    if (top) prof_end(b, EQ_KERNEL_SMOOTH, nodes * 3 * sizeof(T));
+                    if (first)
+                        k_prolong_smooth<R, T, T, T><<<gs, kBlock, 0, s>>>(L.smooth(), C.d_probs, xc, bl, X[cur],
+                                                                           X[cur ^ 1], omega, 0, nullptr, st);
+                    else
+                        k_smooth<R, T, T, T><<<gs, kBlock, 0, s>>>(L.smooth(), bl, X[cur], X[cur ^ 1], nullptr, omega, 0,
+                                                                   0, nullptr, st);
                     cur ^= 1;
                 }
+                if (top) prof_end(b, EQ_KERNEL_SMOOTH, nodes * (sizeof(T) * 2 + bytes_b));
             }
         }
         if (!L.bottom.empty()) {
