@@ -6,20 +6,22 @@
 // q_c/|S_c| to every covered node of every species (writeHSL, 338-359).  A
 // cell therefore samples the deposits of all earlier cells.
 //
-// GPU plan (deterministic, no atomics anywhere):
-//   1. k_cell_geo    one wavefront per cell: pole window, |S_c| (ballot counts)
-//   2. k_scan        exclusive scan of (window rows x row segments) per cell
-//   3. k_band_masks  one wavefront per band of kBand rows: bit c of word c/64
-//                    says cell c's window meets the band (ballots)
-//   4. k_cell_rows   one wavefront per (row, column segment, species): the
-//                    segment lives in LDS; the wave visits the band's cells
-//                    in list order, and for each cell covering the row sums
-//                    the covered values (the read) and then adds q_c/|S_c|
-//                    (the write).  Per node the additions happen in list
-//                    order, so the deposited field is bit-identical to the
-//                    sequential loop.
-//   5. k_cell_sample one thread per cell: mean over its row partials.
+// GPU plan (deterministic: every node sees its cells in list order, every
+// sum is formed in a fixed order), batched over a range of traps:
+//   1. k_cell_geo     one wavefront per cell: pole window, |S_c| (ballot
+//                     counts), and the 64x16-node tiles the window touches
+//   2. k_pair_scan    exclusive scan of the per-cell tile counts
+//   3. k_pairs        one (tile, cell) pair per touched tile, cell-major
+//   4. radix sort     pairs by global tile id, stable, so every tile's cells
+//                     come out in list order
+//   5. k_tile_bounds  first/last pair of every tile
+//   6. k_tile_couple  one block per tile, four rows of one column per
+//                     thread, all species in registers: for each of the
+//                     tile's cells in order, the covered nodes' sum (block
+//                     reduction, one partial per pair) and then the deposit
+//   7. k_cell_sample  per cell: its pair partials in tile order / |S_c|
 #include <hip/hip_runtime.h>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -31,21 +33,30 @@
 
 namespace eqk {
 
-constexpr int kSegMax = 512;  // LDS row segment capacity (4 KB of FP64)
-constexpr int kBand = 32;     // rows per band of the cell bitmasks
+constexpr int kCTileW = 64;                 // tile columns (one wavefront)
+constexpr int kCRows = 4;                   // rows per thread
+constexpr int kCTileH = 4 * kCRows;         // tile rows (4 wavefronts)
+constexpr int kCThreads = 256;
+constexpr int kSpeciesPass = 4;             // species held in registers per pass
 
 struct CellGeo {
     double cx, cy, cs, sn, o, rad; // centre, cos/sin of the angle, box half-length/half-width
     int32_t i1, i2, j1, j2;        // search window (inclusive); the centre node for fallback cells
     int32_t cnt;                   // nodes found by the scan (0: centre-node fallback)
-    int32_t nseg0;                 // first row segment touched
-    int64_t poff;                  // offset of this cell's row partials
+    int32_t trap;                  // trap index (in the batch)
+    int32_t tx1, ty1, ntx;         // first tile column/row of the window, tile columns
+    int32_t npairs;                // tiles the window touches
+    int64_t poff;                  // first pair of this cell
 };
 
-struct GridDesc {
+// per trap of the batch (device table)
+struct TrapGrid {
     int32_t nodes_w, nodes_h;
-    double npm, trap_w, trap_h;
-    int32_t segw, nseg;            // column segment width and count
+    int32_t tiles_x, tiles_y;
+    int32_t tile_base;             // first global tile id
+    int32_t prob0;                 // problem of species 0
+    int32_t pitch;
+    double trap_w, trap_h;
 };
 
 // size_t(round(v)) of eQ::data::ij_from_xy (src/eQ.h:119-124); negative values clamp to 0
@@ -63,19 +74,28 @@ __device__ __forceinline__ int32_t to_index(double v, double npm)
 // cpvcross(edge, p - vertex) < 0 tests are exactly -o < lx < o and
 // -r < ly < r (products with a zero edge component are +-0, and the sign of
 // a rounded difference is the sign of the exact difference).
-__device__ __forceinline__ bool inside_cell(double cx, double cy, double cs, double sn, double o, double rad,
-                                            double X, double Y)
+__device__ __forceinline__ double inv_tx(double cx, double cy, double cs, double sn) { return (-sn) * cy - cs * cx; }
+__device__ __forceinline__ double inv_ty(double cx, double cy, double cs, double sn) { return sn * cx - cs * cy; }
+
+// the test with the products of the node coordinates formed by the caller:
+// csX = cs*X, snX = (-sn)*X, snY = sn*Y, csY = cs*Y
+__device__ __forceinline__ bool inside_box(double csX, double snX, double snY, double csY, double itx, double ity,
+                                           double o, double rad)
 {
-    const double itx = (-sn) * cy - cs * cx;
-    const double ity = sn * cx - cs * cy;
-    const double lx = cs * X + sn * Y + itx;
-    const double ly = (-sn) * X + cs * Y + ity;
+    const double lx = csX + snY + itx;
+    const double ly = snX + csY + ity;
     return (lx - o < 0.0) && (ly - rad < 0.0) && (ly + rad > 0.0) && (lx + o > 0.0);
 }
 
+__device__ __forceinline__ bool inside_cell(double cx, double cy, double cs, double sn, double o, double rad,
+                                            double X, double Y)
+{
+    return inside_box(cs * X, (-sn) * X, sn * Y, cs * Y, inv_tx(cx, cy, cs, sn), inv_ty(cx, cy, cs, sn), o, rad);
+}
+
 // Ecoli::updatePoleCenters (src/abm/Ecoli.cpp:36-63)
-__device__ __forceinline__ void pole(const GridDesc& G, double cx, double cy, double cs, double sn, double L, double w,
-                                     int r, double& px, double& py)
+__device__ __forceinline__ void pole(const TrapGrid& G, double cx, double cy, double cs, double sn, double L,
+                                     double w, int r, double& px, double& py)
 {
     const double s = ((double)r * 0.5) * L - w / 2.0;
     px = cx + cs * s;
@@ -87,12 +107,45 @@ __device__ __forceinline__ void pole(const GridDesc& G, double cx, double cy, do
 }
 #pragma clang fp contract(on)
 
-__global__ __launch_bounds__(256) void k_cell_geo(const eq_cell* __restrict__ cells, int nc, GridDesc G,
+// row_shr:n of both halves of a double, zero where the source lane is
+// outside the 16-lane row
+template <int N>
+__device__ __forceinline__ double dpp_shr(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x110 + N, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x110 + N, 0xf, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double readlane_d(double v, int k)
+{
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), k);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), k);
+    return __hiloint2double(hi, lo);
+}
+
+// trap holding cell c: offsets[t] <= c < offsets[t+1] (offsets of the traps
+// of this call, ntr + 1 entries)
+__device__ __forceinline__ int trap_of(const int64_t* __restrict__ off, int ntr, int64_t c)
+{
+    int lo = 0, hi = ntr - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= c) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void k_cell_geo(const eq_cell* __restrict__ cells, int64_t nc,
+                                                  const int64_t* __restrict__ off, int ntr, int t0,
+                                                  const TrapGrid* __restrict__ traps, double npm,
                                                   CellGeo* __restrict__ geo)
 {
     const int lane = threadIdx.x & 63;
-    const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int64_t c = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (c >= nc) return;
+    const int t = t0 + trap_of(off, ntr, c);
+    const TrapGrid G = traps[t];
     const eq_cell cell = cells[c];
     CellGeo g;
     g.cx = cell.x;
@@ -103,198 +156,271 @@ __global__ __launch_bounds__(256) void k_cell_geo(const eq_cell* __restrict__ ce
     g.o = (cell.length - cell.width) * 0.5;
     g.rad = cell.width * 0.5;
     // search window (src/abm/eQabm.cpp:268-287); nodesToEdge from the default 1 um width (eQabm.cpp:75)
-    const int32_t nte = (int32_t)round(G.npm * 1.0 / 2.0);
+    const int32_t nte = (int32_t)round(npm * 1.0 / 2.0);
     double ax, ay, bx, by;
     pole(G, g.cx, g.cy, g.cs, g.sn, cell.length, cell.width, 1, ax, ay);
     pole(G, g.cx, g.cy, g.cs, g.sn, cell.length, cell.width, -1, bx, by);
-    const int32_t Ai = to_index(ay, G.npm), Aj = to_index(ax, G.npm);
-    const int32_t Bi = to_index(by, G.npm), Bj = to_index(bx, G.npm);
+    const int32_t Ai = to_index(ay, npm), Aj = to_index(ax, npm);
+    const int32_t Bi = to_index(by, npm), Bj = to_index(bx, npm);
     int32_t i1 = min(Ai, Bi), i2 = max(Ai, Bi), j1 = min(Aj, Bj), j2 = max(Aj, Bj);
     i1 = (i1 >= nte) ? i1 - nte : 0;
     j1 = (j1 >= nte) ? j1 - nte : 0;
     i2 = (i2 + nte >= G.nodes_h - 1) ? G.nodes_h - 1 : i2 + nte;
     j2 = (j2 + nte >= G.nodes_w - 1) ? G.nodes_w - 1 : j2 + nte;
-    const int32_t ww = j2 - j1 + 1;
-    const int64_t area = (i2 >= i1 && j2 >= j1) ? (int64_t)(i2 - i1 + 1) * ww : 0;
+    // Only nodes of the window inside the box's axis-aligned bounding box
+    // (one node of margin; the rounded transform is off by ~1e-15) can pass
+    // the strict inside test, so the scan, and everything after it, uses
+    // the window clipped to that box: the covered set is unchanged.
+    {
+        const double ex = fabs(g.o * g.cs) + fabs(g.rad * g.sn), ey = fabs(g.o * g.sn) + fabs(g.rad * g.cs);
+        const double bj1 = floor((g.cx - ex) * npm) - 1.0, bj2 = ceil((g.cx + ex) * npm) + 1.0;
+        const double bi1 = floor((g.cy - ey) * npm) - 1.0, bi2 = ceil((g.cy + ey) * npm) + 1.0;
+        if (bj1 > (double)j1) j1 = bj1 > (double)j2 ? j2 + 1 : (int32_t)bj1;
+        if (bj2 < (double)j2) j2 = bj2 < (double)j1 ? j1 - 1 : (int32_t)bj2;
+        if (bi1 > (double)i1) i1 = bi1 > (double)i2 ? i2 + 1 : (int32_t)bi1;
+        if (bi2 < (double)i2) i2 = bi2 < (double)i1 ? i1 - 1 : (int32_t)bi2;
+    }
     int32_t cnt = 0;
-    for (int64_t t0 = 0; t0 < area; t0 += 64) {
-        const int64_t t = t0 + lane;
-        bool in = false;
-        if (t < area) {
-            const int32_t i = i1 + (int32_t)(t / ww), j = j1 + (int32_t)(t % ww);
-            in = inside_cell(g.cx, g.cy, g.cs, g.sn, g.o, g.rad, (double)j / G.npm, (double)i / G.npm);
+    {
+        const double itx = inv_tx(g.cx, g.cy, g.cs, g.sn), ity = inv_ty(g.cx, g.cy, g.cs, g.sn);
+        const double msn = -g.sn;
+        for (int32_t jb = j1; jb <= j2; jb += 64) {
+            const int32_t j = jb + lane;
+            const double X = (double)j / npm;
+            const double csX = g.cs * X, snX = msn * X;
+            for (int32_t ib = i1; ib <= i2; ib += 64) {
+                // lane k holds Y of row ib + k; rows are taken one by one
+                const double Yl = (double)(ib + lane) / npm;
+                const int nr = min(64, i2 - ib + 1);
+                for (int k = 0; k < nr; ++k) {
+                    const double Y = readlane_d(Yl, k);
+                    const bool in = (j <= j2) && inside_box(csX, snX, g.sn * Y, g.cs * Y, itx, ity, g.o, g.rad);
+                    cnt += __popcll(__ballot(in));
+                }
+            }
         }
-        cnt += __popcll(__ballot(in));
     }
     g.cnt = cnt;
     if (cnt == 0) {
         // fallback: the centre node (src/abm/eQabm.cpp:299-303)
-        i1 = i2 = min(to_index(g.cy, G.npm), G.nodes_h - 1);
-        j1 = j2 = min(to_index(g.cx, G.npm), G.nodes_w - 1);
+        i1 = i2 = min(to_index(g.cy, npm), G.nodes_h - 1);
+        j1 = j2 = min(to_index(g.cx, npm), G.nodes_w - 1);
     }
     g.i1 = i1; g.i2 = i2; g.j1 = j1; g.j2 = j2;
-    g.nseg0 = j1 / G.segw;
+    g.trap = t;
+    g.tx1 = j1 / kCTileW;
+    g.ty1 = i1 / kCTileH;
+    g.ntx = j2 / kCTileW - g.tx1 + 1;
+    g.npairs = (i2 / kCTileH - g.ty1 + 1) * g.ntx;
     g.poff = 0;
     if (lane == 0) geo[c] = g;
 }
 
-__device__ __forceinline__ int64_t cell_slots(const CellGeo& g, int segw)
-{
-    return (int64_t)(g.i2 - g.i1 + 1) * (g.j2 / segw - g.j1 / segw + 1);
-}
-
-// exclusive scan of the per-cell partial slots (single block, deterministic)
-__global__ __launch_bounds__(1024) void k_scan(CellGeo* geo, int nc, int segw, int64_t* total)
+// exclusive scan of the per-cell pair counts (single block, deterministic)
+__global__ __launch_bounds__(1024) void k_pair_scan(CellGeo* geo, int64_t nc, int64_t* total)
 {
     __shared__ int64_t part[1024];
     const int t = threadIdx.x, nt = blockDim.x;
-    const int per = (nc + nt - 1) / nt;
-    const int lo = min(nc, t * per), hi = min(nc, lo + per);
+    const int64_t per = (nc + nt - 1) / nt;
+    const int64_t lo = min(nc, (int64_t)t * per), hi = min(nc, lo + per);
     int64_t s = 0;
-    for (int c = lo; c < hi; ++c) s += cell_slots(geo[c], segw);
+    for (int64_t c = lo; c < hi; ++c) s += geo[c].npairs;
     part[t] = s;
     __syncthreads();
-    if (t == 0) {
-        int64_t acc = 0;
-        for (int k = 0; k < nt; ++k) {
-            const int64_t v = part[k];
-            part[k] = acc;
-            acc += v;
-        }
-        *total = acc;
+    // inclusive Hillis-Steele scan of the thread sums
+    for (int off = 1; off < nt; off <<= 1) {
+        const int64_t v = (t >= off) ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
     }
-    __syncthreads();
-    int64_t acc = part[t];
-    for (int c = lo; c < hi; ++c) {
+    if (t == nt - 1) *total = part[t];
+    int64_t acc = part[t] - s;
+    for (int64_t c = lo; c < hi; ++c) {
         geo[c].poff = acc;
-        acc += cell_slots(geo[c], segw);
+        acc += geo[c].npairs;
     }
 }
 
-// one wavefront per band: bit (c % 64) of word c/64 = cell c's window meets the band
-__global__ __launch_bounds__(64) void k_band_masks(const CellGeo* __restrict__ geo, int nc, int nwords,
-                                                   uint64_t* __restrict__ masks)
+// pair p of cell c: key = global tile id, value = p; the cell's tiles row by row
+__global__ __launch_bounds__(256) void k_pairs(const CellGeo* __restrict__ geo, int64_t nc,
+                                               const TrapGrid* __restrict__ traps, uint32_t* __restrict__ keys,
+                                               uint32_t* __restrict__ vals, uint32_t* __restrict__ pair_cell)
 {
-    const int band = blockIdx.x, lane = threadIdx.x;
-    const int r0 = band * kBand, r1 = r0 + kBand - 1;
-    for (int w = 0; w < nwords; ++w) {
-        const int c = w * 64 + lane;
-        bool hit = false;
-        if (c < nc) hit = (geo[c].i1 <= r1 && geo[c].i2 >= r0);
-        const uint64_t m = __ballot(hit);
-        if (lane == 0) masks[(int64_t)band * nwords + w] = m;
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const CellGeo& g = geo[c];
+    const TrapGrid& G = traps[g.trap];
+    for (int k = 0; k < g.npairs; ++k) {
+        const int ty = g.ty1 + k / g.ntx, tx = g.tx1 + k % g.ntx;
+        const int64_t p = g.poff + k;
+        keys[p] = (uint32_t)(G.tile_base + ty * G.tiles_x + tx);
+        vals[p] = (uint32_t)p;
+        pair_cell[p] = (uint32_t)c;
     }
 }
 
+// [start, end) of every tile's run in the sorted pairs (tiles without pairs
+// keep the zeros they were cleared to)
+__global__ __launch_bounds__(256) void k_tile_bounds(const uint32_t* __restrict__ keys, int64_t n, uint32_t tile0,
+                                                     uint32_t* __restrict__ start, uint32_t* __restrict__ end)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t key = keys[k];
+    if (k == 0 || keys[k - 1] != key) start[key - tile0] = (uint32_t)k;
+    if (k == n - 1 || keys[k + 1] != key) end[key - tile0] = (uint32_t)(k + 1);
+}
+
+// sum over the wavefront in a fixed order: prefix sums within each 16-lane
+// row by DPP shifts, then the four row totals in lane order (no LDS round
+// trips, unlike a shuffle tree)
 __device__ __forceinline__ double wave_sum64(double v)
 {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    v += dpp_shr<1>(v);
+    v += dpp_shr<2>(v);
+    v += dpp_shr<4>(v);
+    v += dpp_shr<8>(v);
+    return ((readlane_d(v, 15) + readlane_d(v, 31)) + readlane_d(v, 47)) + readlane_d(v, 63);
 }
 
-__device__ __forceinline__ double bcast(double v, int k)
-{
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), k);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
+// one block per tile of the call's trap range
+constexpr int kChunk = 128; // cells staged in LDS at a time
 
-__device__ __forceinline__ int bcasti(int v, int k) { return __builtin_amdgcn_readlane(v, k); }
-
-__device__ __forceinline__ int64_t bcastl(int64_t v, int k)
+__global__ __launch_bounds__(kCThreads) void k_tile_couple(
+    const TrapGrid* __restrict__ traps, int t0, int ntr, uint32_t tile0, const uint32_t* __restrict__ start,
+    const uint32_t* __restrict__ end, const uint32_t* __restrict__ sorted_pairs, const uint32_t* __restrict__ pair_cell,
+    const CellGeo* __restrict__ geo, double* const* __restrict__ fields, int ns, const double* __restrict__ q,
+    double* __restrict__ partial, double npm, unsigned flags)
 {
-    const int lo = __builtin_amdgcn_readlane((int)(v & 0xffffffffll), k);
-    const int hi = __builtin_amdgcn_readlane((int)(v >> 32), k);
-    return ((int64_t)hi << 32) | (unsigned int)lo;
-}
-
-// one wavefront per (row, segment); blockIdx.y = species
-__global__ __launch_bounds__(64) void k_cell_rows(const CellGeo* __restrict__ geo, int nc, GridDesc G,
-                                                  const uint64_t* __restrict__ masks, int nwords,
-                                                  double* const* __restrict__ fields, const double* __restrict__ q,
-                                                  int ns, double* __restrict__ partial, int64_t total,
-                                                  unsigned flags, int pitch)
-{
-    __shared__ double row[kSegMax];
-    const int lane = threadIdx.x;
-    const int i = blockIdx.x / G.nseg, seg = blockIdx.x - i * G.nseg;
-    const int s = blockIdx.y;
-    const int jb = seg * G.segw, je = min(G.nodes_w, jb + G.segw) - 1;
-    double* U = fields[s] + (int64_t)i * pitch;
-    for (int j = jb + lane; j <= je; j += 64) row[j - jb] = U[j];
-    __syncthreads();
+    __shared__ double red[2][kCThreads / 64][kSpeciesPass]; // double-buffered wave sums
+    __shared__ CellGeo sg[kChunk];      // cx, cy replaced by the inverse translation
+    __shared__ uint32_t sp[kChunk];
+    __shared__ double sq[kChunk][kSpeciesPass]; // q_c / |S_c|
+    const uint32_t tl = blockIdx.x;
+    const uint32_t e0 = start[tl], e1 = end[tl];
+    if (e0 >= e1) return;
+    // the tile's trap: traps are numbered in tile order
+    int lo = t0, hi = t0 + ntr - 1;
+    const uint32_t tg = tile0 + tl;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((uint32_t)traps[mid].tile_base <= tg) lo = mid; else hi = mid - 1;
+    }
+    const TrapGrid G = traps[lo];
+    const int lt = (int)(tg - (uint32_t)G.tile_base);
+    const int ty = lt / G.tiles_x, tx = lt - ty * G.tiles_x;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int j = tx * kCTileW + lane;
+    const int ib = ty * kCTileH + w * kCRows;
     const bool dep = (flags & EQ_COUPLE_DEPOSIT) != 0;
-    const double Y = (double)i / G.npm;
-    double* part = partial + (int64_t)s * total;
-    const uint64_t* bm = masks + (int64_t)(i / kBand) * nwords;
-    for (int wb = 0; wb < nwords; wb += 64) {
-        const uint64_t myword = (wb + lane < nwords) ? bm[wb + lane] : 0ull;
-        uint64_t nz = __ballot(myword != 0ull);
-        while (nz) {
-            const int wl = __ffsll((unsigned long long)nz) - 1;
-            nz &= nz - 1;
-            const uint64_t word = (uint64_t)bcastl((int64_t)myword, wl);
-            const int c = (wb + wl) * 64 + lane;
-            bool hit = false;
-            CellGeo g{};
-            double qc = 0.0;
-            if ((word >> lane) & 1ull) {
-                g = geo[c];
-                hit = (i >= g.i1 && i <= g.i2 && g.j1 <= je && g.j2 >= jb);
-                if (hit) qc = q[(int64_t)c * ns + s];
+    const double X = (double)j / npm;
+    double Y[kCRows];
+#pragma unroll
+    for (int r = 0; r < kCRows; ++r) Y[r] = (double)(ib + r) / npm;
+    int buf = 0;
+    for (int s0 = 0; s0 < ns; s0 += kSpeciesPass) {
+        const int nsp = min(kSpeciesPass, ns - s0);
+        double u[kSpeciesPass][kCRows];
+        bool in[kCRows];
+#pragma unroll
+        for (int r = 0; r < kCRows; ++r) {
+            in[r] = (ib + r < G.nodes_h) && (j < G.nodes_w);
+#pragma unroll
+            for (int s = 0; s < kSpeciesPass; ++s)
+                u[s][r] = (s < nsp && in[r]) ? fields[G.prob0 + s0 + s][(int64_t)(ib + r) * G.pitch + j] : 0.0;
+        }
+        for (uint32_t cb = e0; cb < e1; cb += kChunk) {
+            const int n = (int)min((uint32_t)kChunk, e1 - cb);
+            __syncthreads(); // the previous chunk's entries are no longer read
+            if ((int)threadIdx.x < n) {
+                const uint32_t p = sorted_pairs[cb + threadIdx.x];
+                const uint32_t c = pair_cell[p];
+                CellGeo g = geo[c];
+                const double itx = inv_tx(g.cx, g.cy, g.cs, g.sn), ity = inv_ty(g.cx, g.cy, g.cs, g.sn);
+                g.cx = itx;
+                g.cy = ity;
+                sg[threadIdx.x] = g;
+                sp[threadIdx.x] = p;
+                const double cntd = (double)(g.cnt > 0 ? g.cnt : 1);
+#pragma unroll
+                for (int s = 0; s < kSpeciesPass; ++s)
+                    sq[threadIdx.x][s] = (s < nsp && dep) ? q[(int64_t)c * ns + s0 + s] / cntd : 0.0;
             }
-            uint64_t mask = __ballot(hit);
-            while (mask) {
-                const int k = __ffsll((unsigned long long)mask) - 1;
-                mask &= mask - 1;
-                const double cx = bcast(g.cx, k), cy = bcast(g.cy, k), cs = bcast(g.cs, k), sn = bcast(g.sn, k);
-                const double o = bcast(g.o, k), rad = bcast(g.rad, k);
-                const int i1 = bcasti(g.i1, k), j1 = bcasti(g.j1, k), j2 = bcasti(g.j2, k);
-                const int cnt = bcasti(g.cnt, k), nseg0 = bcasti(g.nseg0, k);
-                const int64_t poff = bcastl(g.poff, k);
-                const double d = bcast(qc, k) / (double)(cnt > 0 ? cnt : 1);
-                const int lo = max(j1, jb), hi = min(j2, je);
-                double sum = 0.0;
-                for (int j = lo + lane; j <= hi; j += 64) {
-                    if (cnt == 0 || inside_cell(cx, cy, cs, sn, o, rad, (double)j / G.npm, Y)) {
-                        const double v = row[j - jb];
-                        sum += v;
-                        if (dep) row[j - jb] = v + d;
-                    }
+            __syncthreads();
+            for (int k = 0; k < n; ++k) {
+                const CellGeo& g = sg[k];
+                const double cs = g.cs, sn = g.sn, itx = g.cx, ity = g.cy, o = g.o, rad = g.rad;
+                const bool fallback = g.cnt == 0;
+                const bool colin = j >= g.j1 && j <= g.j2;
+                const double csX = cs * X, snX = (-sn) * X;
+                bool cov[kCRows];
+                bool anyc = false;
+#pragma unroll
+                for (int r = 0; r < kCRows; ++r) {
+                    const int i = ib + r;
+                    bool v = in[r] && colin && i >= g.i1 && i <= g.i2;
+                    if (v && !fallback) v = inside_box(csX, snX, sn * Y[r], cs * Y[r], itx, ity, o, rad);
+                    cov[r] = v;
+                    anyc = anyc || v;
                 }
-                const int nsegc = j2 / G.segw - j1 / G.segw + 1;
-                const int64_t slot = poff + (int64_t)(i - i1) * nsegc + (seg - nseg0);
-                sum = wave_sum64(sum);
-                if (lane == 0) part[slot] = sum;
+                const bool wave_any = __ballot(anyc) != 0ull; // uniform over the wavefront
+#pragma unroll
+                for (int s = 0; s < kSpeciesPass; ++s) {
+                    if (s >= nsp) break;
+                    double sum = 0.0;
+                    const double d = sq[k][s];
+#pragma unroll
+                    for (int r = 0; r < kCRows; ++r) {
+                        if (cov[r]) {
+                            sum += u[s][r];
+                            if (dep) u[s][r] += d;
+                        }
+                    }
+                    if (wave_any) sum = wave_sum64(sum);
+                    if (lane == 0) red[buf][w][s] = wave_any ? sum : 0.0;
+                }
+                __syncthreads();
+                // per-pair partial sums, waves in order; the other buffer takes
+                // the next cell, so no second barrier is needed
+                if ((int)threadIdx.x < nsp) {
+                    const int s = threadIdx.x;
+                    double acc = 0.0;
+                    for (int q4 = 0; q4 < kCThreads / 64; ++q4) acc += red[buf][q4][s];
+                    partial[(int64_t)sp[k] * ns + s0 + s] = acc;
+                }
+                buf ^= 1;
             }
         }
+        if (dep) {
+#pragma unroll
+            for (int r = 0; r < kCRows; ++r)
+#pragma unroll
+                for (int s = 0; s < kSpeciesPass; ++s)
+                    if (s < nsp && in[r]) fields[G.prob0 + s0 + s][(int64_t)(ib + r) * G.pitch + j] = u[s][r];
+        }
     }
-    __syncthreads();
-    if (dep)
-        for (int j = jb + lane; j <= je; j += 64) U[j] = row[j - jb];
 }
 
-// sampled[c,s] = (sum of the cell's row partials in row order) / |S_c|
-__global__ __launch_bounds__(256) void k_cell_sample(const CellGeo* __restrict__ geo, int nc, int ns, int segw,
-                                                     const double* __restrict__ partial, int64_t total,
-                                                     double* __restrict__ sampled, int32_t* __restrict__ counts)
+// sampled[c,s] = (sum of the cell's pair partials in tile order) / |S_c|
+__global__ __launch_bounds__(256) void k_cell_sample(const CellGeo* __restrict__ geo, int64_t nc, int ns,
+                                                     const double* __restrict__ partial, double* __restrict__ sampled,
+                                                     int32_t* __restrict__ counts)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
-    const CellGeo g = geo[c];
-    const int64_t n = cell_slots(g, segw);
+    const CellGeo& g = geo[c];
     const int cnt = g.cnt > 0 ? g.cnt : 1;
     if (counts) counts[c] = cnt;
     if (!sampled) return;
     for (int s = 0; s < ns; ++s) {
-        const double* part = partial + (int64_t)s * total + g.poff;
         double acc = 0.0;
-        for (int64_t k = 0; k < n; ++k) acc += part[k];
-        sampled[(int64_t)c * ns + s] = acc / (double)cnt;
+        for (int k = 0; k < g.npairs; ++k) acc += partial[(g.poff + k) * ns + s];
+        sampled[c * ns + s] = acc / (double)cnt;
     }
 }
+
+static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
 static int ensure_scratch(eq_batch* b, size_t bytes)
 {
@@ -311,70 +437,125 @@ static int ensure_scratch(eq_batch* b, size_t bytes)
     return EQ_OK;
 }
 
-static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+// trap grids (first use): the tile tables of every trap, in trap order
+static int ensure_traps(eq_batch* b)
+{
+    if (b->d_cell_traps) return EQ_OK;
+    const int ns = b->nspecies;
+    std::vector<TrapGrid> tg(b->ntraps);
+    int64_t tiles = 0;
+    for (int t = 0; t < b->ntraps; ++t) {
+        const LevelProb& P = b->levels[0].probs[t * ns];
+        TrapGrid& G = tg[t];
+        G.nodes_w = P.nx + 1;
+        G.nodes_h = P.ny + 1;
+        G.tiles_x = (G.nodes_w + kCTileW - 1) / kCTileW;
+        G.tiles_y = (G.nodes_h + kCTileH - 1) / kCTileH;
+        G.tile_base = (int32_t)tiles;
+        G.prob0 = t * ns;
+        G.pitch = P.pitch;
+        G.trap_w = b->descs[t * ns].width;
+        G.trap_h = b->descs[t * ns].height;
+        tiles += (int64_t)G.tiles_x * G.tiles_y;
+    }
+    if (tiles >= ((int64_t)1 << 31)) return set_error(EQ_ERR_INVALID, "too many coupling tiles");
+    b->cell_tile_base.resize(b->ntraps + 1);
+    for (int t = 0; t < b->ntraps; ++t) b->cell_tile_base[t] = tg[t].tile_base;
+    b->cell_tile_base[b->ntraps] = (int32_t)tiles;
+    if (hipMalloc((void**)&b->d_cell_traps, sizeof(TrapGrid) * b->ntraps) != hipSuccess ||
+        hipHostMalloc((void**)&b->h_cell_off, sizeof(int64_t) * (b->ntraps + 1), hipHostMallocDefault) != hipSuccess ||
+        hipMalloc((void**)&b->d_cell_off, sizeof(int64_t) * (b->ntraps + 1)) != hipSuccess)
+        return set_error(EQ_ERR_NOMEM, "coupling tables");
+    EQ_HIP(hipMemcpy(b->d_cell_traps, tg.data(), sizeof(TrapGrid) * b->ntraps, hipMemcpyHostToDevice));
+    return EQ_OK;
+}
 
-static int couple_impl(eq_batch* b, int32_t trap, const eq_cell* d_cells, size_t nc, double npm, const double* d_q,
-                       double* d_sampled, int32_t* d_counts, uint32_t flags)
+// cells of traps t0 .. t0+ntr-1, concatenated; cell_off[k] (host) = first cell
+// of trap t0+k, cell_off[ntr] = total
+static int couple_impl(eq_batch* b, int32_t t0, int32_t ntr, const int64_t* cell_off, const eq_cell* d_cells,
+                       double npm, const double* d_q, double* d_sampled, int32_t* d_counts, uint32_t flags)
 {
     if (!b) return set_error(EQ_ERR_INVALID, "NULL batch");
-    if (trap < 0 || trap >= b->ntraps) return set_error(EQ_ERR_INVALID, "trap index out of range");
+    if (ntr < 1 || t0 < 0 || t0 + ntr > b->ntraps) return set_error(EQ_ERR_INVALID, "trap index out of range");
     if (!(npm > 0)) return set_error(EQ_ERR_INVALID, "npm must be > 0");
+    for (int k = 0; k < ntr; ++k)
+        if (cell_off[k + 1] < cell_off[k]) return set_error(EQ_ERR_INVALID, "cell offsets must not decrease");
+    const int64_t nc = cell_off[ntr] - cell_off[0];
     if (nc == 0) return EQ_OK;
-    if (nc > (size_t)1 << 30) return set_error(EQ_ERR_INVALID, "too many cells");
+    if (nc > ((int64_t)1 << 30)) return set_error(EQ_ERR_INVALID, "too many cells");
     if (!d_cells || (!d_q && (flags & EQ_COUPLE_DEPOSIT))) return set_error(EQ_ERR_INVALID, "NULL cells or q");
-    const int ns = b->nspecies;
-    const int p0 = trap * ns;
-    const LevelProb& P = b->levels[0].probs[p0];
-    const eq_field_desc& d = b->descs[p0];
-    GridDesc G;
-    G.nodes_w = P.nx + 1;
-    G.nodes_h = P.ny + 1;
-    G.npm = npm;
-    G.trap_w = d.width;
-    G.trap_h = d.height;
-    // column segments of at most kSegMax nodes (more wavefronts in flight)
-    G.nseg = (G.nodes_w + kSegMax - 1) / kSegMax;
-    G.segw = (G.nodes_w + G.nseg - 1) / G.nseg;
-    const int nwords = (int)((nc + 63) / 64);
-    const int nbands = (G.nodes_h + kBand - 1) / kBand;
-    hipStream_t s = b->stream;
-    // scratch: geo | total | band masks | partials
-    const size_t geo_bytes = align256(sizeof(CellGeo) * nc);
-    const size_t mask_bytes = align256(sizeof(uint64_t) * (size_t)nwords * nbands);
-    const size_t head = geo_bytes + 256 + mask_bytes;
-    int rc = ensure_scratch(b, head + 256);
+    int rc = ensure_traps(b);
     if (rc) return rc;
+    const int ns = b->nspecies;
+    hipStream_t s = b->stream;
+    const uint32_t tile0 = (uint32_t)b->cell_tile_base[t0];
+    const uint32_t ntiles = (uint32_t)(b->cell_tile_base[t0 + ntr] - b->cell_tile_base[t0]);
+    // cell offsets relative to the first cell of the call
+    for (int k = 0; k <= ntr; ++k) b->h_cell_off[k] = cell_off[k] - cell_off[0];
+    EQ_HIP(hipMemcpyAsync(b->d_cell_off, b->h_cell_off, sizeof(int64_t) * (ntr + 1), hipMemcpyHostToDevice, s));
+    // scratch: geo | total | tile start | tile end | (pairs: keys, vals, keys', vals', pair_cell, partial, sort temp)
+    const size_t geo_bytes = align256(sizeof(CellGeo) * nc);
+    const size_t tb_bytes = align256(sizeof(uint32_t) * ntiles);
+    const size_t head = geo_bytes + 256 + 2 * tb_bytes;
+    if ((rc = ensure_scratch(b, head + 256))) return rc;
     char* base = (char*)b->cell_scratch;
     CellGeo* geo = (CellGeo*)base;
     int64_t* d_total = (int64_t*)(base + geo_bytes);
-    k_cell_geo<<<(unsigned)((nc + 3) / 4), 256, 0, s>>>(d_cells, (int)nc, G, geo);
-    k_scan<<<1, 1024, 0, s>>>(geo, (int)nc, G.segw, d_total);
+    const TrapGrid* traps = (const TrapGrid*)b->d_cell_traps;
+    auto geometry = [&]() {
+        k_cell_geo<<<(unsigned)((nc + 3) / 4), 256, 0, s>>>(d_cells, nc, b->d_cell_off, ntr, t0, traps, npm, geo);
+        k_pair_scan<<<1, 1024, 0, s>>>(geo, nc, d_total);
+    };
+    geometry();
     int64_t total = 0;
     EQ_HIP(hipMemcpyAsync(&total, d_total, sizeof(int64_t), hipMemcpyDeviceToHost, s));
     EQ_HIP(hipStreamSynchronize(s));
-    const size_t need = head + align256(sizeof(double) * (size_t)total * ns);
+    if (total >= ((int64_t)1 << 32)) return set_error(EQ_ERR_INVALID, "too many cell/tile pairs");
+    const size_t pb = align256(sizeof(uint32_t) * (size_t)total);
+    size_t sort_bytes = 0;
+    EQ_HIP(rocprim::radix_sort_pairs(nullptr, sort_bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                     (const uint32_t*)nullptr, (uint32_t*)nullptr, (size_t)total, 0,
+                                     32, s));
+    const size_t need = head + 5 * pb + align256(sizeof(double) * (size_t)total * ns) + align256(sort_bytes);
     if (need > b->cell_scratch_bytes) {
         // grow (keeps nothing: recompute the geometry in the new buffer)
         if ((rc = ensure_scratch(b, need))) return rc;
         base = (char*)b->cell_scratch;
         geo = (CellGeo*)base;
         d_total = (int64_t*)(base + geo_bytes);
-        k_cell_geo<<<(unsigned)((nc + 3) / 4), 256, 0, s>>>(d_cells, (int)nc, G, geo);
-        k_scan<<<1, 1024, 0, s>>>(geo, (int)nc, G.segw, d_total);
+        geometry();
     }
-    uint64_t* masks = (uint64_t*)(base + geo_bytes + 256);
-    double* const* d_fields = b->d_field_ptrs + p0; // the trap's species, in order
-    double* partial = (double*)(base + head);
-    k_band_masks<<<nbands, 64, 0, s>>>(geo, (int)nc, nwords, masks);
+    uint32_t* t_start = (uint32_t*)(base + geo_bytes + 256);
+    uint32_t* t_end = (uint32_t*)(base + geo_bytes + 256 + tb_bytes);
+    char* pbase = base + head;
+    uint32_t* keys = (uint32_t*)pbase;
+    uint32_t* vals = (uint32_t*)(pbase + pb);
+    uint32_t* keys2 = (uint32_t*)(pbase + 2 * pb);
+    uint32_t* vals2 = (uint32_t*)(pbase + 3 * pb);
+    uint32_t* pair_cell = (uint32_t*)(pbase + 4 * pb);
+    double* partial = (double*)(pbase + 5 * pb);
+    void* sort_tmp = pbase + 5 * pb + align256(sizeof(double) * (size_t)total * ns);
+    k_pairs<<<(unsigned)((nc + 255) / 256), 256, 0, s>>>(geo, nc, traps, keys, vals, pair_cell);
+    // key bits: global tile ids up to the last tile of the batch
+    unsigned bits = 1;
+    while (bits < 32 && ((uint64_t)b->cell_tile_base[b->ntraps] >> bits) != 0) ++bits;
+    EQ_HIP(rocprim::radix_sort_pairs(sort_tmp, sort_bytes, (const uint32_t*)keys, keys2, (const uint32_t*)vals, vals2,
+                                     (size_t)total, 0, bits, s));
+    EQ_HIP(hipMemsetAsync(t_start, 0, 2 * tb_bytes, s));
+    k_tile_bounds<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(keys2, total, tile0, t_start, t_end);
     prof_begin(b, EQ_KERNEL_COUPLE);
-    k_cell_rows<<<dim3((unsigned)(G.nodes_h * G.nseg), (unsigned)ns), 64, 0, s>>>(
-        geo, (int)nc, G, masks, nwords, d_fields, d_q, ns, partial, total, flags, P.pitch);
-    // algorithmic bytes: every row read (and written when depositing) once per species
-    prof_end(b, EQ_KERNEL_COUPLE,
-             (double)G.nodes_w * G.nodes_h * ns * sizeof(double) * ((flags & EQ_COUPLE_DEPOSIT) ? 2 : 1));
+    k_tile_couple<<<ntiles, kCThreads, 0, s>>>(traps, t0, ntr, tile0, t_start, t_end, vals2, pair_cell, geo,
+                                               b->d_field_ptrs, ns, d_q, partial, npm, flags);
+    // algorithmic bytes: every node of the call's traps read (and written when depositing) once per species
+    double nodes = 0.0;
+    for (int t = t0; t < t0 + ntr; ++t) {
+        const LevelProb& P = b->levels[0].probs[t * ns];
+        nodes += (double)(P.nx + 1) * (P.ny + 1);
+    }
+    prof_end(b, EQ_KERNEL_COUPLE, nodes * ns * sizeof(double) * ((flags & EQ_COUPLE_DEPOSIT) ? 2 : 1));
     if (d_sampled || d_counts)
         k_cell_sample<<<(unsigned)((nc + 255) / 256), 256, 0, s>>>(
-            geo, (int)nc, ns, G.segw, partial, total, (flags & EQ_COUPLE_SAMPLE) ? d_sampled : nullptr, d_counts);
+            geo, nc, ns, partial, (flags & EQ_COUPLE_SAMPLE) ? d_sampled : nullptr, d_counts);
     EQ_HIP(hipGetLastError());
     return EQ_OK;
 }
@@ -390,7 +571,18 @@ int eq_batch_couple_dev(eq_batch* b, int32_t trap, const eq_cell* d_cells, size_
 {
     if (!b) return set_error(EQ_ERR_INVALID, "NULL batch");
     hipSetDevice(b->device);
-    return couple_impl(b, trap, d_cells, nc, npm, d_q, d_sampled, d_counts, flags);
+    const int64_t off[2] = {0, (int64_t)nc};
+    return couple_impl(b, trap, 1, off, d_cells, npm, d_q, d_sampled, d_counts, flags);
+}
+
+int eq_batch_couple_traps_dev(eq_batch* b, int32_t trap0, int32_t ntraps, const int64_t* cell_offsets,
+                              const eq_cell* d_cells, double npm, const double* d_q, double* d_sampled,
+                              int32_t* d_counts, uint32_t flags)
+{
+    if (!b) return set_error(EQ_ERR_INVALID, "NULL batch");
+    if (!cell_offsets) return set_error(EQ_ERR_INVALID, "NULL cell offsets");
+    hipSetDevice(b->device);
+    return couple_impl(b, trap0, ntraps, cell_offsets, d_cells, npm, d_q, d_sampled, d_counts, flags);
 }
 
 int eq_batch_couple(eq_batch* b, int32_t trap, const eq_cell* cells, size_t nc, double npm, const double* q,
@@ -418,8 +610,9 @@ int eq_batch_couple(eq_batch* b, int32_t trap, const eq_cell* cells, size_t nc, 
     } else if (!q) {
         hipMemsetAsync(dq, 0, qb, s);
     }
+    const int64_t off[2] = {0, (int64_t)nc};
     if (rc == EQ_OK)
-        rc = couple_impl(b, trap, dc, nc, npm, dq, sampled ? ds : nullptr, counts ? dk : nullptr, flags);
+        rc = couple_impl(b, trap, 1, off, dc, npm, dq, sampled ? ds : nullptr, counts ? dk : nullptr, flags);
     if (rc == EQ_OK && sampled && hipMemcpyAsync(sampled, ds, sb, hipMemcpyDeviceToHost, s) != hipSuccess)
         rc = set_error(EQ_ERR_HIP, "couple download");
     if (rc == EQ_OK && counts && hipMemcpyAsync(counts, dk, kb, hipMemcpyDeviceToHost, s) != hipSuccess)

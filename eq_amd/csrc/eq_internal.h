@@ -105,6 +105,10 @@ struct eq_batch {
     void* cell_scratch = nullptr;
     size_t cell_scratch_bytes = 0;
     double** d_field_ptrs = nullptr; // device table: node (0,0) of every problem's field
+    void* d_cell_traps = nullptr;    // coupling tile grid of every trap (device)
+    std::vector<int32_t> cell_tile_base; // first coupling tile of every trap, and the total
+    int64_t* h_cell_off = nullptr;   // pinned staging of a call's cell offsets
+    int64_t* d_cell_off = nullptr;
     double* flux_out = nullptr;      // device scalar
     double* h_flux = nullptr;        // pinned
     eqk::ProfileSlot prof[EQ_KERNEL_COUNT_];

@@ -1785,6 +1785,9 @@ void eq_batch_destroy(eq_batch* b)
     hipFree(b->flux_out);
     hipFree(b->d_field_ptrs);
     hipFree(b->cell_scratch);
+    hipFree(b->d_cell_traps);
+    hipFree(b->d_cell_off);
+    if (b->h_cell_off) hipHostFree(b->h_cell_off);
     if (b->h_state) hipHostFree(b->h_state);
     if (b->h_flux) hipHostFree(b->h_flux);
     for (int k = 0; k < EQ_KERNEL_COUNT_; ++k) {

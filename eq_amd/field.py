@@ -32,7 +32,8 @@ EXPORTED = [
     "eq_batch_create", "eq_batch_destroy", "eq_batch_num_problems", "eq_batch_num_nodes",
     "eq_batch_stream", "eq_batch_sync", "eq_batch_upload", "eq_batch_download",
     "eq_batch_upload_dev", "eq_batch_download_dev", "eq_batch_step", "eq_batch_problem_stats",
-    "eq_batch_couple", "eq_batch_couple_dev", "eq_batch_boundary_flux", "eq_batch_vertex_map",
+    "eq_batch_couple", "eq_batch_couple_dev", "eq_batch_couple_traps_dev", "eq_batch_boundary_flux",
+    "eq_batch_vertex_map",
     "eq_field_create", "eq_field_destroy", "eq_field_upload", "eq_field_download",
     "eq_field_step", "eq_field_vertex_map", "eq_field_boundary_flux", "eq_cells_couple",
     "eq_batch_profile", "eq_batch_profile_read", "eq_batch_apply", "eq_batch_rhs",
@@ -77,7 +78,11 @@ _lib = None
 
 
 def load_library():
-    """Load libeq_field.so; raises if it has not been built."""
+    """Load libeq_field.so; raises if it has not been built.
+
+    A process that also uses torch on the GPU must initialise torch first:
+    torch ships its own HIP runtime under the same soname (libamdhip64.so.7)
+    and whichever is loaded first serves the process."""
     global _lib
     if _lib is not None:
         return _lib
@@ -103,6 +108,8 @@ def load_library():
     L.eq_batch_step.argtypes = [vp, P(StepStats)]
     L.eq_batch_problem_stats.argtypes = [vp, ip32, dp]
     L.eq_batch_couple.argtypes = [vp, C.c_int32, vp, C.c_size_t, C.c_double, vp, vp, vp, C.c_uint32]
+    L.eq_batch_couple_traps_dev.argtypes = [vp, C.c_int32, C.c_int32, P(C.c_int64), vp, C.c_double, vp, vp, vp,
+                                            C.c_uint32]
     L.eq_batch_couple_dev.argtypes = [vp, C.c_int32, vp, C.c_size_t, C.c_double, vp, vp, vp,
                                       C.c_uint32]
     L.eq_batch_boundary_flux.argtypes = [vp, C.c_int32, dp]
@@ -250,6 +257,16 @@ class FieldBatch:
         _check(load_library().eq_batch_couple_dev(
             self._h, trap, C.c_void_p(cells_ptr), nc, npm, C.c_void_p(q_ptr),
             C.c_void_p(sampled_ptr) if sampled_ptr else None,
+            C.c_void_p(counts_ptr) if counts_ptr else None, flags))
+
+    def couple_traps_dev(self, trap0: int, cell_offsets, cells_ptr: int, npm: float, q_ptr: int,
+                         sampled_ptr: int = 0, counts_ptr: int = 0, flags: int = SAMPLE | DEPOSIT):
+        """Traps trap0 .. trap0+len(cell_offsets)-2 in one pass; cell_offsets (host) gives
+        the first cell of each trap in the concatenated device arrays, and the total."""
+        off = np.ascontiguousarray(cell_offsets, dtype=np.int64)
+        _check(load_library().eq_batch_couple_traps_dev(
+            self._h, trap0, off.size - 1, off.ctypes.data_as(C.POINTER(C.c_int64)), C.c_void_p(cells_ptr), npm,
+            C.c_void_p(q_ptr), C.c_void_p(sampled_ptr) if sampled_ptr else None,
             C.c_void_p(counts_ptr) if counts_ptr else None, flags))
 
     def boundary_flux(self, prob: int) -> float:

@@ -145,6 +145,13 @@ int eq_batch_couple(eq_batch* b, int32_t trap, const eq_cell* cells, size_t nc, 
 /* device-pointer variant: d_cells/d_q/d_sampled as above, on the batch stream */
 int eq_batch_couple_dev(eq_batch* b, int32_t trap, const eq_cell* d_cells, size_t nc, double npm,
                         const double* d_q, double* d_sampled, int32_t* d_counts, uint32_t flags);
+/* The same for the traps trap0 .. trap0+ntraps-1 in one pass (one eQabm::updateCells
+ * per trap, src/abm/eQabm.cpp:234-449, the traps being independent): their cells are
+ * concatenated in d_cells, cell_offsets (HOST array, ntraps+1 entries) gives the first
+ * cell of each trap and the total; q/sampled/counts follow the concatenated order. */
+int eq_batch_couple_traps_dev(eq_batch* b, int32_t trap0, int32_t ntraps, const int64_t* cell_offsets,
+                              const eq_cell* d_cells, double npm, const double* d_q, double* d_sampled,
+                              int32_t* d_counts, uint32_t flags);
 
 /* totalBoundaryFlux = D*dt * (-int grad(u).n ds)  (src/fHSL.cpp:156-160,
  * fenics/boundary.ufl:9-12) of the current field */

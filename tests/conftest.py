@@ -29,7 +29,14 @@ def oracle():
 
 @pytest.fixture(scope="session")
 def eqf():
-    """The product binding; on a GPU test run the HIP library must load."""
+    """The product binding; on a GPU test run the HIP library must load.
+
+    torch bundles its own HIP runtime under the same soname as ROCm's; the one
+    loaded first serves the whole process.  Tests that hand torch device
+    buffers to the library need torch's, so torch is initialised first."""
+    if gpu_available():
+        import torch
+        torch.cuda.init()
     from eq_amd import field
     field.load_library()
     return field

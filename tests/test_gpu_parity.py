@@ -252,3 +252,51 @@ def test_degenerate_inputs(eqf, oracle):
         fb.upload(0, np.full(180, 0.25))
         fb.step()
         assert np.abs(fb.download(0) - 0.25).max() <= 1e-12
+
+
+def test_couple_traps_batched_matches_per_trap_and_oracle(eqf, oracle):
+    """Several uneven traps coupled in one pass: fields bit-identical to the
+    sequential loop per trap, samples to n*eps, and identical to trap-by-trap
+    calls."""
+    import torch
+    from eq_amd import workloads as Wl
+    shapes = [(120, 60, 6.0, 3.0, 150), (64, 200, 3.2, 10.0, 400), (300, 90, 15.0, 4.5, 60)]
+    npm = 20.0
+    rng = Wl.SplitMix64(21)
+    descs, cells, rates, u0, grids = [], [], [], [], []
+    for (nx, ny, W, H, nc) in shapes:
+        c, r = Wl.seeded_cells(rng, nc, W, H, 2)
+        cells.append(c)
+        rates.append(r)
+        grids.append(oracle.grid(nx + 1, ny + 1, npm, W, H))
+        g = np.random.default_rng(nx)
+        for s in range(2):
+            descs.append(eqf.field_desc(nx, ny, W, H, 0.1, 100.0))
+            u0.append(g.random((nx + 1) * (ny + 1)))
+    off = np.concatenate([[0], np.cumsum([c.shape[0] for c in cells])]).astype(np.int64)
+    dev = torch.device("cuda")
+    cells_d = torch.from_numpy(np.concatenate(cells)).to(dev).contiguous()
+    q_d = torch.from_numpy(np.concatenate(rates)).to(dev).contiguous()
+    samp_d = torch.zeros((int(off[-1]), 2), dtype=torch.float64, device=dev)
+    cnt_d = torch.zeros(int(off[-1]), dtype=torch.int32, device=dev)
+    with eqf.FieldBatch(descs, nspecies=2, solver=eqf.solver_desc(eqf.JACOBI_CG)) as fb:
+        for k, u in enumerate(u0):
+            fb.upload(k, u)
+        fb.couple_traps_dev(0, off, cells_d.data_ptr(), npm, q_d.data_ptr(), samp_d.data_ptr(), cnt_d.data_ptr())
+        fb.sync()
+        got = [fb.download(k) for k in range(len(descs))]
+    sampled, counts = samp_d.cpu().numpy(), cnt_d.cpu().numpy()
+    with eqf.FieldBatch(descs, nspecies=2, solver=eqf.solver_desc(eqf.JACOBI_CG)) as fb:
+        for k, u in enumerate(u0):
+            fb.upload(k, u)
+        one = [fb.couple(t, cells[t], npm, rates[t]) for t in range(len(shapes))]
+        got1 = [fb.download(k) for k in range(len(descs))]
+    for t in range(len(shapes)):
+        ref, s_ref, c_ref = oracle.couple(grids[t], cells[t], [u0[2 * t], u0[2 * t + 1]], rates[t])
+        a, b = off[t], off[t + 1]
+        assert np.array_equal(counts[a:b], c_ref)
+        assert np.allclose(sampled[a:b], s_ref, rtol=2e-13, atol=1e-300)
+        assert np.array_equal(sampled[a:b], one[t][0])
+        for s in range(2):
+            assert np.array_equal(got[2 * t + s], ref[s]), (t, s)
+            assert np.array_equal(got1[2 * t + s], ref[s]), (t, s)
