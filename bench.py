@@ -168,6 +168,67 @@ def config2_drift(F, W, cells_dev, q_dev, nsteps):
     return res
 
 
+def config3_batched(F, W, nsteps, warmup=2, ntraps=512, sigma=0.05):
+    """Config 3: 512 uneven traps (seed 3) x 2 species in one batch, mixed
+    solver, cycle shape picked by the autotuner after the first step; every
+    step couples all traps' cells in one pass and solves all 1024 problems
+    together.  Cells random-walk (sigma 0.05 um, the config-1 walk) inside
+    their own trap."""
+    import torch
+    traps = W.config3_traps(ntraps, 3)
+    descs = [F.field_desc(t.nx, t.ny, t.width, t.height, 0.1, D, g, 0.0, W.TOP_DIRICHLET)
+             for t in traps for D, g in zip(t.D, t.gamma)]
+    nodes = sum(t.nodes for t in traps) * 2
+    counts = [t.cells.shape[0] for t in traps]
+    off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    cells0 = np.concatenate([t.cells for t in traps])
+    wlim = np.concatenate([np.full(t.cells.shape[0], t.width) for t in traps])
+    hlim = np.concatenate([np.full(t.cells.shape[0], t.height) for t in traps])
+    q = torch.from_numpy(np.concatenate([t.rates for t in traps]) * 0.1).to("cuda").contiguous()
+    total = warmup + nsteps
+    walk = np.empty((total,) + cells0.shape)
+    cur = cells0
+    for k in range(total):
+        cur = W.random_walk_step(cur, 3, k, sigma, wlim, hlim)
+        walk[k] = cur
+    cells_d = torch.from_numpy(walk).to("cuda").contiguous()
+    sampled = torch.empty((cells0.shape[0], 2), dtype=torch.float64, device="cuda")
+    t_create = time.perf_counter()
+    fb = F.FieldBatch(descs, nspecies=2, solver=F.solver_desc(F.MIXED_GMG_CG))
+    t_create = time.perf_counter() - t_create
+    stride = cells0.shape[0] * 5 * 8
+
+    def step(k):
+        fb.couple_traps_dev(0, off, cells_d.data_ptr() + k * stride, 10.0, q.data_ptr(), sampled.data_ptr())
+        return fb.step()
+
+    step(0)
+    t_tune = time.perf_counter()
+    sd = fb.autotune(1)
+    t_tune = time.perf_counter() - t_tune
+    for k in range(1, warmup):
+        step(k)
+    torch.cuda.synchronize()
+    iters, rel = [], []
+    t0 = time.perf_counter()
+    for k in range(warmup, total):
+        st = step(k)
+        iters.append(st.iterations_max)
+        rel.append(st.relres_max)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    res = {"workload": f"{ntraps} uneven traps (seed 3), h=0.1 um, 2 species, {int(off[-1])} rods",
+           "value": nodes * nsteps / dt, "unit": "DOF-timesteps/s", "steps": nsteps,
+           "ms_per_step": dt / nsteps * 1e3, "dofs": nodes, "problems": len(descs),
+           "iterations_max": int(max(iters)), "relres_max": float(max(rel)),
+           "autotuned_cycle": {"pre": int(sd.mg_pre), "post": int(sd.mg_post)},
+           "autotune_seconds": t_tune, "create_seconds": t_create}
+    log(f"bench: config 3: {res['value']:.4e} DOF-timesteps/s, {res['ms_per_step']:.2f} ms/step, "
+        f"V({sd.mg_pre},{sd.mg_post}), iterations max {res['iterations_max']}")
+    fb.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -178,6 +239,8 @@ def main():
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--c3-steps", type=int, default=5,
+                    help="timed steps of the config-3 batch (512 uneven traps), part of the variants")
     ap.add_argument("--c2-steps", type=int, default=50,
                     help="steps of the config-2 drift comparison (mixed vs FP64), part of the variants")
     ap.add_argument("--copy-probe", action="store_true",
@@ -315,6 +378,8 @@ def main():
             log(f"bench: variant {name}: {variants[name]['value']:.4e} DOF-timesteps/s")
         out["variants"] = variants
         out["config2"] = config2_drift(F, W, cells_dev, q_dev, args.c2_steps)
+        if args.c3_steps > 0:
+            out["config3"] = config3_batched(F, W, args.c3_steps)
 
     if not args.no_cpu_baseline:
         jac_iters = out.get("variants", {}).get("jacobi", {}).get("step0_iterations")

@@ -1120,11 +1120,15 @@ static void fill_level_prob(LevelProb& P, const eq_field_desc& d, int nx, int ny
     set_interior_row(P);
 }
 
-// Dense inverse of the eliminated operator on a coarsest level (FP64 Gauss-Jordan
-// with partial pivoting; SPD so it is well posed).
+// Inverse of the eliminated operator on a coarsest level, column by column
+// (inv[m*n + k] = (A^-1)_{k,m}).  A is SPD (Dirichlet rows and columns are
+// identity) and banded (half-bandwidth nx+2 in natural order), so LU without
+// pivoting inside the band is stable and costs O(n bw^2), and the n
+// solves O(n^2 bw), instead of O(n^3) for a dense elimination.
 static bool dense_inverse(const LevelProb& P, std::vector<double>& inv)
 {
     const int nw = P.nx + 1, n = nw * (P.ny + 1);
+    const int bw = nw + 1;
     std::vector<double> A((size_t)n * n, 0.0);
     for (int i = 0; i <= P.ny; ++i)
         for (int j = 0; j <= P.nx; ++j) {
@@ -1147,32 +1151,35 @@ static bool dense_inverse(const LevelProb& P, std::vector<double>& inv)
             put(i - 1, j - 1, r.msw);
             put(i + 1, j + 1, r.mne);
         }
-    inv.assign((size_t)n * n, 0.0);
-    for (int k = 0; k < n; ++k) inv[(size_t)k * n + k] = 1.0;
-    for (int col = 0; col < n; ++col) {
-        int piv = col;
-        for (int r = col + 1; r < n; ++r)
-            if (std::fabs(A[(size_t)r * n + col]) > std::fabs(A[(size_t)piv * n + col])) piv = r;
-        if (A[(size_t)piv * n + col] == 0.0) return false;
-        if (piv != col)
-            for (int c = 0; c < n; ++c) {
-                std::swap(A[(size_t)piv * n + c], A[(size_t)col * n + c]);
-                std::swap(inv[(size_t)piv * n + c], inv[(size_t)col * n + c]);
-            }
-        const double d = A[(size_t)col * n + col];
-        for (int c = 0; c < n; ++c) {
-            A[(size_t)col * n + c] /= d;
-            inv[(size_t)col * n + c] /= d;
-        }
-        for (int r = 0; r < n; ++r) {
-            if (r == col) continue;
-            const double f = A[(size_t)r * n + col];
+    // in-place banded LU (multipliers below the diagonal)
+    for (int c = 0; c < n; ++c) {
+        const double d = A[(size_t)c * n + c];
+        if (!(d > 0.0)) return false;
+        const int rmax = std::min(n - 1, c + bw);
+        for (int r = c + 1; r <= rmax; ++r) {
+            const double f = A[(size_t)r * n + c] / d;
             if (f == 0.0) continue;
-            for (int c = 0; c < n; ++c) {
-                A[(size_t)r * n + c] -= f * A[(size_t)col * n + c];
-                inv[(size_t)r * n + c] -= f * inv[(size_t)col * n + c];
-            }
+            A[(size_t)r * n + c] = f;
+            for (int q = c + 1; q <= rmax; ++q) A[(size_t)r * n + q] -= f * A[(size_t)c * n + q];
         }
+    }
+    inv.assign((size_t)n * n, 0.0);
+    std::vector<double> x(n);
+    for (int m = 0; m < n; ++m) {
+        std::fill(x.begin(), x.end(), 0.0);
+        x[m] = 1.0;
+        for (int r = m + 1; r < n; ++r) { // L y = e_m (y_r = 0 for r < m)
+            double v = x[r];
+            for (int q = std::max(m, r - bw); q < r; ++q) v -= A[(size_t)r * n + q] * x[q];
+            x[r] = v;
+        }
+        for (int r = n - 1; r >= 0; --r) { // U x = y
+            double v = x[r];
+            const int qmax = std::min(n - 1, r + bw);
+            for (int q = r + 1; q <= qmax; ++q) v -= A[(size_t)r * n + q] * x[q];
+            x[r] = v / A[(size_t)r * n + r];
+        }
+        std::copy(x.begin(), x.end(), inv.begin() + (size_t)m * n);
     }
     return true;
 }
@@ -1325,9 +1332,7 @@ static int build_hierarchy(eq_batch* b)
                 std::vector<double> inv;
                 if (!dense_inverse(P, inv)) return set_error(EQ_ERR_INVALID, "singular coarse operator");
                 P.dense_off = (int64_t)dense_all.size();
-                const int n = (P.nx + 1) * (P.ny + 1);
-                for (int m = 0; m < n; ++m) // column-major
-                    for (int k = 0; k < n; ++k) dense_all.push_back((T)inv[(size_t)k * n + m]);
+                for (double v : inv) dense_all.push_back((T)v); // column-major already
             }
         }
     }
@@ -1876,6 +1881,91 @@ int eq_batch_step(eq_batch* b, eq_step_stats* stats)
     if (!b) return set_error(EQ_ERR_INVALID, "NULL batch");
     hipSetDevice(b->device);
     return b->mixed ? solve_all<float>(b, stats) : solve_all<double>(b, stats);
+}
+
+// Candidate V-cycle shapes (pre, post sweeps) for eq_batch_autotune.
+static const int kTuneShapes[][2] = {{1, 1}, {2, 1}, {2, 2}, {3, 3}};
+
+int eq_batch_autotune(eq_batch* b, int32_t repeats, eq_solver_desc* chosen)
+{
+    if (!b) return set_error(EQ_ERR_INVALID, "NULL batch");
+    if (b->solver.kind == EQ_SOLVER_JACOBI_CG) {
+        if (chosen) *chosen = b->solver;
+        return EQ_OK; // nothing to tune
+    }
+    hipSetDevice(b->device);
+    hipStream_t s = b->stream;
+    const int64_t n0 = b->levels[0].pool;
+    // snapshot of everything a step changes: field, kept solution, Krylov state
+    double* save = nullptr;
+    eqk::CgState* save_st = nullptr;
+    if (hipMalloc((void**)&save, sizeof(double) * 2 * n0) != hipSuccess ||
+        hipMalloc((void**)&save_st, sizeof(eqk::CgState) * b->nprob) != hipSuccess) {
+        hipFree(save);
+        return set_error(EQ_ERR_NOMEM, "autotune snapshot");
+    }
+    auto copy_state = [&](bool to_save) -> int {
+        double* u = to_save ? save : b->u;
+        const double* uf = to_save ? b->u : save;
+        double* xp = to_save ? save + n0 : b->xprev;
+        const double* xpf = to_save ? b->xprev : save + n0;
+        EQ_HIP(hipMemcpyAsync(u, uf, sizeof(double) * n0, hipMemcpyDeviceToDevice, s));
+        EQ_HIP(hipMemcpyAsync(xp, xpf, sizeof(double) * n0, hipMemcpyDeviceToDevice, s));
+        EQ_HIP(hipMemcpyAsync(to_save ? save_st : b->d_state, to_save ? b->d_state : save_st,
+                              sizeof(eqk::CgState) * b->nprob, hipMemcpyDeviceToDevice, s));
+        EQ_HIP(hipStreamSynchronize(s));
+        return EQ_OK;
+    };
+    auto drop_graphs = [&]() {
+        if (b->g_iter2) hipGraphExecDestroy(b->g_iter2);
+        if (b->g_init) hipGraphExecDestroy(b->g_init);
+        b->g_iter2 = nullptr;
+        b->g_init = nullptr;
+    };
+    const eq_solver_desc orig = b->solver;
+    const int32_t orig_last = b->last_iters;
+    int rc = copy_state(true);
+    double best_t = 1e300;
+    int best = -1;
+    const int reps = std::max(1, repeats);
+    for (int k = 0; rc == EQ_OK && k < (int)(sizeof(kTuneShapes) / sizeof(kTuneShapes[0])); ++k) {
+        b->solver.mg_pre = kTuneShapes[k][0];
+        b->solver.mg_post = kTuneShapes[k][1];
+        drop_graphs();
+        double t = 0.0;
+        bool ok = true;
+        for (int r = 0; r <= reps && rc == EQ_OK; ++r) { // r = 0 captures the graphs, untimed
+            b->last_iters = orig_last;
+            eq_step_stats st{};
+            const auto t0 = std::chrono::steady_clock::now();
+            const int src = b->mixed ? solve_all<float>(b, &st) : solve_all<double>(b, &st);
+            const hipError_t e = hipStreamSynchronize(s);
+            if (e != hipSuccess) {
+                rc = hip_check(e, "autotune step");
+                break;
+            }
+            const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (src != EQ_OK || st.nconverged != b->nprob) ok = false;
+            if (r > 0) t += dt;
+            rc = copy_state(false);
+        }
+        if (rc == EQ_OK && ok && t < best_t) {
+            best_t = t;
+            best = k;
+        }
+    }
+    b->solver = orig;
+    if (best >= 0) {
+        b->solver.mg_pre = kTuneShapes[best][0];
+        b->solver.mg_post = kTuneShapes[best][1];
+    }
+    b->last_iters = orig_last;
+    drop_graphs();
+    hipFree(save);
+    hipFree(save_st);
+    if (rc != EQ_OK) return rc;
+    if (chosen) *chosen = b->solver;
+    return EQ_OK;
 }
 
 int eq_batch_problem_stats(const eq_batch* b, int32_t* iterations, double* relres)

@@ -32,7 +32,8 @@ EXPORTED = [
     "eq_batch_create", "eq_batch_destroy", "eq_batch_num_problems", "eq_batch_num_nodes",
     "eq_batch_stream", "eq_batch_sync", "eq_batch_upload", "eq_batch_download",
     "eq_batch_upload_dev", "eq_batch_download_dev", "eq_batch_step", "eq_batch_problem_stats",
-    "eq_batch_couple", "eq_batch_couple_dev", "eq_batch_couple_traps_dev", "eq_batch_boundary_flux",
+    "eq_batch_couple", "eq_batch_couple_dev", "eq_batch_couple_traps_dev", "eq_batch_autotune",
+    "eq_batch_boundary_flux",
     "eq_batch_vertex_map",
     "eq_field_create", "eq_field_destroy", "eq_field_upload", "eq_field_download",
     "eq_field_step", "eq_field_vertex_map", "eq_field_boundary_flux", "eq_cells_couple",
@@ -108,6 +109,7 @@ def load_library():
     L.eq_batch_step.argtypes = [vp, P(StepStats)]
     L.eq_batch_problem_stats.argtypes = [vp, ip32, dp]
     L.eq_batch_couple.argtypes = [vp, C.c_int32, vp, C.c_size_t, C.c_double, vp, vp, vp, C.c_uint32]
+    L.eq_batch_autotune.argtypes = [vp, C.c_int32, P(SolverDesc)]
     L.eq_batch_couple_traps_dev.argtypes = [vp, C.c_int32, C.c_int32, P(C.c_int64), vp, C.c_double, vp, vp, vp,
                                             C.c_uint32]
     L.eq_batch_couple_dev.argtypes = [vp, C.c_int32, vp, C.c_size_t, C.c_double, vp, vp, vp,
@@ -268,6 +270,12 @@ class FieldBatch:
             self._h, trap0, off.size - 1, off.ctypes.data_as(C.POINTER(C.c_int64)), C.c_void_p(cells_ptr), npm,
             C.c_void_p(q_ptr), C.c_void_p(sampled_ptr) if sampled_ptr else None,
             C.c_void_p(counts_ptr) if counts_ptr else None, flags))
+
+    def autotune(self, repeats: int = 1) -> SolverDesc:
+        """Pick the fastest multigrid cycle shape on the current state (which is restored)."""
+        sd = SolverDesc()
+        _check(load_library().eq_batch_autotune(self._h, repeats, C.byref(sd)))
+        return sd
 
     def boundary_flux(self, prob: int) -> float:
         v = C.c_double()

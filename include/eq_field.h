@@ -142,6 +142,14 @@ int eq_batch_problem_stats(const eq_batch* b, int32_t* iterations, double* relre
  * Per cell in array order: sample every species, then deposit q/|S_c|. */
 int eq_batch_couple(eq_batch* b, int32_t trap, const eq_cell* cells, size_t nc, double npm,
                     const double* q, double* sampled, int32_t* counts, uint32_t flags);
+/* Autotuner (no counterpart in the reference, which has one fixed direct solve):
+ * times `repeats` steps of each candidate multigrid cycle shape (pre/post sweeps
+ * (1,1), (2,1), (2,2), (3,3)) from the batch's current state, restoring the fields,
+ * the kept solutions and the Krylov state after every trial, and keeps the fastest
+ * shape whose solves all converged.  The state afterwards is the state before.
+ * `chosen` (may be NULL) receives the resulting solver settings. */
+int eq_batch_autotune(eq_batch* b, int32_t repeats, eq_solver_desc* chosen);
+
 /* device-pointer variant: d_cells/d_q/d_sampled as above, on the batch stream */
 int eq_batch_couple_dev(eq_batch* b, int32_t trap, const eq_cell* d_cells, size_t nc, double npm,
                         const double* d_q, double* d_sampled, int32_t* d_counts, uint32_t flags);

@@ -300,3 +300,29 @@ def test_couple_traps_batched_matches_per_trap_and_oracle(eqf, oracle):
         for s in range(2):
             assert np.array_equal(got[2 * t + s], ref[s]), (t, s)
             assert np.array_equal(got1[2 * t + s], ref[s]), (t, s)
+
+
+def test_autotune_restores_state_and_keeps_parity(eqf, oracle):
+    """The autotuner's trial steps leave no trace: fields unchanged, and the
+    next step (with the chosen cycle shape) still matches the oracle."""
+    shapes = [(90, 45, 9.0, 4.5), (40, 70, 4.0, 7.0)]
+    walls = {oracle.TOP: (oracle.DIRICHLET, 0.0, 0, 0)}
+    descs, probs, u0 = [], [], []
+    rng = np.random.default_rng(5)
+    for (nx, ny, W, H) in shapes:
+        for D, g in ((300.0, 0.01), (80.0, 0.05)):
+            descs.append(eqf.field_desc(nx, ny, W, H, 0.1, D, g, 0.0, walls))
+            probs.append(oracle.problem(nx, ny, W, H, 0.1, D, g, 0.0, walls=walls))
+            u0.append(rng.random((nx + 1) * (ny + 1)))
+    with eqf.FieldBatch(descs, nspecies=2, solver=eqf.solver_desc(eqf.MIXED_GMG_CG)) as fb:
+        for k, u in enumerate(u0):
+            fb.upload(k, u)
+        sd = fb.autotune(1)
+        assert (sd.mg_pre, sd.mg_post) in ((1, 1), (2, 1), (2, 2), (3, 3))
+        for k, u in enumerate(u0):
+            assert np.array_equal(fb.download(k), u)
+        st = fb.step()
+        assert st.nconverged == len(descs)
+        for k in range(len(descs)):
+            ref, _ = oracle.step(probs[k], u0[k], "lu")
+            assert _rel(fb.download(k), ref) <= 1e-9, k
