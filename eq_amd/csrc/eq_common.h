@@ -50,6 +50,7 @@ template <typename T>
 struct Row {
     T mc, mw, me, ms, mn, msw, mne; // mass-like couplings (mass, reaction, Robin)
     T kw, ke, ks, kn;               // stiffness difference weights
+    T ksw, kne;                     // diagonal stiffness weights (0 unless D12 != 0)
 };
 
 // One problem (species x trap) on one multigrid level.
@@ -90,6 +91,15 @@ struct LevelProb {
     // 1/nx, 1/ny (grid transfers: hat arguments and index division)
     double rnx, rny;
     float rnxf, rnyf;
+    // per-node diffusion tensor (D11, D22, D12 in natural order, P1 in each
+    // triangle; fenics/hslD.ufl:31): arrays of the level's pool layout,
+    // D11 at tens[k], D22 at tens[k + tstride], D12 at tens[k + 2 tstride]
+    int32_t aniso;         // 1: the tensor arrays below define the stiffness
+    int32_t pad_;
+    double sxy;            // dt*D/2: stiffness per triangle for the mixed derivative
+    const double* tens_d;  // FP64 copy (level 0)
+    const float* tens_f;   // FP32 copy (levels smoothed in FP32)
+    int64_t tstride;
 };
 
 EQ_HD inline bool is_dirichlet(const LevelProb& P, int i, int j)
@@ -145,6 +155,8 @@ EQ_HD inline Row<T> row_coefs_at(const LevelProb& P, int i, int j)
     r.ke = (se + ne) * (T)P.sa;
     r.ks = (sw + se) * (T)P.sb;
     r.kn = (nw + ne) * (T)P.sb;
+    r.ksw = zero;
+    r.kne = zero;
     // Robin wall edges: dt*r*|e|*[[1/3,1/6],[1/6,1/3]]  (fenics/hslD.h:3284-3441)
     const T third = (T)(1.0 / 3.0), sixth = (T)(1.0 / 6.0);
     const T dn = (i > 0) ? one : zero, up = (i < P.ny) ? one : zero;
@@ -176,26 +188,93 @@ EQ_HD inline Row<T> row_coefs_at(const LevelProb& P, int i, int j)
     return r;
 }
 
-template <typename T>
+// TENS = false: no problem of the launch has a tensor field, so the diagonal
+// stiffness weights are zero and their terms are skipped (same values)
+template <bool TENS = true, typename T>
 EQ_HD inline T row_diag(const Row<T>& r)
 {
+    if constexpr (TENS) return r.mc + ((r.kw + r.ke) + (r.ks + r.kn)) + (r.ksw + r.kne);
     return r.mc + ((r.kw + r.ke) + (r.ks + r.kn));
 }
 
-// interior rows come from the per-level table, the rest are built in place
+// Stiffness of node (i,j) from a per-node tensor field: the six triangles
+// around the node, each with the mean of its three vertex tensors (the P1
+// tensor integrated exactly, fenics/hslD.h:3123-3259).  Per triangle of legs
+// hx, hy (area hx*hy/2) the element matrix dt*D*|T| g_a^T Dbar g_b gives, in
+// difference form k = -K_ab,
+//   x legs  dt*D*(hy/(2hx)) d11 - (dt*D/2) d12     y legs  dt*D*(hx/(2hy)) d22 - (dt*D/2) d12
+//   SW-NE hypotenuse (dt*D/2) d12
+// d11, d22, d12 point at node (i,j) of arrays with row pitch `pitch`; the
+// entries one node outside the grid are read only by triangles that exist.
 template <typename T>
-EQ_HD inline Row<T> row_coefs(const LevelProb& P, int i, int j)
+EQ_HD inline void aniso_stiffness(const LevelProb& P, int i, int j, const T* d11, const T* d22, const T* d12,
+                                  int pitch, Row<T>& r)
 {
+    const bool sw = i > 0 && j > 0, se = i > 0 && j < P.nx, nw = i < P.ny && j > 0, ne = i < P.ny && j < P.nx;
+    const T sa = (T)P.sa, sb = (T)P.sb, sx = (T)P.sxy, third = (T)(1.0 / 3.0);
+    // vertex offsets: c, W, E, S, N, SW, NE
+    const int oW = -1, oE = 1, oS = -pitch, oN = pitch, oSW = -pitch - 1, oNE = pitch + 1;
+    auto mean = [&](const T* a, int o1, int o2) { return (a[0] + a[o1] + a[o2]) * third; };
+    T kw = (T)0, ke = (T)0, ks = (T)0, kn = (T)0, ksw = (T)0, kne = (T)0;
+    if (sw) {
+        // lower-right triangle (SW, S, c) and upper-left (SW, W, c) of the SW cell
+        const T b12 = mean(d12, oSW, oS), b22 = mean(d22, oSW, oS);
+        ksw += sx * b12;
+        ks += sb * b22 - sx * b12;
+        const T c12 = mean(d12, oSW, oW), c11 = mean(d11, oSW, oW);
+        ksw += sx * c12;
+        kw += sa * c11 - sx * c12;
+    }
+    if (se) {
+        // upper-left triangle (S, c, E) of the SE cell
+        const T a12 = mean(d12, oS, oE);
+        ks += sb * mean(d22, oS, oE) - sx * a12;
+        ke += sa * mean(d11, oS, oE) - sx * a12;
+    }
+    if (nw) {
+        // lower-right triangle (W, c, N) of the NW cell
+        const T a12 = mean(d12, oW, oN);
+        kw += sa * mean(d11, oW, oN) - sx * a12;
+        kn += sb * mean(d22, oW, oN) - sx * a12;
+    }
+    if (ne) {
+        // lower-right (c, E, NE) and upper-left (c, N, NE) of the node's own cell
+        const T b12 = mean(d12, oE, oNE);
+        ke += sa * mean(d11, oE, oNE) - sx * b12;
+        kne += sx * b12;
+        const T c12 = mean(d12, oN, oNE);
+        kn += sb * mean(d22, oN, oNE) - sx * c12;
+        kne += sx * c12;
+    }
+    r.kw = kw; r.ke = ke; r.ks = ks; r.kn = kn; r.ksw = ksw; r.kne = kne;
+}
+
+template <typename T> EQ_HD inline const T* tens_base(const LevelProb& P);
+template <> EQ_HD inline const double* tens_base<double>(const LevelProb& P) { return P.tens_d; }
+template <> EQ_HD inline const float* tens_base<float>(const LevelProb& P) { return P.tens_f; }
+
+// Row of node (i,j), element k of the level pool.  Constant-coefficient
+// problems take the interior rows from the per-level table and build the
+// rest in place; problems with a tensor field build every row from it.
+template <typename T, bool TENS = true>
+EQ_HD inline Row<T> row_coefs(const LevelProb& P, int i, int j, int64_t k)
+{
+    if (TENS && P.aniso) {
+        Row<T> r = row_coefs_at<T>(P, i, j);
+        const T* t = tens_base<T>(P) + k;
+        aniso_stiffness<T>(P, i, j, t, t + P.tstride, t + 2 * P.tstride, P.pitch, r);
+        return r;
+    }
     if (interior(P, i, j)) return interior_row<T>(P);
     return row_coefs_at<T>(P, i, j);
 }
 
 // 1 / diag(A) at node (i,j) whose row is rc
-template <typename T>
+template <bool TENS = true, typename T>
 EQ_HD inline T row_dinv(const LevelProb& P, int i, int j, const Row<T>& rc)
 {
-    if (interior(P, i, j)) return interior_dinv<T>(P);
-    return (T)1 / row_diag(rc);
+    if ((!TENS || !P.aniso) && interior(P, i, j)) return interior_dinv<T>(P);
+    return (T)1 / row_diag<TENS>(rc);
 }
 
 // fills the interior row of a level (host)
@@ -233,10 +312,11 @@ EQ_HD inline int div_floor_pos(int a, int b, float rb)
 }
 
 // y_c = (A v)_c from the seven values around the node
-template <typename T>
+template <bool TENS = true, typename T>
 EQ_HD inline T row_apply(const Row<T>& r, T vc, T vw, T ve, T vs, T vn, T vsw, T vne)
 {
-    const T stiff = (r.kw * (vc - vw) + r.ke * (vc - ve)) + (r.ks * (vc - vs) + r.kn * (vc - vn));
+    T stiff = (r.kw * (vc - vw) + r.ke * (vc - ve)) + (r.ks * (vc - vs) + r.kn * (vc - vn));
+    if constexpr (TENS) stiff = stiff + (r.ksw * (vc - vsw) + r.kne * (vc - vne));
     const T mass = r.mc * vc + ((r.mw * vw + r.me * ve) + (r.ms * vs + r.mn * vn)) + (r.msw * vsw + r.mne * vne);
     return stiff + mass;
 }

@@ -54,6 +54,7 @@ struct Level {
     void* xa = nullptr;
     void* xb = nullptr;
     void* r = nullptr;  // residual (levels that restrict it in a separate launch)
+    void* tens = nullptr; // tensor field D11|D22|D12 in smoother precision (allocated on first use)
     LevelView all() const { return LevelView{d_probs, d_tiles_all, (int32_t)tiles_all.size()}; }
     LevelView smooth() const { return LevelView{d_probs, d_tiles_smooth, (int32_t)tiles_smooth.size()}; }
     LevelView bottom_tiles() const { return LevelView{d_probs, d_tiles_bottom, (int32_t)tiles_bottom.size()}; }
@@ -79,6 +80,7 @@ struct eq_batch {
     std::vector<eq_field_desc> descs;
     eq_solver_desc solver{};
     int32_t mixed = 0;                 // 1: FP32 V-cycle
+    bool any_tensor = false;           // some problem has a per-node diffusion tensor
     int64_t lds_budget = 64 * 1024;    // LDS bytes the bottom V-cycle may use per workgroup
     std::vector<eqk::Level> levels;
     // level 0 FP64 Krylov vectors (pooled over problems)
@@ -90,6 +92,7 @@ struct eq_batch {
     double* q = nullptr;
     double* z = nullptr;
     double* xprev = nullptr;    // last solution, the next solve's starting iterate
+    double* tens0_d = nullptr;  // level-0 tensor field D11|D22|D12 in FP64 (allocated on first use)
     double* partial = nullptr;  // 4 slots per level-0 tile
     void* dense = nullptr;      // dense inverses of coarsest levels (smoother precision)
     eqk::CgState* d_state = nullptr;

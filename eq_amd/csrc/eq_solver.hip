@@ -195,6 +195,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs(LevelView L, const double* __res
 // from the previous step's solution when it is kept (xprev), else from the
 // field itself: the field carries this step's cell deposits, whose sharp
 // edges would otherwise make the first residual ~1e4 times larger.
+template <bool TENS>
 __global__ __launch_bounds__(kBlock) void k_init(LevelView L, double* x, const double* __restrict__ xprev,
                                                  const double* __restrict__ b, double* __restrict__ r,
                                                  double* __restrict__ z, double* partial, const CgState* st,
@@ -224,12 +225,12 @@ __global__ __launch_bounds__(kBlock) void k_init(LevelView L, double* x, const d
             const double xs = val(i - 1, j, k - pt), xn = val(i + 1, j, k + pt);
             const double xsw = val(i - 1, j - 1, k - pt - 1), xne = val(i + 1, j + 1, k + pt + 1);
             if (prev) x[k] = xc;
-            const Row<double> rc = row_coefs<double>(P, i, j);
-            const double rv = b[k] - row_apply(rc, xc, xw, xe, xs, xn, xsw, xne);
+            const Row<double> rc = row_coefs<double, TENS>(P, i, j, k);
+            const double rv = b[k] - row_apply<TENS>(rc, xc, xw, xe, xs, xn, xsw, xne);
             r[k] = rv;
             v[0] += rv * rv;
             if (jacobi) {
-                const double zv = rv * row_dinv(P, i, j, rc);
+                const double zv = rv * row_dinv<TENS>(P, i, j, rc);
                 z[k] = zv;
                 v[1] += rv * zv;
             }
@@ -254,7 +255,7 @@ __global__ __launch_bounds__(kBlock) void k_keep(LevelView L, const double* __re
 // the storage type TP), q = A p_new in FP64 from the stored values, p_new.q.
 // q is exactly A times the direction x will move along, so the recurrence
 // residual tracks b - A x whatever TP is.  p_old and p_new are different buffers.
-template <typename TP>
+template <typename TP, bool TENS>
 __global__ __launch_bounds__(kBlock) void k_ap_dir(LevelView L, const TP* __restrict__ z, const TP* __restrict__ pold,
                                                    TP* __restrict__ pnew, double* __restrict__ q, double* partial,
                                                    const CgState* st)
@@ -275,8 +276,8 @@ __global__ __launch_bounds__(kBlock) void k_ap_dir(LevelView L, const TP* __rest
         const double nc = dir(zc, pc);
         double qv = 0.0;
         if (!is_dirichlet(P, n.i, n.j)) {
-            const Row<double> rc = row_coefs<double>(P, n.i, n.j);
-            qv = row_apply(rc, nc, dir(zw, pw), dir(ze, pe), dir(zs, ps), dir(zn, pn), dir(zsw, psw), dir(zne, pne));
+            const Row<double> rc = row_coefs<double, TENS>(P, n.i, n.j, n.idx);
+            qv = row_apply<TENS>(rc, nc, dir(zw, pw), dir(ze, pe), dir(zs, ps), dir(zn, pn), dir(zsw, psw), dir(zne, pne));
         }
         pnew[n.idx] = (TP)nc;
         q[n.idx] = qv;
@@ -286,7 +287,7 @@ __global__ __launch_bounds__(kBlock) void k_ap_dir(LevelView L, const TP* __rest
 }
 
 // x += alpha p; r -= alpha q; Jacobi: z = r/diag, r.z; flexible: r.z_old
-template <typename TP>
+template <typename TP, bool TENS>
 __global__ __launch_bounds__(kBlock) void k_update(LevelView L, double* __restrict__ x, double* __restrict__ r,
                                                    TP* __restrict__ z, const TP* __restrict__ p,
                                                    const double* __restrict__ q, double* partial, const CgState* st,
@@ -311,8 +312,8 @@ __global__ __launch_bounds__(kBlock) void k_update(LevelView L, double* __restri
         rr += rv * rv;
         double t = 0.0;
         if (jacobi) {
-            const Row<double> rc = row_coefs<double>(P, n.i, n.j);
-            const TP zv = (TP)(rv * row_dinv(P, n.i, n.j, rc));
+            const Row<double> rc = row_coefs<double, TENS>(P, n.i, n.j, n.idx);
+            const TP zv = (TP)(rv * row_dinv<TENS>(P, n.i, n.j, rc));
             z[k] = zv;
             t = rv * (double)zv;
         } else if (flexible) {
@@ -326,6 +327,7 @@ __global__ __launch_bounds__(kBlock) void k_update(LevelView L, double* __restri
 }
 
 // true residual r = b - A x (replaces the recurrence residual); final stop test
+template <bool TENS>
 __global__ __launch_bounds__(kBlock) void k_true_residual(LevelView L, const double* __restrict__ x,
                                                           const double* __restrict__ b, double* __restrict__ r,
                                                           double* partial)
@@ -338,8 +340,8 @@ __global__ __launch_bounds__(kBlock) void k_true_residual(LevelView L, const dou
         if (!n.in || is_dirichlet(P, n.i, n.j)) return;
         double xc, xw, xe, xs, xn, xsw, xne;
         load7(x, n.idx, P.pitch, xc, xw, xe, xs, xn, xsw, xne);
-        const Row<double> rc = row_coefs<double>(P, n.i, n.j);
-        const double rv = b[n.idx] - row_apply(rc, xc, xw, xe, xs, xn, xsw, xne);
+        const Row<double> rc = row_coefs<double, TENS>(P, n.i, n.j, n.idx);
+        const double rv = b[n.idx] - row_apply<TENS>(rc, xc, xw, xe, xs, xn, xsw, xne);
         r[n.idx] = rv;
         v[0] += rv * rv;
     });
@@ -347,6 +349,7 @@ __global__ __launch_bounds__(kBlock) void k_true_residual(LevelView L, const dou
 }
 
 // Jacobi preconditioner on its own (restart after a residual replacement): z = r/diag, r.z
+template <bool TENS>
 __global__ __launch_bounds__(kBlock) void k_jacobi_z(LevelView L, const double* __restrict__ r,
                                                      double* __restrict__ z, double* partial, const CgState* st)
 {
@@ -359,8 +362,8 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_z(LevelView L, const double* 
         if (!n.in) return;
         double zv = 0.0;
         if (!is_dirichlet(P, n.i, n.j)) {
-            const Row<double> rc = row_coefs<double>(P, n.i, n.j);
-            zv = r[n.idx] * row_dinv(P, n.i, n.j, rc);
+            const Row<double> rc = row_coefs<double, TENS>(P, n.i, n.j, n.idx);
+            zv = r[n.idx] * row_dinv<TENS>(P, n.i, n.j, rc);
         }
         z[n.idx] = zv;
         v[0] += r[n.idx] * zv;
@@ -454,28 +457,30 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(const LevelProb* __re
 // LDS copies of the bottom levels.
 
 // damped Jacobi value at a free node: x + omega (b - A x) / diag  (x = 0 when zero_init)
-template <typename T, typename TB>
+// (k indexes b and x with row pitch `pitch`; kt is the node's element in the
+// level pool, where the tensor arrays live)
+template <typename T, typename TB, bool TENS>
 __device__ __forceinline__ T smooth_val(const LevelProb& P, int i, int j, int64_t k, int pitch,
                                         const TB* __restrict__ b, const T* __restrict__ xin, T omega,
-                                        bool zero_init, T& bv)
+                                        bool zero_init, T& bv, int64_t kt)
 {
-    const Row<T> rc = row_coefs<T>(P, i, j);
-    const T wd = omega * row_dinv(P, i, j, rc);
+    const Row<T> rc = row_coefs<T, TENS>(P, i, j, kt);
+    const T wd = omega * row_dinv<TENS>(P, i, j, rc);
     bv = (T)b[k];
     if (zero_init) return wd * bv;
     T xc, xw, xe, xs, xn, xsw, xne;
     load7(xin, k, pitch, xc, xw, xe, xs, xn, xsw, xne);
-    return xc + wd * (bv - row_apply(rc, xc, xw, xe, xs, xn, xsw, xne));
+    return xc + wd * (bv - row_apply<TENS>(rc, xc, xw, xe, xs, xn, xsw, xne));
 }
 
-template <typename T>
+template <typename T, bool TENS>
 __device__ __forceinline__ T residual_val(const LevelProb& P, int i, int j, int64_t k, int pitch,
-                                          const T* __restrict__ b, const T* __restrict__ x)
+                                          const T* __restrict__ b, const T* __restrict__ x, int64_t kt)
 {
     T xc, xw, xe, xs, xn, xsw, xne;
     load7(x, k, pitch, xc, xw, xe, xs, xn, xsw, xne);
-    const Row<T> rc = row_coefs<T>(P, i, j);
-    return b[k] - row_apply(rc, xc, xw, xe, xs, xn, xsw, xne);
+    const Row<T> rc = row_coefs<T, TENS>(P, i, j, kt);
+    return b[k] - row_apply<TENS>(rc, xc, xw, xe, xs, xn, xsw, xne);
 }
 
 // b_c(I,J) = sum_f phi_IJ(f) r_f over the fine nodes within one coarse cell of
@@ -548,7 +553,7 @@ __device__ __forceinline__ T prolong_val(const LevelProb& Pf, const LevelProb& P
 // Damped Jacobi sweep over the launched tiles.  bcopy (optional) keeps b in
 // the smoother precision; dot (last level-0 sweep) adds the partial r.z with
 // b the FP64 residual and x_out the preconditioned residual.
-template <int R, typename T, typename TB, typename TO>
+template <int R, typename T, typename TB, typename TO, bool TENS>
 __global__ __launch_bounds__(kBlock) void k_smooth(LevelView L, const TB* __restrict__ b, const T* __restrict__ xin,
                                                    TO* __restrict__ xout, T* __restrict__ bcopy, T omega,
                                                    int zero_init, int dot, double* partial, const CgState* st)
@@ -565,7 +570,7 @@ __global__ __launch_bounds__(kBlock) void k_smooth(LevelView L, const TB* __rest
             if (bcopy) bcopy[n.idx] = (T)0;
         } else {
             T bv;
-            const T out = smooth_val<T, TB>(P, n.i, n.j, n.idx, P.pitch, b, xin, omega, zero_init != 0, bv);
+            const T out = smooth_val<T, TB, TENS>(P, n.i, n.j, n.idx, P.pitch, b, xin, omega, zero_init != 0, bv, n.idx);
             if (bcopy) bcopy[n.idx] = bv;
             const TO o = (TO)out;
             xout[n.idx] = o;
@@ -575,7 +580,7 @@ __global__ __launch_bounds__(kBlock) void k_smooth(LevelView L, const TB* __rest
     if (dot) tile_partial<1>(v, partial, c.gt);
 }
 
-template <int R, typename T>
+template <int R, typename T, bool TENS>
 __global__ __launch_bounds__(kBlock) void k_residual(LevelView L, const T* __restrict__ b, const T* __restrict__ x,
                                                      T* __restrict__ r, const CgState* st)
 {
@@ -585,7 +590,7 @@ __global__ __launch_bounds__(kBlock) void k_residual(LevelView L, const T* __res
     if (st[P.prob].done) return;
     each_row<R>(c, [&](const NodeCtx& n) {
         if (!n.in) return;
-        r[n.idx] = is_dirichlet(P, n.i, n.j) ? (T)0 : residual_val<T>(P, n.i, n.j, n.idx, P.pitch, b, x);
+        r[n.idx] = is_dirichlet(P, n.i, n.j) ? (T)0 : residual_val<T, TENS>(P, n.i, n.j, n.idx, P.pitch, b, x, n.idx);
     });
 }
 
@@ -626,18 +631,18 @@ __global__ __launch_bounds__(kBlock) void k_prolong(LevelView Lf, const LevelPro
 
 // x after the first sweep from x = 0: omega b / diag (0 on Dirichlet nodes
 // and outside the grid)
-template <typename T, typename TB>
+template <typename T, typename TB, bool TENS>
 __device__ __forceinline__ T first_sweep_at(const LevelProb& P, int i, int j, int64_t k, const TB* __restrict__ b,
                                             T omega)
 {
     if (i < 0 || j < 0 || i > P.ny || j > P.nx || is_dirichlet(P, i, j)) return (T)0;
-    const Row<T> rc = row_coefs<T>(P, i, j);
-    const T wd = omega * row_dinv(P, i, j, rc);
+    const Row<T> rc = row_coefs<T, TENS>(P, i, j, k);
+    const T wd = omega * row_dinv<TENS>(P, i, j, rc);
     return wd * (T)b[k];
 }
 
 // pre-smoothing sweeps 1 and 2 (sweep 1 from x = 0); bcopy as in k_smooth
-template <int R, typename T, typename TB>
+template <int R, typename T, typename TB, bool TENS>
 __global__ __launch_bounds__(kBlock) void k_presmooth2(LevelView L, const TB* __restrict__ b, T* __restrict__ xout,
                                                        T* __restrict__ bcopy, T omega, const CgState* st)
 {
@@ -654,23 +659,23 @@ __global__ __launch_bounds__(kBlock) void k_presmooth2(LevelView L, const TB* __
             if (bcopy) bcopy[k] = (T)0;
             return;
         }
-        const T xc = first_sweep_at<T, TB>(P, i, j, k, b, omega);
-        const T xw = first_sweep_at<T, TB>(P, i, j - 1, k - 1, b, omega);
-        const T xe = first_sweep_at<T, TB>(P, i, j + 1, k + 1, b, omega);
-        const T xs = first_sweep_at<T, TB>(P, i - 1, j, k - pt, b, omega);
-        const T xn = first_sweep_at<T, TB>(P, i + 1, j, k + pt, b, omega);
-        const T xsw = first_sweep_at<T, TB>(P, i - 1, j - 1, k - pt - 1, b, omega);
-        const T xne = first_sweep_at<T, TB>(P, i + 1, j + 1, k + pt + 1, b, omega);
-        const Row<T> rc = row_coefs<T>(P, i, j);
-        const T wd = omega * row_dinv(P, i, j, rc);
+        const T xc = first_sweep_at<T, TB, TENS>(P, i, j, k, b, omega);
+        const T xw = first_sweep_at<T, TB, TENS>(P, i, j - 1, k - 1, b, omega);
+        const T xe = first_sweep_at<T, TB, TENS>(P, i, j + 1, k + 1, b, omega);
+        const T xs = first_sweep_at<T, TB, TENS>(P, i - 1, j, k - pt, b, omega);
+        const T xn = first_sweep_at<T, TB, TENS>(P, i + 1, j, k + pt, b, omega);
+        const T xsw = first_sweep_at<T, TB, TENS>(P, i - 1, j - 1, k - pt - 1, b, omega);
+        const T xne = first_sweep_at<T, TB, TENS>(P, i + 1, j + 1, k + pt + 1, b, omega);
+        const Row<T> rc = row_coefs<T, TENS>(P, i, j, k);
+        const T wd = omega * row_dinv<TENS>(P, i, j, rc);
         const T bv = (T)b[k];
-        xout[k] = xc + wd * (bv - row_apply(rc, xc, xw, xe, xs, xn, xsw, xne));
+        xout[k] = xc + wd * (bv - row_apply<TENS>(rc, xc, xw, xe, xs, xn, xsw, xne));
         if (bcopy) bcopy[k] = bv;
     });
 }
 
 // residual of the fine level and its restriction, one coarse node per thread
-template <typename T>
+template <typename T, bool TENS>
 __global__ __launch_bounds__(kBlock) void k_residual_restrict(LevelView Lc, const LevelProb* __restrict__ fine,
                                                               const T* __restrict__ bf, const T* __restrict__ xf,
                                                               T* __restrict__ bc, const CgState* st)
@@ -682,7 +687,7 @@ __global__ __launch_bounds__(kBlock) void k_residual_restrict(LevelView Lc, cons
     const LevelProb& Pf = fine[Pc.fidx];
     auto racc = [&](int i, int j, int64_t k) -> T {
         if (i < 0 || j < 0 || i > Pf.ny || j > Pf.nx || is_dirichlet(Pf, i, j)) return (T)0;
-        return residual_val<T>(Pf, i, j, Pf.origin + k, Pf.pitch, bf, xf);
+        return residual_val<T, TENS>(Pf, i, j, Pf.origin + k, Pf.pitch, bf, xf, Pf.origin + k);
     };
     each_row<kRowsC>(c, [&](const NodeCtx& n) {
         if (!n.in) return;
@@ -692,7 +697,7 @@ __global__ __launch_bounds__(kBlock) void k_residual_restrict(LevelView Lc, cons
 
 // coarse-grid correction x += P x_c followed by the first post-smoothing
 // sweep (xin holds x, xout receives the sweep; dot as in k_smooth)
-template <int R, typename T, typename TB, typename TO>
+template <int R, typename T, typename TB, typename TO, bool TENS>
 __global__ __launch_bounds__(kBlock) void k_prolong_smooth(LevelView Lf, const LevelProb* __restrict__ coarse,
                                                            const T* __restrict__ xc, const TB* __restrict__ b,
                                                            const T* __restrict__ xin, TO* __restrict__ xout, T omega,
@@ -719,10 +724,10 @@ __global__ __launch_bounds__(kBlock) void k_prolong_smooth(LevelView Lf, const L
         const T yc = y_at(i, j, k), yw = y_at(i, j - 1, k - 1), ye = y_at(i, j + 1, k + 1);
         const T ys = y_at(i - 1, j, k - pt), yn = y_at(i + 1, j, k + pt);
         const T ysw = y_at(i - 1, j - 1, k - pt - 1), yne = y_at(i + 1, j + 1, k + pt + 1);
-        const Row<T> rc = row_coefs<T>(P, i, j);
-        const T wd = omega * row_dinv(P, i, j, rc);
+        const Row<T> rc = row_coefs<T, TENS>(P, i, j, k);
+        const T wd = omega * row_dinv<TENS>(P, i, j, rc);
         const T bv = (T)b[k];
-        const TO o = (TO)(yc + wd * (bv - row_apply(rc, yc, yw, ye, ys, yn, ysw, yne)));
+        const TO o = (TO)(yc + wd * (bv - row_apply<TENS>(rc, yc, yw, ye, ys, yn, ysw, yne)));
         xout[k] = o;
         if (dot) v[0] += (double)b[k] * (double)o;
     });
@@ -823,7 +828,7 @@ __device__ __forceinline__ void for_nodes_split(const LevelProb& P, FI&& fi, FB&
     }
 }
 
-template <typename T>
+template <typename T, bool TENS>
 __device__ __forceinline__ void lds_sweep(const LevelProb& P, const LdsLevel<T>& S, int from, int to, T omega,
                                           bool zero_init)
 {
@@ -831,9 +836,21 @@ __device__ __forceinline__ void lds_sweep(const LevelProb& P, const LdsLevel<T>&
     const T wdi = omega * interior_dinv<T>(P);
     T* __restrict__ xo = S.x(to);
     const T* __restrict__ xi = S.x(from);
+    auto general = [&](int i, int j) {
+        if (!is_dirichlet(P, i, j)) {
+            T bv;
+            const int64_t k = S.at(i, j);
+            xo[k] = smooth_val<T, T, TENS>(P, i, j, k, S.pitch, S.b, xi, omega, zero_init, bv,
+                                     P.origin + (int64_t)i * P.pitch + j);
+        }
+    };
     for_nodes_split(
         P,
         [&](int i, int j) {
+            if (P.aniso) { // no common interior row
+                general(i, j);
+                return;
+            }
             const int64_t k = S.at(i, j);
             const T bv = S.b[k];
             if (zero_init) {
@@ -841,16 +858,10 @@ __device__ __forceinline__ void lds_sweep(const LevelProb& P, const LdsLevel<T>&
             } else {
                 T xc, xw, xe, xs, xn, xsw, xne;
                 load7(xi, k, S.pitch, xc, xw, xe, xs, xn, xsw, xne);
-                xo[k] = xc + wdi * (bv - row_apply(ri, xc, xw, xe, xs, xn, xsw, xne));
+                xo[k] = xc + wdi * (bv - row_apply<TENS>(ri, xc, xw, xe, xs, xn, xsw, xne));
             }
         },
-        [&](int i, int j) {
-            if (!is_dirichlet(P, i, j)) {
-                T bv;
-                const int64_t k = S.at(i, j);
-                xo[k] = smooth_val<T, T>(P, i, j, k, S.pitch, S.b, xi, omega, zero_init, bv);
-            }
-        });
+        general);
     __syncthreads();
 }
 
@@ -858,7 +869,7 @@ __device__ __forceinline__ void lds_sweep(const LevelProb& P, const LdsLevel<T>&
 // bottom level's right-hand side is read from HBM, its correction is written
 // to `out` (the buffer its finer level's prolongation reads).  Dirichlet
 // entries stay at the zero the LDS is cleared to.
-template <typename T>
+template <typename T, bool TENS>
 __global__ __launch_bounds__(kBottomThreads) void k_bottom(BottomArgs<T> A, int lb, const int32_t* __restrict__ list,
                                                            T* __restrict__ out, T omega, int pre, int post,
                                                            const T* __restrict__ dense, const CgState* st)
@@ -895,10 +906,10 @@ __global__ __launch_bounds__(kBottomThreads) void k_bottom(BottomArgs<T> A, int 
         const LevelProb P = A.probs[lb + k][__builtin_amdgcn_readfirstlane(idx[k])];
         const LevelProb Pc = A.probs[lb + k + 1][__builtin_amdgcn_readfirstlane(idx[k + 1])];
         const LdsLevel<T> S = lds_level(sm, P), C = lds_level(sm, Pc);
-        lds_sweep<T>(P, S, 0, 0, omega, true);
+        lds_sweep<T, TENS>(P, S, 0, 0, omega, true);
         int cur = 0;
         for (int s = 1; s < pre; ++s) {
-            lds_sweep<T>(P, S, cur, cur ^ 1, omega, false);
+            lds_sweep<T, TENS>(P, S, cur, cur ^ 1, omega, false);
             cur ^= 1;
         }
         if (threadIdx.x == 0) fin[k] = cur;
@@ -907,18 +918,24 @@ __global__ __launch_bounds__(kBottomThreads) void k_bottom(BottomArgs<T> A, int 
         {
             const Row<T> ri = interior_row<T>(P);
             const T* __restrict__ xc_ = S.x(cur);
+            auto general = [&](int i, int j) {
+                const int64_t kk = S.at(i, j);
+                r[kk] = is_dirichlet(P, i, j) ? (T)0 : residual_val<T, TENS>(P, i, j, kk, S.pitch, S.b, xc_,
+                                                                          P.origin + (int64_t)i * P.pitch + j);
+            };
             for_nodes_split(
                 P,
                 [&](int i, int j) {
+                    if (P.aniso) {
+                        general(i, j);
+                        return;
+                    }
                     const int64_t kk = S.at(i, j);
                     T xc, xw, xe, xs, xn, xsw, xne;
                     load7(xc_, kk, S.pitch, xc, xw, xe, xs, xn, xsw, xne);
-                    r[kk] = S.b[kk] - row_apply(ri, xc, xw, xe, xs, xn, xsw, xne);
+                    r[kk] = S.b[kk] - row_apply<TENS>(ri, xc, xw, xe, xs, xn, xsw, xne);
                 },
-                [&](int i, int j) {
-                    const int64_t kk = S.at(i, j);
-                    r[kk] = is_dirichlet(P, i, j) ? (T)0 : residual_val<T>(P, i, j, kk, S.pitch, S.b, xc_);
-                });
+                general);
         }
         __syncthreads();
         EQ_FOR_NODES(Pc)
@@ -971,7 +988,7 @@ __global__ __launch_bounds__(kBottomThreads) void k_bottom(BottomArgs<T> A, int 
         }
         __syncthreads();
         for (int s = 0; s < post; ++s) {
-            lds_sweep<T>(P, S, cur, cur ^ 1, omega, false);
+            lds_sweep<T, TENS>(P, S, cur, cur ^ 1, omega, false);
             cur ^= 1;
         }
         from = cur;
@@ -985,6 +1002,7 @@ __global__ __launch_bounds__(kBottomThreads) void k_bottom(BottomArgs<T> A, int 
 // ------------------------------------------------------------ diagnostics
 
 // mode 0: y = A x (Dirichlet rows identity); mode 1: y = b(x) (Dirichlet rows = g)
+template <bool TENS>
 __global__ __launch_bounds__(kBlock) void k_apply_full(LevelView L, int pi_only, const double* __restrict__ x,
                                                        double* __restrict__ y, int mode)
 {
@@ -999,8 +1017,8 @@ __global__ __launch_bounds__(kBlock) void k_apply_full(LevelView L, int pi_only,
         if (is_dirichlet(P, n.i, n.j)) {
             y[n.idx] = (mode == 0) ? vc : dirichlet_value(P, n.i, n.j);
         } else if (mode == 0) {
-            const Row<double> rc = row_coefs<double>(P, n.i, n.j);
-            y[n.idx] = row_apply(rc, vc, vw, ve, vs, vn, vsw, vne);
+            const Row<double> rc = row_coefs<double, TENS>(P, n.i, n.j, n.idx);
+            y[n.idx] = row_apply<TENS>(rc, vc, vw, ve, vs, vn, vsw, vne);
         } else {
             y[n.idx] = rhs_row(P, n.i, n.j, vc, vw, ve, vs, vn, vsw, vne);
         }
@@ -1100,6 +1118,7 @@ static void fill_level_prob(LevelProb& P, const eq_field_desc& d, int nx, int ny
     P.mo = (1.0 + d.dt * d.gamma) * (hx * hy / 24.0);
     P.sa = d.dt * d.D * hy / (2.0 * hx);
     P.sb = d.dt * d.D * hx / (2.0 * hy);
+    P.sxy = d.dt * d.D / 2.0;
     const double elen[4] = {hy, hy, hx, hx};
     P.dmask = 0;
     for (int w = 0; w < 4; ++w) {
@@ -1125,7 +1144,8 @@ static void fill_level_prob(LevelProb& P, const eq_field_desc& d, int nx, int ny
 // identity) and banded (half-bandwidth nx+2 in natural order), so LU without
 // pivoting inside the band is stable and costs O(n bw^2), and the n
 // solves O(n^2 bw), instead of O(n^3) for a dense elimination.
-static bool dense_inverse(const LevelProb& P, std::vector<double>& inv)
+static bool dense_inverse(const LevelProb& P, const double* h11, const double* h22, const double* h12,
+                          std::vector<double>& inv)
 {
     const int nw = P.nx + 1, n = nw * (P.ny + 1);
     const int bw = nw + 1;
@@ -1137,7 +1157,9 @@ static bool dense_inverse(const LevelProb& P, std::vector<double>& inv)
                 A[(size_t)k * n + k] = 1.0;
                 continue;
             }
-            const Row<double> r = row_coefs<double>(P, i, j);
+            // h11/h22/h12: the level's tensor in natural order (NULL: constant coefficients)
+            Row<double> r = row_coefs_at<double>(P, i, j);
+            if (h11) aniso_stiffness<double>(P, i, j, h11 + k, h22 + k, h12 + k, nw, r);
             auto put = [&](int ii, int jj, double v) {
                 if (ii < 0 || jj < 0 || ii > P.ny || jj > P.nx || v == 0.0) return;
                 if (is_dirichlet(P, ii, jj)) return;
@@ -1148,8 +1170,8 @@ static bool dense_inverse(const LevelProb& P, std::vector<double>& inv)
             put(i, j + 1, r.me - r.ke);
             put(i - 1, j, r.ms - r.ks);
             put(i + 1, j, r.mn - r.kn);
-            put(i - 1, j - 1, r.msw);
-            put(i + 1, j + 1, r.mne);
+            put(i - 1, j - 1, r.msw - r.ksw);
+            put(i + 1, j + 1, r.mne - r.kne);
         }
     // in-place banded LU (multipliers below the diagonal)
     for (int c = 0; c < n; ++c) {
@@ -1330,7 +1352,8 @@ static int build_hierarchy(eq_batch* b)
             }
             if (P.coarsest && mg) {
                 std::vector<double> inv;
-                if (!dense_inverse(P, inv)) return set_error(EQ_ERR_INVALID, "singular coarse operator");
+                if (!dense_inverse(P, nullptr, nullptr, nullptr, inv))
+                    return set_error(EQ_ERR_INVALID, "singular coarse operator");
                 P.dense_off = (int64_t)dense_all.size();
                 for (double v : inv) dense_all.push_back((T)v); // column-major already
             }
@@ -1359,7 +1382,10 @@ static int build_hierarchy(eq_batch* b)
     size_t lds_max = 0;
     for (const Level& L : b->levels) lds_max = std::max(lds_max, L.bottom_lds_bytes);
     if (lds_max > 64 * 1024)
-        EQ_HIP(hipFuncSetAttribute((const void*)k_bottom<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max));
+        EQ_HIP(hipFuncSetAttribute((const void*)k_bottom<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds_max));
+        EQ_HIP(hipFuncSetAttribute((const void*)k_bottom<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds_max));
     const int64_t n0 = b->levels[0].pool;
     if ((rc = alloc_zero(&b->u, n0, s)) || (rc = alloc_zero(&b->rhs, n0, s)) || (rc = alloc_zero(&b->r, n0, s)) ||
         (rc = alloc_zero(&b->p, n0, s)) || (rc = alloc_zero(&b->p2, n0, s)) || (rc = alloc_zero(&b->q, n0, s)) ||
@@ -1372,7 +1398,7 @@ static int build_hierarchy(eq_batch* b)
 
 // ------------------------------------------------------------ V-cycle
 
-template <typename T>
+template <typename T, bool TENS>
 struct VCycle {
     eq_batch* b;
     BottomArgs<T> args;
@@ -1438,21 +1464,21 @@ struct VCycle {
             if (S.mg_pre >= 2 && fuse) {
                 // sweeps 1 and 2 in one launch
                 if (top)
-                    k_presmooth2<R, T, double><<<gs, kBlock, 0, s>>>(L.smooth(), b->r, X[1], bcopy, omega, st);
+                    k_presmooth2<R, T, double, TENS><<<gs, kBlock, 0, s>>>(L.smooth(), b->r, X[1], bcopy, omega, st);
                 else
-                    k_presmooth2<R, T, T><<<gs, kBlock, 0, s>>>(L.smooth(), bl, X[1], nullptr, omega, st);
+                    k_presmooth2<R, T, T, TENS><<<gs, kBlock, 0, s>>>(L.smooth(), bl, X[1], nullptr, omega, st);
                 cur = 1;
             } else if (top) {
-                k_smooth<R, T, double, T><<<gs, kBlock, 0, s>>>(L.smooth(), b->r, nullptr, X[0], bcopy, omega, 1, 0,
+                k_smooth<R, T, double, T, TENS><<<gs, kBlock, 0, s>>>(L.smooth(), b->r, nullptr, X[0], bcopy, omega, 1, 0,
                                                                 nullptr, st);
             } else {
-                k_smooth<R, T, T, T><<<gs, kBlock, 0, s>>>(L.smooth(), bl, nullptr, X[0], nullptr, omega, 1, 0, nullptr,
+                k_smooth<R, T, T, T, TENS><<<gs, kBlock, 0, s>>>(L.smooth(), bl, nullptr, X[0], nullptr, omega, 1, 0, nullptr,
                                                            st);
             }
             if (top) prof_end(b, EQ_KERNEL_SMOOTH, nodes * (bytes_b + sizeof(T) * (bcopy ? 2 : 1)));
             for (int k = (S.mg_pre >= 2 && fuse) ? 2 : 1; k < S.mg_pre; ++k) {
                 if (top) prof_begin(b, EQ_KERNEL_SMOOTH);
-                k_smooth<R, T, T, T><<<gs, kBlock, 0, s>>>(L.smooth(), bl, X[cur], X[cur ^ 1], nullptr, omega, 0, 0,
+                k_smooth<R, T, T, T, TENS><<<gs, kBlock, 0, s>>>(L.smooth(), bl, X[cur], X[cur ^ 1], nullptr, omega, 0, 0,
                                                            nullptr, st);
                 if (top) prof_end(b, EQ_KERNEL_SMOOTH, nodes * 3 * sizeof(T));
                 cur ^= 1;
@@ -1461,9 +1487,9 @@ struct VCycle {
             Level& C = b->levels[l + 1];
             const int na = (int)C.tiles_active.size();
             if (fuse) {
-                k_residual_restrict<T><<<grid_for(na), kBlock, 0, s>>>(C.active(), L.d_probs, bl, X[cur], (T*)C.b, st);
+                k_residual_restrict<T, TENS><<<grid_for(na), kBlock, 0, s>>>(C.active(), L.d_probs, bl, X[cur], (T*)C.b, st);
             } else {
-                k_residual<R, T><<<gs, kBlock, 0, s>>>(L.smooth(), bl, X[cur], (T*)L.r, st);
+                k_residual<R, T, TENS><<<gs, kBlock, 0, s>>>(L.smooth(), bl, X[cur], (T*)L.r, st);
                 k_restrict<T><<<grid_for(na), kBlock, 0, s>>>(C.active(), L.d_probs, (const T*)L.r, (T*)C.b, st);
             }
             run(l + 1);
@@ -1479,17 +1505,17 @@ struct VCycle {
                 if (top) prof_begin(b, EQ_KERNEL_SMOOTH);
                 if (top && last) {
                     if (first)
-                        k_prolong_smooth<R, T, double, T><<<gs, kBlock, 0, s>>>(L.smooth(), C.d_probs, xc, b->r, X[cur],
+                        k_prolong_smooth<R, T, double, T, TENS><<<gs, kBlock, 0, s>>>(L.smooth(), C.d_probs, xc, b->r, X[cur],
                                                                                 (T*)b->z, omega, 1, b->partial, st);
                     else
-                        k_smooth<R, T, double, T><<<gs, kBlock, 0, s>>>(L.smooth(), b->r, X[cur], (T*)b->z, nullptr,
+                        k_smooth<R, T, double, T, TENS><<<gs, kBlock, 0, s>>>(L.smooth(), b->r, X[cur], (T*)b->z, nullptr,
                                                                         omega, 0, 1, b->partial, st);
                 } else {
                     if (first)
-                        k_prolong_smooth<R, T, T, T><<<gs, kBlock, 0, s>>>(L.smooth(), C.d_probs, xc, bl, X[cur],
+                        k_prolong_smooth<R, T, T, T, TENS><<<gs, kBlock, 0, s>>>(L.smooth(), C.d_probs, xc, bl, X[cur],
                                                                            X[cur ^ 1], omega, 0, nullptr, st);
                     else
-                        k_smooth<R, T, T, T><<<gs, kBlock, 0, s>>>(L.smooth(), bl, X[cur], X[cur ^ 1], nullptr, omega, 0,
+                        k_smooth<R, T, T, T, TENS><<<gs, kBlock, 0, s>>>(L.smooth(), bl, X[cur], X[cur ^ 1], nullptr, omega, 0,
                                                                    0, nullptr, st);
                     cur ^= 1;
                 }
@@ -1502,7 +1528,7 @@ struct VCycle {
                 const int nb = (int)L.tiles_bottom.size();
                 k_cast_in<T><<<grid_for(nb), kBlock, 0, s>>>(L.bottom_tiles(), b->r, (T*)L.b, st);
             }
-            k_bottom<T><<<(int)L.bottom.size(), kBottomThreads, L.bottom_lds_bytes, s>>>(args, l, L.d_bottom, out, omega, S.mg_pre,
+            k_bottom<T, TENS><<<(int)L.bottom.size(), kBottomThreads, L.bottom_lds_bytes, s>>>(args, l, L.d_bottom, out, omega, S.mg_pre,
                                                                         S.mg_post, (const T*)b->dense, st);
             if (top) {
                 const int nb = (int)L.tiles_bottom.size();
@@ -1521,7 +1547,7 @@ static int read_states(eq_batch* b)
     return EQ_OK;
 }
 
-template <typename T>
+template <typename T, bool TENS>
 static int solve_all(eq_batch* b, eq_step_stats* stats)
 {
     const auto t0 = std::chrono::steady_clock::now();
@@ -1532,7 +1558,7 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
     hipStream_t s = b->stream;
     const bool jacobi = (S.kind == EQ_SOLVER_JACOBI_CG);
     const double fine_nodes = L0.nodes_all;
-    VCycle<T> vc(b);
+    VCycle<T, TENS> vc(b);
     auto finish = [&](int op) {
         k_finish<<<b->nprob, kFinishThreads, 0, s>>>(L0.d_probs, b->partial, b->d_state, op, jacobi ? 1 : 0);
     };
@@ -1543,11 +1569,11 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
             finish(kFinRz);
         }
         prof_begin(b, EQ_KERNEL_STENCIL64);
-        k_ap_dir<T><<<g0, kBlock, 0, s>>>(A, (const T*)b->z, (const T*)pold, (T*)pnew, b->q, b->partial, b->d_state);
+        k_ap_dir<T, TENS><<<g0, kBlock, 0, s>>>(A, (const T*)b->z, (const T*)pold, (T*)pnew, b->q, b->partial, b->d_state);
         prof_end(b, EQ_KERNEL_STENCIL64, fine_nodes * 4 * sizeof(double));
         finish(kFinAp);
         prof_begin(b, EQ_KERNEL_CG_UPDATE);
-        k_update<T><<<g0, kBlock, 0, s>>>(A, b->u, b->r, (T*)b->z, (const T*)pnew, b->q, b->partial, b->d_state,
+        k_update<T, TENS><<<g0, kBlock, 0, s>>>(A, b->u, b->r, (T*)b->z, (const T*)pnew, b->q, b->partial, b->d_state,
                                           jacobi ? 1 : 0);
         prof_end(b, EQ_KERNEL_CG_UPDATE, fine_nodes * sizeof(double) * ((jacobi || b->mixed) ? 7 : 6));
         finish(kFinUpdate);
@@ -1559,7 +1585,7 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
     auto setup = [&]() {
         k_rhs<<<g0, kBlock, 0, s>>>(A, b->u, b->rhs, b->partial);
         finish(kFinRhs);
-        k_init<<<g0, kBlock, 0, s>>>(A, b->u, b->xprev, b->rhs, b->r, b->z, b->partial, b->d_state, jacobi ? 1 : 0);
+        k_init<TENS><<<g0, kBlock, 0, s>>>(A, b->u, b->xprev, b->rhs, b->r, b->z, b->partial, b->d_state, jacobi ? 1 : 0);
         finish(kFinInit);
     };
     // Launch sequences are replayed from hipGraphs unless per-launch event
@@ -1607,14 +1633,14 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
         for (int p = 0; p < b->nprob; ++p) all = all && b->h_state[p].done;
         if (all) {
             // confirm on the true residual; problems above rtol restart CG from it
-            k_true_residual<<<g0, kBlock, 0, s>>>(A, b->u, b->rhs, b->r, b->partial);
+            k_true_residual<TENS><<<g0, kBlock, 0, s>>>(A, b->u, b->rhs, b->r, b->partial);
             finish(kFinTrue);
             if ((rc = read_states(b))) return rc;
             all = true;
             for (int p = 0; p < b->nprob; ++p) all = all && b->h_state[p].done;
             if (all) break;
             if (jacobi) {
-                k_jacobi_z<<<g0, kBlock, 0, s>>>(A, b->r, b->z, b->partial, b->d_state);
+                k_jacobi_z<TENS><<<g0, kBlock, 0, s>>>(A, b->r, b->z, b->partial, b->d_state);
                 finish(kFinRz);
             }
         }
@@ -1654,6 +1680,13 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
 using namespace eqk;
 
 // ------------------------------------------------------------ C ABI
+
+// precision of the multigrid x whether any problem carries a tensor field
+static int run_solve(eq_batch* b, eq_step_stats* stats)
+{
+    if (b->mixed) return b->any_tensor ? solve_all<float, true>(b, stats) : solve_all<float, false>(b, stats);
+    return b->any_tensor ? solve_all<double, true>(b, stats) : solve_all<double, false>(b, stats);
+}
 
 extern "C" {
 
@@ -1775,6 +1808,7 @@ void eq_batch_destroy(eq_batch* b)
         hipFree(L.xa);
         hipFree(L.xb);
         hipFree(L.r);
+        hipFree(L.tens);
     }
     hipFree(b->u);
     hipFree(b->rhs);
@@ -1782,6 +1816,7 @@ void eq_batch_destroy(eq_batch* b)
     hipFree(b->p);
     hipFree(b->p2);
     hipFree(b->xprev);
+    if (b->mixed) hipFree(b->tens0_d); // else it is level 0's own tensor array
     hipFree(b->q);
     hipFree(b->z);
     hipFree(b->partial);
@@ -1880,7 +1915,149 @@ int eq_batch_step(eq_batch* b, eq_step_stats* stats)
 {
     if (!b) return set_error(EQ_ERR_INVALID, "NULL batch");
     hipSetDevice(b->device);
-    return b->mixed ? solve_all<float>(b, stats) : solve_all<double>(b, stats);
+    return run_solve(b, stats);
+}
+
+// ------------------------------------------------------------ tensor field
+
+// Per-node diffusion tensor of one problem (D11, D22, D12 in natural order,
+// fenicsInterface's D11/D22/D12 vectors, src/fHSL.cpp:313-323, 387-392),
+// or constant coefficients again when all three are NULL.  Coarse levels
+// take the value of the nearest fine node; the coarsest level's inverse is
+// rebuilt from its tensor.
+extern "C++" {
+template <typename T>
+static int set_tensor_impl(eq_batch* b, int32_t prob, const double* d11, const double* d22, const double* d12)
+{
+    const int nlev = (int)b->levels.size();
+    hipStream_t s = b->stream;
+    EQ_HIP(hipStreamSynchronize(s)); // no kernel may be reading the tables being replaced
+    const bool on = d11 != nullptr;
+    if (on) {
+        for (int l = 0; l < nlev; ++l) {
+            Level& L = b->levels[l];
+            if (!L.tens) {
+                int rc = alloc_zero((T**)&L.tens, 3 * L.pool, s);
+                if (rc) return rc;
+            }
+        }
+        if (!b->tens0_d) {
+            if (b->mixed) {
+                int rc = alloc_zero(&b->tens0_d, 3 * b->levels[0].pool, s);
+                if (rc) return rc;
+            } else {
+                b->tens0_d = (double*)b->levels[0].tens;
+            }
+        }
+    }
+    // host copies level by level, natural order
+    std::vector<double> cur[3], nxt[3];
+    const LevelProb* Pf = nullptr;
+    std::vector<double> inv;
+    for (int l = 0; l < nlev; ++l) {
+        Level& L = b->levels[l];
+        int pi = -1;
+        for (int k = 0; k < (int)L.probs.size(); ++k)
+            if (L.probs[k].prob == prob) pi = k;
+        if (pi < 0) break;
+        LevelProb& P = L.probs[pi];
+        const int nw = P.nx + 1, nh = P.ny + 1;
+        if (on) {
+            if (l == 0) {
+                for (int c = 0; c < 3; ++c) cur[c].assign((c == 0 ? d11 : c == 1 ? d22 : d12),
+                                                          (c == 0 ? d11 : c == 1 ? d22 : d12) + (size_t)nw * nh);
+            } else {
+                const int fw = Pf->nx + 1;
+                for (int c = 0; c < 3; ++c) {
+                    nxt[c].resize((size_t)nw * nh);
+                    for (int I = 0; I < nh; ++I) {
+                        const int i = (int)((2LL * I * Pf->ny + P.ny) / (2LL * P.ny)); // nearest fine row
+                        for (int J = 0; J < nw; ++J) {
+                            const int j = (int)((2LL * J * Pf->nx + P.nx) / (2LL * P.nx));
+                            nxt[c][(size_t)I * nw + J] = cur[c][(size_t)i * fw + j];
+                        }
+                    }
+                    cur[c].swap(nxt[c]);
+                }
+            }
+            // upload in the level's precision (and in FP64 on level 0)
+            std::vector<T> rowsT((size_t)nw * nh);
+            for (int c = 0; c < 3; ++c) {
+                for (size_t q = 0; q < rowsT.size(); ++q) rowsT[q] = (T)cur[c][q];
+                T* dst = (T*)L.tens + (int64_t)c * L.pool + P.origin;
+                EQ_HIP(hipMemcpy2DAsync(dst, sizeof(T) * P.pitch, rowsT.data(), sizeof(T) * nw, sizeof(T) * nw, nh,
+                                        hipMemcpyHostToDevice, s));
+                EQ_HIP(hipStreamSynchronize(s)); // rowsT is reused
+                if (l == 0 && b->mixed) {
+                    double* dd = b->tens0_d + (int64_t)c * L.pool + P.origin;
+                    EQ_HIP(hipMemcpy2DAsync(dd, sizeof(double) * P.pitch, cur[c].data(), sizeof(double) * nw,
+                                            sizeof(double) * nw, nh, hipMemcpyHostToDevice, s));
+                }
+            }
+            P.aniso = 1;
+            P.tstride = L.pool;
+            P.tens_d = (l == 0) ? b->tens0_d : (b->mixed ? nullptr : (const double*)L.tens);
+            P.tens_f = b->mixed ? (const float*)L.tens : nullptr;
+        } else {
+            P.aniso = 0;
+            P.tens_d = nullptr;
+            P.tens_f = nullptr;
+        }
+        if (P.coarsest && b->solver.kind != EQ_SOLVER_JACOBI_CG && b->dense) {
+            if (!dense_inverse(P, on ? cur[0].data() : nullptr, on ? cur[1].data() : nullptr,
+                               on ? cur[2].data() : nullptr, inv))
+                return set_error(EQ_ERR_INVALID, "singular coarse operator (tensor field not positive definite?)");
+            std::vector<T> invT(inv.size());
+            for (size_t q = 0; q < inv.size(); ++q) invT[q] = (T)inv[q];
+            EQ_HIP(hipMemcpyAsync((T*)b->dense + P.dense_off, invT.data(), sizeof(T) * invT.size(),
+                                  hipMemcpyHostToDevice, s));
+            EQ_HIP(hipStreamSynchronize(s));
+        }
+        EQ_HIP(hipMemcpyAsync(L.d_probs, L.probs.data(), sizeof(LevelProb) * L.probs.size(), hipMemcpyHostToDevice,
+                              s));
+        EQ_HIP(hipStreamSynchronize(s));
+        Pf = &P;
+    }
+    return EQ_OK;
+}
+
+} // extern "C++"
+
+static int set_tensor_checked(eq_batch* b, int32_t prob, const double* D11, const double* D22, const double* D12,
+                              size_t n)
+{
+    if (!b) return set_error(EQ_ERR_INVALID, "NULL batch");
+    if (prob < 0 || prob >= b->nprob) return set_error(EQ_ERR_INVALID, "problem index out of range");
+    const bool any = D11 || D22 || D12, all = D11 && D22 && D12;
+    if (any && !all) return set_error(EQ_ERR_INVALID, "D11, D22, D12: all or none");
+    const LevelProb& P = b->levels[0].probs[prob];
+    if (all && n != (size_t)(P.nx + 1) * (P.ny + 1)) return set_error(EQ_ERR_INVALID, "length must be (nx+1)*(ny+1)");
+    hipSetDevice(b->device);
+    return b->mixed ? set_tensor_impl<float>(b, prob, D11, D22, D12) : set_tensor_impl<double>(b, prob, D11, D22, D12);
+}
+
+int eq_batch_set_tensor(eq_batch* b, int32_t prob, const double* D11, const double* D22, const double* D12,
+                        size_t n)
+{
+    int rc = set_tensor_checked(b, prob, D11, D22, D12, n);
+    if (rc != EQ_OK || !b) return rc;
+    // the kernels specialise on whether any problem carries a tensor
+    bool any = false;
+    for (const LevelProb& P : b->levels[0].probs) any = any || P.aniso;
+    if (any != b->any_tensor) {
+        b->any_tensor = any;
+        if (b->g_iter2) hipGraphExecDestroy(b->g_iter2);
+        if (b->g_init) hipGraphExecDestroy(b->g_init);
+        b->g_iter2 = nullptr;
+        b->g_init = nullptr;
+    }
+    return EQ_OK;
+}
+
+
+int eq_field_set_tensor(eq_field* f, const double* D11, const double* D22, const double* D12, size_t n)
+{
+    return eq_batch_set_tensor(f, 0, D11, D22, D12, n);
 }
 
 // Candidate V-cycle shapes (pre, post sweeps) for eq_batch_autotune.
@@ -1938,7 +2115,7 @@ int eq_batch_autotune(eq_batch* b, int32_t repeats, eq_solver_desc* chosen)
             b->last_iters = orig_last;
             eq_step_stats st{};
             const auto t0 = std::chrono::steady_clock::now();
-            const int src = b->mixed ? solve_all<float>(b, &st) : solve_all<double>(b, &st);
+            const int src = run_solve(b, &st);
             const hipError_t e = hipStreamSynchronize(s);
             if (e != hipSuccess) {
                 rc = hip_check(e, "autotune step");
@@ -1990,7 +2167,10 @@ static int apply_diag(eq_batch* b, int32_t prob, const double* x, double* y, siz
     EQ_HIP(hipMemcpy2DAsync(b->q + P.origin, sizeof(double) * P.pitch, x, sizeof(double) * nw, sizeof(double) * nw,
                             nh, hipMemcpyHostToDevice, s));
     const LevelView A = b->levels[0].all();
-    k_apply_full<<<grid_for(A.ntiles), kBlock, 0, s>>>(A, prob, b->q, b->z, mode);
+    if (b->any_tensor)
+        k_apply_full<true><<<grid_for(A.ntiles), kBlock, 0, s>>>(A, prob, b->q, b->z, mode);
+    else
+        k_apply_full<false><<<grid_for(A.ntiles), kBlock, 0, s>>>(A, prob, b->q, b->z, mode);
     EQ_HIP(hipGetLastError());
     EQ_HIP(hipMemcpy2DAsync(y, sizeof(double) * nw, b->z + P.origin, sizeof(double) * P.pitch, sizeof(double) * nw,
                             nh, hipMemcpyDeviceToHost, s));

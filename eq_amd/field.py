@@ -33,6 +33,7 @@ EXPORTED = [
     "eq_batch_stream", "eq_batch_sync", "eq_batch_upload", "eq_batch_download",
     "eq_batch_upload_dev", "eq_batch_download_dev", "eq_batch_step", "eq_batch_problem_stats",
     "eq_batch_couple", "eq_batch_couple_dev", "eq_batch_couple_traps_dev", "eq_batch_autotune",
+    "eq_batch_set_tensor", "eq_field_set_tensor",
     "eq_batch_boundary_flux",
     "eq_batch_vertex_map",
     "eq_field_create", "eq_field_destroy", "eq_field_upload", "eq_field_download",
@@ -110,6 +111,8 @@ def load_library():
     L.eq_batch_problem_stats.argtypes = [vp, ip32, dp]
     L.eq_batch_couple.argtypes = [vp, C.c_int32, vp, C.c_size_t, C.c_double, vp, vp, vp, C.c_uint32]
     L.eq_batch_autotune.argtypes = [vp, C.c_int32, P(SolverDesc)]
+    L.eq_batch_set_tensor.argtypes = [vp, C.c_int32, vp, vp, vp, C.c_size_t]
+    L.eq_field_set_tensor.argtypes = [vp, vp, vp, vp, C.c_size_t]
     L.eq_batch_couple_traps_dev.argtypes = [vp, C.c_int32, C.c_int32, P(C.c_int64), vp, C.c_double, vp, vp, vp,
                                             C.c_uint32]
     L.eq_batch_couple_dev.argtypes = [vp, C.c_int32, vp, C.c_size_t, C.c_double, vp, vp, vp,
@@ -270,6 +273,14 @@ class FieldBatch:
             self._h, trap0, off.size - 1, off.ctypes.data_as(C.POINTER(C.c_int64)), C.c_void_p(cells_ptr), npm,
             C.c_void_p(q_ptr), C.c_void_p(sampled_ptr) if sampled_ptr else None,
             C.c_void_p(counts_ptr) if counts_ptr else None, flags))
+
+    def set_tensor(self, prob: int, D11=None, D22=None, D12=None):
+        """Per-node diffusion tensor (natural order), or constant coefficients when None."""
+        if D11 is None:
+            _check(load_library().eq_batch_set_tensor(self._h, prob, None, None, None, 0))
+            return
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (D11, D22, D12)]
+        _check(load_library().eq_batch_set_tensor(self._h, prob, _ptr(a[0]), _ptr(a[1]), _ptr(a[2]), a[0].size))
 
     def autotune(self, repeats: int = 1) -> SolverDesc:
         """Pick the fastest multigrid cycle shape on the current state (which is restored)."""

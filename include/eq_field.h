@@ -142,6 +142,15 @@ int eq_batch_problem_stats(const eq_batch* b, int32_t* iterations, double* relre
  * Per cell in array order: sample every species, then deposit q/|S_c|. */
 int eq_batch_couple(eq_batch* b, int32_t trap, const eq_cell* cells, size_t nc, double npm,
                     const double* q, double* sampled, int32_t* counts, uint32_t flags);
+/* Per-node diffusion tensor of problem `prob`: D11, D22, D12 (n = (nx+1)*(ny+1) each,
+ * natural order) are fenicsInterface's D11/D22/D12 vectors (src/fHSL.cpp:313-323,
+ * 387-392), P1-interpolated into A = D*[[D11, D12], [D12, D22]] of fenics/hslD.ufl:31
+ * (AnisotropicDiffusionTensor::eval, src/Expressions.h:95-124).  All three NULL:
+ * back to constant coefficients (the reference's default 1, 1, 0).  Copies the
+ * arrays; call between steps. */
+int eq_batch_set_tensor(eq_batch* b, int32_t prob, const double* D11, const double* D22, const double* D12,
+                        size_t n);
+
 /* Autotuner (no counterpart in the reference, which has one fixed direct solve):
  * times `repeats` steps of each candidate multigrid cycle shape (pre/post sweeps
  * (1,1), (2,1), (2,2), (3,3)) from the batch's current state, restoring the fields,
@@ -188,6 +197,7 @@ int eq_field_download(eq_field* f, double* u, size_t n);
 int eq_field_step(eq_field* f, eq_step_stats* stats);
 int eq_field_vertex_map(const eq_field* f, double* coords, int32_t* dof_from_vertex);
 int eq_field_boundary_flux(eq_field* f, double* total);
+int eq_field_set_tensor(eq_field* f, const double* D11, const double* D22, const double* D12, size_t n);
 int eq_cells_couple(eq_field* f, const eq_cell* cells, size_t nc, double npm, const double* q,
                     double* sampled, int32_t* counts, uint32_t flags);
 

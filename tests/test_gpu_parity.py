@@ -40,12 +40,19 @@ def _rel(a, b):
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
 
 
-def _iso_cases():
+def _cases():
     z = np.load(GOLD)
-    return [str(n) for n in z["names"] if not np.any(z[f"{n}/tensor"][:, 2])]
+    return [str(n) for n in z["names"]]
 
 
-@pytest.mark.parametrize("name", _iso_cases())
+def _set_fixture_tensor(fb, z, name):
+    """Fixtures with a tensor field other than (1, 1, 0) get it uploaded."""
+    t = z[f"{name}/tensor"]
+    if np.any(t[:, 0] != 1.0) or np.any(t[:, 1] != 1.0) or np.any(t[:, 2] != 0.0):
+        fb.set_tensor(0, t[:, 0], t[:, 1], t[:, 2])
+
+
+@pytest.mark.parametrize("name", _cases())
 def test_operator_entries_match_fixture(eqf, oracle, name):
     """Columns of the matrix-free operator = the textbook-assembled matrix."""
     z = np.load(GOLD)
@@ -54,6 +61,7 @@ def test_operator_entries_match_fixture(eqf, oracle, name):
     A_ref = z[f"{name}/A_bc"]
     n = A_ref.shape[0]
     with eqf.FieldBatch([d], solver=eqf.solver_desc(eqf.JACOBI_CG)) as fb:
+        _set_fixture_tensor(fb, z, name)
         A = np.stack([fb.apply(0, np.eye(n)[k]) for k in range(n)], axis=1)
         assert np.abs(A - A_ref).max() <= 1e-14 * np.abs(A_ref).max()
         b = fb.rhs(0, z[f"{name}/u0"])
@@ -81,12 +89,13 @@ def test_operator_matches_oracle_csr(eqf, oracle, shape):
 
 
 @pytest.mark.parametrize("solver", SOLVERS)
-@pytest.mark.parametrize("name", _iso_cases())
+@pytest.mark.parametrize("name", _cases())
 def test_solve_matches_fixture(eqf, oracle, solver, name):
     z = np.load(GOLD)
     nx, ny, W, H, dt, D, gamma, f = z[f"{name}/params"]
     d = eqf.field_desc(int(nx), int(ny), W, H, dt, D, gamma, f, _walls_from(z[f"{name}/walls"]))
     with eqf.FieldBatch([d], solver=eqf.solver_desc(_kind(eqf, solver))) as fb:
+        _set_fixture_tensor(fb, z, name)
         fb.upload(0, z[f"{name}/u0"])
         st = fb.step()
         x = fb.download(0)
@@ -326,3 +335,43 @@ def test_autotune_restores_state_and_keeps_parity(eqf, oracle):
         for k in range(len(descs)):
             ref, _ = oracle.step(probs[k], u0[k], "lu")
             assert _rel(fb.download(k), ref) <= 1e-9, k
+
+
+@pytest.mark.parametrize("solver", SOLVERS)
+def test_tensor_field_steps_match_oracle(eqf, oracle, solver):
+    """A spatially varying SPD tensor (D12 != 0) on uneven traps with Dirichlet
+    and Robin walls: every solver against the oracle's LU with the same
+    tensor, over two steps; then the tensor is cleared and the constant-
+    coefficient step must match again."""
+    shapes = [(70, 45, 7.0, 4.5), (33, 60, 3.3, 6.0)]
+    walls = {oracle.TOP: (oracle.DIRICHLET, 0.0, 0, 0), oracle.RIGHT: (oracle.ROBIN, 0, 2.0, 0.3)}
+    descs, probs, u0, tens = [], [], [], []
+    rng = np.random.default_rng(17)
+    for (nx, ny, W, H) in shapes:
+        n = (nx + 1) * (ny + 1)
+        for D, g in ((300.0, 0.01), (80.0, 0.05)):
+            descs.append(eqf.field_desc(nx, ny, W, H, 0.1, D, g, 0.0, walls))
+            probs.append(oracle.problem(nx, ny, W, H, 0.1, D, g, 0.0, walls=walls))
+            u0.append(rng.random(n))
+            d11 = 0.5 + rng.random(n)
+            d22 = 0.5 + rng.random(n)
+            d12 = 0.8 * np.sqrt(d11 * d22) * (rng.random(n) - 0.5)
+            tens.append((d11, d22, d12))
+    with eqf.FieldBatch(descs, nspecies=2, solver=eqf.solver_desc(_kind(eqf, solver))) as fb:
+        for k in range(len(descs)):
+            fb.upload(k, u0[k])
+            fb.set_tensor(k, *tens[k])
+        ref = [u.copy() for u in u0]
+        for step in range(2):
+            st = fb.step()
+            assert st.nconverged == len(descs), step
+            for k in range(len(descs)):
+                ref[k], _ = oracle.step(probs[k], ref[k], "lu", D11=tens[k][0], D22=tens[k][1], D12=tens[k][2])
+                assert _rel(fb.download(k), ref[k]) <= 1e-9, (step, k)
+        for k in range(len(descs)):
+            fb.set_tensor(k)
+        st = fb.step()
+        assert st.nconverged == len(descs)
+        for k in range(len(descs)):
+            r2, _ = oracle.step(probs[k], ref[k], "lu")
+            assert _rel(fb.download(k), r2) <= 1e-9, k

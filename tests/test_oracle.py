@@ -202,3 +202,56 @@ def test_couple_sequential_semantics(oracle):
     # constant field is sampled exactly
     (f2,), s2, _ = O.couple(g, cells, [np.full(41 * 21, 0.3)], np.zeros((2, 1)))
     assert np.allclose(s2, 0.3, rtol=1e-15, atol=0)
+
+
+def _p1_tensor_assembly(nx, ny, W, H, dt, D, d11, d22, d12):
+    """Independent assembly of dt*int grad(v)^T (D*Dtensor) grad(u) on the
+    'right' mesh from the P1 gradients of each triangle and the exact
+    integral of a P1 tensor (the mean of its vertex values times |T|)."""
+    hx, hy = W / nx, H / ny
+    n = (nx + 1) * (ny + 1)
+    K = np.zeros((n, n))
+    idx = lambda i, j: i * (nx + 1) + j
+    area = hx * hy / 2.0
+    for i in range(ny):
+        for j in range(nx):
+            bl, br, tl, tr = idx(i, j), idx(i, j + 1), idx(i + 1, j), idx(i + 1, j + 1)
+            # lower-right (BL, BR, TR) and upper-left (BL, TL, TR) triangles
+            for verts, grads in (((bl, br, tr), ((-1 / hx, 0.0), (1 / hx, -1 / hy), (0.0, 1 / hy))),
+                                 ((bl, tl, tr), ((0.0, -1 / hy), (-1 / hx, 1 / hy), (1 / hx, 0.0)))):
+                a11 = np.mean([d11[v] for v in verts])
+                a22 = np.mean([d22[v] for v in verts])
+                a12 = np.mean([d12[v] for v in verts])
+                A = D * np.array([[a11, a12], [a12, a22]])
+                for a, ga in zip(verts, grads):
+                    for b, gb in zip(verts, grads):
+                        K[a, b] += dt * area * np.asarray(ga) @ A @ np.asarray(gb)
+    return K
+
+
+def test_tensor_stiffness_matches_gradient_assembly(oracle):
+    """The oracle's quadrature of the anisotropic cell kernel against an
+    assembly from explicit P1 gradients with the exact triangle mean of the
+    tensor; a spatially varying tensor with D12 != 0 (pins the mixed term,
+    the coefficient order and the hypotenuse coupling)."""
+    nx, ny, W, H, dt, D, gam = 6, 4, 3.0, 2.0, 0.1, 7.0, 0.02
+    rng = np.random.default_rng(3)
+    n = (nx + 1) * (ny + 1)
+    d11 = 1.0 + rng.random(n)
+    d22 = 1.0 + rng.random(n)
+    d12 = 0.4 * (rng.random(n) - 0.5)
+    p = oracle.problem(nx, ny, W, H, dt, D, gam)
+    rp, col, val = oracle.assemble(p, d11, d22, d12)
+    A = oracle.to_scipy(rp, col, val, n).toarray()
+    # the mass part from the isotropic assembly with D = 0
+    p0 = oracle.problem(nx, ny, W, H, dt, 0.0, gam)
+    rp0, col0, val0 = oracle.assemble(p0)
+    M = oracle.to_scipy(rp0, col0, val0, n).toarray()
+    K = _p1_tensor_assembly(nx, ny, W, H, dt, D, d11, d22, d12)
+    assert np.allclose(A - M, K, rtol=0, atol=1e-13 * np.abs(K).max())
+    # a constant tensor diag(a, b) splits into a*Kx + b*Ky of the 5-point stencil
+    K2 = _p1_tensor_assembly(nx, ny, W, H, dt, D, np.full(n, 2.0), np.full(n, 0.5), np.zeros(n))
+    Kx = _p1_tensor_assembly(nx, ny, W, H, dt, D, np.ones(n), np.zeros(n), np.zeros(n))
+    Ky = _p1_tensor_assembly(nx, ny, W, H, dt, D, np.zeros(n), np.ones(n), np.zeros(n))
+    assert np.allclose(K2, 2.0 * Kx + 0.5 * Ky, atol=1e-13)
+    assert abs(Kx[0, 1] + dt * D * (H / ny) / (2 * W / nx)) < 1e-14  # boundary edge: one triangle
