@@ -95,8 +95,13 @@ struct LevelProb {
     // triangle; fenics/hslD.ufl:31): arrays of the level's pool layout,
     // D11 at tens[k], D22 at tens[k + tstride], D12 at tens[k + 2 tstride]
     int32_t aniso;         // 1: the tensor arrays below define the stiffness
-    int32_t pad_;
+    int32_t fd;            // 1: 5-point finite-difference scheme (diffusionPETSc)
     double sxy;            // dt*D/2: stiffness per triangle for the mixed derivative
+    double fdm;            // FD: (1+dt*gamma)*hx*hy, the scaled identity row
+    double fkx, fky;       // FD: dt*D*hy/hx, dt*D*hx/hy (x and y couplings, interior)
+    double frob[4];        // FD: 2*dt*r*(length of the wall's node segment) per Robin wall
+    double fsrc[4];        // FD: the matching right-hand side 2*dt*r*s*(...)
+    double fsrcf;          // FD: dt*f*hx*hy
     const double* tens_d;  // FP64 copy (level 0)
     const float* tens_f;   // FP32 copy (levels smoothed in FP32)
     int64_t tstride;
@@ -114,11 +119,46 @@ EQ_HD inline bool is_dirichlet(const LevelProb& P, int i, int j)
 EQ_HD inline double dirichlet_value(const LevelProb& P, int i, int j)
 {
     double v = 0.0;
+    if (P.fd) {
+        // diffusionPETSc::ApplyBoundaryConditions sets top/bottom first, then
+        // right/left (diffuclass.cpp:221-290): left/right win at corners
+        if ((P.dmask & 4) && i == P.ny) v = P.g[kTop];
+        if ((P.dmask & 8) && i == 0) v = P.g[kBottom];
+        if ((P.dmask & 2) && j == P.nx) v = P.g[kRight];
+        if ((P.dmask & 1) && j == 0) v = P.g[kLeft];
+        return v;
+    }
     if ((P.dmask & 1) && j == 0) v = P.g[kLeft];
     if ((P.dmask & 2) && j == P.nx) v = P.g[kRight];
     if ((P.dmask & 4) && i == P.ny) v = P.g[kTop];
     if ((P.dmask & 8) && i == 0) v = P.g[kBottom];
     return v;
+}
+
+// FD row of a free node (diffusionPETSc's ghost-reflection rows times
+// hx*hy*wx*wy, wx = 1/2 on the left/right walls, wy = 1/2 on top/bottom:
+// the scaled system is symmetric and has the same solution)
+template <typename T>
+EQ_HD inline Row<T> fd_row(const LevelProb& P, int i, int j)
+{
+    const T one = (T)1, half = (T)0.5, zero = (T)0;
+    const bool L = j == 0, R = j == P.nx, B = i == 0, Tp = i == P.ny;
+    const T wx = (L || R) ? half : one, wy = (B || Tp) ? half : one;
+    Row<T> r;
+    double m = P.fdm;
+    if (L) m += P.frob[kLeft];
+    if (R) m += P.frob[kRight];
+    if (B) m += P.frob[kBottom];
+    if (Tp) m += P.frob[kTop];
+    r.mc = wx * wy * (T)m;
+    r.mw = r.me = r.ms = r.mn = r.msw = r.mne = zero;
+    const T kx = wy * (T)P.fkx, ky = wx * (T)P.fky;
+    r.kw = L ? zero : kx;
+    r.ke = R ? zero : kx;
+    r.ks = B ? zero : ky;
+    r.kn = Tp ? zero : ky;
+    r.ksw = r.kne = zero;
+    return r;
 }
 
 
@@ -137,6 +177,7 @@ template <> EQ_HD inline float interior_dinv<float>(const LevelProb& P) { return
 template <typename T>
 EQ_HD inline Row<T> row_coefs_at(const LevelProb& P, int i, int j)
 {
+    if (P.fd) return fd_row<T>(P, i, j);
     const T one = (T)1, zero = (T)0, two = (T)2;
     const T mh = (T)P.mh, mo = (T)P.mo;
     const T sw = (i > 0 && j > 0) ? one : zero;
@@ -326,6 +367,17 @@ EQ_HD inline T row_apply(const Row<T>& r, T vc, T vw, T ve, T vs, T vn, T vsw, T
 EQ_HD inline double rhs_row(const LevelProb& P, int i, int j, double uc, double uw, double ue, double us,
                             double un, double usw, double une)
 {
+    if (P.fd) {
+        // u_old plus the Robin ghost sources, scaled as the FD rows
+        const bool L = j == 0, R = j == P.nx, B = i == 0, Tp = i == P.ny;
+        const double wx = (L || R) ? 0.5 : 1.0, wy = (B || Tp) ? 0.5 : 1.0;
+        double b = P.mh0 * uc + P.fsrcf;
+        if (L) b += P.fsrc[kLeft];
+        if (R) b += P.fsrc[kRight];
+        if (B) b += P.fsrc[kBottom];
+        if (Tp) b += P.fsrc[kTop];
+        return wx * wy * b;
+    }
     const double sw = (i > 0 && j > 0) ? 1.0 : 0.0;
     const double se = (i > 0 && j < P.nx) ? 1.0 : 0.0;
     const double nw = (i < P.ny && j > 0) ? 1.0 : 0.0;

@@ -1119,6 +1119,20 @@ static void fill_level_prob(LevelProb& P, const eq_field_desc& d, int nx, int ny
     P.sa = d.dt * d.D * hy / (2.0 * hx);
     P.sb = d.dt * d.D * hx / (2.0 * hy);
     P.sxy = d.dt * d.D / 2.0;
+    if (d.scheme == EQ_SCHEME_FD) {
+        // diffusionPETSc rows times hx*hy (the wall/corner halving is per node)
+        P.fd = 1;
+        P.fdm = (1.0 + d.dt * d.gamma) * hx * hy;
+        P.fkx = d.dt * d.D * hy / hx;
+        P.fky = d.dt * d.D * hx / hy;
+        const double seg[4] = {hy, hy, hx, hx}; // the wall-normal ghost term times hx*hy
+        for (int w = 0; w < 4; ++w)
+            if (d.wall[w].kind == EQ_BC_ROBIN) {
+                P.frob[w] = 2.0 * d.dt * d.wall[w].r * seg[w];
+                P.fsrc[w] = 2.0 * d.dt * d.wall[w].r * d.wall[w].s * seg[w];
+            }
+        P.fsrcf = d.dt * d.f * hx * hy;
+    }
     const double elen[4] = {hy, hy, hx, hx};
     P.dmask = 0;
     for (int w = 0; w < 4; ++w) {
@@ -1130,7 +1144,7 @@ static void fill_level_prob(LevelProb& P, const eq_field_desc& d, int nx, int ny
         P.g[w] = (d.wall[w].kind == EQ_BC_DIRICHLET) ? d.wall[w].value : 0.0;
     }
     if (level0) {
-        P.mh0 = area6;
+        P.mh0 = P.fd ? hx * hy : area6; // FD: the scaled identity
         P.mo0 = hx * hy / 24.0;
         P.src = d.dt * d.f * hx * hy / 6.0;
     }
@@ -1233,6 +1247,7 @@ static int validate(const eq_field_desc& d, std::string& why)
         return 1;
     }
     if (!(d.width > 0) || !(d.height > 0)) { why = "width and height must be > 0"; return 1; }
+    if (d.scheme != EQ_SCHEME_P1 && d.scheme != EQ_SCHEME_FD) { why = "unknown scheme"; return 1; }
     if (!(d.dt > 0) || !(d.D >= 0) || !(d.gamma >= 0) || !std::isfinite(d.f)) {
         why = "need dt > 0, D >= 0, gamma >= 0, finite f";
         return 1;
@@ -2032,6 +2047,7 @@ static int set_tensor_checked(eq_batch* b, int32_t prob, const double* D11, cons
     if (any && !all) return set_error(EQ_ERR_INVALID, "D11, D22, D12: all or none");
     const LevelProb& P = b->levels[0].probs[prob];
     if (all && n != (size_t)(P.nx + 1) * (P.ny + 1)) return set_error(EQ_ERR_INVALID, "length must be (nx+1)*(ny+1)");
+    if (all && P.fd) return set_error(EQ_ERR_INVALID, "the finite-difference scheme has no tensor field");
     hipSetDevice(b->device);
     return b->mixed ? set_tensor_impl<float>(b, prob, D11, D22, D12) : set_tensor_impl<double>(b, prob, D11, D22, D12);
 }

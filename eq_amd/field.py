@@ -17,6 +17,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libeq_field.so")
 
+SCHEME_P1, SCHEME_FD = 0, 1
 NEUMANN, DIRICHLET, ROBIN = 0, 1, 2
 LEFT, RIGHT, TOP, BOTTOM = 0, 1, 2, 3
 JACOBI_CG, GMG_CG, MIXED_GMG_CG = 0, 1, 2
@@ -50,7 +51,8 @@ class Bc(C.Structure):
 class FieldDesc(C.Structure):
     _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("width", C.c_double),
                 ("height", C.c_double), ("dt", C.c_double), ("D", C.c_double),
-                ("gamma", C.c_double), ("f", C.c_double), ("wall", Bc * 4)]
+                ("gamma", C.c_double), ("f", C.c_double), ("wall", Bc * 4), ("scheme", C.c_int32),
+                ("pad_", C.c_int32)]
 
 
 class SolverDesc(C.Structure):
@@ -140,11 +142,13 @@ def _check(rc: int):
         raise EqError(rc, load_library().eq_last_error().decode())
 
 
-def field_desc(nx, ny, width, height, dt, D, gamma=0.0, f=0.0, walls=None) -> FieldDesc:
-    """walls: {wall: (kind, value, r, s)}; missing walls are Neumann."""
+def field_desc(nx, ny, width, height, dt, D, gamma=0.0, f=0.0, walls=None, scheme=0) -> FieldDesc:
+    """walls: {wall: (kind, value, r, s)}; missing walls are Neumann.
+    scheme: SCHEME_P1 (fenicsInterface's P1 form) or SCHEME_FD (diffusionPETSc)."""
     d = FieldDesc()
     d.nx, d.ny, d.width, d.height = nx, ny, width, height
     d.dt, d.D, d.gamma, d.f = dt, D, gamma, f
+    d.scheme = scheme
     for w in range(4):
         kind, value, r, s = (walls or {}).get(w, (NEUMANN, 0.0, 0.0, 0.0))
         d.wall[w].kind, d.wall[w].value, d.wall[w].r, d.wall[w].s = kind, value, r, s

@@ -123,11 +123,44 @@ public:
         }
         host_[s] = v;
     }
+    // Per-species tensor vectors (D11, D22, D12 of the fenicsInterface views,
+    // src/fHSL.cpp:313-323): sent to the device before a solve whenever they
+    // differ from what the device holds; all (1, 1, 0) means constant
+    // coefficients, the reference's default.
+    void attachTensor(int s, const std::shared_ptr<std::vector<double>>* d11,
+                      const std::shared_ptr<std::vector<double>>* d22, const std::shared_ptr<std::vector<double>>* d12)
+    {
+        if ((int)tens_.size() <= s) tens_.resize(s + 1);
+        tens_[s].src[0] = d11;
+        tens_[s].src[1] = d22;
+        tens_[s].src[2] = d12;
+    }
+    void syncTensors()
+    {
+        for (size_t k = 0; k < tens_.size(); ++k) {
+            Tens& t = tens_[k];
+            if (!t.src[0] || !*t.src[0] || !*t.src[1] || !*t.src[2]) continue;
+            const std::vector<double>& a = **t.src[0];
+            const std::vector<double>& b = **t.src[1];
+            const std::vector<double>& c = **t.src[2];
+            if (a == t.sent[0] && b == t.sent[1] && c == t.sent[2]) continue;
+            bool iso = true;
+            for (size_t q = 0; q < a.size() && iso; ++q) iso = a[q] == 1.0 && b[q] == 1.0 && c[q] == 0.0;
+            if (iso)
+                check(eq_batch_set_tensor(batch_.get(), (int)k, nullptr, nullptr, nullptr, 0), "set tensor");
+            else
+                check(eq_batch_set_tensor(batch_.get(), (int)k, a.data(), b.data(), c.data(), a.size()), "set tensor");
+            t.sent[0] = a;
+            t.sent[1] = b;
+            t.sent[2] = c;
+        }
+    }
     void stepSpecies(int s)
     {
         if (!solved_[s]) {
             for (size_t k = 0; k < host_.size(); ++k)
                 if (host_[k]) upload((int)k, *host_[k]);
+            syncTensors();
             step();
             for (size_t k = 0; k < host_.size(); ++k)
                 if (host_[k]) {
@@ -140,6 +173,11 @@ public:
     const eq_step_stats& lastStats() const { return last_; }
 
 private:
+    struct Tens {
+        const std::shared_ptr<std::vector<double>>* src[3] = {nullptr, nullptr, nullptr};
+        std::vector<double> sent[3]; // last values sent (empty: constant coefficients)
+    };
+    std::vector<Tens> tens_;
     std::vector<std::vector<double>*> host_;
     std::vector<char> solved_;
     eq_step_stats last_{};
@@ -182,6 +220,7 @@ public:
         D11 = std::make_shared<std::vector<double>>(n, 1.0);
         D22 = std::make_shared<std::vector<double>>(n, 1.0);
         D12 = std::make_shared<std::vector<double>>(n, 0.0);
+        field_->attachTensor(s_, &D11, &D22, &D12);
     }
 
     // u0 <- solution_vector; solve; solution_vector <- u; boundary flux

@@ -58,11 +58,23 @@ typedef struct {
  * (RectangleMesh (0,0)-(width,height), nx x ny cells, "right" diagonal,
  * src/fHSL.cpp:164-173) and the hslD form (fenics/hslD.ufl:31-49) with the
  * reaction term added:  (M + dt*D*K + dt*gamma*M) u+ = M u + dt*f*int(phi). */
+typedef enum {
+    EQ_SCHEME_P1 = 0, /* P1 finite elements of fenicsInterface (fenics/hslD.ufl)            */
+    EQ_SCHEME_FD = 1  /* 5-point finite differences of diffusionPETSc (diffuclass.cpp):
+                       * interior (1 + 2Fx + 2Fy) u - Fx (u_W + u_E) - Fy (u_S + u_N) = u_old,
+                       * Fx = D dt/hx^2; walls by ghost reflection (MyMatMult,
+                       * diffuclass.cpp:787-865; ApplyBoundaryConditions 191-339), Robin
+                       * walls as D du/dn = -r (u - s); solved in the equivalent symmetric
+                       * form (rows scaled by hx*hy/2 on walls, hx*hy/4 in corners) */
+} eq_scheme;
+
 typedef struct {
     int32_t nx, ny;
     double width, height; /* microns */
     double dt, D, gamma, f;
     eq_bc wall[4];
+    int32_t scheme;       /* eq_scheme (0: the P1 form above)  */
+    int32_t pad_;
 } eq_field_desc;
 
 typedef enum {

@@ -331,6 +331,88 @@ int orc_solve_banded(const orc_problem* p, const int64_t* rowptr, const int32_t*
     return 0;
 }
 
+int orc_fd_step(const orc_problem* p, double* u)
+{
+    const int64_t nx = p->nx, ny = p->ny, nw = nx + 1, n = orc_num_nodes(p);
+    const double hx = p->width / nx, hy = p->height / ny;
+    const double Fx = p->D * p->dt / (hx * hx), Fy = p->D * p->dt / (hy * hy);
+    /* per wall: Nc, Dc, BV of the reference's Robin form Dc u + Nc du/dn = BV */
+    double Nc[4], Dc[4], BV[4];
+    for (int w = 0; w < 4; ++w) {
+        const orc_bc& bc = p->wall[w];
+        if (bc.kind == ORC_DIRICHLET) {
+            Nc[w] = 0.0; Dc[w] = 1.0; BV[w] = bc.value;
+        } else if (bc.kind == ORC_ROBIN) {
+            Nc[w] = 1.0; Dc[w] = bc.r / p->D; BV[w] = bc.r * bc.s / p->D;
+        } else {
+            Nc[w] = 1.0; Dc[w] = 0.0; BV[w] = 0.0;
+        }
+    }
+    std::vector<int64_t> rowptr(n + 1, 0);
+    std::vector<int32_t> col;
+    std::vector<double> val, b(n);
+    col.reserve(5 * n);
+    val.reserve(5 * n);
+    for (int64_t i = 0; i <= ny; ++i)
+        for (int64_t j = 0; j <= nx; ++j) {
+            const int64_t k = j + i * nw;
+            const bool L = j == 0, R = j == nx, B = i == 0, T = i == ny;
+            const bool dirx = (L && Nc[ORC_LEFT] == 0.0) || (R && Nc[ORC_RIGHT] == 0.0);
+            const bool diry = (B && Nc[ORC_BOTTOM] == 0.0) || (T && Nc[ORC_TOP] == 0.0);
+            std::vector<std::pair<int32_t, double>> row;
+            if (dirx || diry) {
+                row.push_back({(int32_t)k, 1.0});
+                double g = 0.0;
+                if (T && Nc[ORC_TOP] == 0.0) g = BV[ORC_TOP];
+                if (B && Nc[ORC_BOTTOM] == 0.0) g = BV[ORC_BOTTOM];
+                if (R && Nc[ORC_RIGHT] == 0.0) g = BV[ORC_RIGHT];
+                if (L && Nc[ORC_LEFT] == 0.0) g = BV[ORC_LEFT];
+                b[k] = g;
+            } else {
+                double diag = 1.0 + 2.0 * Fx + 2.0 * Fy;
+                double rhs = u[k];
+                /* x direction: a wall reflects the ghost onto the inward neighbour */
+                if (L) {
+                    row.push_back({(int32_t)(k + 1), -2.0 * Fx});
+                    diag += 2.0 * hx * Fx * Dc[ORC_LEFT] / Nc[ORC_LEFT];
+                    rhs += 2.0 * Fx * hx * BV[ORC_LEFT] / Nc[ORC_LEFT];
+                } else if (R) {
+                    row.push_back({(int32_t)(k - 1), -2.0 * Fx});
+                    diag += 2.0 * hx * Fx * Dc[ORC_RIGHT] / Nc[ORC_RIGHT];
+                    rhs += 2.0 * Fx * hx * BV[ORC_RIGHT] / Nc[ORC_RIGHT];
+                } else {
+                    row.push_back({(int32_t)(k - 1), -Fx});
+                    row.push_back({(int32_t)(k + 1), -Fx});
+                }
+                if (B) {
+                    row.push_back({(int32_t)(k + nw), -2.0 * Fy});
+                    diag += 2.0 * hy * Fy * Dc[ORC_BOTTOM] / Nc[ORC_BOTTOM];
+                    rhs += 2.0 * Fy * hy * BV[ORC_BOTTOM] / Nc[ORC_BOTTOM];
+                } else if (T) {
+                    row.push_back({(int32_t)(k - nw), -2.0 * Fy});
+                    diag += 2.0 * hy * Fy * Dc[ORC_TOP] / Nc[ORC_TOP];
+                    rhs += 2.0 * Fy * hy * BV[ORC_TOP] / Nc[ORC_TOP];
+                } else {
+                    row.push_back({(int32_t)(k - nw), -Fy});
+                    row.push_back({(int32_t)(k + nw), -Fy});
+                }
+                row.push_back({(int32_t)k, diag});
+                b[k] = rhs;
+            }
+            std::sort(row.begin(), row.end());
+            for (auto& e : row) {
+                col.push_back(e.first);
+                val.push_back(e.second);
+            }
+            rowptr[k + 1] = (int64_t)col.size();
+        }
+    std::vector<double> x(n);
+    const int rc = orc_solve_banded(p, rowptr.data(), col.data(), val.data(), b.data(), x.data());
+    if (rc) return rc;
+    std::copy(x.begin(), x.end(), u);
+    return 0;
+}
+
 int orc_solve_pcg(const orc_problem* p, const int64_t* rowptr, const int32_t* col, const double* val,
                   const double* b, double* x, double rtol, int32_t maxit, orc_solve_info* info)
 {

@@ -129,6 +129,25 @@ void orc_cell_window(const orc_grid* g, const orc_cell* c, int32_t* win4);
 void orc_couple(const orc_grid* g, const orc_cell* cells, int64_t nc, int32_t nspecies,
                 double* const* fields, const double* q, double* sampled, int32_t* counts);
 
+/* ---- finite-difference alternative: diffusionPETSc ----
+ * One backward-Euler step of the 5-point scheme of diffusionPETSc
+ * (diffuclass.cpp): interior rows (1 + 2Fx + 2Fy) u - Fx (u_W + u_E) -
+ * Fy (u_S + u_N) = u_old with Fx = D dt/hx^2, Fy = D dt/hy^2 (MyMatMult,
+ * 787-865; the reference has hx = hy = h, 361); wall rows by ghost
+ * reflection: the inward coupling doubles and a Robin wall adds
+ * 2 h F Dc/Nc to the diagonal and 2 F h BV/Nc to the right-hand side
+ * (ApplyBoundaryConditions, 191-339); Dirichlet wall rows are identity with
+ * the wall value, the left/right value winning at corners (the right/left
+ * block of ApplyBoundaryConditions runs after the top/bottom one).  Wall
+ * kinds map as Dirichlet: Nc = 0; Neumann: Nc = 1, Dc = BV = 0; Robin
+ * (flux D du/dn = -r (u - s)): Nc = 1, Dc = r/D, BV = r s/D.  A corner where
+ * a Neumann/Robin wall meets a Dirichlet wall is a Dirichlet node (the
+ * reference's matrix row there reads past the grid).  Solved exactly (banded
+ * LU without pivoting: the rows are diagonally dominant); the reference runs
+ * KSPFBCGSR without preconditioner at PETSc's default tolerances (389-391).
+ * u is updated in place; returns 0 on success. */
+int orc_fd_step(const orc_problem* p, double* u);
+
 /* boundary-flux functional of stepDiffusion (fenics/boundary.ufl:9-12):
  * -int_{boundary} grad(u).n ds over all walls, P1 gradient per boundary triangle */
 double orc_boundary_flux(const orc_problem* p, const double* u);

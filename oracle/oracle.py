@@ -80,6 +80,8 @@ def lib():
         L.orc_solve_pcg.restype = C.c_int
         L.orc_solve_pcg.argtypes = [P(Problem), ip64, ip32, dp, dp, dp, C.c_double, C.c_int32,
                                     P(SolveInfo)]
+        L.orc_fd_step.restype = C.c_int
+        L.orc_fd_step.argtypes = [P(Problem), dp]
         L.orc_step.restype = C.c_int
         L.orc_step.argtypes = [P(Problem), dp, dp, dp, dp, C.c_int32, C.c_double, C.c_int32, P(SolveInfo)]
         L.orc_cell_coverage.restype = C.c_int64
@@ -189,6 +191,15 @@ def step(p: Problem, u, method="lu", rtol=1e-14, D11=None, D22=None, D12=None, m
     # PCG may stop at the rounding floor of the CSR residual above rtol;
     # callers read info.relres
     return u, info
+
+
+def fd_step(p: Problem, u):
+    """One step of the diffusionPETSc finite-difference scheme (exact solve) on a copy of u."""
+    u = np.array(u, dtype=np.float64, copy=True)
+    rc = lib().orc_fd_step(C.byref(p), _dp(u))
+    if rc:
+        raise RuntimeError(f"oracle FD step failed rc={rc}")
+    return u
 
 
 def grid(nodes_w, nodes_h, npm, trap_w, trap_h) -> Grid:

@@ -375,3 +375,33 @@ def test_tensor_field_steps_match_oracle(eqf, oracle, solver):
         for k in range(len(descs)):
             r2, _ = oracle.step(probs[k], ref[k], "lu")
             assert _rel(fb.download(k), r2) <= 1e-9, k
+
+
+@pytest.mark.parametrize("solver", SOLVERS)
+def test_fd_scheme_matches_oracle(eqf, oracle, solver):
+    """diffusionPETSc's 5-point scheme (ghost-reflection walls, Robin, Dirichlet
+    corners) on uneven traps in one batch, three steps against the oracle's
+    exact solve of the reference rows."""
+    cases = [
+        ((48, 30, 4.8, 3.0), {oracle.TOP: (oracle.DIRICHLET, 0.0, 0, 0)}),
+        ((65, 40, 6.5, 2.0), {oracle.LEFT: (oracle.DIRICHLET, 1.0, 0, 0), oracle.TOP: (oracle.DIRICHLET, 0.5, 0, 0),
+                              oracle.RIGHT: (oracle.ROBIN, 0, 2.0, 0.3)}),
+        ((20, 90, 2.0, 9.0), {}),  # all Neumann
+    ]
+    descs, probs, u0 = [], [], []
+    rng = np.random.default_rng(23)
+    for (nx, ny, W, H), walls in cases:
+        for D in (300.0, 60.0):
+            descs.append(eqf.field_desc(nx, ny, W, H, 0.1, D, 0.0, 0.0, walls, scheme=eqf.SCHEME_FD))
+            probs.append(oracle.problem(nx, ny, W, H, 0.1, D, 0.0, 0.0, walls=walls))
+            u0.append(rng.random((nx + 1) * (ny + 1)))
+    with eqf.FieldBatch(descs, nspecies=2, solver=eqf.solver_desc(_kind(eqf, solver))) as fb:
+        ref = [u.copy() for u in u0]
+        for k in range(len(descs)):
+            fb.upload(k, u0[k])
+        for step in range(3):
+            st = fb.step()
+            assert st.nconverged == len(descs), step
+            for k in range(len(descs)):
+                ref[k] = oracle.fd_step(probs[k], ref[k])
+                assert _rel(fb.download(k), ref[k]) <= 1e-9, (step, k)

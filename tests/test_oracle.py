@@ -255,3 +255,34 @@ def test_tensor_stiffness_matches_gradient_assembly(oracle):
     Ky = _p1_tensor_assembly(nx, ny, W, H, dt, D, np.zeros(n), np.ones(n), np.zeros(n))
     assert np.allclose(K2, 2.0 * Kx + 0.5 * Ky, atol=1e-13)
     assert abs(Kx[0, 1] + dt * D * (H / ny) / (2 * W / nx)) < 1e-14  # boundary edge: one triangle
+
+
+def test_fd_scheme_closed_forms(oracle):
+    """The diffusionPETSc scheme: with Neumann walls the discrete cosine
+    modes cos(pi k j/nx) are eigenvectors of the ghost-reflection operator,
+    so one step divides a mode by exactly 1 + 2F(1 - cos(pi k/nx)); the
+    weighted sum (weights 1/2 on walls, 1/4 in corners) is conserved; a
+    Dirichlet wall holds its value (left/right win at corners)."""
+    nx, ny, W, H, dt, D = 24, 10, 2.4, 1.0, 0.1, 0.3
+    p = oracle.problem(nx, ny, W, H, dt, D)
+    n = (nx + 1) * (ny + 1)
+    j = np.tile(np.arange(nx + 1), ny + 1)
+    i = np.repeat(np.arange(ny + 1), nx + 1)
+    Fx, Fy = D * dt / (W / nx) ** 2, D * dt / (H / ny) ** 2
+    for k in (1, 3):
+        u0 = np.cos(np.pi * k * j / nx)
+        u1 = oracle.fd_step(p, u0)
+        assert np.allclose(u1, u0 / (1.0 + 2.0 * Fx * (1.0 - np.cos(np.pi * k / nx))), atol=1e-14)
+        v0 = np.cos(np.pi * k * i / ny)
+        v1 = oracle.fd_step(p, v0)
+        assert np.allclose(v1, v0 / (1.0 + 2.0 * Fy * (1.0 - np.cos(np.pi * k / ny))), atol=1e-14)
+    w = np.where((j == 0) | (j == nx), 0.5, 1.0) * np.where((i == 0) | (i == ny), 0.5, 1.0)
+    u0 = np.random.default_rng(2).random(n)
+    u1 = oracle.fd_step(p, u0)
+    assert abs(w @ u1 - w @ u0) < 1e-12 * (w @ u0)
+    # Dirichlet top 0.5, left 2.0: the wall nodes take their values
+    walls = {oracle.TOP: (oracle.DIRICHLET, 0.5, 0, 0), oracle.LEFT: (oracle.DIRICHLET, 2.0, 0, 0)}
+    p2 = oracle.problem(nx, ny, W, H, dt, D, walls=walls)
+    u2 = oracle.fd_step(p2, u0)
+    assert np.all(u2[(j == 0)] == 2.0)
+    assert np.all(u2[(i == ny) & (j > 0)] == 0.5)
