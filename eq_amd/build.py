@@ -15,7 +15,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libeq_field.so")
-SOURCES = ["eq_solver.hip", "eq_cells.hip"]
+SOURCES = ["eq_solver.hip", "eq_cells.hip", "eq_channels.hip"]
 HEADERS = ["eq_common.h", "eq_internal.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

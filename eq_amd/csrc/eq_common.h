@@ -80,6 +80,7 @@ struct LevelProb {
     double src;            // dt*f*hx*hy/6 per adjacent triangle
     double robs[4];        // dt*r*s*|e|/2 per Robin wall edge endpoint
     double g[4];           // Dirichlet value per wall
+    const double* gw[4];   // per-node Dirichlet values along a wall (level 0; NULL: g)
     double hx, hy;         // spacing (boundary flux)
     double Ddt;            // dt*D (boundary flux scaling)
     // the row every interior node shares (0 < i < ny, 0 < j < nx), in both
@@ -107,10 +108,20 @@ struct LevelProb {
     int64_t tstride;
 };
 
+// (i,j) must be a node of the grid
 EQ_HD inline bool is_dirichlet(const LevelProb& P, int i, int j)
 {
     return ((P.dmask & 1) && j == 0) || ((P.dmask & 2) && j == P.nx) ||
            ((P.dmask & 4) && i == P.ny) || ((P.dmask & 8) && i == 0);
+}
+
+// The same for a stencil neighbour that may lie one node outside the grid:
+// those are never Dirichlet (dirichlet_value indexes per-node wall arrays by
+// the position along the wall, which must lie on the wall).
+EQ_HD inline bool is_dirichlet_nb(const LevelProb& P, int i, int j)
+{
+    if (i < 0 || j < 0 || i > P.ny || j > P.nx) return false;
+    return is_dirichlet(P, i, j);
 }
 
 // Dirichlet value of a node; walls are applied left, right, top, bottom as
@@ -122,16 +133,16 @@ EQ_HD inline double dirichlet_value(const LevelProb& P, int i, int j)
     if (P.fd) {
         // diffusionPETSc::ApplyBoundaryConditions sets top/bottom first, then
         // right/left (diffuclass.cpp:221-290): left/right win at corners
-        if ((P.dmask & 4) && i == P.ny) v = P.g[kTop];
-        if ((P.dmask & 8) && i == 0) v = P.g[kBottom];
-        if ((P.dmask & 2) && j == P.nx) v = P.g[kRight];
-        if ((P.dmask & 1) && j == 0) v = P.g[kLeft];
+        if ((P.dmask & 4) && i == P.ny) v = P.gw[kTop] ? P.gw[kTop][j] : P.g[kTop];
+        if ((P.dmask & 8) && i == 0) v = P.gw[kBottom] ? P.gw[kBottom][j] : P.g[kBottom];
+        if ((P.dmask & 2) && j == P.nx) v = P.gw[kRight] ? P.gw[kRight][i] : P.g[kRight];
+        if ((P.dmask & 1) && j == 0) v = P.gw[kLeft] ? P.gw[kLeft][i] : P.g[kLeft];
         return v;
     }
-    if ((P.dmask & 1) && j == 0) v = P.g[kLeft];
-    if ((P.dmask & 2) && j == P.nx) v = P.g[kRight];
-    if ((P.dmask & 4) && i == P.ny) v = P.g[kTop];
-    if ((P.dmask & 8) && i == 0) v = P.g[kBottom];
+    if ((P.dmask & 1) && j == 0) v = P.gw[kLeft] ? P.gw[kLeft][i] : P.g[kLeft];
+    if ((P.dmask & 2) && j == P.nx) v = P.gw[kRight] ? P.gw[kRight][i] : P.g[kRight];
+    if ((P.dmask & 4) && i == P.ny) v = P.gw[kTop] ? P.gw[kTop][j] : P.g[kTop];
+    if ((P.dmask & 8) && i == 0) v = P.gw[kBottom] ? P.gw[kBottom][j] : P.g[kBottom];
     return v;
 }
 

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <string>
 #include <vector>
 
@@ -115,6 +116,12 @@ struct eq_batch {
     double* flux_out = nullptr;      // device scalar
     double* h_flux = nullptr;        // pinned
     eqk::ProfileSlot prof[EQ_KERNEL_COUNT_];
+    // per-node wall values and flow channels (allocated on first use)
+    std::vector<std::array<double*, 4>> wall_vals;        // per problem and wall
+    std::vector<std::array<eq_channel_desc, 2>> chan_desc; // per problem: top, bottom (substeps 0: none)
+    void* d_chans = nullptr;                              // device channel table
+    int32_t chan_count = 0;
+    std::vector<double*> chan_scratch;
 };
 
 namespace eqk {
@@ -123,6 +130,9 @@ int hip_check(hipError_t e, const char* what);
 // event bracket for optional per-launch timing
 void prof_begin(eq_batch* b, int kernel);
 void prof_end(eq_batch* b, int kernel, double bytes);
+// flow channels: update after a solve; release on destroy
+int channels_after_step(eq_batch* b);
+void free_channels(eq_batch* b);
 } // namespace eqk
 
 #define EQ_HIP(call)                                                  \

@@ -218,7 +218,7 @@ __global__ __launch_bounds__(kBlock) void k_init(LevelView L, double* x, const d
             const int64_t k = n.idx;
             // Dirichlet neighbours take their wall value (x is being set to g there)
             auto val = [&](int ii, int jj, int64_t kk) -> double {
-                return is_dirichlet(P, ii, jj) ? dirichlet_value(P, ii, jj) : src[kk];
+                return is_dirichlet_nb(P, ii, jj) ? dirichlet_value(P, ii, jj) : src[kk];
             };
             const double xc = src[k];
             const double xw = val(i, j - 1, k - 1), xe = val(i, j + 1, k + 1);
@@ -1840,6 +1840,7 @@ void eq_batch_destroy(eq_batch* b)
     hipFree(b->flux_out);
     hipFree(b->d_field_ptrs);
     hipFree(b->cell_scratch);
+    free_channels(b);
     hipFree(b->d_cell_traps);
     hipFree(b->d_cell_off);
     if (b->h_cell_off) hipHostFree(b->h_cell_off);
@@ -1930,7 +1931,9 @@ int eq_batch_step(eq_batch* b, eq_step_stats* stats)
 {
     if (!b) return set_error(EQ_ERR_INVALID, "NULL batch");
     hipSetDevice(b->device);
-    return run_solve(b, stats);
+    const int rc = run_solve(b, stats);
+    if (rc != EQ_OK) return rc;
+    return channels_after_step(b); // flow channels take the new trap fields' flux
 }
 
 // ------------------------------------------------------------ tensor field

@@ -34,7 +34,8 @@ EXPORTED = [
     "eq_batch_stream", "eq_batch_sync", "eq_batch_upload", "eq_batch_download",
     "eq_batch_upload_dev", "eq_batch_download_dev", "eq_batch_step", "eq_batch_problem_stats",
     "eq_batch_couple", "eq_batch_couple_dev", "eq_batch_couple_traps_dev", "eq_batch_autotune",
-    "eq_batch_set_tensor", "eq_field_set_tensor",
+    "eq_batch_set_tensor", "eq_field_set_tensor", "eq_batch_set_wall_values", "eq_batch_set_channels",
+    "eq_batch_channel_values",
     "eq_batch_boundary_flux",
     "eq_batch_vertex_map",
     "eq_field_create", "eq_field_destroy", "eq_field_upload", "eq_field_download",
@@ -53,6 +54,19 @@ class FieldDesc(C.Structure):
                 ("height", C.c_double), ("dt", C.c_double), ("D", C.c_double),
                 ("gamma", C.c_double), ("f", C.c_double), ("wall", Bc * 4), ("scheme", C.c_int32),
                 ("pad_", C.c_int32)]
+
+
+class ChannelDesc(C.Structure):
+    _fields_ = [("v", C.c_double), ("r_left", C.c_double), ("s_left", C.c_double),
+                ("r_right", C.c_double), ("s_right", C.c_double), ("well_scaling", C.c_double),
+                ("substeps", C.c_int32), ("pad_", C.c_int32)]
+
+
+def channel_desc(v, r_left, s_left, r_right, s_right, well_scaling, substeps=1) -> ChannelDesc:
+    c = ChannelDesc()
+    c.v, c.r_left, c.s_left, c.r_right, c.s_right = v, r_left, s_left, r_right, s_right
+    c.well_scaling, c.substeps = well_scaling, substeps
+    return c
 
 
 class SolverDesc(C.Structure):
@@ -114,6 +128,9 @@ def load_library():
     L.eq_batch_couple.argtypes = [vp, C.c_int32, vp, C.c_size_t, C.c_double, vp, vp, vp, C.c_uint32]
     L.eq_batch_autotune.argtypes = [vp, C.c_int32, P(SolverDesc)]
     L.eq_batch_set_tensor.argtypes = [vp, C.c_int32, vp, vp, vp, C.c_size_t]
+    L.eq_batch_set_wall_values.argtypes = [vp, C.c_int32, C.c_int32, vp, C.c_size_t]
+    L.eq_batch_set_channels.argtypes = [vp, C.c_int32, P(ChannelDesc), P(ChannelDesc)]
+    L.eq_batch_channel_values.argtypes = [vp, C.c_int32, vp, vp]
     L.eq_field_set_tensor.argtypes = [vp, vp, vp, vp, C.c_size_t]
     L.eq_batch_couple_traps_dev.argtypes = [vp, C.c_int32, C.c_int32, P(C.c_int64), vp, C.c_double, vp, vp, vp,
                                             C.c_uint32]
@@ -277,6 +294,25 @@ class FieldBatch:
             self._h, trap0, off.size - 1, off.ctypes.data_as(C.POINTER(C.c_int64)), C.c_void_p(cells_ptr), npm,
             C.c_void_p(q_ptr), C.c_void_p(sampled_ptr) if sampled_ptr else None,
             C.c_void_p(counts_ptr) if counts_ptr else None, flags))
+
+    def set_wall_values(self, prob: int, wall: int, values=None):
+        """Per-node Dirichlet values along a Dirichlet wall, or the wall's constant when None."""
+        if values is None:
+            _check(load_library().eq_batch_set_wall_values(self._h, prob, wall, None, 0))
+            return
+        a = np.ascontiguousarray(values, dtype=np.float64)
+        _check(load_library().eq_batch_set_wall_values(self._h, prob, wall, _ptr(a), a.size))
+
+    def set_channels(self, prob: int, top=None, bottom=None):
+        """Flow channels (ChannelDesc or None) above/below the trap of problem prob."""
+        _check(load_library().eq_batch_set_channels(self._h, prob, C.byref(top) if top is not None else None,
+                                                    C.byref(bottom) if bottom is not None else None))
+
+    def channel_values(self, prob: int):
+        nx = self.descs[prob].nx
+        t, b = np.empty(nx + 1), np.empty(nx + 1)
+        _check(load_library().eq_batch_channel_values(self._h, prob, _ptr(t), _ptr(b)))
+        return t, b
 
     def set_tensor(self, prob: int, D11=None, D22=None, D12=None):
         """Per-node diffusion tensor (natural order), or constant coefficients when None."""

@@ -244,4 +244,65 @@ private:
     solverParams params_;
 };
 
+// diffusionPETSc (diffuclass.h:45-108): the finite-difference alternative
+// behind the same eQ::diffusionSolver surface.  One species per object, as in
+// the reference; the grid is (W/h + 1) x (H/h + 1) nodes with h = 1/npm
+// (diffuclass.cpp:356-358) and the walls default to Dirichlet 0 (the
+// DIRICHLET_0 set-up, diffuclass.cpp:70-89).  solution_vector is in natural
+// order, as the reference's (96-101).
+class diffusionPETSc {
+public:
+    diffusionPETSc() = default;
+
+    void initDiffusion(const solverParams& p, const eq_bc* walls = nullptr, const eq_solver_desc* solver = nullptr)
+    {
+        params_ = p;
+        const double h = 1.0 / p.nodesPerMicron;
+        eq_field_desc d{};
+        d.nx = (int32_t)(p.trapWidthMicrons / h);
+        d.ny = (int32_t)(p.trapHeightMicrons / h);
+        d.width = d.nx * h;
+        d.height = d.ny * h;
+        d.dt = p.dt;
+        d.D = p.D_HSL;
+        d.scheme = EQ_SCHEME_FD;
+        for (int w = 0; w < 4; ++w) {
+            if (walls) {
+                d.wall[w] = walls[w];
+            } else {
+                d.wall[w] = eq_bc{};
+                d.wall[w].kind = EQ_BC_DIRICHLET;
+            }
+        }
+        eq_batch* b = nullptr;
+        check(eq_batch_create(&d, 1, 1, solver, &b), "eq_batch_create (FD)");
+        batch_.reset(b);
+        solution_vector.assign((size_t)(d.nx + 1) * (d.ny + 1), 0.0);
+    }
+    // TimeStep: boundary values into the right-hand side, solve, read back
+    // (diffuclass.cpp:107-117, 405-413)
+    void stepDiffusion()
+    {
+        check(eq_batch_upload(batch_.get(), 0, solution_vector.data(), solution_vector.size()), "upload");
+        eq_step_stats st{};
+        check(eq_batch_step(batch_.get(), &st), "eq_batch_step (FD)");
+        last_ = st;
+        check(eq_batch_download(batch_.get(), 0, solution_vector.data(), solution_vector.size()), "download");
+    }
+    double getDiffusionConstant() const { return params_.D_HSL; }
+    const eq_step_stats& lastStats() const { return last_; }
+    void writeDiffusionFiles(double) {}
+    void finalize() {}
+
+    std::vector<double> solution_vector;
+
+private:
+    struct deleter {
+        void operator()(eq_batch* b) const { eq_batch_destroy(b); }
+    };
+    std::unique_ptr<eq_batch, deleter> batch_;
+    solverParams params_;
+    eq_step_stats last_{};
+};
+
 } // namespace eqgpu

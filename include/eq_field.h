@@ -154,6 +154,34 @@ int eq_batch_problem_stats(const eq_batch* b, int32_t* iterations, double* relre
  * Per cell in array order: sample every species, then deposit q/|S_c|. */
 int eq_batch_couple(eq_batch* b, int32_t trap, const eq_cell* cells, size_t nc, double npm,
                     const double* q, double* sampled, int32_t* counts, uint32_t flags);
+/* Per-node Dirichlet values on one wall of problem `prob` (a DirichletBC bound to a
+ * Function rather than a Constant, src/fHSL.cpp:514, 531): values[k] is the value at
+ * column k (top/bottom walls, n = nx+1) or row k (left/right, n = ny+1); the wall
+ * must be Dirichlet.  NULL reverts to the wall's constant value. */
+int eq_batch_set_wall_values(eq_batch* b, int32_t prob, int32_t wall, const double* values, size_t n);
+
+/* Flow channels above (top) and below (bottom) a trap -- fenicsChannel
+ * (src/fHSL.h:240-332) with the coupling of fenicsInterface::stepDiffusion
+ * (src/fHSL.cpp:110-152): the channel's values are the trap's Dirichlet data on
+ * that wall; after every solve the flux dt*D*(u_inner - u_wall)/well_scaling per
+ * column (computeBoundaryFlux, src/fHSL.cpp:54-96) is added to the channel in
+ * `substeps` equal parts, each followed by a Crank-Nicolson step of dt/substeps of
+ * the P1 advection-diffusion form of fenics/AdvectionDiffusion.ufl:62-64 (velocity
+ * v along +x, Robin ends r*(u - s) at x = 0 and x = width). */
+typedef struct {
+    double v;
+    double r_left, s_left;
+    double r_right, s_right;
+    double well_scaling;
+    int32_t substeps;
+    int32_t pad_;
+} eq_channel_desc;
+/* top/bottom may be NULL (no channel on that side); the walls with a channel must be
+ * Dirichlet.  Channels start at zero (src/fHSL.cpp:423-426). */
+int eq_batch_set_channels(eq_batch* b, int32_t prob, const eq_channel_desc* top, const eq_channel_desc* bottom);
+/* copy the channel values (nx+1 each, natural order) to the host; NULL skips a side */
+int eq_batch_channel_values(eq_batch* b, int32_t prob, double* top, double* bottom);
+
 /* Per-node diffusion tensor of problem `prob`: D11, D22, D12 (n = (nx+1)*(ny+1) each,
  * natural order) are fenicsInterface's D11/D22/D12 vectors (src/fHSL.cpp:313-323,
  * 387-392), P1-interpolated into A = D*[[D11, D12], [D12, D22]] of fenics/hslD.ufl:31

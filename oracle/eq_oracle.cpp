@@ -263,7 +263,14 @@ int64_t orc_dirichlet_nodes(const orc_problem* p, uint8_t* mask, double* g)
         for (const Facet& fc : facets) {
             if (!facet_on_wall(p, fc, w)) continue;
             mask[fc.a] = mask[fc.b] = 1;
-            if (g) g[fc.a] = g[fc.b] = p->wall[w].value;
+            if (g) {
+                const double* gw = p->gwall[w];
+                const int64_t nw = p->nx + 1;
+                /* position along the wall: column on top/bottom, row on left/right */
+                auto pos = [&](int64_t v) { return (w == ORC_TOP || w == ORC_BOTTOM) ? v % nw : v / nw; };
+                g[fc.a] = gw ? gw[pos(fc.a)] : p->wall[w].value;
+                g[fc.b] = gw ? gw[pos(fc.b)] : p->wall[w].value;
+            }
         }
     }
     int64_t cnt = 0;
@@ -411,6 +418,75 @@ int orc_fd_step(const orc_problem* p, double* u)
     if (rc) return rc;
     std::copy(x.begin(), x.end(), u);
     return 0;
+}
+
+/* one channel: u <- CN step of  (M + k/2 (D K + v C + R)) u = (M - k/2 (D K + v C + R)) u0
+ * + k (r1 s1 e_0 + r2 s2 e_N), P1 on a uniform interval mesh (cells of length hc):
+ * M_e = hc/6 [[2,1],[1,2]], K_e = 1/hc [[1,-1],[-1,1]], C_e = [[-1/2,1/2],[-1/2,1/2]]
+ * (row = test function, the advection term v du/dx w) */
+static void channel_cn(int64_t N, double hc, double D, double v, double k, const orc_channel* ch, double* u)
+{
+    const int64_t n = N + 1;
+    std::vector<double> a(n, 0.0), b(n, 0.0), c(n, 0.0), r(n, 0.0);
+    /* operator T = D K + v C + R, assembled per element; A = M + k/2 T, B = M - k/2 T */
+    std::vector<double> Ml(n, 0.0), Md(n, 0.0), Mu(n, 0.0), Tl(n, 0.0), Td(n, 0.0), Tu(n, 0.0);
+    for (int64_t e = 0; e < N; ++e) {
+        const int64_t p0 = e, p1 = e + 1;
+        Md[p0] += 2.0 * hc / 6.0; Mu[p0] += hc / 6.0;
+        Ml[p1] += hc / 6.0;       Md[p1] += 2.0 * hc / 6.0;
+        Td[p0] += D / hc;  Tu[p0] += -D / hc;
+        Tl[p1] += -D / hc; Td[p1] += D / hc;
+        Td[p0] += -0.5 * v; Tu[p0] += 0.5 * v;
+        Tl[p1] += -0.5 * v; Td[p1] += 0.5 * v;
+    }
+    Td[0] += ch->r_left;
+    Td[N] += ch->r_right;
+    for (int64_t i = 0; i < n; ++i) {
+        a[i] = Ml[i] + 0.5 * k * Tl[i];
+        b[i] = Md[i] + 0.5 * k * Td[i];
+        c[i] = Mu[i] + 0.5 * k * Tu[i];
+        double rhs = (Md[i] - 0.5 * k * Td[i]) * u[i];
+        if (i > 0) rhs += (Ml[i] - 0.5 * k * Tl[i]) * u[i - 1];
+        if (i < N) rhs += (Mu[i] - 0.5 * k * Tu[i]) * u[i + 1];
+        r[i] = rhs;
+    }
+    r[0] += k * ch->r_left * ch->s_left;
+    r[N] += k * ch->r_right * ch->s_right;
+    /* Thomas algorithm */
+    std::vector<double> cp(n), dp(n);
+    cp[0] = c[0] / b[0];
+    dp[0] = r[0] / b[0];
+    for (int64_t i = 1; i < n; ++i) {
+        const double m = b[i] - a[i] * cp[i - 1];
+        cp[i] = c[i] / m;
+        dp[i] = (r[i] - a[i] * dp[i - 1]) / m;
+    }
+    u[N] = dp[N];
+    for (int64_t i = N - 1; i >= 0; --i) u[i] = dp[i] - cp[i] * u[i + 1];
+}
+
+void orc_channel_update(const orc_problem* p, const orc_channel* top_ch, const orc_channel* bottom_ch,
+                        const double* u, double* top, double* bottom)
+{
+    const int64_t nx = p->nx, ny = p->ny, nw = nx + 1;
+    const double hc = p->width / nx;
+    for (int side = 0; side < 2; ++side) {
+        const orc_channel* ch = side == 0 ? top_ch : bottom_ch;
+        double* cv = side == 0 ? top : bottom;
+        if (!ch || !cv) continue;
+        const int nsub = ch->substeps > 0 ? ch->substeps : 1;
+        std::vector<double> flux(nw);
+        for (int64_t j = 0; j < nw; ++j) {
+            /* ds * dt * D * gradc / wellScaling with ds = h and gradc = du/h */
+            const double du = side == 0 ? u[(ny - 1) * nw + j] - u[ny * nw + j] : u[nw + j] - u[j];
+            flux[j] = p->dt * p->D * du / ch->well_scaling;
+        }
+        const double k = p->dt / nsub;
+        for (int s = 0; s < nsub; ++s) {
+            for (int64_t j = 0; j < nw; ++j) cv[j] += flux[j] / (double)nsub;
+            channel_cn(nx, hc, p->D, ch->v, k, ch, cv);
+        }
+    }
 }
 
 int orc_solve_pcg(const orc_problem* p, const int64_t* rowptr, const int32_t* col, const double* val,

@@ -56,7 +56,22 @@ typedef struct {
     double width, height;   /* trap size in microns                  */
     double dt, D, gamma, f; /* step, diffusivity, degradation, source */
     orc_bc wall[4];
+    /* per-node Dirichlet values along a wall (NULL: the wall's value): index j
+     * (column) on top/bottom, i (row) on left/right -- a DirichletBC bound to a
+     * Function instead of a Constant, as the channel-coupled walls are
+     * (src/fHSL.cpp:514, 531) */
+    const double* gwall[4];
 } orc_problem;
+
+/* flow channel above or below the trap (fenicsChannel, src/fHSL.h:240-332) */
+typedef struct {
+    double v;                  /* flow velocity along +x (data.v, src/fHSL.cpp:336-337) */
+    double r_left, s_left;     /* Robin rate / exterior value at the channel's left end  */
+    double r_right, s_right;   /* ... right end (ds(1), ds(2) of AdvectionDiffusion.ufl) */
+    double well_scaling;       /* wellScaling (src/fHSL.cpp:47)                           */
+    int32_t substeps;          /* channelSolverNumberIterations (src/main.cpp:414-430)    */
+    int32_t pad_;
+} orc_channel;
 
 typedef struct {
     double x, y, angle, length, width;
@@ -147,6 +162,17 @@ void orc_couple(const orc_grid* g, const orc_cell* cells, int64_t nc, int32_t ns
  * KSPFBCGSR without preconditioner at PETSc's default tolerances (389-391).
  * u is updated in place; returns 0 on success. */
 int orc_fd_step(const orc_problem* p, double* u);
+
+/* The channel part of fenicsInterface::stepDiffusion (src/fHSL.cpp:110-152)
+ * after the trap solve: the flux into each channel from the trap field u
+ * (computeBoundaryFlux, src/fHSL.cpp:54-96: dt*D*(u_inner - u_wall)/wellScaling
+ * per column), then `substeps` Crank-Nicolson steps of dt/substeps of the P1
+ * advection-diffusion form of AdvectionDiffusion.ufl:62-64 on the interval
+ * mesh of the trap's columns, each preceded by adding flux/substeps.  top and
+ * bottom (nx+1 values, natural order) are updated in place; they are the
+ * next trap solve's Dirichlet data on those walls. */
+void orc_channel_update(const orc_problem* p, const orc_channel* top_ch, const orc_channel* bottom_ch,
+                        const double* u, double* top, double* bottom);
 
 /* boundary-flux functional of stepDiffusion (fenics/boundary.ufl:9-12):
  * -int_{boundary} grad(u).n ds over all walls, P1 gradient per boundary triangle */

@@ -30,7 +30,26 @@ class Bc(C.Structure):
 class Problem(C.Structure):
     _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("width", C.c_double),
                 ("height", C.c_double), ("dt", C.c_double), ("D", C.c_double),
-                ("gamma", C.c_double), ("f", C.c_double), ("wall", Bc * 4)]
+                ("gamma", C.c_double), ("f", C.c_double), ("wall", Bc * 4),
+                ("gwall", C.c_void_p * 4)]
+
+    def set_wall_values(self, wall: int, values):
+        """Per-node Dirichlet values along a wall (None: the wall's constant)."""
+        if not hasattr(self, "_keep"):
+            self._keep = {}
+        if values is None:
+            self.gwall[wall] = None
+            self._keep.pop(wall, None)
+            return
+        a = np.ascontiguousarray(values, dtype=np.float64).copy()
+        self._keep[wall] = a
+        self.gwall[wall] = a.ctypes.data
+
+
+class Channel(C.Structure):
+    _fields_ = [("v", C.c_double), ("r_left", C.c_double), ("s_left", C.c_double),
+                ("r_right", C.c_double), ("s_right", C.c_double), ("well_scaling", C.c_double),
+                ("substeps", C.c_int32), ("pad_", C.c_int32)]
 
 
 class Cell(C.Structure):
@@ -80,6 +99,8 @@ def lib():
         L.orc_solve_pcg.restype = C.c_int
         L.orc_solve_pcg.argtypes = [P(Problem), ip64, ip32, dp, dp, dp, C.c_double, C.c_int32,
                                     P(SolveInfo)]
+        L.orc_channel_update.restype = None
+        L.orc_channel_update.argtypes = [P(Problem), P(Channel), P(Channel), dp, dp, dp]
         L.orc_fd_step.restype = C.c_int
         L.orc_fd_step.argtypes = [P(Problem), dp]
         L.orc_step.restype = C.c_int
@@ -191,6 +212,22 @@ def step(p: Problem, u, method="lu", rtol=1e-14, D11=None, D22=None, D12=None, m
     # PCG may stop at the rounding floor of the CSR residual above rtol;
     # callers read info.relres
     return u, info
+
+
+def channel(v, r_left, s_left, r_right, s_right, well_scaling, substeps) -> Channel:
+    ch = Channel()
+    ch.v, ch.r_left, ch.s_left, ch.r_right, ch.s_right = v, r_left, s_left, r_right, s_right
+    ch.well_scaling, ch.substeps = well_scaling, substeps
+    return ch
+
+
+def channel_update(p: Problem, top_ch, bottom_ch, u, top, bottom):
+    """Channel part of the step after the trap solve; returns new (top, bottom)."""
+    t = np.array(top, dtype=np.float64, copy=True)
+    b = np.array(bottom, dtype=np.float64, copy=True)
+    lib().orc_channel_update(C.byref(p), C.byref(top_ch), C.byref(bottom_ch), _dp(np.ascontiguousarray(u, np.float64)),
+                             _dp(t), _dp(b))
+    return t, b
 
 
 def fd_step(p: Problem, u):

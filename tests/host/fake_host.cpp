@@ -12,8 +12,14 @@
 //   stepDiffusion() on every species.
 // mode 1 (device resident): each step trapField::coupleCells, then one batch
 //   step for all species; the fields are downloaded at the end.
+// mode 2: mode 0 with a per-node diffusion tensor: double tensor[ns][3][N]
+//   (D11, D22, D12) follows u0 and is written into each species' D11/D22/D12
+//   vectors before the first step (src/simulation.cpp:311-313).
+// mode 3: mode 0 on eqgpu::diffusionPETSc objects (the finite-difference
+//   solver, one per species) with the given walls.
 // output: double fields[ns*N], double sampled[nc*ns] (mode 1, last step),
-//   double flux[ns], int32 iterations_max.
+//   double flux[ns] (modes 0-2), int32 iterations_max.
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <stdexcept>
@@ -53,6 +59,11 @@ int main(int argc, char** argv)
         const size_t N = (size_t)(nx + 1) * (ny + 1);
         std::vector<std::vector<double>> u0(ns, std::vector<double>(N));
         for (int s = 0; s < ns; ++s) rd(in, u0[s].data(), N);
+        std::vector<std::vector<double>> tens;
+        if (mode == 2) {
+            tens.assign((size_t)ns * 3, std::vector<double>(N));
+            for (auto& t : tens) rd(in, t.data(), N);
+        }
         std::fclose(in);
 
         eqgpu::solverParams p;
@@ -60,6 +71,33 @@ int main(int argc, char** argv)
         p.trapHeightMicrons = g4[1];
         p.nodesPerMicron = g4[2];
         p.dt = g4[3];
+        if (mode == 3) {
+            std::vector<eqgpu::diffusionPETSc> fd(ns);
+            int32_t itmax = 0;
+            for (int s = 0; s < ns; ++s) {
+                eqgpu::solverParams ps = p;
+                ps.D_HSL = D[s];
+                fd[s].initDiffusion(ps, walls);
+                if (fd[s].solution_vector.size() != N) throw std::runtime_error("FD grid size mismatch");
+                fd[s].solution_vector = u0[s];
+            }
+            for (int k = 0; k < nsteps; ++k) {
+                for (int c = 0; c < nc; ++c)
+                    for (int s = 0; s < ns; ++s) fd[s].solution_vector[(size_t)c % N] += q[(size_t)c * ns + s];
+                for (int s = 0; s < ns; ++s) {
+                    fd[s].stepDiffusion();
+                    itmax = std::max(itmax, fd[s].lastStats().iterations_max);
+                }
+            }
+            FILE* fo = std::fopen(argv[2], "wb");
+            if (!fo) throw std::runtime_error("cannot open output");
+            for (int s = 0; s < ns; ++s) std::fwrite(fd[s].solution_vector.data(), sizeof(double), N, fo);
+            std::vector<double> zeros((size_t)nc * ns + ns, 0.0);
+            std::fwrite(zeros.data(), sizeof(double), zeros.size(), fo);
+            std::fwrite(&itmax, sizeof(int32_t), 1, fo);
+            std::fclose(fo);
+            return 0;
+        }
         auto field = std::make_shared<eqgpu::trapField>(p, D, gamma, walls);
         if ((size_t)field->nodes() != N) throw std::runtime_error("grid size mismatch");
         std::vector<eqgpu::fenicsInterface> species;
@@ -69,12 +107,17 @@ int main(int argc, char** argv)
             species.back().initDiffusion(p);
             if (species.back().shell->mesh->num_vertices() != N) throw std::runtime_error("num_vertices");
             species.back().solution_vector = u0[s];
+            if (mode == 2) {
+                *species.back().D11 = tens[(size_t)s * 3 + 0];
+                *species.back().D22 = tens[(size_t)s * 3 + 1];
+                *species.back().D12 = tens[(size_t)s * 3 + 2];
+            }
         }
         std::vector<double> sampled;
         if (mode == 1)
             for (int s = 0; s < ns; ++s) field->upload(s, u0[s]);
         for (int k = 0; k < nsteps; ++k) {
-            if (mode == 0) {
+            if (mode == 0 || mode == 2) {
                 // controller: updateCells writes into its copy of each field,
                 // then the copies travel to the solvers (src/simulation.cpp:447-463)
                 for (int c = 0; c < nc; ++c)

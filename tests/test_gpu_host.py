@@ -14,7 +14,7 @@ EXE = os.path.join(ROOT, "tests", "host", "bin", "fake_host")
 pytestmark = pytest.mark.gpu
 
 
-def _write_input(path, mode, t, nsteps, q, walls):
+def _write_input(path, mode, t, nsteps, q, walls, tensors=None):
     ns = len(t.D)
     N = t.nodes
     with open(path, "wb") as f:
@@ -29,6 +29,9 @@ def _write_input(path, mode, t, nsteps, q, walls):
         f.write(np.ascontiguousarray(q, np.float64).tobytes())
         for u in t.u0:
             f.write(np.ascontiguousarray(u, np.float64).tobytes())
+        for tens in (tensors or []):
+            for a in tens:
+                f.write(np.ascontiguousarray(a, np.float64).tobytes())
     return N
 
 
@@ -72,3 +75,65 @@ def test_fake_host_matches_oracle(oracle, tmp_path, mode):
         assert abs(flux[s] - 0.1 * t.D[s] * oracle.boundary_flux(probs[s], fields[s])) <= 1e-10 * (1 + abs(flux[s]))
     if mode == 1:
         assert np.allclose(sampled, s_ref, rtol=1e-9)
+
+
+def _tensors(N, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(2):
+        d11, d22 = 0.5 + rng.random(N), 0.5 + rng.random(N)
+        out.append((d11, d22, 0.8 * np.sqrt(d11 * d22) * (rng.random(N) - 0.5)))
+    return out
+
+
+def test_fake_host_tensor_field(oracle, tmp_path):
+    """Mode 2: the host writes D11/D22/D12 into the fenicsInterface views (as the
+    controller does, src/simulation.cpp:311-313) and the mirror sends them
+    before the solve."""
+    from eq_amd import workloads as W
+    t = W.config0()
+    walls = W.TOP_DIRICHLET
+    q = t.rates * 0.1
+    nsteps = 2
+    tens = _tensors(t.nodes, 8)
+    inp, outp = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    N = _write_input(inp, 2, t, nsteps, q, walls, tens)
+    print(f"fake_host mode=2 nsteps={nsteps}", flush=True)
+    subprocess.run([EXE, inp, outp], check=True, timeout=120)
+    fields, _, _ = _read_output(outp, 2, N, t.cells.shape[0])
+    probs = [oracle.problem(t.nx, t.ny, t.width, t.height, 0.1, D, g, 0.0, walls=walls) for D, g in zip(t.D, t.gamma)]
+    ref = [u.copy() for u in t.u0]
+    for _ in range(nsteps):
+        for c in range(t.cells.shape[0]):
+            for s in range(2):
+                ref[s][c % N] += q[c, s]
+        ref = [oracle.step(probs[s], ref[s], "lu", D11=tens[s][0], D22=tens[s][1], D12=tens[s][2])[0]
+               for s in range(2)]
+    for s in range(2):
+        err = np.linalg.norm(fields[s] - ref[s]) / np.linalg.norm(ref[s])
+        assert err <= 1e-9, (s, err)
+
+
+def test_fake_host_fd_solver(oracle, tmp_path):
+    """Mode 3: eqgpu::diffusionPETSc, the finite-difference solver behind the
+    same surface, against the oracle's restatement of diffuclass.cpp."""
+    from eq_amd import workloads as W
+    t = W.config0()
+    walls = {oracle.TOP: (oracle.DIRICHLET, 0.0, 0, 0), oracle.RIGHT: (oracle.ROBIN, 0, 2.0, 0.3)}
+    q = t.rates * 0.1
+    nsteps = 3
+    inp, outp = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    N = _write_input(inp, 3, t, nsteps, q, walls)
+    print(f"fake_host mode=3 nsteps={nsteps}", flush=True)
+    subprocess.run([EXE, inp, outp], check=True, timeout=120)
+    fields, _, _ = _read_output(outp, 2, N, t.cells.shape[0])
+    probs = [oracle.problem(t.nx, t.ny, t.width, t.height, 0.1, D, 0.0, 0.0, walls=walls) for D in t.D]
+    ref = [u.copy() for u in t.u0]
+    for _ in range(nsteps):
+        for c in range(t.cells.shape[0]):
+            for s in range(2):
+                ref[s][c % N] += q[c, s]
+        ref = [oracle.fd_step(probs[s], ref[s]) for s in range(2)]
+    for s in range(2):
+        err = np.linalg.norm(fields[s] - ref[s]) / np.linalg.norm(ref[s])
+        assert err <= 1e-9, (s, err)

@@ -405,3 +405,63 @@ def test_fd_scheme_matches_oracle(eqf, oracle, solver):
             for k in range(len(descs)):
                 ref[k] = oracle.fd_step(probs[k], ref[k])
                 assert _rel(fb.download(k), ref[k]) <= 1e-9, (step, k)
+
+
+def _channel_rates(v, D, length_left, length_right):
+    """Robin rates of the channel ends (fenicsInterface::setRobinBoundaryConditions)."""
+    if v > 1e-6:
+        return v / (1.0 - np.exp(-length_left * v / D)), v / (np.exp(length_right * v / D) - 1.0)
+    return D / length_left, D / length_right
+
+
+@pytest.mark.parametrize("solver", SOLVERS)
+def test_flow_channels_match_oracle(eqf, oracle, solver):
+    """Traps whose top and bottom walls are flow channels: each step solves the
+    trap with the channels' values as Dirichlet data, then moves the flux into
+    the channels and advances them by Crank-Nicolson substeps; three steps of
+    fields and channels against the oracle."""
+    shapes = [(60, 30, 6.0, 3.0), (41, 50, 4.1, 5.0)]
+    walls = {oracle.TOP: (oracle.DIRICHLET, 0.0, 0, 0), oracle.BOTTOM: (oracle.DIRICHLET, 0.0, 0, 0),
+             oracle.LEFT: (oracle.ROBIN, 0, 1.5, 0.2)}
+    descs, probs, u0, chans = [], [], [], []
+    rng = np.random.default_rng(31)
+    for (nx, ny, W, H) in shapes:
+        for D in (300.0, 80.0):
+            descs.append(eqf.field_desc(nx, ny, W, H, 0.1, D, 0.01, 0.0, walls))
+            probs.append(oracle.problem(nx, ny, W, H, 0.1, D, 0.01, 0.0, walls=walls))
+            u0.append(rng.random((nx + 1) * (ny + 1)))
+            pair = []
+            for v, subs in ((20.0, 3), (0.0, 1)):
+                rl, rr = _channel_rates(v, D, 5.0, 7.0)
+                pair.append((v, rl, 0.0, rr, 0.0, 2.5, subs))
+            chans.append(pair)
+    with eqf.FieldBatch(descs, nspecies=2, solver=eqf.solver_desc(_kind(eqf, solver))) as fb:
+        for k in range(len(descs)):
+            fb.upload(k, u0[k])
+            fb.set_channels(k, eqf.channel_desc(*chans[k][0]), eqf.channel_desc(*chans[k][1]))
+        ref = [u.copy() for u in u0]
+        tops = [np.zeros(p.nx + 1) for p in probs]
+        bots = [np.zeros(p.nx + 1) for p in probs]
+        for step in range(3):
+            st = fb.step()
+            assert st.nconverged == len(descs), step
+            for k, p in enumerate(probs):
+                p.set_wall_values(oracle.TOP, tops[k])
+                p.set_wall_values(oracle.BOTTOM, bots[k])
+                ref[k], _ = oracle.step(p, ref[k], "lu")
+                tops[k], bots[k] = oracle.channel_update(p, oracle.channel(*chans[k][0]), oracle.channel(*chans[k][1]),
+                                                         ref[k], tops[k], bots[k])
+                assert _rel(fb.download(k), ref[k]) <= 1e-9, (step, k)
+                t, b = fb.channel_values(k)
+                assert _rel(t, tops[k]) <= 1e-9 and _rel(b, bots[k]) <= 1e-9, (step, k)
+    # per-node wall values on their own
+    d, p = _pair(eqf, oracle, 30, 20, 3.0, 2.0, 0.1, 50.0, 0.0, 0.0, {oracle.TOP: (oracle.DIRICHLET, 0.0, 0, 0)})
+    g = np.linspace(0.0, 1.0, 31)
+    u = rng.random(31 * 21)
+    with eqf.FieldBatch([d], solver=eqf.solver_desc(_kind(eqf, solver))) as fb:
+        fb.upload(0, u)
+        fb.set_wall_values(0, oracle.TOP, g)
+        fb.step()
+        p.set_wall_values(oracle.TOP, g)
+        r, _ = oracle.step(p, u, "lu")
+        assert _rel(fb.download(0), r) <= 1e-9

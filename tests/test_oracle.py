@@ -286,3 +286,40 @@ def test_fd_scheme_closed_forms(oracle):
     u2 = oracle.fd_step(p2, u0)
     assert np.all(u2[(j == 0)] == 2.0)
     assert np.all(u2[(i == ny) & (j > 0)] == 0.5)
+
+
+def test_channel_cn_closed_forms(oracle):
+    """The flow-channel Crank-Nicolson step (AdvectionDiffusion.ufl): with v = 0
+    and no Robin ends, nodal cosines are eigenvectors of M and K (boundary rows
+    halved alike), so one step multiplies mode theta by (mu - k D lam/2) /
+    (mu + k D lam/2), mu = hc (4 + 2 cos theta)/6, lam = (2 - 2 cos theta)/hc;
+    constants are invariant for any v (C 1 = 0); the flux from the trap is
+    dt*D*(u_inner - u_wall)/wellScaling per column (src/fHSL.cpp:54-96)."""
+    nx, ny, W, H, dt, D = 40, 6, 8.0, 1.2, 0.1, 50.0
+    p = oracle.problem(nx, ny, W, H, dt, D)
+    hc = W / nx
+    u_flat = np.ones((ny + 1) * (nx + 1))  # no flux: equal rows
+    ch = oracle.channel(0.0, 0.0, 0.0, 0.0, 0.0, 3.0, 1)
+    for kmode in (1, 4):
+        th = np.pi * kmode / nx
+        c0 = np.cos(th * np.arange(nx + 1))
+        t1, b1 = oracle.channel_update(p, ch, ch, u_flat, c0, 2.0 * c0)
+        mu, lam = hc * (4 + 2 * np.cos(th)) / 6, (2 - 2 * np.cos(th)) / hc
+        g = (mu - dt * D * lam / 2) / (mu + dt * D * lam / 2)
+        assert np.allclose(t1, g * c0, atol=1e-13)
+        assert np.allclose(b1, 2.0 * g * c0, atol=1e-13)
+    chv = oracle.channel(35.0, 0.0, 0.0, 0.0, 0.0, 3.0, 3)
+    t1, _ = oracle.channel_update(p, chv, chv, u_flat, np.full(nx + 1, 0.7), np.zeros(nx + 1))
+    assert np.allclose(t1, 0.7, atol=1e-14)
+    # flux: a trap field whose top row is lower than the next row by d_j
+    u = np.zeros((ny + 1, nx + 1))
+    d = np.linspace(0.1, 0.5, nx + 1)
+    u[ny - 1] = d
+    ch1 = oracle.channel(0.0, 0.0, 0.0, 0.0, 0.0, 4.0, 1)
+    # with D = 0 in the channel form there is no diffusion: the channel gains exactly the flux
+    p0 = oracle.problem(nx, ny, W, H, dt, D)
+    t1, b1 = oracle.channel_update(p0, ch1, ch1, u.ravel(), np.zeros(nx + 1), np.zeros(nx + 1))
+    tot = (t1 * np.where((np.arange(nx + 1) == 0) | (np.arange(nx + 1) == nx), 0.5, 1.0)).sum()
+    ref = (dt * D * d / 4.0 * np.where((np.arange(nx + 1) == 0) | (np.arange(nx + 1) == nx), 0.5, 1.0)).sum()
+    assert abs(tot - ref) < 1e-12 * abs(ref)   # diffusion conserves the lumped total M 1
+    assert np.all(b1 == 0.0)
