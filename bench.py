@@ -229,6 +229,53 @@ def config3_batched(F, W, nsteps, warmup=2, ntraps=512, sigma=0.05):
     return res
 
 
+def config4_advection(F, W, nsteps, warmup=1):
+    """Config 4: one 8192x4096 trap (h = 0.025 um), 4 species, parabolic
+    channel velocity along x (peak 50 um/s, zero at the walls), mixed
+    GMG-BiCGStab to 1e-10; 100 000 static rods (seed 4) coupled every step."""
+    import torch
+    trap = W.config4()
+    vx = W.channel_velocity(trap.ny, trap.height)
+    descs = [F.field_desc(trap.nx, trap.ny, trap.width, trap.height, 0.1, D, g, 0.0, W.TOP_DIRICHLET)
+             for D, g in zip(trap.D, trap.gamma)]
+    nodes = (trap.nx + 1) * (trap.ny + 1) * len(descs)
+    nc = trap.cells.shape[0]
+    cells_d = torch.from_numpy(trap.cells).to("cuda").contiguous()
+    q = torch.from_numpy(trap.rates * 0.1).to("cuda").contiguous()
+    sampled = torch.empty((nc, len(descs)), dtype=torch.float64, device="cuda")
+    t_create = time.perf_counter()
+    fb = F.FieldBatch(descs, nspecies=len(descs), solver=F.solver_desc(F.MIXED_GMG_BICGSTAB))
+    for s in range(len(descs)):
+        fb.set_velocity(s, vx)
+    t_create = time.perf_counter() - t_create
+
+    def step():
+        fb.couple_dev(0, cells_d.data_ptr(), nc, trap.npm, q.data_ptr(), sampled.data_ptr(), 0,
+                      F.SAMPLE | F.DEPOSIT)
+        return fb.step()
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    iters, rel = [], []
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        st = step()
+        iters.append(st.iterations_max)
+        rel.append(st.relres_max)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    res = {"workload": "one 204.8x102.4 um trap, 8192x4096 P1 mesh (33 566 721 nodes/species), 4 species, "
+                       "parabolic v_x peak 50 um/s, 100000 static rods (seed 4), mixed GMG-BiCGStab",
+           "value": nodes * nsteps / dt, "unit": "DOF-timesteps/s", "steps": nsteps, "warmup": warmup,
+           "ms_per_step": dt / nsteps * 1e3, "dofs": nodes, "iterations_max": int(max(iters)),
+           "relres_max": float(max(rel)), "create_seconds": t_create}
+    log(f"bench: config 4: {res['value']:.4e} DOF-timesteps/s, {res['ms_per_step']:.2f} ms/step, "
+        f"BiCGStab iterations max {res['iterations_max']}, relres {res['relres_max']:.2e}")
+    fb.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -243,6 +290,10 @@ def main():
                     help="timed steps of the config-3 batch (512 uneven traps), part of the variants")
     ap.add_argument("--c2-steps", type=int, default=50,
                     help="steps of the config-2 drift comparison (mixed vs FP64), part of the variants")
+    ap.add_argument("--c4-steps", type=int, default=3,
+                    help="timed steps of config 4 (8192x4096 trap, 4 species, advection), part of the variants")
+    ap.add_argument("--only", choices=["c3", "c4"], default=None,
+                    help="run only the config-3 batch or the config-4 trap (for profiling) and print its line")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 256 MiB read + 256 MiB write elementwise kernel (torch.mul) three times first, "
                          "as the known-size reference for HBM byte counters")
@@ -264,6 +315,16 @@ def main():
             torch.mul(a, 1.0, out=b)
         torch.cuda.synchronize()
         del a, b
+    if args.only:
+        res = (config3_batched(F, W, args.steps, warmup=max(args.warmup, 2)) if args.only == "c3"
+               else config4_advection(F, W, args.steps, warmup=max(args.warmup, 1)))
+        line = {"metric": "DOF-timesteps/s at 1e-10 rel. residual (FP64 vs mixed/batched); % HBM peak",
+                "value": res["value"], "unit": "DOF-timesteps/s", "n_gpus": 1, "steps": args.steps,
+                "warmup": args.warmup, "ms_per_step": res["ms_per_step"], "higher_is_better": True,
+                "scaling": "weak", "vs_baseline": None, "dtype": "f64 (outer) / f32 (multigrid)",
+                "data": "synthetic", "config": {"workload": res["workload"]}, "detail": res}
+        print(json.dumps(line), flush=True)
+        return
     kinds = {"mixed": F.MIXED_GMG_CG, "gmg": F.GMG_CG, "jacobi": F.JACOBI_CG}
     trap = W.config1()
     nsteps = args.warmup + args.steps
@@ -380,6 +441,8 @@ def main():
         out["config2"] = config2_drift(F, W, cells_dev, q_dev, args.c2_steps)
         if args.c3_steps > 0:
             out["config3"] = config3_batched(F, W, args.c3_steps)
+        if args.c4_steps > 0:
+            out["config4"] = config4_advection(F, W, args.c4_steps)
 
     if not args.no_cpu_baseline:
         jac_iters = out.get("variants", {}).get("jacobi", {}).get("step0_iterations")

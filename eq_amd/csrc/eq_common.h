@@ -106,6 +106,13 @@ struct LevelProb {
     const double* tens_d;  // FP64 copy (level 0)
     const float* tens_f;   // FP32 copy (levels smoothed in FP32)
     int64_t tstride;
+    // advection: x-velocity per grid row (ny+1 values, P1 in y), Galerkin
+    // convection dt * int (v du/dx) phi; cw = dt*hx*hy/(24 hx) = dt*hy/24
+    int32_t vel;           // 1: the row velocities below are set
+    int32_t pad2_;
+    double cw;
+    const double* vel_d;   // FP64 rows (level 0)
+    const float* vel_f;    // FP32 rows (levels smoothed in FP32)
 };
 
 // (i,j) must be a node of the grid
@@ -301,6 +308,45 @@ EQ_HD inline void aniso_stiffness(const LevelProb& P, int i, int j, const T* d11
     r.kw = kw; r.ke = ke; r.ks = ks; r.kn = kn; r.ksw = ksw; r.kne = kne;
 }
 
+// Convection rows of node (i,j) added to r in difference form (k_nb += -dt
+// C[c][nb]; C has zero row sums).  Per triangle around the node with
+// vertices (c, a, b): int_T v phi_c = |T|/12 (2 v_c + v_a + v_b) times
+// dphi_nb/dx = -+1/hx; with v constant along rows the six triangles give
+//   SW cell  lower (SW,S,c): C[SW] -= I/hx, C[S] += I/hx   I ~ 2v_i + 2v_{i-1}
+//            upper (SW,W,c): C[W]  -= I/hx                 I ~ 3v_i +  v_{i-1}
+//   SE cell  upper (S,c,E):  C[E]  += I/hx                 I ~ 3v_i +  v_{i-1}
+//   NW cell  lower (W,c,N):  C[W]  -= I/hx                 I ~ 3v_i +  v_{i+1}
+//   own cell lower (c,E,NE): C[E]  += I/hx                 I ~ 3v_i +  v_{i+1}
+//            upper (c,N,NE): C[N]  -= I/hx, C[NE] += I/hx  I ~ 2v_i + 2v_{i+1}
+// (I = |T|/12 * (...), |T|/(12 hx) = hy/24).  v holds the level's rows.
+template <typename T>
+EQ_HD inline void add_convection(const LevelProb& P, int i, int j, const T* v, Row<T>& r)
+{
+    const bool sw = i > 0 && j > 0, se = i > 0 && j < P.nx, nw = i < P.ny && j > 0, ne = i < P.ny && j < P.nx;
+    const T cw = (T)P.cw, three = (T)3, two = (T)2;
+    const T vi = v[i];
+    if (sw) {
+        const T vs = v[i - 1];
+        const T I1 = cw * (two * vi + two * vs), I2 = cw * (three * vi + vs);
+        r.ksw += I1;
+        r.ks -= I1;
+        r.kw += I2;
+    }
+    if (se) r.ke -= cw * (three * vi + v[i - 1]);
+    if (nw) r.kw += cw * (three * vi + v[i + 1]);
+    if (ne) {
+        const T vn = v[i + 1];
+        const T I5 = cw * (three * vi + vn), I6 = cw * (two * vi + two * vn);
+        r.ke -= I5;
+        r.kn += I6;
+        r.kne -= I6;
+    }
+}
+
+template <typename T> EQ_HD inline const T* vel_base(const LevelProb& P);
+template <> EQ_HD inline const double* vel_base<double>(const LevelProb& P) { return P.vel_d; }
+template <> EQ_HD inline const float* vel_base<float>(const LevelProb& P) { return P.vel_f; }
+
 template <typename T> EQ_HD inline const T* tens_base(const LevelProb& P);
 template <> EQ_HD inline const double* tens_base<double>(const LevelProb& P) { return P.tens_d; }
 template <> EQ_HD inline const float* tens_base<float>(const LevelProb& P) { return P.tens_f; }
@@ -311,10 +357,13 @@ template <> EQ_HD inline const float* tens_base<float>(const LevelProb& P) { ret
 template <typename T, bool TENS = true>
 EQ_HD inline Row<T> row_coefs(const LevelProb& P, int i, int j, int64_t k)
 {
-    if (TENS && P.aniso) {
+    if (TENS && (P.aniso || P.vel)) {
         Row<T> r = row_coefs_at<T>(P, i, j);
-        const T* t = tens_base<T>(P) + k;
-        aniso_stiffness<T>(P, i, j, t, t + P.tstride, t + 2 * P.tstride, P.pitch, r);
+        if (P.aniso) {
+            const T* t = tens_base<T>(P) + k;
+            aniso_stiffness<T>(P, i, j, t, t + P.tstride, t + 2 * P.tstride, P.pitch, r);
+        }
+        if (P.vel) add_convection<T>(P, i, j, vel_base<T>(P), r);
         return r;
     }
     if (interior(P, i, j)) return interior_row<T>(P);
@@ -325,7 +374,7 @@ EQ_HD inline Row<T> row_coefs(const LevelProb& P, int i, int j, int64_t k)
 template <bool TENS = true, typename T>
 EQ_HD inline T row_dinv(const LevelProb& P, int i, int j, const Row<T>& rc)
 {
-    if ((!TENS || !P.aniso) && interior(P, i, j)) return interior_dinv<T>(P);
+    if ((!TENS || !(P.aniso || P.vel)) && interior(P, i, j)) return interior_dinv<T>(P);
     return (T)1 / row_diag<TENS>(rc);
 }
 

@@ -31,6 +31,8 @@ struct CgState {
     int32_t iter, done, max_iter, flexible;
     int32_t restart;        // next beta is 0 (after a residual replacement)
     int32_t have_prev;      // the previous solution is kept: start CG from it
+    // BiCGStab: rho = (r^, r), omega, and the pieces of omega
+    double rho, omega, ts, tt;
 };
 
 struct Level {
@@ -81,7 +83,9 @@ struct eq_batch {
     std::vector<eq_field_desc> descs;
     eq_solver_desc solver{};
     int32_t mixed = 0;                 // 1: FP32 V-cycle
-    bool any_tensor = false;           // some problem has a per-node diffusion tensor
+    bool any_tensor = false;           // some problem has per-node rows (tensor field or advection)
+    std::vector<std::vector<std::vector<double>>> vel_host; // per problem, per level: row velocities
+    std::vector<void*> vel_bufs;       // device copies of the row velocities
     int64_t lds_budget = 64 * 1024;    // LDS bytes the bottom V-cycle may use per workgroup
     std::vector<eqk::Level> levels;
     // level 0 FP64 Krylov vectors (pooled over problems)
@@ -93,6 +97,13 @@ struct eq_batch {
     double* q = nullptr;
     double* z = nullptr;
     double* xprev = nullptr;    // last solution, the next solve's starting iterate
+    // BiCGStab vectors (allocated with the batch for the BiCGStab kinds)
+    double* bs_rhat = nullptr;
+    double* bs_p = nullptr;
+    double* bs_v = nullptr;
+    double* bs_s = nullptr;
+    double* bs_t = nullptr;
+    hipGraphExec_t g_bicg = nullptr;
     double* tens0_d = nullptr;  // level-0 tensor field D11|D22|D12 in FP64 (allocated on first use)
     double* partial = nullptr;  // 4 slots per level-0 tile
     void* dense = nullptr;      // dense inverses of coarsest levels (smoother precision)

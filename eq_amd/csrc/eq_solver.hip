@@ -150,7 +150,11 @@ __device__ __forceinline__ void tile_partial(double (&v)[NV], double* partial, i
         for (int k = 0; k < NV; ++k) partial[(int64_t)gt * 4 + k] = v[k];
 }
 
-enum FinishOp { kFinRhs = 0, kFinInit = 1, kFinAp = 2, kFinUpdate = 3, kFinRz = 4, kFinTrue = 5 };
+enum FinishOp {
+    kFinRhs = 0, kFinInit = 1, kFinAp = 2, kFinUpdate = 3, kFinRz = 4, kFinTrue = 5,
+    // BiCGStab
+    kFinBInit = 6, kFinBAlpha = 7, kFinBOmega = 8, kFinBUpdate = 9, kFinBRestart = 10
+};
 
 template <typename T>
 __device__ __forceinline__ void load7(const T* __restrict__ v, int64_t idx, int pitch, T& c, T& w, T& e, T& s, T& n,
@@ -380,7 +384,9 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(const LevelProb* __re
 {
     const LevelProb& P = probs[blockIdx.x];
     CgState& s = st[P.prob];
-    if ((op == kFinAp || op == kFinUpdate || op == kFinRz) && s.done) return;
+    if ((op == kFinAp || op == kFinUpdate || op == kFinRz || op == kFinBAlpha || op == kFinBOmega ||
+         op == kFinBUpdate) && s.done)
+        return;
     double v[3] = {0.0, 0.0, 0.0};
     const double* base = partial + (int64_t)P.tile_begin * 4;
     int q = threadIdx.x;
@@ -419,7 +425,8 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(const LevelProb* __re
             s.alpha = s.rz / v[0];
         } else {
             s.alpha = 0.0;
-            s.done = 1; // exact solution reached (or breakdown): stop this problem
+            s.iter += 1; // counted, so repeated breakdowns end at max_iter
+            s.done = 1;  // exact solution reached (or breakdown): stop this problem
         }
         break;
     case kFinUpdate:
@@ -448,7 +455,155 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(const LevelProb* __re
         s.done = (v[0] <= s.tol2 * s.bb || s.iter >= s.max_iter) ? 1 : 0;
         s.restart = s.done ? 0 : 1;
         break;
+    case kFinBRestart: // after kFinTrue: restarted problems begin again from r^ = r
+        if (s.restart) {
+            s.rho = s.rr;
+            s.beta = 0.0;
+            s.alpha = 1.0;
+            s.omega = 1.0;
+            s.restart = 0;
+        }
+        break;
+    // BiCGStab (right preconditioned, van der Vorst 1992): r^ = r at the
+    // start and after every restart, so (r^, r) = (r, r)
+    case kFinBInit:
+        s.rr = v[0];
+        s.rho = v[0];
+        s.iter = 0;
+        s.alpha = 1.0;
+        s.omega = 1.0;
+        s.beta = 0.0;
+        s.restart = 0;
+        s.done = (v[0] <= s.tol2 * s.bb) ? 1 : 0;
+        break;
+    case kFinBAlpha: // v[0] = (r^, v)
+        if (v[0] != 0.0 && s.rho != 0.0) {
+            s.alpha = s.rho / v[0];
+        } else {
+            s.alpha = 0.0;
+            s.iter += 1; // counted, so repeated breakdowns end at max_iter
+            s.done = 1;  // breakdown: the true-residual check restarts it
+        }
+        break;
+    case kFinBOmega: // v[0] = (t, s), v[1] = (t, t)
+        s.ts = v[0];
+        s.tt = v[1];
+        s.omega = (v[1] > 0.0) ? v[0] / v[1] : 0.0;
+        break;
+    case kFinBUpdate: { // v[0] = (r^, r_new), v[1] = (r_new, r_new)
+        s.iter += 1;
+        s.rr = v[1];
+        const double rho_new = v[0];
+        if (s.omega != 0.0 && s.rho != 0.0) {
+            s.beta = (rho_new / s.rho) * (s.alpha / s.omega);
+        } else {
+            s.beta = 0.0;
+            s.done = 1; // stagnation: the true-residual check restarts it
+        }
+        s.rho = rho_new;
+        if (v[1] <= s.tol2 * s.bb || s.iter >= s.max_iter) s.done = 1;
+        break;
     }
+    }
+}
+
+// ------------------------------------------------------------ BiCGStab kernels (FP64 vectors)
+
+// r^ = r (start and restarts)
+__global__ __launch_bounds__(kBlock) void k_copy(LevelView L, const double* __restrict__ src, double* __restrict__ dst,
+                                                 const CgState* st)
+{
+    NodeCtx c;
+    if (!decode<kRows0>(L, c)) return;
+    each_row<kRows0>(c, [&](const NodeCtx& n) {
+        if (n.in) dst[n.idx] = src[n.idx];
+    });
+}
+
+// p = r + beta (p - omega v)   (beta = 0: p = r)
+__global__ __launch_bounds__(kBlock) void k_bicg_p(LevelView L, const double* __restrict__ r, double* __restrict__ p,
+                                                   const double* __restrict__ v, const CgState* st)
+{
+    NodeCtx c;
+    if (!decode<kRows0>(L, c)) return;
+    const CgState& s0 = st[c.P->prob];
+    if (s0.done) return;
+    const double beta = s0.beta, omega = s0.omega;
+    each_row<kRows0>(c, [&](const NodeCtx& n) {
+        if (!n.in) return;
+        p[n.idx] = (beta == 0.0) ? r[n.idx] : r[n.idx] + beta * (p[n.idx] - omega * v[n.idx]);
+    });
+}
+
+// y = A xh on the free rows (0 on Dirichlet rows), xh in the preconditioner's
+// precision; partials: mode 0: (a, y); mode 1: (y, a) and (y, y)
+template <typename T, bool TENS>
+__global__ __launch_bounds__(kBlock) void k_bicg_apply(LevelView L, const T* __restrict__ xh, double* __restrict__ y,
+                                                       const double* __restrict__ a, int mode, double* partial,
+                                                       const CgState* st)
+{
+    NodeCtx c;
+    if (!decode<kRows0>(L, c)) return;
+    const LevelProb& P = *c.P;
+    if (st[P.prob].done) return;
+    double d0 = 0.0, d1 = 0.0;
+    each_row<kRows0>(c, [&](const NodeCtx& n) {
+        if (!n.in) return;
+        double yv = 0.0;
+        if (!is_dirichlet(P, n.i, n.j)) {
+            T xc, xw, xe, xs, xn, xsw, xne;
+            load7(xh, n.idx, P.pitch, xc, xw, xe, xs, xn, xsw, xne);
+            const Row<double> rc = row_coefs<double, TENS>(P, n.i, n.j, n.idx);
+            yv = row_apply<TENS>(rc, (double)xc, (double)xw, (double)xe, (double)xs, (double)xn, (double)xsw,
+                                 (double)xne);
+        }
+        y[n.idx] = yv;
+        d0 += yv * a[n.idx];
+        if (mode) d1 += yv * yv;
+    });
+    double v[2] = {d0, d1};
+    tile_partial<2>(v, partial, c.gt);
+}
+
+// s = r - alpha v
+__global__ __launch_bounds__(kBlock) void k_bicg_s(LevelView L, const double* __restrict__ r,
+                                                   const double* __restrict__ v, double* __restrict__ sv,
+                                                   const CgState* st)
+{
+    NodeCtx c;
+    if (!decode<kRows0>(L, c)) return;
+    const CgState& s0 = st[c.P->prob];
+    if (s0.done) return;
+    const double alpha = s0.alpha;
+    each_row<kRows0>(c, [&](const NodeCtx& n) {
+        if (n.in) sv[n.idx] = r[n.idx] - alpha * v[n.idx];
+    });
+}
+
+// x += alpha p^ + omega s^;  r = s - omega t;  partials (r^, r), (r, r)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bicg_x(LevelView L, double* __restrict__ x, double* __restrict__ r,
+                                                   const T* __restrict__ ph, const T* __restrict__ sh,
+                                                   const double* __restrict__ sv, const double* __restrict__ t,
+                                                   const double* __restrict__ rhat, double* partial, const CgState* st)
+{
+    NodeCtx c;
+    if (!decode<kRows0>(L, c)) return;
+    const CgState& s0 = st[c.P->prob];
+    if (s0.done) return;
+    const double alpha = s0.alpha, omega = s0.omega;
+    double d0 = 0.0, d1 = 0.0;
+    each_row<kRows0>(c, [&](const NodeCtx& n) {
+        if (!n.in) return;
+        const int64_t k = n.idx;
+        x[k] += alpha * (double)ph[k] + omega * (double)sh[k];
+        const double rv = sv[k] - omega * t[k];
+        r[k] = rv;
+        d0 += rhat[k] * rv;
+        d1 += rv * rv;
+    });
+    double v[2] = {d0, d1};
+    tile_partial<2>(v, partial, c.gt);
 }
 
 // ------------------------------------------------------------ multigrid node operations
@@ -847,7 +1002,7 @@ __device__ __forceinline__ void lds_sweep(const LevelProb& P, const LdsLevel<T>&
     for_nodes_split(
         P,
         [&](int i, int j) {
-            if (P.aniso) { // no common interior row
+            if (P.aniso || P.vel) { // no common interior row
                 general(i, j);
                 return;
             }
@@ -926,7 +1081,7 @@ __global__ __launch_bounds__(kBottomThreads) void k_bottom(BottomArgs<T> A, int 
             for_nodes_split(
                 P,
                 [&](int i, int j) {
-                    if (P.aniso) {
+                    if (P.aniso || P.vel) {
                         general(i, j);
                         return;
                     }
@@ -1119,6 +1274,7 @@ static void fill_level_prob(LevelProb& P, const eq_field_desc& d, int nx, int ny
     P.sa = d.dt * d.D * hy / (2.0 * hx);
     P.sb = d.dt * d.D * hx / (2.0 * hy);
     P.sxy = d.dt * d.D / 2.0;
+    P.cw = d.dt * hy / 24.0;
     if (d.scheme == EQ_SCHEME_FD) {
         // diffusionPETSc rows times hx*hy (the wall/corner halving is per node)
         P.fd = 1;
@@ -1159,7 +1315,7 @@ static void fill_level_prob(LevelProb& P, const eq_field_desc& d, int nx, int ny
 // pivoting inside the band is stable and costs O(n bw^2), and the n
 // solves O(n^2 bw), instead of O(n^3) for a dense elimination.
 static bool dense_inverse(const LevelProb& P, const double* h11, const double* h22, const double* h12,
-                          std::vector<double>& inv)
+                          const double* hv, std::vector<double>& inv)
 {
     const int nw = P.nx + 1, n = nw * (P.ny + 1);
     const int bw = nw + 1;
@@ -1174,6 +1330,7 @@ static bool dense_inverse(const LevelProb& P, const double* h11, const double* h
             // h11/h22/h12: the level's tensor in natural order (NULL: constant coefficients)
             Row<double> r = row_coefs_at<double>(P, i, j);
             if (h11) aniso_stiffness<double>(P, i, j, h11 + k, h22 + k, h12 + k, nw, r);
+            if (hv) add_convection<double>(P, i, j, hv, r); // hv: the level's row velocities
             auto put = [&](int ii, int jj, double v) {
                 if (ii < 0 || jj < 0 || ii > P.ny || jj > P.nx || v == 0.0) return;
                 if (is_dirichlet(P, ii, jj)) return;
@@ -1367,7 +1524,7 @@ static int build_hierarchy(eq_batch* b)
             }
             if (P.coarsest && mg) {
                 std::vector<double> inv;
-                if (!dense_inverse(P, nullptr, nullptr, nullptr, inv))
+                if (!dense_inverse(P, nullptr, nullptr, nullptr, nullptr, inv))
                     return set_error(EQ_ERR_INVALID, "singular coarse operator");
                 P.dense_off = (int64_t)dense_all.size();
                 for (double v : inv) dense_all.push_back((T)v); // column-major already
@@ -1396,15 +1553,20 @@ static int build_hierarchy(eq_batch* b)
     }
     size_t lds_max = 0;
     for (const Level& L : b->levels) lds_max = std::max(lds_max, L.bottom_lds_bytes);
-    if (lds_max > 64 * 1024)
+    if (lds_max > 64 * 1024) {
         EQ_HIP(hipFuncSetAttribute((const void*)k_bottom<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)lds_max));
         EQ_HIP(hipFuncSetAttribute((const void*)k_bottom<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)lds_max));
+    }
     const int64_t n0 = b->levels[0].pool;
+    const bool bicg = S.kind == EQ_SOLVER_GMG_BICGSTAB || S.kind == EQ_SOLVER_MIXED_GMG_BICGSTAB;
     if ((rc = alloc_zero(&b->u, n0, s)) || (rc = alloc_zero(&b->rhs, n0, s)) || (rc = alloc_zero(&b->r, n0, s)) ||
         (rc = alloc_zero(&b->p, n0, s)) || (rc = alloc_zero(&b->p2, n0, s)) || (rc = alloc_zero(&b->q, n0, s)) ||
         (rc = alloc_zero(&b->z, n0, s)) || (rc = alloc_zero(&b->xprev, n0, s)) ||
+        (bicg && ((rc = alloc_zero(&b->bs_rhat, n0, s)) || (rc = alloc_zero(&b->bs_p, n0, s)) ||
+                  (rc = alloc_zero(&b->bs_v, n0, s)) || (rc = alloc_zero(&b->bs_s, n0, s)) ||
+                  (rc = alloc_zero(&b->bs_t, n0, s)))) ||
         (rc = alloc_zero(&b->partial, 4 * (int64_t)b->levels[0].tiles_all.size(), s)))
         return rc;
     EQ_HIP(hipStreamSynchronize(s));
@@ -1417,8 +1579,10 @@ template <typename T, bool TENS>
 struct VCycle {
     eq_batch* b;
     BottomArgs<T> args;
+    const double* in = nullptr; // level-0 right-hand side (FP64): the vector to precondition
+    T* out = nullptr;           // level-0 result
 
-    explicit VCycle(eq_batch* bb) : b(bb)
+    explicit VCycle(eq_batch* bb) : b(bb), in(bb->r), out((T*)bb->z)
     {
         std::memset(&args, 0, sizeof(args));
         for (size_t l = 0; l < b->levels.size() && l < (size_t)kMaxLev; ++l) {
@@ -1467,7 +1631,7 @@ struct VCycle {
             const double nodes = L.nodes_smooth;
             // pre-smoothing; the first sweep starts from x = 0; level 0 reads
             // the FP64 residual and (mixed precision) keeps an FP32 copy of it
-            const T* bl = top ? (b->mixed ? (const T*)L.b : (const T*)b->r) : (const T*)L.b;
+            const T* bl = top ? (b->mixed ? (const T*)L.b : (const T*)in) : (const T*)L.b;
             T* bcopy = (top && b->mixed) ? (T*)L.b : nullptr;
             const double bytes_b = top ? sizeof(double) : sizeof(T);
             // Small levels run at the launch-latency floor: there the fused
@@ -1479,12 +1643,12 @@ struct VCycle {
             if (S.mg_pre >= 2 && fuse) {
                 // sweeps 1 and 2 in one launch
                 if (top)
-                    k_presmooth2<R, T, double, TENS><<<gs, kBlock, 0, s>>>(L.smooth(), b->r, X[1], bcopy, omega, st);
+                    k_presmooth2<R, T, double, TENS><<<gs, kBlock, 0, s>>>(L.smooth(), in, X[1], bcopy, omega, st);
                 else
                     k_presmooth2<R, T, T, TENS><<<gs, kBlock, 0, s>>>(L.smooth(), bl, X[1], nullptr, omega, st);
                 cur = 1;
             } else if (top) {
-                k_smooth<R, T, double, T, TENS><<<gs, kBlock, 0, s>>>(L.smooth(), b->r, nullptr, X[0], bcopy, omega, 1, 0,
+                k_smooth<R, T, double, T, TENS><<<gs, kBlock, 0, s>>>(L.smooth(), in, nullptr, X[0], bcopy, omega, 1, 0,
                                                                 nullptr, st);
             } else {
                 k_smooth<R, T, T, T, TENS><<<gs, kBlock, 0, s>>>(L.smooth(), bl, nullptr, X[0], nullptr, omega, 1, 0, nullptr,
@@ -1520,10 +1684,10 @@ struct VCycle {
                 if (top) prof_begin(b, EQ_KERNEL_SMOOTH);
                 if (top && last) {
                     if (first)
-                        k_prolong_smooth<R, T, double, T, TENS><<<gs, kBlock, 0, s>>>(L.smooth(), C.d_probs, xc, b->r, X[cur],
-                                                                                (T*)b->z, omega, 1, b->partial, st);
+                        k_prolong_smooth<R, T, double, T, TENS><<<gs, kBlock, 0, s>>>(L.smooth(), C.d_probs, xc, in, X[cur],
+                                                                                out, omega, 1, b->partial, st);
                     else
-                        k_smooth<R, T, double, T, TENS><<<gs, kBlock, 0, s>>>(L.smooth(), b->r, X[cur], (T*)b->z, nullptr,
+                        k_smooth<R, T, double, T, TENS><<<gs, kBlock, 0, s>>>(L.smooth(), in, X[cur], out, nullptr,
                                                                         omega, 0, 1, b->partial, st);
                 } else {
                     if (first)
@@ -1538,16 +1702,18 @@ struct VCycle {
             }
         }
         if (!L.bottom.empty()) {
-            T* out = final_buffer(l);
+            // the bottom solve leaves the level's correction in xl; on level 0
+            // it is copied to `out` with the partial (in, out)
+            T* xl = final_buffer(l);
             if (top) {
                 const int nb = (int)L.tiles_bottom.size();
-                k_cast_in<T><<<grid_for(nb), kBlock, 0, s>>>(L.bottom_tiles(), b->r, (T*)L.b, st);
+                k_cast_in<T><<<grid_for(nb), kBlock, 0, s>>>(L.bottom_tiles(), in, (T*)L.b, st);
             }
-            k_bottom<T, TENS><<<(int)L.bottom.size(), kBottomThreads, L.bottom_lds_bytes, s>>>(args, l, L.d_bottom, out, omega, S.mg_pre,
+            k_bottom<T, TENS><<<(int)L.bottom.size(), kBottomThreads, L.bottom_lds_bytes, s>>>(args, l, L.d_bottom, xl, omega, S.mg_pre,
                                                                         S.mg_post, (const T*)b->dense, st);
             if (top) {
                 const int nb = (int)L.tiles_bottom.size();
-                k_cast_out<T, T><<<grid_for(nb), kBlock, 0, s>>>(L.bottom_tiles(), out, b->r, (T*)b->z, b->partial, st);
+                k_cast_out<T, T><<<grid_for(nb), kBlock, 0, s>>>(L.bottom_tiles(), xl, in, out, b->partial, st);
             }
         }
     }
@@ -1696,9 +1862,139 @@ using namespace eqk;
 
 // ------------------------------------------------------------ C ABI
 
+// Right-preconditioned BiCGStab for the nonsymmetric (advection) operator:
+// two V-cycles and two operator applications per iteration, all vectors FP64
+// except the preconditioned directions p^, s^ (the V-cycle's precision).
+template <typename T, bool TENS>
+static int solve_bicgstab(eq_batch* b, eq_step_stats* stats)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    const eq_solver_desc& S = b->solver;
+    Level& L0 = b->levels[0];
+    const LevelView A = L0.all();
+    const int g0 = grid_for(A.ntiles);
+    hipStream_t s = b->stream;
+    CgState* st = b->d_state;
+    VCycle<T, TENS> vp(b), vsc(b);
+    vp.in = b->bs_p;
+    vp.out = (T*)b->p;   // p^
+    vsc.in = b->bs_s;
+    vsc.out = (T*)b->p2; // s^
+    auto finish = [&](int op) {
+        k_finish<<<b->nprob, kFinishThreads, 0, s>>>(L0.d_probs, b->partial, st, op, 0);
+    };
+    auto iteration = [&]() {
+        k_bicg_p<<<g0, kBlock, 0, s>>>(A, b->r, b->bs_p, b->bs_v, st);
+        vp.run0();
+        prof_begin(b, EQ_KERNEL_STENCIL64);
+        k_bicg_apply<T, TENS><<<g0, kBlock, 0, s>>>(A, (const T*)b->p, b->bs_v, b->bs_rhat, 0, b->partial, st);
+        prof_end(b, EQ_KERNEL_STENCIL64, L0.nodes_all * (sizeof(T) + 2 * sizeof(double)));
+        finish(kFinBAlpha);
+        k_bicg_s<<<g0, kBlock, 0, s>>>(A, b->r, b->bs_v, b->bs_s, st);
+        vsc.run0();
+        k_bicg_apply<T, TENS><<<g0, kBlock, 0, s>>>(A, (const T*)b->p2, b->bs_t, b->bs_s, 1, b->partial, st);
+        finish(kFinBOmega);
+        prof_begin(b, EQ_KERNEL_CG_UPDATE);
+        k_bicg_x<T><<<g0, kBlock, 0, s>>>(A, b->u, b->r, (const T*)b->p, (const T*)b->p2, b->bs_s, b->bs_t,
+                                          b->bs_rhat, b->partial, st);
+        prof_end(b, EQ_KERNEL_CG_UPDATE, L0.nodes_all * (2 * sizeof(T) + 6 * sizeof(double)));
+        finish(kFinBUpdate);
+    };
+    auto setup = [&]() {
+        k_rhs<<<g0, kBlock, 0, s>>>(A, b->u, b->rhs, b->partial);
+        finish(kFinRhs);
+        k_init<TENS><<<g0, kBlock, 0, s>>>(A, b->u, b->xprev, b->rhs, b->r, b->z, b->partial, st, 0);
+        finish(kFinBInit);
+        k_copy<<<g0, kBlock, 0, s>>>(A, b->r, b->bs_rhat, st);
+    };
+    bool profiling = false;
+    for (int k = 0; k < EQ_KERNEL_COUNT_; ++k) profiling = profiling || b->prof[k].enabled;
+    const bool graphs = S.use_graphs && !profiling;
+    auto capture = [&](hipGraphExec_t* exec, auto&& body) -> int {
+        if (*exec) return EQ_OK;
+        hipGraph_t g = nullptr;
+        EQ_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        body();
+        const hipError_t e = hipStreamEndCapture(s, &g);
+        if (e != hipSuccess) return hip_check(e, "hipStreamEndCapture");
+        const hipError_t e2 = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (e2 != hipSuccess) {
+            *exec = nullptr;
+            return hip_check(e2, "hipGraphInstantiate");
+        }
+        return EQ_OK;
+    };
+    int rc;
+    if (graphs) {
+        if ((rc = capture(&b->g_init, setup)) || (rc = capture(&b->g_bicg, iteration))) return rc;
+        EQ_HIP(hipGraphLaunch(b->g_init, s));
+    } else {
+        setup();
+    }
+    EQ_HIP(hipGetLastError());
+    int chunk = std::max(1, b->last_iters > 0 ? b->last_iters : S.check_every);
+    for (int guard = 0; guard < 1000000; ++guard) {
+        for (int k = 0; k < chunk; ++k) {
+            if (graphs) {
+                EQ_HIP(hipGraphLaunch(b->g_bicg, s));
+            } else {
+                iteration();
+            }
+        }
+        EQ_HIP(hipGetLastError());
+        if ((rc = read_states(b))) return rc;
+        bool all = true;
+        for (int p = 0; p < b->nprob; ++p) all = all && b->h_state[p].done;
+        if (all) {
+            // confirm on the true residual; the others restart from r^ = r
+            k_true_residual<TENS><<<g0, kBlock, 0, s>>>(A, b->u, b->rhs, b->r, b->partial);
+            finish(kFinTrue);
+            k_copy<<<g0, kBlock, 0, s>>>(A, b->r, b->bs_rhat, st);
+            finish(kFinBRestart);
+            if ((rc = read_states(b))) return rc;
+            all = true;
+            for (int p = 0; p < b->nprob; ++p) all = all && b->h_state[p].done;
+            if (all) break;
+        }
+        chunk = std::max(1, S.check_every);
+    }
+    k_keep<<<g0, kBlock, 0, s>>>(A, b->u, b->xprev, st);
+    EQ_HIP(hipGetLastError());
+    prof_collect(b);
+    int32_t itmax = 0, nconv = 0;
+    double relmax = 0.0;
+    for (int p = 0; p < b->nprob; ++p) {
+        const CgState& c = b->h_state[p];
+        const double rel = c.bb > 0 ? std::sqrt(c.rr_true / c.bb) : std::sqrt(c.rr_true);
+        b->iters[p] = c.iter;
+        b->relres[p] = rel;
+        itmax = std::max(itmax, c.iter);
+        relmax = std::max(relmax, rel);
+        if (c.rr_true <= c.tol2 * c.bb) ++nconv;
+    }
+    b->last_iters = itmax;
+    if (stats) {
+        stats->iterations_max = itmax;
+        stats->nconverged = nconv;
+        stats->relres_max = relmax;
+        stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return EQ_OK;
+}
+
 // precision of the multigrid x whether any problem carries a tensor field
 static int run_solve(eq_batch* b, eq_step_stats* stats)
 {
+    const bool bicg = b->solver.kind == EQ_SOLVER_GMG_BICGSTAB || b->solver.kind == EQ_SOLVER_MIXED_GMG_BICGSTAB;
+    if (!bicg)
+        for (const LevelProb& P : b->levels[0].probs)
+            if (P.vel) return set_error(EQ_ERR_INVALID, "advection makes the system nonsymmetric: use a BiCGStab solver");
+    if (bicg) {
+        if (b->mixed)
+            return b->any_tensor ? solve_bicgstab<float, true>(b, stats) : solve_bicgstab<float, false>(b, stats);
+        return b->any_tensor ? solve_bicgstab<double, true>(b, stats) : solve_bicgstab<double, false>(b, stats);
+    }
     if (b->mixed) return b->any_tensor ? solve_all<float, true>(b, stats) : solve_all<float, false>(b, stats);
     return b->any_tensor ? solve_all<double, true>(b, stats) : solve_all<double, false>(b, stats);
 }
@@ -1740,7 +2036,7 @@ int eq_batch_create(const eq_field_desc* descs, int32_t nprob, int32_t nspecies,
     }
     eq_solver_desc S;
     if (solver) S = *solver; else eq_solver_default(&S);
-    if (S.kind < EQ_SOLVER_JACOBI_CG || S.kind > EQ_SOLVER_MIXED_GMG_CG) return set_error(EQ_ERR_INVALID, "unknown solver");
+    if (S.kind < EQ_SOLVER_JACOBI_CG || S.kind > EQ_SOLVER_MIXED_GMG_BICGSTAB) return set_error(EQ_ERR_INVALID, "unknown solver");
     if (!(S.rtol > 0) || S.max_iter < 1 || S.check_every < 1) return set_error(EQ_ERR_INVALID, "bad solver tolerances");
     if (S.kind != EQ_SOLVER_JACOBI_CG &&
         (S.mg_pre < 1 || S.mg_post < 1 || !(S.mg_omega > 0) || S.mg_coarse_max_nodes < 4 || S.mg_max_levels < 1))
@@ -1763,7 +2059,7 @@ int eq_batch_create(const eq_field_desc* descs, int32_t nprob, int32_t nspecies,
     b->ntraps = nprob / nspecies;
     b->descs.assign(descs, descs + nprob);
     b->solver = S;
-    b->mixed = (S.kind == EQ_SOLVER_MIXED_GMG_CG) ? 1 : 0;
+    b->mixed = (S.kind == EQ_SOLVER_MIXED_GMG_CG || S.kind == EQ_SOLVER_MIXED_GMG_BICGSTAB) ? 1 : 0;
     b->iters.assign(nprob, 0);
     b->relres.assign(nprob, 0.0);
     int rc = EQ_OK;
@@ -1831,7 +2127,14 @@ void eq_batch_destroy(eq_batch* b)
     hipFree(b->p);
     hipFree(b->p2);
     hipFree(b->xprev);
+    hipFree(b->bs_rhat);
+    hipFree(b->bs_p);
+    hipFree(b->bs_v);
+    hipFree(b->bs_s);
+    hipFree(b->bs_t);
+    if (b->g_bicg) hipGraphExecDestroy(b->g_bicg);
     if (b->mixed) hipFree(b->tens0_d); // else it is level 0's own tensor array
+    for (void* v : b->vel_bufs) hipFree(v);
     hipFree(b->q);
     hipFree(b->z);
     hipFree(b->partial);
@@ -2023,7 +2326,9 @@ static int set_tensor_impl(eq_batch* b, int32_t prob, const double* d11, const d
         }
         if (P.coarsest && b->solver.kind != EQ_SOLVER_JACOBI_CG && b->dense) {
             if (!dense_inverse(P, on ? cur[0].data() : nullptr, on ? cur[1].data() : nullptr,
-                               on ? cur[2].data() : nullptr, inv))
+                               on ? cur[2].data() : nullptr,
+                               (P.vel && (int)b->vel_host[prob].size() > l) ? b->vel_host[prob][l].data() : nullptr,
+                               inv))
                 return set_error(EQ_ERR_INVALID, "singular coarse operator (tensor field not positive definite?)");
             std::vector<T> invT(inv.size());
             for (size_t q = 0; q < inv.size(); ++q) invT[q] = (T)inv[q];
@@ -2040,6 +2345,23 @@ static int set_tensor_impl(eq_batch* b, int32_t prob, const double* d11, const d
 }
 
 } // extern "C++"
+
+// The kernels specialise on whether any problem has per-node rows (tensor
+// field or advection); a change of that drops the captured graphs.
+static void refresh_general(eq_batch* b)
+{
+    bool any = false;
+    for (const LevelProb& P : b->levels[0].probs) any = any || P.aniso || P.vel;
+    if (any != b->any_tensor) {
+        b->any_tensor = any;
+        if (b->g_iter2) hipGraphExecDestroy(b->g_iter2);
+        if (b->g_init) hipGraphExecDestroy(b->g_init);
+        if (b->g_bicg) hipGraphExecDestroy(b->g_bicg);
+        b->g_iter2 = nullptr;
+        b->g_init = nullptr;
+        b->g_bicg = nullptr;
+    }
+}
 
 static int set_tensor_checked(eq_batch* b, int32_t prob, const double* D11, const double* D22, const double* D12,
                               size_t n)
@@ -2060,19 +2382,88 @@ int eq_batch_set_tensor(eq_batch* b, int32_t prob, const double* D11, const doub
 {
     int rc = set_tensor_checked(b, prob, D11, D22, D12, n);
     if (rc != EQ_OK || !b) return rc;
-    // the kernels specialise on whether any problem carries a tensor
-    bool any = false;
-    for (const LevelProb& P : b->levels[0].probs) any = any || P.aniso;
-    if (any != b->any_tensor) {
-        b->any_tensor = any;
-        if (b->g_iter2) hipGraphExecDestroy(b->g_iter2);
-        if (b->g_init) hipGraphExecDestroy(b->g_init);
-        b->g_iter2 = nullptr;
-        b->g_init = nullptr;
-    }
+    if (D11 && b->levels[0].probs[prob].vel)
+        return set_error(EQ_ERR_INVALID, "advection and a tensor field together are not supported");
+    refresh_general(b);
     return EQ_OK;
 }
 
+
+// x-velocity per grid row of one problem (advection, Galerkin P1), or none.
+// Coarse levels take the nearest fine row; the coarsest inverse is rebuilt.
+extern "C++" {
+template <typename T>
+static int set_velocity_impl(eq_batch* b, int32_t prob, const double* vx)
+{
+    hipStream_t s = b->stream;
+    EQ_HIP(hipStreamSynchronize(s));
+    const int nlev = (int)b->levels.size();
+    if ((int)b->vel_host.size() < b->nprob) b->vel_host.resize(b->nprob);
+    std::vector<std::vector<double>>& rows = b->vel_host[prob];
+    rows.clear();
+    const LevelProb* Pf = nullptr;
+    std::vector<double> inv;
+    for (int l = 0; l < nlev; ++l) {
+        Level& L = b->levels[l];
+        int pi = -1;
+        for (int k = 0; k < (int)L.probs.size(); ++k)
+            if (L.probs[k].prob == prob) pi = k;
+        if (pi < 0) break;
+        LevelProb& P = L.probs[pi];
+        if (vx) {
+            std::vector<double> r((size_t)P.ny + 1);
+            for (int I = 0; I <= P.ny; ++I)
+                r[I] = l == 0 ? vx[I] : rows[l - 1][(size_t)((2LL * I * Pf->ny + P.ny) / (2LL * P.ny))];
+            rows.push_back(r);
+            std::vector<T> rt(r.begin(), r.end());
+            void* dT = nullptr;
+            if (hipMalloc(&dT, sizeof(T) * rt.size()) != hipSuccess) return set_error(EQ_ERR_NOMEM, "velocity rows");
+            b->vel_bufs.push_back(dT);
+            EQ_HIP(hipMemcpy(dT, rt.data(), sizeof(T) * rt.size(), hipMemcpyHostToDevice));
+            double* dD = nullptr;
+            if (l == 0 && b->mixed) {
+                if (hipMalloc((void**)&dD, sizeof(double) * r.size()) != hipSuccess)
+                    return set_error(EQ_ERR_NOMEM, "velocity rows");
+                b->vel_bufs.push_back(dD);
+                EQ_HIP(hipMemcpy(dD, r.data(), sizeof(double) * r.size(), hipMemcpyHostToDevice));
+            }
+            P.vel = 1;
+            P.vel_f = b->mixed ? (const float*)dT : nullptr;
+            P.vel_d = b->mixed ? dD : (const double*)dT;
+        } else {
+            P.vel = 0;
+            P.vel_f = nullptr;
+            P.vel_d = nullptr;
+        }
+        if (P.coarsest && b->solver.kind != EQ_SOLVER_JACOBI_CG && b->dense) {
+            // the tensor field, if any, is not kept on the host: velocity and
+            // tensor together keep the tensor's coarsest inverse out of date
+            if (!dense_inverse(P, nullptr, nullptr, nullptr, vx ? rows.back().data() : nullptr, inv))
+                return set_error(EQ_ERR_INVALID, "singular coarse operator");
+            std::vector<T> invT(inv.begin(), inv.end());
+            EQ_HIP(hipMemcpy((T*)b->dense + P.dense_off, invT.data(), sizeof(T) * invT.size(), hipMemcpyHostToDevice));
+        }
+        EQ_HIP(hipMemcpy(L.d_probs, L.probs.data(), sizeof(LevelProb) * L.probs.size(), hipMemcpyHostToDevice));
+        Pf = &P;
+    }
+    return EQ_OK;
+}
+} // extern "C++"
+
+int eq_batch_set_velocity(eq_batch* b, int32_t prob, const double* vx, size_t n)
+{
+    if (!b) return set_error(EQ_ERR_INVALID, "NULL batch");
+    if (prob < 0 || prob >= b->nprob) return set_error(EQ_ERR_INVALID, "problem index out of range");
+    const LevelProb& P = b->levels[0].probs[prob];
+    if (vx && n != (size_t)P.ny + 1) return set_error(EQ_ERR_INVALID, "length must be ny+1");
+    if (vx && P.fd) return set_error(EQ_ERR_INVALID, "the finite-difference scheme has no advection");
+    if (vx && P.aniso) return set_error(EQ_ERR_INVALID, "advection and a tensor field together are not supported");
+    hipSetDevice(b->device);
+    const int rc = b->mixed ? set_velocity_impl<float>(b, prob, vx) : set_velocity_impl<double>(b, prob, vx);
+    if (rc) return rc;
+    refresh_general(b);
+    return EQ_OK;
+}
 
 int eq_field_set_tensor(eq_field* f, const double* D11, const double* D22, const double* D12, size_t n)
 {

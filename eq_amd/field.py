@@ -21,6 +21,7 @@ SCHEME_P1, SCHEME_FD = 0, 1
 NEUMANN, DIRICHLET, ROBIN = 0, 1, 2
 LEFT, RIGHT, TOP, BOTTOM = 0, 1, 2, 3
 JACOBI_CG, GMG_CG, MIXED_GMG_CG = 0, 1, 2
+GMG_BICGSTAB, MIXED_GMG_BICGSTAB = 3, 4
 SAMPLE, DEPOSIT = 1, 2
 KERNEL_STENCIL64, KERNEL_SMOOTH, KERNEL_CG_UPDATE, KERNEL_COUPLE = 0, 1, 2, 3
 
@@ -34,7 +35,7 @@ EXPORTED = [
     "eq_batch_stream", "eq_batch_sync", "eq_batch_upload", "eq_batch_download",
     "eq_batch_upload_dev", "eq_batch_download_dev", "eq_batch_step", "eq_batch_problem_stats",
     "eq_batch_couple", "eq_batch_couple_dev", "eq_batch_couple_traps_dev", "eq_batch_autotune",
-    "eq_batch_set_tensor", "eq_field_set_tensor", "eq_batch_set_wall_values", "eq_batch_set_channels",
+    "eq_batch_set_tensor", "eq_field_set_tensor", "eq_batch_set_velocity", "eq_batch_set_wall_values", "eq_batch_set_channels",
     "eq_batch_channel_values",
     "eq_batch_boundary_flux",
     "eq_batch_vertex_map",
@@ -128,6 +129,7 @@ def load_library():
     L.eq_batch_couple.argtypes = [vp, C.c_int32, vp, C.c_size_t, C.c_double, vp, vp, vp, C.c_uint32]
     L.eq_batch_autotune.argtypes = [vp, C.c_int32, P(SolverDesc)]
     L.eq_batch_set_tensor.argtypes = [vp, C.c_int32, vp, vp, vp, C.c_size_t]
+    L.eq_batch_set_velocity.argtypes = [vp, C.c_int32, vp, C.c_size_t]
     L.eq_batch_set_wall_values.argtypes = [vp, C.c_int32, C.c_int32, vp, C.c_size_t]
     L.eq_batch_set_channels.argtypes = [vp, C.c_int32, P(ChannelDesc), P(ChannelDesc)]
     L.eq_batch_channel_values.argtypes = [vp, C.c_int32, vp, vp]
@@ -313,6 +315,14 @@ class FieldBatch:
         t, b = np.empty(nx + 1), np.empty(nx + 1)
         _check(load_library().eq_batch_channel_values(self._h, prob, _ptr(t), _ptr(b)))
         return t, b
+
+    def set_velocity(self, prob: int, vx_rows=None):
+        """x-velocity per grid row (ny+1 values) for advection, or None."""
+        if vx_rows is None:
+            _check(load_library().eq_batch_set_velocity(self._h, prob, None, 0))
+            return
+        a = np.ascontiguousarray(vx_rows, dtype=np.float64)
+        _check(load_library().eq_batch_set_velocity(self._h, prob, _ptr(a), a.size))
 
     def set_tensor(self, prob: int, D11=None, D22=None, D12=None):
         """Per-node diffusion tensor (natural order), or constant coefficients when None."""

@@ -19,12 +19,14 @@
 
 #include <cmath>
 #include <cstdint>
+#include <filesystem>
 #include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "../../include/eq_field.h"
+#include "field_io.hpp"
 
 namespace eqgpu {
 
@@ -57,6 +59,8 @@ public:
         nx_ = (int)std::ceil(base.trapWidthMicrons * base.nodesPerMicron);
         ny_ = (int)std::ceil(base.trapHeightMicrons * base.nodesPerMicron);
         npm_ = base.nodesPerMicron;
+        width_ = base.trapWidthMicrons;
+        height_ = base.trapHeightMicrons;
         std::vector<eq_field_desc> descs(D.size());
         for (size_t s = 0; s < D.size(); ++s) {
             eq_field_desc& d = descs[s];
@@ -77,6 +81,10 @@ public:
     }
 
     int32_t species() const { return eq_batch_num_problems(batch_.get()); }
+    int nx() const { return nx_; }
+    int ny() const { return ny_; }
+    double width() const { return width_; }
+    double height() const { return height_; }
     size_t nodes() const { return (size_t)(nx_ + 1) * (size_t)(ny_ + 1); }
     eq_batch* handle() const { return batch_.get(); }
 
@@ -186,7 +194,7 @@ private:
     };
     std::unique_ptr<eq_batch, deleter> batch_;
     int nx_ = 0, ny_ = 0;
-    double npm_ = 1.0;
+    double npm_ = 1.0, width_ = 0.0, height_ = 0.0;
 };
 
 // The per-species surface the reference host uses.
@@ -221,6 +229,9 @@ public:
         D22 = std::make_shared<std::vector<double>>(n, 1.0);
         D12 = std::make_shared<std::vector<double>>(n, 0.0);
         field_->attachTensor(s_, &D11, &D22, &D12);
+        // dolfin::File(comm, filePath, "compressed") (src/fHSL.cpp:606-613)
+        if (!p.filePath.empty())
+            files_ = eqio::pvdSeries(p.filePath, field_->nx(), field_->ny(), field_->width(), field_->height());
     }
 
     // u0 <- solution_vector; solve; solution_vector <- u; boundary flux
@@ -230,7 +241,12 @@ public:
         field_->stepSpecies(s_);
         totalBoundaryFlux = field_->boundaryFlux(s_);
     }
-    void writeDiffusionFiles(double) {}
+    // one .vtu of solution_vector per call, listed in the .pvd at timeStamp
+    // (src/fHSL.cpp:628-634)
+    void writeDiffusionFiles(double timeStamp)
+    {
+        if (files_.open()) files_.write(solution_vector.data(), timeStamp);
+    }
     void finalize() {}
 
     std::shared_ptr<shellView> shell;
@@ -242,6 +258,7 @@ private:
     std::shared_ptr<trapField> field_;
     int s_ = 0;
     solverParams params_;
+    eqio::pvdSeries files_;
 };
 
 // diffusionPETSc (diffuclass.h:45-108): the finite-difference alternative
@@ -278,6 +295,15 @@ public:
         check(eq_batch_create(&d, 1, 1, solver, &b), "eq_batch_create (FD)");
         batch_.reset(b);
         solution_vector.assign((size_t)(d.nx + 1) * (d.ny + 1), 0.0);
+        nx_ = d.nx;
+        ny_ = d.ny;
+        h_ = h;
+        step_ = 0;
+        // output folder filePath + "petsc", objects named "grid" (diffuclass.cpp:62-66)
+        if (!p.filePath.empty()) {
+            dir_ = p.filePath + "petsc";
+            std::filesystem::create_directories(dir_);
+        }
     }
     // TimeStep: boundary values into the right-hand side, solve, read back
     // (diffuclass.cpp:107-117, 405-413)
@@ -288,10 +314,16 @@ public:
         check(eq_batch_step(batch_.get(), &st), "eq_batch_step (FD)");
         last_ = st;
         check(eq_batch_download(batch_.get(), 0, solution_vector.data(), solution_vector.size()), "download");
+        ++step_;
     }
     double getDiffusionConstant() const { return params_.D_HSL; }
     const eq_step_stats& lastStats() const { return last_; }
-    void writeDiffusionFiles(double) {}
+    // RecordData: "<filePath>petsc/grid%04d.vtr" numbered by the steps taken
+    // (diffuclass.cpp:168-172, 401-408, 560-583)
+    void writeDiffusionFiles(double)
+    {
+        if (!dir_.empty()) eqio::write_vtr(dir_, "grid", step_, nx_, ny_, h_, solution_vector.data());
+    }
     void finalize() {}
 
     std::vector<double> solution_vector;
@@ -303,6 +335,9 @@ private:
     std::unique_ptr<eq_batch, deleter> batch_;
     solverParams params_;
     eq_step_stats last_{};
+    std::string dir_;
+    int nx_ = 0, ny_ = 0, step_ = 0;
+    double h_ = 1.0;
 };
 
 } // namespace eqgpu

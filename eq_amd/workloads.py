@@ -149,6 +149,21 @@ def config3_traps(ntraps: int = 512, seed: int = 3):
     return traps
 
 
+def config4() -> Trap:
+    """204.8x102.4 um trap, 8192x4096 cells (h=0.025, 33 566 721 nodes/species),
+    4 species D=400,250,150,100, gamma 0.01, 100 000 static rods seed 4; the
+    fields start at zero (u0 left empty: nothing to upload)."""
+    rng = SplitMix64(4)
+    cells, rates = seeded_cells(rng, 100000, 204.8, 102.4, 4)
+    return Trap(8192, 4096, 204.8, 102.4, 40.0, [400.0, 250.0, 150.0, 100.0], [0.01] * 4, cells, rates, [])
+
+
+def channel_velocity(ny: int, height: float, vmax: float = 50.0) -> np.ndarray:
+    """Parabolic x-velocity per grid row, zero at both walls, vmax mid-height."""
+    y = np.arange(ny + 1) * (height / ny)
+    return 4.0 * vmax * y * (height - y) / (height * height)
+
+
 def descs_for(trap: Trap, dt: float = 0.1, walls=None):
     """eq_field_desc-style tuples (nx, ny, W, H, dt, D, gamma, walls) per species."""
     walls = TOP_DIRICHLET if walls is None else walls

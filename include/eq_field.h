@@ -80,7 +80,9 @@ typedef struct {
 typedef enum {
     EQ_SOLVER_JACOBI_CG = 0,    /* FP64 CG, Jacobi preconditioner                    */
     EQ_SOLVER_GMG_CG = 1,       /* FP64 CG, FP64 geometric-multigrid V-cycle         */
-    EQ_SOLVER_MIXED_GMG_CG = 2  /* FP64 CG, FP32 V-cycle (FP64 residual/refinement)  */
+    EQ_SOLVER_MIXED_GMG_CG = 2, /* FP64 CG, FP32 V-cycle (FP64 residual/refinement)  */
+    EQ_SOLVER_GMG_BICGSTAB = 3, /* FP64 BiCGStab, FP64 V-cycle (nonsymmetric: advection) */
+    EQ_SOLVER_MIXED_GMG_BICGSTAB = 4 /* FP64 BiCGStab, FP32 V-cycle                   */
 } eq_solver_kind;
 
 typedef struct {
@@ -190,6 +192,12 @@ int eq_batch_channel_values(eq_batch* b, int32_t prob, double* top, double* bott
  * arrays; call between steps. */
 int eq_batch_set_tensor(eq_batch* b, int32_t prob, const double* D11, const double* D22, const double* D12,
                         size_t n);
+
+/* Advection (no counterpart in the reference's trap form; the channel-flow case of
+ * the config-4 workload): x-velocity per grid row, vx[i] for row i (n = ny+1),
+ * P1 in y, adding dt * int (v du/dx) phi (Galerkin) to the operator.  The system
+ * is then nonsymmetric: solve it with a BiCGStab solver kind.  NULL removes it. */
+int eq_batch_set_velocity(eq_batch* b, int32_t prob, const double* vx, size_t n);
 
 /* Autotuner (no counterpart in the reference, which has one fixed direct solve):
  * times `repeats` steps of each candidate multigrid cycle shape (pre/post sweeps

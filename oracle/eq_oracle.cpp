@@ -94,6 +94,24 @@ void element_matrix(const double xc[3], const double yc[3], const double d11[3],
             Ae[i][j] = mscale * adet * (i == j ? 1.0 / 12.0 : 1.0 / 24.0) + S[i][j];
 }
 
+/* Galerkin convection element matrix C_e[i][j] = int_T v dphi_j/dx phi_i with v
+ * P1 (vertex values vv): int_T v phi_i = |T|/12 (v_i + v_0 + v_1 + v_2) exactly,
+ * and dphi_j/dx constant on T */
+void convection_matrix(const double xc[3], const double yc[3], const double vv[3], double Ce[3][3])
+{
+    const double J00 = xc[1] - xc[0], J01 = xc[2] - xc[0];
+    const double J10 = yc[1] - yc[0], J11 = yc[2] - yc[0];
+    const double det = J00 * J11 - J01 * J10;
+    const double adet = std::fabs(det);
+    const double K00 = J11 / det, K10 = -J10 / det;
+    const double gX[3] = {-1.0, 1.0, 0.0}, gY[3] = {-1.0, 0.0, 1.0};
+    const double vs = vv[0] + vv[1] + vv[2];
+    for (int i = 0; i < 3; ++i) {
+        const double Ii = 0.5 * adet / 12.0 * (vv[i] + vs);
+        for (int j = 0; j < 3; ++j) Ce[i][j] = (K00 * gX[j] + K10 * gY[j]) * Ii;
+    }
+}
+
 /* wall predicates on computed vertex coordinates, DirichletBoundary_TrapEdge::inside
  * (src/fHSL.h:575-590) with near(x, v) = |x - v| < DOLFIN_EPS [ext] */
 const double kDolfinEps = 3.0e-16;
@@ -181,6 +199,13 @@ int64_t orc_assemble(const orc_problem* p, const double* D11, const double* D22,
                 }
                 double Ae[3][3];
                 element_matrix(xc, yc, d11, d22, d12, p->D, p->dt, p->gamma, Ae);
+                if (p->vx) {
+                    double vv[3], Ce[3][3];
+                    for (int c = 0; c < 3; ++c) vv[c] = p->vx[t[k].v[c] / (p->nx + 1)];
+                    convection_matrix(xc, yc, vv, Ce);
+                    for (int a = 0; a < 3; ++a)
+                        for (int b = 0; b < 3; ++b) Ae[a][b] += p->dt * Ce[a][b];
+                }
                 for (int a = 0; a < 3; ++a)
                     for (int b = 0; b < 3; ++b) add(t[k].v[a], t[k].v[b], Ae[a][b]);
             }
@@ -604,6 +629,8 @@ int orc_step(const orc_problem* p, const double* D11, const double* D22, const d
             info->relres = bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
         }
         std::copy(x.begin(), x.end(), u);
+    } else if (p->vx) {
+        rc = 3; /* CG needs a symmetric operator */
     } else {
         rc = orc_solve_pcg(p, rowptr.data(), col.data(), val.data(), b.data(), u, rtol, maxit > 0 ? maxit : 1000000, info);
     }

@@ -61,6 +61,11 @@ typedef struct {
      * Function instead of a Constant, as the channel-coupled walls are
      * (src/fHSL.cpp:514, 531) */
     const double* gwall[4];
+    /* optional advection (the survey's config-4 channel flow; the reference's trap
+     * form has none): x-velocity per grid row (ny+1 values), P1 in y, entering
+     * as dt * int (v du/dx) phi (Galerkin) -- a nonsymmetric operator; the LU
+     * path solves it, PCG refuses it */
+    const double* vx;
 } orc_problem;
 
 /* flow channel above or below the trap (fenicsChannel, src/fHSL.h:240-332) */

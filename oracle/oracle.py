@@ -31,7 +31,19 @@ class Problem(C.Structure):
     _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("width", C.c_double),
                 ("height", C.c_double), ("dt", C.c_double), ("D", C.c_double),
                 ("gamma", C.c_double), ("f", C.c_double), ("wall", Bc * 4),
-                ("gwall", C.c_void_p * 4)]
+                ("gwall", C.c_void_p * 4), ("vx", C.c_void_p)]
+
+    def set_velocity(self, vx_rows):
+        """x-velocity per grid row (ny+1 values), or None."""
+        if not hasattr(self, "_keep"):
+            self._keep = {}
+        if vx_rows is None:
+            self.vx = None
+            self._keep.pop("vx", None)
+            return
+        a = np.ascontiguousarray(vx_rows, dtype=np.float64).copy()
+        self._keep["vx"] = a
+        self.vx = a.ctypes.data
 
     def set_wall_values(self, wall: int, values):
         """Per-node Dirichlet values along a wall (None: the wall's constant)."""

@@ -2,7 +2,7 @@
 // through eqgpu::fenicsInterface / eqgpu::trapField (eq_amd/host/fhsl_gpu.hpp)
 // in the order of Simulation::stepSimulation (src/simulation.cpp:414-510).
 //
-//   usage: fake_host <input.bin> <output.bin>
+//   usage: fake_host <input.bin> <output.bin> [<files dir>]
 //
 // input (little-endian): int32 mode, nx, ny, nspecies, ncells, nsteps;
 //   double W, H, npm, dt; double D[ns], gamma[ns]; 4 x {int32 kind, int32 pad,
@@ -19,10 +19,15 @@
 //   solver, one per species) with the given walls.
 // output: double fields[ns*N], double sampled[nc*ns] (mode 1, last step),
 //   double flux[ns] (modes 0-2), int32 iterations_max.
+// With a files dir (modes 0, 2, 3) every step also writes the solvers' files
+// (writeDiffusionFiles: <dir>/hsl<s>.pvd + .vtu, or <dir>/fd<s>_petsc/grid%04d.vtr)
+// and a frame of the controller's JSON log (<dir>/test-0_0-jsonData.json)
+// whose externalHSL holds the species fields after the step.
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <stdexcept>
+#include <string>
 #include <vector>
 
 #include "../../eq_amd/host/fhsl_gpu.hpp"
@@ -35,10 +40,11 @@ static void rd(FILE* f, T* p, size_t n)
 
 int main(int argc, char** argv)
 {
-    if (argc != 3) {
-        std::fprintf(stderr, "usage: fake_host <input.bin> <output.bin>\n");
+    if (argc != 3 && argc != 4) {
+        std::fprintf(stderr, "usage: fake_host <input.bin> <output.bin> [<files dir>]\n");
         return 2;
     }
+    const std::string files = argc == 4 ? argv[3] : "";
     try {
         FILE* in = std::fopen(argv[1], "rb");
         if (!in) throw std::runtime_error("cannot open input");
@@ -71,12 +77,20 @@ int main(int argc, char** argv)
         p.trapHeightMicrons = g4[1];
         p.nodesPerMicron = g4[2];
         p.dt = g4[3];
+        eqio::jsonRecording rec(0, 0);
+        std::vector<eqio::frameCell> fcells(nc);
+        for (int c = 0; c < nc; ++c) {
+            fcells[c].id = (uint64_t)c + 1;
+            fcells[c].data = {cells[c].x, cells[c].y, cells[c].angle, cells[c].length, 0.0};
+        }
+        if (!files.empty()) rec.init(files, "test", 0, "{\"dt\":" + eqio::json_number(p.dt) + "}");
         if (mode == 3) {
             std::vector<eqgpu::diffusionPETSc> fd(ns);
             int32_t itmax = 0;
             for (int s = 0; s < ns; ++s) {
                 eqgpu::solverParams ps = p;
                 ps.D_HSL = D[s];
+                if (!files.empty()) ps.filePath = files + "/fd" + std::to_string(s) + "_";
                 fd[s].initDiffusion(ps, walls);
                 if (fd[s].solution_vector.size() != N) throw std::runtime_error("FD grid size mismatch");
                 fd[s].solution_vector = u0[s];
@@ -88,7 +102,16 @@ int main(int argc, char** argv)
                     fd[s].stepDiffusion();
                     itmax = std::max(itmax, fd[s].lastStats().iterations_max);
                 }
+                if (!files.empty()) {
+                    std::vector<std::vector<double>> hsl;
+                    for (int s = 0; s < ns; ++s) {
+                        fd[s].writeDiffusionFiles((k + 1) * p.dt);
+                        hsl.push_back(fd[s].solution_vector);
+                    }
+                    rec.recordFrame((k + 1) * p.dt, fcells, hsl);
+                }
             }
+            rec.finalize();
             FILE* fo = std::fopen(argv[2], "wb");
             if (!fo) throw std::runtime_error("cannot open output");
             for (int s = 0; s < ns; ++s) std::fwrite(fd[s].solution_vector.data(), sizeof(double), N, fo);
@@ -104,7 +127,9 @@ int main(int argc, char** argv)
         species.reserve(ns);
         for (int s = 0; s < ns; ++s) {
             species.emplace_back(field, s);
-            species.back().initDiffusion(p);
+            eqgpu::solverParams ps = p;
+            if (!files.empty() && mode != 1) ps.filePath = files + "/hsl" + std::to_string(s) + ".pvd";
+            species.back().initDiffusion(ps);
             if (species.back().shell->mesh->num_vertices() != N) throw std::runtime_error("num_vertices");
             species.back().solution_vector = u0[s];
             if (mode == 2) {
@@ -123,11 +148,20 @@ int main(int argc, char** argv)
                 for (int c = 0; c < nc; ++c)
                     for (int s = 0; s < ns; ++s) species[s].solution_vector[(size_t)c % N] += q[(size_t)c * ns + s];
                 for (int s = 0; s < ns; ++s) species[s].stepDiffusion();
+                if (!files.empty()) {
+                    std::vector<std::vector<double>> hsl;
+                    for (int s = 0; s < ns; ++s) {
+                        species[s].writeDiffusionFiles((k + 1) * p.dt);
+                        hsl.push_back(species[s].solution_vector);
+                    }
+                    rec.recordFrame((k + 1) * p.dt, fcells, hsl);
+                }
             } else {
                 field->coupleCells(cells, q, &sampled);
                 field->step();
             }
         }
+        rec.finalize();
         std::vector<double> out;
         for (int s = 0; s < ns; ++s) {
             std::vector<double> u;

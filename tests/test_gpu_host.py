@@ -137,3 +137,42 @@ def test_fake_host_fd_solver(oracle, tmp_path):
     for s in range(2):
         err = np.linalg.norm(fields[s] - ref[s]) / np.linalg.norm(ref[s])
         assert err <= 1e-9, (s, err)
+
+
+def test_fake_host_writes_files(tmp_path):
+    """writeDiffusionFiles and the controller's JSON log through the mirror:
+    the .pvd/.vtu series of each fenicsInterface, the .vtr files of each
+    diffusionPETSc and the frames' externalHSL hold exactly the fields the
+    solver returned (bit for bit)."""
+    import json
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    import vtk_read
+    from eq_amd import workloads as W
+    t = W.config0()
+    q = t.rates * 0.1
+    nsteps = 2
+    for mode in (0, 3):
+        d = tmp_path / f"m{mode}"
+        d.mkdir()
+        inp, outp = str(d / "in.bin"), str(d / "out.bin")
+        walls = W.TOP_DIRICHLET if mode == 0 else {2: (1, 0.0, 0.0, 0.0), 1: (2, 0.0, 2.0, 0.3)}
+        N = _write_input(inp, mode, t, nsteps, q, walls)
+        print(f"fake_host mode={mode} nsteps={nsteps} with files", flush=True)
+        subprocess.run([EXE, inp, outp, str(d)], check=True, timeout=120)
+        fields, _, _ = _read_output(outp, 2, N, t.cells.shape[0])
+        for s in range(2):
+            if mode == 0:
+                series = vtk_read.read_pvd(d / f"hsl{s}.pvd")
+                assert [ts for ts, _ in series] == [(k + 1) * 0.1 for k in range(nsteps)]
+                last = vtk_read.read(d / series[-1][1])
+                assert last["arrays"]["u"].tobytes() == fields[s].tobytes()
+                assert int(last["piece"]["NumberOfCells"]) == 2 * t.nx * t.ny
+            else:
+                last = vtk_read.read(d / f"fd{s}_petsc" / f"grid{nsteps:04d}.vtr")
+                assert last["arrays"]["grid"].tobytes() == fields[s].tobytes()
+        doc = json.loads((d / "test-0_0-jsonData.json").read_text())
+        assert len(doc["frames"]) == nsteps
+        hsl = np.array(doc["frames"][-1]["externalHSL"], np.float64)
+        assert hsl.tobytes() == fields.tobytes()
+        assert len(doc["frames"][0]["cells"]) == t.cells.shape[0]

@@ -323,3 +323,34 @@ def test_channel_cn_closed_forms(oracle):
     ref = (dt * D * d / 4.0 * np.where((np.arange(nx + 1) == 0) | (np.arange(nx + 1) == nx), 0.5, 1.0)).sum()
     assert abs(tot - ref) < 1e-12 * abs(ref)   # diffusion conserves the lumped total M 1
     assert np.all(b1 == 0.0)
+
+
+def test_convection_matrix_closed_forms(oracle):
+    """Advection (config 4): the Galerkin convection part has zero row sums
+    (constants are not transported), is skew for a constant velocity away from
+    the walls (int v dphi_j/dx phi_i = -int v dphi_i/dx phi_j + boundary terms),
+    and its interior row, summed from the six triangles around the node, is
+    v*hy/3 * (u_E - u_W) + v*hy/6 * ((u_NE - u_N) + (u_S - u_SW)), which
+    gives v*hx*hy = int v du/dx phi for u = x."""
+    nx, ny, W, H, dt, D = 8, 6, 4.0, 3.0, 0.1, 0.0
+    hx, hy = W / nx, H / ny
+    p = oracle.problem(nx, ny, W, H, dt, D)
+    n = (nx + 1) * (ny + 1)
+    rp0, c0, v0 = oracle.assemble(p)
+    M = oracle.to_scipy(rp0, c0, v0, n).toarray()
+    v = 2.5
+    p.set_velocity(np.full(ny + 1, v))
+    rp, c, val = oracle.assemble(p)
+    C = (oracle.to_scipy(rp, c, val, n).toarray() - M) / dt
+    assert np.allclose(C.sum(axis=1), 0.0, atol=1e-13)
+    idx = lambda i, j: i * (nx + 1) + j
+    interior = [idx(i, j) for i in range(1, ny) for j in range(1, nx)]
+    S = C + C.T
+    assert np.allclose(S[np.ix_(interior, interior)], 0.0, atol=1e-13)
+    i, j = 3, 4
+    u = np.random.default_rng(0).random(n)
+    g = lambda a, b: u[idx(a, b)]
+    expect = v * hy / 3 * (g(i, j + 1) - g(i, j - 1)) + v * hy / 6 * ((g(i + 1, j + 1) - g(i + 1, j)) + (g(i - 1, j) - g(i - 1, j - 1)))
+    assert abs((C @ u)[idx(i, j)] - expect) < 1e-12
+    x = np.tile(np.arange(nx + 1) * hx, ny + 1)
+    assert abs((C @ x)[idx(i, j)] - v * hx * hy) < 1e-12
