@@ -27,9 +27,9 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-KERNEL_NAMES = {0: "k_ap (FP64 stencil q=Ap + p.q)", 1: "k_smooth (fine-level Jacobi sweep)",
-                2: "k_update (FP64 x,r update + dots)", 3: "k_cell_rows (coverage/sample/deposit)"}
-KERNEL_SYMBOLS = {0: "k_ap", 1: "k_smooth", 2: "k_update", 3: "k_cell_rows"}
+KERNEL_NAMES = {0: "k_ap_dir (direction update + stencil q=Ap + p.q)", 1: "k_smooth (level-0 Jacobi sweeps of the V-cycle)",
+                2: "k_update (FP64 x,r update + dots)", 3: "k_tile_couple (coverage/sample/deposit per tile)"}
+KERNEL_SYMBOLS = {0: "k_ap_dir", 1: "k_smooth", 2: "k_update", 3: "k_tile_couple"}
 
 
 def log(msg):

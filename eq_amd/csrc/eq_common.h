@@ -358,6 +358,13 @@ template <typename T, bool TENS = true>
 EQ_HD inline Row<T> row_coefs(const LevelProb& P, int i, int j, int64_t k)
 {
     if (TENS && (P.aniso || P.vel)) {
+        if (!P.aniso && interior(P, i, j)) {
+            // velocity only: the level's constant interior row plus this
+            // grid row's convection (no element counting needed)
+            Row<T> r = interior_row<T>(P);
+            add_convection<T>(P, i, j, vel_base<T>(P), r);
+            return r;
+        }
         Row<T> r = row_coefs_at<T>(P, i, j);
         if (P.aniso) {
             const T* t = tens_base<T>(P) + k;

@@ -1118,8 +1118,22 @@ __global__ __launch_bounds__(kBottomThreads) void k_bottom(BottomArgs<T> A, int 
         for (int t = threadIdx.x; t < nch * n; t += kBottomThreads) {
             const int ch = t / n, k = t - ch * n;
             T s = (T)0;
-#pragma unroll 4
-            for (int m = ch; m < n; m += nch) s += D[(int64_t)m * n + k] * cb[m];
+            // the inverse comes from L2 or HBM: 16 independent loads in
+            // flight per batch, then the products in the same m order
+            constexpr int kBatch = 16;
+            for (int m0 = ch; m0 < n; m0 += kBatch * nch) {
+                T dv[kBatch];
+#pragma unroll
+                for (int u = 0; u < kBatch; ++u) {
+                    const int m = m0 + u * nch;
+                    dv[u] = (m < n) ? D[(int64_t)m * n + k] : (T)0;
+                }
+#pragma unroll
+                for (int u = 0; u < kBatch; ++u) {
+                    const int m = m0 + u * nch;
+                    if (m < n) s += dv[u] * cb[m];
+                }
+            }
             red[t] = s;
         }
         __syncthreads();
@@ -1638,7 +1652,7 @@ struct VCycle {
             // kernels (two launches' work in one, the first launch's values
             // recomputed at the stencil points) win; large levels are
             // bandwidth- and issue-bound and keep the separate launches.
-            const bool fuse = nodes < kFuseMaxNodes;
+            const bool fuse = nodes < (S.mg_fuse_max_nodes > 0 ? (double)S.mg_fuse_max_nodes : kFuseMaxNodes);
             if (top) prof_begin(b, EQ_KERNEL_SMOOTH);
             if (S.mg_pre >= 2 && fuse) {
                 // sweeps 1 and 2 in one launch
@@ -1751,12 +1765,15 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
         }
         prof_begin(b, EQ_KERNEL_STENCIL64);
         k_ap_dir<T, TENS><<<g0, kBlock, 0, s>>>(A, (const T*)b->z, (const T*)pold, (T*)pnew, b->q, b->partial, b->d_state);
-        prof_end(b, EQ_KERNEL_STENCIL64, fine_nodes * 4 * sizeof(double));
+        // reads z, p_old; writes p_new, q (FP64)
+        prof_end(b, EQ_KERNEL_STENCIL64, fine_nodes * (3 * sizeof(T) + sizeof(double)));
         finish(kFinAp);
         prof_begin(b, EQ_KERNEL_CG_UPDATE);
         k_update<T, TENS><<<g0, kBlock, 0, s>>>(A, b->u, b->r, (T*)b->z, (const T*)pnew, b->q, b->partial, b->d_state,
                                           jacobi ? 1 : 0);
-        prof_end(b, EQ_KERNEL_CG_UPDATE, fine_nodes * sizeof(double) * ((jacobi || b->mixed) ? 7 : 6));
+        // reads x, r, q (FP64) and p (+ z when flexible); writes x, r (+ z for Jacobi)
+        prof_end(b, EQ_KERNEL_CG_UPDATE,
+                 fine_nodes * (5 * sizeof(double) + sizeof(T) + (jacobi ? sizeof(T) : b->mixed ? sizeof(T) : 0)));
         finish(kFinUpdate);
     };
     auto two_iterations = [&]() {
@@ -2017,6 +2034,7 @@ void eq_solver_default(eq_solver_desc* s)
     s->mg_max_levels = 16;
     s->check_every = 4;
     s->use_graphs = 1;
+    s->mg_fuse_max_nodes = (int32_t)kFuseMaxNodes;
 }
 
 int eq_batch_create(const eq_field_desc* descs, int32_t nprob, int32_t nspecies, const eq_solver_desc* solver,
@@ -2506,18 +2524,28 @@ int eq_batch_autotune(eq_batch* b, int32_t repeats, eq_solver_desc* chosen)
     auto drop_graphs = [&]() {
         if (b->g_iter2) hipGraphExecDestroy(b->g_iter2);
         if (b->g_init) hipGraphExecDestroy(b->g_init);
+        if (b->g_bicg) hipGraphExecDestroy(b->g_bicg);
         b->g_iter2 = nullptr;
         b->g_init = nullptr;
+        b->g_bicg = nullptr;
     };
     const eq_solver_desc orig = b->solver;
     const int32_t orig_last = b->last_iters;
     int rc = copy_state(true);
     double best_t = 1e300;
-    int best = -1;
+    int best = -1, best_fuse = orig.mg_fuse_max_nodes;
     const int reps = std::max(1, repeats);
-    for (int k = 0; rc == EQ_OK && k < (int)(sizeof(kTuneShapes) / sizeof(kTuneShapes[0])); ++k) {
-        b->solver.mg_pre = kTuneShapes[k][0];
-        b->solver.mg_post = kTuneShapes[k][1];
+    // candidates: every cycle shape with the batch's fusion threshold, then
+    // the fastest shape with every level fused
+    const int nshapes = (int)(sizeof(kTuneShapes) / sizeof(kTuneShapes[0]));
+    for (int k = 0; rc == EQ_OK && k <= nshapes; ++k) {
+        if (k == nshapes && best < 0) break;
+        const int shape = k < nshapes ? k : best;
+        const int32_t fuse = k < nshapes ? orig.mg_fuse_max_nodes : INT32_MAX;
+        if (k == nshapes && orig.mg_fuse_max_nodes == INT32_MAX) break;
+        b->solver.mg_pre = kTuneShapes[shape][0];
+        b->solver.mg_post = kTuneShapes[shape][1];
+        b->solver.mg_fuse_max_nodes = fuse;
         drop_graphs();
         double t = 0.0;
         bool ok = true;
@@ -2538,13 +2566,15 @@ int eq_batch_autotune(eq_batch* b, int32_t repeats, eq_solver_desc* chosen)
         }
         if (rc == EQ_OK && ok && t < best_t) {
             best_t = t;
-            best = k;
+            best = shape;
+            best_fuse = fuse;
         }
     }
     b->solver = orig;
     if (best >= 0) {
         b->solver.mg_pre = kTuneShapes[best][0];
         b->solver.mg_post = kTuneShapes[best][1];
+        b->solver.mg_fuse_max_nodes = best_fuse;
     }
     b->last_iters = orig_last;
     drop_graphs();

@@ -95,6 +95,9 @@ typedef struct {
     int32_t mg_max_levels;
     int32_t check_every;         /* iterations between host convergence checks      */
     int32_t use_graphs;          /* capture iteration chunks in hipGraphs (0/1)     */
+    int32_t mg_fuse_max_nodes;   /* levels with fewer nodes (all problems) run the
+                                    fused multigrid kernels; <= 0: 524288            */
+    int32_t pad_;
 } eq_solver_desc;
 
 typedef struct {
