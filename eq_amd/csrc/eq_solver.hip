@@ -1652,9 +1652,11 @@ struct VCycle {
             // kernels (two launches' work in one, the first launch's values
             // recomputed at the stencil points) win; large levels are
             // bandwidth- and issue-bound and keep the separate launches.
-            const bool fuse = nodes < (S.mg_fuse_max_nodes > 0 ? (double)S.mg_fuse_max_nodes : kFuseMaxNodes);
+            const double fmax = S.mg_fuse_max_nodes > 0 ? (double)S.mg_fuse_max_nodes : kFuseMaxNodes;
+            const bool fuse = nodes < fmax;
+            const bool fuse_pre = nodes < (S.mg_fuse_pre_max_nodes > 0 ? (double)S.mg_fuse_pre_max_nodes : fmax);
             if (top) prof_begin(b, EQ_KERNEL_SMOOTH);
-            if (S.mg_pre >= 2 && fuse) {
+            if (S.mg_pre >= 2 && fuse_pre) {
                 // sweeps 1 and 2 in one launch
                 if (top)
                     k_presmooth2<R, T, double, TENS><<<gs, kBlock, 0, s>>>(L.smooth(), in, X[1], bcopy, omega, st);
@@ -1669,7 +1671,7 @@ struct VCycle {
                                                            st);
             }
             if (top) prof_end(b, EQ_KERNEL_SMOOTH, nodes * (bytes_b + sizeof(T) * (bcopy ? 2 : 1)));
-            for (int k = (S.mg_pre >= 2 && fuse) ? 2 : 1; k < S.mg_pre; ++k) {
+            for (int k = (S.mg_pre >= 2 && fuse_pre) ? 2 : 1; k < S.mg_pre; ++k) {
                 if (top) prof_begin(b, EQ_KERNEL_SMOOTH);
                 k_smooth<R, T, T, T, TENS><<<gs, kBlock, 0, s>>>(L.smooth(), bl, X[cur], X[cur ^ 1], nullptr, omega, 0, 0,
                                                            nullptr, st);
@@ -2035,6 +2037,8 @@ void eq_solver_default(eq_solver_desc* s)
     s->check_every = 4;
     s->use_graphs = 1;
     s->mg_fuse_max_nodes = (int32_t)kFuseMaxNodes;
+    s->mg_fuse_pre_max_nodes = (int32_t)kFuseMaxNodes;
+    s->mg_bottom_lds_bytes = (int32_t)kBottomLdsMax;
 }
 
 int eq_batch_create(const eq_field_desc* descs, int32_t nprob, int32_t nspecies, const eq_solver_desc* solver,
@@ -2070,7 +2074,8 @@ int eq_batch_create(const eq_field_desc* descs, int32_t nprob, int32_t nspecies,
             lds = 64 * 1024;
         // leave room for the kernel's static shared memory (tables and the
         // coarsest-level partials)
-        b->lds_budget = std::min<int64_t>(kBottomLdsMax, (int64_t)lds - 1024 - kBottomThreads * (int64_t)sizeof(double));
+        const int64_t cap = S.mg_bottom_lds_bytes > 0 ? (int64_t)S.mg_bottom_lds_bytes : kBottomLdsMax;
+        b->lds_budget = std::min<int64_t>(cap, (int64_t)lds - 1024 - kBottomThreads * (int64_t)sizeof(double));
     }
     b->nprob = nprob;
     b->nspecies = nspecies;

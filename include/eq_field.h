@@ -97,6 +97,10 @@ typedef struct {
     int32_t use_graphs;          /* capture iteration chunks in hipGraphs (0/1)     */
     int32_t mg_fuse_max_nodes;   /* levels with fewer nodes (all problems) run the
                                     fused multigrid kernels; <= 0: 524288            */
+    int32_t mg_fuse_pre_max_nodes; /* the same for the two pre-sweeps in one launch;
+                                      <= 0: mg_fuse_max_nodes                       */
+    int32_t mg_bottom_lds_bytes; /* LDS one workgroup may use for the levels it runs
+                                    (the bottom of the V-cycle); <= 0: 153600        */
     int32_t pad_;
 } eq_solver_desc;
 
