@@ -320,27 +320,32 @@ EQ_HD inline void aniso_stiffness(const LevelProb& P, int i, int j, const T* d11
 //            upper (c,N,NE): C[N]  -= I/hx, C[NE] += I/hx  I ~ 2v_i + 2v_{i+1}
 // (I = |T|/12 * (...), |T|/(12 hx) = hy/24).  v holds the level's rows.
 template <typename T>
-EQ_HD inline void add_convection(const LevelProb& P, int i, int j, const T* v, Row<T>& r)
+EQ_HD inline void add_convection_v(const LevelProb& P, int i, int j, T vs, T vi, T vn, Row<T>& r)
 {
     const bool sw = i > 0 && j > 0, se = i > 0 && j < P.nx, nw = i < P.ny && j > 0, ne = i < P.ny && j < P.nx;
     const T cw = (T)P.cw, three = (T)3, two = (T)2;
-    const T vi = v[i];
     if (sw) {
-        const T vs = v[i - 1];
         const T I1 = cw * (two * vi + two * vs), I2 = cw * (three * vi + vs);
         r.ksw += I1;
         r.ks -= I1;
         r.kw += I2;
     }
-    if (se) r.ke -= cw * (three * vi + v[i - 1]);
-    if (nw) r.kw += cw * (three * vi + v[i + 1]);
+    if (se) r.ke -= cw * (three * vi + vs);
+    if (nw) r.kw += cw * (three * vi + vn);
     if (ne) {
-        const T vn = v[i + 1];
         const T I5 = cw * (three * vi + vn), I6 = cw * (two * vi + two * vn);
         r.ke -= I5;
         r.kn += I6;
         r.kne -= I6;
     }
+}
+
+// the same with the level's row velocities v (rows i-1, i, i+1 as they exist)
+template <typename T>
+EQ_HD inline void add_convection(const LevelProb& P, int i, int j, const T* v, Row<T>& r)
+{
+    const T vs = i > 0 ? v[i - 1] : (T)0, vn = i < P.ny ? v[i + 1] : (T)0;
+    add_convection_v<T>(P, i, j, vs, v[i], vn, r);
 }
 
 template <typename T> EQ_HD inline const T* vel_base(const LevelProb& P);

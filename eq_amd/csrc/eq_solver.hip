@@ -70,6 +70,8 @@ struct NodeCtx {
     int pi, i, j;
     int64_t idx;
     bool in;
+    bool full;  // the block's whole tile is inside the grid, off the walls
+    bool full2; // ... and so is the one-node ring around it
 };
 
 // A thread owns R consecutive rows of one column of its tile (the level's
@@ -90,6 +92,12 @@ __device__ __forceinline__ bool decode(const LevelView& L, NodeCtx& c)
     c.j = tx * kTileW + (int)(threadIdx.x & (kTileW - 1));
     c.i = ty * (kWaves * R) + (int)(threadIdx.x / kTileW) * R;
     c.in = (c.i <= c.P->ny) && (c.j <= c.P->nx);
+    {
+        // rows i0 .. i0+kWaves*R-1 and columns j0 .. j0+kTileW-1 all in 1 .. n-1
+        const int i0 = ty * (kWaves * R), j0 = tx * kTileW;
+        c.full = i0 >= 1 && j0 >= 1 && i0 + kWaves * R <= c.P->ny && j0 + kTileW <= c.P->nx;
+        c.full2 = i0 >= 2 && j0 >= 2 && i0 + kWaves * R < c.P->ny && j0 + kTileW < c.P->nx;
+    }
     c.idx = c.P->origin + (int64_t)c.i * c.P->pitch + c.j;
     return true;
 }
@@ -106,6 +114,58 @@ __device__ __forceinline__ void each_row(const NodeCtx& c, F&& f)
         n.in = (n.i <= P.ny) && (c.j <= P.nx);
         f(n);
     }
+}
+
+// The R+2 rows i-1 .. i+R of columns j-1, j, j+1 around a thread's R nodes,
+// one load per row and column (the rows the R stencils share are read once).
+template <int R, typename T>
+struct Window {
+    T w[R + 2], m[R + 2], e[R + 2];
+    __device__ __forceinline__ void load(const T* __restrict__ x, int64_t k0, int pitch)
+    {
+#pragma unroll
+        for (int q = 0; q < R + 2; ++q) {
+            const T* p = x + k0 + (int64_t)(q - 1) * pitch;
+            w[q] = p[-1];
+            m[q] = p[0];
+            e[q] = p[1];
+        }
+    }
+    // A x at row r of the R (interior constant row): centre, W, E, S, N, SW, NE
+    template <bool TENS>
+    __device__ __forceinline__ T apply(const Row<T>& rc, int r) const
+    {
+        return row_apply<TENS>(rc, m[r + 1], w[r + 1], e[r + 1], m[r], m[r + 2], w[r], e[r + 2]);
+    }
+};
+
+// Rows of the interior nodes i .. i+R-1 (column j) of a problem without a
+// tensor field: the level's constant row, plus the grid row's convection
+// with a velocity (what row_coefs gives there).  load() reads the R+2 row
+// velocities up front, with the other loads.
+template <int R, typename T, bool TENS>
+struct PlainRows {
+    T v[R + 2];
+    __device__ __forceinline__ void load(const LevelProb& P, int i)
+    {
+        if (TENS && P.vel) {
+            const T* vb = vel_base<T>(P);
+#pragma unroll
+            for (int q = 0; q < R + 2; ++q) v[q] = vb[i - 1 + q];
+        }
+    }
+    __device__ __forceinline__ Row<T> row(const LevelProb& P, int r, int i, int j) const
+    {
+        Row<T> rc = interior_row<T>(P);
+        if (TENS && P.vel) add_convection_v<T>(P, i + r, j, v[r], v[r + 1], v[r + 2], rc);
+        return rc;
+    }
+};
+// the inverse diagonal of such a row (row_dinv)
+template <typename T, bool TENS>
+__device__ __forceinline__ T plain_dinv(const LevelProb& P, const Row<T>& rc)
+{
+    return (TENS && P.vel) ? (T)1 / row_diag<TENS>(rc) : interior_dinv<T>(P);
 }
 
 __device__ __forceinline__ double wave_sum(double v)
@@ -272,6 +332,33 @@ __global__ __launch_bounds__(kBlock) void k_ap_dir(LevelView L, const TP* __rest
     const double beta = s.beta;
     double v[1] = {0.0};
     auto dir = [beta](TP zz, TP pp) -> double { return (double)(TP)((double)zz + beta * (double)pp); };
+    if (c.full && !(TENS && P.aniso)) {
+        // interior tile without a tensor field: every load first, then the R rows
+        Window<kRows0, TP> Z, Q;
+        Z.load(z, c.idx, P.pitch);
+        Q.load(pold, c.idx, P.pitch);
+        PlainRows<kRows0, double, TENS> V;
+        V.load(P, c.i);
+        double dw[kRows0 + 2], dm[kRows0 + 2], de[kRows0 + 2];
+#pragma unroll
+        for (int q = 0; q < kRows0 + 2; ++q) {
+            dw[q] = dir(Z.w[q], Q.w[q]);
+            dm[q] = dir(Z.m[q], Q.m[q]);
+            de[q] = dir(Z.e[q], Q.e[q]);
+        }
+#pragma unroll
+        for (int r = 0; r < kRows0; ++r) {
+            const int64_t k = c.idx + (int64_t)r * P.pitch;
+            const double nc = dm[r + 1];
+            const Row<double> ri = V.row(P, r, c.i, c.j);
+            const double qv = row_apply<TENS>(ri, nc, dw[r + 1], de[r + 1], dm[r], dm[r + 2], dw[r], de[r + 2]);
+            pnew[k] = (TP)nc;
+            q[k] = qv;
+            v[0] += nc * qv;
+        }
+        tile_partial<1>(v, partial, c.gt);
+        return;
+    }
     each_row<kRows0>(c, [&](const NodeCtx& n) {
         if (!n.in) return;
         TP zc, zw, ze, zs, zn, zsw, zne, pc, pw, pe, ps, pn, psw, pne;
@@ -307,6 +394,32 @@ __global__ __launch_bounds__(kBlock) void k_update(LevelView L, double* __restri
     // one accumulator for whichever r.z this mode forms (two accumulators
     // picked by the branch end up addressed through a select, in scratch)
     double rr = 0.0, rzx = 0.0;
+    if (c.full && !jacobi) {
+        // interior tile: every load first, then the R rows
+        double xv[kRows0], rv0[kRows0], qv[kRows0];
+        TP pv[kRows0], zv[kRows0];
+#pragma unroll
+        for (int w = 0; w < kRows0; ++w) {
+            const int64_t k = c.idx + (int64_t)w * P.pitch;
+            xv[w] = x[k];
+            rv0[w] = r[k];
+            qv[w] = q[k];
+            pv[w] = p[k];
+            zv[w] = flexible ? z[k] : (TP)0;
+        }
+#pragma unroll
+        for (int w = 0; w < kRows0; ++w) {
+            const int64_t k = c.idx + (int64_t)w * P.pitch;
+            x[k] = xv[w] + alpha * (double)pv[w];
+            const double rv = rv0[w] - alpha * qv[w];
+            r[k] = rv;
+            rr += rv * rv;
+            rzx += flexible ? rv * (double)zv[w] : 0.0;
+        }
+        double v[3] = {rr, 0.0, rzx};
+        tile_partial<3>(v, partial, c.gt);
+        return;
+    }
     each_row<kRows0>(c, [&](const NodeCtx& n) {
         if (!n.in || is_dirichlet(P, n.i, n.j)) return;
         const int64_t k = n.idx;
@@ -547,6 +660,28 @@ __global__ __launch_bounds__(kBlock) void k_bicg_apply(LevelView L, const T* __r
     const LevelProb& P = *c.P;
     if (st[P.prob].done) return;
     double d0 = 0.0, d1 = 0.0;
+    if (c.full && !(TENS && P.aniso)) {
+        // interior tile without a tensor field: every load first
+        Window<kRows0, T> X;
+        X.load(xh, c.idx, P.pitch);
+        double av[kRows0];
+#pragma unroll
+        for (int r = 0; r < kRows0; ++r) av[r] = a[c.idx + (int64_t)r * P.pitch];
+        PlainRows<kRows0, double, TENS> V;
+        V.load(P, c.i);
+#pragma unroll
+        for (int r = 0; r < kRows0; ++r) {
+            const Row<double> rc = V.row(P, r, c.i, c.j);
+            const double yv = row_apply<TENS>(rc, (double)X.m[r + 1], (double)X.w[r + 1], (double)X.e[r + 1],
+                                              (double)X.m[r], (double)X.m[r + 2], (double)X.w[r], (double)X.e[r + 2]);
+            y[c.idx + (int64_t)r * P.pitch] = yv;
+            d0 += yv * av[r];
+            if (mode) d1 += yv * yv;
+        }
+        double v[2] = {d0, d1};
+        tile_partial<2>(v, partial, c.gt);
+        return;
+    }
     each_row<kRows0>(c, [&](const NodeCtx& n) {
         if (!n.in) return;
         double yv = 0.0;
@@ -718,6 +853,32 @@ __global__ __launch_bounds__(kBlock) void k_smooth(LevelView L, const TB* __rest
     const LevelProb& P = *c.P;
     if (st[P.prob].done) return;
     double v[1] = {0.0};
+    if (R > 1 && c.full && !(TENS && P.aniso)) {
+        // interior tile without a tensor field: the same arithmetic as
+        // smooth_val with no per-node tests and one load per row and column;
+        // every load before the first store, so no row waits on the one before
+        TB bl[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) bl[r] = b[c.idx + (int64_t)r * P.pitch];
+        Window<R, T> X;
+        if (!zero_init) X.load(xin, c.idx, P.pitch);
+        PlainRows<R, T, TENS> V;
+        V.load(P, c.i);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int64_t k = c.idx + (int64_t)r * P.pitch;
+            const Row<T> ri = V.row(P, r, c.i, c.j);
+            const T wd = omega * plain_dinv<T, TENS>(P, ri);
+            const T bv = (T)bl[r];
+            const T out = zero_init ? wd * bv : X.m[r + 1] + wd * (bv - X.template apply<TENS>(ri, r));
+            if (bcopy) bcopy[k] = bv;
+            const TO o = (TO)out;
+            xout[k] = o;
+            if (dot) v[0] += (double)bl[r] * (double)o;
+        }
+        if (dot) tile_partial<1>(v, partial, c.gt);
+        return;
+    }
     each_row<R>(c, [&](const NodeCtx& n) {
         if (!n.in) return;
         if (is_dirichlet(P, n.i, n.j)) {
@@ -743,6 +904,18 @@ __global__ __launch_bounds__(kBlock) void k_residual(LevelView L, const T* __res
     if (!decode<R>(L, c)) return;
     const LevelProb& P = *c.P;
     if (st[P.prob].done) return;
+    if (R > 1 && c.full && !(TENS && P.aniso)) {
+        T bl[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) bl[q] = b[c.idx + (int64_t)q * P.pitch];
+        Window<R, T> X;
+        X.load(x, c.idx, P.pitch);
+        PlainRows<R, T, TENS> V;
+        V.load(P, c.i);
+#pragma unroll
+        for (int q = 0; q < R; ++q) r[c.idx + (int64_t)q * P.pitch] = bl[q] - X.template apply<TENS>(V.row(P, q, c.i, c.j), q);
+        return;
+    }
     each_row<R>(c, [&](const NodeCtx& n) {
         if (!n.in) return;
         r[n.idx] = is_dirichlet(P, n.i, n.j) ? (T)0 : residual_val<T, TENS>(P, n.i, n.j, n.idx, P.pitch, b, x, n.idx);
@@ -773,6 +946,18 @@ __global__ __launch_bounds__(kBlock) void k_prolong(LevelView Lf, const LevelPro
     const LevelProb& Pf = *c.P;
     if (st[Pf.prob].done) return;
     const LevelProb& Pc = coarse[Pf.cidx];
+    if (R > 1 && c.full) {
+        // interior tile: the coarse values and the fine x first, then the R rows
+        T add[R], xv[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            add[r] = prolong_val<T>(Pf, Pc, c.i + r, c.j, xc, Pc.origin, Pc.pitch);
+            xv[r] = xf[c.idx + (int64_t)r * Pf.pitch];
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) xf[c.idx + (int64_t)r * Pf.pitch] = xv[r] + add[r];
+        return;
+    }
     each_row<R>(c, [&](const NodeCtx& n) {
         if (!n.in || is_dirichlet(Pf, n.i, n.j)) return;
         xf[n.idx] += prolong_val<T>(Pf, Pc, n.i, n.j, xc, Pc.origin, Pc.pitch);
@@ -805,6 +990,44 @@ __global__ __launch_bounds__(kBlock) void k_presmooth2(LevelView L, const TB* __
     if (!decode<R>(L, c)) return;
     const LevelProb& P = *c.P;
     if (st[P.prob].done) return;
+    if (R > 1 && c.full2 && !(TENS && P.aniso)) {
+        // the tile and its ring are interior, no tensor field: b over the
+        // R+2 rows of the three columns, the first sweep's x there (its row's
+        // damped inverse diagonal times b), then the second sweep -- the
+        // values first_sweep_at and the loop below give, with every load first
+        Window<R, TB> B;
+        B.load(b, c.idx, P.pitch);
+        T vv[R + 4];
+        if (TENS && P.vel) {
+            const T* vb = vel_base<T>(P);
+#pragma unroll
+            for (int q = 0; q < R + 4; ++q) vv[q] = vb[c.i - 2 + q];
+        }
+        auto row_q = [&](int q) -> Row<T> { // grid row c.i - 1 + q
+            Row<T> rc = interior_row<T>(P);
+            if (TENS && P.vel) add_convection_v<T>(P, c.i - 1 + q, c.j, vv[q], vv[q + 1], vv[q + 2], rc);
+            return rc;
+        };
+        T x1w[R + 2], x1m[R + 2], x1e[R + 2];
+#pragma unroll
+        for (int q = 0; q < R + 2; ++q) {
+            const T wq = omega * plain_dinv<T, TENS>(P, row_q(q));
+            x1w[q] = wq * (T)B.w[q];
+            x1m[q] = wq * (T)B.m[q];
+            x1e[q] = wq * (T)B.e[q];
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int64_t k = c.idx + (int64_t)r * P.pitch;
+            const Row<T> rc = row_q(r + 1);
+            const T wd = omega * plain_dinv<T, TENS>(P, rc);
+            const T bv = (T)B.m[r + 1];
+            xout[k] = x1m[r + 1] + wd * (bv - row_apply<TENS>(rc, x1m[r + 1], x1w[r + 1], x1e[r + 1], x1m[r], x1m[r + 2],
+                                                               x1w[r], x1e[r + 2]));
+            if (bcopy) bcopy[k] = bv;
+        }
+        return;
+    }
     each_row<R>(c, [&](const NodeCtx& n) {
         if (!n.in) return;
         const int i = n.i, j = n.j, pt = P.pitch;
@@ -1654,7 +1877,8 @@ struct VCycle {
             // bandwidth- and issue-bound and keep the separate launches.
             const double fmax = S.mg_fuse_max_nodes > 0 ? (double)S.mg_fuse_max_nodes : kFuseMaxNodes;
             const bool fuse = nodes < fmax;
-            const bool fuse_pre = nodes < (S.mg_fuse_pre_max_nodes > 0 ? (double)S.mg_fuse_pre_max_nodes : fmax);
+            const bool fuse_pre =
+                top ? S.mg_fuse_pre_top != 0 : nodes < (S.mg_fuse_pre_max_nodes > 0 ? (double)S.mg_fuse_pre_max_nodes : fmax);
             if (top) prof_begin(b, EQ_KERNEL_SMOOTH);
             if (S.mg_pre >= 2 && fuse_pre) {
                 // sweeps 1 and 2 in one launch
@@ -2540,17 +2764,22 @@ int eq_batch_autotune(eq_batch* b, int32_t repeats, eq_solver_desc* chosen)
     double best_t = 1e300;
     int best = -1, best_fuse = orig.mg_fuse_max_nodes;
     const int reps = std::max(1, repeats);
-    // candidates: every cycle shape with the batch's fusion threshold, then
-    // the fastest shape with every level fused
+    int best_top = orig.mg_fuse_pre_top;
+    // candidates: every cycle shape with the batch's fusion settings, then
+    // the fastest shape with every level fused, then the fastest so far with
+    // the finest level's pre-sweeps fused the other way
     const int nshapes = (int)(sizeof(kTuneShapes) / sizeof(kTuneShapes[0]));
-    for (int k = 0; rc == EQ_OK && k <= nshapes; ++k) {
-        if (k == nshapes && best < 0) break;
+    for (int k = 0; rc == EQ_OK && k <= nshapes + 1; ++k) {
+        if (k >= nshapes && best < 0) break;
         const int shape = k < nshapes ? k : best;
-        const int32_t fuse = k < nshapes ? orig.mg_fuse_max_nodes : INT32_MAX;
-        if (k == nshapes && orig.mg_fuse_max_nodes == INT32_MAX) break;
+        const int32_t fuse = k < nshapes ? orig.mg_fuse_max_nodes : k == nshapes ? INT32_MAX : best_fuse;
+        const int32_t top = k <= nshapes ? orig.mg_fuse_pre_top : !best_top;
+        if (k == nshapes && orig.mg_fuse_max_nodes == INT32_MAX) continue;
+        if (k == nshapes + 1 && kTuneShapes[shape][0] < 2) break; // one pre-sweep: nothing to fuse
         b->solver.mg_pre = kTuneShapes[shape][0];
         b->solver.mg_post = kTuneShapes[shape][1];
         b->solver.mg_fuse_max_nodes = fuse;
+        b->solver.mg_fuse_pre_top = top;
         drop_graphs();
         double t = 0.0;
         bool ok = true;
@@ -2573,6 +2802,7 @@ int eq_batch_autotune(eq_batch* b, int32_t repeats, eq_solver_desc* chosen)
             best_t = t;
             best = shape;
             best_fuse = fuse;
+            best_top = top;
         }
     }
     b->solver = orig;
@@ -2580,6 +2810,7 @@ int eq_batch_autotune(eq_batch* b, int32_t repeats, eq_solver_desc* chosen)
         b->solver.mg_pre = kTuneShapes[best][0];
         b->solver.mg_post = kTuneShapes[best][1];
         b->solver.mg_fuse_max_nodes = best_fuse;
+        b->solver.mg_fuse_pre_top = best_top;
     }
     b->last_iters = orig_last;
     drop_graphs();

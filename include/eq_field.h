@@ -101,7 +101,8 @@ typedef struct {
                                       <= 0: mg_fuse_max_nodes                       */
     int32_t mg_bottom_lds_bytes; /* LDS one workgroup may use for the levels it runs
                                     (the bottom of the V-cycle); <= 0: 153600        */
-    int32_t pad_;
+    int32_t mg_fuse_pre_top;     /* 1: the finest level's first two sweeps run in one
+                                    launch (pays off on large fields)               */
 } eq_solver_desc;
 
 typedef struct {

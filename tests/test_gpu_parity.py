@@ -514,3 +514,44 @@ def test_advection_bicgstab_matches_oracle(eqf, oracle, solver):
             for k in range(len(descs)):
                 ref[k], _ = oracle.step(probs[k], ref[k], "lu")
                 assert _rel(fb.download(k), ref[k]) <= 1e-9, (step, k)
+
+
+@pytest.mark.parametrize("case", ["mixed", "mixed-fused", "gmg-fused", "bicgstab-velocity-fused"])
+def test_interior_tile_paths_match_oracle(eqf, oracle, case):
+    """Traps large enough to hold interior tiles (and tiles whose one-node ring
+    is interior), where the kernels take their register-window paths: the
+    smoother, residual, prolongation, operator, CG update and BiCGStab
+    apply, and -- with the finest level's pre-sweeps fused -- the fused
+    first two sweeps; with a velocity the rows carry the convection.  Two
+    steps against the oracle's LU."""
+    kind = {"mixed": eqf.MIXED_GMG_CG, "mixed-fused": eqf.MIXED_GMG_CG, "gmg-fused": eqf.GMG_CG,
+            "bicgstab-velocity-fused": eqf.MIXED_GMG_BICGSTAB}[case]
+    fused = case.endswith("fused")
+    vel = case.startswith("bicgstab")
+    shapes = [(260, 120, 13.0, 6.0), (200, 90, 10.0, 4.5)]
+    walls = {oracle.TOP: (oracle.DIRICHLET, 0.0, 0, 0), oracle.LEFT: (oracle.ROBIN, 0, 1.0, 0.2)}
+    descs, probs, u0, vels = [], [], [], []
+    rng = np.random.default_rng(23)
+    for (nx, ny, W, H) in shapes:
+        for D in (200.0, 40.0):
+            descs.append(eqf.field_desc(nx, ny, W, H, 0.1, D, 0.01, 0.0, walls))
+            p = oracle.problem(nx, ny, W, H, 0.1, D, 0.01, 0.0, walls=walls)
+            v = _parabolic(ny, H, 30.0) if vel else None
+            p.set_velocity(v)
+            probs.append(p)
+            vels.append(v)
+            u0.append(rng.random((nx + 1) * (ny + 1)))
+    kw = {"mg_fuse_pre_top": 1, "mg_fuse_max_nodes": 2**31 - 1} if fused else {}
+    with eqf.FieldBatch(descs, nspecies=2, solver=eqf.solver_desc(kind, **kw)) as fb:
+        for k in range(len(descs)):
+            fb.upload(k, u0[k])
+            if vels[k] is not None:
+                fb.set_velocity(k, vels[k])
+        ref = [u.copy() for u in u0]
+        for step in range(2):
+            print(f"interior-tile case {case} step {step}", flush=True)
+            st = fb.step()
+            assert st.nconverged == len(descs), step
+            for k in range(len(descs)):
+                ref[k], _ = oracle.step(probs[k], ref[k], "lu")
+                assert _rel(fb.download(k), ref[k]) <= 1e-9, (step, k)
