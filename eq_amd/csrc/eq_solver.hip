@@ -68,6 +68,7 @@ struct NodeCtx {
     int t;      // index in the launched tile table
     int gt;     // tile index in the level's full table (partial sums)
     int pi, i, j;
+    int pitch, nx, ny; // of the problem, in registers for each_row
     int64_t idx;
     bool in;
     bool full;  // the block's whole tile is inside the grid, off the walls
@@ -91,7 +92,10 @@ __device__ __forceinline__ bool decode(const LevelView& L, NodeCtx& c)
     const int ty = tr.local / c.P->tiles_x, tx = tr.local - ty * c.P->tiles_x;
     c.j = tx * kTileW + (int)(threadIdx.x & (kTileW - 1));
     c.i = ty * (kWaves * R) + (int)(threadIdx.x / kTileW) * R;
-    c.in = (c.i <= c.P->ny) && (c.j <= c.P->nx);
+    c.pitch = c.P->pitch;
+    c.nx = c.P->nx;
+    c.ny = c.P->ny;
+    c.in = (c.i <= c.ny) && (c.j <= c.nx);
     {
         // rows i0 .. i0+kWaves*R-1 and columns j0 .. j0+kTileW-1 all in 1 .. n-1
         const int i0 = ty * (kWaves * R), j0 = tx * kTileW;
@@ -105,13 +109,12 @@ __device__ __forceinline__ bool decode(const LevelView& L, NodeCtx& c)
 template <int R, typename F>
 __device__ __forceinline__ void each_row(const NodeCtx& c, F&& f)
 {
-    const LevelProb& P = *c.P;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         NodeCtx n = c;
         n.i = c.i + r;
-        n.idx = c.idx + (int64_t)r * P.pitch;
-        n.in = (n.i <= P.ny) && (c.j <= P.nx);
+        n.idx = c.idx + (int64_t)r * c.pitch;
+        n.in = (n.i <= c.ny) && (c.j <= c.nx);
         f(n);
     }
 }
@@ -237,7 +240,9 @@ __global__ __launch_bounds__(kBlock) void k_rhs(LevelView L, const double* __res
 {
     NodeCtx c;
     if (!decode<kRows0>(L, c)) return;
-    const LevelProb& P = *c.P;
+    // a register copy of the problem record: its fields are read once, not
+    // again after every store (the compiler cannot tell the two apart)
+    const LevelProb P = *c.P;
     double v[1] = {0.0};
     each_row<kRows0>(c, [&](const NodeCtx& n) {
         if (!n.in) return;
@@ -267,7 +272,7 @@ __global__ __launch_bounds__(kBlock) void k_init(LevelView L, double* x, const d
 {
     NodeCtx c;
     if (!decode<kRows0>(L, c)) return;
-    const LevelProb& P = *c.P;
+    const LevelProb P = *c.P;
     const bool prev = st[P.prob].have_prev != 0;
     const double* __restrict__ src = prev ? xprev : x;
     double v[2] = {0.0, 0.0};
@@ -326,7 +331,7 @@ __global__ __launch_bounds__(kBlock) void k_ap_dir(LevelView L, const TP* __rest
 {
     NodeCtx c;
     if (!decode<kRows0>(L, c)) return;
-    const LevelProb& P = *c.P;
+    const LevelProb P = *c.P;
     const CgState& s = st[P.prob];
     if (s.done) return;
     const double beta = s.beta;
@@ -386,7 +391,7 @@ __global__ __launch_bounds__(kBlock) void k_update(LevelView L, double* __restri
 {
     NodeCtx c;
     if (!decode<kRows0>(L, c)) return;
-    const LevelProb& P = *c.P;
+    const LevelProb P = *c.P;
     const CgState& s0 = st[P.prob];
     if (s0.done) return;
     const double alpha = s0.alpha;
@@ -451,7 +456,7 @@ __global__ __launch_bounds__(kBlock) void k_true_residual(LevelView L, const dou
 {
     NodeCtx c;
     if (!decode<kRows0>(L, c)) return;
-    const LevelProb& P = *c.P;
+    const LevelProb P = *c.P;
     double v[1] = {0.0};
     each_row<kRows0>(c, [&](const NodeCtx& n) {
         if (!n.in || is_dirichlet(P, n.i, n.j)) return;
@@ -472,7 +477,7 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_z(LevelView L, const double* 
 {
     NodeCtx c;
     if (!decode<kRows0>(L, c)) return;
-    const LevelProb& P = *c.P;
+    const LevelProb P = *c.P;
     if (st[P.prob].done) return;
     double v[1] = {0.0};
     each_row<kRows0>(c, [&](const NodeCtx& n) {
@@ -657,7 +662,7 @@ __global__ __launch_bounds__(kBlock) void k_bicg_apply(LevelView L, const T* __r
 {
     NodeCtx c;
     if (!decode<kRows0>(L, c)) return;
-    const LevelProb& P = *c.P;
+    const LevelProb P = *c.P;
     if (st[P.prob].done) return;
     double d0 = 0.0, d1 = 0.0;
     if (c.full && !(TENS && P.aniso)) {
@@ -850,7 +855,7 @@ __global__ __launch_bounds__(kBlock) void k_smooth(LevelView L, const TB* __rest
 {
     NodeCtx c;
     if (!decode<R>(L, c)) return;
-    const LevelProb& P = *c.P;
+    const LevelProb P = *c.P;
     if (st[P.prob].done) return;
     double v[1] = {0.0};
     if (R > 1 && c.full && !(TENS && P.aniso)) {
@@ -902,7 +907,7 @@ __global__ __launch_bounds__(kBlock) void k_residual(LevelView L, const T* __res
 {
     NodeCtx c;
     if (!decode<R>(L, c)) return;
-    const LevelProb& P = *c.P;
+    const LevelProb P = *c.P;
     if (st[P.prob].done) return;
     if (R > 1 && c.full && !(TENS && P.aniso)) {
         T bl[R];
@@ -928,9 +933,9 @@ __global__ __launch_bounds__(kBlock) void k_restrict(LevelView Lc, const LevelPr
 {
     NodeCtx c;
     if (!decode<kRowsC>(Lc, c)) return;
-    const LevelProb& Pc = *c.P;
+    const LevelProb Pc = *c.P;
     if (st[Pc.prob].done) return;
-    const LevelProb& Pf = fine[Pc.fidx];
+    const LevelProb Pf = fine[Pc.fidx];
     each_row<kRowsC>(c, [&](const NodeCtx& n) {
         if (!n.in) return;
         bc[n.idx] = is_dirichlet(Pc, n.i, n.j) ? (T)0 : restrict_val<T>(Pc, Pf, n.i, n.j, rf, Pf.origin, Pf.pitch);
@@ -943,9 +948,9 @@ __global__ __launch_bounds__(kBlock) void k_prolong(LevelView Lf, const LevelPro
 {
     NodeCtx c;
     if (!decode<R>(Lf, c)) return;
-    const LevelProb& Pf = *c.P;
+    const LevelProb Pf = *c.P;
     if (st[Pf.prob].done) return;
-    const LevelProb& Pc = coarse[Pf.cidx];
+    const LevelProb Pc = coarse[Pf.cidx];
     if (R > 1 && c.full) {
         // interior tile: the coarse values and the fine x first, then the R rows
         T add[R], xv[R];
@@ -988,7 +993,7 @@ __global__ __launch_bounds__(kBlock) void k_presmooth2(LevelView L, const TB* __
 {
     NodeCtx c;
     if (!decode<R>(L, c)) return;
-    const LevelProb& P = *c.P;
+    const LevelProb P = *c.P;
     if (st[P.prob].done) return;
     if (R > 1 && c.full2 && !(TENS && P.aniso)) {
         // the tile and its ring are interior, no tensor field: b over the
@@ -1060,9 +1065,9 @@ __global__ __launch_bounds__(kBlock) void k_residual_restrict(LevelView Lc, cons
 {
     NodeCtx c;
     if (!decode<kRowsC>(Lc, c)) return;
-    const LevelProb& Pc = *c.P;
+    const LevelProb Pc = *c.P;
     if (st[Pc.prob].done) return;
-    const LevelProb& Pf = fine[Pc.fidx];
+    const LevelProb Pf = fine[Pc.fidx];
     auto racc = [&](int i, int j, int64_t k) -> T {
         if (i < 0 || j < 0 || i > Pf.ny || j > Pf.nx || is_dirichlet(Pf, i, j)) return (T)0;
         return residual_val<T, TENS>(Pf, i, j, Pf.origin + k, Pf.pitch, bf, xf, Pf.origin + k);
@@ -1083,9 +1088,9 @@ __global__ __launch_bounds__(kBlock) void k_prolong_smooth(LevelView Lf, const L
 {
     NodeCtx c;
     if (!decode<R>(Lf, c)) return;
-    const LevelProb& P = *c.P;
+    const LevelProb P = *c.P;
     if (st[P.prob].done) return;
-    const LevelProb& Pc = coarse[P.cidx];
+    const LevelProb Pc = coarse[P.cidx];
     auto y_at = [&](int i, int j, int64_t k) -> T {
         if (i < 0 || j < 0 || i > P.ny || j > P.nx || is_dirichlet(P, i, j)) return xin[k];
         return xin[k] + prolong_val<T>(P, Pc, i, j, xc, Pc.origin, Pc.pitch);
@@ -1401,7 +1406,7 @@ __global__ __launch_bounds__(kBlock) void k_apply_full(LevelView L, int pi_only,
     NodeCtx c;
     if (!decode<kRows0>(L, c)) return;
     if (c.pi != pi_only) return;
-    const LevelProb& P = *c.P;
+    const LevelProb P = *c.P;
     each_row<kRows0>(c, [&](const NodeCtx& n) {
         if (!n.in) return;
         double vc, vw, ve, vs, vn, vsw, vne;
