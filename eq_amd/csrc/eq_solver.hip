@@ -23,6 +23,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "eq_internal.h"
@@ -119,6 +120,17 @@ __device__ __forceinline__ void each_row(const NodeCtx& c, F&& f)
     }
 }
 
+// f(0) .. f(N-1) with each index a compile-time constant (decltype(r)::value),
+// so per-row register arrays are never indexed at run time
+template <int N, typename F>
+__device__ __forceinline__ void static_rows(F&& f)
+{
+    if constexpr (N > 0) {
+        static_rows<N - 1>(f);
+        f(std::integral_constant<int, N - 1>{});
+    }
+}
+
 // The R+2 rows i-1 .. i+R of columns j-1, j, j+1 around a thread's R nodes,
 // one load per row and column (the rows the R stencils share are read once).
 template <int R, typename T>
@@ -129,6 +141,19 @@ struct Window {
 #pragma unroll
         for (int q = 0; q < R + 2; ++q) {
             const T* p = x + k0 + (int64_t)(q - 1) * pitch;
+            w[q] = p[-1];
+            m[q] = p[0];
+            e[q] = p[1];
+        }
+    }
+    // the same where rows past ny + 1 (outside the problem's frame) may be
+    // among them: those read the frame row ny + 1 instead (their values
+    // belong to no node of the grid and are not used)
+    __device__ __forceinline__ void load_rows(const T* __restrict__ x, int64_t k0, int pitch, int i0, int ny)
+    {
+#pragma unroll
+        for (int q = 0; q < R + 2; ++q) {
+            const T* p = x + k0 + (int64_t)(min(i0 - 1 + q, ny + 1) - i0) * pitch;
             w[q] = p[-1];
             m[q] = p[0];
             e[q] = p[1];
@@ -242,7 +267,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs(LevelView L, const double* __res
     if (!decode<kRows0>(L, c)) return;
     // a register copy of the problem record: its fields are read once, not
     // again after every store (the compiler cannot tell the two apart)
-    const LevelProb P = *c.P;
+    const LevelProb& P = *c.P; // reads P.gw[wall]: a register copy would go to scratch
     double v[1] = {0.0};
     each_row<kRows0>(c, [&](const NodeCtx& n) {
         if (!n.in) return;
@@ -272,7 +297,7 @@ __global__ __launch_bounds__(kBlock) void k_init(LevelView L, double* x, const d
 {
     NodeCtx c;
     if (!decode<kRows0>(L, c)) return;
-    const LevelProb P = *c.P;
+    const LevelProb& P = *c.P; // reads P.gw[wall]: a register copy would go to scratch
     const bool prev = st[P.prob].have_prev != 0;
     const double* __restrict__ src = prev ? xprev : x;
     double v[2] = {0.0, 0.0};
@@ -364,19 +389,24 @@ __global__ __launch_bounds__(kBlock) void k_ap_dir(LevelView L, const TP* __rest
         tile_partial<1>(v, partial, c.gt);
         return;
     }
-    each_row<kRows0>(c, [&](const NodeCtx& n) {
-        if (!n.in) return;
-        TP zc, zw, ze, zs, zn, zsw, zne, pc, pw, pe, ps, pn, psw, pne;
-        load7(z, n.idx, P.pitch, zc, zw, ze, zs, zn, zsw, zne);
-        load7(pold, n.idx, P.pitch, pc, pw, pe, ps, pn, psw, pne);
-        const double nc = dir(zc, pc);
+    // the other tiles: every row's loads first as well, then the rows
+    Window<kRows0, TP> Z, Q;
+    Z.load_rows(z, c.idx, c.pitch, c.i, c.ny);
+    Q.load_rows(pold, c.idx, c.pitch, c.i, c.ny);
+    static_rows<kRows0>([&](auto rr) {
+        constexpr int r = decltype(rr)::value;
+        if (!((c.i + r <= c.ny) && (c.j <= c.nx))) return;
+        const int i = c.i + r;
+        const int64_t k = c.idx + (int64_t)r * c.pitch;
+        const double nc = dir(Z.m[r + 1], Q.m[r + 1]);
         double qv = 0.0;
-        if (!is_dirichlet(P, n.i, n.j)) {
-            const Row<double> rc = row_coefs<double, TENS>(P, n.i, n.j, n.idx);
-            qv = row_apply<TENS>(rc, nc, dir(zw, pw), dir(ze, pe), dir(zs, ps), dir(zn, pn), dir(zsw, psw), dir(zne, pne));
+        if (!is_dirichlet(P, i, c.j)) {
+            const Row<double> rc = row_coefs<double, TENS>(P, i, c.j, k);
+            qv = row_apply<TENS>(rc, nc, dir(Z.w[r + 1], Q.w[r + 1]), dir(Z.e[r + 1], Q.e[r + 1]), dir(Z.m[r], Q.m[r]),
+                                 dir(Z.m[r + 2], Q.m[r + 2]), dir(Z.w[r], Q.w[r]), dir(Z.e[r + 2], Q.e[r + 2]));
         }
-        pnew[n.idx] = (TP)nc;
-        q[n.idx] = qv;
+        pnew[k] = (TP)nc;
+        q[k] = qv;
         v[0] += nc * qv;
     });
     tile_partial<1>(v, partial, c.gt);
@@ -425,21 +455,39 @@ __global__ __launch_bounds__(kBlock) void k_update(LevelView L, double* __restri
         tile_partial<3>(v, partial, c.gt);
         return;
     }
-    each_row<kRows0>(c, [&](const NodeCtx& n) {
-        if (!n.in || is_dirichlet(P, n.i, n.j)) return;
-        const int64_t k = n.idx;
-        x[k] += alpha * (double)p[k];
-        const double rv = r[k] - alpha * q[k];
+    // every row's loads first, then the rows
+    bool act[kRows0];
+    double xv[kRows0], rv0[kRows0], qv[kRows0];
+    TP pv[kRows0], zv0[kRows0];
+    static_rows<kRows0>([&](auto ww) {
+        constexpr int w = decltype(ww)::value;
+        const int64_t k = c.idx + (int64_t)w * c.pitch;
+        act[w] = (c.i + w <= c.ny) && (c.j <= c.nx) && !is_dirichlet(P, c.i + w, c.j);
+        if (act[w]) {
+            xv[w] = x[k];
+            rv0[w] = r[k];
+            qv[w] = q[k];
+            pv[w] = p[k];
+            zv0[w] = (!jacobi && flexible) ? z[k] : (TP)0;
+        }
+    });
+    static_rows<kRows0>([&](auto ww) {
+        constexpr int w = decltype(ww)::value;
+        if (!act[w]) return;
+        const int i = c.i + w;
+        const int64_t k = c.idx + (int64_t)w * c.pitch;
+        x[k] = xv[w] + alpha * (double)pv[w];
+        const double rv = rv0[w] - alpha * qv[w];
         r[k] = rv;
         rr += rv * rv;
         double t = 0.0;
         if (jacobi) {
-            const Row<double> rc = row_coefs<double, TENS>(P, n.i, n.j, n.idx);
-            const TP zv = (TP)(rv * row_dinv<TENS>(P, n.i, n.j, rc));
+            const Row<double> rc = row_coefs<double, TENS>(P, i, c.j, k);
+            const TP zv = (TP)(rv * row_dinv<TENS>(P, i, c.j, rc));
             z[k] = zv;
             t = rv * (double)zv;
         } else if (flexible) {
-            t = rv * (double)z[k];
+            t = rv * (double)zv0[w];
         }
         rzx += t;
     });
@@ -884,6 +932,42 @@ __global__ __launch_bounds__(kBlock) void k_smooth(LevelView L, const TB* __rest
         if (dot) tile_partial<1>(v, partial, c.gt);
         return;
     }
+    if (R > 1) {
+        // the other tiles: every row's loads first as well, then the rows
+        // (smooth_val's arithmetic)
+        bool act[R], dirn[R];
+        TB bl[R];
+        Window<R, T> X;
+        if (!zero_init) X.load_rows(xin, c.idx, c.pitch, c.i, c.ny);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = c.i + r;
+            act[r] = (i <= c.ny) && (c.j <= c.nx);
+            dirn[r] = act[r] && is_dirichlet(P, i, c.j);
+            bl[r] = (act[r] && !dirn[r]) ? b[c.idx + (int64_t)r * c.pitch] : (TB)0;
+        }
+        static_rows<R>([&](auto rr) {
+            constexpr int r = decltype(rr)::value;
+            if (!act[r]) return;
+            const int i = c.i + r;
+            const int64_t k = c.idx + (int64_t)r * c.pitch;
+            if (dirn[r]) {
+                xout[k] = (TO)0;
+                if (bcopy) bcopy[k] = (T)0;
+                return;
+            }
+            const Row<T> rc = row_coefs<T, TENS>(P, i, c.j, k);
+            const T wd = omega * row_dinv<TENS>(P, i, c.j, rc);
+            const T bv = (T)bl[r];
+            const T out = zero_init ? wd * bv : X.m[r + 1] + wd * (bv - X.template apply<TENS>(rc, r));
+            if (bcopy) bcopy[k] = bv;
+            const TO o = (TO)out;
+            xout[k] = o;
+            if (dot) v[0] += (double)bl[r] * (double)o;
+        });
+        if (dot) tile_partial<1>(v, partial, c.gt);
+        return;
+    }
     each_row<R>(c, [&](const NodeCtx& n) {
         if (!n.in) return;
         if (is_dirichlet(P, n.i, n.j)) {
@@ -919,6 +1003,32 @@ __global__ __launch_bounds__(kBlock) void k_residual(LevelView L, const T* __res
         V.load(P, c.i);
 #pragma unroll
         for (int q = 0; q < R; ++q) r[c.idx + (int64_t)q * P.pitch] = bl[q] - X.template apply<TENS>(V.row(P, q, c.i, c.j), q);
+        return;
+    }
+    if (R > 1) {
+        // the other tiles: every row's loads first as well (residual_val's arithmetic)
+        bool act[R], dirn[R];
+        T bl[R];
+        Window<R, T> X;
+        X.load_rows(x, c.idx, c.pitch, c.i, c.ny);
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int i = c.i + q;
+            act[q] = (i <= c.ny) && (c.j <= c.nx);
+            dirn[q] = act[q] && is_dirichlet(P, i, c.j);
+            bl[q] = (act[q] && !dirn[q]) ? b[c.idx + (int64_t)q * c.pitch] : (T)0;
+        }
+        static_rows<R>([&](auto qq) {
+            constexpr int q = decltype(qq)::value;
+            if (!act[q]) return;
+            const int64_t k = c.idx + (int64_t)q * c.pitch;
+            if (dirn[q]) {
+                r[k] = (T)0;
+                return;
+            }
+            const Row<T> rc = row_coefs<T, TENS>(P, c.i + q, c.j, k);
+            r[k] = bl[q] - X.template apply<TENS>(rc, q);
+        });
         return;
     }
     each_row<R>(c, [&](const NodeCtx& n) {
@@ -1406,7 +1516,7 @@ __global__ __launch_bounds__(kBlock) void k_apply_full(LevelView L, int pi_only,
     NodeCtx c;
     if (!decode<kRows0>(L, c)) return;
     if (c.pi != pi_only) return;
-    const LevelProb P = *c.P;
+    const LevelProb& P = *c.P; // reads P.gw[wall]: a register copy would go to scratch
     each_row<kRows0>(c, [&](const NodeCtx& n) {
         if (!n.in) return;
         double vc, vw, ve, vs, vn, vsw, vne;

@@ -49,6 +49,50 @@ def test_invalid_descriptor_is_rejected_before_any_device_work():
     assert e.value.code == 1  # EQ_ERR_INVALID
 
 
+@pytest.mark.parametrize("case", ["nx0", "dt0", "negD", "neg_gamma", "nan_f", "bad_wall", "neg_robin", "bad_scheme",
+                                  "huge", "species_grid", "nprob_species", "solver_kind", "rtol0", "mg_pre0"])
+def test_bad_inputs_are_rejected_before_any_device_work(case):
+    """Every descriptor and solver check runs before the library asks for a
+    device, so each bad input comes back as EQ_ERR_INVALID here, without a GPU."""
+    from eq_amd import field as F
+    good = dict(nx=8, ny=4, width=2.0, height=1.0, dt=0.1, D=1.0, gamma=0.0, f=0.0)
+    descs = [F.field_desc(**good)]
+    nspecies, solver = 1, F.solver_desc()
+    d = descs[0]
+    if case == "nx0":
+        d.nx = 0
+    elif case == "dt0":
+        d.dt = 0.0
+    elif case == "negD":
+        d.D = -1.0
+    elif case == "neg_gamma":
+        d.gamma = -0.1
+    elif case == "nan_f":
+        d.f = float("nan")
+    elif case == "bad_wall":
+        d.wall[2].kind = 7
+    elif case == "neg_robin":
+        d.wall[0].kind, d.wall[0].r = 2, -1.0
+    elif case == "bad_scheme":
+        d.scheme = 5
+    elif case == "huge":
+        d.nx, d.ny = 70000, 4
+    elif case == "species_grid":
+        e = F.field_desc(**dict(good, nx=9))
+        descs, nspecies = [d, e], 2
+    elif case == "nprob_species":
+        descs, nspecies = [d, F.field_desc(**good), F.field_desc(**good)], 2
+    elif case == "solver_kind":
+        solver.kind = 9
+    elif case == "rtol0":
+        solver.rtol = 0.0
+    elif case == "mg_pre0":
+        solver.mg_pre = 0
+    with pytest.raises(F.EqError) as e:
+        F.FieldBatch(descs, nspecies=nspecies, solver=solver)
+    assert e.value.code == 1, str(e.value)  # EQ_ERR_INVALID
+
+
 def test_workload_generators_are_deterministic():
     import numpy as np
     from eq_amd import workloads as W

@@ -555,3 +555,69 @@ def test_interior_tile_paths_match_oracle(eqf, oracle, case):
             for k in range(len(descs)):
                 ref[k], _ = oracle.step(probs[k], ref[k], "lu")
                 assert _rel(fb.download(k), ref[k]) <= 1e-9, (step, k)
+
+
+def test_empty_cell_lists_and_rejected_calls(eqf, oracle):
+    """No cells: coupling leaves the fields as they are (host and device
+    entry points, and a batched call whose middle trap has none); a batched
+    call still matches per-trap calls around the empty trap.  Calls the
+    library cannot honour come back as EQ_ERR_INVALID with the state intact:
+    a velocity under a CG solver, a velocity of the wrong length, a velocity
+    on the finite-difference scheme, a problem index out of range."""
+    import torch
+    from eq_amd import workloads as Wl
+    shapes = [(80, 40, 4.0, 2.0, 50), (60, 30, 3.0, 1.5, 0), (70, 50, 3.5, 2.5, 40)]
+    npm = 20.0
+    rng = Wl.SplitMix64(31)
+    descs, cells, rates, u0, grids = [], [], [], [], []
+    for (nx, ny, W, H, nc) in shapes:
+        c, r = Wl.seeded_cells(rng, nc, W, H, 2)
+        cells.append(c)
+        rates.append(r)
+        grids.append(oracle.grid(nx + 1, ny + 1, npm, W, H))
+        g = np.random.default_rng(nx + ny)
+        for s in range(2):
+            descs.append(eqf.field_desc(nx, ny, W, H, 0.1, 100.0))
+            u0.append(g.random((nx + 1) * (ny + 1)))
+    off = np.concatenate([[0], np.cumsum([c.shape[0] for c in cells])]).astype(np.int64)
+    dev = torch.device("cuda")
+    cells_d = torch.from_numpy(np.concatenate(cells)).to(dev).contiguous()
+    q_d = torch.from_numpy(np.concatenate(rates)).to(dev).contiguous()
+    samp_d = torch.zeros((int(off[-1]), 2), dtype=torch.float64, device=dev)
+    cnt_d = torch.zeros(int(off[-1]), dtype=torch.int32, device=dev)
+    with eqf.FieldBatch(descs, nspecies=2, solver=eqf.solver_desc(eqf.MIXED_GMG_CG)) as fb:
+        for k, u in enumerate(u0):
+            fb.upload(k, u)
+        # no cells at all, host entry point
+        fb.couple(1, np.zeros((0, 5)), npm, np.zeros((0, 2)))
+        for k, u in enumerate(u0):
+            assert np.array_equal(fb.download(k), u)
+        fb.couple_traps_dev(0, off, cells_d.data_ptr(), npm, q_d.data_ptr(), samp_d.data_ptr(), cnt_d.data_ptr())
+        fb.sync()
+        got = [fb.download(k) for k in range(len(descs))]
+        # rejected calls leave everything as it was
+        vx = np.linspace(0.0, 1.0, 41)
+        with pytest.raises(eqf.EqError) as e:
+            fb.set_velocity(0, vx[:-1])
+        assert e.value.code == 1
+        with pytest.raises(eqf.EqError) as e:
+            fb.set_velocity(len(descs), vx)
+        assert e.value.code == 1
+        fb.set_velocity(0, vx)
+        with pytest.raises(eqf.EqError) as e:
+            fb.step()
+        assert e.value.code == 1 and "BiCGStab" in str(e.value)
+        fb.set_velocity(0, None)
+        st = fb.step()
+        assert st.nconverged == len(descs)
+    for t in range(len(shapes)):
+        ref, s_ref, c_ref = oracle.couple(grids[t], cells[t], [u0[2 * t], u0[2 * t + 1]], rates[t])
+        a, b = off[t], off[t + 1]
+        assert np.array_equal(cnt_d.cpu().numpy()[a:b], c_ref)
+        for s in range(2):
+            assert np.array_equal(got[2 * t + s], ref[s]), (t, s)
+    fd = eqf.field_desc(20, 10, 2.0, 1.0, 0.1, 10.0, scheme=eqf.SCHEME_FD)
+    with eqf.FieldBatch([fd], solver=eqf.solver_desc(eqf.GMG_BICGSTAB)) as fb:
+        with pytest.raises(eqf.EqError) as e:
+            fb.set_velocity(0, np.ones(11))
+        assert e.value.code == 1
