@@ -39,7 +39,9 @@ constexpr int kBlock = kTileW * kWaves;
 // coarse levels (too few nodes there to give up parallelism)
 constexpr int kRows0 = 4;
 constexpr int kRowsC = 1;
-constexpr int rows_for_level(int l) { return l == 0 ? kRows0 : kRowsC; }
+// coarse levels with at least this many nodes (over the batch) take level 0's
+// tile shape: they are bandwidth-bound, the smaller ones latency-bound
+constexpr double kWideRowsMinNodes = 8.0 * 1024.0 * 1024.0;
 constexpr int kRowPad = 64; // pitch granularity in elements
 
 // walls in eQ::boundaryCondition::wall order (src/eQ.h:335-340)

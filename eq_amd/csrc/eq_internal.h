@@ -48,6 +48,7 @@ struct Level {
     TileRef* d_tiles_bottom = nullptr;
     TileRef* d_tiles_active = nullptr;
     int32_t* d_bottom = nullptr;
+    int R = 1;                        // rows per thread of the level's tiles (tile height 4R)
     int64_t pool = 0;                 // elements per vector on this level
     double nodes_all = 0.0;           // grid nodes summed over the level's problems
     double nodes_smooth = 0.0;        // ... over the problems that smooth here
@@ -85,7 +86,7 @@ struct eq_batch {
     int32_t mixed = 0;                 // 1: FP32 V-cycle
     bool any_tensor = false;           // some problem has per-node rows (tensor field or advection)
     std::vector<std::vector<std::vector<double>>> vel_host; // per problem, per level: row velocities
-    std::vector<void*> vel_bufs;       // device copies of the row velocities
+    std::vector<void*> vel_bufs;       // device copies of the row velocities: [(prob * levels cap + level) * 2 + {0: V-cycle precision, 1: FP64 on level 0}], allocated on first use and reused
     int64_t lds_budget = 64 * 1024;    // LDS bytes the bottom V-cycle may use per workgroup
     std::vector<eqk::Level> levels;
     // level 0 FP64 Krylov vectors (pooled over problems)

@@ -1037,18 +1037,28 @@ __global__ __launch_bounds__(kBlock) void k_residual(LevelView L, const T* __res
     });
 }
 
-template <typename T>
+// RC: rows per thread of the coarse level's tiles; all RC values are gathered
+// before the first store
+template <int RC, typename T>
 __global__ __launch_bounds__(kBlock) void k_restrict(LevelView Lc, const LevelProb* __restrict__ fine,
                                                      const T* __restrict__ rf, T* __restrict__ bc, const CgState* st)
 {
     NodeCtx c;
-    if (!decode<kRowsC>(Lc, c)) return;
+    if (!decode<RC>(Lc, c)) return;
     const LevelProb Pc = *c.P;
     if (st[Pc.prob].done) return;
     const LevelProb Pf = fine[Pc.fidx];
-    each_row<kRowsC>(c, [&](const NodeCtx& n) {
-        if (!n.in) return;
-        bc[n.idx] = is_dirichlet(Pc, n.i, n.j) ? (T)0 : restrict_val<T>(Pc, Pf, n.i, n.j, rf, Pf.origin, Pf.pitch);
+    bool act[RC];
+    T val[RC];
+    static_rows<RC>([&](auto qq) {
+        constexpr int q = decltype(qq)::value;
+        const int i = c.i + q;
+        act[q] = (i <= c.ny) && (c.j <= c.nx);
+        val[q] = (act[q] && !is_dirichlet(Pc, i, c.j)) ? restrict_val<T>(Pc, Pf, i, c.j, rf, Pf.origin, Pf.pitch) : (T)0;
+    });
+    static_rows<RC>([&](auto qq) {
+        constexpr int q = decltype(qq)::value;
+        if (act[q]) bc[c.idx + (int64_t)q * c.pitch] = val[q];
     });
 }
 
@@ -1167,14 +1177,14 @@ __global__ __launch_bounds__(kBlock) void k_presmooth2(LevelView L, const TB* __
     });
 }
 
-// residual of the fine level and its restriction, one coarse node per thread
-template <typename T, bool TENS>
+// residual of the fine level and its restriction, RC coarse nodes per thread
+template <int RC, typename T, bool TENS>
 __global__ __launch_bounds__(kBlock) void k_residual_restrict(LevelView Lc, const LevelProb* __restrict__ fine,
                                                               const T* __restrict__ bf, const T* __restrict__ xf,
                                                               T* __restrict__ bc, const CgState* st)
 {
     NodeCtx c;
-    if (!decode<kRowsC>(Lc, c)) return;
+    if (!decode<RC>(Lc, c)) return;
     const LevelProb Pc = *c.P;
     if (st[Pc.prob].done) return;
     const LevelProb Pf = fine[Pc.fidx];
@@ -1182,9 +1192,17 @@ __global__ __launch_bounds__(kBlock) void k_residual_restrict(LevelView Lc, cons
         if (i < 0 || j < 0 || i > Pf.ny || j > Pf.nx || is_dirichlet(Pf, i, j)) return (T)0;
         return residual_val<T, TENS>(Pf, i, j, Pf.origin + k, Pf.pitch, bf, xf, Pf.origin + k);
     };
-    each_row<kRowsC>(c, [&](const NodeCtx& n) {
-        if (!n.in) return;
-        bc[n.idx] = is_dirichlet(Pc, n.i, n.j) ? (T)0 : restrict_gen<T>(Pc, Pf, n.i, n.j, Pf.pitch, racc);
+    bool act[RC];
+    T val[RC];
+    static_rows<RC>([&](auto qq) {
+        constexpr int q = decltype(qq)::value;
+        const int i = c.i + q;
+        act[q] = (i <= c.ny) && (c.j <= c.nx);
+        val[q] = (act[q] && !is_dirichlet(Pc, i, c.j)) ? restrict_gen<T>(Pc, Pf, i, c.j, Pf.pitch, racc) : (T)0;
+    });
+    static_rows<RC>([&](auto qq) {
+        constexpr int q = decltype(qq)::value;
+        if (act[q]) bc[c.idx + (int64_t)q * c.pitch] = val[q];
     });
 }
 
@@ -1810,6 +1828,12 @@ static int build_hierarchy(eq_batch* b)
         nlev = std::max(nlev, (int)dims[p].size());
     }
     b->levels.assign(nlev, Level());
+    for (int l = 0; l < nlev; ++l) {
+        double nodes = 0.0;
+        for (int p = 0; p < b->nprob; ++p)
+            if ((int)dims[p].size() > l) nodes += (double)(dims[p][l].first + 1) * (double)(dims[p][l].second + 1);
+        b->levels[l].R = (l == 0 || nodes >= kWideRowsMinNodes) ? kRows0 : kRowsC;
+    }
     std::vector<std::vector<int>> where(b->nprob, std::vector<int>(nlev, -1));
     for (int l = 0; l < nlev; ++l) {
         Level& L = b->levels[l];
@@ -1841,7 +1865,7 @@ static int build_hierarchy(eq_batch* b)
             P.origin = off + kRowPad + P.pitch;
             off += round_up(kRowPad + rows * P.pitch + kRowPad, kRowPad);
             P.tiles_x = (P.nx + 1 + kTileW - 1) / kTileW;
-            const int th = kWaves * rows_for_level(l); // tile height on this level
+            const int th = kWaves * L.R; // tile height on this level
             P.tiles_y = (P.ny + 1 + th - 1) / th;
             P.ntiles = P.tiles_x * P.tiles_y;
             P.tile_begin = tile;
@@ -1960,7 +1984,7 @@ struct VCycle {
 
     void run(int l)
     {
-        if (l == 0)
+        if (b->levels[l].R == kRows0)
             run_level<kRows0>(l);
         else
             run_level<kRowsC>(l);
@@ -2021,10 +2045,18 @@ struct VCycle {
             Level& C = b->levels[l + 1];
             const int na = (int)C.tiles_active.size();
             if (fuse) {
-                k_residual_restrict<T, TENS><<<grid_for(na), kBlock, 0, s>>>(C.active(), L.d_probs, bl, X[cur], (T*)C.b, st);
+                if (C.R == kRows0)
+                    k_residual_restrict<kRows0, T, TENS><<<grid_for(na), kBlock, 0, s>>>(C.active(), L.d_probs, bl, X[cur],
+                                                                                        (T*)C.b, st);
+                else
+                    k_residual_restrict<kRowsC, T, TENS><<<grid_for(na), kBlock, 0, s>>>(C.active(), L.d_probs, bl, X[cur],
+                                                                                        (T*)C.b, st);
             } else {
                 k_residual<R, T, TENS><<<gs, kBlock, 0, s>>>(L.smooth(), bl, X[cur], (T*)L.r, st);
-                k_restrict<T><<<grid_for(na), kBlock, 0, s>>>(C.active(), L.d_probs, (const T*)L.r, (T*)C.b, st);
+                if (C.R == kRows0)
+                    k_restrict<kRows0, T><<<grid_for(na), kBlock, 0, s>>>(C.active(), L.d_probs, (const T*)L.r, (T*)C.b, st);
+                else
+                    k_restrict<kRowsC, T><<<grid_for(na), kBlock, 0, s>>>(C.active(), L.d_probs, (const T*)L.r, (T*)C.b, st);
             }
             run(l + 1);
             const T* xc = final_buffer(l + 1);
@@ -2778,15 +2810,23 @@ static int set_velocity_impl(eq_batch* b, int32_t prob, const double* vx)
                 r[I] = l == 0 ? vx[I] : rows[l - 1][(size_t)((2LL * I * Pf->ny + P.ny) / (2LL * P.ny))];
             rows.push_back(r);
             std::vector<T> rt(r.begin(), r.end());
-            void* dT = nullptr;
-            if (hipMalloc(&dT, sizeof(T) * rt.size()) != hipSuccess) return set_error(EQ_ERR_NOMEM, "velocity rows");
-            b->vel_bufs.push_back(dT);
+            // one buffer per (problem, level, precision), kept for later calls
+            const size_t slot = ((size_t)prob * kMaxLev + (size_t)l) * 2;
+            if (b->vel_bufs.size() < (size_t)b->nprob * kMaxLev * 2) b->vel_bufs.resize((size_t)b->nprob * kMaxLev * 2, nullptr);
+            if (!b->vel_bufs[slot] && hipMalloc(&b->vel_bufs[slot], sizeof(T) * rt.size()) != hipSuccess) {
+                b->vel_bufs[slot] = nullptr;
+                return set_error(EQ_ERR_NOMEM, "velocity rows");
+            }
+            void* dT = b->vel_bufs[slot];
             EQ_HIP(hipMemcpy(dT, rt.data(), sizeof(T) * rt.size(), hipMemcpyHostToDevice));
             double* dD = nullptr;
             if (l == 0 && b->mixed) {
-                if (hipMalloc((void**)&dD, sizeof(double) * r.size()) != hipSuccess)
+                if (!b->vel_bufs[slot + 1] &&
+                    hipMalloc(&b->vel_bufs[slot + 1], sizeof(double) * r.size()) != hipSuccess) {
+                    b->vel_bufs[slot + 1] = nullptr;
                     return set_error(EQ_ERR_NOMEM, "velocity rows");
-                b->vel_bufs.push_back(dD);
+                }
+                dD = (double*)b->vel_bufs[slot + 1];
                 EQ_HIP(hipMemcpy(dD, r.data(), sizeof(double) * r.size(), hipMemcpyHostToDevice));
             }
             P.vel = 1;

@@ -478,7 +478,7 @@ def test_advection_bicgstab_matches_oracle(eqf, oracle, solver):
     traps: the nonsymmetric operator is solved by GMG-preconditioned BiCGStab;
     three steps against the oracle's LU of the assembled operator; a trap
     without velocity in the same batch, and the operator itself (apply) vs the
-    oracle's CSR."""
+    oracle's CSR; then the flow is reversed and halved for one more step."""
     kind = {"gmg_bicgstab": eqf.GMG_BICGSTAB, "mixed_bicgstab": eqf.MIXED_GMG_BICGSTAB}[solver]
     shapes = [(80, 40, 8.0, 4.0, 50.0), (37, 60, 3.7, 6.0, 20.0), (50, 25, 5.0, 2.5, 0.0)]
     walls = {oracle.TOP: (oracle.DIRICHLET, 0.0, 0, 0), oracle.LEFT: (oracle.ROBIN, 0, 1.0, 0.1)}
@@ -514,6 +514,17 @@ def test_advection_bicgstab_matches_oracle(eqf, oracle, solver):
             for k in range(len(descs)):
                 ref[k], _ = oracle.step(probs[k], ref[k], "lu")
                 assert _rel(fb.download(k), ref[k]) <= 1e-9, (step, k)
+        # a flow that changes between steps: new row velocities on the same batch
+        for k in range(len(descs)):
+            if vel[k] is not None:
+                v2 = -0.5 * vel[k]
+                fb.set_velocity(k, v2)
+                probs[k].set_velocity(v2)
+        st = fb.step()
+        assert st.nconverged == len(descs)
+        for k in range(len(descs)):
+            ref[k], _ = oracle.step(probs[k], ref[k], "lu")
+            assert _rel(fb.download(k), ref[k]) <= 1e-9, ("new flow", k)
 
 
 @pytest.mark.parametrize("case", ["mixed", "mixed-fused", "gmg-fused", "bicgstab-velocity-fused"])
