@@ -10,7 +10,8 @@
 // sum is formed in a fixed order), batched over a range of traps:
 //   1. k_cell_geo     one wavefront per cell: pole window, |S_c| (ballot
 //                     counts), and the 64x16-node tiles the window touches
-//   2. k_pair_scan    exclusive scan of the per-cell tile counts
+//   2. k_pair_block_sums, k_pair_scan, k_pair_offsets
+//                     exclusive scan of the per-cell tile counts
 //   3. k_pairs        one (tile, cell) pair per touched tile, cell-major
 //   4. radix sort     pairs by global tile id, stable, so every tile's cells
 //                     come out in list order
@@ -216,15 +217,45 @@ __global__ __launch_bounds__(256) void k_cell_geo(const eq_cell* __restrict__ ce
     if (lane == 0) geo[c] = g;
 }
 
-// exclusive scan of the per-cell pair counts (single block, deterministic)
-__global__ __launch_bounds__(1024) void k_pair_scan(CellGeo* geo, int64_t nc, int64_t* total)
+// Exclusive scan of the per-cell pair counts into geo[c].poff, in three
+// launches (integer sums: exact, so the order does not matter):
+//   k_pair_block_sums  one block per kScanBlock cells: the block's total
+//   k_pair_scan        one block: exclusive scan of those totals, and the sum
+//   k_pair_offsets     one block per kScanBlock cells: the block's own scan
+//                      plus its offset
+constexpr int kScanBlock = 256;
+
+__device__ __forceinline__ int64_t block_inclusive_scan(int64_t v, int64_t* sh)
+{
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int off = 1; off < kScanBlock; off <<= 1) {
+        const int64_t u = (t >= off) ? sh[t - off] : 0;
+        __syncthreads();
+        sh[t] += u;
+        __syncthreads();
+    }
+    return sh[t];
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_pair_block_sums(const CellGeo* __restrict__ geo, int64_t nc,
+                                                                int64_t* __restrict__ bsum)
+{
+    __shared__ int64_t sh[kScanBlock];
+    const int64_t c = (int64_t)blockIdx.x * kScanBlock + threadIdx.x;
+    const int64_t v = block_inclusive_scan(c < nc ? (int64_t)geo[c].npairs : 0, sh);
+    if (threadIdx.x == kScanBlock - 1) bsum[blockIdx.x] = v;
+}
+
+__global__ __launch_bounds__(1024) void k_pair_scan(int64_t* bsum, int64_t nb, int64_t* total)
 {
     __shared__ int64_t part[1024];
     const int t = threadIdx.x, nt = blockDim.x;
-    const int64_t per = (nc + nt - 1) / nt;
-    const int64_t lo = min(nc, (int64_t)t * per), hi = min(nc, lo + per);
+    const int64_t per = (nb + nt - 1) / nt;
+    const int64_t lo = min(nb, (int64_t)t * per), hi = min(nb, lo + per);
     int64_t s = 0;
-    for (int64_t c = lo; c < hi; ++c) s += geo[c].npairs;
+    for (int64_t k = lo; k < hi; ++k) s += bsum[k];
     part[t] = s;
     __syncthreads();
     // inclusive Hillis-Steele scan of the thread sums
@@ -236,10 +267,21 @@ __global__ __launch_bounds__(1024) void k_pair_scan(CellGeo* geo, int64_t nc, in
     }
     if (t == nt - 1) *total = part[t];
     int64_t acc = part[t] - s;
-    for (int64_t c = lo; c < hi; ++c) {
-        geo[c].poff = acc;
-        acc += geo[c].npairs;
+    for (int64_t k = lo; k < hi; ++k) {
+        const int64_t v = bsum[k];
+        bsum[k] = acc;
+        acc += v;
     }
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_pair_offsets(CellGeo* __restrict__ geo, int64_t nc,
+                                                             const int64_t* __restrict__ boff)
+{
+    __shared__ int64_t sh[kScanBlock];
+    const int64_t c = (int64_t)blockIdx.x * kScanBlock + threadIdx.x;
+    const int64_t v = c < nc ? (int64_t)geo[c].npairs : 0;
+    const int64_t incl = block_inclusive_scan(v, sh);
+    if (c < nc) geo[c].poff = boff[blockIdx.x] + (incl - v);
 }
 
 // pair p of cell c: key = global tile id, value = p; the cell's tiles row by row
@@ -496,15 +538,20 @@ static int couple_impl(eq_batch* b, int32_t t0, int32_t ntr, const int64_t* cell
     // scratch: geo | total | tile start | tile end | (pairs: keys, vals, keys', vals', pair_cell, partial, sort temp)
     const size_t geo_bytes = align256(sizeof(CellGeo) * nc);
     const size_t tb_bytes = align256(sizeof(uint32_t) * ntiles);
-    const size_t head = geo_bytes + 256 + 2 * tb_bytes;
+    const int64_t nblk = (nc + kScanBlock - 1) / kScanBlock;
+    const size_t blk_bytes = align256(sizeof(int64_t) * (size_t)nblk);
+    const size_t head = geo_bytes + 256 + 2 * tb_bytes + blk_bytes;
     if ((rc = ensure_scratch(b, head + 256))) return rc;
     char* base = (char*)b->cell_scratch;
     CellGeo* geo = (CellGeo*)base;
     int64_t* d_total = (int64_t*)(base + geo_bytes);
+    int64_t* d_blk = (int64_t*)(base + geo_bytes + 256 + 2 * tb_bytes);
     const TrapGrid* traps = (const TrapGrid*)b->d_cell_traps;
     auto geometry = [&]() {
         k_cell_geo<<<(unsigned)((nc + 3) / 4), 256, 0, s>>>(d_cells, nc, b->d_cell_off, ntr, t0, traps, npm, geo);
-        k_pair_scan<<<1, 1024, 0, s>>>(geo, nc, d_total);
+        k_pair_block_sums<<<(unsigned)nblk, kScanBlock, 0, s>>>(geo, nc, d_blk);
+        k_pair_scan<<<1, 1024, 0, s>>>(d_blk, nblk, d_total);
+        k_pair_offsets<<<(unsigned)nblk, kScanBlock, 0, s>>>(geo, nc, d_blk);
     };
     geometry();
     int64_t total = 0;
@@ -523,6 +570,7 @@ static int couple_impl(eq_batch* b, int32_t t0, int32_t ntr, const int64_t* cell
         base = (char*)b->cell_scratch;
         geo = (CellGeo*)base;
         d_total = (int64_t*)(base + geo_bytes);
+        d_blk = (int64_t*)(base + geo_bytes + 256 + 2 * tb_bytes);
         geometry();
     }
     uint32_t* t_start = (uint32_t*)(base + geo_bytes + 256);
