@@ -265,10 +265,24 @@ __global__ __launch_bounds__(kBlock) void k_rhs(LevelView L, const double* __res
 {
     NodeCtx c;
     if (!decode<kRows0>(L, c)) return;
-    // a register copy of the problem record: its fields are read once, not
-    // again after every store (the compiler cannot tell the two apart)
     const LevelProb& P = *c.P; // reads P.gw[wall]: a register copy would go to scratch
     double v[1] = {0.0};
+    if (c.full) {
+        // interior tile: no wall terms, no Dirichlet rows; every load first
+        // (Q: a register copy, no wall lookups here)
+        const LevelProb Q = *c.P;
+        Window<kRows0, double> U;
+        U.load(u, c.idx, c.pitch);
+        static_rows<kRows0>([&](auto rr) {
+            constexpr int r = decltype(rr)::value;
+            const double bv = rhs_row(Q, c.i + r, c.j, U.m[r + 1], U.w[r + 1], U.e[r + 1], U.m[r], U.m[r + 2], U.w[r],
+                                      U.e[r + 2]);
+            b[c.idx + (int64_t)r * c.pitch] = bv;
+            v[0] += bv * bv;
+        });
+        tile_partial<1>(v, partial, c.gt);
+        return;
+    }
     each_row<kRows0>(c, [&](const NodeCtx& n) {
         if (!n.in) return;
         double uc, uw, ue, us, un, usw, une;
@@ -301,6 +315,35 @@ __global__ __launch_bounds__(kBlock) void k_init(LevelView L, double* x, const d
     const bool prev = st[P.prob].have_prev != 0;
     const double* __restrict__ src = prev ? xprev : x;
     double v[2] = {0.0, 0.0};
+    if (c.full2) {
+        // the tile and its ring are interior: no Dirichlet node or neighbour;
+        // every load first (Q: a register copy, no wall lookups here)
+        const LevelProb Q = *c.P;
+        Window<kRows0, double> X;
+        X.load(src, c.idx, c.pitch);
+        double bl[kRows0];
+        static_rows<kRows0>([&](auto rr) {
+            constexpr int w = decltype(rr)::value;
+            bl[w] = b[c.idx + (int64_t)w * c.pitch];
+        });
+        static_rows<kRows0>([&](auto rr) {
+            constexpr int w = decltype(rr)::value;
+            const int i = c.i + w;
+            const int64_t k = c.idx + (int64_t)w * c.pitch;
+            if (prev) x[k] = X.m[w + 1];
+            const Row<double> rc = row_coefs<double, TENS>(Q, i, c.j, k);
+            const double rv = bl[w] - X.template apply<TENS>(rc, w);
+            r[k] = rv;
+            v[0] += rv * rv;
+            if (jacobi) {
+                const double zv = rv * row_dinv<TENS>(Q, i, c.j, rc);
+                z[k] = zv;
+                v[1] += rv * zv;
+            }
+        });
+        tile_partial<2>(v, partial, c.gt);
+        return;
+    }
     each_row<kRows0>(c, [&](const NodeCtx& n) {
         if (!n.in) return;
         if (is_dirichlet(P, n.i, n.j)) {
@@ -339,8 +382,14 @@ __global__ __launch_bounds__(kBlock) void k_keep(LevelView L, const double* __re
 {
     NodeCtx c;
     if (!decode<kRows0>(L, c)) return;
-    each_row<kRows0>(c, [&](const NodeCtx& n) {
-        if (n.in) xprev[n.idx] = x[n.idx];
+    double xv[kRows0];
+    static_rows<kRows0>([&](auto rr) {
+        constexpr int r = decltype(rr)::value;
+        if (c.i + r <= c.ny && c.j <= c.nx) xv[r] = x[c.idx + (int64_t)r * c.pitch];
+    });
+    static_rows<kRows0>([&](auto rr) {
+        constexpr int r = decltype(rr)::value;
+        if (c.i + r <= c.ny && c.j <= c.nx) xprev[c.idx + (int64_t)r * c.pitch] = xv[r];
     });
     if (c.gt == c.P->tile_begin && threadIdx.x == 0) st[c.P->prob].have_prev = 1;
 }
@@ -356,6 +405,8 @@ __global__ __launch_bounds__(kBlock) void k_ap_dir(LevelView L, const TP* __rest
 {
     NodeCtx c;
     if (!decode<kRows0>(L, c)) return;
+    // a register copy of the problem record: its fields are read once, not
+    // again after every store (the compiler cannot tell the two apart)
     const LevelProb P = *c.P;
     const CgState& s = st[P.prob];
     if (s.done) return;
@@ -506,6 +557,26 @@ __global__ __launch_bounds__(kBlock) void k_true_residual(LevelView L, const dou
     if (!decode<kRows0>(L, c)) return;
     const LevelProb P = *c.P;
     double v[1] = {0.0};
+    if (c.full) {
+        // interior tile: every load first
+        Window<kRows0, double> X;
+        X.load(x, c.idx, c.pitch);
+        double bl[kRows0];
+        static_rows<kRows0>([&](auto rr) {
+            constexpr int w = decltype(rr)::value;
+            bl[w] = b[c.idx + (int64_t)w * c.pitch];
+        });
+        static_rows<kRows0>([&](auto rr) {
+            constexpr int w = decltype(rr)::value;
+            const int64_t k = c.idx + (int64_t)w * c.pitch;
+            const Row<double> rc = row_coefs<double, TENS>(P, c.i + w, c.j, k);
+            const double rv = bl[w] - X.template apply<TENS>(rc, w);
+            r[k] = rv;
+            v[0] += rv * rv;
+        });
+        tile_partial<1>(v, partial, c.gt);
+        return;
+    }
     each_row<kRows0>(c, [&](const NodeCtx& n) {
         if (!n.in || is_dirichlet(P, n.i, n.j)) return;
         double xc, xw, xe, xs, xn, xsw, xne;
