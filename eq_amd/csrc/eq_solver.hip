@@ -260,20 +260,21 @@ __device__ __forceinline__ void load7(const T* __restrict__ v, int64_t idx, int 
 // ------------------------------------------------------------ level-0 Krylov kernels (FP64)
 
 // b = M u0 + sources; ||b||^2 of the row-replaced system (b_D = g)
+template <int R0>
 __global__ __launch_bounds__(kBlock) void k_rhs(LevelView L, const double* __restrict__ u, double* __restrict__ b,
                                                 double* partial)
 {
     NodeCtx c;
-    if (!decode<kRows0>(L, c)) return;
+    if (!decode<R0>(L, c)) return;
     const LevelProb& P = *c.P; // reads P.gw[wall]: a register copy would go to scratch
     double v[1] = {0.0};
     if (c.full) {
         // interior tile: no wall terms, no Dirichlet rows; every load first
         // (Q: a register copy, no wall lookups here)
         const LevelProb Q = *c.P;
-        Window<kRows0, double> U;
+        Window<R0, double> U;
         U.load(u, c.idx, c.pitch);
-        static_rows<kRows0>([&](auto rr) {
+        static_rows<R0>([&](auto rr) {
             constexpr int r = decltype(rr)::value;
             const double bv = rhs_row(Q, c.i + r, c.j, U.m[r + 1], U.w[r + 1], U.e[r + 1], U.m[r], U.m[r + 2], U.w[r],
                                       U.e[r + 2]);
@@ -283,7 +284,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs(LevelView L, const double* __res
         tile_partial<1>(v, partial, c.gt);
         return;
     }
-    each_row<kRows0>(c, [&](const NodeCtx& n) {
+    each_row<R0>(c, [&](const NodeCtx& n) {
         if (!n.in) return;
         double uc, uw, ue, us, un, usw, une;
         load7(u, n.idx, P.pitch, uc, uw, ue, us, un, usw, une);
@@ -303,14 +304,14 @@ __global__ __launch_bounds__(kBlock) void k_rhs(LevelView L, const double* __res
 // from the previous step's solution when it is kept (xprev), else from the
 // field itself: the field carries this step's cell deposits, whose sharp
 // edges would otherwise make the first residual ~1e4 times larger.
-template <bool TENS>
+template <int R0, bool TENS>
 __global__ __launch_bounds__(kBlock) void k_init(LevelView L, double* x, const double* __restrict__ xprev,
                                                  const double* __restrict__ b, double* __restrict__ r,
                                                  double* __restrict__ z, double* partial, const CgState* st,
                                                  int jacobi)
 {
     NodeCtx c;
-    if (!decode<kRows0>(L, c)) return;
+    if (!decode<R0>(L, c)) return;
     const LevelProb& P = *c.P; // reads P.gw[wall]: a register copy would go to scratch
     const bool prev = st[P.prob].have_prev != 0;
     const double* __restrict__ src = prev ? xprev : x;
@@ -319,14 +320,14 @@ __global__ __launch_bounds__(kBlock) void k_init(LevelView L, double* x, const d
         // the tile and its ring are interior: no Dirichlet node or neighbour;
         // every load first (Q: a register copy, no wall lookups here)
         const LevelProb Q = *c.P;
-        Window<kRows0, double> X;
+        Window<R0, double> X;
         X.load(src, c.idx, c.pitch);
-        double bl[kRows0];
-        static_rows<kRows0>([&](auto rr) {
+        double bl[R0];
+        static_rows<R0>([&](auto rr) {
             constexpr int w = decltype(rr)::value;
             bl[w] = b[c.idx + (int64_t)w * c.pitch];
         });
-        static_rows<kRows0>([&](auto rr) {
+        static_rows<R0>([&](auto rr) {
             constexpr int w = decltype(rr)::value;
             const int i = c.i + w;
             const int64_t k = c.idx + (int64_t)w * c.pitch;
@@ -344,7 +345,7 @@ __global__ __launch_bounds__(kBlock) void k_init(LevelView L, double* x, const d
         tile_partial<2>(v, partial, c.gt);
         return;
     }
-    each_row<kRows0>(c, [&](const NodeCtx& n) {
+    each_row<R0>(c, [&](const NodeCtx& n) {
         if (!n.in) return;
         if (is_dirichlet(P, n.i, n.j)) {
             x[n.idx] = dirichlet_value(P, n.i, n.j);
@@ -377,17 +378,18 @@ __global__ __launch_bounds__(kBlock) void k_init(LevelView L, double* x, const d
 }
 
 // keep the solution as the next step's starting iterate
+template <int R0>
 __global__ __launch_bounds__(kBlock) void k_keep(LevelView L, const double* __restrict__ x,
                                                  double* __restrict__ xprev, CgState* st)
 {
     NodeCtx c;
-    if (!decode<kRows0>(L, c)) return;
-    double xv[kRows0];
-    static_rows<kRows0>([&](auto rr) {
+    if (!decode<R0>(L, c)) return;
+    double xv[R0];
+    static_rows<R0>([&](auto rr) {
         constexpr int r = decltype(rr)::value;
         if (c.i + r <= c.ny && c.j <= c.nx) xv[r] = x[c.idx + (int64_t)r * c.pitch];
     });
-    static_rows<kRows0>([&](auto rr) {
+    static_rows<R0>([&](auto rr) {
         constexpr int r = decltype(rr)::value;
         if (c.i + r <= c.ny && c.j <= c.nx) xprev[c.idx + (int64_t)r * c.pitch] = xv[r];
     });
@@ -398,13 +400,13 @@ __global__ __launch_bounds__(kBlock) void k_keep(LevelView L, const double* __re
 // the storage type TP), q = A p_new in FP64 from the stored values, p_new.q.
 // q is exactly A times the direction x will move along, so the recurrence
 // residual tracks b - A x whatever TP is.  p_old and p_new are different buffers.
-template <typename TP, bool TENS>
+template <int R0, typename TP, bool TENS>
 __global__ __launch_bounds__(kBlock) void k_ap_dir(LevelView L, const TP* __restrict__ z, const TP* __restrict__ pold,
                                                    TP* __restrict__ pnew, double* __restrict__ q, double* partial,
                                                    const CgState* st)
 {
     NodeCtx c;
-    if (!decode<kRows0>(L, c)) return;
+    if (!decode<R0>(L, c)) return;
     // a register copy of the problem record: its fields are read once, not
     // again after every store (the compiler cannot tell the two apart)
     const LevelProb P = *c.P;
@@ -415,20 +417,20 @@ __global__ __launch_bounds__(kBlock) void k_ap_dir(LevelView L, const TP* __rest
     auto dir = [beta](TP zz, TP pp) -> double { return (double)(TP)((double)zz + beta * (double)pp); };
     if (c.full && !(TENS && P.aniso)) {
         // interior tile without a tensor field: every load first, then the R rows
-        Window<kRows0, TP> Z, Q;
+        Window<R0, TP> Z, Q;
         Z.load(z, c.idx, P.pitch);
         Q.load(pold, c.idx, P.pitch);
-        PlainRows<kRows0, double, TENS> V;
+        PlainRows<R0, double, TENS> V;
         V.load(P, c.i);
-        double dw[kRows0 + 2], dm[kRows0 + 2], de[kRows0 + 2];
+        double dw[R0 + 2], dm[R0 + 2], de[R0 + 2];
 #pragma unroll
-        for (int q = 0; q < kRows0 + 2; ++q) {
+        for (int q = 0; q < R0 + 2; ++q) {
             dw[q] = dir(Z.w[q], Q.w[q]);
             dm[q] = dir(Z.m[q], Q.m[q]);
             de[q] = dir(Z.e[q], Q.e[q]);
         }
 #pragma unroll
-        for (int r = 0; r < kRows0; ++r) {
+        for (int r = 0; r < R0; ++r) {
             const int64_t k = c.idx + (int64_t)r * P.pitch;
             const double nc = dm[r + 1];
             const Row<double> ri = V.row(P, r, c.i, c.j);
@@ -441,10 +443,10 @@ __global__ __launch_bounds__(kBlock) void k_ap_dir(LevelView L, const TP* __rest
         return;
     }
     // the other tiles: every row's loads first as well, then the rows
-    Window<kRows0, TP> Z, Q;
+    Window<R0, TP> Z, Q;
     Z.load_rows(z, c.idx, c.pitch, c.i, c.ny);
     Q.load_rows(pold, c.idx, c.pitch, c.i, c.ny);
-    static_rows<kRows0>([&](auto rr) {
+    static_rows<R0>([&](auto rr) {
         constexpr int r = decltype(rr)::value;
         if (!((c.i + r <= c.ny) && (c.j <= c.nx))) return;
         const int i = c.i + r;
@@ -464,14 +466,14 @@ __global__ __launch_bounds__(kBlock) void k_ap_dir(LevelView L, const TP* __rest
 }
 
 // x += alpha p; r -= alpha q; Jacobi: z = r/diag, r.z; flexible: r.z_old
-template <typename TP, bool TENS>
+template <int R0, typename TP, bool TENS>
 __global__ __launch_bounds__(kBlock) void k_update(LevelView L, double* __restrict__ x, double* __restrict__ r,
                                                    TP* __restrict__ z, const TP* __restrict__ p,
                                                    const double* __restrict__ q, double* partial, const CgState* st,
                                                    int jacobi)
 {
     NodeCtx c;
-    if (!decode<kRows0>(L, c)) return;
+    if (!decode<R0>(L, c)) return;
     const LevelProb P = *c.P;
     const CgState& s0 = st[P.prob];
     if (s0.done) return;
@@ -482,10 +484,10 @@ __global__ __launch_bounds__(kBlock) void k_update(LevelView L, double* __restri
     double rr = 0.0, rzx = 0.0;
     if (c.full && !jacobi) {
         // interior tile: every load first, then the R rows
-        double xv[kRows0], rv0[kRows0], qv[kRows0];
-        TP pv[kRows0], zv[kRows0];
+        double xv[R0], rv0[R0], qv[R0];
+        TP pv[R0], zv[R0];
 #pragma unroll
-        for (int w = 0; w < kRows0; ++w) {
+        for (int w = 0; w < R0; ++w) {
             const int64_t k = c.idx + (int64_t)w * P.pitch;
             xv[w] = x[k];
             rv0[w] = r[k];
@@ -494,7 +496,7 @@ __global__ __launch_bounds__(kBlock) void k_update(LevelView L, double* __restri
             zv[w] = flexible ? z[k] : (TP)0;
         }
 #pragma unroll
-        for (int w = 0; w < kRows0; ++w) {
+        for (int w = 0; w < R0; ++w) {
             const int64_t k = c.idx + (int64_t)w * P.pitch;
             x[k] = xv[w] + alpha * (double)pv[w];
             const double rv = rv0[w] - alpha * qv[w];
@@ -507,10 +509,10 @@ __global__ __launch_bounds__(kBlock) void k_update(LevelView L, double* __restri
         return;
     }
     // every row's loads first, then the rows
-    bool act[kRows0];
-    double xv[kRows0], rv0[kRows0], qv[kRows0];
-    TP pv[kRows0], zv0[kRows0];
-    static_rows<kRows0>([&](auto ww) {
+    bool act[R0];
+    double xv[R0], rv0[R0], qv[R0];
+    TP pv[R0], zv0[R0];
+    static_rows<R0>([&](auto ww) {
         constexpr int w = decltype(ww)::value;
         const int64_t k = c.idx + (int64_t)w * c.pitch;
         act[w] = (c.i + w <= c.ny) && (c.j <= c.nx) && !is_dirichlet(P, c.i + w, c.j);
@@ -522,7 +524,7 @@ __global__ __launch_bounds__(kBlock) void k_update(LevelView L, double* __restri
             zv0[w] = (!jacobi && flexible) ? z[k] : (TP)0;
         }
     });
-    static_rows<kRows0>([&](auto ww) {
+    static_rows<R0>([&](auto ww) {
         constexpr int w = decltype(ww)::value;
         if (!act[w]) return;
         const int i = c.i + w;
@@ -548,25 +550,25 @@ __global__ __launch_bounds__(kBlock) void k_update(LevelView L, double* __restri
 }
 
 // true residual r = b - A x (replaces the recurrence residual); final stop test
-template <bool TENS>
+template <int R0, bool TENS>
 __global__ __launch_bounds__(kBlock) void k_true_residual(LevelView L, const double* __restrict__ x,
                                                           const double* __restrict__ b, double* __restrict__ r,
                                                           double* partial)
 {
     NodeCtx c;
-    if (!decode<kRows0>(L, c)) return;
+    if (!decode<R0>(L, c)) return;
     const LevelProb P = *c.P;
     double v[1] = {0.0};
     if (c.full) {
         // interior tile: every load first
-        Window<kRows0, double> X;
+        Window<R0, double> X;
         X.load(x, c.idx, c.pitch);
-        double bl[kRows0];
-        static_rows<kRows0>([&](auto rr) {
+        double bl[R0];
+        static_rows<R0>([&](auto rr) {
             constexpr int w = decltype(rr)::value;
             bl[w] = b[c.idx + (int64_t)w * c.pitch];
         });
-        static_rows<kRows0>([&](auto rr) {
+        static_rows<R0>([&](auto rr) {
             constexpr int w = decltype(rr)::value;
             const int64_t k = c.idx + (int64_t)w * c.pitch;
             const Row<double> rc = row_coefs<double, TENS>(P, c.i + w, c.j, k);
@@ -577,7 +579,7 @@ __global__ __launch_bounds__(kBlock) void k_true_residual(LevelView L, const dou
         tile_partial<1>(v, partial, c.gt);
         return;
     }
-    each_row<kRows0>(c, [&](const NodeCtx& n) {
+    each_row<R0>(c, [&](const NodeCtx& n) {
         if (!n.in || is_dirichlet(P, n.i, n.j)) return;
         double xc, xw, xe, xs, xn, xsw, xne;
         load7(x, n.idx, P.pitch, xc, xw, xe, xs, xn, xsw, xne);
@@ -590,16 +592,16 @@ __global__ __launch_bounds__(kBlock) void k_true_residual(LevelView L, const dou
 }
 
 // Jacobi preconditioner on its own (restart after a residual replacement): z = r/diag, r.z
-template <bool TENS>
+template <int R0, bool TENS>
 __global__ __launch_bounds__(kBlock) void k_jacobi_z(LevelView L, const double* __restrict__ r,
                                                      double* __restrict__ z, double* partial, const CgState* st)
 {
     NodeCtx c;
-    if (!decode<kRows0>(L, c)) return;
+    if (!decode<R0>(L, c)) return;
     const LevelProb P = *c.P;
     if (st[P.prob].done) return;
     double v[1] = {0.0};
-    each_row<kRows0>(c, [&](const NodeCtx& n) {
+    each_row<R0>(c, [&](const NodeCtx& n) {
         if (!n.in) return;
         double zv = 0.0;
         if (!is_dirichlet(P, n.i, n.j)) {
@@ -747,26 +749,28 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(const LevelProb* __re
 // ------------------------------------------------------------ BiCGStab kernels (FP64 vectors)
 
 // r^ = r (start and restarts)
+template <int R0>
 __global__ __launch_bounds__(kBlock) void k_copy(LevelView L, const double* __restrict__ src, double* __restrict__ dst,
                                                  const CgState* st)
 {
     NodeCtx c;
-    if (!decode<kRows0>(L, c)) return;
-    each_row<kRows0>(c, [&](const NodeCtx& n) {
+    if (!decode<R0>(L, c)) return;
+    each_row<R0>(c, [&](const NodeCtx& n) {
         if (n.in) dst[n.idx] = src[n.idx];
     });
 }
 
 // p = r + beta (p - omega v)   (beta = 0: p = r)
+template <int R0>
 __global__ __launch_bounds__(kBlock) void k_bicg_p(LevelView L, const double* __restrict__ r, double* __restrict__ p,
                                                    const double* __restrict__ v, const CgState* st)
 {
     NodeCtx c;
-    if (!decode<kRows0>(L, c)) return;
+    if (!decode<R0>(L, c)) return;
     const CgState& s0 = st[c.P->prob];
     if (s0.done) return;
     const double beta = s0.beta, omega = s0.omega;
-    each_row<kRows0>(c, [&](const NodeCtx& n) {
+    each_row<R0>(c, [&](const NodeCtx& n) {
         if (!n.in) return;
         p[n.idx] = (beta == 0.0) ? r[n.idx] : r[n.idx] + beta * (p[n.idx] - omega * v[n.idx]);
     });
@@ -774,27 +778,27 @@ __global__ __launch_bounds__(kBlock) void k_bicg_p(LevelView L, const double* __
 
 // y = A xh on the free rows (0 on Dirichlet rows), xh in the preconditioner's
 // precision; partials: mode 0: (a, y); mode 1: (y, a) and (y, y)
-template <typename T, bool TENS>
+template <int R0, typename T, bool TENS>
 __global__ __launch_bounds__(kBlock) void k_bicg_apply(LevelView L, const T* __restrict__ xh, double* __restrict__ y,
                                                        const double* __restrict__ a, int mode, double* partial,
                                                        const CgState* st)
 {
     NodeCtx c;
-    if (!decode<kRows0>(L, c)) return;
+    if (!decode<R0>(L, c)) return;
     const LevelProb P = *c.P;
     if (st[P.prob].done) return;
     double d0 = 0.0, d1 = 0.0;
     if (c.full && !(TENS && P.aniso)) {
         // interior tile without a tensor field: every load first
-        Window<kRows0, T> X;
+        Window<R0, T> X;
         X.load(xh, c.idx, P.pitch);
-        double av[kRows0];
+        double av[R0];
 #pragma unroll
-        for (int r = 0; r < kRows0; ++r) av[r] = a[c.idx + (int64_t)r * P.pitch];
-        PlainRows<kRows0, double, TENS> V;
+        for (int r = 0; r < R0; ++r) av[r] = a[c.idx + (int64_t)r * P.pitch];
+        PlainRows<R0, double, TENS> V;
         V.load(P, c.i);
 #pragma unroll
-        for (int r = 0; r < kRows0; ++r) {
+        for (int r = 0; r < R0; ++r) {
             const Row<double> rc = V.row(P, r, c.i, c.j);
             const double yv = row_apply<TENS>(rc, (double)X.m[r + 1], (double)X.w[r + 1], (double)X.e[r + 1],
                                               (double)X.m[r], (double)X.m[r + 2], (double)X.w[r], (double)X.e[r + 2]);
@@ -806,7 +810,7 @@ __global__ __launch_bounds__(kBlock) void k_bicg_apply(LevelView L, const T* __r
         tile_partial<2>(v, partial, c.gt);
         return;
     }
-    each_row<kRows0>(c, [&](const NodeCtx& n) {
+    each_row<R0>(c, [&](const NodeCtx& n) {
         if (!n.in) return;
         double yv = 0.0;
         if (!is_dirichlet(P, n.i, n.j)) {
@@ -825,34 +829,35 @@ __global__ __launch_bounds__(kBlock) void k_bicg_apply(LevelView L, const T* __r
 }
 
 // s = r - alpha v
+template <int R0>
 __global__ __launch_bounds__(kBlock) void k_bicg_s(LevelView L, const double* __restrict__ r,
                                                    const double* __restrict__ v, double* __restrict__ sv,
                                                    const CgState* st)
 {
     NodeCtx c;
-    if (!decode<kRows0>(L, c)) return;
+    if (!decode<R0>(L, c)) return;
     const CgState& s0 = st[c.P->prob];
     if (s0.done) return;
     const double alpha = s0.alpha;
-    each_row<kRows0>(c, [&](const NodeCtx& n) {
+    each_row<R0>(c, [&](const NodeCtx& n) {
         if (n.in) sv[n.idx] = r[n.idx] - alpha * v[n.idx];
     });
 }
 
 // x += alpha p^ + omega s^;  r = s - omega t;  partials (r^, r), (r, r)
-template <typename T>
+template <int R0, typename T>
 __global__ __launch_bounds__(kBlock) void k_bicg_x(LevelView L, double* __restrict__ x, double* __restrict__ r,
                                                    const T* __restrict__ ph, const T* __restrict__ sh,
                                                    const double* __restrict__ sv, const double* __restrict__ t,
                                                    const double* __restrict__ rhat, double* partial, const CgState* st)
 {
     NodeCtx c;
-    if (!decode<kRows0>(L, c)) return;
+    if (!decode<R0>(L, c)) return;
     const CgState& s0 = st[c.P->prob];
     if (s0.done) return;
     const double alpha = s0.alpha, omega = s0.omega;
     double d0 = 0.0, d1 = 0.0;
-    each_row<kRows0>(c, [&](const NodeCtx& n) {
+    each_row<R0>(c, [&](const NodeCtx& n) {
         if (!n.in) return;
         const int64_t k = n.idx;
         x[k] += alpha * (double)ph[k] + omega * (double)sh[k];
@@ -1318,27 +1323,27 @@ __global__ __launch_bounds__(kBlock) void k_prolong_smooth(LevelView Lf, const L
 
 // level-0 problems whose whole V-cycle runs in one workgroup: cast the FP64
 // residual in, and the result out (with the partial r.z)
-template <typename T>
+template <int R0, typename T>
 __global__ __launch_bounds__(kBlock) void k_cast_in(LevelView L, const double* __restrict__ r, T* __restrict__ b,
                                                     const CgState* st)
 {
     NodeCtx c;
-    if (!decode<kRows0>(L, c)) return;
+    if (!decode<R0>(L, c)) return;
     if (st[c.P->prob].done) return;
-    each_row<kRows0>(c, [&](const NodeCtx& n) {
+    each_row<R0>(c, [&](const NodeCtx& n) {
         if (n.in) b[n.idx] = (T)r[n.idx];
     });
 }
 
-template <typename T, typename TZ>
+template <int R0, typename T, typename TZ>
 __global__ __launch_bounds__(kBlock) void k_cast_out(LevelView L, const T* __restrict__ x, const double* __restrict__ r,
                                                      TZ* __restrict__ z, double* partial, const CgState* st)
 {
     NodeCtx c;
-    if (!decode<kRows0>(L, c)) return;
+    if (!decode<R0>(L, c)) return;
     if (st[c.P->prob].done) return;
     double v[1] = {0.0};
-    each_row<kRows0>(c, [&](const NodeCtx& n) {
+    each_row<R0>(c, [&](const NodeCtx& n) {
         if (!n.in) return;
         const TZ zv = (TZ)x[n.idx];
         z[n.idx] = zv;
@@ -1598,15 +1603,15 @@ __global__ __launch_bounds__(kBottomThreads) void k_bottom(BottomArgs<T> A, int 
 // ------------------------------------------------------------ diagnostics
 
 // mode 0: y = A x (Dirichlet rows identity); mode 1: y = b(x) (Dirichlet rows = g)
-template <bool TENS>
+template <int R0, bool TENS>
 __global__ __launch_bounds__(kBlock) void k_apply_full(LevelView L, int pi_only, const double* __restrict__ x,
                                                        double* __restrict__ y, int mode)
 {
     NodeCtx c;
-    if (!decode<kRows0>(L, c)) return;
+    if (!decode<R0>(L, c)) return;
     if (c.pi != pi_only) return;
     const LevelProb& P = *c.P; // reads P.gw[wall]: a register copy would go to scratch
-    each_row<kRows0>(c, [&](const NodeCtx& n) {
+    each_row<R0>(c, [&](const NodeCtx& n) {
         if (!n.in) return;
         double vc, vw, ve, vs, vn, vsw, vne;
         load7(x, n.idx, P.pitch, vc, vw, ve, vs, vn, vsw, vne);
@@ -1903,7 +1908,9 @@ static int build_hierarchy(eq_batch* b)
         double nodes = 0.0;
         for (int p = 0; p < b->nprob; ++p)
             if ((int)dims[p].size() > l) nodes += (double)(dims[p][l].first + 1) * (double)(dims[p][l].second + 1);
-        b->levels[l].R = (l == 0 || nodes >= kWideRowsMinNodes) ? kRows0 : kRowsC;
+        const double min4 = S.tile4_min_nodes > 0 ? (double)S.tile4_min_nodes : kWideRowsMinNodes;
+        const double min8 = S.tile8_min_nodes > 0 ? (double)S.tile8_min_nodes : kWideRows8MinNodes;
+        b->levels[l].R = nodes >= min8 ? kRowsWide : (l == 0 || nodes >= min4) ? kRows0 : kRowsC;
     }
     std::vector<std::vector<int>> where(b->nprob, std::vector<int>(nlev, -1));
     for (int l = 0; l < nlev; ++l) {
@@ -2055,7 +2062,10 @@ struct VCycle {
 
     void run(int l)
     {
-        if (b->levels[l].R == kRows0)
+        const int R = b->levels[l].R;
+        if (R == kRowsWide)
+            run_level<kRowsWide>(l);
+        else if (R == kRows0)
             run_level<kRows0>(l);
         else
             run_level<kRowsC>(l);
@@ -2116,7 +2126,10 @@ struct VCycle {
             Level& C = b->levels[l + 1];
             const int na = (int)C.tiles_active.size();
             if (fuse) {
-                if (C.R == kRows0)
+                if (C.R == kRowsWide)
+                    k_residual_restrict<kRowsWide, T, TENS><<<grid_for(na), kBlock, 0, s>>>(C.active(), L.d_probs, bl,
+                                                                                           X[cur], (T*)C.b, st);
+                else if (C.R == kRows0)
                     k_residual_restrict<kRows0, T, TENS><<<grid_for(na), kBlock, 0, s>>>(C.active(), L.d_probs, bl, X[cur],
                                                                                         (T*)C.b, st);
                 else
@@ -2124,7 +2137,9 @@ struct VCycle {
                                                                                         (T*)C.b, st);
             } else {
                 k_residual<R, T, TENS><<<gs, kBlock, 0, s>>>(L.smooth(), bl, X[cur], (T*)L.r, st);
-                if (C.R == kRows0)
+                if (C.R == kRowsWide)
+                    k_restrict<kRowsWide, T><<<grid_for(na), kBlock, 0, s>>>(C.active(), L.d_probs, (const T*)L.r, (T*)C.b, st);
+                else if (C.R == kRows0)
                     k_restrict<kRows0, T><<<grid_for(na), kBlock, 0, s>>>(C.active(), L.d_probs, (const T*)L.r, (T*)C.b, st);
                 else
                     k_restrict<kRowsC, T><<<grid_for(na), kBlock, 0, s>>>(C.active(), L.d_probs, (const T*)L.r, (T*)C.b, st);
@@ -2165,13 +2180,13 @@ struct VCycle {
             T* xl = final_buffer(l);
             if (top) {
                 const int nb = (int)L.tiles_bottom.size();
-                k_cast_in<T><<<grid_for(nb), kBlock, 0, s>>>(L.bottom_tiles(), in, (T*)L.b, st);
+                k_cast_in<R, T><<<grid_for(nb), kBlock, 0, s>>>(L.bottom_tiles(), in, (T*)L.b, st);
             }
             k_bottom<T, TENS><<<(int)L.bottom.size(), kBottomThreads, L.bottom_lds_bytes, s>>>(args, l, L.d_bottom, xl, omega, S.mg_pre,
                                                                         S.mg_post, (const T*)b->dense, st);
             if (top) {
                 const int nb = (int)L.tiles_bottom.size();
-                k_cast_out<T, T><<<grid_for(nb), kBlock, 0, s>>>(L.bottom_tiles(), xl, in, out, b->partial, st);
+                k_cast_out<R, T, T><<<grid_for(nb), kBlock, 0, s>>>(L.bottom_tiles(), xl, in, out, b->partial, st);
             }
         }
     }
@@ -2186,7 +2201,7 @@ static int read_states(eq_batch* b)
     return EQ_OK;
 }
 
-template <typename T, bool TENS>
+template <typename T, bool TENS, int R0>
 static int solve_all(eq_batch* b, eq_step_stats* stats)
 {
     const auto t0 = std::chrono::steady_clock::now();
@@ -2208,12 +2223,12 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
             finish(kFinRz);
         }
         prof_begin(b, EQ_KERNEL_STENCIL64);
-        k_ap_dir<T, TENS><<<g0, kBlock, 0, s>>>(A, (const T*)b->z, (const T*)pold, (T*)pnew, b->q, b->partial, b->d_state);
+        k_ap_dir<R0, T, TENS><<<g0, kBlock, 0, s>>>(A, (const T*)b->z, (const T*)pold, (T*)pnew, b->q, b->partial, b->d_state);
         // reads z, p_old; writes p_new, q (FP64)
         prof_end(b, EQ_KERNEL_STENCIL64, fine_nodes * (3 * sizeof(T) + sizeof(double)));
         finish(kFinAp);
         prof_begin(b, EQ_KERNEL_CG_UPDATE);
-        k_update<T, TENS><<<g0, kBlock, 0, s>>>(A, b->u, b->r, (T*)b->z, (const T*)pnew, b->q, b->partial, b->d_state,
+        k_update<R0, T, TENS><<<g0, kBlock, 0, s>>>(A, b->u, b->r, (T*)b->z, (const T*)pnew, b->q, b->partial, b->d_state,
                                           jacobi ? 1 : 0);
         // reads x, r, q (FP64) and p (+ z when flexible); writes x, r (+ z for Jacobi)
         prof_end(b, EQ_KERNEL_CG_UPDATE,
@@ -2225,9 +2240,9 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
         iteration(b->p2, b->p);
     };
     auto setup = [&]() {
-        k_rhs<<<g0, kBlock, 0, s>>>(A, b->u, b->rhs, b->partial);
+        k_rhs<R0><<<g0, kBlock, 0, s>>>(A, b->u, b->rhs, b->partial);
         finish(kFinRhs);
-        k_init<TENS><<<g0, kBlock, 0, s>>>(A, b->u, b->xprev, b->rhs, b->r, b->z, b->partial, b->d_state, jacobi ? 1 : 0);
+        k_init<R0, TENS><<<g0, kBlock, 0, s>>>(A, b->u, b->xprev, b->rhs, b->r, b->z, b->partial, b->d_state, jacobi ? 1 : 0);
         finish(kFinInit);
     };
     // Launch sequences are replayed from hipGraphs unless per-launch event
@@ -2275,20 +2290,20 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
         for (int p = 0; p < b->nprob; ++p) all = all && b->h_state[p].done;
         if (all) {
             // confirm on the true residual; problems above rtol restart CG from it
-            k_true_residual<TENS><<<g0, kBlock, 0, s>>>(A, b->u, b->rhs, b->r, b->partial);
+            k_true_residual<R0, TENS><<<g0, kBlock, 0, s>>>(A, b->u, b->rhs, b->r, b->partial);
             finish(kFinTrue);
             if ((rc = read_states(b))) return rc;
             all = true;
             for (int p = 0; p < b->nprob; ++p) all = all && b->h_state[p].done;
             if (all) break;
             if (jacobi) {
-                k_jacobi_z<TENS><<<g0, kBlock, 0, s>>>(A, b->r, b->z, b->partial, b->d_state);
+                k_jacobi_z<R0, TENS><<<g0, kBlock, 0, s>>>(A, b->r, b->z, b->partial, b->d_state);
                 finish(kFinRz);
             }
         }
         chunk = std::max(1, S.check_every);
     }
-    k_keep<<<g0, kBlock, 0, s>>>(A, b->u, b->xprev, b->d_state);
+    k_keep<R0><<<g0, kBlock, 0, s>>>(A, b->u, b->xprev, b->d_state);
     EQ_HIP(hipGetLastError());
     prof_collect(b);
     int32_t itmax = 0, nconv = 0;
@@ -2326,7 +2341,7 @@ using namespace eqk;
 // Right-preconditioned BiCGStab for the nonsymmetric (advection) operator:
 // two V-cycles and two operator applications per iteration, all vectors FP64
 // except the preconditioned directions p^, s^ (the V-cycle's precision).
-template <typename T, bool TENS>
+template <typename T, bool TENS, int R0>
 static int solve_bicgstab(eq_batch* b, eq_step_stats* stats)
 {
     const auto t0 = std::chrono::steady_clock::now();
@@ -2345,28 +2360,28 @@ static int solve_bicgstab(eq_batch* b, eq_step_stats* stats)
         k_finish<<<b->nprob, kFinishThreads, 0, s>>>(L0.d_probs, b->partial, st, op, 0);
     };
     auto iteration = [&]() {
-        k_bicg_p<<<g0, kBlock, 0, s>>>(A, b->r, b->bs_p, b->bs_v, st);
+        k_bicg_p<R0><<<g0, kBlock, 0, s>>>(A, b->r, b->bs_p, b->bs_v, st);
         vp.run0();
         prof_begin(b, EQ_KERNEL_STENCIL64);
-        k_bicg_apply<T, TENS><<<g0, kBlock, 0, s>>>(A, (const T*)b->p, b->bs_v, b->bs_rhat, 0, b->partial, st);
+        k_bicg_apply<R0, T, TENS><<<g0, kBlock, 0, s>>>(A, (const T*)b->p, b->bs_v, b->bs_rhat, 0, b->partial, st);
         prof_end(b, EQ_KERNEL_STENCIL64, L0.nodes_all * (sizeof(T) + 2 * sizeof(double)));
         finish(kFinBAlpha);
-        k_bicg_s<<<g0, kBlock, 0, s>>>(A, b->r, b->bs_v, b->bs_s, st);
+        k_bicg_s<R0><<<g0, kBlock, 0, s>>>(A, b->r, b->bs_v, b->bs_s, st);
         vsc.run0();
-        k_bicg_apply<T, TENS><<<g0, kBlock, 0, s>>>(A, (const T*)b->p2, b->bs_t, b->bs_s, 1, b->partial, st);
+        k_bicg_apply<R0, T, TENS><<<g0, kBlock, 0, s>>>(A, (const T*)b->p2, b->bs_t, b->bs_s, 1, b->partial, st);
         finish(kFinBOmega);
         prof_begin(b, EQ_KERNEL_CG_UPDATE);
-        k_bicg_x<T><<<g0, kBlock, 0, s>>>(A, b->u, b->r, (const T*)b->p, (const T*)b->p2, b->bs_s, b->bs_t,
+        k_bicg_x<R0, T><<<g0, kBlock, 0, s>>>(A, b->u, b->r, (const T*)b->p, (const T*)b->p2, b->bs_s, b->bs_t,
                                           b->bs_rhat, b->partial, st);
         prof_end(b, EQ_KERNEL_CG_UPDATE, L0.nodes_all * (2 * sizeof(T) + 6 * sizeof(double)));
         finish(kFinBUpdate);
     };
     auto setup = [&]() {
-        k_rhs<<<g0, kBlock, 0, s>>>(A, b->u, b->rhs, b->partial);
+        k_rhs<R0><<<g0, kBlock, 0, s>>>(A, b->u, b->rhs, b->partial);
         finish(kFinRhs);
-        k_init<TENS><<<g0, kBlock, 0, s>>>(A, b->u, b->xprev, b->rhs, b->r, b->z, b->partial, st, 0);
+        k_init<R0, TENS><<<g0, kBlock, 0, s>>>(A, b->u, b->xprev, b->rhs, b->r, b->z, b->partial, st, 0);
         finish(kFinBInit);
-        k_copy<<<g0, kBlock, 0, s>>>(A, b->r, b->bs_rhat, st);
+        k_copy<R0><<<g0, kBlock, 0, s>>>(A, b->r, b->bs_rhat, st);
     };
     bool profiling = false;
     for (int k = 0; k < EQ_KERNEL_COUNT_; ++k) profiling = profiling || b->prof[k].enabled;
@@ -2409,9 +2424,9 @@ static int solve_bicgstab(eq_batch* b, eq_step_stats* stats)
         for (int p = 0; p < b->nprob; ++p) all = all && b->h_state[p].done;
         if (all) {
             // confirm on the true residual; the others restart from r^ = r
-            k_true_residual<TENS><<<g0, kBlock, 0, s>>>(A, b->u, b->rhs, b->r, b->partial);
+            k_true_residual<R0, TENS><<<g0, kBlock, 0, s>>>(A, b->u, b->rhs, b->r, b->partial);
             finish(kFinTrue);
-            k_copy<<<g0, kBlock, 0, s>>>(A, b->r, b->bs_rhat, st);
+            k_copy<R0><<<g0, kBlock, 0, s>>>(A, b->r, b->bs_rhat, st);
             finish(kFinBRestart);
             if ((rc = read_states(b))) return rc;
             all = true;
@@ -2420,7 +2435,7 @@ static int solve_bicgstab(eq_batch* b, eq_step_stats* stats)
         }
         chunk = std::max(1, S.check_every);
     }
-    k_keep<<<g0, kBlock, 0, s>>>(A, b->u, b->xprev, st);
+    k_keep<R0><<<g0, kBlock, 0, s>>>(A, b->u, b->xprev, st);
     EQ_HIP(hipGetLastError());
     prof_collect(b);
     int32_t itmax = 0, nconv = 0;
@@ -2451,13 +2466,28 @@ static int run_solve(eq_batch* b, eq_step_stats* stats)
     if (!bicg)
         for (const LevelProb& P : b->levels[0].probs)
             if (P.vel) return set_error(EQ_ERR_INVALID, "advection makes the system nonsymmetric: use a BiCGStab solver");
+    // level 0's rows per thread (tile height) is a template argument of every level-0 kernel
+    if (b->levels[0].R == kRowsWide) {
+        if (bicg) {
+            if (b->mixed)
+                return b->any_tensor ? solve_bicgstab<float, true, kRowsWide>(b, stats)
+                                     : solve_bicgstab<float, false, kRowsWide>(b, stats);
+            return b->any_tensor ? solve_bicgstab<double, true, kRowsWide>(b, stats)
+                                 : solve_bicgstab<double, false, kRowsWide>(b, stats);
+        }
+        if (b->mixed)
+            return b->any_tensor ? solve_all<float, true, kRowsWide>(b, stats) : solve_all<float, false, kRowsWide>(b, stats);
+        return b->any_tensor ? solve_all<double, true, kRowsWide>(b, stats) : solve_all<double, false, kRowsWide>(b, stats);
+    }
     if (bicg) {
         if (b->mixed)
-            return b->any_tensor ? solve_bicgstab<float, true>(b, stats) : solve_bicgstab<float, false>(b, stats);
-        return b->any_tensor ? solve_bicgstab<double, true>(b, stats) : solve_bicgstab<double, false>(b, stats);
+            return b->any_tensor ? solve_bicgstab<float, true, kRows0>(b, stats)
+                                 : solve_bicgstab<float, false, kRows0>(b, stats);
+        return b->any_tensor ? solve_bicgstab<double, true, kRows0>(b, stats)
+                             : solve_bicgstab<double, false, kRows0>(b, stats);
     }
-    if (b->mixed) return b->any_tensor ? solve_all<float, true>(b, stats) : solve_all<float, false>(b, stats);
-    return b->any_tensor ? solve_all<double, true>(b, stats) : solve_all<double, false>(b, stats);
+    if (b->mixed) return b->any_tensor ? solve_all<float, true, kRows0>(b, stats) : solve_all<float, false, kRows0>(b, stats);
+    return b->any_tensor ? solve_all<double, true, kRows0>(b, stats) : solve_all<double, false, kRows0>(b, stats);
 }
 
 extern "C" {
@@ -2481,6 +2511,8 @@ void eq_solver_default(eq_solver_desc* s)
     s->mg_fuse_max_nodes = (int32_t)kFuseMaxNodes;
     s->mg_fuse_pre_max_nodes = (int32_t)kFuseMaxNodes;
     s->mg_bottom_lds_bytes = (int32_t)kBottomLdsMax;
+    s->tile4_min_nodes = (int32_t)kWideRowsMinNodes;
+    s->tile8_min_nodes = (int32_t)kWideRows8MinNodes;
 }
 
 int eq_batch_create(const eq_field_desc* descs, int32_t nprob, int32_t nspecies, const eq_solver_desc* solver,
@@ -3070,9 +3102,15 @@ static int apply_diag(eq_batch* b, int32_t prob, const double* x, double* y, siz
                             nh, hipMemcpyHostToDevice, s));
     const LevelView A = b->levels[0].all();
     if (b->any_tensor)
-        k_apply_full<true><<<grid_for(A.ntiles), kBlock, 0, s>>>(A, prob, b->q, b->z, mode);
+        if (b->levels[0].R == kRowsWide)
+            k_apply_full<kRowsWide, true><<<grid_for(A.ntiles), kBlock, 0, s>>>(A, prob, b->q, b->z, mode);
+        else
+            k_apply_full<kRows0, true><<<grid_for(A.ntiles), kBlock, 0, s>>>(A, prob, b->q, b->z, mode);
     else
-        k_apply_full<false><<<grid_for(A.ntiles), kBlock, 0, s>>>(A, prob, b->q, b->z, mode);
+        if (b->levels[0].R == kRowsWide)
+            k_apply_full<kRowsWide, false><<<grid_for(A.ntiles), kBlock, 0, s>>>(A, prob, b->q, b->z, mode);
+        else
+            k_apply_full<kRows0, false><<<grid_for(A.ntiles), kBlock, 0, s>>>(A, prob, b->q, b->z, mode);
     EQ_HIP(hipGetLastError());
     EQ_HIP(hipMemcpy2DAsync(y, sizeof(double) * nw, b->z + P.origin, sizeof(double) * P.pitch, sizeof(double) * nw,
                             nh, hipMemcpyDeviceToHost, s));

@@ -76,7 +76,8 @@ class SolverDesc(C.Structure):
                 ("mg_coarse_max_nodes", C.c_int32), ("mg_max_levels", C.c_int32),
                 ("check_every", C.c_int32), ("use_graphs", C.c_int32),
                 ("mg_fuse_max_nodes", C.c_int32), ("mg_fuse_pre_max_nodes", C.c_int32),
-                ("mg_bottom_lds_bytes", C.c_int32), ("mg_fuse_pre_top", C.c_int32)]
+                ("mg_bottom_lds_bytes", C.c_int32), ("mg_fuse_pre_top", C.c_int32),
+                ("tile4_min_nodes", C.c_int32), ("tile8_min_nodes", C.c_int32)]
 
 
 class StepStats(C.Structure):

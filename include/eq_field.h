@@ -103,6 +103,10 @@ typedef struct {
                                     (the bottom of the V-cycle); <= 0: 153600        */
     int32_t mg_fuse_pre_top;     /* 1: the finest level's first two sweeps run in one
                                     launch (pays off on large fields)               */
+    int32_t tile4_min_nodes;     /* coarse levels with at least this many nodes (all
+                                    problems) run 4 rows per thread; <= 0: 8 Mi     */
+    int32_t tile8_min_nodes;     /* levels with at least this many run 8 rows per
+                                    thread (level 0 runs at least 4); <= 0: 64 Mi   */
 } eq_solver_desc;
 
 typedef struct {

@@ -527,17 +527,21 @@ def test_advection_bicgstab_matches_oracle(eqf, oracle, solver):
             assert _rel(fb.download(k), ref[k]) <= 1e-9, ("new flow", k)
 
 
-@pytest.mark.parametrize("case", ["mixed", "mixed-fused", "gmg-fused", "bicgstab-velocity-fused"])
+@pytest.mark.parametrize("case", ["mixed", "mixed-fused", "gmg-fused", "bicgstab-velocity-fused", "mixed-tiles8",
+                                  "gmg-fused-tiles8", "bicgstab-velocity-tiles4"])
 def test_interior_tile_paths_match_oracle(eqf, oracle, case):
     """Traps large enough to hold interior tiles (and tiles whose one-node ring
     is interior), where the kernels take their register-window paths: the
     smoother, residual, prolongation, operator, CG update and BiCGStab
     apply, and -- with the finest level's pre-sweeps fused -- the fused
     first two sweeps; with a velocity the rows carry the convection.  Two
-    steps against the oracle's LU."""
+    steps against the oracle's LU.  The tiles8 / tiles4 cases lower the
+    tile-shape thresholds so these traps run 8 rows per thread on every level,
+    or 4 on every coarse level (the shapes large batches take)."""
     kind = {"mixed": eqf.MIXED_GMG_CG, "mixed-fused": eqf.MIXED_GMG_CG, "gmg-fused": eqf.GMG_CG,
-            "bicgstab-velocity-fused": eqf.MIXED_GMG_BICGSTAB}[case]
-    fused = case.endswith("fused")
+            "bicgstab-velocity-fused": eqf.MIXED_GMG_BICGSTAB, "mixed-tiles8": eqf.MIXED_GMG_CG,
+            "gmg-fused-tiles8": eqf.GMG_CG, "bicgstab-velocity-tiles4": eqf.MIXED_GMG_BICGSTAB}[case]
+    fused = "fused" in case
     vel = case.startswith("bicgstab")
     shapes = [(260, 120, 13.0, 6.0), (200, 90, 10.0, 4.5)]
     walls = {oracle.TOP: (oracle.DIRICHLET, 0.0, 0, 0), oracle.LEFT: (oracle.ROBIN, 0, 1.0, 0.2)}
@@ -553,6 +557,10 @@ def test_interior_tile_paths_match_oracle(eqf, oracle, case):
             vels.append(v)
             u0.append(rng.random((nx + 1) * (ny + 1)))
     kw = {"mg_fuse_pre_top": 1, "mg_fuse_max_nodes": 2**31 - 1} if fused else {}
+    if case.endswith("tiles8"):
+        kw["tile8_min_nodes"] = 1
+    if case.endswith("tiles4"):
+        kw["tile4_min_nodes"] = 1
     with eqf.FieldBatch(descs, nspecies=2, solver=eqf.solver_desc(kind, **kw)) as fb:
         for k in range(len(descs)):
             fb.upload(k, u0[k])
