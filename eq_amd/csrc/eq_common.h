@@ -44,9 +44,9 @@ constexpr int kRowsC = 1;
 constexpr double kWideRowsMinNodes = 8.0 * 1024.0 * 1024.0;
 // levels with at least this many nodes take 8 rows per thread (64 x 32 tiles):
 // more reuse per load and one round of blocks, where the launch is large
-// enough for that to pay (config 3 and 4's finest levels, not a 2000 x 1000 trap)
+// enough for that to pay (config 3 and 4's two finest levels, not a 2000 x 1000 trap)
 constexpr int kRowsWide = 8;
-constexpr double kWideRows8MinNodes = 64.0 * 1024.0 * 1024.0;
+constexpr double kWideRows8MinNodes = 32.0 * 1024.0 * 1024.0;
 constexpr int kRowPad = 64; // pitch granularity in elements
 
 // walls in eQ::boundaryCondition::wall order (src/eQ.h:335-340)

@@ -106,7 +106,7 @@ typedef struct {
     int32_t tile4_min_nodes;     /* coarse levels with at least this many nodes (all
                                     problems) run 4 rows per thread; <= 0: 8 Mi     */
     int32_t tile8_min_nodes;     /* levels with at least this many run 8 rows per
-                                    thread (level 0 runs at least 4); <= 0: 64 Mi   */
+                                    thread (level 0 runs at least 4); <= 0: 32 Mi   */
 } eq_solver_desc;
 
 typedef struct {
