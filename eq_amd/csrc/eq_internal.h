@@ -35,6 +35,13 @@ struct CgState {
     double rho, omega, ts, tt;
 };
 
+// one bottom V-cycle workgroup: its problem's level records (bottom level,
+// then each coarser level down to the coarsest) start at rec in the level's
+// bottom record table; nl coarser levels follow the bottom one
+struct BottomRef {
+    int32_t rec, nl, prob, pad_;
+};
+
 struct Level {
     std::vector<LevelProb> probs;     // host copy
     std::vector<TileRef> tiles_all;   // every problem at this level
@@ -42,12 +49,15 @@ struct Level {
     std::vector<TileRef> tiles_bottom;// problems whose bottom level is this one
     std::vector<TileRef> tiles_active;// problems at or above their bottom level (restriction targets)
     std::vector<int32_t> bottom;      // problems whose bottom level is this one
+    std::vector<BottomRef> bottom_refs;   // one per bottom problem
+    std::vector<LevelProb> bottom_recs;   // copies of their level records, chain by chain
     LevelProb* d_probs = nullptr;
     TileRef* d_tiles_all = nullptr;
     TileRef* d_tiles_smooth = nullptr;
     TileRef* d_tiles_bottom = nullptr;
     TileRef* d_tiles_active = nullptr;
-    int32_t* d_bottom = nullptr;
+    BottomRef* d_bottom = nullptr;
+    LevelProb* d_bottom_recs = nullptr;
     int R = 1;                        // rows per thread of the level's tiles (tile height 4R)
     int64_t pool = 0;                 // elements per vector on this level
     double nodes_all = 0.0;           // grid nodes summed over the level's problems

@@ -1354,11 +1354,30 @@ __global__ __launch_bounds__(kBlock) void k_cast_out(LevelView L, const T* __res
 
 // ------------------------------------------------------------ bottom V-cycle (one workgroup per problem, in LDS)
 
+// byte offset of a bottom chain's level records in the dynamic LDS: after
+// the level arrays, 16-byte aligned
 template <typename T>
-struct BottomArgs {
-    const LevelProb* probs[kMaxLev];
-    T* b[kMaxLev];
-};
+__host__ __device__ inline int64_t bottom_rec_offset(const LevelProb& P0)
+{
+    return ((int64_t)P0.lds_total * (int64_t)sizeof(T) + 15) / 16 * 16;
+}
+
+constexpr int kRecWords = (int)(sizeof(LevelProb) / 4);
+
+// A level record read from LDS into uniform (scalar) registers, word by word:
+// a plain copy would hold every field in vector registers.  Words of fields
+// the caller never reads are dropped by the compiler.
+__device__ __forceinline__ LevelProb uniform_rec(const LevelProb* p)
+{
+    LevelProb r;
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(p);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&r);
+#pragma unroll
+    for (int w = 0; w < kRecWords; ++w) dst[w] = __builtin_amdgcn_readfirstlane(src[w]);
+    return r;
+}
+static_assert(sizeof(LevelProb) % 8 == 0, "level records are copied to LDS by 4-byte words and stay 8-byte aligned");
+static_assert(kMaxLev * kRecWords <= 3 * kBottomThreads, "a chain's records load in three words per thread");
 
 #define EQ_FOR_NODES(P)                                                              \
     for (int i = (int)(threadIdx.x >> 6); i <= (P).ny; i += kBottomThreads / 64)      \
@@ -1455,43 +1474,73 @@ __device__ __forceinline__ void lds_sweep(const LevelProb& P, const LdsLevel<T>&
 // The V-cycle from the problem's bottom level down, entirely in LDS: the
 // bottom level's right-hand side is read from HBM, its correction is written
 // to `out` (the buffer its finer level's prolongation reads).  Dirichlet
-// entries stay at the zero the LDS is cleared to.
+// entries stay at the zero the LDS is cleared to.  The chain's level records
+// are copied into LDS behind the level arrays with the right-hand side, in
+// one batch of loads, so no level waits on a record load from memory.
 template <typename T, bool TENS>
-__global__ __launch_bounds__(kBottomThreads) void k_bottom(BottomArgs<T> A, int lb, const int32_t* __restrict__ list,
+__global__ __launch_bounds__(kBottomThreads) void k_bottom(const BottomRef* __restrict__ refs,
+                                                           const LevelProb* __restrict__ recs, const T* __restrict__ bin,
                                                            T* __restrict__ out, T omega, int pre, int post,
                                                            const T* __restrict__ dense, const CgState* st)
 {
-    extern __shared__ unsigned char eq_smem[];
-    __shared__ int idx[kMaxLev];
+    extern __shared__ __align__(16) unsigned char eq_smem[];
     __shared__ int fin[kMaxLev];
-    __shared__ int s_nl;
     __shared__ T red[kBottomThreads]; // coarsest-level partial products
+    const BottomRef ref = refs[blockIdx.x];
+    if (st[ref.prob].done) return;
+    const int nl = ref.nl;
     T* sm = reinterpret_cast<T*>(eq_smem);
-    // level records are copied into (uniform) registers: through a reference
-    // the compiler would reload them after every LDS store it cannot tell apart
-    const LevelProb P0 = A.probs[lb][list[blockIdx.x]];
-    if (st[P0.prob].done) return;
-    if (threadIdx.x == 0) {
-        int n = 0;
-        idx[0] = list[blockIdx.x];
-        while (!A.probs[lb + n][idx[n]].coarsest && lb + n + 1 < kMaxLev) {
-            idx[n + 1] = A.probs[lb + n][idx[n]].cidx;
-            ++n;
-        }
-        s_nl = n;
-    }
-    __syncthreads();
-    const int nl = s_nl;
-    for (int m = threadIdx.x; m < P0.lds_total; m += kBottomThreads) sm[m] = (T)0;
-    __syncthreads();
+    const LevelProb* lrec;
     {
+        // the bottom level's record from memory (uniform registers): its
+        // sizes place the arrays and the records
+        const LevelProb P0 = recs[ref.rec];
+        lrec = reinterpret_cast<const LevelProb*>(eq_smem + bottom_rec_offset<T>(P0));
         const LdsLevel<T> S = lds_level(sm, P0);
-        EQ_FOR_NODES(P0) S.b[S.at(i, j)] = A.b[lb][P0.origin + (int64_t)i * P0.pitch + j];
+        const uint32_t* __restrict__ ws = reinterpret_cast<const uint32_t*>(recs + ref.rec);
+        const int nwords = (nl + 1) * kRecWords;
+        uint32_t wv[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const int t = (int)threadIdx.x + q * kBottomThreads;
+            wv[q] = t < nwords ? ws[t] : 0u;
+        }
+        for (int m = threadIdx.x; m < P0.lds_total; m += kBottomThreads) sm[m] = (T)0;
+        __syncthreads();
+        // right-hand side by batches of kLoad nodes per thread, each batch's
+        // loads ahead of its stores
+        constexpr int kLoad = 8;
+        const int rw = P0.nx + 1, n = rw * (P0.ny + 1);
+        const T* __restrict__ bs = bin + P0.origin;
+        for (int m0 = 0; m0 < n; m0 += kLoad * kBottomThreads) {
+            T v[kLoad];
+            int at[kLoad];
+#pragma unroll
+            for (int q = 0; q < kLoad; ++q) {
+                const int m = m0 + q * kBottomThreads + (int)threadIdx.x;
+                v[q] = (T)0;
+                at[q] = -1;
+                if (m < n) {
+                    const int i = m / rw, j = m - i * rw;
+                    v[q] = bs[(int64_t)i * P0.pitch + j];
+                    at[q] = (int)S.at(i, j);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < kLoad; ++q)
+                if (at[q] >= 0) S.b[at[q]] = v[q];
+        }
+        uint32_t* wd = reinterpret_cast<uint32_t*>(eq_smem + bottom_rec_offset<T>(P0));
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const int t = (int)threadIdx.x + q * kBottomThreads;
+            if (t < nwords) wd[t] = wv[q];
+        }
         __syncthreads();
     }
     for (int k = 0; k < nl; ++k) {
-        const LevelProb P = A.probs[lb + k][__builtin_amdgcn_readfirstlane(idx[k])];
-        const LevelProb Pc = A.probs[lb + k + 1][__builtin_amdgcn_readfirstlane(idx[k + 1])];
+        const LevelProb P = uniform_rec(lrec + k);
+        const LevelProb Pc = uniform_rec(lrec + k + 1);
         const LdsLevel<T> S = lds_level(sm, P), C = lds_level(sm, Pc);
         lds_sweep<T, TENS>(P, S, 0, 0, omega, true);
         int cur = 0;
@@ -1536,7 +1585,7 @@ __global__ __launch_bounds__(kBottomThreads) void k_bottom(BottomArgs<T> A, int 
     // so a group reads whole columns (coalesced) and every load is
     // independent; the group partials are summed in a fixed order.
     {
-        const LevelProb Pc = A.probs[lb + nl][__builtin_amdgcn_readfirstlane(idx[nl])];
+        const LevelProb Pc = uniform_rec(lrec + nl);
         const LdsLevel<T> C = lds_level(sm, Pc);
         const int nw = Pc.nx + 1, n = nw * (Pc.ny + 1);
         const T* D = dense + Pc.dense_off;
@@ -1550,9 +1599,9 @@ __global__ __launch_bounds__(kBottomThreads) void k_bottom(BottomArgs<T> A, int 
         for (int t = threadIdx.x; t < nch * n; t += kBottomThreads) {
             const int ch = t / n, k = t - ch * n;
             T s = (T)0;
-            // the inverse comes from L2 or HBM: 16 independent loads in
+            // the inverse comes from L2 or HBM: kBatch independent loads in
             // flight per batch, then the products in the same m order
-            constexpr int kBatch = 16;
+            constexpr int kBatch = sizeof(T) == 4 ? 32 : 16;
             for (int m0 = ch; m0 < n; m0 += kBatch * nch) {
                 T dv[kBatch];
 #pragma unroll
@@ -1579,8 +1628,8 @@ __global__ __launch_bounds__(kBottomThreads) void k_bottom(BottomArgs<T> A, int 
     }
     int from = 0; // buffer of the coarser level's result
     for (int k = nl - 1; k >= 0; --k) {
-        const LevelProb P = A.probs[lb + k][__builtin_amdgcn_readfirstlane(idx[k])];
-        const LevelProb Pc = A.probs[lb + k + 1][__builtin_amdgcn_readfirstlane(idx[k + 1])];
+        const LevelProb P = uniform_rec(lrec + k);
+        const LevelProb Pc = uniform_rec(lrec + k + 1);
         const LdsLevel<T> S = lds_level(sm, P), C = lds_level(sm, Pc);
         int cur = fin[k];
         EQ_FOR_NODES(P)
@@ -1595,6 +1644,7 @@ __global__ __launch_bounds__(kBottomThreads) void k_bottom(BottomArgs<T> A, int 
         from = cur;
     }
     {
+        const LevelProb P0 = uniform_rec(lrec);
         const LdsLevel<T> S = lds_level(sm, P0);
         EQ_FOR_NODES(P0) out[P0.origin + (int64_t)i * P0.pitch + j] = S.x(from)[S.at(i, j)];
     }
@@ -1863,6 +1913,50 @@ static int validate(const eq_field_desc& d, std::string& why)
     return 0;
 }
 
+// Each bottom problem's chain of level records (its bottom level, then every
+// coarser level) copied out of the level tables, so that the bottom V-cycle
+// reads them in one batch of loads.  Built with the hierarchy and rebuilt,
+// and uploaded again when upload is set, after a level record changes.
+static int sync_bottom_recs(eq_batch* b, bool upload)
+{
+    for (size_t l = 0; l < b->levels.size(); ++l) {
+        Level& L = b->levels[l];
+        L.bottom_refs.clear();
+        L.bottom_recs.clear();
+        for (int32_t k : L.bottom) {
+            BottomRef r{(int32_t)L.bottom_recs.size(), 0, L.probs[k].prob, 0};
+            size_t lev = l;
+            int idx = k;
+            L.bottom_recs.push_back(L.probs[k]);
+            while (!b->levels[lev].probs[idx].coarsest) {
+                idx = b->levels[lev].probs[idx].cidx;
+                ++lev;
+                L.bottom_recs.push_back(b->levels[lev].probs[idx]);
+                ++r.nl;
+            }
+            L.bottom_refs.push_back(r);
+        }
+        if (upload && !L.bottom_recs.empty())
+            EQ_HIP(hipMemcpyAsync(L.d_bottom_recs, L.bottom_recs.data(), sizeof(LevelProb) * L.bottom_recs.size(),
+                                  hipMemcpyHostToDevice, b->stream));
+    }
+    if (upload) EQ_HIP(hipStreamSynchronize(b->stream));
+    return EQ_OK;
+}
+
+// Cells of the coarse grid over n fine cells when the grid cannot be halved
+// exactly: about n/2, rounded up to a multiple of the largest power of two
+// M <= (n/2)/8 (at most 1/8 more cells), so the levels under it halve exactly
+// log2(M) more times and keep the cheap nested transfers.
+static int coarse_cells(int n)
+{
+    if (n <= 2) return n;
+    const int h = (n + 1) / 2;
+    int m = 1;
+    while (2 * m * 8 <= h) m *= 2;
+    return (h + m - 1) / m * m;
+}
+
 template <typename T>
 static int build_hierarchy(eq_batch* b)
 {
@@ -1878,7 +1972,15 @@ static int build_hierarchy(eq_batch* b)
         dims[p].push_back({nx, ny});
         while (mg && (int)dims[p].size() < maxlev &&
                (int64_t)(nx + 1) * (ny + 1) > (int64_t)S.mg_coarse_max_nodes) {
-            const int cx = nx > 2 ? (nx + 1) / 2 : nx, cy = ny > 2 ? (ny + 1) / 2 : ny;
+            int cx, cy;
+            if (nx > 2 && ny > 2 && nx % 2 == 0 && ny % 2 == 0) {
+                // nested: every coarse node is a fine node
+                cx = nx / 2;
+                cy = ny / 2;
+            } else {
+                cx = coarse_cells(nx);
+                cy = coarse_cells(ny);
+            }
             if (cx == nx && cy == ny) break;
             nx = cx;
             ny = cy;
@@ -1888,11 +1990,13 @@ static int build_hierarchy(eq_batch* b)
             return set_error(EQ_ERR_INVALID, "coarsest multigrid level exceeds 1024 nodes; raise mg_max_levels");
         if (mg) {
             // bottom level: the first level from which every coarser level's
-            // three LDS arrays (x0, x1, b with a zero frame) fit in kBottomLds
-            int64_t foot = 0;
+            // three LDS arrays (x0, x1, b with a zero frame) and its level
+            // record fit in the LDS budget
+            int64_t foot = 16; // alignment of the records after the arrays
             int l = (int)dims[p].size();
             while (l > 0) {
-                const int64_t f = 3 * (int64_t)(dims[p][l - 1].first + 3) * (dims[p][l - 1].second + 3) * (int64_t)sizeof(T);
+                const int64_t f = 3 * (int64_t)(dims[p][l - 1].first + 3) * (dims[p][l - 1].second + 3) * (int64_t)sizeof(T) +
+                                  (int64_t)sizeof(LevelProb);
                 if (foot + f > b->lds_budget) break;
                 foot += f;
                 --l;
@@ -1972,10 +2076,7 @@ static int build_hierarchy(eq_batch* b)
             const double nodes = (double)(P.nx + 1) * (double)(P.ny + 1);
             L.nodes_all += nodes;
             if (smooth) L.nodes_smooth += nodes;
-            if (P.bottom) {
-                L.bottom.push_back((int32_t)k);
-                L.bottom_lds_bytes = std::max<size_t>(L.bottom_lds_bytes, (size_t)P.lds_total * sizeof(T));
-            }
+            if (P.bottom) L.bottom.push_back((int32_t)k);
             if (P.coarsest && mg) {
                 std::vector<double> inv;
                 if (!dense_inverse(P, nullptr, nullptr, nullptr, nullptr, inv))
@@ -1985,6 +2086,12 @@ static int build_hierarchy(eq_batch* b)
             }
         }
     }
+    // bottom chains; dynamic LDS = the level arrays, then the level records
+    sync_bottom_recs(b, false);
+    for (Level& L : b->levels)
+        for (const BottomRef& r : L.bottom_refs)
+            L.bottom_lds_bytes = std::max<size_t>(
+                L.bottom_lds_bytes, (size_t)(bottom_rec_offset<T>(L.bottom_recs[r.rec]) + (r.nl + 1) * (int64_t)sizeof(LevelProb)));
     // device tables and vectors
     hipStream_t s = b->stream;
     int rc;
@@ -1993,7 +2100,8 @@ static int build_hierarchy(eq_batch* b)
         if ((rc = upload_vec(&L.d_probs, L.probs, s)) || (rc = upload_vec(&L.d_tiles_all, L.tiles_all, s)) ||
             (rc = upload_vec(&L.d_tiles_smooth, L.tiles_smooth, s)) ||
             (rc = upload_vec(&L.d_tiles_bottom, L.tiles_bottom, s)) ||
-            (rc = upload_vec(&L.d_tiles_active, L.tiles_active, s)) || (rc = upload_vec(&L.d_bottom, L.bottom, s)))
+            (rc = upload_vec(&L.d_tiles_active, L.tiles_active, s)) || (rc = upload_vec(&L.d_bottom, L.bottom_refs, s)) ||
+            (rc = upload_vec(&L.d_bottom_recs, L.bottom_recs, s)))
             return rc;
         if (mg) {
             if ((rc = alloc_zero((T**)&L.b, L.pool, s)) || (rc = alloc_zero((T**)&L.xa, L.pool, s)) ||
@@ -2032,19 +2140,10 @@ static int build_hierarchy(eq_batch* b)
 template <typename T, bool TENS>
 struct VCycle {
     eq_batch* b;
-    BottomArgs<T> args;
     const double* in = nullptr; // level-0 right-hand side (FP64): the vector to precondition
     T* out = nullptr;           // level-0 result
 
-    explicit VCycle(eq_batch* bb) : b(bb), in(bb->r), out((T*)bb->z)
-    {
-        std::memset(&args, 0, sizeof(args));
-        for (size_t l = 0; l < b->levels.size() && l < (size_t)kMaxLev; ++l) {
-            const Level& L = b->levels[l];
-            args.probs[l] = L.d_probs;
-            args.b[l] = (T*)L.b;
-        }
-    }
+    explicit VCycle(eq_batch* bb) : b(bb), in(bb->r), out((T*)bb->z) {}
 
     // z = V(r) on level 0 (and the partial r.z for the next k_finish)
     void run0() { run(0); }
@@ -2182,8 +2281,8 @@ struct VCycle {
                 const int nb = (int)L.tiles_bottom.size();
                 k_cast_in<R, T><<<grid_for(nb), kBlock, 0, s>>>(L.bottom_tiles(), in, (T*)L.b, st);
             }
-            k_bottom<T, TENS><<<(int)L.bottom.size(), kBottomThreads, L.bottom_lds_bytes, s>>>(args, l, L.d_bottom, xl, omega, S.mg_pre,
-                                                                        S.mg_post, (const T*)b->dense, st);
+            k_bottom<T, TENS><<<(int)L.bottom.size(), kBottomThreads, L.bottom_lds_bytes, s>>>(
+                L.d_bottom, L.d_bottom_recs, (const T*)L.b, xl, omega, S.mg_pre, S.mg_post, (const T*)b->dense, st);
             if (top) {
                 const int nb = (int)L.tiles_bottom.size();
                 k_cast_out<R, T, T><<<grid_for(nb), kBlock, 0, s>>>(L.bottom_tiles(), xl, in, out, b->partial, st);
@@ -2610,6 +2709,7 @@ void eq_batch_destroy(eq_batch* b)
         hipFree(L.d_tiles_all);
         hipFree(L.d_tiles_smooth);
         hipFree(L.d_bottom);
+        hipFree(L.d_bottom_recs);
         hipFree(L.d_tiles_bottom);
         hipFree(L.d_tiles_active);
         hipFree(L.b);
@@ -2838,7 +2938,7 @@ static int set_tensor_impl(eq_batch* b, int32_t prob, const double* d11, const d
         EQ_HIP(hipStreamSynchronize(s));
         Pf = &P;
     }
-    return EQ_OK;
+    return sync_bottom_recs(b, true);
 }
 
 } // extern "C++"
@@ -2951,7 +3051,7 @@ static int set_velocity_impl(eq_batch* b, int32_t prob, const double* vx)
         EQ_HIP(hipMemcpy(L.d_probs, L.probs.data(), sizeof(LevelProb) * L.probs.size(), hipMemcpyHostToDevice));
         Pf = &P;
     }
-    return EQ_OK;
+    return sync_bottom_recs(b, true);
 }
 } // extern "C++"
 
