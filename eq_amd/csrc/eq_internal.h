@@ -13,9 +13,17 @@
 
 namespace eqk {
 
-// tile table entry: problem (index into the level table) and tile inside it
+// tile table entry: the tile's problem and place, precomputed so that a block
+// has its node addresses and its problem's state index after one load
 struct TileRef {
-    int32_t pi, local;
+    int32_t pi;          // problem index in the level table
+    int32_t prob;        // global problem index (Krylov state)
+    int32_t gt;          // tile index in the level's full table (partial sums)
+    int32_t local;       // tile index inside the problem
+    int32_t i0, j0;      // first row and column of the tile
+    int32_t pitch, nx, ny;
+    int32_t flags;       // bit 0: tile inside the grid and off the walls; bit 1: so is the ring around it
+    int64_t base;        // element offset of node (i0, j0) in the level pool
 };
 
 struct LevelView {

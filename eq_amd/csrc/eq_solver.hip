@@ -69,6 +69,7 @@ struct NodeCtx {
     int t;      // index in the launched tile table
     int gt;     // tile index in the level's full table (partial sums)
     int pi, i, j;
+    int prob;          // global problem index (Krylov state)
     int pitch, nx, ny; // of the problem, in registers for each_row
     int64_t idx;
     bool in;
@@ -88,22 +89,19 @@ __device__ __forceinline__ bool decode(const LevelView& L, NodeCtx& c)
     if (c.t >= L.ntiles) return false;
     const TileRef tr = L.tiles[c.t];
     c.pi = tr.pi;
+    c.prob = tr.prob;
     c.P = &L.probs[tr.pi];
-    c.gt = c.P->tile_begin + tr.local;
-    const int ty = tr.local / c.P->tiles_x, tx = tr.local - ty * c.P->tiles_x;
-    c.j = tx * kTileW + (int)(threadIdx.x & (kTileW - 1));
-    c.i = ty * (kWaves * R) + (int)(threadIdx.x / kTileW) * R;
-    c.pitch = c.P->pitch;
-    c.nx = c.P->nx;
-    c.ny = c.P->ny;
+    c.gt = tr.gt;
+    const int dj = (int)(threadIdx.x & (kTileW - 1)), di = (int)(threadIdx.x / kTileW) * R;
+    c.j = tr.j0 + dj;
+    c.i = tr.i0 + di;
+    c.pitch = tr.pitch;
+    c.nx = tr.nx;
+    c.ny = tr.ny;
     c.in = (c.i <= c.ny) && (c.j <= c.nx);
-    {
-        // rows i0 .. i0+kWaves*R-1 and columns j0 .. j0+kTileW-1 all in 1 .. n-1
-        const int i0 = ty * (kWaves * R), j0 = tx * kTileW;
-        c.full = i0 >= 1 && j0 >= 1 && i0 + kWaves * R <= c.P->ny && j0 + kTileW <= c.P->nx;
-        c.full2 = i0 >= 2 && j0 >= 2 && i0 + kWaves * R < c.P->ny && j0 + kTileW < c.P->nx;
-    }
-    c.idx = c.P->origin + (int64_t)c.i * c.P->pitch + c.j;
+    c.full = (tr.flags & 1) != 0;
+    c.full2 = (tr.flags & 2) != 0;
+    c.idx = tr.base + (int64_t)di * tr.pitch + dj;
     return true;
 }
 
@@ -313,7 +311,7 @@ __global__ __launch_bounds__(kBlock) void k_init(LevelView L, double* x, const d
     NodeCtx c;
     if (!decode<R0>(L, c)) return;
     const LevelProb& P = *c.P; // reads P.gw[wall]: a register copy would go to scratch
-    const bool prev = st[P.prob].have_prev != 0;
+    const bool prev = st[c.prob].have_prev != 0;
     const double* __restrict__ src = prev ? xprev : x;
     double v[2] = {0.0, 0.0};
     if (c.full2) {
@@ -393,7 +391,7 @@ __global__ __launch_bounds__(kBlock) void k_keep(LevelView L, const double* __re
         constexpr int r = decltype(rr)::value;
         if (c.i + r <= c.ny && c.j <= c.nx) xprev[c.idx + (int64_t)r * c.pitch] = xv[r];
     });
-    if (c.gt == c.P->tile_begin && threadIdx.x == 0) st[c.P->prob].have_prev = 1;
+    if (c.gt == c.P->tile_begin && threadIdx.x == 0) st[c.prob].have_prev = 1;
 }
 
 // p_new = z + beta p_old (formed at the seven stencil points and rounded to
@@ -410,7 +408,7 @@ __global__ __launch_bounds__(kBlock) void k_ap_dir(LevelView L, const TP* __rest
     // a register copy of the problem record: its fields are read once, not
     // again after every store (the compiler cannot tell the two apart)
     const LevelProb P = *c.P;
-    const CgState& s = st[P.prob];
+    const CgState& s = st[c.prob];
     if (s.done) return;
     const double beta = s.beta;
     double v[1] = {0.0};
@@ -475,7 +473,7 @@ __global__ __launch_bounds__(kBlock) void k_update(LevelView L, double* __restri
     NodeCtx c;
     if (!decode<R0>(L, c)) return;
     const LevelProb P = *c.P;
-    const CgState& s0 = st[P.prob];
+    const CgState& s0 = st[c.prob];
     if (s0.done) return;
     const double alpha = s0.alpha;
     const int flexible = s0.flexible;
@@ -599,7 +597,7 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_z(LevelView L, const double* 
     NodeCtx c;
     if (!decode<R0>(L, c)) return;
     const LevelProb P = *c.P;
-    if (st[P.prob].done) return;
+    if (st[c.prob].done) return;
     double v[1] = {0.0};
     each_row<R0>(c, [&](const NodeCtx& n) {
         if (!n.in) return;
@@ -767,7 +765,7 @@ __global__ __launch_bounds__(kBlock) void k_bicg_p(LevelView L, const double* __
 {
     NodeCtx c;
     if (!decode<R0>(L, c)) return;
-    const CgState& s0 = st[c.P->prob];
+    const CgState& s0 = st[c.prob];
     if (s0.done) return;
     const double beta = s0.beta, omega = s0.omega;
     each_row<R0>(c, [&](const NodeCtx& n) {
@@ -786,7 +784,7 @@ __global__ __launch_bounds__(kBlock) void k_bicg_apply(LevelView L, const T* __r
     NodeCtx c;
     if (!decode<R0>(L, c)) return;
     const LevelProb P = *c.P;
-    if (st[P.prob].done) return;
+    if (st[c.prob].done) return;
     double d0 = 0.0, d1 = 0.0;
     if (c.full && !(TENS && P.aniso)) {
         // interior tile without a tensor field: every load first
@@ -836,7 +834,7 @@ __global__ __launch_bounds__(kBlock) void k_bicg_s(LevelView L, const double* __
 {
     NodeCtx c;
     if (!decode<R0>(L, c)) return;
-    const CgState& s0 = st[c.P->prob];
+    const CgState& s0 = st[c.prob];
     if (s0.done) return;
     const double alpha = s0.alpha;
     each_row<R0>(c, [&](const NodeCtx& n) {
@@ -853,7 +851,7 @@ __global__ __launch_bounds__(kBlock) void k_bicg_x(LevelView L, double* __restri
 {
     NodeCtx c;
     if (!decode<R0>(L, c)) return;
-    const CgState& s0 = st[c.P->prob];
+    const CgState& s0 = st[c.prob];
     if (s0.done) return;
     const double alpha = s0.alpha, omega = s0.omega;
     double d0 = 0.0, d1 = 0.0;
@@ -980,7 +978,7 @@ __global__ __launch_bounds__(kBlock) void k_smooth(LevelView L, const TB* __rest
     NodeCtx c;
     if (!decode<R>(L, c)) return;
     const LevelProb P = *c.P;
-    if (st[P.prob].done) return;
+    if (st[c.prob].done) return;
     double v[1] = {0.0};
     if (R > 1 && c.full && !(TENS && P.aniso)) {
         // interior tile without a tensor field: the same arithmetic as
@@ -1068,7 +1066,7 @@ __global__ __launch_bounds__(kBlock) void k_residual(LevelView L, const T* __res
     NodeCtx c;
     if (!decode<R>(L, c)) return;
     const LevelProb P = *c.P;
-    if (st[P.prob].done) return;
+    if (st[c.prob].done) return;
     if (R > 1 && c.full && !(TENS && P.aniso)) {
         T bl[R];
 #pragma unroll
@@ -1190,7 +1188,7 @@ __global__ __launch_bounds__(kBlock) void k_presmooth2(LevelView L, const TB* __
     NodeCtx c;
     if (!decode<R>(L, c)) return;
     const LevelProb P = *c.P;
-    if (st[P.prob].done) return;
+    if (st[c.prob].done) return;
     if (R > 1 && c.full2 && !(TENS && P.aniso)) {
         // the tile and its ring are interior, no tensor field: b over the
         // R+2 rows of the three columns, the first sweep's x there (its row's
@@ -1293,7 +1291,7 @@ __global__ __launch_bounds__(kBlock) void k_prolong_smooth(LevelView Lf, const L
     NodeCtx c;
     if (!decode<R>(Lf, c)) return;
     const LevelProb P = *c.P;
-    if (st[P.prob].done) return;
+    if (st[c.prob].done) return;
     const LevelProb Pc = coarse[P.cidx];
     auto y_at = [&](int i, int j, int64_t k) -> T {
         if (i < 0 || j < 0 || i > P.ny || j > P.nx || is_dirichlet(P, i, j)) return xin[k];
@@ -1329,7 +1327,7 @@ __global__ __launch_bounds__(kBlock) void k_cast_in(LevelView L, const double* _
 {
     NodeCtx c;
     if (!decode<R0>(L, c)) return;
-    if (st[c.P->prob].done) return;
+    if (st[c.prob].done) return;
     each_row<R0>(c, [&](const NodeCtx& n) {
         if (n.in) b[n.idx] = (T)r[n.idx];
     });
@@ -1341,7 +1339,7 @@ __global__ __launch_bounds__(kBlock) void k_cast_out(LevelView L, const T* __res
 {
     NodeCtx c;
     if (!decode<R0>(L, c)) return;
-    if (st[c.P->prob].done) return;
+    if (st[c.prob].done) return;
     double v[1] = {0.0};
     each_row<R0>(c, [&](const NodeCtx& n) {
         if (!n.in) return;
@@ -2067,11 +2065,28 @@ static int build_hierarchy(eq_batch* b)
             if (!P.coarsest) P.cidx = where[P.prob][l + 1];
             const bool smooth = mg && !P.bottom && !P.below;
             const bool active = !P.below;
+            const int th = kWaves * L.R; // tile height on this level
             for (int t = 0; t < P.ntiles; ++t) {
-                L.tiles_all.push_back(TileRef{(int32_t)k, t});
-                if (smooth) L.tiles_smooth.push_back(TileRef{(int32_t)k, t});
-                if (active && P.bottom) L.tiles_bottom.push_back(TileRef{(int32_t)k, t});
-                if (active) L.tiles_active.push_back(TileRef{(int32_t)k, t});
+                TileRef r{};
+                r.pi = (int32_t)k;
+                r.prob = P.prob;
+                r.local = t;
+                r.gt = P.tile_begin + t;
+                const int ty = t / P.tiles_x, tx = t - ty * P.tiles_x;
+                r.i0 = ty * th;
+                r.j0 = tx * kTileW;
+                r.pitch = P.pitch;
+                r.nx = P.nx;
+                r.ny = P.ny;
+                r.base = P.origin + (int64_t)r.i0 * P.pitch + r.j0;
+                // rows i0 .. i0+th-1 and columns j0 .. j0+kTileW-1 all in 1 .. n-1
+                const bool full = r.i0 >= 1 && r.j0 >= 1 && r.i0 + th <= P.ny && r.j0 + kTileW <= P.nx;
+                const bool full2 = r.i0 >= 2 && r.j0 >= 2 && r.i0 + th < P.ny && r.j0 + kTileW < P.nx;
+                r.flags = (full ? 1 : 0) | (full2 ? 2 : 0);
+                L.tiles_all.push_back(r);
+                if (smooth) L.tiles_smooth.push_back(r);
+                if (active && P.bottom) L.tiles_bottom.push_back(r);
+                if (active) L.tiles_active.push_back(r);
             }
             const double nodes = (double)(P.nx + 1) * (double)(P.ny + 1);
             L.nodes_all += nodes;
