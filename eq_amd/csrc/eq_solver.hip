@@ -468,7 +468,7 @@ template <int R0, typename TP, bool TENS>
 __global__ __launch_bounds__(kBlock) void k_update(LevelView L, double* __restrict__ x, double* __restrict__ r,
                                                    TP* __restrict__ z, const TP* __restrict__ p,
                                                    const double* __restrict__ q, double* partial, const CgState* st,
-                                                   int jacobi)
+                                                   int jacobi, TP* __restrict__ x1, TP* __restrict__ bcopy, TP omega)
 {
     NodeCtx c;
     if (!decode<R0>(L, c)) return;
@@ -484,6 +484,8 @@ __global__ __launch_bounds__(kBlock) void k_update(LevelView L, double* __restri
         // interior tile: every load first, then the R rows
         double xv[R0], rv0[R0], qv[R0];
         TP pv[R0], zv[R0];
+        PlainRows<R0, TP, TENS> V;
+        if (x1) V.load(P, c.i);
 #pragma unroll
         for (int w = 0; w < R0; ++w) {
             const int64_t k = c.idx + (int64_t)w * P.pitch;
@@ -501,6 +503,19 @@ __global__ __launch_bounds__(kBlock) void k_update(LevelView L, double* __restri
             r[k] = rv;
             rr += rv * rv;
             rzx += flexible ? rv * (double)zv[w] : 0.0;
+            if (x1) {
+                // the preconditioner's first sweep from zero, as k_smooth forms it
+                TP wd;
+                if (!(TENS && P.aniso)) {
+                    wd = omega * plain_dinv<TP, TENS>(P, V.row(P, w, c.i, c.j));
+                } else {
+                    const Row<TP> rc = row_coefs<TP, TENS>(P, c.i + w, c.j, k);
+                    wd = omega * row_dinv<TENS>(P, c.i + w, c.j, rc);
+                }
+                const TP bv = (TP)rv;
+                x1[k] = wd * bv;
+                if (bcopy) bcopy[k] = bv;
+            }
         }
         double v[3] = {rr, 0.0, rzx};
         tile_partial<3>(v, partial, c.gt);
@@ -541,6 +556,13 @@ __global__ __launch_bounds__(kBlock) void k_update(LevelView L, double* __restri
             t = rv * (double)zv0[w];
         }
         rzx += t;
+        if (x1) {
+            const Row<TP> rc = row_coefs<TP, TENS>(P, i, c.j, k);
+            const TP wd = omega * row_dinv<TENS>(P, i, c.j, rc);
+            const TP bv = (TP)rv;
+            x1[k] = wd * bv;
+            if (bcopy) bcopy[k] = bv;
+        }
     });
     // slot 1: r.z (Jacobi), slot 2: r_new.z_old (flexible)
     double v[3] = {rr, jacobi ? rzx : 0.0, jacobi ? 0.0 : rzx};
@@ -621,9 +643,9 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(const LevelProb* __re
 {
     const LevelProb& P = probs[blockIdx.x];
     CgState& s = st[P.prob];
-    if ((op == kFinAp || op == kFinUpdate || op == kFinRz || op == kFinBAlpha || op == kFinBOmega ||
-         op == kFinBUpdate) && s.done)
-        return;
+    // the partials are summed before the state is tested, so their loads
+    // and the state's are in flight together (a finished problem's partials
+    // are read and dropped)
     double v[3] = {0.0, 0.0, 0.0};
     const double* base = partial + (int64_t)P.tile_begin * 4;
     int q = threadIdx.x;
@@ -642,6 +664,9 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(const LevelProb* __re
         v[1] += p0[1];
         v[2] += p0[2];
     }
+    if ((op == kFinAp || op == kFinUpdate || op == kFinRz || op == kFinBAlpha || op == kFinBOmega ||
+         op == kFinBUpdate) && s.done)
+        return;
     block_sum<3, kFinishThreads>(v);
     if (threadIdx.x != 0) return;
     switch (op) {
@@ -2157,6 +2182,7 @@ struct VCycle {
     eq_batch* b;
     const double* in = nullptr; // level-0 right-hand side (FP64): the vector to precondition
     T* out = nullptr;           // level-0 result
+    bool pre1 = false;          // level 0's first sweep from zero is already in xa (and its b in L.b)
 
     explicit VCycle(eq_batch* bb) : b(bb), in(bb->r), out((T*)bb->z) {}
 
@@ -2213,23 +2239,29 @@ struct VCycle {
             const bool fuse = nodes < fmax;
             const bool fuse_pre =
                 top ? S.mg_fuse_pre_top != 0 : nodes < (S.mg_fuse_pre_max_nodes > 0 ? (double)S.mg_fuse_pre_max_nodes : fmax);
-            if (top) prof_begin(b, EQ_KERNEL_SMOOTH);
-            if (S.mg_pre >= 2 && fuse_pre) {
-                // sweeps 1 and 2 in one launch
-                if (top)
-                    k_presmooth2<R, T, double, TENS><<<gs, kBlock, 0, s>>>(L.smooth(), in, X[1], bcopy, omega, st);
-                else
-                    k_presmooth2<R, T, T, TENS><<<gs, kBlock, 0, s>>>(L.smooth(), bl, X[1], nullptr, omega, st);
-                cur = 1;
-            } else if (top) {
-                k_smooth<R, T, double, T, TENS><<<gs, kBlock, 0, s>>>(L.smooth(), in, nullptr, X[0], bcopy, omega, 1, 0,
-                                                                nullptr, st);
-            } else {
-                k_smooth<R, T, T, T, TENS><<<gs, kBlock, 0, s>>>(L.smooth(), bl, nullptr, X[0], nullptr, omega, 1, 0, nullptr,
-                                                           st);
+            // pre1: the kernel that formed the level-0 residual also wrote
+            // the first sweep into X[0] (and the FP32 copy of the residual)
+            const bool first_done = top && pre1;
+            const bool pair = !first_done && S.mg_pre >= 2 && fuse_pre;
+            if (!first_done) {
+                if (top) prof_begin(b, EQ_KERNEL_SMOOTH);
+                if (pair) {
+                    // sweeps 1 and 2 in one launch
+                    if (top)
+                        k_presmooth2<R, T, double, TENS><<<gs, kBlock, 0, s>>>(L.smooth(), in, X[1], bcopy, omega, st);
+                    else
+                        k_presmooth2<R, T, T, TENS><<<gs, kBlock, 0, s>>>(L.smooth(), bl, X[1], nullptr, omega, st);
+                    cur = 1;
+                } else if (top) {
+                    k_smooth<R, T, double, T, TENS><<<gs, kBlock, 0, s>>>(L.smooth(), in, nullptr, X[0], bcopy, omega, 1,
+                                                                    0, nullptr, st);
+                } else {
+                    k_smooth<R, T, T, T, TENS><<<gs, kBlock, 0, s>>>(L.smooth(), bl, nullptr, X[0], nullptr, omega, 1, 0,
+                                                               nullptr, st);
+                }
+                if (top) prof_end(b, EQ_KERNEL_SMOOTH, nodes * (bytes_b + sizeof(T) * (bcopy ? 2 : 1)));
             }
-            if (top) prof_end(b, EQ_KERNEL_SMOOTH, nodes * (bytes_b + sizeof(T) * (bcopy ? 2 : 1)));
-            for (int k = (S.mg_pre >= 2 && fuse_pre) ? 2 : 1; k < S.mg_pre; ++k) {
+            for (int k = pair ? 2 : 1; k < S.mg_pre; ++k) {
                 if (top) prof_begin(b, EQ_KERNEL_SMOOTH);
                 k_smooth<R, T, T, T, TENS><<<gs, kBlock, 0, s>>>(L.smooth(), bl, X[cur], X[cur ^ 1], nullptr, omega, 0, 0,
                                                            nullptr, st);
@@ -2330,6 +2362,20 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
     auto finish = [&](int op) {
         k_finish<<<b->nprob, kFinishThreads, 0, s>>>(L0.d_probs, b->partial, b->d_state, op, jacobi ? 1 : 0);
     };
+    // The V-cycle's first level-0 sweep (x = omega D^-1 r from x = 0) is
+    // pointwise in the new residual: k_update forms it with r, and after
+    // k_init or the true residual a sweep launch does.  The V-cycle then
+    // starts from its second sweep.
+    T* x1 = jacobi ? nullptr : (T*)L0.xa;
+    T* bcopy = (!jacobi && b->mixed) ? (T*)L0.b : nullptr;
+    const T omega = (T)S.mg_omega;
+    vc.pre1 = !jacobi;
+    auto first_sweep = [&]() {
+        const int ns = (int)L0.tiles_smooth.size();
+        if (jacobi || ns == 0) return;
+        k_smooth<R0, T, double, T, TENS><<<grid_for(ns), kBlock, 0, s>>>(L0.smooth(), b->r, nullptr, x1, bcopy, omega, 1, 0,
+                                                                  nullptr, b->d_state);
+    };
     // one CG iteration from search direction pold to pnew
     auto iteration = [&](double* pold, double* pnew) {
         if (!jacobi) {
@@ -2343,10 +2389,12 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
         finish(kFinAp);
         prof_begin(b, EQ_KERNEL_CG_UPDATE);
         k_update<R0, T, TENS><<<g0, kBlock, 0, s>>>(A, b->u, b->r, (T*)b->z, (const T*)pnew, b->q, b->partial, b->d_state,
-                                          jacobi ? 1 : 0);
-        // reads x, r, q (FP64) and p (+ z when flexible); writes x, r (+ z for Jacobi)
+                                          jacobi ? 1 : 0, x1, bcopy, omega);
+        // reads x, r, q (FP64) and p (+ z when flexible); writes x, r (+ z for
+        // Jacobi; + the first sweep, and the FP32 residual when mixed, otherwise)
         prof_end(b, EQ_KERNEL_CG_UPDATE,
-                 fine_nodes * (5 * sizeof(double) + sizeof(T) + (jacobi ? sizeof(T) : b->mixed ? sizeof(T) : 0)));
+                 fine_nodes * (5 * sizeof(double) + sizeof(T) +
+                               (jacobi ? sizeof(T) : b->mixed ? 3 * sizeof(T) : sizeof(T))));
         finish(kFinUpdate);
     };
     auto two_iterations = [&]() {
@@ -2358,6 +2406,7 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
         finish(kFinRhs);
         k_init<R0, TENS><<<g0, kBlock, 0, s>>>(A, b->u, b->xprev, b->rhs, b->r, b->z, b->partial, b->d_state, jacobi ? 1 : 0);
         finish(kFinInit);
+        first_sweep();
     };
     // Launch sequences are replayed from hipGraphs unless per-launch event
     // timing is on (events need eager launches).
@@ -2406,6 +2455,7 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
             // confirm on the true residual; problems above rtol restart CG from it
             k_true_residual<R0, TENS><<<g0, kBlock, 0, s>>>(A, b->u, b->rhs, b->r, b->partial);
             finish(kFinTrue);
+            first_sweep();
             if ((rc = read_states(b))) return rc;
             all = true;
             for (int p = 0; p < b->nprob; ++p) all = all && b->h_state[p].done;
