@@ -1402,10 +1402,6 @@ __device__ __forceinline__ LevelProb uniform_rec(const LevelProb* p)
 static_assert(sizeof(LevelProb) % 8 == 0, "level records are copied to LDS by 4-byte words and stay 8-byte aligned");
 static_assert(kMaxLev * kRecWords <= 3 * kBottomThreads, "a chain's records load in three words per thread");
 
-#define EQ_FOR_NODES(P)                                                              \
-    for (int i = (int)(threadIdx.x >> 6); i <= (P).ny; i += kBottomThreads / 64)      \
-        for (int j = (int)(threadIdx.x & 63); j <= (P).nx; j += 64)
-
 // LDS arrays of a bottom level: x0, x1, b, each (ny+3) x (nx+3) with a zero frame
 template <typename T>
 struct LdsLevel {
@@ -1414,6 +1410,7 @@ struct LdsLevel {
     T* b;
     int pitch;
     __device__ int64_t at(int i, int j) const { return (int64_t)(i + 1) * pitch + (j + 1); }
+    __device__ int ati(int i, int j) const { return (i + 1) * pitch + (j + 1); }
     __device__ T* x(int k) const { return k ? x1 : x0; } // select, not an indexed array (no scratch)
 };
 
@@ -1429,14 +1426,10 @@ __device__ __forceinline__ LdsLevel<T> lds_level(T* sm, const LevelProb& P)
     return L;
 }
 
-// Visit every node of a bottom level once: interior nodes by rows (whole
-// waves take the interior row, no boundary lanes mixed in), then the
-// boundary ring, one node per thread.
-template <typename FI, typename FB>
-__device__ __forceinline__ void for_nodes_split(const LevelProb& P, FI&& fi, FB&& fb)
+// The boundary ring of a bottom level, one node per thread.
+template <typename FB>
+__device__ __forceinline__ void for_ring(const LevelProb& P, FB&& fb)
 {
-    for (int i = 1 + (int)(threadIdx.x >> 6); i < P.ny; i += kBottomThreads / 64)
-        for (int j = 1 + (int)(threadIdx.x & 63); j < P.nx; j += 64) fi(i, j);
     const int nw = P.nx + 1, nr = 2 * nw + 2 * (P.ny - 1);
     for (int t = threadIdx.x; t < nr; t += kBottomThreads) {
         int i, j;
@@ -1455,6 +1448,17 @@ __device__ __forceinline__ void for_nodes_split(const LevelProb& P, FI&& fi, FB&
         }
         fb(i, j);
     }
+}
+
+// Visit every node of a bottom level once: interior nodes by rows (whole
+// waves take the interior row, no boundary lanes mixed in), then the
+// boundary ring, one node per thread.
+template <typename FI, typename FB>
+__device__ __forceinline__ void for_nodes_split(const LevelProb& P, FI&& fi, FB&& fb)
+{
+    for (int i = 1 + (int)(threadIdx.x >> 6); i < P.ny; i += kBottomThreads / 64)
+        for (int j = 1 + (int)(threadIdx.x & 63); j < P.nx; j += 64) fi(i, j);
+    for_ring(P, fb);
 }
 
 template <typename T, bool TENS>
@@ -1513,6 +1517,22 @@ __global__ __launch_bounds__(kBottomThreads) void k_bottom(const BottomRef* __re
     if (st[ref.prob].done) return;
     const int nl = ref.nl;
     T* sm = reinterpret_cast<T*>(eq_smem);
+    // The coarsest level's dense inverse (below): this thread's first kPre
+    // entries are loaded now, with the right-hand side, and used after the
+    // down-sweep, so the solve there waits on no load from memory.
+    constexpr int kPre = (sizeof(T) == 4 && !TENS) ? 32 : 0;
+    T dpre[kPre > 0 ? kPre : 1];
+    {
+        const LevelProb& Pz = recs[ref.rec + nl];
+        const int n = (Pz.nx + 1) * (Pz.ny + 1), nch = max(1, kBottomThreads / n);
+        const int t = (int)threadIdx.x, ch = t / n, k = t - ch * n;
+        const T* D = dense + Pz.dense_off;
+#pragma unroll
+        for (int u = 0; u < kPre; ++u) {
+            const int m = ch + u * nch;
+            dpre[u] = (t < nch * n && m < n) ? D[(int64_t)m * n + k] : (T)0;
+        }
+    }
     const LevelProb* lrec;
     {
         // the bottom level's record from memory (uniform registers): its
@@ -1597,9 +1617,14 @@ __global__ __launch_bounds__(kBottomThreads) void k_bottom(const BottomRef* __re
                 general);
         }
         __syncthreads();
-        EQ_FOR_NODES(Pc)
         {
-            if (!is_dirichlet(Pc, i, j)) C.b[C.at(i, j)] = restrict_val<T>(Pc, P, i, j, r, S.at(0, 0), S.pitch);
+            // interior coarse nodes are never Dirichlet
+            const int64_t f0 = S.at(0, 0);
+            for_nodes_split(
+                Pc, [&](int i, int j) { C.b[C.ati(i, j)] = restrict_val<T>(Pc, P, i, j, r, f0, S.pitch); },
+                [&](int i, int j) {
+                    if (!is_dirichlet(Pc, i, j)) C.b[C.ati(i, j)] = restrict_val<T>(Pc, P, i, j, r, f0, S.pitch);
+                });
         }
         __syncthreads();
     }
@@ -1618,14 +1643,21 @@ __global__ __launch_bounds__(kBottomThreads) void k_bottom(const BottomRef* __re
             cb[m] = C.b[C.at(mi, mj)];
         }
         __syncthreads();
-        const int nch = max(1, min(kBottomThreads / n, kBottomThreads));
+        // n <= 1024 nodes, so nch * n <= kBottomThreads: one pass, t = threadIdx.x
+        const int nch = max(1, kBottomThreads / n);
         for (int t = threadIdx.x; t < nch * n; t += kBottomThreads) {
             const int ch = t / n, k = t - ch * n;
             T s = (T)0;
-            // the inverse comes from L2 or HBM: kBatch independent loads in
-            // flight per batch, then the products in the same m order
-            constexpr int kBatch = sizeof(T) == 4 ? 32 : 16;
-            for (int m0 = ch; m0 < n; m0 += kBatch * nch) {
+#pragma unroll
+            for (int u = 0; u < kPre; ++u) {
+                const int m = ch + u * nch;
+                if (m < n) s += dpre[u] * cb[m];
+            }
+            // the rest of the inverse comes from L2 or HBM: kBatch
+            // independent loads in flight per batch, then the products in
+            // the same m order
+            constexpr int kBatch = 16;
+            for (int m0 = ch + kPre * nch; m0 < n; m0 += kBatch * nch) {
                 T dv[kBatch];
 #pragma unroll
                 for (int u = 0; u < kBatch; ++u) {
@@ -1655,9 +1687,16 @@ __global__ __launch_bounds__(kBottomThreads) void k_bottom(const BottomRef* __re
         const LevelProb Pc = uniform_rec(lrec + k + 1);
         const LdsLevel<T> S = lds_level(sm, P), C = lds_level(sm, Pc);
         int cur = fin[k];
-        EQ_FOR_NODES(P)
         {
-            if (!is_dirichlet(P, i, j)) S.x(cur)[S.at(i, j)] += prolong_val<T>(P, Pc, i, j, C.x(from), C.at(0, 0), C.pitch);
+            // interior fine nodes are never Dirichlet
+            T* __restrict__ xf = S.x(cur);
+            const T* __restrict__ xcs = C.x(from);
+            const int64_t c0 = C.at(0, 0);
+            for_nodes_split(
+                P, [&](int i, int j) { xf[S.ati(i, j)] += prolong_val<T>(P, Pc, i, j, xcs, c0, C.pitch); },
+                [&](int i, int j) {
+                    if (!is_dirichlet(P, i, j)) xf[S.ati(i, j)] += prolong_val<T>(P, Pc, i, j, xcs, c0, C.pitch);
+                });
         }
         __syncthreads();
         for (int s = 0; s < post; ++s) {
@@ -1669,7 +1708,9 @@ __global__ __launch_bounds__(kBottomThreads) void k_bottom(const BottomRef* __re
     {
         const LevelProb P0 = uniform_rec(lrec);
         const LdsLevel<T> S = lds_level(sm, P0);
-        EQ_FOR_NODES(P0) out[P0.origin + (int64_t)i * P0.pitch + j] = S.x(from)[S.at(i, j)];
+        const T* __restrict__ xs = S.x(from);
+        auto put = [&](int i, int j) { out[P0.origin + (int64_t)i * P0.pitch + j] = xs[S.ati(i, j)]; };
+        for_nodes_split(P0, put, put);
     }
 }
 
@@ -2317,7 +2358,8 @@ struct VCycle {
                                                                    0, nullptr, st);
                     cur ^= 1;
                 }
-                if (top) prof_end(b, EQ_KERNEL_SMOOTH, nodes * (sizeof(T) * 2 + bytes_b));
+                // the last level-0 sweep reads the FP64 residual, the others b in T
+                if (top) prof_end(b, EQ_KERNEL_SMOOTH, nodes * (sizeof(T) * 2 + (last ? bytes_b : sizeof(T))));
             }
         }
         if (!L.bottom.empty()) {
