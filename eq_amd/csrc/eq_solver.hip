@@ -927,8 +927,10 @@ __device__ __forceinline__ T residual_val(const LevelProb& P, int i, int j, int6
 
 // b_c(I,J) = sum_f phi_IJ(f) r_f over the fine nodes within one coarse cell of
 // (I,J); rf(i, j, k) gives r at fine node (i,j), k = i*fpitch + j, and must
-// give 0 one node outside the grid (the nested form reads there).
-template <typename T, typename RF>
+// give 0 one node outside the grid (the nested form reads there).  FETCH:
+// rf is a plain load (fetch the whole support first); otherwise rf computes
+// (visit only the support's nodes).
+template <typename T, bool FETCH, typename RF>
 __device__ __forceinline__ T restrict_gen(const LevelProb& Pc, const LevelProb& Pf, int I, int J, int fpitch, RF&& rf)
 {
     const int nxf = Pf.nx, nyf = Pf.ny, nxc = Pc.nx, nyc = Pc.ny;
@@ -947,21 +949,48 @@ __device__ __forceinline__ T restrict_gen(const LevelProb& Pc, const LevelProb& 
     const int ilo = I == 0 ? 0 : div_floor_pos((I - 1) * nyf, nyc, Pc.rnyf) + 1;
     const int ihi = min(nyf, div_floor_pos((I + 1) * nyf - 1, nyc, Pc.rnyf));
     const T inx = recip_nx<T>(Pf), iny = recip_ny<T>(Pf);
+    if constexpr (!FETCH) {
+        T sum = (T)0;
+        for (int i = ilo; i <= ihi; ++i) {
+            const int dv = i * nyc - I * nyf;
+            for (int j = jlo; j <= jhi; ++j)
+                sum += hat<T>(j * nxc - J * nxf, dv, inx, iny) * rf(i, j, (int64_t)i * fpitch + j);
+        }
+        return sum;
+    }
+    // n_f <= 2 n_c on every axis (the hierarchy never coarsens by more), so
+    // the support holds at most kSup fine nodes per axis: all values are
+    // fetched first, then summed row by row in the order of the loops above
+    constexpr int kSup = 4;
+    T v[kSup][kSup];
+#pragma unroll
+    for (int a = 0; a < kSup; ++a)
+#pragma unroll
+        for (int q = 0; q < kSup; ++q) {
+            const int i = ilo + a, j = jlo + q;
+            v[a][q] = (i <= ihi && j <= jhi) ? rf(i, j, (int64_t)i * fpitch + j) : (T)0;
+        }
     T sum = (T)0;
-    for (int i = ilo; i <= ihi; ++i) {
-        const int dv = i * nyc - I * nyf;
-        for (int j = jlo; j <= jhi; ++j) sum += hat<T>(j * nxc - J * nxf, dv, inx, iny) * rf(i, j, (int64_t)i * fpitch + j);
+#pragma unroll
+    for (int a = 0; a < kSup; ++a) {
+        const int i = ilo + a, dv = i * nyc - I * nyf;
+#pragma unroll
+        for (int q = 0; q < kSup; ++q) {
+            const int j = jlo + q;
+            if (i <= ihi && j <= jhi) sum += hat<T>(j * nxc - J * nxf, dv, inx, iny) * v[a][q];
+        }
     }
     return sum;
 }
 
 // the same from a stored residual: fine node (i,j) at rf[forg + i*fpitch + j],
-// zero frame around the grid
-template <typename T>
+// zero frame around the grid.  FETCH for residuals in memory (the loads'
+// latency is what counts); the bottom solve's LDS copies take the loop.
+template <typename T, bool FETCH = true>
 __device__ __forceinline__ T restrict_val(const LevelProb& Pc, const LevelProb& Pf, int I, int J,
                                           const T* __restrict__ rf, int64_t forg, int fpitch)
 {
-    return restrict_gen<T>(Pc, Pf, I, J, fpitch, [&](int, int, int64_t k) { return rf[forg + k]; });
+    return restrict_gen<T, FETCH>(Pc, Pf, I, J, fpitch, [&](int, int, int64_t k) { return rf[forg + k]; });
 }
 
 // (P x_c) at fine node (i,j): the four corners of the coarse cell holding it;
@@ -1297,7 +1326,7 @@ __global__ __launch_bounds__(kBlock) void k_residual_restrict(LevelView Lc, cons
         constexpr int q = decltype(qq)::value;
         const int i = c.i + q;
         act[q] = (i <= c.ny) && (c.j <= c.nx);
-        val[q] = (act[q] && !is_dirichlet(Pc, i, c.j)) ? restrict_gen<T>(Pc, Pf, i, c.j, Pf.pitch, racc) : (T)0;
+        val[q] = (act[q] && !is_dirichlet(Pc, i, c.j)) ? restrict_gen<T, false>(Pc, Pf, i, c.j, Pf.pitch, racc) : (T)0;
     });
     static_rows<RC>([&](auto qq) {
         constexpr int q = decltype(qq)::value;
@@ -1621,9 +1650,9 @@ __global__ __launch_bounds__(kBottomThreads) void k_bottom(const BottomRef* __re
             // interior coarse nodes are never Dirichlet
             const int64_t f0 = S.at(0, 0);
             for_nodes_split(
-                Pc, [&](int i, int j) { C.b[C.ati(i, j)] = restrict_val<T>(Pc, P, i, j, r, f0, S.pitch); },
+                Pc, [&](int i, int j) { C.b[C.ati(i, j)] = restrict_val<T, false>(Pc, P, i, j, r, f0, S.pitch); },
                 [&](int i, int j) {
-                    if (!is_dirichlet(Pc, i, j)) C.b[C.ati(i, j)] = restrict_val<T>(Pc, P, i, j, r, f0, S.pitch);
+                    if (!is_dirichlet(Pc, i, j)) C.b[C.ati(i, j)] = restrict_val<T, false>(Pc, P, i, j, r, f0, S.pitch);
                 });
         }
         __syncthreads();
@@ -2045,7 +2074,10 @@ static int build_hierarchy(eq_batch* b)
                 cx = coarse_cells(nx);
                 cy = coarse_cells(ny);
             }
+            // every transfer relies on n_f <= 2 n_c per axis (at most four
+            // fine nodes per axis in a coarse hat's support)
             if (cx == nx && cy == ny) break;
+            if (2 * cx < nx || 2 * cy < ny) return set_error(EQ_ERR_INVALID, "internal: coarsening ratio above 2");
             nx = cx;
             ny = cy;
             dims[p].push_back({nx, ny});
