@@ -67,6 +67,7 @@ struct Level {
     BottomRef* d_bottom = nullptr;
     LevelProb* d_bottom_recs = nullptr;
     int R = 1;                        // rows per thread of the level's tiles (tile height 4R)
+    bool nested_down = true;          // every problem smoothed here halves exactly to the next level
     int64_t pool = 0;                 // elements per vector on this level
     double nodes_all = 0.0;           // grid nodes summed over the level's problems
     double nodes_smooth = 0.0;        // ... over the problems that smooth here

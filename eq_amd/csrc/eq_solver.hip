@@ -2189,6 +2189,7 @@ static int build_hierarchy(eq_batch* b)
             const double nodes = (double)(P.nx + 1) * (double)(P.ny + 1);
             L.nodes_all += nodes;
             if (smooth) L.nodes_smooth += nodes;
+            if (smooth && !(P.nx == 2 * P.nxc && P.ny == 2 * P.nyc)) L.nested_down = false;
             if (P.bottom) L.bottom.push_back((int32_t)k);
             if (P.coarsest && mg) {
                 std::vector<double> inv;
@@ -2309,7 +2310,9 @@ struct VCycle {
             // recomputed at the stencil points) win; large levels are
             // bandwidth- and issue-bound and keep the separate launches.
             const double fmax = S.mg_fuse_max_nodes > 0 ? (double)S.mg_fuse_max_nodes : kFuseMaxNodes;
-            const bool fuse = nodes < fmax;
+            // a non-nested transfer recomputes general interpolation weights
+            // at every stencil point when fused: it stays in separate launches
+            const bool fuse = nodes < fmax && L.nested_down;
             const bool fuse_pre =
                 top ? S.mg_fuse_pre_top != 0 : nodes < (S.mg_fuse_pre_max_nodes > 0 ? (double)S.mg_fuse_pre_max_nodes : fmax);
             // pre1: the kernel that formed the level-0 residual also wrote
