@@ -405,9 +405,10 @@ def main():
     value = nodes * args.steps / elapsed
     iters = run.iters[args.warmup:]
     relres = run.relres[args.warmup:]
-    # roofline of the dominant kernel: HIP events on the batch stream around
-    # every launch of it over the next prof_steps steps (events need eager
-    # launches; the kernels are the same ones the graphs replay)
+    # roofline of the dominant kernel: start/stop HIP events attached to every
+    # launch of it (hipExtLaunchKernelGGL on the batch stream) over the next
+    # prof_steps steps (events need eager launches; the kernels are the same
+    # ones the graphs replay)
     roof = None
     prof_steps = min(10, args.steps)
     if dominant is not None and prof_steps > 0:
@@ -427,7 +428,7 @@ def main():
                     "frac": ach / HBM_PEAK_GBS, "traffic": load_traffic(KERNEL_SYMBOLS[dominant]),
                     "kernel": KERNEL_NAMES[dominant], "launches": n, "avg_us": avg_s * 1e6,
                     "bytes_per_launch": by / n,
-                    "timing": f"HIP events on the batch stream around each launch over {prof_steps} steps "
+                    "timing": f"start/stop HIP events attached to each launch on the batch stream over {prof_steps} steps "
                               f"after the timed region (eager launches, {eager_s * 1e3:.3f} ms/step)"}
     fields_final = [run.fb.download(s) for s in range(2)]
     log(f"bench: {value:.4e} DOF-timesteps/s, {elapsed / args.steps * 1e3:.3f} ms/step, "

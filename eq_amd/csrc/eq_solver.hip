@@ -15,6 +15,7 @@
 // node (i,j) at origin + i*pitch + j, zero ghost frame, rows 64-element
 // aligned.  Kernels tile 64 columns x 4 rows per 256-thread block (one
 // wavefront per row segment, coalesced), one launch covers every problem.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -1823,6 +1824,31 @@ void prof_end(eq_batch* b, int kernel, double bytes)
     s.pending_bytes.push_back(bytes);
 }
 
+// Launch kernel<<<grid, kBlock, 0, s>>>(args...).  When profiling slot
+// `slot` is on, the launch carries start and stop events
+// (hipExtLaunchKernelGGL): they bracket the kernel's own execution, not the
+// stream's gaps around it.  slot < 0: never timed.
+template <typename K, typename... Args>
+static void launch_slot(eq_batch* b, int slot, double bytes, K kernel, int grid, hipStream_t s, Args... args)
+{
+    if (slot < 0 || !b->prof[slot].enabled) {
+        hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), 0, s, nullptr, nullptr, 0, args...);
+        return;
+    }
+    ProfileSlot& ps = b->prof[slot];
+    std::pair<hipEvent_t, hipEvent_t> ev;
+    if (!ps.pool.empty()) {
+        ev = ps.pool.back();
+        ps.pool.pop_back();
+    } else {
+        hipEventCreate(&ev.first);
+        hipEventCreate(&ev.second);
+    }
+    hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), 0, s, ev.first, ev.second, 0, args...);
+    ps.pending.push_back(ev);
+    ps.pending_bytes.push_back(bytes);
+}
+
 static void prof_collect(eq_batch* b)
 {
     for (int k = 0; k < EQ_KERNEL_COUNT_; ++k) {
@@ -2319,29 +2345,30 @@ struct VCycle {
             // the first sweep into X[0] (and the FP32 copy of the residual)
             const bool first_done = top && pre1;
             const bool pair = !first_done && S.mg_pre >= 2 && fuse_pre;
+            // level-0 sweeps go to the sweep profiling slot
+            const int slot = top ? EQ_KERNEL_SMOOTH : -1;
             if (!first_done) {
-                if (top) prof_begin(b, EQ_KERNEL_SMOOTH);
+                const double by = nodes * (bytes_b + sizeof(T) * (bcopy ? 2 : 1));
                 if (pair) {
                     // sweeps 1 and 2 in one launch
                     if (top)
-                        k_presmooth2<R, T, double, TENS><<<gs, kBlock, 0, s>>>(L.smooth(), in, X[1], bcopy, omega, st);
+                        launch_slot(b, slot, by, k_presmooth2<R, T, double, TENS>, gs, s, L.smooth(), in, X[1], bcopy,
+                                    omega, st);
                     else
-                        k_presmooth2<R, T, T, TENS><<<gs, kBlock, 0, s>>>(L.smooth(), bl, X[1], nullptr, omega, st);
+                        launch_slot(b, slot, by, k_presmooth2<R, T, T, TENS>, gs, s, L.smooth(), bl, X[1], (T*)nullptr,
+                                    omega, st);
                     cur = 1;
                 } else if (top) {
-                    k_smooth<R, T, double, T, TENS><<<gs, kBlock, 0, s>>>(L.smooth(), in, nullptr, X[0], bcopy, omega, 1,
-                                                                    0, nullptr, st);
+                    launch_slot(b, slot, by, k_smooth<R, T, double, T, TENS>, gs, s, L.smooth(), in, (const T*)nullptr, X[0],
+                                bcopy, omega, 1, 0, (double*)nullptr, st);
                 } else {
-                    k_smooth<R, T, T, T, TENS><<<gs, kBlock, 0, s>>>(L.smooth(), bl, nullptr, X[0], nullptr, omega, 1, 0,
-                                                               nullptr, st);
+                    launch_slot(b, slot, by, k_smooth<R, T, T, T, TENS>, gs, s, L.smooth(), bl, (const T*)nullptr, X[0],
+                                (T*)nullptr, omega, 1, 0, (double*)nullptr, st);
                 }
-                if (top) prof_end(b, EQ_KERNEL_SMOOTH, nodes * (bytes_b + sizeof(T) * (bcopy ? 2 : 1)));
             }
             for (int k = pair ? 2 : 1; k < S.mg_pre; ++k) {
-                if (top) prof_begin(b, EQ_KERNEL_SMOOTH);
-                k_smooth<R, T, T, T, TENS><<<gs, kBlock, 0, s>>>(L.smooth(), bl, X[cur], X[cur ^ 1], nullptr, omega, 0, 0,
-                                                           nullptr, st);
-                if (top) prof_end(b, EQ_KERNEL_SMOOTH, nodes * 3 * sizeof(T));
+                launch_slot(b, slot, nodes * 3 * sizeof(T), k_smooth<R, T, T, T, TENS>, gs, s, L.smooth(), bl,
+                            (const T*)X[cur], X[cur ^ 1], (T*)nullptr, omega, 0, 0, (double*)nullptr, st);
                 cur ^= 1;
             }
             // residual and its restriction over the coarse nodes
@@ -2376,25 +2403,24 @@ struct VCycle {
                 const bool first = (k == 0) && fuse;
                 if (k == 0 && !fuse)
                     k_prolong<R, T><<<gs, kBlock, 0, s>>>(L.smooth(), C.d_probs, xc, X[cur], st);
-                if (top) prof_begin(b, EQ_KERNEL_SMOOTH);
+                // the last level-0 sweep reads the FP64 residual, the others b in T
+                const double by = nodes * (sizeof(T) * 2 + (last ? bytes_b : sizeof(T)));
                 if (top && last) {
                     if (first)
-                        k_prolong_smooth<R, T, double, T, TENS><<<gs, kBlock, 0, s>>>(L.smooth(), C.d_probs, xc, in, X[cur],
-                                                                                out, omega, 1, b->partial, st);
+                        launch_slot(b, slot, by, k_prolong_smooth<R, T, double, T, TENS>, gs, s, L.smooth(), C.d_probs, xc,
+                                    in, (const T*)X[cur], out, omega, 1, b->partial, st);
                     else
-                        k_smooth<R, T, double, T, TENS><<<gs, kBlock, 0, s>>>(L.smooth(), in, X[cur], out, nullptr,
-                                                                        omega, 0, 1, b->partial, st);
+                        launch_slot(b, slot, by, k_smooth<R, T, double, T, TENS>, gs, s, L.smooth(), in, (const T*)X[cur],
+                                    out, (T*)nullptr, omega, 0, 1, b->partial, st);
                 } else {
                     if (first)
-                        k_prolong_smooth<R, T, T, T, TENS><<<gs, kBlock, 0, s>>>(L.smooth(), C.d_probs, xc, bl, X[cur],
-                                                                           X[cur ^ 1], omega, 0, nullptr, st);
+                        launch_slot(b, slot, by, k_prolong_smooth<R, T, T, T, TENS>, gs, s, L.smooth(), C.d_probs, xc, bl,
+                                    (const T*)X[cur], X[cur ^ 1], omega, 0, (double*)nullptr, st);
                     else
-                        k_smooth<R, T, T, T, TENS><<<gs, kBlock, 0, s>>>(L.smooth(), bl, X[cur], X[cur ^ 1], nullptr, omega, 0,
-                                                                   0, nullptr, st);
+                        launch_slot(b, slot, by, k_smooth<R, T, T, T, TENS>, gs, s, L.smooth(), bl, (const T*)X[cur],
+                                    X[cur ^ 1], (T*)nullptr, omega, 0, 0, (double*)nullptr, st);
                     cur ^= 1;
                 }
-                // the last level-0 sweep reads the FP64 residual, the others b in T
-                if (top) prof_end(b, EQ_KERNEL_SMOOTH, nodes * (sizeof(T) * 2 + (last ? bytes_b : sizeof(T))));
             }
         }
         if (!L.bottom.empty()) {
@@ -2459,19 +2485,16 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
             vc.run0();
             finish(kFinRz);
         }
-        prof_begin(b, EQ_KERNEL_STENCIL64);
-        k_ap_dir<R0, T, TENS><<<g0, kBlock, 0, s>>>(A, (const T*)b->z, (const T*)pold, (T*)pnew, b->q, b->partial, b->d_state);
         // reads z, p_old; writes p_new, q (FP64)
-        prof_end(b, EQ_KERNEL_STENCIL64, fine_nodes * (3 * sizeof(T) + sizeof(double)));
+        launch_slot(b, EQ_KERNEL_STENCIL64, fine_nodes * (3 * sizeof(T) + sizeof(double)), k_ap_dir<R0, T, TENS>, g0, s, A,
+                    (const T*)b->z, (const T*)pold, (T*)pnew, b->q, b->partial, (const CgState*)b->d_state);
         finish(kFinAp);
-        prof_begin(b, EQ_KERNEL_CG_UPDATE);
-        k_update<R0, T, TENS><<<g0, kBlock, 0, s>>>(A, b->u, b->r, (T*)b->z, (const T*)pnew, b->q, b->partial, b->d_state,
-                                          jacobi ? 1 : 0, x1, bcopy, omega);
         // reads x, r, q (FP64) and p (+ z when flexible); writes x, r (+ z for
         // Jacobi; + the first sweep, and the FP32 residual when mixed, otherwise)
-        prof_end(b, EQ_KERNEL_CG_UPDATE,
-                 fine_nodes * (5 * sizeof(double) + sizeof(T) +
-                               (jacobi ? sizeof(T) : b->mixed ? 3 * sizeof(T) : sizeof(T))));
+        launch_slot(b, EQ_KERNEL_CG_UPDATE,
+                    fine_nodes * (5 * sizeof(double) + sizeof(T) + (jacobi ? sizeof(T) : b->mixed ? 3 * sizeof(T) : sizeof(T))),
+                    k_update<R0, T, TENS>, g0, s, A, b->u, b->r, (T*)b->z, (const T*)pnew, (const double*)b->q, b->partial,
+                    (const CgState*)b->d_state, jacobi ? 1 : 0, x1, bcopy, omega);
         finish(kFinUpdate);
     };
     auto two_iterations = [&]() {
@@ -2603,18 +2626,15 @@ static int solve_bicgstab(eq_batch* b, eq_step_stats* stats)
     auto iteration = [&]() {
         k_bicg_p<R0><<<g0, kBlock, 0, s>>>(A, b->r, b->bs_p, b->bs_v, st);
         vp.run0();
-        prof_begin(b, EQ_KERNEL_STENCIL64);
-        k_bicg_apply<R0, T, TENS><<<g0, kBlock, 0, s>>>(A, (const T*)b->p, b->bs_v, b->bs_rhat, 0, b->partial, st);
-        prof_end(b, EQ_KERNEL_STENCIL64, L0.nodes_all * (sizeof(T) + 2 * sizeof(double)));
+        launch_slot(b, EQ_KERNEL_STENCIL64, L0.nodes_all * (sizeof(T) + 2 * sizeof(double)), k_bicg_apply<R0, T, TENS>, g0, s,
+                    A, (const T*)b->p, b->bs_v, (const double*)b->bs_rhat, 0, b->partial, st);
         finish(kFinBAlpha);
         k_bicg_s<R0><<<g0, kBlock, 0, s>>>(A, b->r, b->bs_v, b->bs_s, st);
         vsc.run0();
         k_bicg_apply<R0, T, TENS><<<g0, kBlock, 0, s>>>(A, (const T*)b->p2, b->bs_t, b->bs_s, 1, b->partial, st);
         finish(kFinBOmega);
-        prof_begin(b, EQ_KERNEL_CG_UPDATE);
-        k_bicg_x<R0, T><<<g0, kBlock, 0, s>>>(A, b->u, b->r, (const T*)b->p, (const T*)b->p2, b->bs_s, b->bs_t,
-                                          b->bs_rhat, b->partial, st);
-        prof_end(b, EQ_KERNEL_CG_UPDATE, L0.nodes_all * (2 * sizeof(T) + 6 * sizeof(double)));
+        launch_slot(b, EQ_KERNEL_CG_UPDATE, L0.nodes_all * (2 * sizeof(T) + 6 * sizeof(double)), k_bicg_x<R0, T>, g0, s, A,
+                    b->u, b->r, (const T*)b->p, (const T*)b->p2, b->bs_s, b->bs_t, b->bs_rhat, b->partial, st);
         finish(kFinBUpdate);
     };
     auto setup = [&]() {
