@@ -576,6 +576,48 @@ def test_interior_tile_paths_match_oracle(eqf, oracle, case):
                 assert _rel(fb.download(k), ref[k]) <= 1e-9, (step, k)
 
 
+@pytest.mark.parametrize("solver", ["gmg", "mixed", "mixed_bicgstab"])
+def test_regridded_hierarchy_matches_oracle(eqf, oracle, solver):
+    """Grids whose cell counts stop halving exactly: the hierarchy regrids
+    once (about half, rounded up so the levels under it nest again), and the
+    regridded transfer runs in separate launches while nested levels fuse.
+    One batch mixes shapes that regrid on different levels (260x132 from
+    its second coarse level, 65x33, on; 203x97 and 99x33 at once) with one
+    that stays nested (128x64), so a level holds nested and regridded
+    problems together.  Two steps against the oracle's LU; the BiCGStab case
+    adds a velocity.  The spacing is 1/16 um, so every node coordinate is
+    exact and the oracle's coordinate test finds every wall (at 203 cells
+    over 10.15 um it misses the right wall, which this path marks by index)."""
+    kind = {"gmg": eqf.GMG_CG, "mixed": eqf.MIXED_GMG_CG, "mixed_bicgstab": eqf.MIXED_GMG_BICGSTAB}[solver]
+    vel = solver.endswith("bicgstab")
+    shapes = [(260, 132, 16.25, 8.25), (203, 97, 12.6875, 6.0625), (128, 64, 8.0, 4.0), (99, 33, 6.1875, 2.0625)]
+    walls = {oracle.TOP: (oracle.DIRICHLET, 0.0, 0, 0), oracle.RIGHT: (oracle.ROBIN, 0, 2.0, 0.1)}
+    descs, probs, u0, vels = [], [], [], []
+    rng = np.random.default_rng(31)
+    for (nx, ny, W, H) in shapes:
+        for D in (250.0, 60.0):
+            descs.append(eqf.field_desc(nx, ny, W, H, 0.1, D, 0.02, 0.0, walls))
+            p = oracle.problem(nx, ny, W, H, 0.1, D, 0.02, 0.0, walls=walls)
+            v = _parabolic(ny, H, 20.0) if vel else None
+            p.set_velocity(v)
+            probs.append(p)
+            vels.append(v)
+            u0.append(rng.random((nx + 1) * (ny + 1)))
+    with eqf.FieldBatch(descs, nspecies=2, solver=eqf.solver_desc(kind)) as fb:
+        for k in range(len(descs)):
+            fb.upload(k, u0[k])
+            if vels[k] is not None:
+                fb.set_velocity(k, vels[k])
+        ref = [u.copy() for u in u0]
+        for step in range(2):
+            print(f"regridded hierarchy {solver} step {step}", flush=True)
+            st = fb.step()
+            assert st.nconverged == len(descs), step
+            for k in range(len(descs)):
+                ref[k], _ = oracle.step(probs[k], ref[k], "lu")
+                assert _rel(fb.download(k), ref[k]) <= 1e-9, (step, k)
+
+
 def test_empty_cell_lists_and_rejected_calls(eqf, oracle):
     """No cells: coupling leaves the fields as they are (host and device
     entry points, and a batched call whose middle trap has none); a batched
