@@ -547,11 +547,16 @@ static int couple_impl(eq_batch* b, int32_t t0, int32_t ntr, const int64_t* cell
     int64_t* d_total = (int64_t*)(base + geo_bytes);
     int64_t* d_blk = (int64_t*)(base + geo_bytes + 256 + 2 * tb_bytes);
     const TrapGrid* traps = (const TrapGrid*)b->d_cell_traps;
+    const int kid = EQ_KERNEL_COUPLE_SETUP;
+    const double gb = (double)sizeof(CellGeo) * (double)nc;
     auto geometry = [&]() {
-        k_cell_geo<<<(unsigned)((nc + 3) / 4), 256, 0, s>>>(d_cells, nc, b->d_cell_off, ntr, t0, traps, npm, geo);
-        k_pair_block_sums<<<(unsigned)nblk, kScanBlock, 0, s>>>(geo, nc, d_blk);
-        k_pair_scan<<<1, 1024, 0, s>>>(d_blk, nblk, d_total);
-        k_pair_offsets<<<(unsigned)nblk, kScanBlock, 0, s>>>(geo, nc, d_blk);
+        launch(b, kid, (double)sizeof(eq_cell) * nc + gb, k_cell_geo, dim3((unsigned)((nc + 3) / 4)), dim3(256), 0, s,
+               d_cells, (int64_t)nc, (const int64_t*)b->d_cell_off, ntr, t0, traps, npm, geo);
+        launch(b, kid, gb + 8.0 * nblk, k_pair_block_sums, dim3((unsigned)nblk), dim3(kScanBlock), 0, s,
+               (const CellGeo*)geo, (int64_t)nc, d_blk);
+        launch(b, kid, 16.0 * nblk, k_pair_scan, dim3(1), dim3(1024), 0, s, d_blk, (int64_t)nblk, d_total);
+        launch(b, kid, 2.0 * gb + 8.0 * nblk, k_pair_offsets, dim3((unsigned)nblk), dim3(kScanBlock), 0, s, geo,
+               (int64_t)nc, (const int64_t*)d_blk);
     };
     geometry();
     int64_t total = 0;
@@ -583,27 +588,35 @@ static int couple_impl(eq_batch* b, int32_t t0, int32_t ntr, const int64_t* cell
     uint32_t* pair_cell = (uint32_t*)(pbase + 4 * pb);
     double* partial = (double*)(pbase + 5 * pb);
     void* sort_tmp = pbase + 5 * pb + align256(sizeof(double) * (size_t)total * ns);
-    k_pairs<<<(unsigned)((nc + 255) / 256), 256, 0, s>>>(geo, nc, traps, keys, vals, pair_cell);
+    launch(b, kid, gb + 12.0 * total, k_pairs, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s, (const CellGeo*)geo,
+           (int64_t)nc, traps, keys, vals, pair_cell);
     // key bits: global tile ids up to the last tile of the batch
     unsigned bits = 1;
     while (bits < 32 && ((uint64_t)b->cell_tile_base[b->ntraps] >> bits) != 0) ++bits;
     EQ_HIP(rocprim::radix_sort_pairs(sort_tmp, sort_bytes, (const uint32_t*)keys, keys2, (const uint32_t*)vals, vals2,
                                      (size_t)total, 0, bits, s));
+    b->acct_to->bytes += 16.0 * total; // the sort's keys and values, read and written once
+    b->acct_to->launches += 1;
     EQ_HIP(hipMemsetAsync(t_start, 0, 2 * tb_bytes, s));
-    k_tile_bounds<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(keys2, total, tile0, t_start, t_end);
-    prof_begin(b, EQ_KERNEL_COUPLE);
-    k_tile_couple<<<ntiles, kCThreads, 0, s>>>(traps, t0, ntr, tile0, t_start, t_end, vals2, pair_cell, geo,
-                                               b->d_field_ptrs, ns, d_q, partial, npm, flags);
-    // algorithmic bytes: every node of the call's traps read (and written when depositing) once per species
+    launch(b, kid, 4.0 * total + 8.0 * ntiles, k_tile_bounds, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
+           (const uint32_t*)keys2, (int64_t)total, tile0, t_start, t_end);
+    // algorithmic bytes: every node of the call's traps read (and written when
+    // depositing) once per species, the pairs, the cell records and amounts,
+    // and one partial per pair and species
     double nodes = 0.0;
     for (int t = t0; t < t0 + ntr; ++t) {
         const LevelProb& P = b->levels[0].probs[t * ns];
         nodes += (double)(P.nx + 1) * (P.ny + 1);
     }
-    prof_end(b, EQ_KERNEL_COUPLE, nodes * ns * sizeof(double) * ((flags & EQ_COUPLE_DEPOSIT) ? 2 : 1));
+    const double cb = nodes * ns * sizeof(double) * ((flags & EQ_COUPLE_DEPOSIT) ? 2 : 1) + 8.0 * total + gb +
+                      8.0 * ns * nc + 8.0 * ns * total + 8.0 * ntiles;
+    launch(b, EQ_KERNEL_COUPLE, cb, k_tile_couple, dim3(ntiles), dim3(kCThreads), 0, s, traps, t0, ntr, tile0,
+           (const uint32_t*)t_start, (const uint32_t*)t_end, (const uint32_t*)vals2, (const uint32_t*)pair_cell,
+           (const CellGeo*)geo, (double* const*)b->d_field_ptrs, ns, d_q, partial, npm, flags);
     if (d_sampled || d_counts)
-        k_cell_sample<<<(unsigned)((nc + 255) / 256), 256, 0, s>>>(
-            geo, nc, ns, partial, (flags & EQ_COUPLE_SAMPLE) ? d_sampled : nullptr, d_counts);
+        launch(b, kid, gb + 8.0 * ns * total + 12.0 * ns * nc, k_cell_sample, dim3((unsigned)((nc + 255) / 256)),
+               dim3(256), 0, s, (const CellGeo*)geo, (int64_t)nc, ns, (const double*)partial,
+               (flags & EQ_COUPLE_SAMPLE) ? d_sampled : nullptr, d_counts);
     EQ_HIP(hipGetLastError());
     return EQ_OK;
 }

@@ -105,6 +105,8 @@ int upload_level0(eq_batch* b)
 {
     Level& L = b->levels[0];
     EQ_HIP(hipStreamSynchronize(b->stream));
+    const int rc = pull_well_values(b); // keep the wall values the wells wrote
+    if (rc) return rc;
     EQ_HIP(hipMemcpy(L.d_probs, L.probs.data(), sizeof(LevelProb) * L.probs.size(), hipMemcpyHostToDevice));
     return EQ_OK;
 }
@@ -119,7 +121,13 @@ namespace eqk {
 int channels_after_step(eq_batch* b)
 {
     if (b->chan_count == 0) return EQ_OK;
-    k_channels<<<(b->chan_count + 63) / 64, 64, 0, b->stream>>>((const ChannelDev*)b->d_chans, b->chan_count);
+    double bytes = 0.0; // per channel: its values read and written, the trap's two rows read, per substep
+    for (size_t p = 0; p < b->chan_desc.size(); ++p)
+        for (int k = 0; k < 2; ++k)
+            if (b->chan_desc[p][k].substeps > 0)
+                bytes += 32.0 * (b->levels[0].probs[p].nx + 1) * b->chan_desc[p][k].substeps;
+    launch(b, EQ_KERNEL_BOUNDARY, bytes, k_channels, dim3((b->chan_count + 63) / 64), dim3(64), 0, b->stream,
+           (const ChannelDev*)b->d_chans, b->chan_count);
     EQ_HIP(hipGetLastError());
     return EQ_OK;
 }

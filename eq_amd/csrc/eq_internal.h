@@ -2,6 +2,7 @@
 // libeq_field.so.  Not part of the ABI.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <array>
@@ -50,6 +51,18 @@ struct BottomRef {
     int32_t rec, nl, prob, pad_;
 };
 
+// one boundary well (Simulation::computeBoundaryWell, src/simulation.cpp:581-604)
+struct WellDev {
+    int32_t prob;        // problem index (its level-0 table entry)
+    int32_t pad_;
+    double scaling;      // wellScaling
+    double decay;        // boundaryDecayRate
+    double dt;
+    double w;            // boundaryWellConcentration
+    double flux;         // the last totalBoundaryFlux it took
+    int64_t underflows;  // boundaryUnderFlow
+};
+
 struct Level {
     std::vector<LevelProb> probs;     // host copy
     std::vector<TileRef> tiles_all;   // every problem at this level
@@ -71,6 +84,9 @@ struct Level {
     int64_t pool = 0;                 // elements per vector on this level
     double nodes_all = 0.0;           // grid nodes summed over the level's problems
     double nodes_smooth = 0.0;        // ... over the problems that smooth here
+    double nodes_active = 0.0;        // ... over the problems at or above their bottom level
+    double nodes_bottom = 0.0;        // ... over the problems whose bottom level this is
+    double bottom_dense_bytes = 0.0;  // their coarsest inverses (read by the bottom V-cycle)
     size_t bottom_lds_bytes = 0;      // dynamic LDS of the bottom V-cycle launched from this level
     // multigrid vectors (element type = smoother precision)
     void* b = nullptr;  // level >= 1, or level 0 FP32 copy of r
@@ -82,6 +98,12 @@ struct Level {
     LevelView smooth() const { return LevelView{d_probs, d_tiles_smooth, (int32_t)tiles_smooth.size()}; }
     LevelView bottom_tiles() const { return LevelView{d_probs, d_tiles_bottom, (int32_t)tiles_bottom.size()}; }
     LevelView active() const { return LevelView{d_probs, d_tiles_active, (int32_t)tiles_active.size()}; }
+};
+
+// algorithmic bytes and launches of a launch sequence
+struct Acct {
+    double bytes = 0.0;
+    int64_t launches = 0;
 };
 
 struct ProfileSlot {
@@ -153,17 +175,66 @@ struct eq_batch {
     void* d_chans = nullptr;                              // device channel table
     int32_t chan_count = 0;
     std::vector<double*> chan_scratch;
+    // boundary wells (allocated on first use): the device table is the live
+    // state; the host keeps which problems have one and their parameters
+    std::vector<eqk::WellDev> wells;
+    eqk::WellDev* d_wells = nullptr;
+    int32_t d_wells_cap = 0;
+    // algorithmic traffic: launches add to *acct_to, which is `acct` except
+    // while a graph is captured (then the graph's own count, added to `acct`
+    // at every replay)
+    eqk::Acct acct;
+    eqk::Acct* acct_to = &acct;
+    eqk::Acct g_init_acct, g_iter2_acct, g_bicg_acct;
 };
 
 namespace eqk {
 int set_error(int code, const std::string& msg);
 int hip_check(hipError_t e, const char* what);
-// event bracket for optional per-launch timing
-void prof_begin(eq_batch* b, int kernel);
-void prof_end(eq_batch* b, int kernel, double bytes);
 // flow channels: update after a solve; release on destroy
 int channels_after_step(eq_batch* b);
 void free_channels(eq_batch* b);
+// the level-0 Dirichlet values a well wrote on the device, copied back into
+// the host records before the host uploads its copy of the table
+int pull_well_values(eq_batch* b);
+// boundary wells: update after a step
+int wells_after_step(eq_batch* b);
+} // namespace eqk
+
+namespace eqk {
+// Every kernel launch of the library: counts its algorithmic bytes, and when
+// timing is on for its family `kid`, carries start and stop events
+// (hipExtLaunchKernelGGL) that bracket the kernel's own execution, not the
+// stream's gaps around it.
+template <typename K, typename... Args>
+inline void launch(eq_batch* b, int kid, double bytes, K kernel, dim3 grid, dim3 block, size_t lds, hipStream_t s,
+                   Args... args)
+{
+    b->acct_to->bytes += bytes;
+    b->acct_to->launches += 1;
+    if (kid < 0 || kid >= EQ_KERNEL_COUNT_ || !b->prof[kid].enabled) {
+        hipExtLaunchKernelGGL(kernel, grid, block, lds, s, nullptr, nullptr, 0, args...);
+        return;
+    }
+    ProfileSlot& ps = b->prof[kid];
+    std::pair<hipEvent_t, hipEvent_t> ev;
+    if (!ps.pool.empty()) {
+        ev = ps.pool.back();
+        ps.pool.pop_back();
+    } else {
+        hipEventCreate(&ev.first);
+        hipEventCreate(&ev.second);
+    }
+    hipExtLaunchKernelGGL(kernel, grid, block, lds, s, ev.first, ev.second, 0, args...);
+    ps.pending.push_back(ev);
+    ps.pending_bytes.push_back(bytes);
+}
+// a replayed graph's traffic
+inline void replayed(eq_batch* b, const Acct& g)
+{
+    b->acct.bytes += g.bytes;
+    b->acct.launches += g.launches;
+}
 } // namespace eqk
 
 #define EQ_HIP(call)                                                  \

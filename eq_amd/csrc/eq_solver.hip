@@ -1775,10 +1775,9 @@ __global__ __launch_bounds__(kBlock) void k_apply_full(LevelView L, int pi_only,
 // -int grad(u).n ds over all walls with the P1 gradient of the triangle that
 // owns each boundary edge (fenics/boundary.h:2652-2742), times dt*D
 // (src/fHSL.cpp:156-160).  One block, one problem.
-__global__ __launch_bounds__(256) void k_flux(const LevelProb* __restrict__ probs, int pi,
-                                              const double* __restrict__ u, double* out)
+// Block sum of the boundary terms; the total is valid in thread 0.
+__device__ double flux_sum(const LevelProb& P, const double* __restrict__ u)
 {
-    const LevelProb& P = probs[pi];
     const int nx = P.nx, ny = P.ny, pt = P.pitch;
     const double* U = u + P.origin;
     double v[1] = {0.0};
@@ -1795,58 +1794,85 @@ __global__ __launch_bounds__(256) void k_flux(const LevelProb* __restrict__ prob
         v[0] += P.hy * (U[(int64_t)i * pt + nx - 1] - U[(int64_t)i * pt + nx]) / P.hx;
     }
     block_sum<1, 256>(v);
-    if (threadIdx.x == 0) *out = P.Ddt * v[0];
+    return v[0];
+}
+
+__global__ __launch_bounds__(256) void k_flux(const LevelProb* __restrict__ probs, int pi,
+                                              const double* __restrict__ u, double* out)
+{
+    const LevelProb& P = probs[pi];
+    const double v = flux_sum(P, u);
+    if (threadIdx.x == 0) *out = P.Ddt * v;
+}
+
+// One block per boundary well: the flux of the new field, the well update of
+// Simulation::computeBoundaryWell (src/simulation.cpp:581-604) in the
+// reference's operation order (w += flux/scaling; w -= (dt*rate)*w; clamp at
+// 0), and setBoundaryValues(w) on every Dirichlet wall of the problem's level-0
+// record (src/fHSL.cpp:601-603), read by the next solve's kernels.
+__global__ __launch_bounds__(256) void k_wells(LevelProb* probs, WellDev* wells, const double* __restrict__ u)
+{
+    WellDev& W = wells[blockIdx.x];
+    LevelProb& P = probs[W.prob];
+    const double v = flux_sum(P, u);
+    if (threadIdx.x != 0) return;
+    const double flux = P.Ddt * v;
+    double w = __dadd_rn(W.w, flux / W.scaling);
+    w = __dsub_rn(w, __dmul_rn(__dmul_rn(W.dt, W.decay), w));
+    if (w < 0.0) {
+        w = 0.0;
+        W.underflows += 1;
+    }
+    W.w = w;
+    W.flux = flux;
+    for (int k = 0; k < 4; ++k)
+        if ((P.dmask >> k) & 1) P.g[k] = w;
+}
+
+// setBoundaryValues: every Dirichlet wall of one problem takes `value`
+__global__ void k_set_walls(LevelProb* probs, int pi, double value)
+{
+    if (threadIdx.x != 0) return;
+    LevelProb& P = probs[pi];
+    for (int k = 0; k < 4; ++k)
+        if ((P.dmask >> k) & 1) P.g[k] = value;
+}
+
+// the wells after a step (called by eq_batch_step, after the channels)
+int wells_after_step(eq_batch* b)
+{
+    if (b->wells.empty()) return EQ_OK;
+    double nodes = 0.0; // the boundary rows and columns each well's flux reads
+    for (const WellDev& W : b->wells) {
+        const LevelProb& P = b->levels[0].probs[W.prob];
+        nodes += 4.0 * (P.nx + P.ny);
+    }
+    launch(b, EQ_KERNEL_BOUNDARY, nodes * sizeof(double), k_wells, dim3((int)b->wells.size()), dim3(256), 0, b->stream,
+           b->levels[0].d_probs, b->d_wells, (const double*)b->u);
+    EQ_HIP(hipGetLastError());
+    return EQ_OK;
+}
+
+int pull_well_values(eq_batch* b)
+{
+    if (b->wells.empty()) return EQ_OK;
+    Level& L = b->levels[0];
+    std::vector<LevelProb> dev(L.probs.size());
+    EQ_HIP(hipStreamSynchronize(b->stream));
+    EQ_HIP(hipMemcpy(dev.data(), L.d_probs, sizeof(LevelProb) * dev.size(), hipMemcpyDeviceToHost));
+    EQ_HIP(hipMemcpy(b->wells.data(), b->d_wells, sizeof(WellDev) * b->wells.size(), hipMemcpyDeviceToHost));
+    for (const WellDev& W : b->wells)
+        for (int k = 0; k < 4; ++k) L.probs[W.prob].g[k] = dev[W.prob].g[k];
+    return EQ_OK;
 }
 
 // ------------------------------------------------------------ profiling
 
-void prof_begin(eq_batch* b, int kernel)
-{
-    ProfileSlot& s = b->prof[kernel];
-    if (!s.enabled) return;
-    std::pair<hipEvent_t, hipEvent_t> ev;
-    if (!s.pool.empty()) {
-        ev = s.pool.back();
-        s.pool.pop_back();
-    } else {
-        hipEventCreate(&ev.first);
-        hipEventCreate(&ev.second);
-    }
-    hipEventRecord(ev.first, b->stream);
-    s.pending.push_back(ev);
-}
-
-void prof_end(eq_batch* b, int kernel, double bytes)
-{
-    ProfileSlot& s = b->prof[kernel];
-    if (!s.enabled) return;
-    hipEventRecord(s.pending.back().second, b->stream);
-    s.pending_bytes.push_back(bytes);
-}
-
-// Launch kernel<<<grid, kBlock, 0, s>>>(args...).  When profiling slot
-// `slot` is on, the launch carries start and stop events
-// (hipExtLaunchKernelGGL): they bracket the kernel's own execution, not the
-// stream's gaps around it.  slot < 0: never timed.
+// launch on the standard tile block: eqk::launch with kBlock threads
 template <typename K, typename... Args>
-static void launch_slot(eq_batch* b, int slot, double bytes, K kernel, int grid, hipStream_t s, Args... args)
+static void launch_tiles(eq_batch* b, int kid, double bytes, K kernel, int grid, hipStream_t s, Args... args)
 {
-    if (slot < 0 || !b->prof[slot].enabled) {
-        hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), 0, s, nullptr, nullptr, 0, args...);
-        return;
-    }
-    ProfileSlot& ps = b->prof[slot];
-    std::pair<hipEvent_t, hipEvent_t> ev;
-    if (!ps.pool.empty()) {
-        ev = ps.pool.back();
-        ps.pool.pop_back();
-    } else {
-        hipEventCreate(&ev.first);
-        hipEventCreate(&ev.second);
-    }
-    hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), 0, s, ev.first, ev.second, 0, args...);
-    ps.pending.push_back(ev);
-    ps.pending_bytes.push_back(bytes);
+    launch(b, kid, bytes, kernel, dim3(grid), dim3(kBlock), 0, s, args...);
 }
 
 static void prof_collect(eq_batch* b)
@@ -2020,6 +2046,7 @@ static int validate(const eq_field_desc& d, std::string& why)
     }
     if (!(d.width > 0) || !(d.height > 0)) { why = "width and height must be > 0"; return 1; }
     if (d.scheme != EQ_SCHEME_P1 && d.scheme != EQ_SCHEME_FD) { why = "unknown scheme"; return 1; }
+    if (d.marking != EQ_WALLS_GRID && d.marking != EQ_WALLS_COORD) { why = "unknown wall marking"; return 1; }
     if (!(d.dt > 0) || !(d.D >= 0) || !(d.gamma >= 0) || !std::isfinite(d.f)) {
         why = "need dt > 0, D >= 0, gamma >= 0, finite f";
         return 1;
@@ -2030,6 +2057,22 @@ static int validate(const eq_field_desc& d, std::string& why)
         if (k == EQ_BC_ROBIN && !(d.wall[w].r >= 0)) { why = "Robin rate must be >= 0"; return 1; }
     }
     return 0;
+}
+
+// The reference's wall test on DOLFIN's vertex coordinates: near(x_j, W) with
+// x_j = 0 + (j*(W - 0))/nx, likewise in y (RectangleMesh [ext], src/fHSL.cpp:169-172;
+// DirichletBoundary_TrapEdge::inside, src/fHSL.h:575-590).  The left and bottom
+// walls sit at exact zeros.  Every facet of a wall has the wall's coordinate at
+// both ends, so a wall is found whole or not at all.
+static void walls_found(const eq_field_desc& d, int32_t found[4])
+{
+    const double kDolfinEps = 3.0e-16;
+    const double xr = 0.0 + ((double)d.nx * (d.width - 0.0)) / (double)d.nx;
+    const double yt = 0.0 + ((double)d.ny * (d.height - 0.0)) / (double)d.ny;
+    found[EQ_WALL_LEFT] = 1;
+    found[EQ_WALL_BOTTOM] = 1;
+    found[EQ_WALL_RIGHT] = std::fabs(xr - d.width) < kDolfinEps ? 1 : 0;
+    found[EQ_WALL_TOP] = std::fabs(yt - d.height) < kDolfinEps ? 1 : 0;
 }
 
 // Each bottom problem's chain of level records (its bottom level, then every
@@ -2215,6 +2258,8 @@ static int build_hierarchy(eq_batch* b)
             const double nodes = (double)(P.nx + 1) * (double)(P.ny + 1);
             L.nodes_all += nodes;
             if (smooth) L.nodes_smooth += nodes;
+            if (active) L.nodes_active += nodes;
+            if (active && P.bottom) L.nodes_bottom += nodes;
             if (smooth && !(P.nx == 2 * P.nxc && P.ny == 2 * P.nyc)) L.nested_down = false;
             if (P.bottom) L.bottom.push_back((int32_t)k);
             if (P.coarsest && mg) {
@@ -2228,6 +2273,12 @@ static int build_hierarchy(eq_batch* b)
     }
     // bottom chains; dynamic LDS = the level arrays, then the level records
     sync_bottom_recs(b, false);
+    for (Level& L : b->levels)
+        for (const BottomRef& r : L.bottom_refs) {
+            const LevelProb& Pc = L.bottom_recs[r.rec + r.nl];
+            const double nc = (double)(Pc.nx + 1) * (double)(Pc.ny + 1);
+            L.bottom_dense_bytes += nc * nc * sizeof(T);
+        }
     for (Level& L : b->levels)
         for (const BottomRef& r : L.bottom_refs)
             L.bottom_lds_bytes = std::max<size_t>(
@@ -2345,53 +2396,61 @@ struct VCycle {
             // the first sweep into X[0] (and the FP32 copy of the residual)
             const bool first_done = top && pre1;
             const bool pair = !first_done && S.mg_pre >= 2 && fuse_pre;
-            // level-0 sweeps go to the sweep profiling slot
-            const int slot = top ? EQ_KERNEL_SMOOTH : -1;
+            // sweeps time as the finest level's or the coarse levels' family
+            const int slot = top ? EQ_KERNEL_SMOOTH : EQ_KERNEL_SMOOTH_COARSE;
+            const int xfer = top ? EQ_KERNEL_TRANSFER0 : EQ_KERNEL_TRANSFER_COARSE;
             if (!first_done) {
                 const double by = nodes * (bytes_b + sizeof(T) * (bcopy ? 2 : 1));
                 if (pair) {
                     // sweeps 1 and 2 in one launch
                     if (top)
-                        launch_slot(b, slot, by, k_presmooth2<R, T, double, TENS>, gs, s, L.smooth(), in, X[1], bcopy,
+                        launch_tiles(b, slot, by, k_presmooth2<R, T, double, TENS>, gs, s, L.smooth(), in, X[1], bcopy,
                                     omega, st);
                     else
-                        launch_slot(b, slot, by, k_presmooth2<R, T, T, TENS>, gs, s, L.smooth(), bl, X[1], (T*)nullptr,
+                        launch_tiles(b, slot, by, k_presmooth2<R, T, T, TENS>, gs, s, L.smooth(), bl, X[1], (T*)nullptr,
                                     omega, st);
                     cur = 1;
                 } else if (top) {
-                    launch_slot(b, slot, by, k_smooth<R, T, double, T, TENS>, gs, s, L.smooth(), in, (const T*)nullptr, X[0],
+                    launch_tiles(b, slot, by, k_smooth<R, T, double, T, TENS>, gs, s, L.smooth(), in, (const T*)nullptr, X[0],
                                 bcopy, omega, 1, 0, (double*)nullptr, st);
                 } else {
-                    launch_slot(b, slot, by, k_smooth<R, T, T, T, TENS>, gs, s, L.smooth(), bl, (const T*)nullptr, X[0],
+                    launch_tiles(b, slot, by, k_smooth<R, T, T, T, TENS>, gs, s, L.smooth(), bl, (const T*)nullptr, X[0],
                                 (T*)nullptr, omega, 1, 0, (double*)nullptr, st);
                 }
             }
             for (int k = pair ? 2 : 1; k < S.mg_pre; ++k) {
-                launch_slot(b, slot, nodes * 3 * sizeof(T), k_smooth<R, T, T, T, TENS>, gs, s, L.smooth(), bl,
+                launch_tiles(b, slot, nodes * 3 * sizeof(T), k_smooth<R, T, T, T, TENS>, gs, s, L.smooth(), bl,
                             (const T*)X[cur], X[cur ^ 1], (T*)nullptr, omega, 0, 0, (double*)nullptr, st);
                 cur ^= 1;
             }
             // residual and its restriction over the coarse nodes
             Level& C = b->levels[l + 1];
             const int na = (int)C.tiles_active.size();
+            const double cbytes = C.nodes_active * sizeof(T);
             if (fuse) {
+                const double by = nodes * 2 * sizeof(T) + cbytes; // b (the FP32 copy on level 0) and x
                 if (C.R == kRowsWide)
-                    k_residual_restrict<kRowsWide, T, TENS><<<grid_for(na), kBlock, 0, s>>>(C.active(), L.d_probs, bl,
-                                                                                           X[cur], (T*)C.b, st);
+                    launch_tiles(b, xfer, by, k_residual_restrict<kRowsWide, T, TENS>, grid_for(na), s, C.active(),
+                                 (const LevelProb*)L.d_probs, bl, (const T*)X[cur], (T*)C.b, st);
                 else if (C.R == kRows0)
-                    k_residual_restrict<kRows0, T, TENS><<<grid_for(na), kBlock, 0, s>>>(C.active(), L.d_probs, bl, X[cur],
-                                                                                        (T*)C.b, st);
+                    launch_tiles(b, xfer, by, k_residual_restrict<kRows0, T, TENS>, grid_for(na), s, C.active(),
+                                 (const LevelProb*)L.d_probs, bl, (const T*)X[cur], (T*)C.b, st);
                 else
-                    k_residual_restrict<kRowsC, T, TENS><<<grid_for(na), kBlock, 0, s>>>(C.active(), L.d_probs, bl, X[cur],
-                                                                                        (T*)C.b, st);
+                    launch_tiles(b, xfer, by, k_residual_restrict<kRowsC, T, TENS>, grid_for(na), s, C.active(),
+                                 (const LevelProb*)L.d_probs, bl, (const T*)X[cur], (T*)C.b, st);
             } else {
-                k_residual<R, T, TENS><<<gs, kBlock, 0, s>>>(L.smooth(), bl, X[cur], (T*)L.r, st);
+                launch_tiles(b, xfer, nodes * 3 * sizeof(T), k_residual<R, T, TENS>, gs, s, L.smooth(), bl,
+                             (const T*)X[cur], (T*)L.r, st);
+                const double by = nodes * sizeof(T) + cbytes;
                 if (C.R == kRowsWide)
-                    k_restrict<kRowsWide, T><<<grid_for(na), kBlock, 0, s>>>(C.active(), L.d_probs, (const T*)L.r, (T*)C.b, st);
+                    launch_tiles(b, xfer, by, k_restrict<kRowsWide, T>, grid_for(na), s, C.active(),
+                                 (const LevelProb*)L.d_probs, (const T*)L.r, (T*)C.b, st);
                 else if (C.R == kRows0)
-                    k_restrict<kRows0, T><<<grid_for(na), kBlock, 0, s>>>(C.active(), L.d_probs, (const T*)L.r, (T*)C.b, st);
+                    launch_tiles(b, xfer, by, k_restrict<kRows0, T>, grid_for(na), s, C.active(), (const LevelProb*)L.d_probs,
+                                 (const T*)L.r, (T*)C.b, st);
                 else
-                    k_restrict<kRowsC, T><<<grid_for(na), kBlock, 0, s>>>(C.active(), L.d_probs, (const T*)L.r, (T*)C.b, st);
+                    launch_tiles(b, xfer, by, k_restrict<kRowsC, T>, grid_for(na), s, C.active(), (const LevelProb*)L.d_probs,
+                                 (const T*)L.r, (T*)C.b, st);
             }
             run(l + 1);
             const T* xc = final_buffer(l + 1);
@@ -2402,22 +2461,23 @@ struct VCycle {
                 const bool last = (k == S.mg_post - 1);
                 const bool first = (k == 0) && fuse;
                 if (k == 0 && !fuse)
-                    k_prolong<R, T><<<gs, kBlock, 0, s>>>(L.smooth(), C.d_probs, xc, X[cur], st);
+                    launch_tiles(b, xfer, nodes * 2 * sizeof(T) + cbytes, k_prolong<R, T>, gs, s, L.smooth(),
+                                 (const LevelProb*)C.d_probs, xc, X[cur], st);
                 // the last level-0 sweep reads the FP64 residual, the others b in T
-                const double by = nodes * (sizeof(T) * 2 + (last ? bytes_b : sizeof(T)));
+                const double by = nodes * (sizeof(T) * 2 + (last ? bytes_b : sizeof(T))) + (first ? cbytes : 0.0);
                 if (top && last) {
                     if (first)
-                        launch_slot(b, slot, by, k_prolong_smooth<R, T, double, T, TENS>, gs, s, L.smooth(), C.d_probs, xc,
+                        launch_tiles(b, slot, by, k_prolong_smooth<R, T, double, T, TENS>, gs, s, L.smooth(), C.d_probs, xc,
                                     in, (const T*)X[cur], out, omega, 1, b->partial, st);
                     else
-                        launch_slot(b, slot, by, k_smooth<R, T, double, T, TENS>, gs, s, L.smooth(), in, (const T*)X[cur],
+                        launch_tiles(b, slot, by, k_smooth<R, T, double, T, TENS>, gs, s, L.smooth(), in, (const T*)X[cur],
                                     out, (T*)nullptr, omega, 0, 1, b->partial, st);
                 } else {
                     if (first)
-                        launch_slot(b, slot, by, k_prolong_smooth<R, T, T, T, TENS>, gs, s, L.smooth(), C.d_probs, xc, bl,
+                        launch_tiles(b, slot, by, k_prolong_smooth<R, T, T, T, TENS>, gs, s, L.smooth(), C.d_probs, xc, bl,
                                     (const T*)X[cur], X[cur ^ 1], omega, 0, (double*)nullptr, st);
                     else
-                        launch_slot(b, slot, by, k_smooth<R, T, T, T, TENS>, gs, s, L.smooth(), bl, (const T*)X[cur],
+                        launch_tiles(b, slot, by, k_smooth<R, T, T, T, TENS>, gs, s, L.smooth(), bl, (const T*)X[cur],
                                     X[cur ^ 1], (T*)nullptr, omega, 0, 0, (double*)nullptr, st);
                     cur ^= 1;
                 }
@@ -2427,16 +2487,18 @@ struct VCycle {
             // the bottom solve leaves the level's correction in xl; on level 0
             // it is copied to `out` with the partial (in, out)
             T* xl = final_buffer(l);
-            if (top) {
-                const int nb = (int)L.tiles_bottom.size();
-                k_cast_in<R, T><<<grid_for(nb), kBlock, 0, s>>>(L.bottom_tiles(), in, (T*)L.b, st);
-            }
-            k_bottom<T, TENS><<<(int)L.bottom.size(), kBottomThreads, L.bottom_lds_bytes, s>>>(
-                L.d_bottom, L.d_bottom_recs, (const T*)L.b, xl, omega, S.mg_pre, S.mg_post, (const T*)b->dense, st);
-            if (top) {
-                const int nb = (int)L.tiles_bottom.size();
-                k_cast_out<R, T, T><<<grid_for(nb), kBlock, 0, s>>>(L.bottom_tiles(), xl, in, out, b->partial, st);
-            }
+            const int nb = (int)L.tiles_bottom.size();
+            if (top)
+                launch_tiles(b, EQ_KERNEL_BOTTOM, L.nodes_bottom * (sizeof(double) + sizeof(T)), k_cast_in<R, T>,
+                             grid_for(nb), s, L.bottom_tiles(), in, (T*)L.b, st);
+            // the bottom levels live in LDS: HBM sees the level's b and x and the coarsest inverses
+            launch(b, EQ_KERNEL_BOTTOM, L.nodes_bottom * 2 * sizeof(T) + L.bottom_dense_bytes, k_bottom<T, TENS>,
+                   dim3((int)L.bottom.size()), dim3(kBottomThreads), L.bottom_lds_bytes, s,
+                   (const BottomRef*)L.d_bottom, (const LevelProb*)L.d_bottom_recs, (const T*)L.b, xl, omega, S.mg_pre,
+                   S.mg_post, (const T*)b->dense, st);
+            if (top)
+                launch_tiles(b, EQ_KERNEL_BOTTOM, L.nodes_bottom * (2 * sizeof(T) + sizeof(double)), k_cast_out<R, T, T>,
+                             grid_for(nb), s, L.bottom_tiles(), (const T*)xl, in, out, b->partial, st);
         }
     }
 };
@@ -2462,8 +2524,10 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
     const bool jacobi = (S.kind == EQ_SOLVER_JACOBI_CG);
     const double fine_nodes = L0.nodes_all;
     VCycle<T, TENS> vc(b);
+    const double fin_bytes = (double)L0.tiles_all.size() * 3 * sizeof(double) + 2.0 * b->nprob * sizeof(CgState);
     auto finish = [&](int op) {
-        k_finish<<<b->nprob, kFinishThreads, 0, s>>>(L0.d_probs, b->partial, b->d_state, op, jacobi ? 1 : 0);
+        launch(b, EQ_KERNEL_FINISH, fin_bytes, k_finish, dim3(b->nprob), dim3(kFinishThreads), 0, s,
+               (const LevelProb*)L0.d_probs, (const double*)b->partial, b->d_state, op, jacobi ? 1 : 0);
     };
     // The V-cycle's first level-0 sweep (x = omega D^-1 r from x = 0) is
     // pointwise in the new residual: k_update forms it with r, and after
@@ -2476,8 +2540,9 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
     auto first_sweep = [&]() {
         const int ns = (int)L0.tiles_smooth.size();
         if (jacobi || ns == 0) return;
-        k_smooth<R0, T, double, T, TENS><<<grid_for(ns), kBlock, 0, s>>>(L0.smooth(), b->r, nullptr, x1, bcopy, omega, 1, 0,
-                                                                  nullptr, b->d_state);
+        launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, L0.nodes_smooth * (sizeof(double) + sizeof(T) * (bcopy ? 2 : 1)),
+                     k_smooth<R0, T, double, T, TENS>, grid_for(ns), s, L0.smooth(), (const double*)b->r,
+                     (const T*)nullptr, x1, bcopy, omega, 1, 0, (double*)nullptr, (const CgState*)b->d_state);
     };
     // one CG iteration from search direction pold to pnew
     auto iteration = [&](double* pold, double* pnew) {
@@ -2486,12 +2551,12 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
             finish(kFinRz);
         }
         // reads z, p_old; writes p_new, q (FP64)
-        launch_slot(b, EQ_KERNEL_STENCIL64, fine_nodes * (3 * sizeof(T) + sizeof(double)), k_ap_dir<R0, T, TENS>, g0, s, A,
+        launch_tiles(b, EQ_KERNEL_STENCIL64, fine_nodes * (3 * sizeof(T) + sizeof(double)), k_ap_dir<R0, T, TENS>, g0, s, A,
                     (const T*)b->z, (const T*)pold, (T*)pnew, b->q, b->partial, (const CgState*)b->d_state);
         finish(kFinAp);
         // reads x, r, q (FP64) and p (+ z when flexible); writes x, r (+ z for
         // Jacobi; + the first sweep, and the FP32 residual when mixed, otherwise)
-        launch_slot(b, EQ_KERNEL_CG_UPDATE,
+        launch_tiles(b, EQ_KERNEL_CG_UPDATE,
                     fine_nodes * (5 * sizeof(double) + sizeof(T) + (jacobi ? sizeof(T) : b->mixed ? 3 * sizeof(T) : sizeof(T))),
                     k_update<R0, T, TENS>, g0, s, A, b->u, b->r, (T*)b->z, (const T*)pnew, (const double*)b->q, b->partial,
                     (const CgState*)b->d_state, jacobi ? 1 : 0, x1, bcopy, omega);
@@ -2502,9 +2567,12 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
         iteration(b->p2, b->p);
     };
     auto setup = [&]() {
-        k_rhs<R0><<<g0, kBlock, 0, s>>>(A, b->u, b->rhs, b->partial);
+        launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, fine_nodes * 16.0, k_rhs<R0>, g0, s, A, (const double*)b->u, b->rhs,
+                     b->partial);
         finish(kFinRhs);
-        k_init<R0, TENS><<<g0, kBlock, 0, s>>>(A, b->u, b->xprev, b->rhs, b->r, b->z, b->partial, b->d_state, jacobi ? 1 : 0);
+        launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, fine_nodes * 40.0, k_init<R0, TENS>, g0, s, A, b->u,
+                     (const double*)b->xprev, (const double*)b->rhs, b->r, b->z, b->partial, b->d_state,
+                     jacobi ? 1 : 0);
         finish(kFinInit);
         first_sweep();
     };
@@ -2513,11 +2581,14 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
     bool profiling = false;
     for (int k = 0; k < EQ_KERNEL_COUNT_; ++k) profiling = profiling || b->prof[k].enabled;
     const bool graphs = S.use_graphs && !profiling;
-    auto capture = [&](hipGraphExec_t* exec, auto&& body) -> int {
+    auto capture = [&](hipGraphExec_t* exec, Acct* acct, auto&& body) -> int {
         if (*exec) return EQ_OK;
         hipGraph_t g = nullptr;
         EQ_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        *acct = Acct{};
+        b->acct_to = acct; // the captured launches count toward the graph
         body();
+        b->acct_to = &b->acct;
         const hipError_t e = hipStreamEndCapture(s, &g);
         if (e != hipSuccess) return hip_check(e, "hipStreamEndCapture");
         const hipError_t e2 = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
@@ -2530,8 +2601,11 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
     };
     int rc;
     if (graphs) {
-        if ((rc = capture(&b->g_init, setup)) || (rc = capture(&b->g_iter2, two_iterations))) return rc;
+        if ((rc = capture(&b->g_init, &b->g_init_acct, setup)) ||
+            (rc = capture(&b->g_iter2, &b->g_iter2_acct, two_iterations)))
+            return rc;
         EQ_HIP(hipGraphLaunch(b->g_init, s));
+        replayed(b, b->g_init_acct);
     } else {
         setup();
     }
@@ -2543,6 +2617,7 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
         for (int k = 0; k < chunk; k += 2) {
             if (graphs) {
                 EQ_HIP(hipGraphLaunch(b->g_iter2, s));
+                replayed(b, b->g_iter2_acct);
             } else {
                 two_iterations();
             }
@@ -2553,7 +2628,8 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
         for (int p = 0; p < b->nprob; ++p) all = all && b->h_state[p].done;
         if (all) {
             // confirm on the true residual; problems above rtol restart CG from it
-            k_true_residual<R0, TENS><<<g0, kBlock, 0, s>>>(A, b->u, b->rhs, b->r, b->partial);
+            launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, fine_nodes * 24.0, k_true_residual<R0, TENS>, g0, s, A,
+                         (const double*)b->u, (const double*)b->rhs, b->r, b->partial);
             finish(kFinTrue);
             first_sweep();
             if ((rc = read_states(b))) return rc;
@@ -2561,13 +2637,15 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
             for (int p = 0; p < b->nprob; ++p) all = all && b->h_state[p].done;
             if (all) break;
             if (jacobi) {
-                k_jacobi_z<R0, TENS><<<g0, kBlock, 0, s>>>(A, b->r, b->z, b->partial, b->d_state);
+                launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, fine_nodes * 16.0, k_jacobi_z<R0, TENS>, g0, s, A,
+                             (const double*)b->r, b->z, b->partial, (const CgState*)b->d_state);
                 finish(kFinRz);
             }
         }
         chunk = std::max(1, S.check_every);
     }
-    k_keep<R0><<<g0, kBlock, 0, s>>>(A, b->u, b->xprev, b->d_state);
+    launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, fine_nodes * 16.0, k_keep<R0>, g0, s, A, (const double*)b->u, b->xprev,
+                 b->d_state);
     EQ_HIP(hipGetLastError());
     prof_collect(b);
     int32_t itmax = 0, nconv = 0;
@@ -2620,38 +2698,49 @@ static int solve_bicgstab(eq_batch* b, eq_step_stats* stats)
     vp.out = (T*)b->p;   // p^
     vsc.in = b->bs_s;
     vsc.out = (T*)b->p2; // s^
+    const double fin_bytes = (double)L0.tiles_all.size() * 3 * sizeof(double) + 2.0 * b->nprob * sizeof(CgState);
+    const double n0 = L0.nodes_all;
     auto finish = [&](int op) {
-        k_finish<<<b->nprob, kFinishThreads, 0, s>>>(L0.d_probs, b->partial, st, op, 0);
+        launch(b, EQ_KERNEL_FINISH, fin_bytes, k_finish, dim3(b->nprob), dim3(kFinishThreads), 0, s,
+               (const LevelProb*)L0.d_probs, (const double*)b->partial, st, op, 0);
     };
     auto iteration = [&]() {
-        k_bicg_p<R0><<<g0, kBlock, 0, s>>>(A, b->r, b->bs_p, b->bs_v, st);
+        launch_tiles(b, EQ_KERNEL_BICG_VEC, n0 * 32.0, k_bicg_p<R0>, g0, s, A, (const double*)b->r, b->bs_p,
+                     (const double*)b->bs_v, (const CgState*)st);
         vp.run0();
-        launch_slot(b, EQ_KERNEL_STENCIL64, L0.nodes_all * (sizeof(T) + 2 * sizeof(double)), k_bicg_apply<R0, T, TENS>, g0, s,
-                    A, (const T*)b->p, b->bs_v, (const double*)b->bs_rhat, 0, b->partial, st);
+        launch_tiles(b, EQ_KERNEL_STENCIL64, n0 * (sizeof(T) + 2 * sizeof(double)), k_bicg_apply<R0, T, TENS>, g0, s,
+                    A, (const T*)b->p, b->bs_v, (const double*)b->bs_rhat, 0, b->partial, (const CgState*)st);
         finish(kFinBAlpha);
-        k_bicg_s<R0><<<g0, kBlock, 0, s>>>(A, b->r, b->bs_v, b->bs_s, st);
+        launch_tiles(b, EQ_KERNEL_BICG_VEC, n0 * 24.0, k_bicg_s<R0>, g0, s, A, (const double*)b->r,
+                     (const double*)b->bs_v, b->bs_s, (const CgState*)st);
         vsc.run0();
-        k_bicg_apply<R0, T, TENS><<<g0, kBlock, 0, s>>>(A, (const T*)b->p2, b->bs_t, b->bs_s, 1, b->partial, st);
+        launch_tiles(b, EQ_KERNEL_STENCIL64, n0 * (sizeof(T) + 2 * sizeof(double)), k_bicg_apply<R0, T, TENS>, g0, s,
+                     A, (const T*)b->p2, b->bs_t, (const double*)b->bs_s, 1, b->partial, (const CgState*)st);
         finish(kFinBOmega);
-        launch_slot(b, EQ_KERNEL_CG_UPDATE, L0.nodes_all * (2 * sizeof(T) + 6 * sizeof(double)), k_bicg_x<R0, T>, g0, s, A,
+        launch_tiles(b, EQ_KERNEL_CG_UPDATE, n0 * (2 * sizeof(T) + 6 * sizeof(double)), k_bicg_x<R0, T>, g0, s, A,
                     b->u, b->r, (const T*)b->p, (const T*)b->p2, b->bs_s, b->bs_t, b->bs_rhat, b->partial, st);
         finish(kFinBUpdate);
     };
     auto setup = [&]() {
-        k_rhs<R0><<<g0, kBlock, 0, s>>>(A, b->u, b->rhs, b->partial);
+        launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, n0 * 16.0, k_rhs<R0>, g0, s, A, (const double*)b->u, b->rhs, b->partial);
         finish(kFinRhs);
-        k_init<R0, TENS><<<g0, kBlock, 0, s>>>(A, b->u, b->xprev, b->rhs, b->r, b->z, b->partial, st, 0);
+        launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, n0 * 40.0, k_init<R0, TENS>, g0, s, A, b->u, (const double*)b->xprev,
+                     (const double*)b->rhs, b->r, b->z, b->partial, (const CgState*)st, 0);
         finish(kFinBInit);
-        k_copy<R0><<<g0, kBlock, 0, s>>>(A, b->r, b->bs_rhat, st);
+        launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, n0 * 16.0, k_copy<R0>, g0, s, A, (const double*)b->r, b->bs_rhat,
+                     (const CgState*)st);
     };
     bool profiling = false;
     for (int k = 0; k < EQ_KERNEL_COUNT_; ++k) profiling = profiling || b->prof[k].enabled;
     const bool graphs = S.use_graphs && !profiling;
-    auto capture = [&](hipGraphExec_t* exec, auto&& body) -> int {
+    auto capture = [&](hipGraphExec_t* exec, Acct* acct, auto&& body) -> int {
         if (*exec) return EQ_OK;
         hipGraph_t g = nullptr;
         EQ_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        *acct = Acct{};
+        b->acct_to = acct; // the captured launches count toward the graph
         body();
+        b->acct_to = &b->acct;
         const hipError_t e = hipStreamEndCapture(s, &g);
         if (e != hipSuccess) return hip_check(e, "hipStreamEndCapture");
         const hipError_t e2 = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
@@ -2664,8 +2753,10 @@ static int solve_bicgstab(eq_batch* b, eq_step_stats* stats)
     };
     int rc;
     if (graphs) {
-        if ((rc = capture(&b->g_init, setup)) || (rc = capture(&b->g_bicg, iteration))) return rc;
+        if ((rc = capture(&b->g_init, &b->g_init_acct, setup)) || (rc = capture(&b->g_bicg, &b->g_bicg_acct, iteration)))
+            return rc;
         EQ_HIP(hipGraphLaunch(b->g_init, s));
+        replayed(b, b->g_init_acct);
     } else {
         setup();
     }
@@ -2675,6 +2766,7 @@ static int solve_bicgstab(eq_batch* b, eq_step_stats* stats)
         for (int k = 0; k < chunk; ++k) {
             if (graphs) {
                 EQ_HIP(hipGraphLaunch(b->g_bicg, s));
+                replayed(b, b->g_bicg_acct);
             } else {
                 iteration();
             }
@@ -2685,9 +2777,11 @@ static int solve_bicgstab(eq_batch* b, eq_step_stats* stats)
         for (int p = 0; p < b->nprob; ++p) all = all && b->h_state[p].done;
         if (all) {
             // confirm on the true residual; the others restart from r^ = r
-            k_true_residual<R0, TENS><<<g0, kBlock, 0, s>>>(A, b->u, b->rhs, b->r, b->partial);
+            launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, n0 * 24.0, k_true_residual<R0, TENS>, g0, s, A, (const double*)b->u,
+                         (const double*)b->rhs, b->r, b->partial);
             finish(kFinTrue);
-            k_copy<R0><<<g0, kBlock, 0, s>>>(A, b->r, b->bs_rhat, st);
+            launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, n0 * 16.0, k_copy<R0>, g0, s, A, (const double*)b->r, b->bs_rhat,
+                         (const CgState*)st);
             finish(kFinBRestart);
             if ((rc = read_states(b))) return rc;
             all = true;
@@ -2696,7 +2790,7 @@ static int solve_bicgstab(eq_batch* b, eq_step_stats* stats)
         }
         chunk = std::max(1, S.check_every);
     }
-    k_keep<R0><<<g0, kBlock, 0, s>>>(A, b->u, b->xprev, st);
+    launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, n0 * 16.0, k_keep<R0>, g0, s, A, (const double*)b->u, b->xprev, st);
     EQ_HIP(hipGetLastError());
     prof_collect(b);
     int32_t itmax = 0, nconv = 0;
@@ -2716,6 +2810,11 @@ static int solve_bicgstab(eq_batch* b, eq_step_stats* stats)
         stats->nconverged = nconv;
         stats->relres_max = relmax;
         stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    if (nconv < b->nprob) {
+        char msg[128];
+        std::snprintf(msg, sizeof(msg), "max_iter reached above rtol (relres %.3e)", relmax);
+        return set_error(EQ_ERR_NOT_CONVERGED, msg);
     }
     return EQ_OK;
 }
@@ -2754,6 +2853,15 @@ static int run_solve(eq_batch* b, eq_step_stats* stats)
 extern "C" {
 
 const char* eq_last_error(void) { return g_last_error.c_str(); }
+
+int eq_walls_found(const eq_field_desc* desc, int32_t found[4])
+{
+    if (!desc || !found) return set_error(EQ_ERR_INVALID, "NULL argument");
+    std::string why;
+    if (validate(*desc, why)) return set_error(EQ_ERR_INVALID, why);
+    walls_found(*desc, found);
+    return EQ_OK;
+}
 
 void eq_solver_default(eq_solver_desc* s)
 {
@@ -2816,6 +2924,13 @@ int eq_batch_create(const eq_field_desc* descs, int32_t nprob, int32_t nspecies,
     b->nspecies = nspecies;
     b->ntraps = nprob / nspecies;
     b->descs.assign(descs, descs + nprob);
+    for (eq_field_desc& d : b->descs) {
+        if (d.marking != EQ_WALLS_COORD) continue;
+        int32_t found[4];
+        walls_found(d, found);
+        for (int w = 0; w < 4; ++w)
+            if (!found[w]) d.wall[w] = eq_bc{EQ_BC_NEUMANN, 0, 0.0, 0.0, 0.0};
+    }
     b->solver = S;
     b->mixed = (S.kind == EQ_SOLVER_MIXED_GMG_CG || S.kind == EQ_SOLVER_MIXED_GMG_BICGSTAB) ? 1 : 0;
     b->iters.assign(nprob, 0);
@@ -2902,6 +3017,7 @@ void eq_batch_destroy(eq_batch* b)
     hipFree(b->flux_out);
     hipFree(b->d_field_ptrs);
     hipFree(b->cell_scratch);
+    hipFree(b->d_wells);
     free_channels(b);
     hipFree(b->d_cell_traps);
     hipFree(b->d_cell_off);
@@ -2993,9 +3109,10 @@ int eq_batch_step(eq_batch* b, eq_step_stats* stats)
 {
     if (!b) return set_error(EQ_ERR_INVALID, "NULL batch");
     hipSetDevice(b->device);
-    const int rc = run_solve(b, stats);
+    int rc = run_solve(b, stats);
     if (rc != EQ_OK) return rc;
-    return channels_after_step(b); // flow channels take the new trap fields' flux
+    if ((rc = channels_after_step(b))) return rc; // flow channels take the new trap fields' flux
+    return wells_after_step(b);                   // then the boundary wells
 }
 
 // ------------------------------------------------------------ tensor field
@@ -3012,6 +3129,8 @@ static int set_tensor_impl(eq_batch* b, int32_t prob, const double* d11, const d
     const int nlev = (int)b->levels.size();
     hipStream_t s = b->stream;
     EQ_HIP(hipStreamSynchronize(s)); // no kernel may be reading the tables being replaced
+    int prc = pull_well_values(b);
+    if (prc) return prc;
     const bool on = d11 != nullptr;
     if (on) {
         for (int l = 0; l < nlev; ++l) {
@@ -3132,6 +3251,7 @@ static int set_tensor_checked(eq_batch* b, int32_t prob, const double* D11, cons
     const LevelProb& P = b->levels[0].probs[prob];
     if (all && n != (size_t)(P.nx + 1) * (P.ny + 1)) return set_error(EQ_ERR_INVALID, "length must be (nx+1)*(ny+1)");
     if (all && P.fd) return set_error(EQ_ERR_INVALID, "the finite-difference scheme has no tensor field");
+    if (all && P.vel) return set_error(EQ_ERR_INVALID, "advection and a tensor field together are not supported");
     hipSetDevice(b->device);
     return b->mixed ? set_tensor_impl<float>(b, prob, D11, D22, D12) : set_tensor_impl<double>(b, prob, D11, D22, D12);
 }
@@ -3141,8 +3261,6 @@ int eq_batch_set_tensor(eq_batch* b, int32_t prob, const double* D11, const doub
 {
     int rc = set_tensor_checked(b, prob, D11, D22, D12, n);
     if (rc != EQ_OK || !b) return rc;
-    if (D11 && b->levels[0].probs[prob].vel)
-        return set_error(EQ_ERR_INVALID, "advection and a tensor field together are not supported");
     refresh_general(b);
     return EQ_OK;
 }
@@ -3156,6 +3274,8 @@ static int set_velocity_impl(eq_batch* b, int32_t prob, const double* vx)
 {
     hipStream_t s = b->stream;
     EQ_HIP(hipStreamSynchronize(s));
+    int prc = pull_well_values(b);
+    if (prc) return prc;
     const int nlev = (int)b->levels.size();
     if ((int)b->vel_host.size() < b->nprob) b->vel_host.resize(b->nprob);
     std::vector<std::vector<double>>& rows = b->vel_host[prob];
@@ -3365,14 +3485,14 @@ static int apply_diag(eq_batch* b, int32_t prob, const double* x, double* y, siz
     const LevelView A = b->levels[0].all();
     if (b->any_tensor)
         if (b->levels[0].R == kRowsWide)
-            k_apply_full<kRowsWide, true><<<grid_for(A.ntiles), kBlock, 0, s>>>(A, prob, b->q, b->z, mode);
+            launch_tiles(b, -1, 0.0, k_apply_full<kRowsWide, true>, grid_for(A.ntiles), s, A, (int)prob, (const double*)b->q, b->z, mode);
         else
-            k_apply_full<kRows0, true><<<grid_for(A.ntiles), kBlock, 0, s>>>(A, prob, b->q, b->z, mode);
+            launch_tiles(b, -1, 0.0, k_apply_full<kRows0, true>, grid_for(A.ntiles), s, A, (int)prob, (const double*)b->q, b->z, mode);
     else
         if (b->levels[0].R == kRowsWide)
-            k_apply_full<kRowsWide, false><<<grid_for(A.ntiles), kBlock, 0, s>>>(A, prob, b->q, b->z, mode);
+            launch_tiles(b, -1, 0.0, k_apply_full<kRowsWide, false>, grid_for(A.ntiles), s, A, (int)prob, (const double*)b->q, b->z, mode);
         else
-            k_apply_full<kRows0, false><<<grid_for(A.ntiles), kBlock, 0, s>>>(A, prob, b->q, b->z, mode);
+            launch_tiles(b, -1, 0.0, k_apply_full<kRows0, false>, grid_for(A.ntiles), s, A, (int)prob, (const double*)b->q, b->z, mode);
     EQ_HIP(hipGetLastError());
     EQ_HIP(hipMemcpy2DAsync(y, sizeof(double) * nw, b->z + P.origin, sizeof(double) * P.pitch, sizeof(double) * nw,
                             nh, hipMemcpyDeviceToHost, s));
@@ -3394,12 +3514,90 @@ int eq_batch_boundary_flux(eq_batch* b, int32_t prob, double* total)
 {
     if (!b || !total) return set_error(EQ_ERR_INVALID, "NULL argument");
     if (prob < 0 || prob >= b->nprob) return set_error(EQ_ERR_INVALID, "problem index out of range");
-    k_flux<<<1, 256, 0, b->stream>>>(b->levels[0].d_probs, prob, b->u, b->flux_out);
+    const LevelProb& P = b->levels[0].probs[prob];
+    launch(b, EQ_KERNEL_BOUNDARY, 4.0 * (P.nx + P.ny) * sizeof(double), k_flux, dim3(1), dim3(256), 0, b->stream,
+           (const LevelProb*)b->levels[0].d_probs, (int)prob, (const double*)b->u, b->flux_out);
     EQ_HIP(hipGetLastError());
     EQ_HIP(hipMemcpyAsync(b->h_flux, b->flux_out, sizeof(double), hipMemcpyDeviceToHost, b->stream));
     EQ_HIP(hipStreamSynchronize(b->stream));
     *total = *b->h_flux;
     return EQ_OK;
+}
+
+int eq_batch_set_boundary_value(eq_batch* b, int32_t prob, double value)
+{
+    if (!b) return set_error(EQ_ERR_INVALID, "NULL batch");
+    if (prob < 0 || prob >= b->nprob) return set_error(EQ_ERR_INVALID, "problem index out of range");
+    if (!std::isfinite(value)) return set_error(EQ_ERR_INVALID, "boundary value must be finite");
+    LevelProb& P = b->levels[0].probs[prob];
+    if (!P.dmask) return set_error(EQ_ERR_INVALID, "the problem has no Dirichlet wall");
+    hipSetDevice(b->device);
+    for (int k = 0; k < 4; ++k)
+        if ((P.dmask >> k) & 1) P.g[k] = value; // the host copy stays current
+    launch(b, EQ_KERNEL_BOUNDARY, 0.0, k_set_walls, dim3(1), dim3(64), 0, b->stream, b->levels[0].d_probs, (int)prob,
+           value);
+    EQ_HIP(hipGetLastError());
+    return EQ_OK;
+}
+
+int eq_batch_set_well(eq_batch* b, int32_t prob, const eq_well_desc* well)
+{
+    if (!b) return set_error(EQ_ERR_INVALID, "NULL batch");
+    if (prob < 0 || prob >= b->nprob) return set_error(EQ_ERR_INVALID, "problem index out of range");
+    const LevelProb& P = b->levels[0].probs[prob];
+    if (well) {
+        if (P.dmask != 15) return set_error(EQ_ERR_INVALID, "a boundary well needs all four walls Dirichlet");
+        for (int k = 0; k < 4; ++k)
+            if (P.gw[k]) return set_error(EQ_ERR_INVALID, "a boundary well's walls cannot carry per-node values");
+        if (!(well->well_scaling > 0) || !std::isfinite(well->well_scaling) || !(well->decay_rate >= 0) ||
+            !std::isfinite(well->decay_rate) || !std::isfinite(well->w0))
+            return set_error(EQ_ERR_INVALID, "need well_scaling > 0, decay_rate >= 0, finite w0");
+    }
+    hipSetDevice(b->device);
+    int rc = pull_well_values(b); // also brings the device's well states into b->wells
+    if (rc) return rc;
+    for (size_t k = 0; k < b->wells.size(); ++k)
+        if (b->wells[k].prob == prob) {
+            b->wells.erase(b->wells.begin() + (ptrdiff_t)k);
+            break;
+        }
+    if (well) {
+        WellDev W{};
+        W.prob = prob;
+        W.scaling = well->well_scaling;
+        W.decay = well->decay_rate;
+        W.dt = b->descs[prob].dt;
+        W.w = well->w0;
+        b->wells.push_back(W);
+    }
+    if (!b->d_wells && !b->wells.empty()) {
+        if (hipMalloc((void**)&b->d_wells, sizeof(WellDev) * b->nprob) != hipSuccess) {
+            b->d_wells = nullptr;
+            return set_error(EQ_ERR_NOMEM, "well table");
+        }
+        b->d_wells_cap = b->nprob;
+    }
+    if (!b->wells.empty())
+        EQ_HIP(hipMemcpy(b->d_wells, b->wells.data(), sizeof(WellDev) * b->wells.size(), hipMemcpyHostToDevice));
+    // the first solve sees the starting concentration on the walls
+    return well ? eq_batch_set_boundary_value(b, prob, well->w0) : EQ_OK;
+}
+
+int eq_batch_well_state(eq_batch* b, int32_t prob, double* w, double* last_flux, int64_t* underflows)
+{
+    if (!b) return set_error(EQ_ERR_INVALID, "NULL batch");
+    if (prob < 0 || prob >= b->nprob) return set_error(EQ_ERR_INVALID, "problem index out of range");
+    hipSetDevice(b->device);
+    int rc = pull_well_values(b);
+    if (rc) return rc;
+    for (const WellDev& W : b->wells)
+        if (W.prob == prob) {
+            if (w) *w = W.w;
+            if (last_flux) *last_flux = W.flux;
+            if (underflows) *underflows = W.underflows;
+            return EQ_OK;
+        }
+    return set_error(EQ_ERR_INVALID, "the problem has no boundary well");
 }
 
 int eq_batch_vertex_map(const eq_batch* b, int32_t prob, double* coords, int32_t* dof_from_vertex)
@@ -3438,6 +3636,15 @@ int eq_batch_profile_read(eq_batch* b, int32_t kernel, int64_t* launches, double
     if (launches) *launches = b->prof[kernel].launches;
     if (total_ms) *total_ms = b->prof[kernel].total_ms;
     if (bytes) *bytes = b->prof[kernel].bytes;
+    return EQ_OK;
+}
+
+int eq_batch_traffic(eq_batch* b, double* bytes, int64_t* launches, int32_t reset)
+{
+    if (!b) return set_error(EQ_ERR_INVALID, "NULL batch");
+    if (bytes) *bytes = b->acct.bytes;
+    if (launches) *launches = b->acct.launches;
+    if (reset) b->acct = Acct{};
     return EQ_OK;
 }
 

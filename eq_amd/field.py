@@ -18,30 +18,49 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libeq_field.so")
 
 SCHEME_P1, SCHEME_FD = 0, 1
+WALLS_GRID, WALLS_COORD = 0, 1
 NEUMANN, DIRICHLET, ROBIN = 0, 1, 2
 LEFT, RIGHT, TOP, BOTTOM = 0, 1, 2, 3
 JACOBI_CG, GMG_CG, MIXED_GMG_CG = 0, 1, 2
 GMG_BICGSTAB, MIXED_GMG_BICGSTAB = 3, 4
 SAMPLE, DEPOSIT = 1, 2
 KERNEL_STENCIL64, KERNEL_SMOOTH, KERNEL_CG_UPDATE, KERNEL_COUPLE = 0, 1, 2, 3
+KERNEL_SMOOTH_COARSE, KERNEL_TRANSFER0, KERNEL_TRANSFER_COARSE, KERNEL_BOTTOM = 4, 5, 6, 7
+KERNEL_FINISH, KERNEL_SOLVE_ONCE, KERNEL_BICG_VEC, KERNEL_COUPLE_SETUP, KERNEL_BOUNDARY = 8, 9, 10, 11, 12
+KERNEL_COUNT = 13
+KERNEL_FAMILIES = {
+    KERNEL_STENCIL64: "Krylov operator apply (k_ap_dir, k_bicg_apply)",
+    KERNEL_SMOOTH: "finest-level Jacobi sweeps (k_smooth, k_presmooth2, k_prolong_smooth on level 0)",
+    KERNEL_CG_UPDATE: "Krylov x, r update + dots (k_update, k_bicg_x)",
+    KERNEL_COUPLE: "cell sample/deposit per tile (k_tile_couple)",
+    KERNEL_SMOOTH_COARSE: "coarse-level Jacobi sweeps (k_smooth, k_presmooth2, k_prolong_smooth)",
+    KERNEL_TRANSFER0: "finest-level residual and transfers (k_residual, k_restrict, k_residual_restrict, k_prolong)",
+    KERNEL_TRANSFER_COARSE: "coarse-level residual and transfers",
+    KERNEL_BOTTOM: "LDS bottom V-cycle (k_bottom, k_cast_in, k_cast_out)",
+    KERNEL_FINISH: "per-problem reduction finish (k_finish)",
+    KERNEL_SOLVE_ONCE: "once per solve (k_rhs, k_init, k_true_residual, k_keep, first sweep, k_copy)",
+    KERNEL_BICG_VEC: "BiCGStab vector updates (k_bicg_p, k_bicg_s)",
+    KERNEL_COUPLE_SETUP: "cell geometry, pair scan and sort, tile bounds, per-cell sums",
+    KERNEL_BOUNDARY: "channels, boundary wells and values, flux",
+}
 
 STATUS = {0: "EQ_OK", 1: "EQ_ERR_INVALID", 2: "EQ_ERR_HIP", 3: "EQ_ERR_NOMEM",
           4: "EQ_ERR_NOT_CONVERGED", 5: "EQ_ERR_NO_DEVICE"}
 
 # every symbol include/eq_field.h declares
 EXPORTED = [
-    "eq_last_error", "eq_solver_default",
+    "eq_last_error", "eq_solver_default", "eq_walls_found",
     "eq_batch_create", "eq_batch_destroy", "eq_batch_num_problems", "eq_batch_num_nodes",
     "eq_batch_stream", "eq_batch_sync", "eq_batch_upload", "eq_batch_download",
     "eq_batch_upload_dev", "eq_batch_download_dev", "eq_batch_step", "eq_batch_problem_stats",
     "eq_batch_couple", "eq_batch_couple_dev", "eq_batch_couple_traps_dev", "eq_batch_autotune",
     "eq_batch_set_tensor", "eq_field_set_tensor", "eq_batch_set_velocity", "eq_batch_set_wall_values", "eq_batch_set_channels",
     "eq_batch_channel_values",
-    "eq_batch_boundary_flux",
+    "eq_batch_boundary_flux", "eq_batch_set_boundary_value", "eq_batch_set_well", "eq_batch_well_state",
     "eq_batch_vertex_map",
     "eq_field_create", "eq_field_destroy", "eq_field_upload", "eq_field_download",
     "eq_field_step", "eq_field_vertex_map", "eq_field_boundary_flux", "eq_cells_couple",
-    "eq_batch_profile", "eq_batch_profile_read", "eq_batch_apply", "eq_batch_rhs",
+    "eq_batch_profile", "eq_batch_profile_read", "eq_batch_traffic", "eq_batch_apply", "eq_batch_rhs",
 ]
 
 
@@ -54,7 +73,7 @@ class FieldDesc(C.Structure):
     _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("width", C.c_double),
                 ("height", C.c_double), ("dt", C.c_double), ("D", C.c_double),
                 ("gamma", C.c_double), ("f", C.c_double), ("wall", Bc * 4), ("scheme", C.c_int32),
-                ("pad_", C.c_int32)]
+                ("marking", C.c_int32)]
 
 
 class ChannelDesc(C.Structure):
@@ -68,6 +87,10 @@ def channel_desc(v, r_left, s_left, r_right, s_right, well_scaling, substeps=1) 
     c.v, c.r_left, c.s_left, c.r_right, c.s_right = v, r_left, s_left, r_right, s_right
     c.well_scaling, c.substeps = well_scaling, substeps
     return c
+
+
+class WellDesc(C.Structure):
+    _fields_ = [("well_scaling", C.c_double), ("decay_rate", C.c_double), ("w0", C.c_double)]
 
 
 class SolverDesc(C.Structure):
@@ -116,6 +139,7 @@ def load_library():
     vp, dp, ip32, ip64 = C.c_void_p, P(C.c_double), P(C.c_int32), P(C.c_int64)
     L.eq_last_error.restype = C.c_char_p
     L.eq_solver_default.argtypes = [P(SolverDesc)]
+    L.eq_walls_found.argtypes = [P(FieldDesc), ip32]
     L.eq_batch_create.argtypes = [P(FieldDesc), C.c_int32, C.c_int32, P(SolverDesc), P(vp)]
     L.eq_batch_destroy.argtypes = [vp]
     L.eq_batch_num_problems.argtypes = [vp]
@@ -142,9 +166,13 @@ def load_library():
     L.eq_batch_couple_dev.argtypes = [vp, C.c_int32, vp, C.c_size_t, C.c_double, vp, vp, vp,
                                       C.c_uint32]
     L.eq_batch_boundary_flux.argtypes = [vp, C.c_int32, dp]
+    L.eq_batch_set_boundary_value.argtypes = [vp, C.c_int32, C.c_double]
+    L.eq_batch_set_well.argtypes = [vp, C.c_int32, P(WellDesc)]
+    L.eq_batch_well_state.argtypes = [vp, C.c_int32, dp, dp, ip64]
     L.eq_batch_vertex_map.argtypes = [vp, C.c_int32, dp, ip32]
     L.eq_batch_profile.argtypes = [vp, C.c_int32, C.c_int32]
     L.eq_batch_profile_read.argtypes = [vp, C.c_int32, ip64, dp, dp]
+    L.eq_batch_traffic.argtypes = [vp, dp, ip64, C.c_int32]
     L.eq_batch_apply.argtypes = [vp, C.c_int32, vp, vp, C.c_size_t]
     L.eq_batch_rhs.argtypes = [vp, C.c_int32, vp, vp, C.c_size_t]
     L.eq_field_create.argtypes = [P(FieldDesc), P(SolverDesc), P(vp)]
@@ -164,17 +192,28 @@ def _check(rc: int):
         raise EqError(rc, load_library().eq_last_error().decode())
 
 
-def field_desc(nx, ny, width, height, dt, D, gamma=0.0, f=0.0, walls=None, scheme=0) -> FieldDesc:
+def field_desc(nx, ny, width, height, dt, D, gamma=0.0, f=0.0, walls=None, scheme=0,
+               marking=WALLS_GRID) -> FieldDesc:
     """walls: {wall: (kind, value, r, s)}; missing walls are Neumann.
-    scheme: SCHEME_P1 (fenicsInterface's P1 form) or SCHEME_FD (diffusionPETSc)."""
+    scheme: SCHEME_P1 (fenicsInterface's P1 form) or SCHEME_FD (diffusionPETSc).
+    marking: WALLS_GRID (outer grid rows/columns) or WALLS_COORD (the
+    reference's near() test on DOLFIN's computed vertex coordinates)."""
     d = FieldDesc()
     d.nx, d.ny, d.width, d.height = nx, ny, width, height
     d.dt, d.D, d.gamma, d.f = dt, D, gamma, f
     d.scheme = scheme
+    d.marking = marking
     for w in range(4):
         kind, value, r, s = (walls or {}).get(w, (NEUMANN, 0.0, 0.0, 0.0))
         d.wall[w].kind, d.wall[w].value, d.wall[w].r, d.wall[w].s = kind, value, r, s
     return d
+
+
+def walls_found(desc: FieldDesc) -> list:
+    """Per wall (left, right, top, bottom): found by the reference's coordinate test."""
+    f = np.empty(4, np.int32)
+    _check(load_library().eq_walls_found(C.byref(desc), f.ctypes.data_as(C.POINTER(C.c_int32))))
+    return [bool(v) for v in f]
 
 
 def solver_desc(kind=MIXED_GMG_CG, **kw) -> SolverDesc:
@@ -346,6 +385,25 @@ class FieldBatch:
         _check(load_library().eq_batch_boundary_flux(self._h, prob, C.byref(v)))
         return v.value
 
+    def set_boundary_value(self, prob: int, value: float):
+        """setBoundaryValues: every Dirichlet wall of `prob` takes `value` (stream ordered)."""
+        _check(load_library().eq_batch_set_boundary_value(self._h, prob, value))
+
+    def set_well(self, prob: int, well_scaling=None, decay_rate=0.0, w0=0.0):
+        """Boundary well of the DIRICHLET_UPDATE boundary, updated after every step;
+        well_scaling None removes it."""
+        if well_scaling is None:
+            _check(load_library().eq_batch_set_well(self._h, prob, None))
+            return
+        w = WellDesc(well_scaling, decay_rate, w0)
+        _check(load_library().eq_batch_set_well(self._h, prob, C.byref(w)))
+
+    def well_state(self, prob: int):
+        """(w, last flux, underflows) of the problem's boundary well."""
+        w, f, n = C.c_double(), C.c_double(), C.c_int64()
+        _check(load_library().eq_batch_well_state(self._h, prob, C.byref(w), C.byref(f), C.byref(n)))
+        return w.value, f.value, n.value
+
     def vertex_map(self, prob: int):
         n = self.num_nodes(prob)
         xy = np.empty(2 * n)
@@ -370,6 +428,12 @@ class FieldBatch:
 
     def profile(self, kernel: int, enable: bool = True):
         _check(load_library().eq_batch_profile(self._h, kernel, 1 if enable else 0))
+
+    def traffic(self, reset: bool = False):
+        """(algorithmic bytes, launches) issued since the last reset, graph replays included."""
+        by, n = C.c_double(), C.c_int64()
+        _check(load_library().eq_batch_traffic(self._h, C.byref(by), C.byref(n), 1 if reset else 0))
+        return by.value, n.value
 
     def profile_read(self, kernel: int):
         n = C.c_int64()

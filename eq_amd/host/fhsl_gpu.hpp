@@ -9,7 +9,10 @@
 //   shell->dof_from_vertex                       src/simulation.cpp:298-308, src/fHSL.h:82-88
 //   D11 / D22 / D12                              src/simulation.cpp:311-313, 503-505
 //   solution_vector                              src/simulation.cpp:492, 501
-//   totalBoundaryFlux, writeDiffusionFiles, finalize
+//   totalBoundaryFlux, getBoundaryFlux()          src/simulation.cpp:546, 583; src/eQ.h:327
+//   setBoundaryValues(bval)                       src/simulation.cpp:602; src/fHSL.cpp:601-603
+//   writeDiffusionFiles (+ channel series), writeDataFiles,
+//   the data-recording constructor, finalize      src/simulation.cpp:536, 569, 578, 683
 // eqgpu::fenicsInterface keeps those names and meanings.  All species of a
 // trap share one device batch (eqgpu::trapField), so the solve of every
 // species is one set of launches, and the controller's cell loop
@@ -17,9 +20,11 @@
 // trapField::coupleCells.
 #pragma once
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <filesystem>
+#include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -35,6 +40,28 @@ inline void check(int rc, const char* what)
     if (rc != EQ_OK) throw std::runtime_error(std::string(what) + ": " + eq_last_error());
 }
 
+// One data file of the controller's data-recording grid (eQ::data::record,
+// src/eQ.h:95-104): a file name and the eQ::data::tensor it samples.  The
+// tensor's grids are natural-order arrays of (y*n + 1) x (x*n + 1) nodes at n
+// nodes per micron (src/eQ.h:147-157); a vector tensor has a second grid.
+struct dataRecord {
+    std::string fileName;
+    const std::vector<double>* grid0 = nullptr;
+    const std::vector<double>* grid1 = nullptr; // non-null: VECTOR rank
+    size_t nodesWide = 0, nodesHigh = 0;
+    double nodesPerMicron = 1.0;
+    // tensor::eval / evalVector: the grid value at ij_from_xy(x, y, n)
+    // (src/eQ.h:119-124, 162-175); indices past the grid are clamped to it
+    double at(const std::vector<double>& g, double x, double y) const
+    {
+        size_t j = (size_t)std::llround(std::max(0.0, x * nodesPerMicron));
+        size_t i = (size_t)std::llround(std::max(0.0, y * nodesPerMicron));
+        j = std::min(j, nodesWide - 1);
+        i = std::min(i, nodesHigh - 1);
+        return g[j + i * nodesWide];
+    }
+};
+
 // eQ::diffusionSolver::params (src/eQ.h:305-321) without the MPI and JSON
 // handles, which a single-GPU process does not need
 struct solverParams {
@@ -42,6 +69,9 @@ struct solverParams {
     double dt = 0.1;
     double D_HSL = 0.0;
     std::string filePath;
+    std::string filePathTopChannel;    // channel file series (MICROFLUIDIC_TRAP, src/fHSL.cpp:613-627)
+    std::string filePathBottomChannel;
+    std::vector<dataRecord> dataFiles; // data-recording grid (src/fHSL.cpp:288-307)
     double trapHeightMicrons = 0.0;
     double trapWidthMicrons = 0.0;
     double nodesPerMicron = 1.0;
@@ -118,6 +148,36 @@ public:
         check(eq_batch_boundary_flux(batch_.get(), s, &v), "boundary flux");
         return v;
     }
+    // setBoundaryValues of one species (stream ordered on the device)
+    void setBoundaryValue(int s, double v) { check(eq_batch_set_boundary_value(batch_.get(), s, v), "boundary value"); }
+    // Simulation::computeBoundaryWell run on the device after every step
+    // (DIRICHLET_UPDATE): wellScaling, boundaryDecayRate, starting value
+    void setBoundaryWell(int s, double wellScaling, double decayRate, double w0 = 0.0)
+    {
+        const eq_well_desc w{wellScaling, decayRate, w0};
+        check(eq_batch_set_well(batch_.get(), s, &w), "boundary well");
+    }
+    double boundaryWellConcentration(int s)
+    {
+        double w = 0.0;
+        check(eq_batch_well_state(batch_.get(), s, &w, nullptr, nullptr), "well state");
+        return w;
+    }
+    // flow channels above/below the trap (MICROFLUIDIC_TRAP); null: no channel
+    void setChannels(int s, const eq_channel_desc* top, const eq_channel_desc* bottom)
+    {
+        check(eq_batch_set_channels(batch_.get(), s, top, bottom), "channels");
+        if ((int)chan_.size() <= s) chan_.resize(s + 1, 0);
+        chan_[s] = (top ? 1 : 0) | (bottom ? 2 : 0);
+    }
+    int channels(int s) const { return s < (int)chan_.size() ? chan_[s] : 0; }
+    // topChannelData / bottomChannelData (nx + 1 values each)
+    void channelValues(int s, std::vector<double>& top, std::vector<double>& bottom)
+    {
+        top.resize((size_t)nx_ + 1);
+        bottom.resize((size_t)nx_ + 1);
+        check(eq_batch_channel_values(batch_.get(), s, top.data(), bottom.data()), "channel values");
+    }
 
     // Per-species host vectors registered by the fenicsInterface views.  The
     // first species to step in a time step uploads every registered vector,
@@ -186,6 +246,7 @@ private:
         std::vector<double> sent[3]; // last values sent (empty: constant coefficients)
     };
     std::vector<Tens> tens_;
+    std::vector<int> chan_;
     std::vector<std::vector<double>*> host_;
     std::vector<char> solved_;
     eq_step_stats last_{};
@@ -212,6 +273,21 @@ public:
 
     fenicsInterface(std::shared_ptr<trapField> field, int species) : field_(std::move(field)), s_(species) {}
 
+    // Data recording only (src/fHSL.cpp:27-33, 288-307): no field and no
+    // solve; a P1 mesh of ceil(W*n) x ceil(H*n) cells at the params' nodes per
+    // micron and one .pvd series per data file at filePath + fileName.
+    explicit fenicsInterface(const solverParams& p) : params_(p), recording_(true)
+    {
+        nx_ = (int)std::ceil(p.trapWidthMicrons * p.nodesPerMicron);
+        ny_ = (int)std::ceil(p.trapHeightMicrons * p.nodesPerMicron);
+        const size_t n = (size_t)(nx_ + 1) * (ny_ + 1);
+        shell = std::make_shared<shellView>();
+        shell->mesh->n = n;
+        for (const dataRecord& d : p.dataFiles)
+            data_.emplace_back(p.filePath + d.fileName, nx_, ny_, p.trapWidthMicrons, p.trapHeightMicrons,
+                               d.grid1 ? "v" : "u");
+    }
+
     void initDiffusion(const solverParams& p)
     {
         params_ = p;
@@ -232,6 +308,8 @@ public:
         // dolfin::File(comm, filePath, "compressed") (src/fHSL.cpp:606-613)
         if (!p.filePath.empty())
             files_ = eqio::pvdSeries(p.filePath, field_->nx(), field_->ny(), field_->width(), field_->height());
+        nx_ = field_->nx();
+        ny_ = field_->ny();
     }
 
     // u0 <- solution_vector; solve; solution_vector <- u; boundary flux
@@ -241,11 +319,55 @@ public:
         field_->stepSpecies(s_);
         totalBoundaryFlux = field_->boundaryFlux(s_);
     }
-    // one .vtu of solution_vector per call, listed in the .pvd at timeStamp
-    // (src/fHSL.cpp:628-634)
+    // one .vtu of solution_vector per call, listed in the .pvd at timeStamp,
+    // then the top and bottom channel fields when this species has channels
+    // and the params name their files (src/fHSL.cpp:613-634)
     void writeDiffusionFiles(double timeStamp)
     {
         if (files_.open()) files_.write(solution_vector.data(), timeStamp);
+        const int ch = field_->channels(s_);
+        if (!ch) return;
+        const bool top = (ch & 1) && !params_.filePathTopChannel.empty();
+        const bool bottom = (ch & 2) && !params_.filePathBottomChannel.empty();
+        if (!top && !bottom) return;
+        std::vector<double> t, b;
+        field_->channelValues(s_, t, b);
+        if (top) {
+            if (!topFiles_.open())
+                topFiles_ = eqio::pvdSeries(params_.filePathTopChannel, field_->nx(), 0, field_->width(), 0.0);
+            topFiles_.write(t.data(), timeStamp);
+        }
+        if (bottom) {
+            if (!bottomFiles_.open())
+                bottomFiles_ = eqio::pvdSeries(params_.filePathBottomChannel, field_->nx(), 0, field_->width(), 0.0);
+            bottomFiles_.write(b.data(), timeStamp);
+        }
+    }
+    // setBoundaryValues(bval) (src/fHSL.cpp:601-603): every Dirichlet wall of
+    // this species takes bval from the next solve on
+    void setBoundaryValues(double bval) { field_->setBoundaryValue(s_, bval); }
+    // getBoundaryFlux (src/eQ.h:327): at least "totalFlux"
+    std::map<std::string, double> getBoundaryFlux() const { return {{"totalFlux", totalBoundaryFlux}}; }
+    // writeDataFiles (src/fHSL.cpp:637-654): each data tensor interpolated on
+    // the mesh vertices (x_j, y_i) = (j W/nx, i H/ny), one .vtu per file per call
+    void writeDataFiles(double t)
+    {
+        if (!recording_) return;
+        const size_t n = (size_t)(nx_ + 1) * (ny_ + 1);
+        std::vector<double> a(n), b;
+        for (size_t k = 0; k < params_.dataFiles.size(); ++k) {
+            const dataRecord& d = params_.dataFiles[k];
+            if (d.grid1) b.resize(n);
+            for (int i = 0; i <= ny_; ++i)
+                for (int j = 0; j <= nx_; ++j) {
+                    const double x = 0.0 + ((double)j * (params_.trapWidthMicrons - 0.0)) / (double)nx_;
+                    const double y = 0.0 + ((double)i * (params_.trapHeightMicrons - 0.0)) / (double)ny_;
+                    const size_t q = (size_t)j + (size_t)i * (nx_ + 1);
+                    a[q] = d.at(*d.grid0, x, y);
+                    if (d.grid1) b[q] = d.at(*d.grid1, x, y);
+                }
+            data_[k].write(a.data(), t, d.grid1 ? b.data() : nullptr);
+        }
     }
     void finalize() {}
 
@@ -258,7 +380,10 @@ private:
     std::shared_ptr<trapField> field_;
     int s_ = 0;
     solverParams params_;
-    eqio::pvdSeries files_;
+    bool recording_ = false;
+    int nx_ = 0, ny_ = 0;
+    eqio::pvdSeries files_, topFiles_, bottomFiles_;
+    std::vector<eqio::pvdSeries> data_;
 };
 
 // diffusionPETSc (diffuclass.h:45-108): the finite-difference alternative

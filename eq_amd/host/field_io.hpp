@@ -100,8 +100,10 @@ inline const char* vtk_header(bool compressed)
 // (i, j) splits along its bottom-left -> top-right diagonal into
 // (v00, v01, v11) and (v00, v11, v10), 2 nx ny triangles; points are the
 // (nx + 1)(ny + 1) nodes in natural order at (j W / nx, i H / ny, 0).
+// uy non-null: a vector field (ux, uy), written as 3 components with z = 0,
+// the layout dolfin::File uses for a 2-D vector function.
 inline void write_vtu(const std::string& path, int nx, int ny, double W, double H, const double* u,
-                      const std::string& name, bool compressed = true)
+                      const std::string& name, bool compressed = true, const double* uy = nullptr)
 {
     if (nx < 1 || ny < 1) throw std::runtime_error("write_vtu: empty grid");
     const size_t np = (size_t)(nx + 1) * (ny + 1), nc = 2 * (size_t)nx * ny;
@@ -134,10 +136,53 @@ inline void write_vtu(const std::string& path, int nx, int ny, double W, double 
     put_array(f, "Int64", "connectivity", 1, conn.data(), conn.size() * 8, compressed);
     put_array(f, "Int64", "offsets", 1, offs.data(), offs.size() * 8, compressed);
     put_array(f, "UInt8", "types", 1, types.data(), types.size(), compressed);
+    if (uy) {
+        std::vector<double> v(3 * np);
+        for (size_t k = 0; k < np; ++k) {
+            v[3 * k] = u[k];
+            v[3 * k + 1] = uy[k];
+            v[3 * k + 2] = 0.0;
+        }
+        f << "</Cells>\n<PointData Vectors=\"" << name << "\">\n";
+        put_array(f, "Float64", name.c_str(), 3, v.data(), v.size() * 8, compressed);
+    } else {
+        f << "</Cells>\n<PointData Scalars=\"" << name << "\">\n";
+        put_array(f, "Float64", name.c_str(), 1, u, np * 8, compressed);
+    }
+    f << "</PointData>\n</Piece>\n</UnstructuredGrid>\n</VTKFile>\n";
+    if (!f) throw std::runtime_error("write_vtu: write failed for " + path);
+}
+
+// A 1-D P1 field on the interval mesh of a flow channel (fenicsChannel's
+// IntervalMesh over the trap's columns, src/fHSL.h:240-332): nx + 1 points at
+// (j W / nx, 0, 0) and nx line cells.
+inline void write_vtu_line(const std::string& path, int nx, double W, const double* u, const std::string& name,
+                           bool compressed = true)
+{
+    if (nx < 1) throw std::runtime_error("write_vtu_line: empty mesh");
+    const size_t np = (size_t)nx + 1, nc = (size_t)nx;
+    std::vector<double> pts(3 * np, 0.0);
+    for (size_t j = 0; j < np; ++j) pts[3 * j] = W * (double)j / nx;
+    std::vector<int64_t> conn(2 * nc), offs(nc);
+    std::vector<uint8_t> types(nc, 3); // VTK_LINE
+    for (size_t c = 0; c < nc; ++c) {
+        conn[2 * c] = (int64_t)c;
+        conn[2 * c + 1] = (int64_t)c + 1;
+        offs[c] = 2 * (int64_t)(c + 1);
+    }
+    std::ofstream f(path, std::ios::trunc);
+    if (!f) throw std::runtime_error("write_vtu_line: cannot open " + path);
+    f << "<?xml version=\"1.0\"?>\n<VTKFile type=\"UnstructuredGrid\"" << vtk_header(compressed) << ">\n"
+      << "<UnstructuredGrid>\n<Piece NumberOfPoints=\"" << np << "\" NumberOfCells=\"" << nc << "\">\n<Points>\n";
+    put_array(f, "Float64", "Points", 3, pts.data(), pts.size() * 8, compressed);
+    f << "</Points>\n<Cells>\n";
+    put_array(f, "Int64", "connectivity", 1, conn.data(), conn.size() * 8, compressed);
+    put_array(f, "Int64", "offsets", 1, offs.data(), offs.size() * 8, compressed);
+    put_array(f, "UInt8", "types", 1, types.data(), types.size(), compressed);
     f << "</Cells>\n<PointData Scalars=\"" << name << "\">\n";
     put_array(f, "Float64", name.c_str(), 1, u, np * 8, compressed);
     f << "</PointData>\n</Piece>\n</UnstructuredGrid>\n</VTKFile>\n";
-    if (!f) throw std::runtime_error("write_vtu: write failed for " + path);
+    if (!f) throw std::runtime_error("write_vtu_line: write failed for " + path);
 }
 
 // A .pvd collection with one .vtu per write, named <stem>NNNNNN.vtu beside
@@ -157,12 +202,16 @@ public:
     bool open() const { return !path_.empty(); }
     const std::string& path() const { return path_; }
 
-    std::string write(const double* u, double t)
+    // ny = 0: a 1-D channel field (write_vtu_line); uy non-null: a vector field
+    std::string write(const double* u, double t, const double* uy = nullptr)
     {
         char num[16];
         std::snprintf(num, sizeof(num), "%06d", (int)steps_.size());
         const std::string vtu = stem_ + num + ".vtu";
-        write_vtu(vtu, nx_, ny_, W_, H_, u, name_, compressed_);
+        if (ny_ == 0)
+            write_vtu_line(vtu, nx_, W_, u, name_, compressed_);
+        else
+            write_vtu(vtu, nx_, ny_, W_, H_, u, name_, compressed_, uy);
         const size_t slash = vtu.find_last_of('/');
         steps_.emplace_back(t, slash == std::string::npos ? vtu : vtu.substr(slash + 1));
         std::ofstream f(path_, std::ios::trunc);

@@ -68,13 +68,27 @@ typedef enum {
                        * form (rows scaled by hx*hy/2 on walls, hx*hy/4 in corners) */
 } eq_scheme;
 
+/* How the walls are found on the mesh.  The reference marks a wall's facets
+ * with near(coordinate, W or H) on the vertex coordinates DOLFIN computes,
+ * x_j = (j*W)/nx and y_i = (i*H)/ny (DirichletBoundary_TrapEdge::inside,
+ * src/fHSL.h:575-590; RectangleMesh, src/fHSL.cpp:169-172), with
+ * near(a, b) = |a - b| < DOLFIN_EPS = 3e-16.  For an integer trap size these
+ * coordinates are exact and both modes find every wall.  For other sizes the
+ * right or top coordinate can miss by an ulp (e.g. 218 cells over 21.8 um),
+ * and the reference then applies no condition on that wall. */
+typedef enum {
+    EQ_WALLS_GRID = 0,  /* the outer rows and columns of the grid (default) */
+    EQ_WALLS_COORD = 1  /* the reference's coordinate test: a wall it misses
+                         * becomes Neumann (no Dirichlet, no Robin term) */
+} eq_wall_marking;
+
 typedef struct {
     int32_t nx, ny;
     double width, height; /* microns */
     double dt, D, gamma, f;
     eq_bc wall[4];
     int32_t scheme;       /* eq_scheme (0: the P1 form above)  */
-    int32_t pad_;
+    int32_t marking;      /* eq_wall_marking (0: grid rows and columns) */
 } eq_field_desc;
 
 typedef enum {
@@ -131,6 +145,11 @@ typedef struct eq_batch eq_batch;
 typedef struct eq_batch eq_field;
 
 const char* eq_last_error(void);
+/* found[w] = 1 when wall w (left, right, top, bottom) of this mesh is found by
+ * the reference's coordinate test (see eq_wall_marking); left and bottom
+ * always are.  A host can call it before eq_batch_create to learn which walls
+ * a reference run would have dropped. */
+int eq_walls_found(const eq_field_desc* desc, int32_t found[4]);
 /* fills the defaults: mixed GMG-CG, rtol 1e-10, V(2,2), omega 0.8, coarse <= 256 nodes,
  * graphs on */
 void eq_solver_default(eq_solver_desc* s);
@@ -233,6 +252,38 @@ int eq_batch_couple_traps_dev(eq_batch* b, int32_t trap0, int32_t ntraps, const 
 /* totalBoundaryFlux = D*dt * (-int grad(u).n ds)  (src/fHSL.cpp:156-160,
  * fenics/boundary.ufl:9-12) of the current field */
 int eq_batch_boundary_flux(eq_batch* b, int32_t prob, double* total);
+
+/* setBoundaryValues(bval) (src/fHSL.cpp:601-603): every Dirichlet wall of
+ * problem `prob` takes the value `value` from the next solve on (the one
+ * updatingDirchletBoundary that all four DirichletBCs of the DIRICHLET_UPDATE
+ * boundary share, src/fHSL.cpp:545-554, src/Expressions.h:13-44).  Walls with
+ * per-node values or a flow channel keep those.  Stream ordered: one small
+ * kernel on the batch stream, no table upload and no host synchronisation, so
+ * a host may call it between steps or while capturing its own graph. */
+int eq_batch_set_boundary_value(eq_batch* b, int32_t prob, double value);
+
+/* The boundary well of Simulation::computeBoundaryWell with the DIRICHLET_UPDATE
+ * boundary (src/simulation.cpp:581-604, 612-624), run on the device at the end
+ * of every eq_batch_step, after the solve and the channels:
+ *   flux = totalBoundaryFlux of the new field        (src/fHSL.cpp:156-160)
+ *   w   += flux / well_scaling
+ *   w   -= dt * decay_rate * w
+ *   w < 0: w = 0 and one underflow is counted
+ *   every Dirichlet wall takes w for the next solve  (setBoundaryValues)
+ * well_scaling is the reference's wellScaling (2*10*(15/lengthScaling)*(W+H)),
+ * decay_rate its boundaryDecayRate (simulationFlowRate / trap width), w0 the
+ * starting concentration (0 in initBoundaryWell).  The problem's four walls
+ * must be Dirichlet, as in the DIRICHLET_UPDATE set-up.  NULL removes the
+ * well.  Set-up call: synchronises the batch stream. */
+typedef struct {
+    double well_scaling;
+    double decay_rate;
+    double w0;
+} eq_well_desc;
+int eq_batch_set_well(eq_batch* b, int32_t prob, const eq_well_desc* well);
+/* the well's concentration, the last boundary flux it took and the underflow
+ * count (src/simulation.cpp:591-595); any pointer may be NULL.  Synchronises. */
+int eq_batch_well_state(eq_batch* b, int32_t prob, double* w, double* last_flux, int64_t* underflows);
 /* shell->mesh_coords (2N, vertex order) and shell->dof_from_vertex (N)
  * (src/fHSL.h:82-88); vertex order = natural order, dof_from_vertex = identity */
 int eq_batch_vertex_map(const eq_batch* b, int32_t prob, double* coords, int32_t* dof_from_vertex);
@@ -262,13 +313,26 @@ int eq_cells_couple(eq_field* f, const eq_cell* cells, size_t nc, double npm, co
                     double* sampled, int32_t* counts, uint32_t flags);
 
 /* ---------------- measurement ---------------- */
-/* Kernel ids for per-launch HIP-event timing on the batch stream. */
+/* Kernel families for per-launch HIP-event timing on the batch stream.  Every
+ * launch of the library belongs to one. */
 typedef enum {
-    EQ_KERNEL_STENCIL64 = 0,  /* FP64 CG operator apply q = A p          */
-    EQ_KERNEL_SMOOTH = 1,     /* fine-level Jacobi sweep of the V-cycle   */
-    EQ_KERNEL_CG_UPDATE = 2,  /* FP64 x,r update + dot products           */
-    EQ_KERNEL_COUPLE = 3,     /* cell coverage/sample/deposit row pass    */
-    EQ_KERNEL_COUNT_ = 4
+    EQ_KERNEL_STENCIL64 = 0,      /* Krylov operator apply q = A p (CG), v, t = A x^ (BiCGStab) */
+    EQ_KERNEL_SMOOTH = 1,         /* finest-level Jacobi sweeps of the V-cycle               */
+    EQ_KERNEL_CG_UPDATE = 2,      /* x, r update + dot products (CG, BiCGStab)               */
+    EQ_KERNEL_COUPLE = 3,         /* cell sample/deposit per tile (k_tile_couple)            */
+    EQ_KERNEL_SMOOTH_COARSE = 4,  /* Jacobi sweeps on the coarse levels                      */
+    EQ_KERNEL_TRANSFER0 = 5,      /* finest-level residual, restriction from and prolongation
+                                     to it (fused forms included)                          */
+    EQ_KERNEL_TRANSFER_COARSE = 6,/* the same between coarse levels                          */
+    EQ_KERNEL_BOTTOM = 7,         /* the LDS bottom V-cycle (one workgroup per problem)      */
+    EQ_KERNEL_FINISH = 8,         /* per-problem reductions of the tile partials             */
+    EQ_KERNEL_SOLVE_ONCE = 9,     /* once per solve: right-hand side, initial and true
+                                     residuals, first sweeps, copies, the kept solution    */
+    EQ_KERNEL_BICG_VEC = 10,      /* BiCGStab direction and s updates                        */
+    EQ_KERNEL_COUPLE_SETUP = 11,  /* cell geometry, pair scan, pairs, sort, tile bounds,
+                                     per-cell sample sums                                  */
+    EQ_KERNEL_BOUNDARY = 12,      /* channels, boundary wells, boundary values               */
+    EQ_KERNEL_COUNT_ = 13
 } eq_kernel_id;
 /* enable (1) / disable (0) event timing around every launch of `kernel`;
  * timing forces eager launches (no graph replay) while enabled */
@@ -276,6 +340,11 @@ int eq_batch_profile(eq_batch* b, int32_t kernel, int32_t enable);
 /* launches, summed event time (ms) and algorithmic bytes summed over launches */
 int eq_batch_profile_read(eq_batch* b, int32_t kernel, int64_t* launches, double* total_ms,
                           double* bytes);
+/* Algorithmic HBM bytes (each operand array read once and each output written
+ * once per launch, at its element size) and launches issued by the batch since
+ * the last reset, graph replays included: the numerator of a whole-step
+ * roofline fraction.  reset = 1 zeroes the counters after reading them. */
+int eq_batch_traffic(eq_batch* b, double* bytes, int64_t* launches, int32_t reset);
 
 #ifdef __cplusplus
 }

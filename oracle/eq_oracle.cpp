@@ -126,6 +126,18 @@ inline bool on_wall(const orc_problem* p, int w, double x, double y)
     }
 }
 
+/* a vertex on the outer row or column of wall w, by grid index */
+inline bool on_wall_index(const orc_problem* p, int w, int64_t v)
+{
+    const int64_t nw = p->nx + 1, i = v / nw, j = v % nw;
+    switch (w) {
+    case ORC_LEFT: return j == 0;
+    case ORC_RIGHT: return j == p->nx;
+    case ORC_TOP: return i == p->ny;
+    default: return i == 0;
+    }
+}
+
 /* exterior facets of the mesh as (v_a, v_b, wall), each with both endpoints on the wall */
 struct Facet { int64_t a, b; int wall; };
 std::vector<Facet> boundary_facets(const orc_problem* p)
@@ -143,6 +155,7 @@ std::vector<Facet> boundary_facets(const orc_problem* p)
  * (SubDomain::mark over facet vertices [ext]) */
 inline bool facet_on_wall(const orc_problem* p, const Facet& fc, int w)
 {
+    if (p->grid_walls) return on_wall_index(p, w, fc.a) && on_wall_index(p, w, fc.b);
     double xa, ya, xb, yb;
     coords_of(p, fc.a, xa, ya);
     coords_of(p, fc.b, xb, yb);
@@ -163,6 +176,16 @@ inline double facet_length(const orc_problem* p, const Facet& fc)
 extern "C" {
 
 int64_t orc_num_nodes(const orc_problem* p) { return (int64_t)(p->nx + 1) * (p->ny + 1); }
+
+void orc_walls_found(const orc_problem* p, int32_t* found4)
+{
+    /* every facet of a wall shares the wall's computed coordinate, so the
+     * wall is found on all of its facets or on none */
+    const std::vector<Facet> facets = boundary_facets(p);
+    for (int w = 0; w < 4; ++w) found4[w] = 0;
+    for (const Facet& fc : facets)
+        if (facet_on_wall(p, fc, fc.wall)) found4[fc.wall] = 1;
+}
 
 void orc_mesh_coords(const orc_problem* p, double* xy)
 {
@@ -782,6 +805,18 @@ void orc_couple(const orc_grid* g, const orc_cell* cells, int64_t nc, int32_t ns
     }
 }
 
+double orc_boundary_well(double w, double flux, double well_scaling, double decay_rate, double dt,
+                         int64_t* underflows)
+{
+    w += flux / well_scaling;
+    w -= dt * decay_rate * w;
+    if (w < 0.0) {
+        w = 0.0;
+        if (underflows) ++*underflows;
+    }
+    return w;
+}
+
 /* Boundary-flux functional, boundary_exterior_facet_integral_0_otherwise
  * (fenics/boundary.h:2652-2742): for every exterior facet, -grad(u).n*|e|
  * with grad(u) the P1 gradient of the triangle owning the facet. */
@@ -802,11 +837,13 @@ double orc_boundary_flux(const orc_problem* p, const double* u)
                 const double dX = u[t[k].v[1]] - u[t[k].v[0]], dY = u[t[k].v[2]] - u[t[k].v[0]];
                 const double gx = K00 * dX + K10 * dY, gy = K01 * dX + K11 * dY;
                 /* triangle edges (0,1),(1,2),(2,0); an edge is exterior when both
-                 * endpoints lie on the same wall of the rectangle */
+                 * endpoints lie on the same outer row or column of the grid (ds
+                 * without a marker covers every exterior facet, whatever the
+                 * wall tests find) */
                 for (int e = 0; e < 3; ++e) {
                     const int a = e, bq = (e + 1) % 3, o = (e + 2) % 3;
                     for (int w = 0; w < 4; ++w) {
-                        if (!(on_wall(p, w, xc[a], yc[a]) && on_wall(p, w, xc[bq], yc[bq]))) continue;
+                        if (!(on_wall_index(p, w, t[k].v[a]) && on_wall_index(p, w, t[k].v[bq]))) continue;
                         const double ex = xc[bq] - xc[a], ey = yc[bq] - yc[a];
                         const double len = std::sqrt(ex * ex + ey * ey);
                         double nx = ey / len, ny = -ex / len;

@@ -66,6 +66,14 @@ typedef struct {
      * as dt * int (v du/dx) phi (Galerkin) -- a nonsymmetric operator; the LU
      * path solves it, PCG refuses it */
     const double* vx;
+    /* how walls are found: 0 = the reference's test, near() on the computed
+     * vertex coordinates (DirichletBoundary_TrapEdge::inside, src/fHSL.h:575-590,
+     * with DOLFIN_EPS [ext]), under which a wall whose computed coordinate
+     * misses W or H by more than DOLFIN_EPS carries no condition; 1 = by grid
+     * index (the outer rows and columns; the same walls whenever the computed
+     * coordinates are exact, e.g. for every integer trap size) */
+    int32_t grid_walls;
+    int32_t pad_;
 } orc_problem;
 
 /* flow channel above or below the trap (fenicsChannel, src/fHSL.h:240-332) */
@@ -97,6 +105,8 @@ typedef struct {
 
 /* ---- mesh ---- */
 int64_t orc_num_nodes(const orc_problem* p);
+/* found[w] = 1 when wall w carries its condition under p->grid_walls */
+void orc_walls_found(const orc_problem* p, int32_t* found4);
 void orc_mesh_coords(const orc_problem* p, double* xy);
 
 /* ---- assembly ---- */
@@ -178,6 +188,15 @@ int orc_fd_step(const orc_problem* p, double* u);
  * next trap solve's Dirichlet data on those walls. */
 void orc_channel_update(const orc_problem* p, const orc_channel* top_ch, const orc_channel* bottom_ch,
                         const double* u, double* top, double* bottom);
+
+/* Simulation::computeBoundaryWell (src/simulation.cpp:581-604): the well
+ * concentration after one step that produced totalBoundaryFlux = flux:
+ *   w += flux / well_scaling;  w -= dt * decay_rate * w;
+ *   w < 0: w = 0 and *underflows is incremented.
+ * With the DIRICHLET_UPDATE boundary the result is every wall's Dirichlet
+ * value for the next step (setBoundaryValues, src/fHSL.cpp:601-603). */
+double orc_boundary_well(double w, double flux, double well_scaling, double decay_rate, double dt,
+                         int64_t* underflows);
 
 /* boundary-flux functional of stepDiffusion (fenics/boundary.ufl:9-12):
  * -int_{boundary} grad(u).n ds over all walls, P1 gradient per boundary triangle */

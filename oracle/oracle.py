@@ -31,7 +31,8 @@ class Problem(C.Structure):
     _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("width", C.c_double),
                 ("height", C.c_double), ("dt", C.c_double), ("D", C.c_double),
                 ("gamma", C.c_double), ("f", C.c_double), ("wall", Bc * 4),
-                ("gwall", C.c_void_p * 4), ("vx", C.c_void_p)]
+                ("gwall", C.c_void_p * 4), ("vx", C.c_void_p), ("grid_walls", C.c_int32),
+                ("pad_", C.c_int32)]
 
     def set_velocity(self, vx_rows):
         """x-velocity per grid row (ny+1 values), or None."""
@@ -99,6 +100,7 @@ def lib():
         L.orc_num_nodes.restype = C.c_int64
         L.orc_num_nodes.argtypes = [P(Problem)]
         L.orc_mesh_coords.argtypes = [P(Problem), dp]
+        L.orc_walls_found.argtypes = [P(Problem), ip32]
         L.orc_assemble.restype = C.c_int64
         L.orc_assemble.argtypes = [P(Problem), dp, dp, dp, ip64, ip32, dp]
         L.orc_assemble_rhs.argtypes = [P(Problem), dp, dp]
@@ -121,6 +123,8 @@ def lib():
         L.orc_cell_coverage.argtypes = [P(Grid), P(Cell), ip32, C.c_int64]
         L.orc_cell_window.argtypes = [P(Grid), P(Cell), ip32]
         L.orc_couple.argtypes = [P(Grid), P(Cell), C.c_int64, C.c_int32, P(dp), dp, dp, ip32]
+        L.orc_boundary_well.restype = C.c_double
+        L.orc_boundary_well.argtypes = [C.c_double] * 5 + [ip64]
         L.orc_boundary_flux.restype = C.c_double
         L.orc_boundary_flux.argtypes = [P(Problem), dp]
         _lib = L
@@ -131,9 +135,12 @@ def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
 
 
-def problem(nx, ny, width, height, dt, D, gamma=0.0, f=0.0, walls=None) -> Problem:
-    """walls: dict wall-index -> (kind, value, r, s); missing walls are Neumann."""
+def problem(nx, ny, width, height, dt, D, gamma=0.0, f=0.0, walls=None, grid_walls=False) -> Problem:
+    """walls: dict wall-index -> (kind, value, r, s); missing walls are Neumann.
+    grid_walls: find the walls by grid index instead of the reference's
+    near() test on the computed vertex coordinates."""
     p = Problem()
+    p.grid_walls = 1 if grid_walls else 0
     p.nx, p.ny, p.width, p.height = nx, ny, width, height
     p.dt, p.D, p.gamma, p.f = dt, D, gamma, f
     for w in range(4):
@@ -144,6 +151,13 @@ def problem(nx, ny, width, height, dt, D, gamma=0.0, f=0.0, walls=None) -> Probl
 
 def num_nodes(p: Problem) -> int:
     return (p.nx + 1) * (p.ny + 1)
+
+
+def walls_found(p: Problem) -> list:
+    """Per wall (left, right, top, bottom): does the wall carry its condition."""
+    f = np.empty(4, np.int32)
+    lib().orc_walls_found(C.byref(p), f.ctypes.data_as(C.POINTER(C.c_int32)))
+    return [bool(v) for v in f]
 
 
 def mesh_coords(p: Problem) -> np.ndarray:
@@ -297,3 +311,10 @@ def couple(g: Grid, cells: np.ndarray, fields, q: np.ndarray):
 
 def boundary_flux(p: Problem, u) -> float:
     return lib().orc_boundary_flux(C.byref(p), _dp(np.ascontiguousarray(u, dtype=np.float64)))
+
+
+def boundary_well(w, flux, well_scaling, decay_rate, dt):
+    """One computeBoundaryWell update; returns (w_new, underflowed)."""
+    n = C.c_int64(0)
+    w2 = lib().orc_boundary_well(w, flux, well_scaling, decay_rate, dt, C.byref(n))
+    return w2, n.value

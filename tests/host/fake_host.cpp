@@ -17,8 +17,20 @@
 //   vectors before the first step (src/simulation.cpp:311-313).
 // mode 3: mode 0 on eqgpu::diffusionPETSc objects (the finite-difference
 //   solver, one per species) with the given walls.
+// mode 4: mode 0 with the DIRICHLET_UPDATE boundary well computed by the host
+//   after every step, as Simulation::computeBoundaryWell does
+//   (src/simulation.cpp:581-604), and sent with setBoundaryValues; input adds
+//   double wellScaling, decayRate, w0.  With a files dir, a data-recording
+//   fenicsInterface (src/fHSL.cpp:27-33) also writes <dir>/data_s.pvd (scalar:
+//   species 0's field) and <dir>/data_v.pvd (vector: species 0 and 1) per step.
+// mode 5: mode 1 with the same well run on the device (trapField::setBoundaryWell).
+// mode 6: mode 0 with flow channels above and below the trap: input adds two
+//   eq_channel_desc (top, bottom); with a files dir writeDiffusionFiles also
+//   writes <dir>/top<s>.pvd and <dir>/bottom<s>.pvd.
 // output: double fields[ns*N], double sampled[nc*ns] (mode 1, last step),
-//   double flux[ns] (modes 0-2), int32 iterations_max.
+//   double flux[ns] (modes 0-2, 4-6), int32 iterations_max; then modes 4 and 5:
+//   double w[ns] (the well after the last step); mode 6: double top[ns][nx+1],
+//   bottom[ns][nx+1].
 // With a files dir (modes 0, 2, 3) every step also writes the solvers' files
 // (writeDiffusionFiles: <dir>/hsl<s>.pvd + .vtu, or <dir>/fd<s>_petsc/grid%04d.vtr)
 // and a frame of the controller's JSON log (<dir>/test-0_0-jsonData.json)
@@ -70,6 +82,10 @@ int main(int argc, char** argv)
             tens.assign((size_t)ns * 3, std::vector<double>(N));
             for (auto& t : tens) rd(in, t.data(), N);
         }
+        double well[3] = {0.0, 0.0, 0.0};
+        if (mode == 4 || mode == 5) rd(in, well, 3);
+        eq_channel_desc chans[2];
+        if (mode == 6) rd(in, chans, 2);
         std::fclose(in);
 
         eqgpu::solverParams p;
@@ -128,7 +144,12 @@ int main(int argc, char** argv)
         for (int s = 0; s < ns; ++s) {
             species.emplace_back(field, s);
             eqgpu::solverParams ps = p;
-            if (!files.empty() && mode != 1) ps.filePath = files + "/hsl" + std::to_string(s) + ".pvd";
+            if (!files.empty() && mode != 1 && mode != 5) ps.filePath = files + "/hsl" + std::to_string(s) + ".pvd";
+            if (!files.empty() && mode == 6) {
+                ps.filePathTopChannel = files + "/top" + std::to_string(s) + ".pvd";
+                ps.filePathBottomChannel = files + "/bottom" + std::to_string(s) + ".pvd";
+            }
+            if (mode == 6) field->setChannels(s, &chans[0], &chans[1]);
             species.back().initDiffusion(ps);
             if (species.back().shell->mesh->num_vertices() != N) throw std::runtime_error("num_vertices");
             species.back().solution_vector = u0[s];
@@ -139,10 +160,40 @@ int main(int argc, char** argv)
             }
         }
         std::vector<double> sampled;
-        if (mode == 1)
+        if (mode == 1 || mode == 5)
             for (int s = 0; s < ns; ++s) field->upload(s, u0[s]);
+        // the host's boundary well (Simulation::initBoundaryWell / computeBoundaryWell)
+        std::vector<double> wellConc(ns, well[2]);
+        if (mode == 4)
+            for (int s = 0; s < ns; ++s) species[s].setBoundaryValues(wellConc[s]);
+        if (mode == 5)
+            for (int s = 0; s < ns; ++s) field->setBoundaryWell(s, well[0], well[1], well[2]);
+        // data recording on the controller (src/simulation.cpp:557-571)
+        std::unique_ptr<eqgpu::fenicsInterface> recorder;
+        if (mode == 4 && !files.empty()) {
+            eqgpu::solverParams pd = p;
+            pd.filePath = files + "/";
+            eqgpu::dataRecord sc{"data_s.pvd", &species[0].solution_vector, nullptr, (size_t)nx + 1, (size_t)ny + 1,
+                                 p.nodesPerMicron};
+            eqgpu::dataRecord vc{"data_v.pvd", &species[0].solution_vector, &species[ns > 1 ? 1 : 0].solution_vector,
+                                 (size_t)nx + 1, (size_t)ny + 1, p.nodesPerMicron};
+            pd.dataFiles = {sc, vc};
+            recorder = std::make_unique<eqgpu::fenicsInterface>(pd);
+        }
         for (int k = 0; k < nsteps; ++k) {
-            if (mode == 0 || mode == 2) {
+            if (mode == 4) {
+                for (int c = 0; c < nc; ++c)
+                    for (int s = 0; s < ns; ++s) species[s].solution_vector[(size_t)c % N] += q[(size_t)c * ns + s];
+                for (int s = 0; s < ns; ++s) {
+                    species[s].stepDiffusion();
+                    const double flux = species[s].getBoundaryFlux().at("totalFlux");
+                    wellConc[s] += flux / well[0];
+                    wellConc[s] -= p.dt * well[1] * wellConc[s];
+                    if (wellConc[s] < 0.0) wellConc[s] = 0.0;
+                    species[s].setBoundaryValues(wellConc[s]);
+                }
+                if (recorder) recorder->writeDataFiles((k + 1) * p.dt);
+            } else if (mode == 0 || mode == 2 || mode == 6) {
                 // controller: updateCells writes into its copy of each field,
                 // then the copies travel to the solvers (src/simulation.cpp:447-463)
                 for (int c = 0; c < nc; ++c)
@@ -165,7 +216,7 @@ int main(int argc, char** argv)
         std::vector<double> out;
         for (int s = 0; s < ns; ++s) {
             std::vector<double> u;
-            if (mode == 1) field->download(s, u);
+            if (mode == 1 || mode == 5) field->download(s, u);
             else u = species[s].solution_vector;
             out.insert(out.end(), u.begin(), u.end());
         }
@@ -180,6 +231,14 @@ int main(int argc, char** argv)
         }
         const int32_t it = field->lastStats().iterations_max;
         std::fwrite(&it, sizeof(int32_t), 1, fo);
+        if (mode == 5)
+            for (int s = 0; s < ns; ++s) wellConc[s] = field->boundaryWellConcentration(s);
+        if (mode == 4 || mode == 5) std::fwrite(wellConc.data(), sizeof(double), ns, fo);
+        if (mode == 6) {
+            std::vector<std::vector<double>> tb(2 * ns);
+            for (int s = 0; s < ns; ++s) field->channelValues(s, tb[s], tb[ns + s]);
+            for (auto& v : tb) std::fwrite(v.data(), sizeof(double), v.size(), fo);
+        }
         std::fclose(fo);
     } catch (const std::exception& e) {
         std::fprintf(stderr, "fake_host: %s\n", e.what());

@@ -106,3 +106,40 @@ def test_workload_generators_are_deterministic():
     assert np.array_equal(c1, c2)
     d = (c1[:, :2] - a.cells[:, :2]).ravel()
     assert abs(d.std() - 0.05) < 0.01 and np.abs(d).max() <= 0.3 + 1e-12
+
+
+def test_walls_found_matches_the_oracle_on_config3_traps():
+    """The library's restatement of the reference's coordinate wall test
+    agrees with the oracle's on all 512 config-3 traps: 41 lose the top wall
+    (the computed y of the top row misses H by more than DOLFIN_EPS), 39 the
+    right wall.  Integer trap sizes lose none, because (n*W)/n is then exact.
+    Host logic only: no device is asked for."""
+    from eq_amd import field as F
+    from eq_amd import workloads as W
+    from oracle import oracle as O
+    lost_top = lost_right = 0
+    for t in W.config3_traps(512, 3):
+        d = F.field_desc(t.nx, t.ny, t.width, t.height, 0.1, 1.0)
+        found = F.walls_found(d)
+        assert found == O.walls_found(O.problem(t.nx, t.ny, t.width, t.height, 0.1, 1.0))
+        assert found[F.LEFT] and found[F.BOTTOM]
+        lost_top += not found[F.TOP]
+        lost_right += not found[F.RIGHT]
+    assert (lost_top, lost_right) == (41, 39)
+    for nx, ny, W_, H in [(160, 80, 40.0, 20.0), (2000, 1000, 100.0, 50.0), (8192, 4096, 204.8, 102.4),
+                          (1000, 500, 100.0, 50.0), (37, 23, 37.0, 23.0)]:
+        assert all(F.walls_found(F.field_desc(nx, ny, W_, H, 0.1, 1.0)))
+    # the grid rule finds every wall in the oracle, whatever the coordinates
+    assert all(O.walls_found(O.problem(218, 440, 21.8, 44.0, 0.1, 1.0, grid_walls=True)))
+
+
+def test_bad_wall_marking_is_rejected():
+    from eq_amd import field as F
+    d = F.field_desc(8, 4, 2.0, 1.0, 0.1, 1.0)
+    d.marking = 3
+    with pytest.raises(F.EqError) as e:
+        F.walls_found(d)
+    assert e.value.code == 1
+    with pytest.raises(F.EqError) as e:
+        F.FieldBatch([d])
+    assert e.value.code == 1

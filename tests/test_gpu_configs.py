@@ -1,0 +1,316 @@
+"""GPU parity at the workload configurations' own scale, through the C ABI.
+
+  config 3 (512 uneven traps, h = 0.1 um, 2 species): a subset of the real
+    generated traps -- the largest, the smallest, traps with odd cell counts
+    in both directions, and traps whose top or right wall the reference's
+    coordinate test misses -- coupled in one batched pass and stepped twice;
+    every coupled field bit-identical to the oracle's sequential loop, every
+    solve within 1e-9 relative L2 of a direct LU (SuperLU) of the oracle's
+    assembled, Dirichlet-row-replaced matrix.  The same traps under the
+    reference's coordinate wall test.  The whole 512-trap batch coupled in
+    one pass (787 402 rods), bit-identical per trap.
+  config 4 (8192 x 4096, parabolic velocity): one species at full size after
+    coupling its 100 000 rods at npm 40 (bit-identical to the oracle's loop);
+    FP64 and mixed GMG-BiCGStab agree to 1e-9, and each residual recomputed
+    in FP64 with the oracle's CSR is <= 1.2e-10.
+  config 1 (2000 x 1000, 5000 random-walk rods): three coupled steps, each
+    coupling bit-identical to the oracle's loop on the same input fields.
+
+Tolerances as in test_gpu_parity.py: coverage counts exact, deposits
+bit-identical, sampled means 2e-13 relative, solves 1e-9 relative L2.
+"""
+import os
+import time
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+# config-3 traps (index into config3_traps(512, 3)) chosen for their shapes
+C3_LARGEST, C3_SMALLEST = 75, 475
+C3_ODD = (130, 408)           # odd nx and odd ny
+C3_TOP_MISSED = (11, 14)      # (ny*H)/ny != H: the reference finds no top wall
+C3_RIGHT_MISSED = (10,)       # (nx*W)/nx != W: no right wall
+C3_SUBSET = (C3_LARGEST, C3_SMALLEST) + C3_ODD + C3_TOP_MISSED + C3_RIGHT_MISSED + (0,)
+
+
+def _workers():
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+class LuStep:
+    """The reference step with a direct solve: the oracle assembles A and b
+    (hslD element tensors, Robin, reaction) and replaces the Dirichlet rows;
+    SuperLU factors A once (the operator does not change between steps)."""
+
+    def __init__(self, oracle, p):
+        import scipy.sparse.linalg as sla
+        self.O, self.p = oracle, p
+        n = oracle.num_nodes(p)
+        self.rp, self.c, self.v = oracle.assemble(p)
+        v2, _ = oracle.apply_dirichlet(p, self.rp, self.c, self.v, np.zeros(n))
+        self.lu = sla.splu(oracle.to_scipy(self.rp, self.c, v2, n).tocsc(), permc_spec="COLAMD")
+
+    def __call__(self, u):
+        b = self.O.assemble_rhs(self.p, u)
+        _, b2 = self.O.apply_dirichlet(self.p, self.rp, self.c, self.v, b)
+        return self.lu.solve(b2)
+
+
+def _c3_setup(eqf, oracle, W, ids, marking):
+    traps = W.config3_traps(512, 3)
+    sel = [traps[k] for k in ids]
+    descs, probs = [], []
+    for t in sel:
+        for D, g in zip(t.D, t.gamma):
+            descs.append(eqf.field_desc(t.nx, t.ny, t.width, t.height, 0.1, D, g, 0.0, W.TOP_DIRICHLET,
+                                        marking=marking))
+            probs.append(oracle.problem(t.nx, t.ny, t.width, t.height, 0.1, D, g, 0.0, walls=W.TOP_DIRICHLET,
+                                        grid_walls=(marking == eqf.WALLS_GRID)))
+    return sel, descs, probs
+
+
+@pytest.mark.parametrize("marking", ["grid", "coord"])
+def test_config3_subset_two_coupled_steps(eqf, oracle, marking):
+    import torch
+    from eq_amd import workloads as W
+    mk = eqf.WALLS_GRID if marking == "grid" else eqf.WALLS_COORD
+    ids = C3_SUBSET if marking == "grid" else (C3_SMALLEST,) + C3_TOP_MISSED + C3_RIGHT_MISSED
+    sel, descs, probs = _c3_setup(eqf, oracle, W, ids, mk)
+    # the chosen traps do lose the walls they are chosen for under the coordinate test
+    for k, t in zip(ids, sel):
+        found = eqf.walls_found(descs[2 * ids.index(k)])
+        assert found == oracle.walls_found(oracle.problem(t.nx, t.ny, t.width, t.height, 0.1, 1.0,
+                                                          walls=W.TOP_DIRICHLET))
+        assert found[eqf.TOP] == (k not in C3_TOP_MISSED) and found[eqf.RIGHT] == (k not in C3_RIGHT_MISSED), k
+    t0 = time.time()
+    with ThreadPoolExecutor(_workers()) as ex:
+        steppers = list(ex.map(lambda p: LuStep(oracle, p), probs))
+    print(f"config 3 subset ({marking}): {len(probs)} LU factorisations in {time.time() - t0:.1f} s", flush=True)
+    grids = [oracle.grid(t.nx + 1, t.ny + 1, t.npm, t.width, t.height) for t in sel]
+    off = np.concatenate([[0], np.cumsum([t.cells.shape[0] for t in sel])]).astype(np.int64)
+    wlim = np.concatenate([np.full(t.cells.shape[0], t.width) for t in sel])
+    hlim = np.concatenate([np.full(t.cells.shape[0], t.height) for t in sel])
+    q = np.concatenate([t.rates for t in sel]) * 0.1
+    cells = np.concatenate([t.cells for t in sel])
+    dev = torch.device("cuda")
+    q_d = torch.from_numpy(q).to(dev).contiguous()
+    samp_d = torch.zeros((int(off[-1]), 2), dtype=torch.float64, device=dev)
+    cnt_d = torch.zeros(int(off[-1]), dtype=torch.int32, device=dev)
+    ref = [np.zeros((t.nx + 1) * (t.ny + 1)) for t in sel for _ in range(2)]
+    with eqf.FieldBatch(descs, nspecies=2, solver=eqf.solver_desc(eqf.MIXED_GMG_CG)) as fb:
+        for step in range(2):
+            print(f"config 3 subset ({marking}) step {step}", flush=True)
+            cells = W.random_walk_step(cells, 3, step, 0.05, wlim, hlim)
+            cells_d = torch.from_numpy(cells).to(dev).contiguous()
+            for k in range(len(descs)):
+                fb.upload(k, ref[k])
+            fb.couple_traps_dev(0, off, cells_d.data_ptr(), 10.0, q_d.data_ptr(), samp_d.data_ptr(),
+                                cnt_d.data_ptr())
+            fb.sync()
+            coupled = [fb.download(k) for k in range(len(descs))]
+            cnt, samp = cnt_d.cpu().numpy(), samp_d.cpu().numpy()
+            for t in range(len(sel)):
+                a, b = off[t], off[t + 1]
+                fields, s_ref, c_ref = oracle.couple(grids[t], cells[a:b], ref[2 * t:2 * t + 2], q[a:b])
+                assert np.array_equal(cnt[a:b], c_ref), (ids[t], step)
+                assert np.allclose(samp[a:b], s_ref, rtol=2e-13, atol=1e-300), (ids[t], step)
+                for s in range(2):
+                    assert np.array_equal(coupled[2 * t + s], fields[s]), (ids[t], s, step)
+                    ref[2 * t + s] = fields[s]
+            st = fb.step()
+            assert st.nconverged == len(descs) and st.relres_max <= 1e-10
+            with ThreadPoolExecutor(_workers()) as ex:
+                ref = list(ex.map(lambda kv: kv[0](kv[1]), zip(steppers, ref)))
+            worst = 0.0
+            for k in range(len(descs)):
+                e = _rel(fb.download(k), ref[k])
+                worst = max(worst, e)
+                assert e <= 1e-9, (ids[k // 2], k % 2, step, e)
+            print(f"config 3 subset ({marking}) step {step}: iterations {st.iterations_max}, "
+                  f"worst rel L2 vs LU {worst:.2e}", flush=True)
+
+
+def test_config3_whole_batch_coupling_bit_exact(eqf, oracle):
+    """All 512 traps' 787 402 rods coupled in one pass; every trap's two
+    fields bit-identical to the oracle's sequential loop, counts exact."""
+    import torch
+    from eq_amd import workloads as W
+    traps = W.config3_traps(512, 3)
+    descs = [eqf.field_desc(t.nx, t.ny, t.width, t.height, 0.1, D, g, 0.0, W.TOP_DIRICHLET)
+             for t in traps for D, g in zip(t.D, t.gamma)]
+    off = np.concatenate([[0], np.cumsum([t.cells.shape[0] for t in traps])]).astype(np.int64)
+    assert int(off[-1]) == 787402
+    cells = np.concatenate([t.cells for t in traps])
+    q = np.concatenate([t.rates for t in traps]) * 0.1
+    rng = np.random.default_rng(3)
+    u0 = [rng.random(t.nodes) for t in traps for _ in range(2)]
+    dev = torch.device("cuda")
+    cells_d = torch.from_numpy(cells).to(dev).contiguous()
+    q_d = torch.from_numpy(q).to(dev).contiguous()
+    samp_d = torch.zeros((int(off[-1]), 2), dtype=torch.float64, device=dev)
+    cnt_d = torch.zeros(int(off[-1]), dtype=torch.int32, device=dev)
+    with eqf.FieldBatch(descs, nspecies=2, solver=eqf.solver_desc(eqf.JACOBI_CG)) as fb:
+        for k, u in enumerate(u0):
+            fb.upload(k, u)
+        fb.couple_traps_dev(0, off, cells_d.data_ptr(), 10.0, q_d.data_ptr(), samp_d.data_ptr(), cnt_d.data_ptr())
+        fb.sync()
+        got = [fb.download(k) for k in range(len(descs))]
+    cnt, samp = cnt_d.cpu().numpy(), samp_d.cpu().numpy()
+
+    def check(t):
+        tr = traps[t]
+        g = oracle.grid(tr.nx + 1, tr.ny + 1, tr.npm, tr.width, tr.height)
+        a, b = off[t], off[t + 1]
+        fields, s_ref, c_ref = oracle.couple(g, cells[a:b], u0[2 * t:2 * t + 2], q[a:b])
+        ok = (np.array_equal(cnt[a:b], c_ref) and np.allclose(samp[a:b], s_ref, rtol=2e-13, atol=1e-300)
+              and all(np.array_equal(got[2 * t + s], fields[s]) for s in range(2)))
+        return t, ok
+
+    with ThreadPoolExecutor(_workers()) as ex:
+        bad = [t for t, ok in ex.map(check, range(len(traps))) if not ok]
+    assert not bad, bad
+
+
+def test_config4_full_size_species(eqf, oracle):
+    """8192x4096 trap, the first species (D = 400) with the parabolic channel
+    velocity: couple the 100 000 rods at npm 40 into a zero field (bit-exact
+    against the oracle), then one step with FP64 and with mixed
+    GMG-BiCGStab."""
+    import torch
+    from eq_amd import workloads as W
+    t = W.config4()
+    vx = W.channel_velocity(t.ny, t.height)
+    D, g = t.D[0], t.gamma[0]
+    n = t.nodes
+    p = oracle.problem(t.nx, t.ny, t.width, t.height, 0.1, D, g, 0.0, walls=W.TOP_DIRICHLET)
+    p.set_velocity(vx)
+    grid = oracle.grid(t.nx + 1, t.ny + 1, t.npm, t.width, t.height)
+    q = (t.rates[:, :1] * 0.1).copy()
+    print("config 4: oracle coupling of 100000 rods", flush=True)
+    fields, s_ref, c_ref = oracle.couple(grid, t.cells, [np.zeros(n)], q)
+    u0 = fields[0]
+    print("config 4: oracle assembly", flush=True)
+    rp, c, v = oracle.assemble(p)
+    b = oracle.assemble_rhs(p, u0)
+    v2, b2 = oracle.apply_dirichlet(p, rp, c, v, b)
+    del v
+    dev = torch.device("cuda")
+    cells_d = torch.from_numpy(t.cells).to(dev).contiguous()
+    q_d = torch.from_numpy(q).to(dev).contiguous()
+    samp_d = torch.zeros((t.cells.shape[0], 1), dtype=torch.float64, device=dev)
+    cnt_d = torch.zeros(t.cells.shape[0], dtype=torch.int32, device=dev)
+    sols = {}
+    for name, kind in (("gmg_bicgstab", eqf.GMG_BICGSTAB), ("mixed_bicgstab", eqf.MIXED_GMG_BICGSTAB)):
+        print(f"config 4: {name}", flush=True)
+        d = eqf.field_desc(t.nx, t.ny, t.width, t.height, 0.1, D, g, 0.0, W.TOP_DIRICHLET)
+        with eqf.FieldBatch([d], nspecies=1, solver=eqf.solver_desc(kind)) as fb:
+            fb.set_velocity(0, vx)
+            fb.couple_dev(0, cells_d.data_ptr(), t.cells.shape[0], t.npm, q_d.data_ptr(), samp_d.data_ptr(),
+                          cnt_d.data_ptr())
+            fb.sync()
+            assert np.array_equal(fb.download(0), u0)
+            assert np.array_equal(cnt_d.cpu().numpy(), c_ref)
+            assert np.allclose(samp_d.cpu().numpy(), s_ref, rtol=2e-13, atol=1e-300)
+            st = fb.step()
+            x = fb.download(0)
+        res = np.linalg.norm(b2 - oracle.spmv(rp, c, v2, x)) / np.linalg.norm(b2)
+        print(f"config 4: {name}: iterations={st.iterations_max} relres(gpu)={st.relres_max:.3e} "
+              f"relres(csr)={res:.3e}", flush=True)
+        assert st.nconverged == 1 and st.relres_max <= 1e-10
+        assert res <= 1.2e-10
+        sols[name] = x
+    assert _rel(sols["mixed_bicgstab"], sols["gmg_bicgstab"]) <= 1e-9
+
+
+def test_config1_walk_coupling_bit_exact(eqf, oracle):
+    """The 2000x1000 trap with its 5000 random-walk rods: three steps of
+    couple + mixed solve; each coupling is bit-identical to the oracle's
+    sequential loop applied to the same input fields."""
+    import torch
+    from eq_amd import workloads as W
+    t = W.config1()
+    grid = oracle.grid(t.nx + 1, t.ny + 1, t.npm, t.width, t.height)
+    descs = [eqf.field_desc(t.nx, t.ny, t.width, t.height, 0.1, D, g, 0.0, W.TOP_DIRICHLET)
+             for D, g in zip(t.D, t.gamma)]
+    q = t.rates * 0.1
+    dev = torch.device("cuda")
+    q_d = torch.from_numpy(q).to(dev).contiguous()
+    samp_d = torch.zeros((t.cells.shape[0], 2), dtype=torch.float64, device=dev)
+    cnt_d = torch.zeros(t.cells.shape[0], dtype=torch.int32, device=dev)
+    cells = t.cells
+    with eqf.FieldBatch(descs, nspecies=2, solver=eqf.solver_desc(eqf.MIXED_GMG_CG)) as fb:
+        for step in range(3):
+            cells = W.random_walk_step(cells, 0, step, 0.05, t.width, t.height)
+            cells_d = torch.from_numpy(cells).to(dev).contiguous()
+            before = [fb.download(s) for s in range(2)]
+            fb.couple_dev(0, cells_d.data_ptr(), cells.shape[0], t.npm, q_d.data_ptr(), samp_d.data_ptr(),
+                          cnt_d.data_ptr())
+            fb.sync()
+            after = [fb.download(s) for s in range(2)]
+            fields, s_ref, c_ref = oracle.couple(grid, cells, before, q)
+            assert np.array_equal(cnt_d.cpu().numpy(), c_ref), step
+            assert np.allclose(samp_d.cpu().numpy(), s_ref, rtol=2e-13, atol=1e-300), step
+            for s in range(2):
+                assert np.array_equal(after[s], fields[s]), (step, s)
+            st = fb.step()
+            assert st.nconverged == 2 and st.relres_max <= 1e-10
+            print(f"config 1 walk step {step}: coupling bit-exact, iterations {st.iterations_max}", flush=True)
+
+
+def test_bicgstab_reports_non_convergence(eqf, oracle):
+    """A BiCGStab solve stopped by max_iter above rtol returns
+    EQ_ERR_NOT_CONVERGED (as the CG kinds do); the batch stays usable."""
+    walls = {oracle.TOP: (oracle.DIRICHLET, 0.0, 0, 0)}
+    nx, ny, W_, H = 120, 60, 6.0, 3.0
+    d = eqf.field_desc(nx, ny, W_, H, 0.1, 400.0, 0.01, 0.0, walls)
+    p = oracle.problem(nx, ny, W_, H, 0.1, 400.0, 0.01, 0.0, walls=walls)
+    vx = 4.0 * 30.0 * np.arange(ny + 1) * (H / ny) * (H - np.arange(ny + 1) * (H / ny)) / (H * H)
+    p.set_velocity(vx)
+    u0 = np.random.default_rng(5).random((nx + 1) * (ny + 1))
+    for kind in (eqf.GMG_BICGSTAB, eqf.MIXED_GMG_BICGSTAB):
+        with eqf.FieldBatch([d], solver=eqf.solver_desc(kind, max_iter=1, check_every=1)) as fb:
+            fb.set_velocity(0, vx)
+            fb.upload(0, u0)
+            with pytest.raises(eqf.EqError) as e:
+                fb.step()
+            assert e.value.code == 4
+            st = fb.step(check=False)
+            assert st.nconverged == 0
+    with eqf.FieldBatch([d], solver=eqf.solver_desc(eqf.MIXED_GMG_CG, max_iter=1, check_every=1)) as fb:
+        fb.upload(0, u0)
+        with pytest.raises(eqf.EqError) as e:
+            fb.step()
+        assert e.value.code == 4
+
+
+def test_tensor_with_velocity_rejected_before_any_change(eqf, oracle):
+    """A tensor field on a problem that carries a velocity is refused before
+    anything is uploaded: the next steps still match the oracle's advection
+    step without a tensor."""
+    walls = {oracle.TOP: (oracle.DIRICHLET, 0.0, 0, 0)}
+    nx, ny, W_, H = 80, 40, 8.0, 4.0
+    d = eqf.field_desc(nx, ny, W_, H, 0.1, 100.0, 0.01, 0.0, walls)
+    p = oracle.problem(nx, ny, W_, H, 0.1, 100.0, 0.01, 0.0, walls=walls)
+    y = np.arange(ny + 1) * (H / ny)
+    vx = 4.0 * 40.0 * y * (H - y) / (H * H)
+    p.set_velocity(vx)
+    n = (nx + 1) * (ny + 1)
+    u = np.random.default_rng(9).random(n)
+    with eqf.FieldBatch([d], solver=eqf.solver_desc(eqf.MIXED_GMG_BICGSTAB)) as fb:
+        fb.set_velocity(0, vx)
+        fb.upload(0, u)
+        with pytest.raises(eqf.EqError) as e:
+            fb.set_tensor(0, np.full(n, 2.0), np.full(n, 0.5), np.full(n, 0.1))
+        assert e.value.code == 1
+        for _ in range(2):
+            st = fb.step()
+            assert st.nconverged == 1
+            u, _ = oracle.step(p, u, "lu")
+            assert _rel(fb.download(0), u) <= 1e-9

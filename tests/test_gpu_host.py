@@ -176,3 +176,126 @@ def test_fake_host_writes_files(tmp_path):
         hsl = np.array(doc["frames"][-1]["externalHSL"], np.float64)
         assert hsl.tobytes() == fields.tobytes()
         assert len(doc["frames"][0]["cells"]) == t.cells.shape[0]
+
+
+def _append(path, *arrays):
+    with open(path, "ab") as f:
+        for a in arrays:
+            f.write(np.ascontiguousarray(a).tobytes())
+
+
+def _tail(path, ns, N, nc):
+    """what follows the common output: after fields, sampled, flux and iterations"""
+    raw = open(path, "rb").read()
+    return raw[(ns * N + nc * ns + ns) * 8 + 4:]
+
+
+def _small_trap(W):
+    t = W.config0()
+    t.u0 = [5.0 * np.random.default_rng(s + 2).random(t.nodes) for s in range(2)]
+    return t
+
+
+@pytest.mark.parametrize("mode", [4, 5])
+def test_fake_host_boundary_well(oracle, tmp_path, mode):
+    """DIRICHLET_UPDATE: mode 4 runs Simulation::computeBoundaryWell on the host
+    and sends setBoundaryValues through the mirror; mode 5 couples on the
+    device and runs the same well there (trapField::setBoundaryWell).  Fields
+    and well values against the oracle's LU steps and boundary-well update.
+    Mode 4 also records the data-recording grid (writeDataFiles): its scalar
+    and vector files read back bit-exact."""
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    import vtk_read
+    from eq_amd import workloads as W
+    t = _small_trap(W)
+    walls = {w: (1, 0.0, 0.0, 0.0) for w in range(4)}
+    q = t.rates * 0.1
+    nsteps, dt = 3, 0.1
+    scaling, rate, w0 = 40.0, 0.5, 0.2
+    inp, outp = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    N = _write_input(inp, mode, t, nsteps, q, walls)
+    _append(inp, np.array([scaling, rate, w0]))
+    print(f"fake_host mode={mode} nsteps={nsteps}", flush=True)
+    subprocess.run([EXE, inp, outp, str(tmp_path)], check=True, timeout=120)
+    fields, _, _ = _read_output(outp, 2, N, t.cells.shape[0])
+    wgot = np.frombuffer(_tail(outp, 2, N, t.cells.shape[0]), np.float64, 2)
+    grid = oracle.grid(t.nx + 1, t.ny + 1, t.npm, t.width, t.height)
+    ref = [u.copy() for u in t.u0]
+    w = [w0, w0]
+    for _ in range(nsteps):
+        if mode == 4:
+            for c in range(t.cells.shape[0]):
+                for s in range(2):
+                    ref[s][c % N] += q[c, s]
+        else:
+            ref, _, _ = oracle.couple(grid, t.cells, ref, q)
+        for s in range(2):
+            p = oracle.problem(t.nx, t.ny, t.width, t.height, dt, t.D[s], t.gamma[s], 0.0,
+                               walls={k: (1, w[s], 0.0, 0.0) for k in range(4)})
+            ref[s], _ = oracle.step(p, ref[s], "lu")
+            w[s], _ = oracle.boundary_well(w[s], dt * t.D[s] * oracle.boundary_flux(p, ref[s]), scaling, rate, dt)
+    for s in range(2):
+        err = np.linalg.norm(fields[s] - ref[s]) / np.linalg.norm(ref[s])
+        assert err <= 1e-9, (mode, s, err)
+        assert abs(wgot[s] - w[s]) <= 1e-8 * abs(w[s]), (mode, s, wgot[s], w[s])
+    if mode == 4:
+        series = vtk_read.read_pvd(tmp_path / "data_s.pvd")
+        assert [ts for ts, _ in series] == [(k + 1) * dt for k in range(nsteps)]
+        last = vtk_read.read(tmp_path / series[-1][1])
+        assert last["arrays"]["u"].tobytes() == fields[0].tobytes()
+        vec = vtk_read.read(tmp_path / vtk_read.read_pvd(tmp_path / "data_v.pvd")[-1][1])["arrays"]["v"]
+        assert vec.shape == (N, 3)
+        assert vec[:, 0].tobytes() == fields[0].tobytes() and vec[:, 1].tobytes() == fields[1].tobytes()
+        assert not vec[:, 2].any()
+
+
+def test_fake_host_channel_files(oracle, tmp_path):
+    """Mode 6: flow channels above and below the trap through the mirror; the
+    trap fields and the channel values against the oracle's channel loop, and
+    the channel file series (writeDiffusionFiles, src/fHSL.cpp:613-634) read
+    back bit-exact against the values the mirror returns."""
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    import vtk_read
+    from eq_amd import field as F
+    from eq_amd import workloads as W
+    t = _small_trap(W)
+    walls = {2: (1, 0.0, 0.0, 0.0), 3: (1, 0.0, 0.0, 0.0)}
+    q = t.rates * 0.1
+    nsteps, dt = 3, 0.1
+    top = F.channel_desc(30.0, 1.0, 0.0, 2.0, 0.0, 12.0, 2)
+    bot = F.channel_desc(-10.0, 0.5, 0.1, 1.0, 0.0, 8.0, 3)
+    inp, outp = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    N = _write_input(inp, 6, t, nsteps, q, walls)
+    with open(inp, "ab") as f:
+        f.write(bytes(top))
+        f.write(bytes(bot))
+    print(f"fake_host mode=6 nsteps={nsteps}", flush=True)
+    subprocess.run([EXE, inp, outp, str(tmp_path)], check=True, timeout=120)
+    fields, _, _ = _read_output(outp, 2, N, t.cells.shape[0])
+    tb = np.frombuffer(_tail(outp, 2, N, t.cells.shape[0]), np.float64, 4 * (t.nx + 1)).reshape(4, t.nx + 1)
+    och = [oracle.channel(c.v, c.r_left, c.s_left, c.r_right, c.s_right, c.well_scaling, c.substeps) for c in (top, bot)]
+    ref = [u.copy() for u in t.u0]
+    ct = [np.zeros(t.nx + 1), np.zeros(t.nx + 1)]
+    cb = [np.zeros(t.nx + 1), np.zeros(t.nx + 1)]
+    for _ in range(nsteps):
+        for c in range(t.cells.shape[0]):
+            for s in range(2):
+                ref[s][c % N] += q[c, s]
+        for s in range(2):
+            p = oracle.problem(t.nx, t.ny, t.width, t.height, dt, t.D[s], t.gamma[s], 0.0, walls=walls)
+            p.set_wall_values(2, ct[s])
+            p.set_wall_values(3, cb[s])
+            ref[s], _ = oracle.step(p, ref[s], "lu")
+            ct[s], cb[s] = oracle.channel_update(p, och[0], och[1], ref[s], ct[s], cb[s])
+    for s in range(2):
+        assert np.linalg.norm(fields[s] - ref[s]) <= 1e-9 * np.linalg.norm(ref[s]), s
+        assert np.linalg.norm(tb[s] - ct[s]) <= 1e-9 * np.linalg.norm(ct[s]), s
+        assert np.linalg.norm(tb[2 + s] - cb[s]) <= 1e-9 * np.linalg.norm(cb[s]), s
+        for name, vals in (("top", tb[s]), ("bottom", tb[2 + s])):
+            series = vtk_read.read_pvd(tmp_path / f"{name}{s}.pvd")
+            assert [ts for ts, _ in series] == [(k + 1) * dt for k in range(nsteps)]
+            last = vtk_read.read(tmp_path / series[-1][1])
+            assert last["arrays"]["u"].tobytes() == vals.tobytes()
+            assert int(last["piece"]["NumberOfCells"]) == t.nx

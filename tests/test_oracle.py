@@ -354,3 +354,32 @@ def test_convection_matrix_closed_forms(oracle):
     assert abs((C @ u)[idx(i, j)] - expect) < 1e-12
     x = np.tile(np.arange(nx + 1) * hx, ny + 1)
     assert abs((C @ x)[idx(i, j)] - v * hx * hy) < 1e-12
+
+
+def test_boundary_well_closed_forms(oracle):
+    """computeBoundaryWell (src/simulation.cpp:581-604) pinned by its closed
+    forms: no flux gives geometric decay w0*(1-dt*k)^n; a constant flux F gives
+    w_n = w* + (w0 - w*)(1-dt*k)^n with w* = (F/s)(1-dt*k)/(dt*k), because the
+    flux is added before the decay (the other order gives a different fixed
+    point, F/(s*dt*k)); a large negative flux clamps at zero and counts one
+    underflow per clamped step."""
+    O = oracle
+    w, n = O.boundary_well(1.0, 1.0, 1.0, 0.5, 1.0)   # (1 + 1)(1 - 0.5)
+    assert (w, n) == (1.0, 0)
+    dt, k, s, F, w0 = 0.1, 0.05, 750.0, 3.0, 0.7
+    a = 1.0 - dt * k
+    w = w0
+    for step in range(1, 400):
+        w, n = O.boundary_well(w, 0.0, s, k, dt)
+        assert n == 0
+        assert math.isclose(w, w0 * a ** step, rel_tol=1e-12)
+    ws = (F / s) * a / (dt * k)
+    w = w0
+    for step in range(1, 400):
+        w, _ = O.boundary_well(w, F, s, k, dt)
+        assert math.isclose(w, ws + (w0 - ws) * a ** step, rel_tol=1e-12)
+    assert not math.isclose(ws, F / (s * dt * k), rel_tol=1e-6)
+    w, n = O.boundary_well(0.2, -1e6, s, k, dt)
+    assert (w, n) == (0.0, 1)
+    w, n = O.boundary_well(0.2, -0.2 * s, s, k, dt)       # lands exactly on zero: no underflow
+    assert (w, n) == (0.0, 0)
