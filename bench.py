@@ -27,9 +27,9 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-KERNEL_NAMES = {0: "k_ap_dir (direction update + stencil q=Ap + p.q)", 1: "k_smooth (level-0 Jacobi sweeps of the V-cycle)",
-                2: "k_update (FP64 x,r update + dots)", 3: "k_tile_couple (coverage/sample/deposit per tile)"}
-KERNEL_SYMBOLS = {0: "k_ap_dir", 1: "k_smooth", 2: "k_update", 3: "k_tile_couple"}
+# the kernel symbol of each family whose counter run (profiles/traffic.json)
+# stands for the family's traffic
+KERNEL_SYMBOLS = {0: "k_ap_dir", 1: "k_smooth", 2: "k_update", 3: "k_tile_couple", 7: "k_bottom", 8: "k_finish"}
 
 
 def log(msg):
@@ -146,6 +146,124 @@ def cpu_baseline(trap, gpu_jacobi_iters, budget_iters=(40, 240)):
             "host": {"cpu_model": _cpu_model(), "cpus_in_affinity": affinity, "os_cpu_count": os.cpu_count()}}
 
 
+def cpu_baseline_config0(nsteps=100):
+    """Config 0 in full on the host: the reference's algorithm per step (the
+    sequential cell loop over 200 static rods, then per species reassembly from
+    the element tensors, Dirichlet rows and a direct banded LU), in the
+    reference's decomposition (src/simulation.cpp:38-41): the cell loop on one
+    core, then one thread per species on its own core."""
+    import threading
+    from oracle import oracle as O
+    from eq_amd import workloads as W
+    t = W.config0()
+    grid = O.grid(t.nx + 1, t.ny + 1, t.npm, t.width, t.height)
+    probs = [O.problem(t.nx, t.ny, t.width, t.height, 0.1, D, g, 0.0, walls=W.TOP_DIRICHLET)
+             for D, g in zip(t.D, t.gamma)]
+    q = t.rates * 0.1
+    fields = [u.copy() for u in t.u0]
+    t_cells = t_solve = 0.0
+    out = [None] * len(probs)
+
+    def species(k):
+        out[k] = O.step(probs[k], fields[k], "lu")[0]
+
+    for _ in range(nsteps):
+        t0 = time.perf_counter()
+        fields, _, _ = O.couple(grid, t.cells, fields, q)
+        t1 = time.perf_counter()
+        th = [threading.Thread(target=species, args=(k,)) for k in range(len(probs))]
+        for x in th:
+            x.start()
+        for x in th:
+            x.join()
+        fields = list(out)
+        t_solve += time.perf_counter() - t1
+        t_cells += t1 - t0
+    total = t_cells + t_solve
+    return {"value": t.nodes * len(probs) * nsteps / total, "unit": "DOF-timesteps/s", "cores": len(probs),
+            "kind": "port", "steps": nsteps, "seconds": total, "seconds_cell_loop": t_cells,
+            "seconds_solves": t_solve,
+            "sample": f"config 0 in full: 160x80 trap, 2 species, 200 static rods, {nsteps} steps; per step the "
+                      f"oracle's sequential cell loop on 1 core, then each species on its own thread: reassembly "
+                      f"from the element tensors, Dirichlet rows, direct banded LU (the reference's LU solve)"}, fields
+
+
+def gpu_config0(F, W, nsteps=100):
+    """Config 0 on the GPU through the host entry points (cells and q from
+    host memory every step, as the reference's controller hands them over),
+    mixed GMG-CG to 1e-10; returns throughput and the fields after nsteps."""
+    import torch
+    t = W.config0()
+    descs = [F.field_desc(t.nx, t.ny, t.width, t.height, 0.1, D, g, 0.0, W.TOP_DIRICHLET)
+             for D, g in zip(t.D, t.gamma)]
+    q = t.rates * 0.1
+    with F.FieldBatch(descs, nspecies=2, solver=F.solver_desc(F.MIXED_GMG_CG)) as fb:
+        for s_ in range(2):
+            fb.upload(s_, t.u0[s_])
+        fb.couple(0, t.cells, t.npm, q)
+        fb.step()
+        for s_ in range(2):
+            fb.upload(s_, t.u0[s_])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(nsteps):
+            fb.couple(0, t.cells, t.npm, q)
+            fb.step()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        fields = [fb.download(s_) for s_ in range(2)]
+    return {"value": t.nodes * 2 * nsteps / dt, "unit": "DOF-timesteps/s", "steps": nsteps,
+            "ms_per_step": dt / nsteps * 1e3}, fields
+
+
+def family_profile(F, fb, step_fn, nsteps):
+    """Every kernel family of the library timed over nsteps eager steps:
+    start/stop HIP events attached to each launch on the batch stream
+    (hipExtLaunchKernelGGL), so each duration is the kernel's own execution.
+    Returns ({family: record}, eager seconds per step)."""
+    import torch
+    for k in range(F.KERNEL_COUNT):
+        fb.profile(k, True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(nsteps):
+        step_fn(k)
+    torch.cuda.synchronize()
+    eager = (time.perf_counter() - t0) / nsteps
+    fams = {}
+    for k in range(F.KERNEL_COUNT):
+        n, ms, by = fb.profile_read(k)
+        fb.profile(k, False)
+        if n:
+            fams[k] = {"family": F.KERNEL_FAMILIES[k], "launches_per_step": n / nsteps, "ms_per_step": ms / nsteps,
+                       "bytes_per_step": by / nsteps, "avg_us": ms / n * 1e3,
+                       "achieved_gbs": (by / (ms / 1e3) / 1e9) if ms > 0 else 0.0}
+    total = sum(f["ms_per_step"] for f in fams.values())
+    for f in fams.values():
+        f["share"] = f["ms_per_step"] / total if total > 0 else 0.0
+    return fams, eager
+
+
+def roofline_from(fams, step_bytes, step_s, nsteps_profiled, eager_s):
+    """Roofline record of the family with the largest summed kernel time, with
+    the whole step's fraction (all launches' algorithmic bytes over the timed
+    step) beside it."""
+    dom = max(fams, key=lambda k: fams[k]["ms_per_step"])
+    f = fams[dom]
+    ach = f["achieved_gbs"]
+    step_ach = step_bytes / step_s / 1e9
+    return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+            "traffic": load_traffic(KERNEL_SYMBOLS.get(dom, "")),
+            "kernel": f["family"], "family_id": dom, "share_of_kernel_time": f["share"],
+            "launches_per_step": f["launches_per_step"], "avg_us": f["avg_us"],
+            "bytes_per_launch": f["bytes_per_step"] / f["launches_per_step"],
+            "step_frac": step_ach / HBM_PEAK_GBS, "step_achieved": step_ach, "step_bytes": step_bytes,
+            "timing": f"start/stop HIP events attached to every launch of every family on the batch stream over "
+                      f"{nsteps_profiled} eager steps after the timed region ({eager_s * 1e3:.3f} ms/step eager); "
+                      f"step_frac = algorithmic bytes of every launch of the timed graph-replayed steps / their "
+                      f"wall time / peak"}
+
+
 def load_traffic(symbol):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
@@ -202,7 +320,7 @@ def config2_drift(F, W, cells_dev, q_dev, nsteps):
     return res
 
 
-def config3_batched(F, W, nsteps, warmup=2, ntraps=512, sigma=0.05):
+def config3_batched(F, W, nsteps, warmup=2, ntraps=512, sigma=0.05, prof=2):
     """Config 3: 512 uneven traps (seed 3) x 2 species in one batch, mixed
     solver, cycle shape picked by the autotuner after the first step; every
     step couples all traps' cells in one pass and solves all 1024 problems
@@ -219,7 +337,7 @@ def config3_batched(F, W, nsteps, warmup=2, ntraps=512, sigma=0.05):
     wlim = np.concatenate([np.full(t.cells.shape[0], t.width) for t in traps])
     hlim = np.concatenate([np.full(t.cells.shape[0], t.height) for t in traps])
     q = torch.from_numpy(np.concatenate([t.rates for t in traps]) * 0.1).to("cuda").contiguous()
-    total = warmup + nsteps
+    total = warmup + nsteps + prof
     walk = np.empty((total,) + cells0.shape)
     cur = cells0
     for k in range(total):
@@ -244,27 +362,32 @@ def config3_batched(F, W, nsteps, warmup=2, ntraps=512, sigma=0.05):
         step(k)
     torch.cuda.synchronize()
     iters, rel = [], []
+    fb.traffic(reset=True)
     t0 = time.perf_counter()
-    for k in range(warmup, total):
+    for k in range(warmup, total - prof):
         st = step(k)
         iters.append(st.iterations_max)
         rel.append(st.relres_max)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    step_bytes = fb.traffic(reset=True)[0] / nsteps
+    fams, eager = family_profile(F, fb, lambda k: step(total - prof + k), prof) if prof else ({}, 0.0)
+    roof = roofline_from(fams, step_bytes, dt / nsteps, prof, eager) if fams else None
     res = {"workload": f"{ntraps} uneven traps (seed 3), h=0.1 um, 2 species, {int(off[-1])} rods",
            "value": nodes * nsteps / dt, "unit": "DOF-timesteps/s", "steps": nsteps,
            "ms_per_step": dt / nsteps * 1e3, "dofs": nodes, "problems": len(descs),
            "iterations_max": int(max(iters)), "relres_max": float(max(rel)),
            "autotuned_cycle": {"pre": int(sd.mg_pre), "post": int(sd.mg_post),
                                "fuse_max_nodes": int(sd.mg_fuse_max_nodes), "fuse_pre_finest": int(sd.mg_fuse_pre_top)},
-           "autotune_seconds": t_tune, "create_seconds": t_create}
+           "autotune_seconds": t_tune, "create_seconds": t_create, "roofline": roof,
+           "kernel_families": {f["family"]: f for f in fams.values()} or None}
     log(f"bench: config 3: {res['value']:.4e} DOF-timesteps/s, {res['ms_per_step']:.2f} ms/step, "
         f"V({sd.mg_pre},{sd.mg_post}), iterations max {res['iterations_max']}")
     fb.close()
     return res
 
 
-def config4_advection(F, W, nsteps, warmup=2):
+def config4_advection(F, W, nsteps, warmup=2, prof=1):
     """Config 4: one 8192x4096 trap (h = 0.025 um), 4 species, parabolic
     channel velocity along x (peak 50 um/s, zero at the walls), mixed
     GMG-BiCGStab to 1e-10, cycle shape and fusion picked by the autotuner
@@ -298,6 +421,7 @@ def config4_advection(F, W, nsteps, warmup=2):
         step()
     torch.cuda.synchronize()
     iters, rel = [], []
+    fb.traffic(reset=True)
     t0 = time.perf_counter()
     for _ in range(nsteps):
         st = step()
@@ -305,6 +429,9 @@ def config4_advection(F, W, nsteps, warmup=2):
         rel.append(st.relres_max)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    step_bytes = fb.traffic(reset=True)[0] / nsteps
+    fams, eager = family_profile(F, fb, lambda k: step(), prof) if prof else ({}, 0.0)
+    roof = roofline_from(fams, step_bytes, dt / nsteps, prof, eager) if fams else None
     res = {"workload": "one 204.8x102.4 um trap, 8192x4096 P1 mesh (33 566 721 nodes/species), 4 species, "
                        "parabolic v_x peak 50 um/s, 100000 static rods (seed 4), mixed GMG-BiCGStab",
            "value": nodes * nsteps / dt, "unit": "DOF-timesteps/s", "steps": nsteps, "warmup": warmup,
@@ -312,7 +439,8 @@ def config4_advection(F, W, nsteps, warmup=2):
            "relres_max": float(max(rel)), "create_seconds": t_create,
            "autotuned_cycle": {"pre": int(sd.mg_pre), "post": int(sd.mg_post),
                                "fuse_max_nodes": int(sd.mg_fuse_max_nodes), "fuse_pre_finest": int(sd.mg_fuse_pre_top)},
-           "autotune_seconds": t_tune}
+           "autotune_seconds": t_tune, "roofline": roof,
+           "kernel_families": {f["family"]: f for f in fams.values()} or None}
     log(f"bench: config 4: {res['value']:.4e} DOF-timesteps/s, {res['ms_per_step']:.2f} ms/step, "
         f"BiCGStab iterations max {res['iterations_max']}, relres {res['relres_max']:.2e}, "
         f"V({sd.mg_pre},{sd.mg_post}) fused-finest-pre {sd.mg_fuse_pre_top}")
@@ -381,55 +509,34 @@ def main():
     torch.cuda.synchronize()
 
     run = Run(F, trap, kinds[args.solver], cells_dev, q_dev, sampled_dev)
-    # warmup with every kernel timed, to pick the dominant kernel
-    for k in range(4):
-        run.fb.profile(k, not args.no_profile)
     for k in range(args.warmup):
         run.step(k)
-    dominant, best = None, -1.0
-    for k in range(4):
-        n, ms, by = run.fb.profile_read(k)
-        log(f"bench: warmup {KERNEL_NAMES[k]}: launches={n} total={ms:.3f} ms")
-        if n and ms > best:
-            dominant, best = k, ms
-        run.fb.profile(k, False)
-    # timed region: graph replays, no per-launch events
+    # timed region: graph replays, no per-launch events; the traffic counter
+    # sums the algorithmic bytes of every launch (graph replays included)
     log(f"bench: timing {args.steps} steps ({args.solver})")
     torch.cuda.synchronize()
+    run.fb.traffic(reset=True)
     t0 = time.perf_counter()
     for k in range(args.steps):
         run.step(args.warmup + k)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    step_bytes, step_launches = run.fb.traffic(reset=True)
+    step_bytes /= args.steps
     nodes = trap.nodes * len(trap.D)
     value = nodes * args.steps / elapsed
     iters = run.iters[args.warmup:]
     relres = run.relres[args.warmup:]
-    # roofline of the dominant kernel: start/stop HIP events attached to every
-    # launch of it (hipExtLaunchKernelGGL on the batch stream) over the next
-    # prof_steps steps (events need eager launches; the kernels are the same
-    # ones the graphs replay)
-    roof = None
+    # per-family kernel time over the next prof_steps steps; the roofline is
+    # the family with the most kernel time
+    roof, fams = None, None
     prof_steps = min(10, args.steps)
-    if dominant is not None and prof_steps > 0:
-        run.fb.profile(dominant, True)
-        torch.cuda.synchronize()
-        tp = time.perf_counter()
-        for k in range(prof_steps):
-            run.step(args.warmup + args.steps + k)
-        torch.cuda.synchronize()
-        eager_s = (time.perf_counter() - tp) / prof_steps
-        n, ms, by = run.fb.profile_read(dominant)
-        run.fb.profile(dominant, False)
-        if n:
-            avg_s = ms / n / 1e3
-            ach = by / n / avg_s / 1e9
-            roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": ach / HBM_PEAK_GBS, "traffic": load_traffic(KERNEL_SYMBOLS[dominant]),
-                    "kernel": KERNEL_NAMES[dominant], "launches": n, "avg_us": avg_s * 1e6,
-                    "bytes_per_launch": by / n,
-                    "timing": f"start/stop HIP events attached to each launch on the batch stream over {prof_steps} steps "
-                              f"after the timed region (eager launches, {eager_s * 1e3:.3f} ms/step)"}
+    if not args.no_profile and prof_steps > 0:
+        fams, eager_s = family_profile(F, run.fb, lambda k: run.step(args.warmup + args.steps + k), prof_steps)
+        roof = roofline_from(fams, step_bytes, elapsed / args.steps, prof_steps, eager_s)
+        for k, f in sorted(fams.items(), key=lambda kv: -kv[1]["ms_per_step"]):
+            log(f"bench: {f['family']}: {f['ms_per_step']:.3f} ms/step ({f['share']:.1%}), "
+                f"{f['achieved_gbs']:.0f} GB/s")
     fields_final = [run.fb.download(s) for s in range(2)]
     log(f"bench: {value:.4e} DOF-timesteps/s, {elapsed / args.steps * 1e3:.3f} ms/step, "
         f"iterations max {max(iters)}, relres max {max(relres):.2e}")
@@ -457,6 +564,10 @@ def main():
         "iterations": {"max": int(max(iters)), "mean": float(np.mean(iters))},
         "relres_max": float(max(relres)),
         "roofline": roof,
+        "kernel_families": {F.KERNEL_FAMILIES[k]: v for k, v in fams.items()} if fams else None,
+        "step_traffic": {"bytes_per_step": step_bytes, "launches_per_step": step_launches / args.steps,
+                         "achieved_gbs": step_bytes / (elapsed / args.steps) / 1e9,
+                         "frac": step_bytes / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS},
     }
 
     if not args.no_variants:
@@ -499,6 +610,14 @@ def main():
             jr.fb.close()
         log("bench: timing the CPU baseline sample")
         out["cpu_baseline"] = cpu_baseline(trap, jac_iters)
+        out["cpu_baseline"]["extrapolated"] = True
+        log("bench: config 0 in full on the CPU (oracle, banded LU) and on the GPU")
+        c0, f_cpu = cpu_baseline_config0()
+        g0, f_gpu = gpu_config0(F, W)
+        c0["gpu_same_workload"] = g0
+        c0["gpu_vs_cpu_rel_l2_after_100_steps"] = [float(np.linalg.norm(a - b) / np.linalg.norm(b))
+                                                   for a, b in zip(f_gpu, f_cpu)]
+        out["cpu_baseline_config0"] = c0
     del fields_final
     print(json.dumps(out), flush=True)
 

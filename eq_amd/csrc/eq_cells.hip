@@ -57,7 +57,9 @@ struct TrapGrid {
     int32_t tile_base;             // first global tile id
     int32_t prob0;                 // problem of species 0
     int32_t pitch;
+    int32_t tens;                  // 1: the coupling sets this trap's diffusion tensor from the cells
     double trap_w, trap_h;
+    double Dx, Dy;                 // axial and transverse diffusion factors (tens = 1)
 };
 
 // size_t(round(v)) of eQ::data::ij_from_xy (src/eQ.h:119-124); negative values clamp to 0
@@ -333,7 +335,8 @@ __global__ __launch_bounds__(kCThreads) void k_tile_couple(
     const TrapGrid* __restrict__ traps, int t0, int ntr, uint32_t tile0, const uint32_t* __restrict__ start,
     const uint32_t* __restrict__ end, const uint32_t* __restrict__ sorted_pairs, const uint32_t* __restrict__ pair_cell,
     const CellGeo* __restrict__ geo, double* const* __restrict__ fields, int ns, const double* __restrict__ q,
-    double* __restrict__ partial, double npm, unsigned flags)
+    double* __restrict__ partial, double npm, unsigned flags, double* const* __restrict__ tens64,
+    float* const* __restrict__ tens32, int64_t tstride)
 {
     __shared__ double red[2][kCThreads / 64][kSpeciesPass]; // double-buffered wave sums
     __shared__ CellGeo sg[kChunk];      // cx, cy replaced by the inverse translation
@@ -341,7 +344,8 @@ __global__ __launch_bounds__(kCThreads) void k_tile_couple(
     __shared__ double sq[kChunk][kSpeciesPass]; // q_c / |S_c|
     const uint32_t tl = blockIdx.x;
     const uint32_t e0 = start[tl], e1 = end[tl];
-    if (e0 >= e1) return;
+    const bool want_tens = (flags & EQ_COUPLE_TENSOR) != 0;
+    if (e0 >= e1 && !want_tens) return;
     // the tile's trap: traps are numbered in tile order
     int lo = t0, hi = t0 + ntr - 1;
     const uint32_t tg = tile0 + tl;
@@ -360,8 +364,18 @@ __global__ __launch_bounds__(kCThreads) void k_tile_couple(
     double Y[kCRows];
 #pragma unroll
     for (int r = 0; r < kCRows; ++r) Y[r] = (double)(ib + r) / npm;
+    // setDiffusionTensor (src/abm/eQabm.cpp:241-248, 306-325): the nodes start
+    // isotropic and each covering cell, in list order, overwrites them
+    const bool tens = want_tens && G.tens;
+    double t11[kCRows], t22[kCRows], t12[kCRows];
+#pragma unroll
+    for (int r = 0; r < kCRows; ++r) {
+        t11[r] = 1.0;
+        t22[r] = 1.0;
+        t12[r] = 0.0;
+    }
     int buf = 0;
-    for (int s0 = 0; s0 < ns; s0 += kSpeciesPass) {
+    for (int s0 = 0; s0 < ns && e0 < e1; s0 += kSpeciesPass) {
         const int nsp = min(kSpeciesPass, ns - s0);
         double u[kSpeciesPass][kCRows];
         bool in[kCRows];
@@ -406,6 +420,20 @@ __global__ __launch_bounds__(kCThreads) void k_tile_couple(
                     cov[r] = v;
                     anyc = anyc || v;
                 }
+                if (tens && s0 == 0 && anyc) {
+                    // cos^2, sin^2 and sin cos as the reference forms them, each product rounded
+                    const double c2 = __dmul_rn(cs, cs), s2 = __dmul_rn(sn, sn), sc = __dmul_rn(sn, cs);
+                    const double d11 = __dadd_rn(__dmul_rn(G.Dx, c2), __dmul_rn(G.Dy, s2));
+                    const double d22 = __dadd_rn(__dmul_rn(G.Dx, s2), __dmul_rn(G.Dy, c2));
+                    const double d12 = __dmul_rn(__dsub_rn(G.Dx, G.Dy), sc);
+#pragma unroll
+                    for (int r = 0; r < kCRows; ++r)
+                        if (cov[r]) {
+                            t11[r] = d11;
+                            t22[r] = d22;
+                            t12[r] = d12;
+                        }
+                }
                 const bool wave_any = __ballot(anyc) != 0ull; // uniform over the wavefront
 #pragma unroll
                 for (int s = 0; s < kSpeciesPass; ++s) {
@@ -440,6 +468,26 @@ __global__ __launch_bounds__(kCThreads) void k_tile_couple(
 #pragma unroll
                 for (int s = 0; s < kSpeciesPass; ++s)
                     if (s < nsp && in[r]) fields[G.prob0 + s0 + s][(int64_t)(ib + r) * G.pitch + j] = u[s][r];
+        }
+    }
+    if (!tens) return;
+    // every species of the trap carries the same tensor, in FP64 (the Krylov
+    // operator) and, for an FP32 V-cycle, in FP32
+    for (int s = 0; s < ns; ++s) {
+        double* d = tens64[G.prob0 + s];
+        float* f = tens32 ? tens32[G.prob0 + s] : nullptr;
+#pragma unroll
+        for (int r = 0; r < kCRows; ++r) {
+            if (!((ib + r < G.nodes_h) && (j < G.nodes_w))) continue;
+            const int64_t k = (int64_t)(ib + r) * G.pitch + j;
+            d[k] = t11[r];
+            d[k + tstride] = t22[r];
+            d[k + 2 * tstride] = t12[r];
+            if (f) {
+                f[k] = (float)t11[r];
+                f[k + tstride] = (float)t22[r];
+                f[k + 2 * tstride] = (float)t12[r];
+            }
         }
     }
 }
@@ -498,6 +546,8 @@ static int ensure_traps(eq_batch* b)
         G.pitch = P.pitch;
         G.trap_w = b->descs[t * ns].width;
         G.trap_h = b->descs[t * ns].height;
+        G.tens = 0;
+        G.Dx = G.Dy = 1.0;
         tiles += (int64_t)G.tiles_x * G.tiles_y;
     }
     if (tiles >= ((int64_t)1 << 31)) return set_error(EQ_ERR_INVALID, "too many coupling tiles");
@@ -610,14 +660,37 @@ static int couple_impl(eq_batch* b, int32_t t0, int32_t ntr, const int64_t* cell
     }
     const double cb = nodes * ns * sizeof(double) * ((flags & EQ_COUPLE_DEPOSIT) ? 2 : 1) + 8.0 * total + gb +
                       8.0 * ns * nc + 8.0 * ns * total + 8.0 * ntiles;
-    launch(b, EQ_KERNEL_COUPLE, cb, k_tile_couple, dim3(ntiles), dim3(kCThreads), 0, s, traps, t0, ntr, tile0,
+    // the tensor is written only where a trap of the call has one set from its cells
+    bool tens = false;
+    if (flags & EQ_COUPLE_TENSOR)
+        for (int t : b->cell_tensor_traps) tens = tens || (t >= t0 && t < t0 + ntr);
+    const unsigned kflags = tens ? flags : (flags & ~EQ_COUPLE_TENSOR);
+    const double tb = tens ? nodes * ns * 3.0 * (sizeof(double) + (b->mixed ? sizeof(float) : 0)) : 0.0;
+    launch(b, EQ_KERNEL_COUPLE, cb + tb, k_tile_couple, dim3(ntiles), dim3(kCThreads), 0, s, traps, t0, ntr, tile0,
            (const uint32_t*)t_start, (const uint32_t*)t_end, (const uint32_t*)vals2, (const uint32_t*)pair_cell,
-           (const CellGeo*)geo, (double* const*)b->d_field_ptrs, ns, d_q, partial, npm, flags);
+           (const CellGeo*)geo, (double* const*)b->d_field_ptrs, ns, d_q, partial, npm, kflags,
+           (double* const*)b->d_tens64, (float* const*)b->d_tens32, (int64_t)b->levels[0].pool);
     if (d_sampled || d_counts)
         launch(b, kid, gb + 8.0 * ns * total + 12.0 * ns * nc, k_cell_sample, dim3((unsigned)((nc + 255) / 256)),
                dim3(256), 0, s, (const CellGeo*)geo, (int64_t)nc, ns, (const double*)partial,
                (flags & EQ_COUPLE_SAMPLE) ? d_sampled : nullptr, d_counts);
     EQ_HIP(hipGetLastError());
+    if (tens) return refresh_cell_tensors(b);
+    return EQ_OK;
+}
+
+int set_trap_tensor(eq_batch* b, int trap, double Dx, double Dy, int on)
+{
+    int rc = ensure_traps(b);
+    if (rc) return rc;
+    TrapGrid G;
+    TrapGrid* d = (TrapGrid*)b->d_cell_traps + trap;
+    EQ_HIP(hipStreamSynchronize(b->stream));
+    EQ_HIP(hipMemcpy(&G, d, sizeof(TrapGrid), hipMemcpyDeviceToHost));
+    G.tens = on ? 1 : 0;
+    G.Dx = on ? Dx : 1.0;
+    G.Dy = on ? Dy : 1.0;
+    EQ_HIP(hipMemcpy(d, &G, sizeof(TrapGrid), hipMemcpyHostToDevice));
     return EQ_OK;
 }
 

@@ -116,7 +116,7 @@ struct LevelProb {
     // advection: x-velocity per grid row (ny+1 values, P1 in y), Galerkin
     // convection dt * int (v du/dx) phi; cw = dt*hx*hy/(24 hx) = dt*hy/24
     int32_t vel;           // 1: the row velocities below are set
-    int32_t pad2_;
+    int32_t celltens;      // 1: the tensor comes from the cells' angles (refreshed by the coupling)
     double cw;
     const double* vel_d;   // FP64 rows (level 0)
     const float* vel_f;    // FP32 rows (levels smoothed in FP32)

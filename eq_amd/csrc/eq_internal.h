@@ -51,6 +51,12 @@ struct BottomRef {
     int32_t rec, nl, prob, pad_;
 };
 
+// one coarsest-level inverse rebuilt on the device (cell-driven tensor)
+struct InvJob {
+    int32_t lev, idx;    // the problem's coarsest level and its index in that level's table
+    int64_t scratch;     // offset (doubles) of its assembly scratch: A (n*n), X (n*n), tensor (3n)
+};
+
 // one boundary well (Simulation::computeBoundaryWell, src/simulation.cpp:581-604)
 struct WellDev {
     int32_t prob;        // problem index (its level-0 table entry)
@@ -186,6 +192,16 @@ struct eq_batch {
     eqk::Acct acct;
     eqk::Acct* acct_to = &acct;
     eqk::Acct g_init_acct, g_iter2_acct, g_bicg_acct;
+    // tensors set from the cells' angles (allocated on first use): per problem,
+    // node (0,0) of its finest-level tensor in FP64 and (FP32 V-cycle) FP32;
+    // the coarsest inverses to rebuild after each such coupling
+    double** d_tens64 = nullptr;
+    float** d_tens32 = nullptr;
+    std::vector<int32_t> cell_tensor_traps;
+    std::vector<eqk::InvJob> inv_jobs;
+    eqk::InvJob* d_inv_jobs = nullptr;
+    double* inv_scratch = nullptr;
+    eqk::LevelProb** d_level_tables = nullptr;
 };
 
 namespace eqk {
@@ -199,6 +215,11 @@ void free_channels(eq_batch* b);
 int pull_well_values(eq_batch* b);
 // boundary wells: update after a step
 int wells_after_step(eq_batch* b);
+// cell-driven tensors: after the coupling wrote the finest level, carry it to
+// the coarse levels and rebuild the coarsest inverses (eq_solver.hip)
+int refresh_cell_tensors(eq_batch* b);
+// the coupling's trap table entry of `trap` (eq_cells.hip)
+int set_trap_tensor(eq_batch* b, int trap, double Dx, double Dy, int on);
 } // namespace eqk
 
 namespace eqk {

@@ -1829,6 +1829,123 @@ __global__ __launch_bounds__(256) void k_wells(LevelProb* probs, WellDev* wells,
         if ((P.dmask >> k) & 1) P.g[k] = w;
 }
 
+// Cell-driven tensor, coarse level: each node takes the tensor of the nearest
+// node of the next finer level (the injection set_tensor applies on the host),
+// for the problems whose tensor comes from the cells.  One block per tile.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_tensor_inject(LevelView L, const LevelProb* __restrict__ fine, int R)
+{
+    const TileRef t = L.tiles[blockIdx.x];
+    const LevelProb& P = L.probs[t.pi];
+    if (!P.celltens) return;
+    const LevelProb& F = fine[P.fidx];
+    const T* src = std::is_same<T, float>::value ? (const T*)F.tens_f : (const T*)F.tens_d;
+    T* dst = std::is_same<T, float>::value ? (T*)P.tens_f : (T*)P.tens_d;
+    const int j = t.j0 + (int)(threadIdx.x & 63);
+    if (j > P.nx) return;
+    const int jf = (int)((2LL * j * F.nx + P.nx) / (2LL * P.nx));
+    for (int r = 0; r < R; ++r) {
+        const int i = t.i0 + (int)(threadIdx.x >> 6) * R + r;
+        if (i > P.ny) break;
+        const int i_f = (int)((2LL * i * F.ny + P.ny) / (2LL * P.ny));
+        const int64_t kf = F.origin + (int64_t)i_f * F.pitch + jf, kc = P.origin + (int64_t)i * P.pitch + j;
+        for (int c = 0; c < 3; ++c) dst[kc + c * P.tstride] = src[kf + c * F.tstride];
+    }
+}
+
+// Cell-driven tensor, coarsest level: the dense inverse of the level's
+// operator (Dirichlet rows identity), as the host builds it for set_tensor:
+// assemble in FP64 from the level's tensor, banded LU without pivoting, then
+// one column of the inverse per thread, stored column-major in the smoother's
+// precision.  One block per problem; scratch A (n*n), X (n*n), tensor (3n).
+template <typename T>
+__global__ __launch_bounds__(256) void k_coarse_inverse(LevelProb* const* __restrict__ levels,
+                                                        const InvJob* __restrict__ jobs, double* scratch,
+                                                        T* __restrict__ dense)
+{
+    const InvJob J = jobs[blockIdx.x];
+    const LevelProb& P = levels[J.lev][J.idx];
+    const int nw = P.nx + 1, n = nw * (P.ny + 1), bw = nw + 1;
+    double* A = scratch + J.scratch;
+    double* X = A + (int64_t)n * n;
+    double* t11 = X + (int64_t)n * n;
+    double* t22 = t11 + n;
+    double* t12 = t22 + n;
+    const T* ts = std::is_same<T, float>::value ? (const T*)P.tens_f : (const T*)P.tens_d;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        const int i = k / nw, j = k - i * nw;
+        const int64_t q = P.origin + (int64_t)i * P.pitch + j;
+        t11[k] = (double)ts[q];
+        t22[k] = (double)ts[q + P.tstride];
+        t12[k] = (double)ts[q + 2 * P.tstride];
+    }
+    for (int64_t k = threadIdx.x; k < (int64_t)n * n; k += blockDim.x) A[k] = 0.0;
+    __syncthreads();
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        const int i = k / nw, j = k - i * nw;
+        if (is_dirichlet(P, i, j)) {
+            A[(int64_t)k * n + k] = 1.0;
+            continue;
+        }
+        Row<double> r = row_coefs_at<double>(P, i, j);
+        aniso_stiffness<double>(P, i, j, t11 + k, t22 + k, t12 + k, nw, r);
+        double* row = A + (int64_t)k * n;
+        row[k] += row_diag(r);
+        auto put = [&](int ii, int jj, double v) {
+            if (ii < 0 || jj < 0 || ii > P.ny || jj > P.nx || v == 0.0) return;
+            if (is_dirichlet(P, ii, jj)) return;
+            row[jj + ii * nw] += v;
+        };
+        put(i, j - 1, r.mw - r.kw);
+        put(i, j + 1, r.me - r.ke);
+        put(i - 1, j, r.ms - r.ks);
+        put(i + 1, j, r.mn - r.kn);
+        put(i - 1, j - 1, r.msw - r.ksw);
+        put(i + 1, j + 1, r.mne - r.kne);
+    }
+    __syncthreads();
+    // banded LU in place, multipliers below the diagonal
+    __shared__ int bad;
+    if (threadIdx.x == 0) bad = 0;
+    for (int c = 0; c < n; ++c) {
+        const double d = A[(int64_t)c * n + c];
+        if (!(d > 0.0)) {
+            if (threadIdx.x == 0) bad = 1;
+            break; // uniform: every thread reads the same pivot
+        }
+        const int rmax = min(n - 1, c + bw);
+        for (int r = c + 1 + (int)threadIdx.x; r <= rmax; r += blockDim.x) A[(int64_t)r * n + c] /= d;
+        __syncthreads();
+        const int w = rmax - c;
+        for (int e = threadIdx.x; e < w * w; e += blockDim.x) {
+            const int r = c + 1 + e / w, q = c + 1 + e % w;
+            const double f = A[(int64_t)r * n + c];
+            if (f != 0.0) A[(int64_t)r * n + q] -= f * A[(int64_t)c * n + q];
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+    if (bad) return; // cannot happen for Dx, Dy > 0 (D11 D22 - D12^2 = Dx Dy): the old inverse stays
+    // column m of the inverse: L y = e_m, U x = y
+    for (int m = threadIdx.x; m < n; m += blockDim.x) {
+        double* x = X + (int64_t)m * n;
+        for (int r = 0; r < n; ++r) x[r] = (r == m) ? 1.0 : 0.0;
+        for (int r = m + 1; r < n; ++r) {
+            double v = x[r];
+            for (int q = max(m, r - bw); q < r; ++q) v -= A[(int64_t)r * n + q] * x[q];
+            x[r] = v;
+        }
+        for (int r = n - 1; r >= 0; --r) {
+            double v = x[r];
+            const int qmax = min(n - 1, r + bw);
+            for (int q = r + 1; q <= qmax; ++q) v -= A[(int64_t)r * n + q] * x[q];
+            x[r] = v / A[(int64_t)r * n + r];
+        }
+        T* out = dense + P.dense_off + (int64_t)m * n;
+        for (int r = 0; r < n; ++r) out[r] = (T)x[r];
+    }
+}
+
 // setBoundaryValues: every Dirichlet wall of one problem takes `value`
 __global__ void k_set_walls(LevelProb* probs, int pi, double value)
 {
@@ -3018,6 +3135,11 @@ void eq_batch_destroy(eq_batch* b)
     hipFree(b->d_field_ptrs);
     hipFree(b->cell_scratch);
     hipFree(b->d_wells);
+    hipFree(b->d_tens64);
+    hipFree(b->d_tens32);
+    hipFree(b->d_level_tables);
+    hipFree(b->d_inv_jobs);
+    hipFree(b->inv_scratch);
     free_channels(b);
     hipFree(b->d_cell_traps);
     hipFree(b->d_cell_off);
@@ -3254,6 +3376,155 @@ static int set_tensor_checked(eq_batch* b, int32_t prob, const double* D11, cons
     if (all && P.vel) return set_error(EQ_ERR_INVALID, "advection and a tensor field together are not supported");
     hipSetDevice(b->device);
     return b->mixed ? set_tensor_impl<float>(b, prob, D11, D22, D12) : set_tensor_impl<double>(b, prob, D11, D22, D12);
+}
+
+} // extern "C"
+
+namespace eqk {
+
+template <typename T>
+static int refresh_cell_tensors_t(eq_batch* b)
+{
+    hipStream_t s = b->stream;
+    for (size_t l = 1; l < b->levels.size(); ++l) {
+        Level& L = b->levels[l];
+        const Level& Fn = b->levels[l - 1];
+        const int nt = (int)L.tiles_all.size();
+        if (nt == 0) continue;
+        // each coarse node reads and writes the three entries
+        launch(b, EQ_KERNEL_COUPLE_SETUP, L.nodes_all * 6.0 * sizeof(T), k_tensor_inject<T>, dim3(nt), dim3(kBlock), 0, s,
+               L.all(), (const LevelProb*)Fn.d_probs, L.R);
+    }
+    if (b->inv_jobs.empty()) return EQ_OK;
+    double by = 0.0;
+    for (const InvJob& J : b->inv_jobs) {
+        const LevelProb& P = b->levels[J.lev].probs[J.idx];
+        const double n = (double)(P.nx + 1) * (P.ny + 1);
+        by += n * n * (3.0 * sizeof(double) + sizeof(T));
+    }
+    launch(b, EQ_KERNEL_COUPLE_SETUP, by, k_coarse_inverse<T>, dim3((int)b->inv_jobs.size()), dim3(256), 0, s,
+           (LevelProb* const*)b->d_level_tables, (const InvJob*)b->d_inv_jobs, b->inv_scratch, (T*)b->dense);
+    return EQ_OK;
+}
+
+int refresh_cell_tensors(eq_batch* b)
+{
+    return b->mixed ? refresh_cell_tensors_t<float>(b) : refresh_cell_tensors_t<double>(b);
+}
+
+} // namespace eqk
+
+extern "C" {
+
+int eq_batch_cell_tensor(eq_batch* b, int32_t trap, double Dx, double Dy, int32_t enable)
+{
+    if (!b) return set_error(EQ_ERR_INVALID, "NULL batch");
+    if (trap < 0 || trap >= b->ntraps) return set_error(EQ_ERR_INVALID, "trap index out of range");
+    if (enable && (!(Dx > 0) || !(Dy > 0) || !std::isfinite(Dx) || !std::isfinite(Dy)))
+        return set_error(EQ_ERR_INVALID, "Dx and Dy must be finite and > 0");
+    const int ns = b->nspecies, p0 = trap * ns;
+    for (int p = p0; p < p0 + ns; ++p) {
+        const LevelProb& P = b->levels[0].probs[p];
+        if (enable && P.fd) return set_error(EQ_ERR_INVALID, "the finite-difference scheme has no tensor field");
+        if (enable && P.vel) return set_error(EQ_ERR_INVALID, "advection and a tensor field together are not supported");
+    }
+    hipSetDevice(b->device);
+    const bool was = std::find(b->cell_tensor_traps.begin(), b->cell_tensor_traps.end(), trap) !=
+                     b->cell_tensor_traps.end();
+    int rc;
+    if ((rc = set_trap_tensor(b, trap, Dx, Dy, enable ? 1 : 0))) return rc;
+    const LevelProb& Q = b->levels[0].probs[p0];
+    const size_t n = (size_t)(Q.nx + 1) * (Q.ny + 1);
+    if (enable) {
+        // start from the reference's isotropic grids; this allocates the tensor
+        // arrays of every level and marks the problems anisotropic
+        std::vector<double> one(n, 1.0), zero(n, 0.0);
+        for (int p = p0; p < p0 + ns; ++p)
+            if ((rc = set_tensor_checked(b, p, one.data(), one.data(), zero.data(), n))) return rc;
+    } else {
+        for (int p = p0; p < p0 + ns; ++p)
+            if ((rc = set_tensor_checked(b, p, nullptr, nullptr, nullptr, 0))) return rc;
+    }
+    // mark (or clear) the problems on every level, and rebuild the job list
+    for (size_t l = 0; l < b->levels.size(); ++l) {
+        Level& L = b->levels[l];
+        for (LevelProb& P : L.probs)
+            if (P.prob >= p0 && P.prob < p0 + ns) P.celltens = enable ? 1 : 0;
+        EQ_HIP(hipMemcpy(L.d_probs, L.probs.data(), sizeof(LevelProb) * L.probs.size(), hipMemcpyHostToDevice));
+    }
+    if ((rc = sync_bottom_recs(b, true))) return rc;
+    if (enable && !was) b->cell_tensor_traps.push_back(trap);
+    if (!enable && was) b->cell_tensor_traps.erase(std::find(b->cell_tensor_traps.begin(), b->cell_tensor_traps.end(), trap));
+    // device tables: per problem, node (0,0) of its finest-level tensor
+    if (!b->d_tens64) {
+        if (hipMalloc((void**)&b->d_tens64, sizeof(double*) * b->nprob) != hipSuccess ||
+            hipMalloc((void**)&b->d_tens32, sizeof(float*) * b->nprob) != hipSuccess ||
+            hipMalloc((void**)&b->d_level_tables, sizeof(LevelProb*) * b->levels.size()) != hipSuccess)
+            return set_error(EQ_ERR_NOMEM, "tensor tables");
+        std::vector<LevelProb*> lt;
+        for (Level& L : b->levels) lt.push_back(L.d_probs);
+        EQ_HIP(hipMemcpy(b->d_level_tables, lt.data(), sizeof(LevelProb*) * lt.size(), hipMemcpyHostToDevice));
+    }
+    {
+        std::vector<double*> t64(b->nprob, nullptr);
+        std::vector<float*> t32(b->nprob, nullptr);
+        for (int p = 0; p < b->nprob; ++p) {
+            const LevelProb& P = b->levels[0].probs[p];
+            if (!P.celltens) continue;
+            t64[p] = b->tens0_d + P.origin;
+            if (b->mixed) t32[p] = (float*)b->levels[0].tens + P.origin;
+        }
+        EQ_HIP(hipMemcpy(b->d_tens64, t64.data(), sizeof(double*) * b->nprob, hipMemcpyHostToDevice));
+        EQ_HIP(hipMemcpy(b->d_tens32, t32.data(), sizeof(float*) * b->nprob, hipMemcpyHostToDevice));
+    }
+    // coarsest inverses of every cell-driven problem (multigrid kinds only)
+    b->inv_jobs.clear();
+    int64_t need = 0;
+    if (b->solver.kind != EQ_SOLVER_JACOBI_CG && b->dense)
+        for (size_t l = 0; l < b->levels.size(); ++l) {
+            const Level& L = b->levels[l];
+            for (size_t k = 0; k < L.probs.size(); ++k) {
+                const LevelProb& P = L.probs[k];
+                if (!P.celltens || !P.coarsest) continue;
+                const int64_t nn = (int64_t)(P.nx + 1) * (P.ny + 1);
+                b->inv_jobs.push_back(InvJob{(int32_t)l, (int32_t)k, need});
+                need += 2 * nn * nn + 3 * nn;
+            }
+        }
+    hipFree(b->d_inv_jobs);
+    hipFree(b->inv_scratch);
+    b->d_inv_jobs = nullptr;
+    b->inv_scratch = nullptr;
+    if (!b->inv_jobs.empty()) {
+        if (hipMalloc((void**)&b->d_inv_jobs, sizeof(InvJob) * b->inv_jobs.size()) != hipSuccess ||
+            hipMalloc((void**)&b->inv_scratch, sizeof(double) * need) != hipSuccess)
+            return set_error(EQ_ERR_NOMEM, "coarsest inverse scratch");
+        EQ_HIP(hipMemcpy(b->d_inv_jobs, b->inv_jobs.data(), sizeof(InvJob) * b->inv_jobs.size(), hipMemcpyHostToDevice));
+    }
+    refresh_general(b);
+    return EQ_OK;
+}
+
+int eq_batch_tensor_values(eq_batch* b, int32_t prob, double* D11, double* D22, double* D12, size_t n)
+{
+    if (!b || !D11 || !D22 || !D12) return set_error(EQ_ERR_INVALID, "NULL argument");
+    if (prob < 0 || prob >= b->nprob) return set_error(EQ_ERR_INVALID, "problem index out of range");
+    const LevelProb& P = b->levels[0].probs[prob];
+    const size_t nw = (size_t)P.nx + 1, nh = (size_t)P.ny + 1;
+    if (n != nw * nh) return set_error(EQ_ERR_INVALID, "length must be (nx+1)*(ny+1)");
+    hipSetDevice(b->device);
+    EQ_HIP(hipStreamSynchronize(b->stream));
+    double* out[3] = {D11, D22, D12};
+    if (!P.aniso) {
+        std::fill(D11, D11 + n, 1.0);
+        std::fill(D22, D22 + n, 1.0);
+        std::fill(D12, D12 + n, 0.0);
+        return EQ_OK;
+    }
+    for (int c = 0; c < 3; ++c)
+        EQ_HIP(hipMemcpy2D(out[c], sizeof(double) * nw, b->tens0_d + (int64_t)c * b->levels[0].pool + P.origin,
+                           sizeof(double) * P.pitch, sizeof(double) * nw, nh, hipMemcpyDeviceToHost));
+    return EQ_OK;
 }
 
 int eq_batch_set_tensor(eq_batch* b, int32_t prob, const double* D11, const double* D22, const double* D12,

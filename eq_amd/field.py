@@ -23,7 +23,7 @@ NEUMANN, DIRICHLET, ROBIN = 0, 1, 2
 LEFT, RIGHT, TOP, BOTTOM = 0, 1, 2, 3
 JACOBI_CG, GMG_CG, MIXED_GMG_CG = 0, 1, 2
 GMG_BICGSTAB, MIXED_GMG_BICGSTAB = 3, 4
-SAMPLE, DEPOSIT = 1, 2
+SAMPLE, DEPOSIT, TENSOR = 1, 2, 4
 KERNEL_STENCIL64, KERNEL_SMOOTH, KERNEL_CG_UPDATE, KERNEL_COUPLE = 0, 1, 2, 3
 KERNEL_SMOOTH_COARSE, KERNEL_TRANSFER0, KERNEL_TRANSFER_COARSE, KERNEL_BOTTOM = 4, 5, 6, 7
 KERNEL_FINISH, KERNEL_SOLVE_ONCE, KERNEL_BICG_VEC, KERNEL_COUPLE_SETUP, KERNEL_BOUNDARY = 8, 9, 10, 11, 12
@@ -54,7 +54,7 @@ EXPORTED = [
     "eq_batch_stream", "eq_batch_sync", "eq_batch_upload", "eq_batch_download",
     "eq_batch_upload_dev", "eq_batch_download_dev", "eq_batch_step", "eq_batch_problem_stats",
     "eq_batch_couple", "eq_batch_couple_dev", "eq_batch_couple_traps_dev", "eq_batch_autotune",
-    "eq_batch_set_tensor", "eq_field_set_tensor", "eq_batch_set_velocity", "eq_batch_set_wall_values", "eq_batch_set_channels",
+    "eq_batch_set_tensor", "eq_field_set_tensor", "eq_batch_cell_tensor", "eq_batch_tensor_values", "eq_batch_set_velocity", "eq_batch_set_wall_values", "eq_batch_set_channels",
     "eq_batch_channel_values",
     "eq_batch_boundary_flux", "eq_batch_set_boundary_value", "eq_batch_set_well", "eq_batch_well_state",
     "eq_batch_vertex_map",
@@ -157,6 +157,8 @@ def load_library():
     L.eq_batch_autotune.argtypes = [vp, C.c_int32, P(SolverDesc)]
     L.eq_batch_set_tensor.argtypes = [vp, C.c_int32, vp, vp, vp, C.c_size_t]
     L.eq_batch_set_velocity.argtypes = [vp, C.c_int32, vp, C.c_size_t]
+    L.eq_batch_cell_tensor.argtypes = [vp, C.c_int32, C.c_double, C.c_double, C.c_int32]
+    L.eq_batch_tensor_values.argtypes = [vp, C.c_int32, vp, vp, vp, C.c_size_t]
     L.eq_batch_set_wall_values.argtypes = [vp, C.c_int32, C.c_int32, vp, C.c_size_t]
     L.eq_batch_set_channels.argtypes = [vp, C.c_int32, P(ChannelDesc), P(ChannelDesc)]
     L.eq_batch_channel_values.argtypes = [vp, C.c_int32, vp, vp]
@@ -373,6 +375,18 @@ class FieldBatch:
             return
         a = [np.ascontiguousarray(x, dtype=np.float64) for x in (D11, D22, D12)]
         _check(load_library().eq_batch_set_tensor(self._h, prob, _ptr(a[0]), _ptr(a[1]), _ptr(a[2]), a[0].size))
+
+    def cell_tensor(self, trap: int, Dx: float = 1.0, Dy: float = 1.0, enable: bool = True):
+        """Set the trap's diffusion tensor from the cells' angles in every coupling
+        call carrying TENSOR (setDiffusionTensor), or back to constant coefficients."""
+        _check(load_library().eq_batch_cell_tensor(self._h, trap, Dx, Dy, 1 if enable else 0))
+
+    def tensor_values(self, prob: int):
+        """(D11, D22, D12) of the finest level in natural order."""
+        n = self.num_nodes(prob)
+        t = [np.empty(n) for _ in range(3)]
+        _check(load_library().eq_batch_tensor_values(self._h, prob, _ptr(t[0]), _ptr(t[1]), _ptr(t[2]), n))
+        return t
 
     def autotune(self, repeats: int = 1) -> SolverDesc:
         """Pick the fastest multigrid cycle shape on the current state (which is restored)."""

@@ -140,6 +140,8 @@ typedef struct {
 /* coupling flags */
 #define EQ_COUPLE_SAMPLE 1u  /* write per-cell means into `sampled`  */
 #define EQ_COUPLE_DEPOSIT 2u /* add q_c/|S_c| to every covered node   */
+#define EQ_COUPLE_TENSOR 4u  /* set the diffusion tensor of the traps enabled by
+                                eq_batch_cell_tensor from the cells' angles   */
 
 typedef struct eq_batch eq_batch;
 typedef struct eq_batch eq_field;
@@ -223,6 +225,23 @@ int eq_batch_channel_values(eq_batch* b, int32_t prob, double* top, double* bott
  * arrays; call between steps. */
 int eq_batch_set_tensor(eq_batch* b, int32_t prob, const double* D11, const double* D22, const double* D12,
                         size_t n);
+
+/* setDiffusionTensor of eQabm::updateCells on the device (src/abm/eQabm.cpp:241-248,
+ * 306-325, 407).  enable = 1: every species of trap `trap` takes its D11/D22/D12
+ * from the coupling calls that carry EQ_COUPLE_TENSOR: the tensor starts from
+ * (1, 1, 0) and each cell, in list order, writes on its covered nodes
+ *   D11 = Dx cos^2 a + Dy sin^2 a,  D22 = Dx sin^2 a + Dy cos^2 a,
+ *   D12 = (Dx - Dy) sin a cos a          (a = the cell's angle)
+ * so the last cell covering a node wins.  The coupling call then carries the
+ * tensor to the coarse levels (nearest fine node) and rebuilds the coarsest
+ * inverses, all on the batch stream with no host round trip.  Dx, Dy are
+ * AnisotropicDiffusion_Axial / _Transverse (src/main.cpp:569-570).  enable = 0:
+ * constant coefficients again.  Set-up call (synchronises); P1 scheme only, not
+ * with a velocity. */
+int eq_batch_cell_tensor(eq_batch* b, int32_t trap, double Dx, double Dy, int32_t enable);
+/* the finest-level tensor of problem `prob` in natural order ((nx+1)(ny+1)
+ * values each; constant coefficients read as 1, 1, 0).  Synchronises. */
+int eq_batch_tensor_values(eq_batch* b, int32_t prob, double* D11, double* D22, double* D12, size_t n);
 
 /* Advection (no counterpart in the reference's trap form; the channel-flow case of
  * the config-4 workload): x-velocity per grid row, vx[i] for row i (n = ny+1),

@@ -805,6 +805,36 @@ void orc_couple(const orc_grid* g, const orc_cell* cells, int64_t nc, int32_t ns
     }
 }
 
+void orc_cell_tensor(const orc_grid* g, const orc_cell* cells, int64_t nc, double Dx, double Dy, double* D11,
+                     double* D22, double* D12)
+{
+    /* updateCells (src/abm/eQabm.cpp:241-248): the grids start isotropic;
+     * setDiffusionTensor (306-325), called per cell in list order after its
+     * read and write (407), overwrites the cell's covered nodes. */
+    const int64_t nw = g->nodes_w, n = nw * g->nodes_h;
+    std::fill(D11, D11 + n, 1.0);
+    std::fill(D22, D22 + n, 1.0);
+    std::fill(D12, D12 + n, 0.0);
+    std::vector<int32_t> ij;
+    for (int64_t k = 0; k < nc; ++k) {
+        int32_t w[4];
+        orc_cell_window(g, &cells[k], w);
+        const int64_t cap = (int64_t)(w[1] - w[0] + 1) * (w[3] - w[2] + 1) + 1;
+        ij.resize(2 * cap);
+        const int64_t cnt = orc_cell_coverage(g, &cells[k], ij.data(), cap);
+        const double theta = cells[k].angle;
+        const double cos2t = std::cos(theta) * std::cos(theta);
+        const double sin2t = std::sin(theta) * std::sin(theta);
+        const double sincost = std::sin(theta) * std::cos(theta);
+        for (int64_t t = 0; t < cnt; ++t) {
+            const int64_t v = ij[2 * t + 1] + ij[2 * t] * nw;
+            D11[v] = Dx * cos2t + Dy * sin2t;
+            D22[v] = Dx * sin2t + Dy * cos2t;
+            D12[v] = (Dx - Dy) * sincost;
+        }
+    }
+}
+
 double orc_boundary_well(double w, double flux, double well_scaling, double decay_rate, double dt,
                          int64_t* underflows)
 {

@@ -159,6 +159,15 @@ void orc_cell_window(const orc_grid* g, const orc_cell* c, int32_t* win4);
 void orc_couple(const orc_grid* g, const orc_cell* cells, int64_t nc, int32_t nspecies,
                 double* const* fields, const double* q, double* sampled, int32_t* counts);
 
+/* setDiffusionTensor of eQabm::updateCells (src/abm/eQabm.cpp:241-248,
+ * 306-325, 407): D11, D22, D12 (natural order, nodes_w*nodes_h each) reset to
+ * (1, 1, 0), then for each cell in list order every node of S_c (the centre
+ * fallback included) takes D11 = Dx cos^2 a + Dy sin^2 a, D22 = Dx sin^2 a +
+ * Dy cos^2 a, D12 = (Dx - Dy) sin a cos a for the cell's angle a: the last
+ * cell covering a node wins.  Dx, Dy: AnisotropicDiffusion_Axial/_Transverse. */
+void orc_cell_tensor(const orc_grid* g, const orc_cell* cells, int64_t nc, double Dx, double Dy, double* D11,
+                     double* D22, double* D12);
+
 /* ---- finite-difference alternative: diffusionPETSc ----
  * One backward-Euler step of the 5-point scheme of diffusionPETSc
  * (diffuclass.cpp): interior rows (1 + 2Fx + 2Fy) u - Fx (u_W + u_E) -

@@ -123,6 +123,7 @@ def lib():
         L.orc_cell_coverage.argtypes = [P(Grid), P(Cell), ip32, C.c_int64]
         L.orc_cell_window.argtypes = [P(Grid), P(Cell), ip32]
         L.orc_couple.argtypes = [P(Grid), P(Cell), C.c_int64, C.c_int32, P(dp), dp, dp, ip32]
+        L.orc_cell_tensor.argtypes = [P(Grid), P(Cell), C.c_int64, C.c_double, C.c_double, dp, dp, dp]
         L.orc_boundary_well.restype = C.c_double
         L.orc_boundary_well.argtypes = [C.c_double] * 5 + [ip64]
         L.orc_boundary_flux.restype = C.c_double
@@ -318,3 +319,12 @@ def boundary_well(w, flux, well_scaling, decay_rate, dt):
     n = C.c_int64(0)
     w2 = lib().orc_boundary_well(w, flux, well_scaling, decay_rate, dt, C.byref(n))
     return w2, n.value
+
+
+def cell_tensor(g: Grid, cells: np.ndarray, Dx: float, Dy: float):
+    """setDiffusionTensor over the cell list: (D11, D22, D12) in natural order."""
+    arr, ptr = cells_array(cells)
+    n = g.nodes_w * g.nodes_h
+    t = [np.empty(n) for _ in range(3)]
+    lib().orc_cell_tensor(C.byref(g), ptr, arr.shape[0], Dx, Dy, _dp(t[0]), _dp(t[1]), _dp(t[2]))
+    return t

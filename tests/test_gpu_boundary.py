@@ -62,7 +62,7 @@ def test_boundary_well_steps_match_oracle(eqf, oracle, solver):
     nx, ny, W, H = 120, 60, 30.0, 15.0
     dt, lengthScaling = 0.1, 5.0
     scaling = 2.0 * 10.0 * (15.0 / lengthScaling) * (W + H)      # initBoundaryWell
-    decay = 100.0 * 60.0 / lengthScaling / W                      # flowRate / trap width
+    decay = 0.5   # boundaryDecayRate (1/s); dt * decay < 1, as setBoundaryRate requires
     Ds = (400.0, 100.0)
     descs = [eqf.field_desc(nx, ny, W, H, dt, D, 0.01, 0.0, ALL_DIRICHLET) for D in Ds]
     n = (nx + 1) * (ny + 1)

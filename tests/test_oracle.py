@@ -383,3 +383,112 @@ def test_boundary_well_closed_forms(oracle):
     assert (w, n) == (0.0, 1)
     w, n = O.boundary_well(0.2, -0.2 * s, s, k, dt)       # lands exactly on zero: no underflow
     assert (w, n) == (0.0, 0)
+
+
+def test_channel_pure_advection_of_a_linear_profile(oracle):
+    """Sign and size of the channel advection term (AdvectionDiffusion.ufl:62-64,
+    v c' w not integrated by parts): with D = 0 and no Robin ends, a linear
+    profile c = a + b x is transported exactly, c -> c - dt*v*b at every node
+    (C c = b * int(phi) and M 1 = int(phi) row by row, so the Crank-Nicolson
+    rows hold with c1 = c0 - k v b for each substep).  A flipped advection
+    sign would give c0 + dt*v*b."""
+    nx, ny, W, H, dt = 50, 4, 10.0, 1.0, 0.1
+    p = oracle.problem(nx, ny, W, H, dt, 0.0)   # D = 0: no diffusion, no flux from the trap
+    x = np.arange(nx + 1) * (W / nx)
+    a, bslope = 0.4, 0.03
+    c0 = a + bslope * x
+    u = np.random.default_rng(1).random((ny + 1) * (nx + 1))
+    for v, subs in ((20.0, 1), (-35.0, 4)):
+        ch = oracle.channel(v, 0.0, 0.0, 0.0, 0.0, 2.0, subs)
+        t1, b1 = oracle.channel_update(p, ch, ch, u, c0, 2.0 * c0)
+        assert np.allclose(t1, c0 - dt * v * bslope, rtol=0, atol=5e-13), v
+        assert np.allclose(b1, 2.0 * c0 - dt * v * 2.0 * bslope, rtol=0, atol=5e-13), v
+        assert np.abs(t1 - (c0 + dt * v * bslope)).max() > 0.05   # the other sign is far off
+
+
+def test_fd_robin_wall_holds_its_linear_steady_state(oracle):
+    """diffusionPETSc's Robin wall by ghost reflection (ApplyBoundaryConditions,
+    diffuclass.cpp:191-339): for D du/dn = -r (u - s) on the left wall and a
+    Dirichlet right wall g, u = A + B x with B = r (A - s)/D and A + B W = g is
+    a steady state of the discrete scheme (its 5-point Laplacian is zero and
+    the ghost u_-1 = u_1 - 2 h r (u_0 - s)/D lies on the same line), so one
+    backward-Euler step returns it unchanged.  A wrong factor in the Robin
+    row (the 2 h, or r/D) moves the wall column."""
+    nx, ny, W, H, dt, D = 30, 12, 3.0, 1.2, 0.1, 0.7
+    r, s, g = 1.3, 0.25, 2.0
+    walls = {oracle.LEFT: (oracle.ROBIN, 0.0, r, s), oracle.RIGHT: (oracle.DIRICHLET, g, 0, 0)}
+    p = oracle.problem(nx, ny, W, H, dt, D, walls=walls)
+    A = (g + r * s * W / D) / (1.0 + r * W / D)
+    B = r * (A - s) / D
+    x = np.tile(np.arange(nx + 1) * (W / nx), ny + 1)
+    u0 = A + B * x
+    u1 = oracle.fd_step(p, u0)
+    assert np.allclose(u1, u0, rtol=0, atol=1e-12)
+    # the same line with a Robin rate off by a factor 2 is not steady
+    B2 = 2.0 * r * (A - s) / D
+    u2 = A + B2 * x
+    assert np.abs(oracle.fd_step(p, u2) - u2).max() > 1e-3
+
+
+def test_pole_window_is_asymmetric(oracle):
+    """The search window runs between the poles of Ecoli::updatePoleCenters
+    (src/abm/Ecoli.cpp:44-56): centre + (L/2 - w/2) along the axis and
+    centre - (L/2 + w/2) against it, then +/- nodesToEdge = round(npm/2)
+    (eQabm.cpp:75, 274-290).  Hand-computed for a 4 um x 2 um rod at (5, 5),
+    npm 4: angle 0 gives columns [round(2*4) - 2, round(6*4) + 2] = [6, 26]
+    (a symmetric window would be [10, 30]); angle pi mirrors it to [14, 34];
+    rows are round(5*4) -/+ 2.  The box |dx| < 1, |dy| < 1 (strict) holds
+    columns 17..23 and rows 17..23, and the window keeps rows 18..22: 35 nodes.
+    Near the left wall the far pole is clamped to x = 0."""
+    O = oracle
+    g = O.grid(41, 41, 4.0, 10.0, 10.0)
+    assert O.cell_window(g, [5.0, 5.0, 0.0, 4.0, 2.0]) == (18, 22, 6, 26)
+    assert O.cell_window(g, [5.0, 5.0, math.pi, 4.0, 2.0]) == (18, 22, 14, 34)
+    cov = set(map(tuple, O.cell_coverage(g, [5.0, 5.0, 0.0, 4.0, 2.0]).tolist()))
+    assert cov == {(i, j) for i in range(18, 23) for j in range(17, 24)}
+    # angle pi/2: the poles lie along y; rows [round(2*4) - 2, round(6*4) + 2]
+    assert O.cell_window(g, [5.0, 5.0, math.pi / 2, 4.0, 2.0]) == (6, 26, 18, 22)
+    # centre 1.5 um from the left wall: the pole at x = 1.5 - 3 = -1.5 is clamped to 0
+    assert O.cell_window(g, [1.5, 5.0, 0.0, 4.0, 2.0]) == (18, 22, 0, 12)
+
+
+def test_cell_tensor_hand_counted(oracle):
+    """setDiffusionTensor (src/abm/eQabm.cpp:241-248, 306-325): isotropic
+    (1, 1, 0) off the cells; on the 21 nodes of a 3 x 1 um rod at angle 0,
+    (Dx, Dy, 0); turned by pi/2, (Dy, Dx, ~0); at pi/4 all three entries from
+    cos^2 = sin^2 = sin cos = 1/2 (to rounding); where two rods overlap, the
+    later one in list order wins; a rod too short for any node writes its
+    centre node (the fallback of findInteriorPoints)."""
+    O = oracle
+    g = O.grid(41, 21, 4.0, 10.0, 5.0)
+    Dx, Dy = 3.0, 0.5
+    rod = {(i, j) for i in (9, 10, 11) for j in range(17, 24)}
+    D11, D22, D12 = O.cell_tensor(g, np.array([[5.0, 2.5, 0.0, 3.0, 1.0]]), Dx, Dy)
+    on = np.zeros(41 * 21, bool)
+    for (i, j) in rod:
+        on[j + 41 * i] = True
+    assert np.all(D11[on] == Dx) and np.all(D22[on] == Dy) and np.all(D12[on] == 0.0)
+    assert np.all(D11[~on] == 1.0) and np.all(D22[~on] == 1.0) and np.all(D12[~on] == 0.0)
+    D11, D22, D12 = O.cell_tensor(g, np.array([[5.0, 2.5, math.pi / 2, 3.0, 1.0]]), Dx, Dy)
+    rod2 = [(i, j) for i in range(7, 14) for j in (19, 20, 21)]
+    for (i, j) in rod2:
+        v = j + 41 * i
+        assert math.isclose(D11[v], Dy, rel_tol=1e-15) and math.isclose(D22[v], Dx, rel_tol=1e-15)
+        assert abs(D12[v]) < 1e-15
+    D11, D22, D12 = O.cell_tensor(g, np.array([[5.0, 2.5, math.pi / 4, 3.0, 1.0]]), Dx, Dy)
+    cov = O.cell_coverage(g, [5.0, 2.5, math.pi / 4, 3.0, 1.0])
+    for (i, j) in cov:
+        v = j + 41 * i
+        assert math.isclose(D11[v], (Dx + Dy) / 2, rel_tol=1e-15) and math.isclose(D22[v], (Dx + Dy) / 2, rel_tol=1e-15)
+        assert math.isclose(D12[v], (Dx - Dy) / 2, rel_tol=1e-15)
+    # overlap: the second rod (angle pi/2 through the same centre) wins on the shared nodes
+    cells = np.array([[5.0, 2.5, 0.0, 3.0, 1.0], [5.0, 2.5, math.pi / 2, 3.0, 1.0]])
+    D11, _, _ = O.cell_tensor(g, cells, Dx, Dy)
+    shared = rod & set(rod2)
+    assert shared and all(math.isclose(D11[j + 41 * i], Dy, rel_tol=1e-15) for (i, j) in shared)
+    assert all(D11[j + 41 * i] == Dx for (i, j) in rod - set(rod2))
+    D11, _, _ = O.cell_tensor(g, cells[::-1].copy(), Dx, Dy)
+    assert all(D11[j + 41 * i] == Dx for (i, j) in rod)
+    # fallback: a zero-length rod writes only its centre node
+    D11, D22, D12 = O.cell_tensor(g, np.array([[3.3, 1.2, 0.0, 1.0, 1.0]]), Dx, Dy)
+    assert np.count_nonzero(D11 != 1.0) == 1 and D11[round(3.3 * 4) + 41 * round(1.2 * 4)] == Dx
