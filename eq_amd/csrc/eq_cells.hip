@@ -26,7 +26,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -512,6 +514,31 @@ __global__ __launch_bounds__(256) void k_cell_sample(const CellGeo* __restrict__
 
 static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
+// zero n 32-bit words (tile bounds); a kernel rather than a memset, so the
+// same sequence can sit in a captured graph
+__global__ __launch_bounds__(256) void k_zero_u32(uint32_t* __restrict__ p, int64_t n)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = 0u;
+}
+
+// The allocation-free coupling's scratch: one fixed buffer per device, held
+// from the first use of the library's code on that device until the process
+// ends.  Host code never passes `scratch` itself to a kernel (on the host it
+// names a stand-in, not device memory): each device's address comes from
+// hipGetSymbolAddress, taken once with that device current.
+constexpr size_t kFixedScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kFixedScratchBytes];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices];
+
+// a call's scratch: the batch's growable buffer, or the fixed one
+struct CoupleScratch {
+    char* base = nullptr;
+    size_t cap = 0;
+};
+
 static int ensure_scratch(eq_batch* b, size_t bytes)
 {
     if (b->cell_scratch_bytes >= bytes) return EQ_OK;
@@ -565,7 +592,8 @@ static int ensure_traps(eq_batch* b)
 // cells of traps t0 .. t0+ntr-1, concatenated; cell_off[k] (host) = first cell
 // of trap t0+k, cell_off[ntr] = total
 static int couple_impl(eq_batch* b, int32_t t0, int32_t ntr, const int64_t* cell_off, const eq_cell* d_cells,
-                       double npm, const double* d_q, double* d_sampled, int32_t* d_counts, uint32_t flags)
+                       double npm, const double* d_q, double* d_sampled, int32_t* d_counts, uint32_t flags,
+                       const CoupleScratch* fixed = nullptr)
 {
     if (!b) return set_error(EQ_ERR_INVALID, "NULL batch");
     if (ntr < 1 || t0 < 0 || t0 + ntr > b->ntraps) return set_error(EQ_ERR_INVALID, "trap index out of range");
@@ -591,8 +619,19 @@ static int couple_impl(eq_batch* b, int32_t t0, int32_t ntr, const int64_t* cell
     const int64_t nblk = (nc + kScanBlock - 1) / kScanBlock;
     const size_t blk_bytes = align256(sizeof(int64_t) * (size_t)nblk);
     const size_t head = geo_bytes + 256 + 2 * tb_bytes + blk_bytes;
-    if ((rc = ensure_scratch(b, head + 256))) return rc;
-    char* base = (char*)b->cell_scratch;
+    auto too_big = [&](size_t need) {
+        char msg[160];
+        std::snprintf(msg, sizeof(msg),
+                      "the coupling needs %zu bytes of scratch and the fixed buffer holds %zu: couple fewer traps "
+                      "per call", need, fixed->cap);
+        return set_error(EQ_ERR_INVALID, msg);
+    };
+    if (fixed) {
+        if (head + 256 > fixed->cap) return too_big(head + 256);
+    } else if ((rc = ensure_scratch(b, head + 256))) {
+        return rc;
+    }
+    char* base = fixed ? fixed->base : (char*)b->cell_scratch;
     CellGeo* geo = (CellGeo*)base;
     int64_t* d_total = (int64_t*)(base + geo_bytes);
     int64_t* d_blk = (int64_t*)(base + geo_bytes + 256 + 2 * tb_bytes);
@@ -619,7 +658,9 @@ static int couple_impl(eq_batch* b, int32_t t0, int32_t ntr, const int64_t* cell
                                      (const uint32_t*)nullptr, (uint32_t*)nullptr, (size_t)total, 0,
                                      32, s));
     const size_t need = head + 5 * pb + align256(sizeof(double) * (size_t)total * ns) + align256(sort_bytes);
-    if (need > b->cell_scratch_bytes) {
+    if (fixed) {
+        if (need > fixed->cap) return too_big(need); // nothing written outside the scratch yet
+    } else if (need > b->cell_scratch_bytes) {
         // grow (keeps nothing: recompute the geometry in the new buffer)
         if ((rc = ensure_scratch(b, need))) return rc;
         base = (char*)b->cell_scratch;
@@ -647,7 +688,8 @@ static int couple_impl(eq_batch* b, int32_t t0, int32_t ntr, const int64_t* cell
                                      (size_t)total, 0, bits, s));
     b->acct_to->bytes += 16.0 * total; // the sort's keys and values, read and written once
     b->acct_to->launches += 1;
-    EQ_HIP(hipMemsetAsync(t_start, 0, 2 * tb_bytes, s));
+    launch(b, kid, (double)(2 * tb_bytes), k_zero_u32, dim3((unsigned)((2 * tb_bytes / 4 + 255) / 256)), dim3(256), 0,
+           s, t_start, (int64_t)(2 * tb_bytes / 4));
     launch(b, kid, 4.0 * total + 8.0 * ntiles, k_tile_bounds, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
            (const uint32_t*)keys2, (int64_t)total, tile0, t_start, t_end);
     // algorithmic bytes: every node of the call's traps read (and written when
@@ -676,6 +718,24 @@ static int couple_impl(eq_batch* b, int32_t t0, int32_t ntr, const int64_t* cell
                (flags & EQ_COUPLE_SAMPLE) ? d_sampled : nullptr, d_counts);
     EQ_HIP(hipGetLastError());
     if (tens) return refresh_cell_tensors(b);
+    return EQ_OK;
+}
+
+// set-up of the coupling when the batch is created: the trap tables, and the
+// fixed scratch's address on the batch's device (current here)
+int init_coupling(eq_batch* b)
+{
+    int rc = ensure_traps(b);
+    if (rc) return rc;
+    if (b->device < 0 || b->device >= kMaxDevices)
+        return set_error(EQ_ERR_INVALID, "device index beyond the fixed scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[b->device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_check(e, "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[b->device] = p;
+    }
     return EQ_OK;
 }
 
@@ -717,6 +777,21 @@ int eq_batch_couple_traps_dev(eq_batch* b, int32_t trap0, int32_t ntraps, const 
     if (!cell_offsets) return set_error(EQ_ERR_INVALID, "NULL cell offsets");
     hipSetDevice(b->device);
     return couple_impl(b, trap0, ntraps, cell_offsets, d_cells, npm, d_q, d_sampled, d_counts, flags);
+}
+
+int eq_batch_couple_traps_dev_noalloc(eq_batch* b, int32_t trap0, int32_t ntraps, const int64_t* cell_offsets,
+                                      const eq_cell* d_cells, double npm, const double* d_q, double* d_sampled,
+                                      int32_t* d_counts, uint32_t flags)
+{
+    if (!b) return set_error(EQ_ERR_INVALID, "NULL batch");
+    if (!cell_offsets) return set_error(EQ_ERR_INVALID, "NULL cell offsets");
+    if (b->device < 0 || b->device >= kMaxDevices || !g_scratch_ptr[b->device])
+        return set_error(EQ_ERR_INVALID, "no fixed scratch on the batch's device");
+    hipSetDevice(b->device);
+    CoupleScratch fx;
+    fx.base = (char*)g_scratch_ptr[b->device];
+    fx.cap = kFixedScratchBytes;
+    return couple_impl(b, trap0, ntraps, cell_offsets, d_cells, npm, d_q, d_sampled, d_counts, flags, &fx);
 }
 
 int eq_batch_couple(eq_batch* b, int32_t trap, const eq_cell* cells, size_t nc, double npm, const double* q,

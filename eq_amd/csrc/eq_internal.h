@@ -220,6 +220,8 @@ int wells_after_step(eq_batch* b);
 int refresh_cell_tensors(eq_batch* b);
 // the coupling's trap table entry of `trap` (eq_cells.hip)
 int set_trap_tensor(eq_batch* b, int trap, double Dx, double Dy, int on);
+// the coupling's tables and the fixed scratch's device address (at batch creation)
+int init_coupling(eq_batch* b);
 } // namespace eqk
 
 namespace eqk {

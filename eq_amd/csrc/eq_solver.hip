@@ -3081,6 +3081,7 @@ int eq_batch_create(const eq_field_desc* descs, int32_t nprob, int32_t nspecies,
                 rc = set_error(EQ_ERR_NOMEM, "field table");
         }
     }
+    if (rc == EQ_OK) rc = init_coupling(b);
     if (rc != EQ_OK) {
         std::string keep = g_last_error;
         eq_batch_destroy(b);

@@ -53,7 +53,8 @@ EXPORTED = [
     "eq_batch_create", "eq_batch_destroy", "eq_batch_num_problems", "eq_batch_num_nodes",
     "eq_batch_stream", "eq_batch_sync", "eq_batch_upload", "eq_batch_download",
     "eq_batch_upload_dev", "eq_batch_download_dev", "eq_batch_step", "eq_batch_problem_stats",
-    "eq_batch_couple", "eq_batch_couple_dev", "eq_batch_couple_traps_dev", "eq_batch_autotune",
+    "eq_batch_couple", "eq_batch_couple_dev", "eq_batch_couple_traps_dev", "eq_batch_couple_traps_dev_noalloc",
+    "eq_batch_autotune",
     "eq_batch_set_tensor", "eq_field_set_tensor", "eq_batch_cell_tensor", "eq_batch_tensor_values", "eq_batch_set_velocity", "eq_batch_set_wall_values", "eq_batch_set_channels",
     "eq_batch_channel_values",
     "eq_batch_boundary_flux", "eq_batch_set_boundary_value", "eq_batch_set_well", "eq_batch_well_state",
@@ -167,6 +168,7 @@ def load_library():
                                             C.c_uint32]
     L.eq_batch_couple_dev.argtypes = [vp, C.c_int32, vp, C.c_size_t, C.c_double, vp, vp, vp,
                                       C.c_uint32]
+    L.eq_batch_couple_traps_dev_noalloc.argtypes = L.eq_batch_couple_traps_dev.argtypes
     L.eq_batch_boundary_flux.argtypes = [vp, C.c_int32, dp]
     L.eq_batch_set_boundary_value.argtypes = [vp, C.c_int32, C.c_double]
     L.eq_batch_set_well.argtypes = [vp, C.c_int32, P(WellDesc)]
@@ -337,6 +339,15 @@ class FieldBatch:
         the first cell of each trap in the concatenated device arrays, and the total."""
         off = np.ascontiguousarray(cell_offsets, dtype=np.int64)
         _check(load_library().eq_batch_couple_traps_dev(
+            self._h, trap0, off.size - 1, off.ctypes.data_as(C.POINTER(C.c_int64)), C.c_void_p(cells_ptr), npm,
+            C.c_void_p(q_ptr), C.c_void_p(sampled_ptr) if sampled_ptr else None,
+            C.c_void_p(counts_ptr) if counts_ptr else None, flags))
+
+    def couple_traps_dev_noalloc(self, trap0: int, cell_offsets, cells_ptr: int, npm: float, q_ptr: int,
+                                 sampled_ptr: int = 0, counts_ptr: int = 0, flags: int = SAMPLE | DEPOSIT):
+        """couple_traps_dev with the fixed per-device scratch (no allocation)."""
+        off = np.ascontiguousarray(cell_offsets, dtype=np.int64)
+        _check(load_library().eq_batch_couple_traps_dev_noalloc(
             self._h, trap0, off.size - 1, off.ctypes.data_as(C.POINTER(C.c_int64)), C.c_void_p(cells_ptr), npm,
             C.c_void_p(q_ptr), C.c_void_p(sampled_ptr) if sampled_ptr else None,
             C.c_void_p(counts_ptr) if counts_ptr else None, flags))

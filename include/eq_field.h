@@ -268,6 +268,21 @@ int eq_batch_couple_traps_dev(eq_batch* b, int32_t trap0, int32_t ntraps, const 
                               const eq_cell* d_cells, double npm, const double* d_q, double* d_sampled,
                               int32_t* d_counts, uint32_t flags);
 
+/* No counterpart in the reference.  eq_batch_couple_traps_dev for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): same arguments, same results, and it
+ * calls no hipMalloc, hipFree or allocating library routine.  The coupling's
+ * scratch (cell records, cell/tile pairs, sort buffers, per-pair sums) lives in
+ * one fixed 64 MiB device buffer per GPU, part of the library's device code and
+ * held until the process ends.  A call that needs more returns EQ_ERR_INVALID
+ * before it touches any field (couple fewer traps per call: about 300 000 rods
+ * of 2-4 um at 10 nodes per micron fit).  Every call on one GPU shares that
+ * GPU's buffer: calls that overlap on one GPU (other threads, other streams)
+ * race, so a caller serialises them. */
+int eq_batch_couple_traps_dev_noalloc(eq_batch* b, int32_t trap0, int32_t ntraps, const int64_t* cell_offsets,
+                                      const eq_cell* d_cells, double npm, const double* d_q, double* d_sampled,
+                                      int32_t* d_counts, uint32_t flags);
+
 /* totalBoundaryFlux = D*dt * (-int grad(u).n ds)  (src/fHSL.cpp:156-160,
  * fenics/boundary.ufl:9-12) of the current field */
 int eq_batch_boundary_flux(eq_batch* b, int32_t prob, double* total);
