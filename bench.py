@@ -458,6 +458,8 @@ def main():
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--extrapolate", type=int, default=0, choices=[0, 1],
+                    help="start each solve from 2 u_n - u_(n-1) (1) or from u_n (0)")
     ap.add_argument("--c3-steps", type=int, default=5,
                     help="timed steps of the config-3 batch (512 uneven traps), part of the variants")
     ap.add_argument("--c2-steps", type=int, default=1000,
@@ -508,7 +510,7 @@ def main():
     sampled_dev = torch.empty((trap.cells.shape[0], 2), dtype=torch.float64, device="cuda")
     torch.cuda.synchronize()
 
-    run = Run(F, trap, kinds[args.solver], cells_dev, q_dev, sampled_dev)
+    run = Run(F, trap, kinds[args.solver], cells_dev, q_dev, sampled_dev, extrapolate=args.extrapolate)
     for k in range(args.warmup):
         run.step(k)
     # timed region: graph replays, no per-launch events; the traffic counter

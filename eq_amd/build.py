@@ -36,17 +36,35 @@ FLAGS = [
 
 
 def build(verbose: bool = False) -> str:
-    """Compile every HIP source into eq_amd/lib/libeq_field.so (always rebuilds)."""
+    """Compile every HIP source into eq_amd/lib/libeq_field.so (always rebuilds).
+    The translation units compile concurrently, then link into one library."""
+    from concurrent.futures import ThreadPoolExecutor
     os.makedirs(LIB_DIR, exist_ok=True)
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    inc = ["-I", os.path.join(ROOT, "include")]
+    cflags = [f for f in FLAGS if f != "-shared"]
+
+    def compile_one(src):
+        obj = os.path.join(LIB_DIR, os.path.splitext(src)[0] + ".o")
+        cmd = [HIPCC] + cflags + inc + ["-c", os.path.join(CSRC, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        proc = subprocess.run(cmd, capture_output=True, text=True)
+        return obj, proc
+
+    with ThreadPoolExecutor(len(SOURCES)) as ex:
+        results = list(ex.map(compile_one, SOURCES))
+    for obj, proc in results:
+        if proc.returncode != 0:
+            sys.stderr.write(proc.stdout + proc.stderr)
+            raise RuntimeError(f"hipcc failed compiling {os.path.basename(obj)}")
     tmp = LIB + ".tmp"
-    cmd = [HIPCC] + FLAGS + ["-I", os.path.join(ROOT, "include"), "-o", tmp] + srcs
-    if verbose:
-        print(" ".join(cmd), flush=True)
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "--hip-link", "-o", tmp] + [o for o, _ in results]
     proc = subprocess.run(cmd, capture_output=True, text=True)
     if proc.returncode != 0:
         sys.stderr.write(proc.stdout + proc.stderr)
-        raise RuntimeError("hipcc failed building libeq_field.so")
+        raise RuntimeError("hipcc failed linking libeq_field.so")
+    for o, _ in results:
+        os.remove(o)
     os.replace(tmp, LIB)
     return LIB
 

@@ -40,6 +40,8 @@ struct CgState {
     int32_t iter, done, max_iter, flexible;
     int32_t restart;        // next beta is 0 (after a residual replacement)
     int32_t have_prev;      // the previous solution is kept: start CG from it
+    int32_t have_prev2;     // so is the one before: start from their linear extrapolation
+    int32_t extrap;         // extrapolation allowed (solver setting)
     // BiCGStab: rho = (r^, r), omega, and the pieces of omega
     double rho, omega, ts, tt;
 };
@@ -145,6 +147,7 @@ struct eq_batch {
     double* q = nullptr;
     double* z = nullptr;
     double* xprev = nullptr;    // last solution, the next solve's starting iterate
+    double* xprev2 = nullptr;   // the solution before it (extrapolated start)
     // BiCGStab vectors (allocated with the batch for the BiCGStab kinds)
     double* bs_rhat = nullptr;
     double* bs_p = nullptr;
@@ -157,6 +160,16 @@ struct eq_batch {
     void* dense = nullptr;      // dense inverses of coarsest levels (smoother precision)
     eqk::CgState* d_state = nullptr;
     eqk::CgState* h_state = nullptr; // pinned mirror
+    // iterative refinement (EQ_SOLVER_IR_GMG_CG): the inner FP32 solve's
+    // Krylov state, its correction and residual (FP32), and the refinement's
+    // scale per problem (FP64 residual norm before the inner solve)
+    eqk::CgState* d_state_in = nullptr;
+    eqk::CgState* h_state_in = nullptr;
+    float* ir_d = nullptr;
+    float* ir_r = nullptr;
+    double* ir_scale = nullptr;
+    hipGraphExec_t g_ir2 = nullptr;
+    eqk::Acct g_ir2_acct;
     int32_t last_iters = 0;
     // captured launch sequences (created on first use): two CG iterations
     // (the search-direction buffers swap back after two), and the set-up

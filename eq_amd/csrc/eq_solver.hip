@@ -240,7 +240,9 @@ __device__ __forceinline__ void tile_partial(double (&v)[NV], double* partial, i
 enum FinishOp {
     kFinRhs = 0, kFinInit = 1, kFinAp = 2, kFinUpdate = 3, kFinRz = 4, kFinTrue = 5,
     // BiCGStab
-    kFinBInit = 6, kFinBAlpha = 7, kFinBOmega = 8, kFinBUpdate = 9, kFinBRestart = 10
+    kFinBInit = 6, kFinBAlpha = 7, kFinBOmega = 8, kFinBUpdate = 9, kFinBRestart = 10,
+    // iterative refinement: the inner solve's start, the outer residual check
+    kFinIrInit = 11, kFinIrOuter = 12
 };
 
 template <typename T>
@@ -305,14 +307,16 @@ __global__ __launch_bounds__(kBlock) void k_rhs(LevelView L, const double* __res
 // edges would otherwise make the first residual ~1e4 times larger.
 template <int R0, bool TENS>
 __global__ __launch_bounds__(kBlock) void k_init(LevelView L, double* x, const double* __restrict__ xprev,
-                                                 const double* __restrict__ b, double* __restrict__ r,
-                                                 double* __restrict__ z, double* partial, const CgState* st,
-                                                 int jacobi)
+                                                 const double* __restrict__ xprev2, const double* __restrict__ b,
+                                                 double* __restrict__ r, double* __restrict__ z, double* partial,
+                                                 const CgState* st, int jacobi)
 {
     NodeCtx c;
     if (!decode<R0>(L, c)) return;
     const LevelProb& P = *c.P; // reads P.gw[wall]: a register copy would go to scratch
     const bool prev = st[c.prob].have_prev != 0;
+    // the starting iterate: the last solution, or the line through the last two
+    const bool prev2 = prev && st[c.prob].have_prev2 != 0 && st[c.prob].extrap != 0;
     const double* __restrict__ src = prev ? xprev : x;
     double v[2] = {0.0, 0.0};
     if (c.full2) {
@@ -321,6 +325,16 @@ __global__ __launch_bounds__(kBlock) void k_init(LevelView L, double* x, const d
         const LevelProb Q = *c.P;
         Window<R0, double> X;
         X.load(src, c.idx, c.pitch);
+        if (prev2) {
+            Window<R0, double> X2;
+            X2.load(xprev2, c.idx, c.pitch);
+#pragma unroll
+            for (int q = 0; q < R0 + 2; ++q) {
+                X.w[q] = 2.0 * X.w[q] - X2.w[q];
+                X.m[q] = 2.0 * X.m[q] - X2.m[q];
+                X.e[q] = 2.0 * X.e[q] - X2.e[q];
+            }
+        }
         double bl[R0];
         static_rows<R0>([&](auto rr) {
             constexpr int w = decltype(rr)::value;
@@ -354,10 +368,11 @@ __global__ __launch_bounds__(kBlock) void k_init(LevelView L, double* x, const d
             const int i = n.i, j = n.j, pt = P.pitch;
             const int64_t k = n.idx;
             // Dirichlet neighbours take their wall value (x is being set to g there)
+            auto start = [&](int64_t kk) -> double { return prev2 ? 2.0 * xprev[kk] - xprev2[kk] : src[kk]; };
             auto val = [&](int ii, int jj, int64_t kk) -> double {
-                return is_dirichlet_nb(P, ii, jj) ? dirichlet_value(P, ii, jj) : src[kk];
+                return is_dirichlet_nb(P, ii, jj) ? dirichlet_value(P, ii, jj) : start(kk);
             };
-            const double xc = src[k];
+            const double xc = start(k);
             const double xw = val(i, j - 1, k - 1), xe = val(i, j + 1, k + 1);
             const double xs = val(i - 1, j, k - pt), xn = val(i + 1, j, k + pt);
             const double xsw = val(i - 1, j - 1, k - pt - 1), xne = val(i + 1, j + 1, k + pt + 1);
@@ -376,32 +391,43 @@ __global__ __launch_bounds__(kBlock) void k_init(LevelView L, double* x, const d
     tile_partial<2>(v, partial, c.gt);
 }
 
-// keep the solution as the next step's starting iterate
+// keep the solution as the next step's starting iterate, and the one before
+// it when the start is extrapolated
 template <int R0>
 __global__ __launch_bounds__(kBlock) void k_keep(LevelView L, const double* __restrict__ x,
-                                                 double* __restrict__ xprev, CgState* st)
+                                                 double* __restrict__ xprev, double* __restrict__ xprev2, CgState* st)
 {
     NodeCtx c;
     if (!decode<R0>(L, c)) return;
-    double xv[R0];
+    const bool two = st[c.prob].extrap != 0;
+    double xv[R0], pv[R0];
     static_rows<R0>([&](auto rr) {
         constexpr int r = decltype(rr)::value;
-        if (c.i + r <= c.ny && c.j <= c.nx) xv[r] = x[c.idx + (int64_t)r * c.pitch];
+        if (c.i + r <= c.ny && c.j <= c.nx) {
+            xv[r] = x[c.idx + (int64_t)r * c.pitch];
+            if (two) pv[r] = xprev[c.idx + (int64_t)r * c.pitch];
+        }
     });
     static_rows<R0>([&](auto rr) {
         constexpr int r = decltype(rr)::value;
-        if (c.i + r <= c.ny && c.j <= c.nx) xprev[c.idx + (int64_t)r * c.pitch] = xv[r];
+        if (c.i + r <= c.ny && c.j <= c.nx) {
+            xprev[c.idx + (int64_t)r * c.pitch] = xv[r];
+            if (two) xprev2[c.idx + (int64_t)r * c.pitch] = pv[r];
+        }
     });
-    if (c.gt == c.P->tile_begin && threadIdx.x == 0) st[c.prob].have_prev = 1;
+    if (c.gt == c.P->tile_begin && threadIdx.x == 0) {
+        st[c.prob].have_prev2 = st[c.prob].have_prev;
+        st[c.prob].have_prev = 1;
+    }
 }
 
 // p_new = z + beta p_old (formed at the seven stencil points and rounded to
 // the storage type TP), q = A p_new in FP64 from the stored values, p_new.q.
 // q is exactly A times the direction x will move along, so the recurrence
 // residual tracks b - A x whatever TP is.  p_old and p_new are different buffers.
-template <int R0, typename TP, bool TENS>
+template <int R0, typename TP, bool TENS, typename TV = double>
 __global__ __launch_bounds__(kBlock) void k_ap_dir(LevelView L, const TP* __restrict__ z, const TP* __restrict__ pold,
-                                                   TP* __restrict__ pnew, double* __restrict__ q, double* partial,
+                                                   TP* __restrict__ pnew, TV* __restrict__ q, double* partial,
                                                    const CgState* st)
 {
     NodeCtx c;
@@ -413,15 +439,15 @@ __global__ __launch_bounds__(kBlock) void k_ap_dir(LevelView L, const TP* __rest
     if (s.done) return;
     const double beta = s.beta;
     double v[1] = {0.0};
-    auto dir = [beta](TP zz, TP pp) -> double { return (double)(TP)((double)zz + beta * (double)pp); };
+    auto dir = [beta](TP zz, TP pp) -> TV { return (TV)(TP)((double)zz + beta * (double)pp); };
     if (c.full && !(TENS && P.aniso)) {
         // interior tile without a tensor field: every load first, then the R rows
         Window<R0, TP> Z, Q;
         Z.load(z, c.idx, P.pitch);
         Q.load(pold, c.idx, P.pitch);
-        PlainRows<R0, double, TENS> V;
+        PlainRows<R0, TV, TENS> V;
         V.load(P, c.i);
-        double dw[R0 + 2], dm[R0 + 2], de[R0 + 2];
+        TV dw[R0 + 2], dm[R0 + 2], de[R0 + 2];
 #pragma unroll
         for (int q = 0; q < R0 + 2; ++q) {
             dw[q] = dir(Z.w[q], Q.w[q]);
@@ -431,12 +457,12 @@ __global__ __launch_bounds__(kBlock) void k_ap_dir(LevelView L, const TP* __rest
 #pragma unroll
         for (int r = 0; r < R0; ++r) {
             const int64_t k = c.idx + (int64_t)r * P.pitch;
-            const double nc = dm[r + 1];
-            const Row<double> ri = V.row(P, r, c.i, c.j);
-            const double qv = row_apply<TENS>(ri, nc, dw[r + 1], de[r + 1], dm[r], dm[r + 2], dw[r], de[r + 2]);
+            const TV nc = dm[r + 1];
+            const Row<TV> ri = V.row(P, r, c.i, c.j);
+            const TV qv = row_apply<TENS>(ri, nc, dw[r + 1], de[r + 1], dm[r], dm[r + 2], dw[r], de[r + 2]);
             pnew[k] = (TP)nc;
             q[k] = qv;
-            v[0] += nc * qv;
+            v[0] += (double)nc * (double)qv;
         }
         tile_partial<1>(v, partial, c.gt);
         return;
@@ -450,25 +476,25 @@ __global__ __launch_bounds__(kBlock) void k_ap_dir(LevelView L, const TP* __rest
         if (!((c.i + r <= c.ny) && (c.j <= c.nx))) return;
         const int i = c.i + r;
         const int64_t k = c.idx + (int64_t)r * c.pitch;
-        const double nc = dir(Z.m[r + 1], Q.m[r + 1]);
-        double qv = 0.0;
+        const TV nc = dir(Z.m[r + 1], Q.m[r + 1]);
+        TV qv = (TV)0;
         if (!is_dirichlet(P, i, c.j)) {
-            const Row<double> rc = row_coefs<double, TENS>(P, i, c.j, k);
+            const Row<TV> rc = row_coefs<TV, TENS>(P, i, c.j, k);
             qv = row_apply<TENS>(rc, nc, dir(Z.w[r + 1], Q.w[r + 1]), dir(Z.e[r + 1], Q.e[r + 1]), dir(Z.m[r], Q.m[r]),
                                  dir(Z.m[r + 2], Q.m[r + 2]), dir(Z.w[r], Q.w[r]), dir(Z.e[r + 2], Q.e[r + 2]));
         }
         pnew[k] = (TP)nc;
         q[k] = qv;
-        v[0] += nc * qv;
+        v[0] += (double)nc * (double)qv;
     });
     tile_partial<1>(v, partial, c.gt);
 }
 
 // x += alpha p; r -= alpha q; Jacobi: z = r/diag, r.z; flexible: r.z_old
-template <int R0, typename TP, bool TENS>
-__global__ __launch_bounds__(kBlock) void k_update(LevelView L, double* __restrict__ x, double* __restrict__ r,
+template <int R0, typename TP, bool TENS, typename TV = double>
+__global__ __launch_bounds__(kBlock) void k_update(LevelView L, TV* __restrict__ x, TV* __restrict__ r,
                                                    TP* __restrict__ z, const TP* __restrict__ p,
-                                                   const double* __restrict__ q, double* partial, const CgState* st,
+                                                   const TV* __restrict__ q, double* partial, const CgState* st,
                                                    int jacobi, TP* __restrict__ x1, TP* __restrict__ bcopy, TP omega)
 {
     NodeCtx c;
@@ -476,14 +502,14 @@ __global__ __launch_bounds__(kBlock) void k_update(LevelView L, double* __restri
     const LevelProb P = *c.P;
     const CgState& s0 = st[c.prob];
     if (s0.done) return;
-    const double alpha = s0.alpha;
+    const TV alpha = (TV)s0.alpha;
     const int flexible = s0.flexible;
     // one accumulator for whichever r.z this mode forms (two accumulators
     // picked by the branch end up addressed through a select, in scratch)
     double rr = 0.0, rzx = 0.0;
     if (c.full && !jacobi) {
         // interior tile: every load first, then the R rows
-        double xv[R0], rv0[R0], qv[R0];
+        TV xv[R0], rv0[R0], qv[R0];
         TP pv[R0], zv[R0];
         PlainRows<R0, TP, TENS> V;
         if (x1) V.load(P, c.i);
@@ -499,11 +525,11 @@ __global__ __launch_bounds__(kBlock) void k_update(LevelView L, double* __restri
 #pragma unroll
         for (int w = 0; w < R0; ++w) {
             const int64_t k = c.idx + (int64_t)w * P.pitch;
-            x[k] = xv[w] + alpha * (double)pv[w];
-            const double rv = rv0[w] - alpha * qv[w];
+            x[k] = xv[w] + alpha * (TV)pv[w];
+            const TV rv = rv0[w] - alpha * qv[w];
             r[k] = rv;
-            rr += rv * rv;
-            rzx += flexible ? rv * (double)zv[w] : 0.0;
+            rr += (double)rv * (double)rv;
+            rzx += flexible ? (double)rv * (double)zv[w] : 0.0;
             if (x1) {
                 // the preconditioner's first sweep from zero, as k_smooth forms it
                 TP wd;
@@ -524,7 +550,7 @@ __global__ __launch_bounds__(kBlock) void k_update(LevelView L, double* __restri
     }
     // every row's loads first, then the rows
     bool act[R0];
-    double xv[R0], rv0[R0], qv[R0];
+    TV xv[R0], rv0[R0], qv[R0];
     TP pv[R0], zv0[R0];
     static_rows<R0>([&](auto ww) {
         constexpr int w = decltype(ww)::value;
@@ -543,18 +569,18 @@ __global__ __launch_bounds__(kBlock) void k_update(LevelView L, double* __restri
         if (!act[w]) return;
         const int i = c.i + w;
         const int64_t k = c.idx + (int64_t)w * c.pitch;
-        x[k] = xv[w] + alpha * (double)pv[w];
-        const double rv = rv0[w] - alpha * qv[w];
+        x[k] = xv[w] + alpha * (TV)pv[w];
+        const TV rv = rv0[w] - alpha * qv[w];
         r[k] = rv;
-        rr += rv * rv;
+        rr += (double)rv * (double)rv;
         double t = 0.0;
         if (jacobi) {
             const Row<double> rc = row_coefs<double, TENS>(P, i, c.j, k);
-            const TP zv = (TP)(rv * row_dinv<TENS>(P, i, c.j, rc));
+            const TP zv = (TP)((double)rv * row_dinv<TENS>(P, i, c.j, rc));
             z[k] = zv;
-            t = rv * (double)zv;
+            t = (double)rv * (double)zv;
         } else if (flexible) {
-            t = rv * (double)zv0[w];
+            t = (double)rv * (double)zv0[w];
         }
         rzx += t;
         if (x1) {
@@ -613,6 +639,77 @@ __global__ __launch_bounds__(kBlock) void k_true_residual(LevelView L, const dou
 }
 
 // Jacobi preconditioner on its own (restart after a residual replacement): z = r/diag, r.z
+// Iterative refinement, start of an inner solve: the FP64 residual scaled by
+// 1/||r|| into FP32 (so the FP32 solve works near unit size whatever the
+// residual's magnitude), the correction zeroed, the scale kept, the inner
+// Krylov state reset; problems whose refinement has converged sit out.
+template <int R0>
+__global__ __launch_bounds__(kBlock) void k_ir_begin(LevelView L, const double* __restrict__ r64,
+                                                     float* __restrict__ r32, float* __restrict__ d32,
+                                                     double* __restrict__ scale, const CgState* outer, CgState* inner,
+                                                     double* partial)
+{
+    NodeCtx c;
+    if (!decode<R0>(L, c)) return;
+    const CgState& so = outer[c.prob];
+    const bool first = c.gt == c.P->tile_begin && threadIdx.x == 0;
+    if (so.done) {
+        if (first) inner[c.prob].done = 1;
+        return;
+    }
+    const double nrm = sqrt(so.rr);
+    const double inv = nrm > 0.0 ? 1.0 / nrm : 0.0;
+    double v[1] = {0.0};
+    double rv[R0];
+    static_rows<R0>([&](auto rr) {
+        constexpr int r = decltype(rr)::value;
+        if (c.i + r <= c.ny && c.j <= c.nx) rv[r] = r64[c.idx + (int64_t)r * c.pitch];
+    });
+    static_rows<R0>([&](auto rr) {
+        constexpr int r = decltype(rr)::value;
+        if (!(c.i + r <= c.ny && c.j <= c.nx)) return;
+        const int64_t k = c.idx + (int64_t)r * c.pitch;
+        const float f = (float)(rv[r] * inv);
+        r32[k] = f;
+        d32[k] = 0.0f;
+        v[0] += (double)f * (double)f;
+    });
+    if (first) {
+        scale[c.prob] = nrm;
+        CgState& si = inner[c.prob];
+        si.iter = 0;
+        si.beta = 0.0;
+        si.restart = 0;
+        si.rzold = 0.0;
+        si.done = 0;
+    }
+    tile_partial<1>(v, partial, c.gt);
+}
+
+// Iterative refinement, end of an inner solve: x += ||r|| d in FP64
+template <int R0>
+__global__ __launch_bounds__(kBlock) void k_ir_apply(LevelView L, double* __restrict__ x, const float* __restrict__ d32,
+                                                     const double* __restrict__ scale, const CgState* outer)
+{
+    NodeCtx c;
+    if (!decode<R0>(L, c)) return;
+    if (outer[c.prob].done) return;
+    const double sc = scale[c.prob];
+    double xv[R0];
+    float dv[R0];
+    static_rows<R0>([&](auto rr) {
+        constexpr int r = decltype(rr)::value;
+        if (c.i + r <= c.ny && c.j <= c.nx) {
+            xv[r] = x[c.idx + (int64_t)r * c.pitch];
+            dv[r] = d32[c.idx + (int64_t)r * c.pitch];
+        }
+    });
+    static_rows<R0>([&](auto rr) {
+        constexpr int r = decltype(rr)::value;
+        if (c.i + r <= c.ny && c.j <= c.nx) x[c.idx + (int64_t)r * c.pitch] = xv[r] + sc * (double)dv[r];
+    });
+}
+
 template <int R0, bool TENS>
 __global__ __launch_bounds__(kBlock) void k_jacobi_z(LevelView L, const double* __restrict__ r,
                                                      double* __restrict__ z, double* partial, const CgState* st)
@@ -666,7 +763,7 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(const LevelProb* __re
         v[2] += p0[2];
     }
     if ((op == kFinAp || op == kFinUpdate || op == kFinRz || op == kFinBAlpha || op == kFinBOmega ||
-         op == kFinBUpdate) && s.done)
+         op == kFinBUpdate || op == kFinIrInit) && s.done)
         return;
     block_sum<3, kFinishThreads>(v);
     if (threadIdx.x != 0) return;
@@ -717,6 +814,17 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(const LevelProb* __re
         s.rr = v[0];
         s.done = (v[0] <= s.tol2 * s.bb || s.iter >= s.max_iter) ? 1 : 0;
         s.restart = s.done ? 0 : 1;
+        break;
+    case kFinIrInit: // inner FP32 solve from the scaled residual: its norm is the reference
+        s.bb = v[0];
+        s.rr = v[0];
+        s.done = (v[0] > 0.0) ? 0 : 1;
+        break;
+    case kFinIrOuter: // the FP64 residual after a correction: one refinement step done
+        s.rr_true = v[0];
+        s.rr = v[0];
+        s.iter += 1;
+        s.done = (v[0] <= s.tol2 * s.bb || s.iter >= s.max_iter) ? 1 : 0;
         break;
     case kFinBRestart: // after kFinTrue: restarted problems begin again from r^ = r
         if (s.restart) {
@@ -1376,8 +1484,8 @@ __global__ __launch_bounds__(kBlock) void k_prolong_smooth(LevelView Lf, const L
 
 // level-0 problems whose whole V-cycle runs in one workgroup: cast the FP64
 // residual in, and the result out (with the partial r.z)
-template <int R0, typename T>
-__global__ __launch_bounds__(kBlock) void k_cast_in(LevelView L, const double* __restrict__ r, T* __restrict__ b,
+template <int R0, typename T, typename TI = double>
+__global__ __launch_bounds__(kBlock) void k_cast_in(LevelView L, const TI* __restrict__ r, T* __restrict__ b,
                                                     const CgState* st)
 {
     NodeCtx c;
@@ -1388,8 +1496,8 @@ __global__ __launch_bounds__(kBlock) void k_cast_in(LevelView L, const double* _
     });
 }
 
-template <int R0, typename T, typename TZ>
-__global__ __launch_bounds__(kBlock) void k_cast_out(LevelView L, const T* __restrict__ x, const double* __restrict__ r,
+template <int R0, typename T, typename TZ, typename TI = double>
+__global__ __launch_bounds__(kBlock) void k_cast_out(LevelView L, const T* __restrict__ x, const TI* __restrict__ r,
                                                      TZ* __restrict__ z, double* partial, const CgState* st)
 {
     NodeCtx c;
@@ -1400,7 +1508,7 @@ __global__ __launch_bounds__(kBlock) void k_cast_out(LevelView L, const T* __res
         if (!n.in) return;
         const TZ zv = (TZ)x[n.idx];
         z[n.idx] = zv;
-        v[0] += r[n.idx] * (double)zv;
+        v[0] += (double)r[n.idx] * (double)zv;
     });
     tile_partial<1>(v, partial, c.gt);
 }
@@ -2434,6 +2542,7 @@ static int build_hierarchy(eq_batch* b)
     if ((rc = alloc_zero(&b->u, n0, s)) || (rc = alloc_zero(&b->rhs, n0, s)) || (rc = alloc_zero(&b->r, n0, s)) ||
         (rc = alloc_zero(&b->p, n0, s)) || (rc = alloc_zero(&b->p2, n0, s)) || (rc = alloc_zero(&b->q, n0, s)) ||
         (rc = alloc_zero(&b->z, n0, s)) || (rc = alloc_zero(&b->xprev, n0, s)) ||
+        (S.extrapolate && (rc = alloc_zero(&b->xprev2, n0, s))) ||
         (bicg && ((rc = alloc_zero(&b->bs_rhat, n0, s)) || (rc = alloc_zero(&b->bs_p, n0, s)) ||
                   (rc = alloc_zero(&b->bs_v, n0, s)) || (rc = alloc_zero(&b->bs_s, n0, s)) ||
                   (rc = alloc_zero(&b->bs_t, n0, s)))) ||
@@ -2445,14 +2554,24 @@ static int build_hierarchy(eq_batch* b)
 
 // ------------------------------------------------------------ V-cycle
 
-template <typename T, bool TENS>
+template <typename T, bool TENS, typename TI = double>
 struct VCycle {
     eq_batch* b;
-    const double* in = nullptr; // level-0 right-hand side (FP64): the vector to precondition
+    const TI* in = nullptr;     // level-0 right-hand side: the vector to precondition (FP64, or FP32 when
+                                // it is already in the smoother's precision)
     T* out = nullptr;           // level-0 result
     bool pre1 = false;          // level 0's first sweep from zero is already in xa (and its b in L.b)
+    const CgState* st_ = nullptr; // Krylov state whose `done` the launches test (default: the batch's)
+    // an input in the smoother's precision is read directly; an FP64 one is
+    // copied to L.b by the first sweep when the smoother runs in FP32
+    static constexpr bool kDirect = std::is_same<TI, T>::value;
 
-    explicit VCycle(eq_batch* bb) : b(bb), in(bb->r), out((T*)bb->z) {}
+    explicit VCycle(eq_batch* bb) : b(bb), in(initial_in(bb)), out((T*)bb->z) {}
+    static const TI* initial_in(eq_batch* bb)
+    {
+        if constexpr (std::is_same<TI, double>::value) return bb->r;
+        else return nullptr;
+    }
 
     // z = V(r) on level 0 (and the partial r.z for the next k_finish)
     void run0() { run(0); }
@@ -2486,7 +2605,7 @@ struct VCycle {
         Level& L = b->levels[l];
         const T omega = (T)S.mg_omega;
         hipStream_t s = b->stream;
-        const CgState* st = b->d_state;
+        const CgState* st = st_ ? st_ : b->d_state;
         const bool top = (l == 0);
         const int ns = (int)L.tiles_smooth.size();
         const int gs = grid_for(ns);
@@ -2496,9 +2615,9 @@ struct VCycle {
             const double nodes = L.nodes_smooth;
             // pre-smoothing; the first sweep starts from x = 0; level 0 reads
             // the FP64 residual and (mixed precision) keeps an FP32 copy of it
-            const T* bl = top ? (b->mixed ? (const T*)L.b : (const T*)in) : (const T*)L.b;
-            T* bcopy = (top && b->mixed) ? (T*)L.b : nullptr;
-            const double bytes_b = top ? sizeof(double) : sizeof(T);
+            const T* bl = top ? ((kDirect || !b->mixed) ? (const T*)in : (const T*)L.b) : (const T*)L.b;
+            T* bcopy = (top && b->mixed && !kDirect) ? (T*)L.b : nullptr;
+            const double bytes_b = top ? sizeof(TI) : sizeof(T);
             // Small levels run at the launch-latency floor: there the fused
             // kernels (two launches' work in one, the first launch's values
             // recomputed at the stencil points) win; large levels are
@@ -2521,14 +2640,14 @@ struct VCycle {
                 if (pair) {
                     // sweeps 1 and 2 in one launch
                     if (top)
-                        launch_tiles(b, slot, by, k_presmooth2<R, T, double, TENS>, gs, s, L.smooth(), in, X[1], bcopy,
+                        launch_tiles(b, slot, by, k_presmooth2<R, T, TI, TENS>, gs, s, L.smooth(), in, X[1], bcopy,
                                     omega, st);
                     else
                         launch_tiles(b, slot, by, k_presmooth2<R, T, T, TENS>, gs, s, L.smooth(), bl, X[1], (T*)nullptr,
                                     omega, st);
                     cur = 1;
                 } else if (top) {
-                    launch_tiles(b, slot, by, k_smooth<R, T, double, T, TENS>, gs, s, L.smooth(), in, (const T*)nullptr, X[0],
+                    launch_tiles(b, slot, by, k_smooth<R, T, TI, T, TENS>, gs, s, L.smooth(), in, (const T*)nullptr, X[0],
                                 bcopy, omega, 1, 0, (double*)nullptr, st);
                 } else {
                     launch_tiles(b, slot, by, k_smooth<R, T, T, T, TENS>, gs, s, L.smooth(), bl, (const T*)nullptr, X[0],
@@ -2584,10 +2703,10 @@ struct VCycle {
                 const double by = nodes * (sizeof(T) * 2 + (last ? bytes_b : sizeof(T))) + (first ? cbytes : 0.0);
                 if (top && last) {
                     if (first)
-                        launch_tiles(b, slot, by, k_prolong_smooth<R, T, double, T, TENS>, gs, s, L.smooth(), C.d_probs, xc,
+                        launch_tiles(b, slot, by, k_prolong_smooth<R, T, TI, T, TENS>, gs, s, L.smooth(), C.d_probs, xc,
                                     in, (const T*)X[cur], out, omega, 1, b->partial, st);
                     else
-                        launch_tiles(b, slot, by, k_smooth<R, T, double, T, TENS>, gs, s, L.smooth(), in, (const T*)X[cur],
+                        launch_tiles(b, slot, by, k_smooth<R, T, TI, T, TENS>, gs, s, L.smooth(), in, (const T*)X[cur],
                                     out, (T*)nullptr, omega, 0, 1, b->partial, st);
                 } else {
                     if (first)
@@ -2606,7 +2725,7 @@ struct VCycle {
             T* xl = final_buffer(l);
             const int nb = (int)L.tiles_bottom.size();
             if (top)
-                launch_tiles(b, EQ_KERNEL_BOTTOM, L.nodes_bottom * (sizeof(double) + sizeof(T)), k_cast_in<R, T>,
+                launch_tiles(b, EQ_KERNEL_BOTTOM, L.nodes_bottom * (sizeof(TI) + sizeof(T)), k_cast_in<R, T, TI>,
                              grid_for(nb), s, L.bottom_tiles(), in, (T*)L.b, st);
             // the bottom levels live in LDS: HBM sees the level's b and x and the coarsest inverses
             launch(b, EQ_KERNEL_BOTTOM, L.nodes_bottom * 2 * sizeof(T) + L.bottom_dense_bytes, k_bottom<T, TENS>,
@@ -2614,7 +2733,7 @@ struct VCycle {
                    (const BottomRef*)L.d_bottom, (const LevelProb*)L.d_bottom_recs, (const T*)L.b, xl, omega, S.mg_pre,
                    S.mg_post, (const T*)b->dense, st);
             if (top)
-                launch_tiles(b, EQ_KERNEL_BOTTOM, L.nodes_bottom * (2 * sizeof(T) + sizeof(double)), k_cast_out<R, T, T>,
+                launch_tiles(b, EQ_KERNEL_BOTTOM, L.nodes_bottom * (2 * sizeof(T) + sizeof(TI)), k_cast_out<R, T, T, TI>,
                              grid_for(nb), s, L.bottom_tiles(), (const T*)xl, in, out, b->partial, st);
         }
     }
@@ -2687,9 +2806,9 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
         launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, fine_nodes * 16.0, k_rhs<R0>, g0, s, A, (const double*)b->u, b->rhs,
                      b->partial);
         finish(kFinRhs);
-        launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, fine_nodes * 40.0, k_init<R0, TENS>, g0, s, A, b->u,
-                     (const double*)b->xprev, (const double*)b->rhs, b->r, b->z, b->partial, b->d_state,
-                     jacobi ? 1 : 0);
+        launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, fine_nodes * (S.extrapolate ? 48.0 : 40.0), k_init<R0, TENS>, g0, s, A,
+                     b->u, (const double*)b->xprev, (const double*)b->xprev2, (const double*)b->rhs, b->r, b->z,
+                     b->partial, (const CgState*)b->d_state, jacobi ? 1 : 0);
         finish(kFinInit);
         first_sweep();
     };
@@ -2761,8 +2880,8 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
         }
         chunk = std::max(1, S.check_every);
     }
-    launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, fine_nodes * 16.0, k_keep<R0>, g0, s, A, (const double*)b->u, b->xprev,
-                 b->d_state);
+    launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, fine_nodes * (S.extrapolate ? 32.0 : 16.0), k_keep<R0>, g0, s, A,
+                 (const double*)b->u, b->xprev, b->xprev2, b->d_state);
     EQ_HIP(hipGetLastError());
     prof_collect(b);
     int32_t itmax = 0, nconv = 0;
@@ -2841,8 +2960,9 @@ static int solve_bicgstab(eq_batch* b, eq_step_stats* stats)
     auto setup = [&]() {
         launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, n0 * 16.0, k_rhs<R0>, g0, s, A, (const double*)b->u, b->rhs, b->partial);
         finish(kFinRhs);
-        launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, n0 * 40.0, k_init<R0, TENS>, g0, s, A, b->u, (const double*)b->xprev,
-                     (const double*)b->rhs, b->r, b->z, b->partial, (const CgState*)st, 0);
+        launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, n0 * (S.extrapolate ? 48.0 : 40.0), k_init<R0, TENS>, g0, s, A, b->u,
+                     (const double*)b->xprev, (const double*)b->xprev2, (const double*)b->rhs, b->r, b->z, b->partial,
+                     (const CgState*)st, 0);
         finish(kFinBInit);
         launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, n0 * 16.0, k_copy<R0>, g0, s, A, (const double*)b->r, b->bs_rhat,
                      (const CgState*)st);
@@ -2907,7 +3027,8 @@ static int solve_bicgstab(eq_batch* b, eq_step_stats* stats)
         }
         chunk = std::max(1, S.check_every);
     }
-    launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, n0 * 16.0, k_keep<R0>, g0, s, A, (const double*)b->u, b->xprev, st);
+    launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, n0 * (S.extrapolate ? 32.0 : 16.0), k_keep<R0>, g0, s, A,
+                 (const double*)b->u, b->xprev, b->xprev2, st);
     EQ_HIP(hipGetLastError());
     prof_collect(b);
     int32_t itmax = 0, nconv = 0;
@@ -2936,6 +3057,171 @@ static int solve_bicgstab(eq_batch* b, eq_step_stats* stats)
     return EQ_OK;
 }
 
+// Iterative refinement (EQ_SOLVER_IR_GMG_CG): FP64 holds only the residual
+// b - A x and the iterate x; each correction A d = r is solved in FP32 by
+// flexible CG preconditioned with the FP32 V-cycle (FP32 vectors, FP32 stencil
+// apply), from the residual scaled to unit norm, until its residual falls by
+// ir_inner_rtol; then x += ||r|| d and the FP64 residual is formed again.
+template <int R0, bool TENS>
+static int solve_ir(eq_batch* b, eq_step_stats* stats)
+{
+    using T = float;
+    const auto t0 = std::chrono::steady_clock::now();
+    const eq_solver_desc& S = b->solver;
+    Level& L0 = b->levels[0];
+    const LevelView A = L0.all();
+    const int g0 = grid_for(A.ntiles);
+    hipStream_t s = b->stream;
+    const double n0 = L0.nodes_all;
+    CgState* so = b->d_state;
+    CgState* si = b->d_state_in;
+    const double fin_bytes = (double)L0.tiles_all.size() * 3 * sizeof(double) + 2.0 * b->nprob * sizeof(CgState);
+    auto finish = [&](CgState* st, int op) {
+        launch(b, EQ_KERNEL_FINISH, fin_bytes, k_finish, dim3(b->nprob), dim3(kFinishThreads), 0, s,
+               (const LevelProb*)L0.d_probs, (const double*)b->partial, st, op, 0);
+    };
+    T* x1 = (T*)L0.xa;
+    const T omega = (T)S.mg_omega;
+    VCycle<T, TENS, T> vc(b);
+    vc.in = b->ir_r;
+    vc.out = (T*)b->z;
+    vc.st_ = si;
+    vc.pre1 = true;
+    // one inner iteration from direction pold to pnew (FP32 throughout)
+    auto iteration = [&](T* pold, T* pnew) {
+        vc.run0();
+        finish(si, kFinRz);
+        launch_tiles(b, EQ_KERNEL_STENCIL64, n0 * 4.0 * sizeof(T), k_ap_dir<R0, T, TENS, T>, g0, s, A, (const T*)b->z,
+                     (const T*)pold, pnew, (T*)b->q, b->partial, (const CgState*)si);
+        finish(si, kFinAp);
+        launch_tiles(b, EQ_KERNEL_CG_UPDATE, n0 * 7.0 * sizeof(T), k_update<R0, T, TENS, T>, g0, s, A, b->ir_d, b->ir_r,
+                     (T*)b->z, (const T*)pnew, (const T*)b->q, b->partial, (const CgState*)si, 0, x1, (T*)nullptr, omega);
+        finish(si, kFinUpdate);
+    };
+    auto two_iterations = [&]() {
+        iteration((T*)b->p, (T*)b->p2);
+        iteration((T*)b->p2, (T*)b->p);
+    };
+    auto setup = [&]() {
+        launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, n0 * 16.0, k_rhs<R0>, g0, s, A, (const double*)b->u, b->rhs, b->partial);
+        finish(so, kFinRhs);
+        launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, n0 * (S.extrapolate ? 48.0 : 40.0), k_init<R0, TENS>, g0, s, A, b->u,
+                     (const double*)b->xprev, (const double*)b->xprev2, (const double*)b->rhs, b->r, b->z, b->partial,
+                     (const CgState*)so, 0);
+        finish(so, kFinInit);
+    };
+    bool profiling = false;
+    for (int k = 0; k < EQ_KERNEL_COUNT_; ++k) profiling = profiling || b->prof[k].enabled;
+    const bool graphs = S.use_graphs && !profiling;
+    auto capture = [&](hipGraphExec_t* exec, Acct* acct, auto&& body) -> int {
+        if (*exec) return EQ_OK;
+        hipGraph_t g = nullptr;
+        EQ_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        *acct = Acct{};
+        b->acct_to = acct;
+        body();
+        b->acct_to = &b->acct;
+        const hipError_t e = hipStreamEndCapture(s, &g);
+        if (e != hipSuccess) return hip_check(e, "hipStreamEndCapture");
+        const hipError_t e2 = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (e2 != hipSuccess) {
+            *exec = nullptr;
+            return hip_check(e2, "hipGraphInstantiate");
+        }
+        return EQ_OK;
+    };
+    int rc;
+    if (graphs) {
+        if ((rc = capture(&b->g_init, &b->g_init_acct, setup)) || (rc = capture(&b->g_ir2, &b->g_ir2_acct, two_iterations)))
+            return rc;
+        EQ_HIP(hipGraphLaunch(b->g_init, s));
+        replayed(b, b->g_init_acct);
+    } else {
+        setup();
+    }
+    EQ_HIP(hipGetLastError());
+    auto read_both = [&]() -> int {
+        EQ_HIP(hipMemcpyAsync(b->h_state, so, sizeof(CgState) * b->nprob, hipMemcpyDeviceToHost, s));
+        EQ_HIP(hipMemcpyAsync(b->h_state_in, si, sizeof(CgState) * b->nprob, hipMemcpyDeviceToHost, s));
+        EQ_HIP(hipStreamSynchronize(s));
+        return EQ_OK;
+    };
+    if ((rc = read_both())) return rc;
+    std::vector<int32_t> inner_total(b->nprob, 0);
+    const int ns = (int)L0.tiles_smooth.size();
+    for (int outer = 0; outer < 1000; ++outer) {
+        bool all = true;
+        for (int p = 0; p < b->nprob; ++p) all = all && b->h_state[p].done;
+        if (all) break;
+        // FP32 correction problem from the scaled FP64 residual
+        launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, n0 * (8.0 + 2.0 * sizeof(T)), k_ir_begin<R0>, g0, s, A,
+                     (const double*)b->r, b->ir_r, b->ir_d, b->ir_scale, (const CgState*)so, si, b->partial);
+        finish(si, kFinIrInit);
+        if (ns > 0)
+            launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, L0.nodes_smooth * 2.0 * sizeof(T), k_smooth<R0, T, T, T, TENS>,
+                         grid_for(ns), s, L0.smooth(), (const T*)b->ir_r, (const T*)nullptr, x1, (T*)nullptr, omega, 1, 0,
+                         (double*)nullptr, (const CgState*)si);
+        int chunk = std::max(2, b->last_iters > 0 ? b->last_iters : S.check_every);
+        for (int guard = 0; guard < 1000000; ++guard) {
+            for (int k = 0; k < chunk; k += 2) {
+                if (graphs) {
+                    EQ_HIP(hipGraphLaunch(b->g_ir2, s));
+                    replayed(b, b->g_ir2_acct);
+                } else {
+                    two_iterations();
+                }
+            }
+            EQ_HIP(hipGetLastError());
+            if ((rc = read_both())) return rc;
+            bool inner_done = true;
+            for (int p = 0; p < b->nprob; ++p) inner_done = inner_done && b->h_state_in[p].done;
+            if (inner_done) break;
+            chunk = std::max(2, S.check_every);
+        }
+        int32_t imax = 0;
+        for (int p = 0; p < b->nprob; ++p) {
+            if (!b->h_state[p].done) inner_total[p] += b->h_state_in[p].iter;
+            imax = std::max(imax, b->h_state_in[p].iter);
+        }
+        b->last_iters = imax;
+        // x += ||r|| d, then the FP64 residual and the outer test
+        launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, n0 * (16.0 + sizeof(T)), k_ir_apply<R0>, g0, s, A, b->u,
+                     (const T*)b->ir_d, (const double*)b->ir_scale, (const CgState*)so);
+        launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, n0 * 24.0, k_true_residual<R0, TENS>, g0, s, A, (const double*)b->u,
+                     (const double*)b->rhs, b->r, b->partial);
+        finish(so, kFinIrOuter);
+        if ((rc = read_both())) return rc;
+    }
+    launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, n0 * (S.extrapolate ? 32.0 : 16.0), k_keep<R0>, g0, s, A,
+                 (const double*)b->u, b->xprev, b->xprev2, so);
+    EQ_HIP(hipGetLastError());
+    prof_collect(b);
+    int32_t itmax = 0, nconv = 0;
+    double relmax = 0.0;
+    for (int p = 0; p < b->nprob; ++p) {
+        const CgState& c = b->h_state[p];
+        const double rel = c.bb > 0 ? std::sqrt(c.rr_true / c.bb) : std::sqrt(c.rr_true);
+        b->iters[p] = inner_total[p];
+        b->relres[p] = rel;
+        itmax = std::max(itmax, inner_total[p]);
+        relmax = std::max(relmax, rel);
+        if (c.rr_true <= c.tol2 * c.bb) ++nconv;
+    }
+    if (stats) {
+        stats->iterations_max = itmax;
+        stats->nconverged = nconv;
+        stats->relres_max = relmax;
+        stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    if (nconv < b->nprob) {
+        char msg[128];
+        std::snprintf(msg, sizeof(msg), "refinement steps exhausted above rtol (relres %.3e)", relmax);
+        return set_error(EQ_ERR_NOT_CONVERGED, msg);
+    }
+    return EQ_OK;
+}
+
 // precision of the multigrid x whether any problem carries a tensor field
 static int run_solve(eq_batch* b, eq_step_stats* stats)
 {
@@ -2943,6 +3229,11 @@ static int run_solve(eq_batch* b, eq_step_stats* stats)
     if (!bicg)
         for (const LevelProb& P : b->levels[0].probs)
             if (P.vel) return set_error(EQ_ERR_INVALID, "advection makes the system nonsymmetric: use a BiCGStab solver");
+    if (b->solver.kind == EQ_SOLVER_IR_GMG_CG) {
+        if (b->levels[0].R == kRowsWide)
+            return b->any_tensor ? solve_ir<kRowsWide, true>(b, stats) : solve_ir<kRowsWide, false>(b, stats);
+        return b->any_tensor ? solve_ir<kRows0, true>(b, stats) : solve_ir<kRows0, false>(b, stats);
+    }
     // level 0's rows per thread (tile height) is a template argument of every level-0 kernel
     if (b->levels[0].R == kRowsWide) {
         if (bicg) {
@@ -2999,6 +3290,9 @@ void eq_solver_default(eq_solver_desc* s)
     s->mg_bottom_lds_bytes = (int32_t)kBottomLdsMax;
     s->tile4_min_nodes = (int32_t)kWideRowsMinNodes;
     s->tile8_min_nodes = (int32_t)kWideRows8MinNodes;
+    s->extrapolate = 0; // measured: no fewer iterations on the 2000x1000 workload (10 either way)
+    s->ir_max_outer = 20;
+    s->ir_inner_rtol = 1e-4;
 }
 
 int eq_batch_create(const eq_field_desc* descs, int32_t nprob, int32_t nspecies, const eq_solver_desc* solver,
@@ -3018,7 +3312,9 @@ int eq_batch_create(const eq_field_desc* descs, int32_t nprob, int32_t nspecies,
     }
     eq_solver_desc S;
     if (solver) S = *solver; else eq_solver_default(&S);
-    if (S.kind < EQ_SOLVER_JACOBI_CG || S.kind > EQ_SOLVER_MIXED_GMG_BICGSTAB) return set_error(EQ_ERR_INVALID, "unknown solver");
+    if (S.kind < EQ_SOLVER_JACOBI_CG || S.kind > EQ_SOLVER_IR_GMG_CG) return set_error(EQ_ERR_INVALID, "unknown solver");
+    if (S.kind == EQ_SOLVER_IR_GMG_CG && (!(S.ir_inner_rtol > 0 && S.ir_inner_rtol < 1) || S.ir_max_outer < 1))
+        return set_error(EQ_ERR_INVALID, "refinement needs 0 < ir_inner_rtol < 1 and ir_max_outer >= 1");
     if (!(S.rtol > 0) || S.max_iter < 1 || S.check_every < 1) return set_error(EQ_ERR_INVALID, "bad solver tolerances");
     if (S.kind != EQ_SOLVER_JACOBI_CG &&
         (S.mg_pre < 1 || S.mg_post < 1 || !(S.mg_omega > 0) || S.mg_coarse_max_nodes < 4 || S.mg_max_levels < 1))
@@ -3049,7 +3345,8 @@ int eq_batch_create(const eq_field_desc* descs, int32_t nprob, int32_t nspecies,
             if (!found[w]) d.wall[w] = eq_bc{EQ_BC_NEUMANN, 0, 0.0, 0.0, 0.0};
     }
     b->solver = S;
-    b->mixed = (S.kind == EQ_SOLVER_MIXED_GMG_CG || S.kind == EQ_SOLVER_MIXED_GMG_BICGSTAB) ? 1 : 0;
+    b->mixed = (S.kind == EQ_SOLVER_MIXED_GMG_CG || S.kind == EQ_SOLVER_MIXED_GMG_BICGSTAB ||
+                S.kind == EQ_SOLVER_IR_GMG_CG) ? 1 : 0;
     b->iters.assign(nprob, 0);
     b->relres.assign(nprob, 0.0);
     int rc = EQ_OK;
@@ -3065,6 +3362,29 @@ int eq_batch_create(const eq_field_desc* descs, int32_t nprob, int32_t nspecies,
             st[p].tol2 = S.rtol * S.rtol;
             st[p].max_iter = S.max_iter;
             st[p].flexible = b->mixed;
+            st[p].extrap = S.extrapolate ? 1 : 0;
+            if (S.kind == EQ_SOLVER_IR_GMG_CG) st[p].max_iter = S.ir_max_outer; // refinement steps
+        }
+        if (S.kind == EQ_SOLVER_IR_GMG_CG && rc == EQ_OK) {
+            // the inner FP32 solves: flexible CG to ir_inner_rtol (relative to the scaled residual)
+            std::vector<CgState> si(nprob);
+            for (int p = 0; p < nprob; ++p) {
+                std::memset(&si[p], 0, sizeof(CgState));
+                si[p].tol2 = S.ir_inner_rtol * S.ir_inner_rtol;
+                si[p].max_iter = S.max_iter;
+                si[p].flexible = 1;
+            }
+            const int64_t n0 = b->levels[0].pool;
+            if (hipMalloc((void**)&b->d_state_in, sizeof(CgState) * nprob) != hipSuccess ||
+                hipHostMalloc((void**)&b->h_state_in, sizeof(CgState) * nprob, hipHostMallocDefault) != hipSuccess ||
+                hipMalloc((void**)&b->ir_d, sizeof(float) * n0) != hipSuccess ||
+                hipMalloc((void**)&b->ir_r, sizeof(float) * n0) != hipSuccess ||
+                hipMalloc((void**)&b->ir_scale, sizeof(double) * nprob) != hipSuccess)
+                rc = set_error(EQ_ERR_NOMEM, "refinement buffers");
+            else if (hipMemcpy(b->d_state_in, si.data(), sizeof(CgState) * nprob, hipMemcpyHostToDevice) != hipSuccess ||
+                     hipMemset(b->ir_d, 0, sizeof(float) * n0) != hipSuccess ||
+                     hipMemset(b->ir_r, 0, sizeof(float) * n0) != hipSuccess)
+                rc = set_error(EQ_ERR_HIP, "refinement state upload");
         }
         if (hipMalloc((void**)&b->d_state, sizeof(CgState) * nprob) != hipSuccess ||
             hipHostMalloc((void**)&b->h_state, sizeof(CgState) * nprob, hipHostMallocDefault) != hipSuccess ||
@@ -3119,6 +3439,13 @@ void eq_batch_destroy(eq_batch* b)
     hipFree(b->p);
     hipFree(b->p2);
     hipFree(b->xprev);
+    hipFree(b->xprev2);
+    hipFree(b->d_state_in);
+    if (b->h_state_in) hipHostFree(b->h_state_in);
+    hipFree(b->ir_d);
+    hipFree(b->ir_r);
+    hipFree(b->ir_scale);
+    if (b->g_ir2) hipGraphExecDestroy(b->g_ir2);
     hipFree(b->bs_rhat);
     hipFree(b->bs_p);
     hipFree(b->bs_v);
@@ -3185,7 +3512,7 @@ static int copy_field(eq_batch* b, int32_t prob, const double* src, double* dst,
     const bool to_dev = (kind == hipMemcpyHostToDevice) || (dst == nullptr);
     if (to_dev) {
         // a new state from the host: the kept solution is no starting point for it
-        EQ_HIP(hipMemsetAsync(&b->d_state[prob].have_prev, 0, sizeof(int32_t), b->stream));
+        EQ_HIP(hipMemsetAsync(&b->d_state[prob].have_prev, 0, 2 * sizeof(int32_t), b->stream)); // and have_prev2
         EQ_HIP(hipMemcpy2DAsync(dev, sizeof(double) * P.pitch, src, sizeof(double) * nw, sizeof(double) * nw, nh,
                                 kind, b->stream));
     } else {
@@ -3358,9 +3685,11 @@ static void refresh_general(eq_batch* b)
         if (b->g_iter2) hipGraphExecDestroy(b->g_iter2);
         if (b->g_init) hipGraphExecDestroy(b->g_init);
         if (b->g_bicg) hipGraphExecDestroy(b->g_bicg);
+        if (b->g_ir2) hipGraphExecDestroy(b->g_ir2);
         b->g_iter2 = nullptr;
         b->g_init = nullptr;
         b->g_bicg = nullptr;
+        b->g_ir2 = nullptr;
     }
 }
 
@@ -3645,7 +3974,7 @@ int eq_batch_autotune(eq_batch* b, int32_t repeats, eq_solver_desc* chosen)
     // snapshot of everything a step changes: field, kept solution, Krylov state
     double* save = nullptr;
     eqk::CgState* save_st = nullptr;
-    if (hipMalloc((void**)&save, sizeof(double) * 2 * n0) != hipSuccess ||
+    if (hipMalloc((void**)&save, sizeof(double) * 3 * n0) != hipSuccess ||
         hipMalloc((void**)&save_st, sizeof(eqk::CgState) * b->nprob) != hipSuccess) {
         hipFree(save);
         return set_error(EQ_ERR_NOMEM, "autotune snapshot");
@@ -3657,6 +3986,9 @@ int eq_batch_autotune(eq_batch* b, int32_t repeats, eq_solver_desc* chosen)
         const double* xpf = to_save ? b->xprev : save + n0;
         EQ_HIP(hipMemcpyAsync(u, uf, sizeof(double) * n0, hipMemcpyDeviceToDevice, s));
         EQ_HIP(hipMemcpyAsync(xp, xpf, sizeof(double) * n0, hipMemcpyDeviceToDevice, s));
+        if (b->xprev2)
+            EQ_HIP(hipMemcpyAsync(to_save ? save + 2 * n0 : b->xprev2, to_save ? b->xprev2 : save + 2 * n0,
+                                  sizeof(double) * n0, hipMemcpyDeviceToDevice, s));
         EQ_HIP(hipMemcpyAsync(to_save ? save_st : b->d_state, to_save ? b->d_state : save_st,
                               sizeof(eqk::CgState) * b->nprob, hipMemcpyDeviceToDevice, s));
         EQ_HIP(hipStreamSynchronize(s));
@@ -3666,9 +3998,11 @@ int eq_batch_autotune(eq_batch* b, int32_t repeats, eq_solver_desc* chosen)
         if (b->g_iter2) hipGraphExecDestroy(b->g_iter2);
         if (b->g_init) hipGraphExecDestroy(b->g_init);
         if (b->g_bicg) hipGraphExecDestroy(b->g_bicg);
+        if (b->g_ir2) hipGraphExecDestroy(b->g_ir2);
         b->g_iter2 = nullptr;
         b->g_init = nullptr;
         b->g_bicg = nullptr;
+        b->g_ir2 = nullptr;
     };
     const eq_solver_desc orig = b->solver;
     const int32_t orig_last = b->last_iters;

@@ -23,6 +23,7 @@ NEUMANN, DIRICHLET, ROBIN = 0, 1, 2
 LEFT, RIGHT, TOP, BOTTOM = 0, 1, 2, 3
 JACOBI_CG, GMG_CG, MIXED_GMG_CG = 0, 1, 2
 GMG_BICGSTAB, MIXED_GMG_BICGSTAB = 3, 4
+IR_GMG_CG = 5
 SAMPLE, DEPOSIT, TENSOR = 1, 2, 4
 KERNEL_STENCIL64, KERNEL_SMOOTH, KERNEL_CG_UPDATE, KERNEL_COUPLE = 0, 1, 2, 3
 KERNEL_SMOOTH_COARSE, KERNEL_TRANSFER0, KERNEL_TRANSFER_COARSE, KERNEL_BOTTOM = 4, 5, 6, 7
@@ -101,7 +102,8 @@ class SolverDesc(C.Structure):
                 ("check_every", C.c_int32), ("use_graphs", C.c_int32),
                 ("mg_fuse_max_nodes", C.c_int32), ("mg_fuse_pre_max_nodes", C.c_int32),
                 ("mg_bottom_lds_bytes", C.c_int32), ("mg_fuse_pre_top", C.c_int32),
-                ("tile4_min_nodes", C.c_int32), ("tile8_min_nodes", C.c_int32)]
+                ("tile4_min_nodes", C.c_int32), ("tile8_min_nodes", C.c_int32),
+                ("extrapolate", C.c_int32), ("ir_max_outer", C.c_int32), ("ir_inner_rtol", C.c_double)]
 
 
 class StepStats(C.Structure):

@@ -96,7 +96,11 @@ typedef enum {
     EQ_SOLVER_GMG_CG = 1,       /* FP64 CG, FP64 geometric-multigrid V-cycle         */
     EQ_SOLVER_MIXED_GMG_CG = 2, /* FP64 CG, FP32 V-cycle (FP64 residual/refinement)  */
     EQ_SOLVER_GMG_BICGSTAB = 3, /* FP64 BiCGStab, FP64 V-cycle (nonsymmetric: advection) */
-    EQ_SOLVER_MIXED_GMG_BICGSTAB = 4 /* FP64 BiCGStab, FP32 V-cycle                   */
+    EQ_SOLVER_MIXED_GMG_BICGSTAB = 4, /* FP64 BiCGStab, FP32 V-cycle                  */
+    EQ_SOLVER_IR_GMG_CG = 5     /* FP64 iterative refinement around FP32 GMG-CG: the
+                                   residual b - A x and the iterate x in FP64; each
+                                   correction solved by CG in FP32 (FP32 vectors and
+                                   stencil, FP32 V-cycle) to ir_inner_rtol          */
 } eq_solver_kind;
 
 typedef struct {
@@ -121,6 +125,11 @@ typedef struct {
                                     problems) run 4 rows per thread; <= 0: 8 Mi     */
     int32_t tile8_min_nodes;     /* levels with at least this many run 8 rows per
                                     thread (level 0 runs at least 4); <= 0: 32 Mi   */
+    int32_t extrapolate;         /* 1: start each solve from 2 u_n - u_(n-1) once two
+                                    solutions are kept; 0 (default): from u_n       */
+    int32_t ir_max_outer;        /* EQ_SOLVER_IR_GMG_CG: refinement steps per solve  */
+    double ir_inner_rtol;        /* EQ_SOLVER_IR_GMG_CG: each inner FP32 solve stops at
+                                    this residual reduction (default 1e-4)          */
 } eq_solver_desc;
 
 typedef struct {

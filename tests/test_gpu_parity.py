@@ -20,11 +20,12 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden", "p1_tiny.npz")
-SOLVERS = ["jacobi", "gmg", "mixed"]
+SOLVERS = ["jacobi", "gmg", "mixed", "ir"]
 
 
 def _kind(field, name):
-    return {"jacobi": field.JACOBI_CG, "gmg": field.GMG_CG, "mixed": field.MIXED_GMG_CG}[name]
+    return {"jacobi": field.JACOBI_CG, "gmg": field.GMG_CG, "mixed": field.MIXED_GMG_CG,
+            "ir": field.IR_GMG_CG}[name]
 
 
 def _walls_from(arr):
@@ -241,6 +242,7 @@ def test_full_size_trap_residual_and_solver_agreement(eqf, oracle):
         sols[solver] = x
     assert _rel(sols["gmg"], sols["jacobi"]) <= 1e-9
     assert _rel(sols["mixed"], sols["jacobi"]) <= 1e-9
+    assert _rel(sols["ir"], sols["jacobi"]) <= 1e-9
 
 
 def test_degenerate_inputs(eqf, oracle):
