@@ -273,51 +273,120 @@ def load_traffic(symbol):
         return None
 
 
-def config2_drift(F, W, cells_dev, q_dev, nsteps):
+class OracleLU:
+    """The reference step with a direct solve for one species of the
+    2000x1000 trap: the oracle assembles A (element tensors, Robin, reaction)
+    and replaces the Dirichlet rows; SuperLU factors it once (the operator is
+    the same every step), in a background thread, so later steps cost one
+    triangular solve each.  The cell loop is the oracle's sequential one."""
+
+    def __init__(self, trap, species):
+        import threading
+        self.trap, self.s = trap, species
+        self.lu = None
+        self.thread = threading.Thread(target=self._factor, daemon=True)
+        self.thread.start()
+
+    def _factor(self):
+        import scipy.sparse.linalg as sla
+        from oracle import oracle as O
+        from eq_amd import workloads as W
+        t = self.trap
+        self.O = O
+        self.p = O.problem(t.nx, t.ny, t.width, t.height, 0.1, t.D[self.s], t.gamma[self.s], 0.0,
+                           walls=W.TOP_DIRICHLET)
+        n = O.num_nodes(self.p)
+        self.rp, self.c, self.v = O.assemble(self.p)
+        v2, _ = O.apply_dirichlet(self.p, self.rp, self.c, self.v, np.zeros(n))
+        self.lu = sla.splu(O.to_scipy(self.rp, self.c, v2, n).tocsc(), permc_spec="COLAMD")
+
+    def step(self, u):
+        self.thread.join()
+        b = self.O.assemble_rhs(self.p, u)
+        _, b2 = self.O.apply_dirichlet(self.p, self.rp, self.c, self.v, b)
+        return self.lu.solve(b2)
+
+
+def _errors(a, b):
+    return {"rel_l2": float(np.linalg.norm(a - b) / np.linalg.norm(b)), "max_abs": float(np.abs(a - b).max()),
+            "max_rel": float(np.abs(a - b).max() / np.abs(b).max())}
+
+
+def config2_drift(F, W, cells_host, cells_dev, q_dev, nsteps, oracle_lu=None):
     """Config 2: the config-1 trap started from u0 ~ U[0,1] (seed 2), stepped
-    independently by the mixed solver and by the FP64 GMG-CG solver at
-    rtol 1e-13 (at most 30 iterations: its rounding floor, ~3e-12, is reached
-    well before), with the same cells; per-species error of the mixed fields
-    against the FP64 fields."""
+    independently by the mixed solver, by the refinement solver (FP32 inner
+    solves) and by the FP64 GMG-CG solver at rtol 1e-13 (at most 30
+    iterations: its rounding floor, ~3e-12, is reached well before), with the
+    same cells.  At checkpoints: each solver's fields against the FP64 run
+    (the drift of the trajectory), and -- when oracle_lu is given -- each
+    solver's step against the oracle's direct solve from the same input
+    (per-step error, the input being that solver's own state; computed by
+    finalize() once the factorisations are done)."""
     import torch
     trap2 = W.config1(u0_seed=2)
     nc = trap2.cells.shape[0]
-    s_ref = torch.empty((nc, 2), dtype=torch.float64, device="cuda")
-    s_mix = torch.empty((nc, 2), dtype=torch.float64, device="cuda")
-    ref = Run(F, trap2, F.GMG_CG, cells_dev, q_dev, s_ref, rtol=1e-13, max_iter=30)
-    mix = Run(F, trap2, F.MIXED_GMG_CG, cells_dev, q_dev, s_mix)
+    q = trap2.rates * 0.1
+    samp = {k: torch.empty((nc, 2), dtype=torch.float64, device="cuda") for k in ("ref", "mixed", "ir")}
+    runs = {"ref": Run(F, trap2, F.GMG_CG, cells_dev, q_dev, samp["ref"], rtol=1e-13, max_iter=30),
+            "mixed": Run(F, trap2, F.MIXED_GMG_CG, cells_dev, q_dev, samp["mixed"]),
+            "ir": Run(F, trap2, F.IR_GMG_CG, cells_dev, q_dev, samp["ir"])}
     checkpoints = sorted({1, 2, 5, 10, 20, 50, 100, 200, 500, 1000, nsteps})
-    hist = []
-    t_mix = 0.0
+    oracle_points = sorted({1, 10, 100, nsteps})
+    hist, snaps = [], []
+    t_run = {"mixed": 0.0, "ir": 0.0}
     for k in range(nsteps):
-        ref.step(k, check=False)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        mix.step(k)
-        torch.cuda.synchronize()
-        t_mix += time.perf_counter() - t0
+        runs["ref"].step(k, check=False)
+        for name in ("mixed", "ir"):
+            before = [runs[name].fb.download(s) for s in range(2)] if (oracle_lu and k + 1 in oracle_points) else None
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            runs[name].step(k)
+            torch.cuda.synchronize()
+            t_run[name] += time.perf_counter() - t0
+            if before is not None:
+                snaps.append({"solver": name, "step": k + 1, "before": before, "cells": cells_host[k].copy(),
+                              "after": [runs[name].fb.download(s) for s in range(2)]})
         if k + 1 in checkpoints:
-            rec = {"step": k + 1, "species": []}
-            for s in range(2):
-                a, b = mix.fb.download(s), ref.fb.download(s)
-                rec["species"].append({"rel_l2": float(np.linalg.norm(a - b) / np.linalg.norm(b)),
-                                       "max_abs": float(np.abs(a - b).max()),
-                                       "max_rel": float(np.abs(a - b).max() / np.abs(b).max())})
-            sm, sr = s_mix.cpu().numpy(), s_ref.cpu().numpy()
-            rec["sampled_max_rel"] = float(np.abs(sm - sr).max() / max(np.abs(sr).max(), 1e-300))
+            rec = {"step": k + 1}
+            ref_f = [runs["ref"].fb.download(s) for s in range(2)]
+            for name in ("mixed", "ir"):
+                rec[name] = [_errors(runs[name].fb.download(s), ref_f[s]) for s in range(2)]
+            sr = samp["ref"].cpu().numpy()
+            rec["sampled_max_rel"] = {n: float(np.abs(samp[n].cpu().numpy() - sr).max() / max(np.abs(sr).max(), 1e-300))
+                                      for n in ("mixed", "ir")}
             hist.append(rec)
-            log(f"bench: config 2 step {k + 1}: rel L2 " +
-                ", ".join(f"{e['rel_l2']:.2e}" for e in rec["species"]))
+            log(f"bench: config 2 step {k + 1}: rel L2 vs FP64 run: mixed " +
+                ", ".join(f"{e['rel_l2']:.2e}" for e in rec["mixed"]) + "; refinement " +
+                ", ".join(f"{e['rel_l2']:.2e}" for e in rec["ir"]))
     res = {"workload": "config-1 trap, u0 ~ U[0,1] seed 2, 5000 random-walk rods",
            "steps": nsteps,
            "reference": "FP64 GMG-CG, rtol 1e-13 capped at 30 iterations (its rounding floor), stepped independently",
-           "mixed_value": trap2.nodes * 2 * nsteps / t_mix,
-           "mixed_iterations_max": int(max(mix.iters)), "mixed_relres_max": float(max(mix.relres)),
-           "reference_relres_max": float(max(ref.relres)),
-           "errors": hist}
-    ref.fb.close()
-    mix.fb.close()
-    return res
+           "mixed_value": trap2.nodes * 2 * nsteps / t_run["mixed"],
+           "ir_value": trap2.nodes * 2 * nsteps / t_run["ir"],
+           "mixed_iterations_max": int(max(runs["mixed"].iters)), "mixed_relres_max": float(max(runs["mixed"].relres)),
+           "ir_inner_iterations_max": int(max(runs["ir"].iters)), "ir_relres_max": float(max(runs["ir"].relres)),
+           "reference_relres_max": float(max(runs["ref"].relres)),
+           "drift_vs_fp64_run": hist}
+    for r in runs.values():
+        r.fb.close()
+
+    def finalize():
+        """per-step errors against the oracle's direct solve, from each solver's own input state"""
+        from oracle import oracle as O
+        grid = O.grid(trap2.nx + 1, trap2.ny + 1, trap2.npm, trap2.width, trap2.height)
+        out = []
+        for sn in snaps:
+            fields, _, _ = O.couple(grid, sn["cells"], sn["before"], q)
+            ref = [oracle_lu[s].step(fields[s]) for s in range(2)]
+            out.append({"solver": sn["solver"], "step": sn["step"],
+                        "species": [_errors(sn["after"][s], ref[s]) for s in range(2)]})
+        res["per_step_vs_oracle_lu"] = out
+        res["per_step_vs_oracle_lu_max_rel_l2"] = max(e["rel_l2"] for o in out for e in o["species"]) if out else None
+        log("bench: config 2 per-step error vs the oracle's LU: max rel L2 "
+            f"{res['per_step_vs_oracle_lu_max_rel_l2']:.2e}" if out else "bench: config 2: no oracle checkpoints")
+        return res
+
+    return res, finalize
 
 
 def config3_batched(F, W, nsteps, warmup=2, ntraps=512, sigma=0.05, prof=2):
@@ -453,7 +522,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--solver", default="mixed", choices=["mixed", "gmg", "jacobi"])
+    ap.add_argument("--solver", default="mixed", choices=["mixed", "gmg", "jacobi", "ir"])
     ap.add_argument("--variant-steps", type=int, default=5)
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -464,6 +533,9 @@ def main():
                     help="timed steps of the config-3 batch (512 uneven traps), part of the variants")
     ap.add_argument("--c2-steps", type=int, default=1000,
                     help="steps of the config-2 drift comparison (mixed vs FP64), part of the variants")
+    ap.add_argument("--c2-oracle", type=int, default=1, choices=[0, 1],
+                    help="per-step errors of config 2 against the oracle's direct solve (SuperLU factorisation in "
+                         "background threads, ~80 s and ~6 GB per species)")
     ap.add_argument("--c4-steps", type=int, default=3,
                     help="timed steps of config 4 (8192x4096 trap, 4 species, advection), part of the variants")
     ap.add_argument("--only", choices=["c3", "c4"], default=None,
@@ -499,7 +571,7 @@ def main():
                 "data": "synthetic", "config": {"workload": res["workload"]}, "detail": res}
         print(json.dumps(line), flush=True)
         return
-    kinds = {"mixed": F.MIXED_GMG_CG, "gmg": F.GMG_CG, "jacobi": F.JACOBI_CG}
+    kinds = {"mixed": F.MIXED_GMG_CG, "gmg": F.GMG_CG, "jacobi": F.JACOBI_CG, "ir": F.IR_GMG_CG}
     trap = W.config1()
     nsteps = args.warmup + args.steps
     log(f"bench: building {nsteps} steps of cell positions for {trap.cells.shape[0]} cells")
@@ -554,13 +626,15 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.solver != "mixed" else "f64 (outer) / f32 (multigrid)",
+        "dtype": {"mixed": "f64 (Krylov) / f32 (multigrid)", "ir": "f64 (refinement) / f32 (inner Krylov, multigrid)"}
+                 .get(args.solver, "f64"),
         "data": "synthetic",
         "config": {"workload": "single 100x50 um trap, 2000x1000 P1 mesh (2 003 001 nodes/species), 2 species, "
                                "5000 random-walk rods, dt 0.1 s, top Dirichlet, rtol 1e-10",
                    "solver": {"mixed": "FP64 CG + FP32 GMG V(2,2) preconditioner",
                               "gmg": "FP64 CG + FP64 GMG V(2,2) preconditioner",
-                              "jacobi": "FP64 Jacobi-CG"}[args.solver],
+                              "jacobi": "FP64 Jacobi-CG",
+                              "ir": "FP64 iterative refinement around FP32 GMG-CG (FP32 vectors and stencil)"}[args.solver],
                    "species": 2, "nodes_per_species": trap.nodes, "cells": int(trap.cells.shape[0]),
                    "parallelism": "single GPU, species batched in one launch per kernel"},
         "iterations": {"max": int(max(iters)), "mean": float(np.mean(iters))},
@@ -573,8 +647,11 @@ def main():
     }
 
     if not args.no_variants:
+        # the oracle's direct solves for config 2's checkpoints factor in the
+        # background while the GPU sections run
+        oracle_lu = [OracleLU(W.config1(u0_seed=2), s) for s in range(2)] if args.c2_oracle else None
         variants = {}
-        for name in ("gmg", "jacobi", "mixed"):
+        for name in ("gmg", "jacobi", "mixed", "ir"):
             if name == args.solver:
                 continue
             r = Run(F, trap, kinds[name], cells_dev, q_dev, sampled_dev, max_iter=20000,
@@ -597,11 +674,26 @@ def main():
             r.fb.close()
             log(f"bench: variant {name}: {variants[name]['value']:.4e} DOF-timesteps/s")
         out["variants"] = variants
-        out["config2"] = config2_drift(F, W, cells_dev, q_dev, args.c2_steps)
+        # the mixed solver against the FP64 GPU solvers of the same workload
+        fp64 = variants.get("gmg", {}).get("value") if args.solver != "gmg" else value
+        jac = variants.get("jacobi", {}).get("value")
+        mixv = value if args.solver == "mixed" else variants.get("mixed", {}).get("value")
+        irv = variants.get("ir", {}).get("value")
+        out["speedup_vs_fp64"] = {
+            "mixed_over_fp64_gmg_cg": mixv / fp64 if (mixv and fp64) else None,
+            "mixed_over_fp64_jacobi_cg": mixv / jac if (mixv and jac) else None,
+            "refinement_over_fp64_gmg_cg": irv / fp64 if (irv and fp64) else None,
+            "refinement_over_fp64_jacobi_cg": irv / jac if (irv and jac) else None}
+        c2, c2_finalize = config2_drift(F, W, cells_all, cells_dev, q_dev, args.c2_steps, oracle_lu)
+        out["config2"] = c2
         if args.c3_steps > 0:
             out["config3"] = config3_batched(F, W, args.c3_steps)
         if args.c4_steps > 0:
             out["config4"] = config4_advection(F, W, args.c4_steps)
+        if oracle_lu:
+            log("bench: config 2 per-step errors against the oracle's direct solve")
+            out["config2"] = c2_finalize()
+            del oracle_lu
 
     if not args.no_cpu_baseline:
         jac_iters = out.get("variants", {}).get("jacobi", {}).get("step0_iterations")

@@ -647,7 +647,7 @@ template <int R0>
 __global__ __launch_bounds__(kBlock) void k_ir_begin(LevelView L, const double* __restrict__ r64,
                                                      float* __restrict__ r32, float* __restrict__ d32,
                                                      double* __restrict__ scale, const CgState* outer, CgState* inner,
-                                                     double* partial)
+                                                     double* partial, double inner_rtol)
 {
     NodeCtx c;
     if (!decode<R0>(L, c)) return;
@@ -677,6 +677,12 @@ __global__ __launch_bounds__(kBlock) void k_ir_begin(LevelView L, const double* 
     if (first) {
         scale[c.prob] = nrm;
         CgState& si = inner[c.prob];
+        // reduce the residual by inner_rtol, or by no more than the outer
+        // tolerance still asks (with a margin of 3): the last correction is
+        // not solved past what the stop test needs
+        const double need = (so.rr > 0.0) ? 0.3 * sqrt(so.tol2 * so.bb / so.rr) : 1.0;
+        const double tgt = fmin(0.5, fmax(inner_rtol, need));
+        si.tol2 = tgt * tgt;
         si.iter = 0;
         si.beta = 0.0;
         si.restart = 0;
@@ -3156,7 +3162,8 @@ static int solve_ir(eq_batch* b, eq_step_stats* stats)
         if (all) break;
         // FP32 correction problem from the scaled FP64 residual
         launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, n0 * (8.0 + 2.0 * sizeof(T)), k_ir_begin<R0>, g0, s, A,
-                     (const double*)b->r, b->ir_r, b->ir_d, b->ir_scale, (const CgState*)so, si, b->partial);
+                     (const double*)b->r, b->ir_r, b->ir_d, b->ir_scale, (const CgState*)so, si, b->partial,
+                     S.ir_inner_rtol);
         finish(si, kFinIrInit);
         if (ns > 0)
             launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, L0.nodes_smooth * 2.0 * sizeof(T), k_smooth<R0, T, T, T, TENS>,

@@ -128,8 +128,9 @@ typedef struct {
     int32_t extrapolate;         /* 1: start each solve from 2 u_n - u_(n-1) once two
                                     solutions are kept; 0 (default): from u_n       */
     int32_t ir_max_outer;        /* EQ_SOLVER_IR_GMG_CG: refinement steps per solve  */
-    double ir_inner_rtol;        /* EQ_SOLVER_IR_GMG_CG: each inner FP32 solve stops at
-                                    this residual reduction (default 1e-4)          */
+    double ir_inner_rtol;        /* EQ_SOLVER_IR_GMG_CG: each inner FP32 solve reduces its
+                                    residual by this much (default 1e-4), or by less
+                                    when less reaches rtol                          */
 } eq_solver_desc;
 
 typedef struct {
