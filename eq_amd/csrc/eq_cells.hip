@@ -514,6 +514,81 @@ __global__ __launch_bounds__(256) void k_cell_sample(const CellGeo* __restrict__
 
 static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
+// The coupling of a cell whose amount depends on its own sample, q = a + b e
+// (a strain's membrane exchange, src/Strain.h:312-313): per trap, one
+// workgroup walks the cells in list order; for each cell every thread scans
+// its share of the cell's window, the workgroup sums the covered nodes of
+// every species (read before the cell's own write, after every earlier
+// cell's), forms e = sum/|S_c| and q, and the covered nodes take q/|S_c|
+// (src/abm/eQabm.cpp:326-359, 384-405).  A barrier separates each cell's
+// reads from its writes and its writes from the next cell's reads.
+constexpr int kAThreads = 1024;
+__global__ __launch_bounds__(kAThreads) void k_couple_affine(const TrapGrid* __restrict__ traps, int t0,
+                                                             const int64_t* __restrict__ off,
+                                                             const CellGeo* __restrict__ geo,
+                                                             double* const* __restrict__ fields, int ns,
+                                                             const double* __restrict__ qa, const double* __restrict__ qb,
+                                                             double* __restrict__ sampled, int32_t* __restrict__ counts,
+                                                             double npm)
+{
+    __shared__ double red[kAThreads / 64][kSpeciesPass];
+    __shared__ double dep[kSpeciesPass];
+    const int t = blockIdx.x;
+    const TrapGrid G = traps[t0 + t];
+    const int64_t c0 = off[t], c1 = off[t + 1];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int64_t c = c0; c < c1; ++c) {
+        const CellGeo g = geo[c];
+        const int cnt = g.cnt > 0 ? g.cnt : 1;
+        const int wj = g.j2 - g.j1 + 1, wi = g.i2 - g.i1 + 1;
+        const int nwin = (wi > 0 && wj > 0) ? wi * wj : 0;
+        if (counts && threadIdx.x == 0) counts[c] = cnt;
+        for (int s0 = 0; s0 < ns; s0 += kSpeciesPass) {
+            const int nsp = min(kSpeciesPass, ns - s0);
+            double sum[kSpeciesPass];
+#pragma unroll
+            for (int s = 0; s < kSpeciesPass; ++s) sum[s] = 0.0;
+            for (int k = threadIdx.x; k < nwin; k += kAThreads) {
+                const int i = g.i1 + k / wj, j = g.j1 + k % wj;
+                const bool in = g.cnt == 0 ||
+                                inside_cell(g.cx, g.cy, g.cs, g.sn, g.o, g.rad, (double)j / npm, (double)i / npm);
+                if (!in) continue;
+                const int64_t q = (int64_t)i * G.pitch + j;
+#pragma unroll
+                for (int s = 0; s < kSpeciesPass; ++s)
+                    if (s < nsp) sum[s] += fields[G.prob0 + s0 + s][q];
+            }
+#pragma unroll
+            for (int s = 0; s < kSpeciesPass; ++s) {
+                const double v = wave_sum64(sum[s]);
+                if (lane == 0) red[w][s] = v;
+            }
+            __syncthreads();
+            if (threadIdx.x < (unsigned)nsp) {
+                const int s = threadIdx.x;
+                double tot = 0.0;
+                for (int q4 = 0; q4 < kAThreads / 64; ++q4) tot += red[q4][s];
+                const double e = tot / (double)cnt;
+                if (sampled) sampled[c * ns + s0 + s] = e;
+                const double amount = qa[c * ns + s0 + s] + qb[c * ns + s0 + s] * e;
+                dep[s] = amount / (double)cnt;
+            }
+            __syncthreads();
+            for (int k = threadIdx.x; k < nwin; k += kAThreads) {
+                const int i = g.i1 + k / wj, j = g.j1 + k % wj;
+                const bool in = g.cnt == 0 ||
+                                inside_cell(g.cx, g.cy, g.cs, g.sn, g.o, g.rad, (double)j / npm, (double)i / npm);
+                if (!in) continue;
+                const int64_t q = (int64_t)i * G.pitch + j;
+#pragma unroll
+                for (int s = 0; s < kSpeciesPass; ++s)
+                    if (s < nsp) fields[G.prob0 + s0 + s][q] += dep[s];
+            }
+            __syncthreads(); // this cell's writes before the next cell's reads
+        }
+    }
+}
+
 // zero n 32-bit words (tile bounds); a kernel rather than a memset, so the
 // same sequence can sit in a captured graph
 __global__ __launch_bounds__(256) void k_zero_u32(uint32_t* __restrict__ p, int64_t n)
@@ -777,6 +852,42 @@ int eq_batch_couple_traps_dev(eq_batch* b, int32_t trap0, int32_t ntraps, const 
     if (!cell_offsets) return set_error(EQ_ERR_INVALID, "NULL cell offsets");
     hipSetDevice(b->device);
     return couple_impl(b, trap0, ntraps, cell_offsets, d_cells, npm, d_q, d_sampled, d_counts, flags);
+}
+
+int eq_batch_couple_affine_dev(eq_batch* b, int32_t trap0, int32_t ntraps, const int64_t* cell_offsets,
+                               const eq_cell* d_cells, double npm, const double* d_a, const double* d_b,
+                               double* d_sampled, int32_t* d_counts)
+{
+    if (!b) return set_error(EQ_ERR_INVALID, "NULL batch");
+    if (!cell_offsets) return set_error(EQ_ERR_INVALID, "NULL cell offsets");
+    if (ntraps < 1 || trap0 < 0 || trap0 + ntraps > b->ntraps) return set_error(EQ_ERR_INVALID, "trap index out of range");
+    if (!(npm > 0)) return set_error(EQ_ERR_INVALID, "npm must be > 0");
+    for (int k = 0; k < ntraps; ++k)
+        if (cell_offsets[k + 1] < cell_offsets[k]) return set_error(EQ_ERR_INVALID, "cell offsets must not decrease");
+    const int64_t nc = cell_offsets[ntraps] - cell_offsets[0];
+    if (nc == 0) return EQ_OK;
+    if (nc > ((int64_t)1 << 30)) return set_error(EQ_ERR_INVALID, "too many cells");
+    if (!d_cells || !d_a || !d_b) return set_error(EQ_ERR_INVALID, "NULL cells, a or b");
+    hipSetDevice(b->device);
+    int rc = ensure_traps(b);
+    if (rc) return rc;
+    hipStream_t s = b->stream;
+    for (int k = 0; k <= ntraps; ++k) b->h_cell_off[k] = cell_offsets[k] - cell_offsets[0];
+    EQ_HIP(hipMemcpyAsync(b->d_cell_off, b->h_cell_off, sizeof(int64_t) * (ntraps + 1), hipMemcpyHostToDevice, s));
+    const size_t geo_bytes = align256(sizeof(CellGeo) * nc);
+    if ((rc = ensure_scratch(b, geo_bytes))) return rc;
+    CellGeo* geo = (CellGeo*)b->cell_scratch;
+    const TrapGrid* traps = (const TrapGrid*)b->d_cell_traps;
+    const int ns = b->nspecies;
+    launch(b, EQ_KERNEL_COUPLE_SETUP, (double)sizeof(eq_cell) * nc + (double)sizeof(CellGeo) * nc, k_cell_geo,
+           dim3((unsigned)((nc + 3) / 4)), dim3(256), 0, s, d_cells, (int64_t)nc, (const int64_t*)b->d_cell_off,
+           (int)ntraps, (int)trap0, traps, npm, geo);
+    // every covered node read and written once per species, the cells' records and amounts
+    launch(b, EQ_KERNEL_COUPLE, (double)sizeof(CellGeo) * nc + 24.0 * ns * nc, k_couple_affine, dim3((unsigned)ntraps),
+           dim3(kAThreads), 0, s, traps, (int)trap0, (const int64_t*)b->d_cell_off, (const CellGeo*)geo,
+           (double* const*)b->d_field_ptrs, ns, d_a, d_b, d_sampled, d_counts, npm);
+    EQ_HIP(hipGetLastError());
+    return EQ_OK;
 }
 
 int eq_batch_couple_traps_dev_noalloc(eq_batch* b, int32_t trap0, int32_t ntraps, const int64_t* cell_offsets,

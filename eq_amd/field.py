@@ -55,6 +55,7 @@ EXPORTED = [
     "eq_batch_stream", "eq_batch_sync", "eq_batch_upload", "eq_batch_download",
     "eq_batch_upload_dev", "eq_batch_download_dev", "eq_batch_step", "eq_batch_problem_stats",
     "eq_batch_couple", "eq_batch_couple_dev", "eq_batch_couple_traps_dev", "eq_batch_couple_traps_dev_noalloc",
+    "eq_batch_couple_affine_dev",
     "eq_batch_autotune",
     "eq_batch_set_tensor", "eq_field_set_tensor", "eq_batch_cell_tensor", "eq_batch_tensor_values", "eq_batch_set_velocity", "eq_batch_set_wall_values", "eq_batch_set_channels",
     "eq_batch_channel_values",
@@ -171,6 +172,7 @@ def load_library():
     L.eq_batch_couple_dev.argtypes = [vp, C.c_int32, vp, C.c_size_t, C.c_double, vp, vp, vp,
                                       C.c_uint32]
     L.eq_batch_couple_traps_dev_noalloc.argtypes = L.eq_batch_couple_traps_dev.argtypes
+    L.eq_batch_couple_affine_dev.argtypes = [vp, C.c_int32, C.c_int32, P(C.c_int64), vp, C.c_double, vp, vp, vp, vp]
     L.eq_batch_boundary_flux.argtypes = [vp, C.c_int32, dp]
     L.eq_batch_set_boundary_value.argtypes = [vp, C.c_int32, C.c_double]
     L.eq_batch_set_well.argtypes = [vp, C.c_int32, P(WellDesc)]
@@ -353,6 +355,15 @@ class FieldBatch:
             self._h, trap0, off.size - 1, off.ctypes.data_as(C.POINTER(C.c_int64)), C.c_void_p(cells_ptr), npm,
             C.c_void_p(q_ptr), C.c_void_p(sampled_ptr) if sampled_ptr else None,
             C.c_void_p(counts_ptr) if counts_ptr else None, flags))
+
+    def couple_affine_dev(self, trap0: int, cell_offsets, cells_ptr: int, npm: float, a_ptr: int, b_ptr: int,
+                          sampled_ptr: int = 0, counts_ptr: int = 0):
+        """Coupling with q = a + b * (the cell's own sample), in list order."""
+        off = np.ascontiguousarray(cell_offsets, dtype=np.int64)
+        _check(load_library().eq_batch_couple_affine_dev(
+            self._h, trap0, off.size - 1, off.ctypes.data_as(C.POINTER(C.c_int64)), C.c_void_p(cells_ptr), npm,
+            C.c_void_p(a_ptr), C.c_void_p(b_ptr), C.c_void_p(sampled_ptr) if sampled_ptr else None,
+            C.c_void_p(counts_ptr) if counts_ptr else None))
 
     def set_wall_values(self, prob: int, wall: int, values=None):
         """Per-node Dirichlet values along a Dirichlet wall, or the wall's constant when None."""

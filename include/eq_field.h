@@ -278,6 +278,20 @@ int eq_batch_couple_traps_dev(eq_batch* b, int32_t trap0, int32_t ntraps, const 
                               const eq_cell* d_cells, double npm, const double* d_q, double* d_sampled,
                               int32_t* d_counts, uint32_t flags);
 
+/* updateCells for strains whose amount depends on the cell's own sample
+ * (Strain::computeProteins with membrane exchange, dt*membraneRate*(iHSL - eHSL),
+ * src/Strain.h:312-313, is affine in the sampled eHSL): per cell in list order
+ * and per species, e = the mean over S_c of the current field (after every
+ * earlier cell's deposit), then q = a + b*e is added as q/|S_c| on S_c -- the
+ * reference's read -> compute -> write order inside one step
+ * (src/abm/eQabm.cpp:384-405).  a, b, sampled: nc x nspecies (device); cells and
+ * offsets as eq_batch_couple_traps_dev.  One workgroup per trap walks its cells
+ * in order, so a trap's cells are a sequential chain: use the q-upfront entry
+ * points when the amounts do not depend on the sample. */
+int eq_batch_couple_affine_dev(eq_batch* b, int32_t trap0, int32_t ntraps, const int64_t* cell_offsets,
+                               const eq_cell* d_cells, double npm, const double* d_a, const double* d_b,
+                               double* d_sampled, int32_t* d_counts);
+
 /* No counterpart in the reference.  eq_batch_couple_traps_dev for callers that
  * cannot allocate device memory (code under HIP graph capture, tight serving
  * loops, harnesses that forbid hipMalloc): same arguments, same results, and it

@@ -805,6 +805,37 @@ void orc_couple(const orc_grid* g, const orc_cell* cells, int64_t nc, int32_t ns
     }
 }
 
+void orc_couple_affine(const orc_grid* g, const orc_cell* cells, int64_t nc, int32_t ns, double* const* fields,
+                       const double* qa, const double* qb, double* sampled, int32_t* counts)
+{
+    /* updateCells (src/abm/eQabm.cpp:384-405) with the strain's amount formed
+     * from the cell's own reading: readHSL for every species, then
+     * q = a + b*e (the membrane term dt*membraneRate*(iHSL - eHSL) of
+     * src/Strain.h:312-313 is of this form), then writeHSL. */
+    const int64_t nw = g->nodes_w;
+    std::vector<int32_t> ij;
+    std::vector<double> e(ns);
+    for (int64_t k = 0; k < nc; ++k) {
+        int32_t w[4];
+        orc_cell_window(g, &cells[k], w);
+        const int64_t cap = (int64_t)(w[1] - w[0] + 1) * (w[3] - w[2] + 1) + 1;
+        ij.resize(2 * cap);
+        const int64_t cnt = orc_cell_coverage(g, &cells[k], ij.data(), cap);
+        if (counts) counts[k] = (int32_t)cnt;
+        for (int32_t s = 0; s < ns; ++s) {
+            double hsl = 0.0;
+            for (int64_t t = 0; t < cnt; ++t) hsl += fields[s][ij[2 * t + 1] + ij[2 * t] * nw];
+            e[s] = hsl / double(cnt);
+            if (sampled) sampled[k * ns + s] = e[s];
+        }
+        for (int32_t s = 0; s < ns; ++s) {
+            const double q = qa[k * ns + s] + qb[k * ns + s] * e[s];
+            const double d = q / double(cnt);
+            for (int64_t t = 0; t < cnt; ++t) fields[s][ij[2 * t + 1] + ij[2 * t] * nw] += d;
+        }
+    }
+}
+
 void orc_cell_tensor(const orc_grid* g, const orc_cell* cells, int64_t nc, double Dx, double Dy, double* D11,
                      double* D22, double* D12)
 {

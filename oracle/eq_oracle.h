@@ -159,6 +159,13 @@ void orc_cell_window(const orc_grid* g, const orc_cell* c, int32_t* win4);
 void orc_couple(const orc_grid* g, const orc_cell* cells, int64_t nc, int32_t nspecies,
                 double* const* fields, const double* q, double* sampled, int32_t* counts);
 
+/* orc_couple with each cell's amount formed from its own reading: per cell in
+ * list order, e_s = mean of species s over S_c, then q_s = a_s + b_s e_s is
+ * written as q_s/|S_c| (a strain whose membrane exchange dt*rate*(iHSL - eHSL),
+ * src/Strain.h:312-313, is affine in the sample; src/abm/eQabm.cpp:384-405). */
+void orc_couple_affine(const orc_grid* g, const orc_cell* cells, int64_t nc, int32_t ns, double* const* fields,
+                       const double* qa, const double* qb, double* sampled, int32_t* counts);
+
 /* setDiffusionTensor of eQabm::updateCells (src/abm/eQabm.cpp:241-248,
  * 306-325, 407): D11, D22, D12 (natural order, nodes_w*nodes_h each) reset to
  * (1, 1, 0), then for each cell in list order every node of S_c (the centre

@@ -124,6 +124,7 @@ def lib():
         L.orc_cell_window.argtypes = [P(Grid), P(Cell), ip32]
         L.orc_couple.argtypes = [P(Grid), P(Cell), C.c_int64, C.c_int32, P(dp), dp, dp, ip32]
         L.orc_cell_tensor.argtypes = [P(Grid), P(Cell), C.c_int64, C.c_double, C.c_double, dp, dp, dp]
+        L.orc_couple_affine.argtypes = [P(Grid), P(Cell), C.c_int64, C.c_int32, P(dp), dp, dp, dp, ip32]
         L.orc_boundary_well.restype = C.c_double
         L.orc_boundary_well.argtypes = [C.c_double] * 5 + [ip64]
         L.orc_boundary_flux.restype = C.c_double
@@ -328,3 +329,20 @@ def cell_tensor(g: Grid, cells: np.ndarray, Dx: float, Dy: float):
     t = [np.empty(n) for _ in range(3)]
     lib().orc_cell_tensor(C.byref(g), ptr, arr.shape[0], Dx, Dy, _dp(t[0]), _dp(t[1]), _dp(t[2]))
     return t
+
+
+def couple_affine(g: Grid, cells: np.ndarray, fields, qa: np.ndarray, qb: np.ndarray):
+    """Sequential updateCells loop with q = a + b * (the cell's own sample).
+    Returns (new_fields, sampled (nc, ns), counts (nc,))."""
+    arr, ptr = cells_array(cells)
+    nc = arr.shape[0]
+    fs = [np.array(f, dtype=np.float64, copy=True) for f in fields]
+    ns = len(fs)
+    qa = np.ascontiguousarray(qa, dtype=np.float64).reshape(nc, ns)
+    qb = np.ascontiguousarray(qb, dtype=np.float64).reshape(nc, ns)
+    sampled = np.empty((nc, ns))
+    counts = np.empty(nc, np.int32)
+    ptrs = (C.POINTER(C.c_double) * ns)(*[_dp(f) for f in fs])
+    lib().orc_couple_affine(C.byref(g), ptr, nc, ns, ptrs, _dp(qa), _dp(qb), _dp(sampled),
+                            counts.ctypes.data_as(C.POINTER(C.c_int32)))
+    return fs, sampled, counts

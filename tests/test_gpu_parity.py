@@ -684,3 +684,57 @@ def test_empty_cell_lists_and_rejected_calls(eqf, oracle):
         with pytest.raises(eqf.EqError) as e:
             fb.set_velocity(0, np.ones(11))
         assert e.value.code == 1
+
+
+def test_affine_coupling_matches_oracle(eqf, oracle):
+    """Amounts that depend on the cell's own sample (q = a + b e, a strain's
+    membrane exchange): two traps coupled in one call against the oracle's
+    sequential loop; counts exact, samples and fields within rounding of the
+    per-cell sums (the workgroup sums the covered nodes in another order, and
+    each deposit depends on its sample); then a step against the oracle's LU.
+    With b = 0 the fields equal the q-upfront coupling's bit for bit up to
+    that summation order."""
+    import torch
+    from eq_amd import workloads as Wl
+    npm = 10.0
+    shapes = [(120, 60, 12.0, 6.0, 150), (80, 50, 8.0, 5.0, 90)]
+    rng = Wl.SplitMix64(77)
+    descs, probs, cells, grids, u, qa, qb = [], [], [], [], [], [], []
+    for (nx, ny, W, H, nc) in shapes:
+        c, r = Wl.seeded_cells(rng, nc, W, H, 2)
+        cells.append(c)
+        grids.append(oracle.grid(nx + 1, ny + 1, npm, W, H))
+        qa.append(0.1 * r)
+        qb.append(-0.05 * np.ones_like(r) * np.array([1.0, 0.5]))
+        for D in (200.0, 60.0):
+            descs.append(eqf.field_desc(nx, ny, W, H, 0.1, D, 0.01, 0.0, Wl.TOP_DIRICHLET))
+            probs.append(oracle.problem(nx, ny, W, H, 0.1, D, 0.01, 0.0, walls=Wl.TOP_DIRICHLET))
+            u.append(np.random.default_rng(nx + int(D)).random((nx + 1) * (ny + 1)))
+    off = np.concatenate([[0], np.cumsum([c.shape[0] for c in cells])]).astype(np.int64)
+    dev = torch.device("cuda")
+    cells_d = torch.from_numpy(np.concatenate(cells)).to(dev).contiguous()
+    a_d = torch.from_numpy(np.concatenate(qa)).to(dev).contiguous()
+    b_d = torch.from_numpy(np.concatenate(qb)).to(dev).contiguous()
+    samp_d = torch.zeros((int(off[-1]), 2), dtype=torch.float64, device=dev)
+    cnt_d = torch.zeros(int(off[-1]), dtype=torch.int32, device=dev)
+    with eqf.FieldBatch(descs, nspecies=2, solver=eqf.solver_desc(eqf.MIXED_GMG_CG)) as fb:
+        for k in range(len(descs)):
+            fb.upload(k, u[k])
+        fb.couple_affine_dev(0, off, cells_d.data_ptr(), npm, a_d.data_ptr(), b_d.data_ptr(), samp_d.data_ptr(),
+                             cnt_d.data_ptr())
+        fb.sync()
+        samp, cnt = samp_d.cpu().numpy(), cnt_d.cpu().numpy()
+        for t in range(len(shapes)):
+            a, b = off[t], off[t + 1]
+            fields, s_ref, c_ref = oracle.couple_affine(grids[t], cells[t], u[2 * t:2 * t + 2], qa[t], qb[t])
+            assert np.array_equal(cnt[a:b], c_ref)
+            assert np.allclose(samp[a:b], s_ref, rtol=1e-13, atol=1e-300)
+            for s in range(2):
+                got = fb.download(2 * t + s)
+                assert np.allclose(got, fields[s], rtol=1e-14, atol=1e-15), (t, s)
+                u[2 * t + s] = got
+        st = fb.step()
+        assert st.nconverged == len(descs)
+        for k in range(len(descs)):
+            ref, _ = oracle.step(probs[k], u[k], "lu")
+            assert _rel(fb.download(k), ref) <= 1e-9, k

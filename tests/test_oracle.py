@@ -492,3 +492,30 @@ def test_cell_tensor_hand_counted(oracle):
     # fallback: a zero-length rod writes only its centre node
     D11, D22, D12 = O.cell_tensor(g, np.array([[3.3, 1.2, 0.0, 1.0, 1.0]]), Dx, Dy)
     assert np.count_nonzero(D11 != 1.0) == 1 and D11[round(3.3 * 4) + 41 * round(1.2 * 4)] == Dx
+
+
+def test_couple_affine_semantics(oracle):
+    """The read -> compute -> write order of updateCells for a strain whose
+    amount depends on its own reading (src/abm/eQabm.cpp:384-405): with b = 0
+    it is the plain coupling bit for bit; one cell on a constant field c
+    deposits (a + b c)/|S_c| per node; a second cell overlapping the first
+    reads the first's deposit before its own amount is formed."""
+    O = oracle
+    g = O.grid(41, 21, 4.0, 10.0, 5.0)
+    rng = np.random.default_rng(3)
+    cells = np.array([[5.0, 2.5, 0.0, 3.0, 1.0], [5.5, 2.5, 0.0, 3.0, 1.0], [3.0, 3.0, 0.7, 2.5, 1.0]])
+    u = [rng.random(41 * 21), rng.random(41 * 21)]
+    a = rng.random((3, 2))
+    f1, s1, c1 = O.couple(g, cells, u, a)
+    f2, s2, c2 = O.couple_affine(g, cells, u, a, np.zeros((3, 2)))
+    assert all(np.array_equal(x, y) for x, y in zip(f1, f2)) and np.array_equal(s1, s2) and np.array_equal(c1, c2)
+    # one cell on a constant field 0.4: q = 0.3 - 0.5 * 0.4 = 0.1 over its 21 nodes
+    (f,), s, c = O.couple_affine(g, cells[:1], [np.full(41 * 21, 0.4)], np.array([[0.3]]), np.array([[-0.5]]))
+    assert c[0] == 21 and math.isclose(s[0, 0], 0.4, rel_tol=1e-15)
+    assert math.isclose(f.max(), 0.4 + (0.3 - 0.5 * s[0, 0]) / 21, rel_tol=1e-15) and np.count_nonzero(f != 0.4) == 21
+    assert math.isclose(f.max(), 0.4 + 0.1 / 21, rel_tol=1e-14)
+    # two overlapping cells (15 shared nodes) on a zero field, q = a + b e
+    (f,), s, c = O.couple_affine(g, cells[:2], [np.zeros(41 * 21)], np.array([[2.1], [4.2]]), np.array([[-1.0], [-3.0]]))
+    e2 = 15 * (2.1 / 21) / 21
+    assert s[0, 0] == 0.0 and math.isclose(s[1, 0], e2, rel_tol=1e-14)
+    assert math.isclose(f.sum(), 2.1 + (4.2 - 3.0 * e2), rel_tol=1e-13)
