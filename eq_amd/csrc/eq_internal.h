@@ -27,6 +27,25 @@ struct TileRef {
     int64_t base;        // element offset of node (i0, j0) in the level pool
 };
 
+// A tile of the fused coarse-level kernels (k_level_down, k_level_up):
+// kFusedTileW x kFusedTileH fine nodes, the coarse nodes it owns and their
+// restriction support, and both levels' geometry, so a workgroup's first
+// loads wait on this record alone.
+constexpr int kFusedTileW = 56, kFusedTileH = 16;
+struct FusedTile {
+    int32_t pi, prob;             // problem in the level's table, global problem (Krylov state)
+    int32_t gt;                   // level 0: slot of the tile's partial sums (after the level's own tiles')
+    int32_t i0, j0;               // first fine row and column
+    int32_t Ia, Ib, Ja, Jb;       // coarse rows [Ia, Ib) and columns [Ja, Jb) owned
+    int32_t ilo, ihi, jlo, jhi;   // fine rows and columns of their restriction support
+    int32_t inner;                // bit 0: the way down's region holds interior nodes only; bit 1: the way up's
+    int32_t nx, ny, pitch, dmask; // fine level
+    int32_t cnx, cny, cpitch, cdmask; // coarse level
+    float rnxf, rnyf, crnxf, crnyf;
+    double rnx, rny;
+    int64_t origin, corigin;
+};
+
 struct LevelView {
     const LevelProb* probs;
     const TileRef* tiles;
@@ -77,6 +96,8 @@ struct Level {
     std::vector<TileRef> tiles_smooth;// problems smoothed here by grid launches (above their bottom level)
     std::vector<TileRef> tiles_bottom;// problems whose bottom level is this one
     std::vector<TileRef> tiles_active;// problems at or above their bottom level (restriction targets)
+    std::vector<FusedTile> fused;     // problems smoothed here, in the fused kernels' tiles
+    std::vector<LevelProb> probs_fin; // level 0: host copy of the table behind d_probs_fin
     std::vector<int32_t> bottom;      // problems whose bottom level is this one
     std::vector<BottomRef> bottom_refs;   // one per bottom problem
     std::vector<LevelProb> bottom_recs;   // copies of their level records, chain by chain
@@ -85,6 +106,8 @@ struct Level {
     TileRef* d_tiles_smooth = nullptr;
     TileRef* d_tiles_bottom = nullptr;
     TileRef* d_tiles_active = nullptr;
+    FusedTile* d_fused = nullptr;
+    LevelProb* d_probs_fin = nullptr; // level 0: the level table with each fused problem's partial-sum slots
     BottomRef* d_bottom = nullptr;
     LevelProb* d_bottom_recs = nullptr;
     int R = 1;                        // rows per thread of the level's tiles (tile height 4R)

@@ -1642,6 +1642,210 @@ __device__ __forceinline__ void lds_sweep(const LevelProb& P, const LdsLevel<T>&
     __syncthreads();
 }
 
+// ---- bottom levels without a tensor field or velocity (the common case)
+//
+// A free ring node's row depends only on the walls it lies on (row_coefs_at:
+// the element counts around the node and the Robin edges of its walls), so
+// one table of 16 rows, indexed by wall_code, holds every ring row of a level
+// together with the reciprocal of its diagonal -- the values row_coefs and
+// row_dinv give at those nodes.  The interior runs in strips of kBotRows
+// rows per thread from a register window (one load per row and column), with
+// 32-bit LDS indices; the level's strips are dealt round-robin to the waves
+// and its ring nodes start at the first wave with one strip fewer.
+
+constexpr int kBotWaves = kBottomThreads / 64;
+constexpr int kBotRows = 4;
+
+__device__ __forceinline__ int wall_code(const LevelProb& P, int i, int j)
+{
+    return (j == 0 ? 1 : 0) | (j == P.nx ? 2 : 0) | (i == 0 ? 4 : 0) | (i == P.ny ? 8 : 0);
+}
+
+template <typename T>
+struct RingTable {
+    static constexpr int kRowWords = (int)(sizeof(Row<T>) / sizeof(T));
+    static constexpr int kWords = 16 * (kRowWords + 1);
+    T* mem;
+    __device__ Row<T>* rows() const { return reinterpret_cast<Row<T>*>(mem); }
+    __device__ T* dinv() const { return mem + 16 * kRowWords; }
+    // threads 0..15 fill the table (entry c from a node with wall code c);
+    // the caller synchronises before the first use
+    template <bool TENS>
+    __device__ void fill(const LevelProb& P) const
+    {
+        const int c = (int)threadIdx.x;
+        if (c == 0 || c >= 16 || (c & 3) == 3 || (c & 12) == 12) return;
+        const int j = (c & 1) ? 0 : (c & 2) ? P.nx : min(1, P.nx);
+        const int i = (c & 4) ? 0 : (c & 8) ? P.ny : min(1, P.ny);
+        const Row<T> rc = row_coefs_at<T>(P, i, j);
+        rows()[c] = rc;
+        dinv()[c] = (T)1 / row_diag<TENS>(rc);
+    }
+};
+static_assert(sizeof(Row<float>) == 13 * sizeof(float) && sizeof(Row<double>) == 13 * sizeof(double),
+              "ring rows are stored as 13 coefficients");
+
+struct BotSplit {
+    int ncb, items, ring0;
+};
+__device__ __forceinline__ BotSplit bot_split(const LevelProb& P)
+{
+    BotSplit s;
+    s.ncb = (P.nx - 1 + 63) / 64; // column blocks over the interior columns 1 .. nx-1
+    const int nst = (P.ny - 1 + kBotRows - 1) / kBotRows;
+    s.items = (P.nx > 1 && P.ny > 1) ? s.ncb * nst : 0;
+    s.ring0 = (s.items % kBotWaves) * 64;
+    return s;
+}
+
+// f(i0, j, a0): interior strip of this thread's item (rows i0 .. i0+kBotRows-1,
+// those below ny; column j < nx; a0 the LDS index of (i0, j))
+template <typename T, typename F>
+__device__ __forceinline__ void for_strips(const LevelProb& P, const LdsLevel<T>& S, const BotSplit& sp, F&& f)
+{
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lane = (int)(threadIdx.x & 63);
+    for (int it = wave; it < sp.items; it += kBotWaves) {
+        const int cb = it % sp.ncb, st = it / sp.ncb;
+        const int j = 1 + cb * 64 + lane;
+        const int i0 = 1 + st * kBotRows;
+        if (j < P.nx) f(i0, j, S.ati(i0, j));
+    }
+}
+
+// g(i, j, a): this thread's ring nodes
+template <typename T, typename G>
+__device__ __forceinline__ void for_ring_nodes(const LevelProb& P, const LdsLevel<T>& S, const BotSplit& sp, G&& g)
+{
+    const int nw = P.nx + 1, nr = 2 * nw + 2 * (P.ny - 1);
+    const int t0 = (int)((threadIdx.x + kBottomThreads - sp.ring0) & (kBottomThreads - 1));
+    for (int t = t0; t < nr; t += kBottomThreads) {
+        int i, j;
+        if (t < nw) {
+            i = 0;
+            j = t;
+        } else if (t < 2 * nw) {
+            i = P.ny;
+            j = t - nw;
+        } else if (t < 2 * nw + P.ny - 1) {
+            i = 1 + t - 2 * nw;
+            j = 0;
+        } else {
+            i = 1 + t - 2 * nw - (P.ny - 1);
+            j = P.nx;
+        }
+        g(i, j, S.ati(i, j));
+    }
+}
+
+// the kBotRows + 2 rows i0-1 .. i0+kBotRows of columns j-1, j, j+1 (rows past
+// ny + 1 read the frame row ny + 1; their values are not used)
+template <typename T>
+struct BotWindow {
+    T w[kBotRows + 2], m[kBotRows + 2], e[kBotRows + 2];
+    __device__ __forceinline__ void load(const T* __restrict__ x, int a0, int pitch, int i0, int ny)
+    {
+#pragma unroll
+        for (int q = 0; q < kBotRows + 2; ++q) {
+            const int a = a0 + (min(i0 - 1 + q, ny + 1) - i0) * pitch;
+            w[q] = x[a - 1];
+            m[q] = x[a];
+            e[q] = x[a + 1];
+        }
+    }
+};
+
+template <typename T>
+__device__ __forceinline__ void lds_load7(const T* __restrict__ v, int a, int pitch, T& c, T& w, T& e, T& s, T& n,
+                                          T& sw, T& ne)
+{
+    c = v[a];
+    w = v[a - 1];
+    e = v[a + 1];
+    s = v[a - pitch];
+    n = v[a + pitch];
+    sw = v[a - pitch - 1];
+    ne = v[a + pitch + 1];
+}
+
+// damped Jacobi sweep of a plain bottom level (lds_sweep's values)
+template <typename T, bool TENS>
+__device__ __forceinline__ void lds_sweep_plain(const LevelProb& P, const LdsLevel<T>& S, const RingTable<T>& rt,
+                                                int from, int to, T omega, bool zero_init)
+{
+    const BotSplit sp = bot_split(P);
+    const Row<T> ri = interior_row<T>(P);
+    const T wdi = omega * interior_dinv<T>(P);
+    T* __restrict__ xo = S.x(to);
+    const T* __restrict__ xi = S.x(from);
+    const int pt = S.pitch;
+    for_strips(P, S, sp, [&](int i0, int j, int a0) {
+        T bl[kBotRows];
+#pragma unroll
+        for (int r = 0; r < kBotRows; ++r) bl[r] = S.b[a0 + (min(i0 + r, P.ny) - i0) * pt];
+        if (zero_init) {
+#pragma unroll
+            for (int r = 0; r < kBotRows; ++r)
+                if (i0 + r < P.ny) xo[a0 + r * pt] = wdi * bl[r];
+            return;
+        }
+        BotWindow<T> X;
+        X.load(xi, a0, pt, i0, P.ny);
+#pragma unroll
+        for (int r = 0; r < kBotRows; ++r)
+            if (i0 + r < P.ny)
+                xo[a0 + r * pt] = X.m[r + 1] + wdi * (bl[r] - row_apply<TENS>(ri, X.m[r + 1], X.w[r + 1], X.e[r + 1],
+                                                                              X.m[r], X.m[r + 2], X.w[r], X.e[r + 2]));
+    });
+    for_ring_nodes(P, S, sp, [&](int i, int j, int a) {
+        if (is_dirichlet(P, i, j)) return;
+        const int c = wall_code(P, i, j);
+        const Row<T> rc = rt.rows()[c];
+        const T wd = omega * rt.dinv()[c];
+        const T bv = S.b[a];
+        if (zero_init) {
+            xo[a] = wd * bv;
+            return;
+        }
+        T xc, xw, xe, xs, xn, xsw, xne;
+        lds_load7(xi, a, pt, xc, xw, xe, xs, xn, xsw, xne);
+        xo[a] = xc + wd * (bv - row_apply<TENS>(rc, xc, xw, xe, xs, xn, xsw, xne));
+    });
+    __syncthreads();
+}
+
+// residual of a plain bottom level into r (0 on Dirichlet nodes)
+template <typename T, bool TENS>
+__device__ __forceinline__ void lds_residual_plain(const LevelProb& P, const LdsLevel<T>& S, const RingTable<T>& rt,
+                                                   const T* __restrict__ x, T* __restrict__ r)
+{
+    const BotSplit sp = bot_split(P);
+    const Row<T> ri = interior_row<T>(P);
+    const int pt = S.pitch;
+    for_strips(P, S, sp, [&](int i0, int j, int a0) {
+        T bl[kBotRows];
+#pragma unroll
+        for (int q = 0; q < kBotRows; ++q) bl[q] = S.b[a0 + (min(i0 + q, P.ny) - i0) * pt];
+        BotWindow<T> X;
+        X.load(x, a0, pt, i0, P.ny);
+#pragma unroll
+        for (int q = 0; q < kBotRows; ++q)
+            if (i0 + q < P.ny)
+                r[a0 + q * pt] = bl[q] - row_apply<TENS>(ri, X.m[q + 1], X.w[q + 1], X.e[q + 1], X.m[q], X.m[q + 2],
+                                                         X.w[q], X.e[q + 2]);
+    });
+    for_ring_nodes(P, S, sp, [&](int i, int j, int a) {
+        if (is_dirichlet(P, i, j)) {
+            r[a] = (T)0;
+            return;
+        }
+        const Row<T> rc = rt.rows()[wall_code(P, i, j)];
+        T xc, xw, xe, xs, xn, xsw, xne;
+        lds_load7(x, a, pt, xc, xw, xe, xs, xn, xsw, xne);
+        r[a] = S.b[a] - row_apply<TENS>(rc, xc, xw, xe, xs, xn, xsw, xne);
+    });
+}
+
 // The V-cycle from the problem's bottom level down, entirely in LDS: the
 // bottom level's right-hand side is read from HBM, its correction is written
 // to `out` (the buffer its finer level's prolongation reads).  Dirichlet
@@ -1657,6 +1861,10 @@ __global__ __launch_bounds__(kBottomThreads) void k_bottom(const BottomRef* __re
     extern __shared__ __align__(16) unsigned char eq_smem[];
     __shared__ int fin[kMaxLev];
     __shared__ T red[kBottomThreads]; // coarsest-level partial products
+    // the ring rows of the level being smoothed share red's storage (red is
+    // used only by the coarsest solve, between barriers)
+    static_assert(RingTable<T>::kWords <= kBottomThreads, "ring table fits the partials' array");
+    const RingTable<T> rt{red};
     const BottomRef ref = refs[blockIdx.x];
     if (st[ref.prob].done) return;
     const int nl = ref.nl;
@@ -1698,6 +1906,7 @@ __global__ __launch_bounds__(kBottomThreads) void k_bottom(const BottomRef* __re
         // loads ahead of its stores
         constexpr int kLoad = 8;
         const int rw = P0.nx + 1, n = rw * (P0.ny + 1);
+        const float rrw = 1.0f / (float)rw;
         const T* __restrict__ bs = bin + P0.origin;
         for (int m0 = 0; m0 < n; m0 += kLoad * kBottomThreads) {
             T v[kLoad];
@@ -1708,7 +1917,7 @@ __global__ __launch_bounds__(kBottomThreads) void k_bottom(const BottomRef* __re
                 v[q] = (T)0;
                 at[q] = -1;
                 if (m < n) {
-                    const int i = m / rw, j = m - i * rw;
+                    const int i = div_floor_pos(m, rw, rrw), j = m - i * rw;
                     v[q] = bs[(int64_t)i * P0.pitch + j];
                     at[q] = (int)S.at(i, j);
                 }
@@ -1729,16 +1938,29 @@ __global__ __launch_bounds__(kBottomThreads) void k_bottom(const BottomRef* __re
         const LevelProb P = uniform_rec(lrec + k);
         const LevelProb Pc = uniform_rec(lrec + k + 1);
         const LdsLevel<T> S = lds_level(sm, P), C = lds_level(sm, Pc);
-        lds_sweep<T, TENS>(P, S, 0, 0, omega, true);
+        const bool plain = !(TENS && (P.aniso || P.vel));
+        if (plain) {
+            rt.template fill<TENS>(P);
+            __syncthreads();
+        }
+        auto sweep = [&](int from, int to, bool zero_init) {
+            if (plain)
+                lds_sweep_plain<T, TENS>(P, S, rt, from, to, omega, zero_init);
+            else
+                lds_sweep<T, TENS>(P, S, from, to, omega, zero_init);
+        };
+        sweep(0, 0, true);
         int cur = 0;
         for (int s = 1; s < pre; ++s) {
-            lds_sweep<T, TENS>(P, S, cur, cur ^ 1, omega, false);
+            sweep(cur, cur ^ 1, false);
             cur ^= 1;
         }
         if (threadIdx.x == 0) fin[k] = cur;
         // residual into the free buffer, then restrict it into the coarser b
         T* r = S.x(cur ^ 1);
-        {
+        if (plain) {
+            lds_residual_plain<T, TENS>(P, S, rt, S.x(cur), r);
+        } else {
             const Row<T> ri = interior_row<T>(P);
             const T* __restrict__ xc_ = S.x(cur);
             auto general = [&](int i, int j) {
@@ -1831,6 +2053,8 @@ __global__ __launch_bounds__(kBottomThreads) void k_bottom(const BottomRef* __re
         const LevelProb Pc = uniform_rec(lrec + k + 1);
         const LdsLevel<T> S = lds_level(sm, P), C = lds_level(sm, Pc);
         int cur = fin[k];
+        const bool plain = !(TENS && (P.aniso || P.vel));
+        if (plain) rt.template fill<TENS>(P); // read after the barrier below
         {
             // interior fine nodes are never Dirichlet
             T* __restrict__ xf = S.x(cur);
@@ -1844,7 +2068,10 @@ __global__ __launch_bounds__(kBottomThreads) void k_bottom(const BottomRef* __re
         }
         __syncthreads();
         for (int s = 0; s < post; ++s) {
-            lds_sweep<T, TENS>(P, S, cur, cur ^ 1, omega, false);
+            if (plain)
+                lds_sweep_plain<T, TENS>(P, S, rt, cur, cur ^ 1, omega, false);
+            else
+                lds_sweep<T, TENS>(P, S, cur, cur ^ 1, omega, false);
             cur ^= 1;
         }
         from = cur;
@@ -1856,6 +2083,357 @@ __global__ __launch_bounds__(kBottomThreads) void k_bottom(const BottomRef* __re
         auto put = [&](int i, int j) { out[P0.origin + (int64_t)i * P0.pitch + j] = xs[S.ati(i, j)]; };
         for_nodes_split(P0, put, put);
     }
+}
+
+// ------------------------------------------------------------ fused coarse levels
+//
+// One launch for a coarse level's way down (pre-sweeps 1 and 2 from x = 0,
+// the residual, its restriction) and one for its way up (the coarse-grid
+// correction, post-sweeps 1 and 2), in place of the four to seven launches of
+// the kernels above; for launches without a tensor field or velocity.  A
+// workgroup takes a tile of kFW x kFH fine nodes and stages the region within
+// kFM nodes around it in LDS.  The way down owns the coarse nodes (I, J)
+// whose fine position (floor(I nyf/nyc), floor(J nxf/nxc)) lies in its tile;
+// with a coarsening ratio in (1, 2] their restriction support reaches at most
+// one fine node before the tile and two past it, and the two sweeps' halos
+// two more, so every value the tile needs lies in the region.  Values outside
+// the grid are zero, as in the pools' frames.  Each node's arithmetic is that
+// of smooth_val, residual_val, restrict_gen and prolong_val.  The tile record
+// (FusedTile) carries the owned ranges and both levels' geometry, so the
+// first loads of a workgroup wait on that record alone; the level record is
+// read alongside them, for the rows.
+
+constexpr int kFW = kFusedTileW, kFH = kFusedTileH, kFM = 3;
+constexpr int kFRW = 64;                // LDS row of a region: columns j0-3 .. j0+kFW+3
+constexpr int kFRH = kFH + 2 * kFM + 1; // rows i0-3 .. i0+kFH+3
+constexpr int kFRows = (kFRH + kWaves - 1) / kWaves; // region rows per wave
+static_assert(kFW + 2 * kFM + 1 <= kFRW, "a region row fits one wave");
+
+// the two levels' geometry as level records (the fields the wall tests and
+// the grid transfers read)
+__device__ __forceinline__ LevelProb fused_fine(const FusedTile& f)
+{
+    LevelProb G{};
+    G.nx = f.nx;
+    G.ny = f.ny;
+    G.pitch = f.pitch;
+    G.dmask = f.dmask;
+    G.origin = f.origin;
+    G.rnx = f.rnx;
+    G.rny = f.rny;
+    G.rnxf = f.rnxf;
+    G.rnyf = f.rnyf;
+    return G;
+}
+__device__ __forceinline__ LevelProb fused_coarse(const FusedTile& f)
+{
+    LevelProb G{};
+    G.nx = f.cnx;
+    G.ny = f.cny;
+    G.pitch = f.cpitch;
+    G.dmask = f.cdmask;
+    G.origin = f.corigin;
+    G.rnxf = f.crnxf;
+    G.rnyf = f.crnyf;
+    return G;
+}
+
+// A x at a free node of a plain problem (its row: the level's interior row,
+// or the ring table's), with its damped inverse diagonal
+template <typename T>
+__device__ __forceinline__ T fused_apply(const LevelProb& P, const RingTable<T>& rt, int i, int j, T& wd, T omega,
+                                         T vc, T vw, T ve, T vs, T vn, T vsw, T vne)
+{
+    if (interior(P, i, j)) {
+        wd = omega * interior_dinv<T>(P);
+        return row_apply<false>(interior_row<T>(P), vc, vw, ve, vs, vn, vsw, vne);
+    }
+    const int c = wall_code(P, i, j);
+    wd = omega * rt.dinv()[c];
+    return row_apply<false>(rt.rows()[c], vc, vw, ve, vs, vn, vsw, vne);
+}
+
+template <typename T>
+__device__ __forceinline__ T fused_wd(const LevelProb& P, const RingTable<T>& rt, int i, int j, T omega)
+{
+    return interior(P, i, j) ? omega * interior_dinv<T>(P) : omega * rt.dinv()[wall_code(P, i, j)];
+}
+
+// Region rows ra .. rb of this lane's column, K consecutive rows per wave:
+// the wave's 3 x (K+2) window of src is loaded first, then f(row, c, w, e,
+// s, n, sw, ne) runs for each of its rows (rows past rb + 1 read row rb + 1;
+// their values are not used).
+template <int K, typename T, typename F>
+__device__ __forceinline__ void region_strips(const T* __restrict__ src, int ra, int rb, int lane, F&& f)
+{
+    const int r0 = ra + (int)(threadIdx.x >> 6) * K;
+    if (r0 > rb) return;
+    T wv[K + 2], mv[K + 2], ev[K + 2];
+#pragma unroll
+    for (int q = 0; q < K + 2; ++q) {
+        const int a = min(r0 - 1 + q, rb + 1) * kFRW + lane;
+        wv[q] = src[a - 1];
+        mv[q] = src[a];
+        ev[q] = src[a + 1];
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        if (r0 + k <= rb) f(r0 + k, mv[k + 1], wv[k + 1], ev[k + 1], mv[k], mv[k + 2], wv[k], ev[k + 2]);
+}
+
+// X1: sweep 1 is already in x1 (the finest level, where the Krylov update
+// forms it); otherwise it is formed here from b
+template <typename T, bool X1>
+__global__ __launch_bounds__(kBlock) void k_level_down(const FusedTile* __restrict__ tiles, int ntiles,
+                                                       const LevelProb* __restrict__ probs, const T* __restrict__ b,
+                                                       T* __restrict__ xout, T* __restrict__ bc, T omega,
+                                                       const CgState* st, const T* __restrict__ x1)
+{
+    __shared__ T sb[kFRH * kFRW], s1[kFRH * kFRW], s2[kFRH * kFRW];
+    __shared__ T ring[RingTable<T>::kWords];
+    const int t = tile_index();
+    if (t >= ntiles) return;
+    const FusedTile f = tiles[t];
+    const LevelProb G = fused_fine(f), Gc = fused_coarse(f);
+    const int i0 = f.i0, j0 = f.j0, ri = i0 - kFM, rj = j0 - kFM;
+    const int ilo = f.ilo, ihi = f.ihi, jlo = f.jlo, jhi = f.jhi;
+    const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
+    const int j = rj + lane;
+    const bool jin = j >= 0 && j <= G.nx;
+    // b (and x1) over the support and two nodes around it, every load first
+    T bv[kFRows], xv[kFRows];
+#pragma unroll
+    for (int q = 0; q < kFRows; ++q) {
+        const int i = ilo - 2 + w + q * kWaves;
+        bv[q] = xv[q] = (T)0;
+        if (i <= ihi + 2 && j >= jlo - 2 && j <= jhi + 2 && jin && i >= 0 && i <= G.ny) {
+            const int64_t g = G.origin + (int64_t)i * G.pitch + j;
+            bv[q] = b[g];
+            if (X1) xv[q] = x1[g];
+        }
+    }
+    if (st[f.prob].done) return;
+    const LevelProb P = probs[f.pi];
+    const RingTable<T> rt{ring};
+    if (f.inner & 1) {
+        // every node of the region interior (none on a wall, none outside the
+        // grid): the level's shared row, no per-node tests, rows in strips
+        const Row<T> rin = interior_row<T>(P);
+        const T wdi = omega * interior_dinv<T>(P);
+#pragma unroll
+        for (int q = 0; q < kFRows; ++q) {
+            const int i = ilo - 2 + w + q * kWaves;
+            if (i > ihi + 2 || j < jlo - 2 || j > jhi + 2) continue;
+            const int a = (i - ri) * kFRW + lane;
+            sb[a] = bv[q];
+            s1[a] = X1 ? xv[q] : wdi * bv[q];
+        }
+        __syncthreads();
+        if (j >= jlo - 1 && j <= jhi + 1)
+            region_strips<(kFH + 5 + kWaves - 1) / kWaves>(
+                s1, ilo - 1 - ri, ihi + 1 - ri, lane, [&](int r, T c, T vw, T ve, T vs, T vn, T vsw, T vne) {
+                    const int a = r * kFRW + lane, i = r + ri;
+                    const T x2 = c + wdi * (sb[a] - row_apply<false>(rin, c, vw, ve, vs, vn, vsw, vne));
+                    s2[a] = x2;
+                    if (i >= i0 && i < i0 + kFH && j >= j0 && j < j0 + kFW)
+                        xout[G.origin + (int64_t)i * G.pitch + j] = x2;
+                });
+        __syncthreads();
+        if (j >= jlo && j <= jhi)
+            region_strips<(kFH + 3 + kWaves - 1) / kWaves>(
+                s2, ilo - ri, ihi - ri, lane, [&](int r, T c, T vw, T ve, T vs, T vn, T vsw, T vne) {
+                    const int a = r * kFRW + lane;
+                    s1[a] = sb[a] - row_apply<false>(rin, c, vw, ve, vs, vn, vsw, vne);
+                });
+        __syncthreads();
+    } else {
+        rt.template fill<false>(P);
+        __syncthreads();
+        // sweep 1 from x = 0 there
+#pragma unroll
+        for (int q = 0; q < kFRows; ++q) {
+            const int i = ilo - 2 + w + q * kWaves;
+            if (i > ihi + 2 || j < jlo - 2 || j > jhi + 2) continue;
+            const int a = (i - ri) * kFRW + lane;
+            const bool in = jin && i >= 0 && i <= G.ny;
+            sb[a] = bv[q];
+            s1[a] = (in && !is_dirichlet(G, i, j)) ? (X1 ? xv[q] : fused_wd(P, rt, i, j, omega) * bv[q]) : (T)0;
+        }
+        __syncthreads();
+        // sweep 2 one node around the support; the tile's own nodes go to xout
+#pragma unroll
+        for (int q = 0; q < kFRows; ++q) {
+            const int i = ilo - 1 + w + q * kWaves;
+            if (i > ihi + 1 || j < jlo - 1 || j > jhi + 1) continue;
+            const int a = (i - ri) * kFRW + lane;
+            const bool in = jin && i >= 0 && i <= G.ny;
+            T x2 = (T)0;
+            if (in && !is_dirichlet(G, i, j)) {
+                T wd;
+                const T xc = s1[a];
+                const T ax = fused_apply(P, rt, i, j, wd, omega, xc, s1[a - 1], s1[a + 1], s1[a - kFRW], s1[a + kFRW],
+                                         s1[a - kFRW - 1], s1[a + kFRW + 1]);
+                x2 = xc + wd * (sb[a] - ax);
+            }
+            s2[a] = x2;
+            if (in && i >= i0 && i < i0 + kFH && j >= j0 && j < j0 + kFW)
+                xout[G.origin + (int64_t)i * G.pitch + j] = x2;
+        }
+        __syncthreads();
+        // residual over the support (zero outside the grid, where the nested
+        // restriction reads one node past the edge); it replaces x1
+#pragma unroll
+        for (int q = 0; q < kFRows; ++q) {
+            const int i = ilo - 1 + w + q * kWaves;
+            if (i > ihi + 1 || j < jlo - 1 || j > jhi + 1) continue;
+            const int a = (i - ri) * kFRW + lane;
+            const bool in = jin && i >= 0 && i <= G.ny;
+            if (!in) {
+                s1[a] = (T)0;
+                continue;
+            }
+            if (i < ilo || i > ihi || j < jlo || j > jhi) continue;
+            T rv = (T)0;
+            if (!is_dirichlet(G, i, j)) {
+                T wd;
+                const T ax = fused_apply(P, rt, i, j, wd, omega, s2[a], s2[a - 1], s2[a + 1], s2[a - kFRW],
+                                         s2[a + kFRW], s2[a - kFRW - 1], s2[a + kFRW + 1]);
+                rv = sb[a] - ax;
+            }
+            s1[a] = rv;
+        }
+        __syncthreads();
+    }
+    // the owned coarse nodes: the restricted residual (0 on Dirichlet nodes)
+    auto rf = [&](int fi, int fj, int64_t) -> T { return s1[(fi - ri) * kFRW + (fj - rj)]; };
+    const int J = f.Ja + lane;
+    if (J < f.Jb)
+        for (int I = f.Ia + w; I < f.Ib; I += kWaves) {
+            const T v = is_dirichlet(Gc, I, J) ? (T)0 : restrict_gen<T, false>(Gc, G, I, J, 0, rf);
+            bc[Gc.origin + (int64_t)I * Gc.pitch + J] = v;
+        }
+}
+
+// DOT (the finest level): the last sweep's b is r (type TI) rounded to T, as
+// in k_smooth, and the tile adds r.x_out to its partial sum in slot gt
+template <typename T, typename TI, bool DOT>
+__global__ __launch_bounds__(kBlock) void k_level_up(const FusedTile* __restrict__ tiles, int ntiles,
+                                                     const LevelProb* __restrict__ probs, const T* __restrict__ xc,
+                                                     const T* __restrict__ b, const T* __restrict__ xin,
+                                                     T* __restrict__ xout, T omega, const CgState* st,
+                                                     const TI* __restrict__ rin, double* partial)
+{
+    __shared__ T sb[kFRH * kFRW], sy[kFRH * kFRW], ss[kFRH * kFRW];
+    __shared__ T ring[RingTable<T>::kWords];
+    const int t = tile_index();
+    if (t >= ntiles) return;
+    const FusedTile f = tiles[t];
+    const LevelProb G = fused_fine(f), Gc = fused_coarse(f);
+    const int i0 = f.i0, j0 = f.j0, ri = i0 - kFM, rj = j0 - kFM;
+    const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
+    const int j = rj + lane;
+    const bool jq = j >= j0 - 2 && j <= j0 + kFW + 1 && j >= 0 && j <= G.nx;
+    // x + P x_c two nodes around the tile (x alone on Dirichlet nodes), and b
+    constexpr int kUpRows = (kFH + 4 + kWaves - 1) / kWaves;
+    T xv[kUpRows], bv[kUpRows], cv[kUpRows];
+#pragma unroll
+    for (int q = 0; q < kUpRows; ++q) {
+        const int i = i0 - 2 + w + q * kWaves;
+        xv[q] = bv[q] = cv[q] = (T)0;
+        if (jq && i <= i0 + kFH + 1 && i >= 0 && i <= G.ny) {
+            const int64_t g = G.origin + (int64_t)i * G.pitch + j;
+            xv[q] = xin[g];
+            bv[q] = b[g];
+            if (!is_dirichlet(G, i, j)) cv[q] = prolong_val<T>(G, Gc, i, j, xc, Gc.origin, Gc.pitch);
+        }
+    }
+    if (st[f.prob].done) return;
+    const LevelProb P = probs[f.pi];
+    const RingTable<T> rt{ring};
+    const bool inner = (f.inner & 2) != 0;
+    if (!inner) rt.template fill<false>(P);
+#pragma unroll
+    for (int q = 0; q < kUpRows; ++q) {
+        const int i = i0 - 2 + w + q * kWaves;
+        if (i > i0 + kFH + 1 || j < j0 - 2 || j > j0 + kFW + 1) continue;
+        const int a = (i - ri) * kFRW + lane;
+        sy[a] = xv[q] + cv[q];
+        sb[a] = bv[q];
+    }
+    __syncthreads();
+    double dot[1] = {0.0};
+    if (inner) {
+        // every node of the region interior: the shared row, rows in strips
+        const Row<T> rw = interior_row<T>(P);
+        const T wdi = omega * interior_dinv<T>(P);
+        if (j >= j0 - 1 && j <= j0 + kFW)
+            region_strips<(kFH + 2 + kWaves - 1) / kWaves>(
+                sy, i0 - 1 - ri, i0 + kFH - ri, lane, [&](int r, T c, T vw, T ve, T vs, T vn, T vsw, T vne) {
+                    const int a = r * kFRW + lane;
+                    ss[a] = c + wdi * (sb[a] - row_apply<false>(rw, c, vw, ve, vs, vn, vsw, vne));
+                });
+        __syncthreads();
+        if (j >= j0 && j < j0 + kFW)
+            region_strips<(kFH + kWaves - 1) / kWaves>(
+                ss, i0 - ri, i0 + kFH - 1 - ri, lane, [&](int r, T c, T vw, T ve, T vs, T vn, T vsw, T vne) {
+                    const int a = r * kFRW + lane, i = r + ri;
+                    const int64_t g = G.origin + (int64_t)i * G.pitch + j;
+                    T bb = sb[a];
+                    TI rv = (TI)0;
+                    if (DOT) {
+                        rv = rin[g];
+                        bb = (T)rv;
+                    }
+                    const T o = c + wdi * (bb - row_apply<false>(rw, c, vw, ve, vs, vn, vsw, vne));
+                    xout[g] = o;
+                    if (DOT) dot[0] += (double)rv * (double)o;
+                });
+        if (DOT) tile_partial<1>(dot, partial, f.gt);
+        return;
+    }
+    // post-sweep 1 one node around the tile
+#pragma unroll
+    for (int q = 0; q < kUpRows; ++q) {
+        const int i = i0 - 1 + w + q * kWaves;
+        if (i > i0 + kFH || j < j0 - 1 || j > j0 + kFW) continue;
+        const int a = (i - ri) * kFRW + lane;
+        const bool in = j >= 0 && j <= G.nx && i >= 0 && i <= G.ny;
+        T s = (T)0;
+        if (in && !is_dirichlet(G, i, j)) {
+            T wd;
+            const T yc = sy[a];
+            const T ay = fused_apply(P, rt, i, j, wd, omega, yc, sy[a - 1], sy[a + 1], sy[a - kFRW], sy[a + kFRW],
+                                     sy[a - kFRW - 1], sy[a + kFRW + 1]);
+            s = yc + wd * (sb[a] - ay);
+        }
+        ss[a] = s;
+    }
+    __syncthreads();
+    // post-sweep 2 on the tile
+#pragma unroll
+    for (int q = 0; q < (kFH + kWaves - 1) / kWaves; ++q) {
+        const int i = i0 + w + q * kWaves;
+        if (i >= i0 + kFH || i > G.ny || j < j0 || j >= j0 + kFW || j > G.nx) continue;
+        const int a = (i - ri) * kFRW + lane;
+        const int64_t g = G.origin + (int64_t)i * G.pitch + j;
+        T o = (T)0;
+        if (!is_dirichlet(G, i, j)) {
+            T wd;
+            const T sc = ss[a];
+            const T as = fused_apply(P, rt, i, j, wd, omega, sc, ss[a - 1], ss[a + 1], ss[a - kFRW], ss[a + kFRW],
+                                     ss[a - kFRW - 1], ss[a + kFRW + 1]);
+            T bb = sb[a];
+            TI rv = (TI)0;
+            if (DOT) {
+                rv = rin[g];
+                bb = (T)rv;
+            }
+            o = sc + wd * (bb - as);
+            if (DOT) dot[0] += (double)rv * (double)o;
+        }
+        xout[g] = o;
+    }
+    if (DOT) tile_partial<1>(dot, partial, f.gt);
 }
 
 // ------------------------------------------------------------ diagnostics
@@ -2350,6 +2928,18 @@ static int coarse_cells(int n)
     return (h + m - 1) / m * m;
 }
 
+// coarse nodes [Ia, Ib) owned by the fine nodes [i0, i0 + n) -- those whose
+// fine position floor(I nf / nc) lies there -- and the fine nodes [lo, hi]
+// of their restriction support (restrict_gen's ranges)
+static void fused_owned(int i0, int n, int nf, int nc, int32_t& Ia, int32_t& Ib, int32_t& lo, int32_t& hi)
+{
+    const int64_t f = nf, c = nc;
+    Ia = (int32_t)(((int64_t)i0 * c + f - 1) / f);
+    Ib = (int32_t)std::min<int64_t>(c + 1, ((int64_t)(i0 + n) * c + f - 1) / f);
+    lo = Ia == 0 ? 0 : (int32_t)(((int64_t)(Ia - 1) * f) / c) + 1;
+    hi = (int32_t)std::min<int64_t>(f, ((int64_t)Ib * f - 1) / c);
+}
+
 template <typename T>
 static int build_hierarchy(eq_batch* b)
 {
@@ -2486,6 +3076,43 @@ static int build_hierarchy(eq_batch* b)
                 if (active && P.bottom) L.tiles_bottom.push_back(r);
                 if (active) L.tiles_active.push_back(r);
             }
+            if (smooth && !P.coarsest) {
+                // the fused coarse-level kernels' tiles
+                const LevelProb& Pc = b->levels[l + 1].probs[P.cidx];
+                const int fx = (P.nx + 1 + kFusedTileW - 1) / kFusedTileW;
+                const int fy = (P.ny + 1 + kFusedTileH - 1) / kFusedTileH;
+                for (int t = 0; t < fx * fy; ++t) {
+                    FusedTile f{};
+                    f.pi = (int32_t)k;
+                    f.prob = P.prob;
+                    f.gt = -1;
+                    f.i0 = (t / fx) * kFusedTileH;
+                    f.j0 = (t % fx) * kFusedTileW;
+                    fused_owned(f.i0, kFusedTileH, P.ny, Pc.ny, f.Ia, f.Ib, f.ilo, f.ihi);
+                    fused_owned(f.j0, kFusedTileW, P.nx, Pc.nx, f.Ja, f.Jb, f.jlo, f.jhi);
+                    if (f.ilo - 2 >= 1 && f.ihi + 2 <= P.ny - 1 && f.jlo - 2 >= 1 && f.jhi + 2 <= P.nx - 1) f.inner |= 1;
+                    if (f.i0 - 2 >= 1 && f.i0 + kFusedTileH + 1 <= P.ny - 1 && f.j0 - 2 >= 1 &&
+                        f.j0 + kFusedTileW + 1 <= P.nx - 1)
+                        f.inner |= 2;
+                    f.nx = P.nx;
+                    f.ny = P.ny;
+                    f.pitch = P.pitch;
+                    f.dmask = P.dmask;
+                    f.cnx = Pc.nx;
+                    f.cny = Pc.ny;
+                    f.cpitch = Pc.pitch;
+                    f.cdmask = Pc.dmask;
+                    f.rnxf = P.rnxf;
+                    f.rnyf = P.rnyf;
+                    f.crnxf = Pc.rnxf;
+                    f.crnyf = Pc.rnyf;
+                    f.rnx = P.rnx;
+                    f.rny = P.rny;
+                    f.origin = P.origin;
+                    f.corigin = Pc.origin;
+                    L.fused.push_back(f);
+                }
+            }
             const double nodes = (double)(P.nx + 1) * (double)(P.ny + 1);
             L.nodes_all += nodes;
             if (smooth) L.nodes_smooth += nodes;
@@ -2500,6 +3127,23 @@ static int build_hierarchy(eq_batch* b)
                 P.dense_off = (int64_t)dense_all.size();
                 for (double v : inv) dense_all.push_back((T)v); // column-major already
             }
+        }
+    }
+    // level 0: the fused tiles' partial sums follow the level's own tiles',
+    // and the finish of r.z after a fused V-cycle reads them there
+    {
+        Level& L0 = b->levels[0];
+        L0.probs_fin = L0.probs;
+        const int32_t base = (int32_t)L0.tiles_all.size();
+        for (size_t k = 0; k < L0.fused.size(); ++k) {
+            FusedTile& f = L0.fused[k];
+            f.gt = base + (int32_t)k;
+            LevelProb& Q = L0.probs_fin[f.pi];
+            if (Q.tile_begin < base) {
+                Q.tile_begin = f.gt;
+                Q.ntiles = 0;
+            }
+            Q.ntiles += 1;
         }
     }
     // bottom chains; dynamic LDS = the level arrays, then the level records
@@ -2522,7 +3166,9 @@ static int build_hierarchy(eq_batch* b)
         if ((rc = upload_vec(&L.d_probs, L.probs, s)) || (rc = upload_vec(&L.d_tiles_all, L.tiles_all, s)) ||
             (rc = upload_vec(&L.d_tiles_smooth, L.tiles_smooth, s)) ||
             (rc = upload_vec(&L.d_tiles_bottom, L.tiles_bottom, s)) ||
-            (rc = upload_vec(&L.d_tiles_active, L.tiles_active, s)) || (rc = upload_vec(&L.d_bottom, L.bottom_refs, s)) ||
+            (rc = upload_vec(&L.d_tiles_active, L.tiles_active, s)) || (rc = upload_vec(&L.d_fused, L.fused, s)) ||
+            (l == 0 && (rc = upload_vec(&L.d_probs_fin, L.probs_fin, s))) ||
+            (rc = upload_vec(&L.d_bottom, L.bottom_refs, s)) ||
             (rc = upload_vec(&L.d_bottom_recs, L.bottom_recs, s)))
             return rc;
         if (mg) {
@@ -2552,7 +3198,7 @@ static int build_hierarchy(eq_batch* b)
         (bicg && ((rc = alloc_zero(&b->bs_rhat, n0, s)) || (rc = alloc_zero(&b->bs_p, n0, s)) ||
                   (rc = alloc_zero(&b->bs_v, n0, s)) || (rc = alloc_zero(&b->bs_s, n0, s)) ||
                   (rc = alloc_zero(&b->bs_t, n0, s)))) ||
-        (rc = alloc_zero(&b->partial, 4 * (int64_t)b->levels[0].tiles_all.size(), s)))
+        (rc = alloc_zero(&b->partial, 4 * (int64_t)(b->levels[0].tiles_all.size() + b->levels[0].fused.size()), s)))
         return rc;
     EQ_HIP(hipStreamSynchronize(s));
     return EQ_OK;
@@ -2589,8 +3235,27 @@ struct VCycle {
     {
         const eq_solver_desc& S = b->solver;
         const Level& L = b->levels[l];
+        if (fused_level(l)) return (T*)L.xa;
         const int f = ((S.mg_pre - 1) + S.mg_post) & 1;
         return f ? (T*)L.xb : (T*)L.xa;
+    }
+
+    // A level goes down in one launch (k_level_down: pre-sweeps into xb, the
+    // restricted residual into the next level's b) and up in another
+    // (k_level_up: xb plus the correction, post-sweeps into xa; level 0: into
+    // `out`, with the partial r.z in the fused tiles' slots).  Level 0 does
+    // so when its first sweep is already in xa (pre1).
+    bool fused_level(int l) const
+    {
+        const eq_solver_desc& S = b->solver;
+        if (TENS || S.mg_level_fused == 0 || S.mg_pre != 2 || S.mg_post != 2 || b->levels[l].fused.empty()) return false;
+        return l > 0 || pre1;
+    }
+    // the level table whose tile ranges locate the partial r.z of the last run0()
+    const LevelProb* rz_table() const
+    {
+        const Level& L0 = b->levels[0];
+        return fused_level(0) ? L0.d_probs_fin : L0.d_probs;
     }
 
     void run(int l)
@@ -2617,7 +3282,35 @@ struct VCycle {
         const int gs = grid_for(ns);
         T* X[2] = {(T*)L.xa, (T*)L.xb};
         int cur = 0;
-        if (ns > 0) {
+        if (ns > 0 && top && fused_level(0)) {
+            Level& C = b->levels[1];
+            const int nf = (int)L.fused.size(), gf = grid_for(nf);
+            const double nodes = L.nodes_smooth, cnodes = C.nodes_active;
+            // the smoother's b: the residual itself in the smoother's precision,
+            // or the FP32 copy the Krylov update wrote with the first sweep
+            const T* b0 = kDirect ? (const T*)in : (const T*)L.b;
+            launch_tiles(b, EQ_KERNEL_TRANSFER0, nodes * 3 * sizeof(T) + cnodes * sizeof(T), k_level_down<T, true>, gf,
+                         s, (const FusedTile*)L.d_fused, nf, (const LevelProb*)L.d_probs, b0, (T*)L.xb, (T*)C.b, omega,
+                         st, (const T*)L.xa);
+            run(1);
+            launch_tiles(b, EQ_KERNEL_SMOOTH, nodes * (3 * sizeof(T) + sizeof(TI)) + cnodes * sizeof(T),
+                         k_level_up<T, TI, true>, gf, s, (const FusedTile*)L.d_fused, nf, (const LevelProb*)L.d_probs,
+                         (const T*)final_buffer(1), b0, (const T*)L.xb, out, omega, st, in, b->partial);
+        } else if (ns > 0 && fused_level(l)) {
+            Level& C = b->levels[l + 1];
+            const int nf = (int)L.fused.size(), gf = grid_for(nf);
+            const double nodes = L.nodes_smooth, cnodes = C.nodes_active;
+            // b read, x written; the coarse b written
+            launch_tiles(b, EQ_KERNEL_TRANSFER_COARSE, nodes * 2 * sizeof(T) + cnodes * sizeof(T),
+                         k_level_down<T, false>, gf, s, (const FusedTile*)L.d_fused, nf, (const LevelProb*)L.d_probs,
+                         (const T*)L.b, (T*)L.xb, (T*)C.b, omega, st, (const T*)nullptr);
+            run(l + 1);
+            // x and b read, x written; the coarse correction read
+            launch_tiles(b, EQ_KERNEL_SMOOTH_COARSE, nodes * 3 * sizeof(T) + cnodes * sizeof(T),
+                         k_level_up<T, T, false>, gf, s, (const FusedTile*)L.d_fused, nf, (const LevelProb*)L.d_probs,
+                         (const T*)final_buffer(l + 1), (const T*)L.b, (const T*)L.xb, (T*)L.xa, omega, st,
+                         (const T*)nullptr, (double*)nullptr);
+        } else if (ns > 0) {
             const double nodes = L.nodes_smooth;
             // pre-smoothing; the first sweep starts from x = 0; level 0 reads
             // the FP64 residual and (mixed precision) keeps an FP32 copy of it
@@ -2790,7 +3483,8 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
     auto iteration = [&](double* pold, double* pnew) {
         if (!jacobi) {
             vc.run0();
-            finish(kFinRz);
+            launch(b, EQ_KERNEL_FINISH, fin_bytes, k_finish, dim3(b->nprob), dim3(kFinishThreads), 0, s, vc.rz_table(),
+                   (const double*)b->partial, b->d_state, (int)kFinRz, 0);
         }
         // reads z, p_old; writes p_new, q (FP64)
         launch_tiles(b, EQ_KERNEL_STENCIL64, fine_nodes * (3 * sizeof(T) + sizeof(double)), k_ap_dir<R0, T, TENS>, g0, s, A,
@@ -3096,7 +3790,8 @@ static int solve_ir(eq_batch* b, eq_step_stats* stats)
     // one inner iteration from direction pold to pnew (FP32 throughout)
     auto iteration = [&](T* pold, T* pnew) {
         vc.run0();
-        finish(si, kFinRz);
+        launch(b, EQ_KERNEL_FINISH, fin_bytes, k_finish, dim3(b->nprob), dim3(kFinishThreads), 0, s, vc.rz_table(),
+               (const double*)b->partial, si, (int)kFinRz, 0);
         launch_tiles(b, EQ_KERNEL_STENCIL64, n0 * 4.0 * sizeof(T), k_ap_dir<R0, T, TENS, T>, g0, s, A, (const T*)b->z,
                      (const T*)pold, pnew, (T*)b->q, b->partial, (const CgState*)si);
         finish(si, kFinAp);
@@ -3300,6 +3995,7 @@ void eq_solver_default(eq_solver_desc* s)
     s->extrapolate = 0; // measured: no fewer iterations on the 2000x1000 workload (10 either way)
     s->ir_max_outer = 20;
     s->ir_inner_rtol = 1e-4;
+    s->mg_level_fused = 1;
 }
 
 int eq_batch_create(const eq_field_desc* descs, int32_t nprob, int32_t nspecies, const eq_solver_desc* solver,
@@ -3434,6 +4130,8 @@ void eq_batch_destroy(eq_batch* b)
         hipFree(L.d_bottom_recs);
         hipFree(L.d_tiles_bottom);
         hipFree(L.d_tiles_active);
+        hipFree(L.d_fused);
+        hipFree(L.d_probs_fin);
         hipFree(L.b);
         hipFree(L.xa);
         hipFree(L.xb);

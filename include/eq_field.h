@@ -131,6 +131,11 @@ typedef struct {
     double ir_inner_rtol;        /* EQ_SOLVER_IR_GMG_CG: each inner FP32 solve reduces its
                                     residual by this much (default 1e-4), or by less
                                     when less reaches rtol                          */
+    int32_t mg_level_fused;      /* 1 (default): a coarse level smoothed with two pre-
+                                    and two post-sweeps runs its way down (sweeps,
+                                    residual, restriction) in one launch and its way
+                                    up (correction, sweeps) in another; 0: the
+                                    separate launches                               */
 } eq_solver_desc;
 
 typedef struct {
