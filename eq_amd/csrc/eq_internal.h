@@ -27,11 +27,13 @@ struct TileRef {
     int64_t base;        // element offset of node (i0, j0) in the level pool
 };
 
-// A tile of the fused coarse-level kernels (k_level_down, k_level_up):
-// kFusedTileW x kFusedTileH fine nodes, the coarse nodes it owns and their
+// A tile of the fused level kernels (k_level_down, k_level_up): kFusedTileW
+// columns by kFusedTileH0 rows on level 0, kFusedTileHc rows on the coarse
+// levels (measured: the coarse levels' short launches favour more, shorter
+// workgroups, level 0 fewer halo rows), the coarse nodes it owns and their
 // restriction support, and both levels' geometry, so a workgroup's first
 // loads wait on this record alone.
-constexpr int kFusedTileW = 56, kFusedTileH = 16;
+constexpr int kFusedTileW = 56, kFusedTileH0 = 16, kFusedTileHc = 8;
 struct FusedTile {
     int32_t pi, prob;             // problem in the level's table, global problem (Krylov state)
     int32_t gt;                   // level 0: slot of the tile's partial sums (after the level's own tiles')

@@ -2103,11 +2103,14 @@ __global__ __launch_bounds__(kBottomThreads) void k_bottom(const BottomRef* __re
 // first loads of a workgroup wait on that record alone; the level record is
 // read alongside them, for the rows.
 
-constexpr int kFW = kFusedTileW, kFH = kFusedTileH, kFM = 3;
-constexpr int kFRW = 64;                // LDS row of a region: columns j0-3 .. j0+kFW+3
-constexpr int kFRH = kFH + 2 * kFM + 1; // rows i0-3 .. i0+kFH+3
-constexpr int kFRows = (kFRH + kWaves - 1) / kWaves; // region rows per wave
+constexpr int kFW = kFusedTileW, kFM = 3;
+constexpr int kFRW = 64; // LDS row of a region: columns j0-3 .. j0+kFW+3
 static_assert(kFW + 2 * kFM + 1 <= kFRW, "a region row fits one wave");
+// a tile of TH rows: region rows i0-3 .. i0+TH+3, and region rows per wave
+#define EQ_FUSED_DIMS(TH)                                                                                              \
+    constexpr int kFH = (TH);                                                                                          \
+    constexpr int kFRH = kFH + 2 * kFM + 1;                                                                            \
+    constexpr int kFRows = (kFRH + kWaves - 1) / kWaves;
 
 // the two levels' geometry as level records (the fields the wall tests and
 // the grid transfers read)
@@ -2160,13 +2163,13 @@ __device__ __forceinline__ T fused_wd(const LevelProb& P, const RingTable<T>& rt
 }
 
 // Region rows ra .. rb of this lane's column, K consecutive rows per wave:
-// the wave's 3 x (K+2) window of src is loaded first, then f(row, c, w, e,
-// s, n, sw, ne) runs for each of its rows (rows past rb + 1 read row rb + 1;
-// their values are not used).
+// the wave's 3 x (K+2) window of src is loaded first, then f(k, row, c, w,
+// e, s, n, sw, ne) runs for each of its rows, k = 0 .. K-1 (rows past rb + 1
+// read row rb + 1; their values are not used).
 template <int K, typename T, typename F>
 __device__ __forceinline__ void region_strips(const T* __restrict__ src, int ra, int rb, int lane, F&& f)
 {
-    const int r0 = ra + (int)(threadIdx.x >> 6) * K;
+    const int r0 = ra + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) * K;
     if (r0 > rb) return;
     T wv[K + 2], mv[K + 2], ev[K + 2];
 #pragma unroll
@@ -2178,17 +2181,18 @@ __device__ __forceinline__ void region_strips(const T* __restrict__ src, int ra,
     }
 #pragma unroll
     for (int k = 0; k < K; ++k)
-        if (r0 + k <= rb) f(r0 + k, mv[k + 1], wv[k + 1], ev[k + 1], mv[k], mv[k + 2], wv[k], ev[k + 2]);
+        if (r0 + k <= rb) f(k, r0 + k, mv[k + 1], wv[k + 1], ev[k + 1], mv[k], mv[k + 2], wv[k], ev[k + 2]);
 }
 
 // X1: sweep 1 is already in x1 (the finest level, where the Krylov update
-// forms it); otherwise it is formed here from b
-template <typename T, bool X1>
+// forms it); otherwise it is formed here from b.  TH: tile rows.
+template <typename T, int TH, bool X1>
 __global__ __launch_bounds__(kBlock) void k_level_down(const FusedTile* __restrict__ tiles, int ntiles,
                                                        const LevelProb* __restrict__ probs, const T* __restrict__ b,
                                                        T* __restrict__ xout, T* __restrict__ bc, T omega,
                                                        const CgState* st, const T* __restrict__ x1)
 {
+    EQ_FUSED_DIMS(TH)
     __shared__ T sb[kFRH * kFRW], s1[kFRH * kFRW], s2[kFRH * kFRW];
     __shared__ T ring[RingTable<T>::kWords];
     const int t = tile_index();
@@ -2197,7 +2201,7 @@ __global__ __launch_bounds__(kBlock) void k_level_down(const FusedTile* __restri
     const LevelProb G = fused_fine(f), Gc = fused_coarse(f);
     const int i0 = f.i0, j0 = f.j0, ri = i0 - kFM, rj = j0 - kFM;
     const int ilo = f.ilo, ihi = f.ihi, jlo = f.jlo, jhi = f.jhi;
-    const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
+    const int lane = (int)(threadIdx.x & 63), w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int j = rj + lane;
     const bool jin = j >= 0 && j <= G.nx;
     // b (and x1) over the support and two nodes around it, every load first
@@ -2212,8 +2216,8 @@ __global__ __launch_bounds__(kBlock) void k_level_down(const FusedTile* __restri
             if (X1) xv[q] = x1[g];
         }
     }
-    if (st[f.prob].done) return;
     const LevelProb P = probs[f.pi];
+    if (st[f.prob].done) return;
     const RingTable<T> rt{ring};
     if (f.inner & 1) {
         // every node of the region interior (none on a wall, none outside the
@@ -2229,19 +2233,22 @@ __global__ __launch_bounds__(kBlock) void k_level_down(const FusedTile* __restri
             s1[a] = X1 ? xv[q] : wdi * bv[q];
         }
         __syncthreads();
+        const bool jt = j >= j0 && j < j0 + kFW; // a column of the tile
         if (j >= jlo - 1 && j <= jhi + 1)
             region_strips<(kFH + 5 + kWaves - 1) / kWaves>(
-                s1, ilo - 1 - ri, ihi + 1 - ri, lane, [&](int r, T c, T vw, T ve, T vs, T vn, T vsw, T vne) {
+                s1, ilo - 1 - ri, ihi + 1 - ri, lane, [&](int, int r, T c, T vw, T ve, T vs, T vn, T vsw, T vne) {
                     const int a = r * kFRW + lane, i = r + ri;
                     const T x2 = c + wdi * (sb[a] - row_apply<false>(rin, c, vw, ve, vs, vn, vsw, vne));
                     s2[a] = x2;
-                    if (i >= i0 && i < i0 + kFH && j >= j0 && j < j0 + kFW)
-                        xout[G.origin + (int64_t)i * G.pitch + j] = x2;
+                    if (i >= i0 && i < i0 + kFH) {
+                        T* __restrict__ row = xout + (G.origin + (int64_t)i * G.pitch);
+                        if (jt) row[j] = x2;
+                    }
                 });
         __syncthreads();
         if (j >= jlo && j <= jhi)
             region_strips<(kFH + 3 + kWaves - 1) / kWaves>(
-                s2, ilo - ri, ihi - ri, lane, [&](int r, T c, T vw, T ve, T vs, T vn, T vsw, T vne) {
+                s2, ilo - ri, ihi - ri, lane, [&](int, int r, T c, T vw, T ve, T vs, T vn, T vsw, T vne) {
                     const int a = r * kFRW + lane;
                     s1[a] = sb[a] - row_apply<false>(rin, c, vw, ve, vs, vn, vsw, vne);
                 });
@@ -2306,23 +2313,27 @@ __global__ __launch_bounds__(kBlock) void k_level_down(const FusedTile* __restri
     }
     // the owned coarse nodes: the restricted residual (0 on Dirichlet nodes)
     auto rf = [&](int fi, int fj, int64_t) -> T { return s1[(fi - ri) * kFRW + (fj - rj)]; };
-    const int J = f.Ja + lane;
-    if (J < f.Jb)
-        for (int I = f.Ia + w; I < f.Ib; I += kWaves) {
+    // (two coarse rows a pass per wave: lanes 0-31 and 32-63, 32 columns each)
+    for (int J0 = f.Ja; J0 < f.Jb; J0 += 32) {
+        const int J = J0 + (lane & 31);
+        for (int I = f.Ia + 2 * w + (lane >> 5); I < f.Ib; I += 2 * kWaves) {
+            if (J >= f.Jb) continue;
             const T v = is_dirichlet(Gc, I, J) ? (T)0 : restrict_gen<T, false>(Gc, G, I, J, 0, rf);
             bc[Gc.origin + (int64_t)I * Gc.pitch + J] = v;
         }
+    }
 }
 
 // DOT (the finest level): the last sweep's b is r (type TI) rounded to T, as
 // in k_smooth, and the tile adds r.x_out to its partial sum in slot gt
-template <typename T, typename TI, bool DOT>
+template <typename T, typename TI, int TH, bool DOT>
 __global__ __launch_bounds__(kBlock) void k_level_up(const FusedTile* __restrict__ tiles, int ntiles,
                                                      const LevelProb* __restrict__ probs, const T* __restrict__ xc,
                                                      const T* __restrict__ b, const T* __restrict__ xin,
                                                      T* __restrict__ xout, T omega, const CgState* st,
                                                      const TI* __restrict__ rin, double* partial)
 {
+    EQ_FUSED_DIMS(TH)
     __shared__ T sb[kFRH * kFRW], sy[kFRH * kFRW], ss[kFRH * kFRW];
     __shared__ T ring[RingTable<T>::kWords];
     const int t = tile_index();
@@ -2330,7 +2341,7 @@ __global__ __launch_bounds__(kBlock) void k_level_up(const FusedTile* __restrict
     const FusedTile f = tiles[t];
     const LevelProb G = fused_fine(f), Gc = fused_coarse(f);
     const int i0 = f.i0, j0 = f.j0, ri = i0 - kFM, rj = j0 - kFM;
-    const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
+    const int lane = (int)(threadIdx.x & 63), w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int j = rj + lane;
     const bool jq = j >= j0 - 2 && j <= j0 + kFW + 1 && j >= 0 && j <= G.nx;
     // x + P x_c two nodes around the tile (x alone on Dirichlet nodes), and b
@@ -2347,10 +2358,12 @@ __global__ __launch_bounds__(kBlock) void k_level_up(const FusedTile* __restrict
             if (!is_dirichlet(G, i, j)) cv[q] = prolong_val<T>(G, Gc, i, j, xc, Gc.origin, Gc.pitch);
         }
     }
-    if (st[f.prob].done) return;
-    const LevelProb P = probs[f.pi];
-    const RingTable<T> rt{ring};
     const bool inner = (f.inner & 2) != 0;
+    constexpr int kUp = (kFH + kWaves - 1) / kWaves; // tile rows per wave in the inner path's last sweep
+    const bool jt = j >= j0 && j < j0 + kFW;
+    const LevelProb P = probs[f.pi];
+    if (st[f.prob].done) return;
+    const RingTable<T> rt{ring};
     if (!inner) rt.template fill<false>(P);
 #pragma unroll
     for (int q = 0; q < kUpRows; ++q) {
@@ -2368,24 +2381,24 @@ __global__ __launch_bounds__(kBlock) void k_level_up(const FusedTile* __restrict
         const T wdi = omega * interior_dinv<T>(P);
         if (j >= j0 - 1 && j <= j0 + kFW)
             region_strips<(kFH + 2 + kWaves - 1) / kWaves>(
-                sy, i0 - 1 - ri, i0 + kFH - ri, lane, [&](int r, T c, T vw, T ve, T vs, T vn, T vsw, T vne) {
+                sy, i0 - 1 - ri, i0 + kFH - ri, lane, [&](int, int r, T c, T vw, T ve, T vs, T vn, T vsw, T vne) {
                     const int a = r * kFRW + lane;
                     ss[a] = c + wdi * (sb[a] - row_apply<false>(rw, c, vw, ve, vs, vn, vsw, vne));
                 });
         __syncthreads();
-        if (j >= j0 && j < j0 + kFW)
-            region_strips<(kFH + kWaves - 1) / kWaves>(
-                ss, i0 - ri, i0 + kFH - 1 - ri, lane, [&](int r, T c, T vw, T ve, T vs, T vn, T vsw, T vne) {
+        if (jt)
+            region_strips<kUp>(
+                ss, i0 - ri, i0 + kFH - 1 - ri, lane, [&](int, int r, T c, T vw, T ve, T vs, T vn, T vsw, T vne) {
                     const int a = r * kFRW + lane, i = r + ri;
-                    const int64_t g = G.origin + (int64_t)i * G.pitch + j;
+                    const int64_t g0 = G.origin + (int64_t)i * G.pitch;
                     T bb = sb[a];
                     TI rv = (TI)0;
                     if (DOT) {
-                        rv = rin[g];
+                        rv = (rin + g0)[j];
                         bb = (T)rv;
                     }
                     const T o = c + wdi * (bb - row_apply<false>(rw, c, vw, ve, vs, vn, vsw, vne));
-                    xout[g] = o;
+                    (xout + g0)[j] = o;
                     if (DOT) dot[0] += (double)rv * (double)o;
                 });
         if (DOT) tile_partial<1>(dot, partial, f.gt);
@@ -3080,18 +3093,19 @@ static int build_hierarchy(eq_batch* b)
                 // the fused coarse-level kernels' tiles
                 const LevelProb& Pc = b->levels[l + 1].probs[P.cidx];
                 const int fx = (P.nx + 1 + kFusedTileW - 1) / kFusedTileW;
-                const int fy = (P.ny + 1 + kFusedTileH - 1) / kFusedTileH;
+                const int th = l == 0 ? kFusedTileH0 : kFusedTileHc;
+                const int fy = (P.ny + 1 + th - 1) / th;
                 for (int t = 0; t < fx * fy; ++t) {
                     FusedTile f{};
                     f.pi = (int32_t)k;
                     f.prob = P.prob;
                     f.gt = -1;
-                    f.i0 = (t / fx) * kFusedTileH;
+                    f.i0 = (t / fx) * th;
                     f.j0 = (t % fx) * kFusedTileW;
-                    fused_owned(f.i0, kFusedTileH, P.ny, Pc.ny, f.Ia, f.Ib, f.ilo, f.ihi);
+                    fused_owned(f.i0, th, P.ny, Pc.ny, f.Ia, f.Ib, f.ilo, f.ihi);
                     fused_owned(f.j0, kFusedTileW, P.nx, Pc.nx, f.Ja, f.Jb, f.jlo, f.jhi);
                     if (f.ilo - 2 >= 1 && f.ihi + 2 <= P.ny - 1 && f.jlo - 2 >= 1 && f.jhi + 2 <= P.nx - 1) f.inner |= 1;
-                    if (f.i0 - 2 >= 1 && f.i0 + kFusedTileH + 1 <= P.ny - 1 && f.j0 - 2 >= 1 &&
+                    if (f.i0 - 2 >= 1 && f.i0 + th + 1 <= P.ny - 1 && f.j0 - 2 >= 1 &&
                         f.j0 + kFusedTileW + 1 <= P.nx - 1)
                         f.inner |= 2;
                     f.nx = P.nx;
@@ -3289,12 +3303,13 @@ struct VCycle {
             // the smoother's b: the residual itself in the smoother's precision,
             // or the FP32 copy the Krylov update wrote with the first sweep
             const T* b0 = kDirect ? (const T*)in : (const T*)L.b;
-            launch_tiles(b, EQ_KERNEL_TRANSFER0, nodes * 3 * sizeof(T) + cnodes * sizeof(T), k_level_down<T, true>, gf,
+            launch_tiles(b, EQ_KERNEL_TRANSFER0, nodes * 3 * sizeof(T) + cnodes * sizeof(T),
+                         k_level_down<T, kFusedTileH0, true>, gf,
                          s, (const FusedTile*)L.d_fused, nf, (const LevelProb*)L.d_probs, b0, (T*)L.xb, (T*)C.b, omega,
                          st, (const T*)L.xa);
             run(1);
             launch_tiles(b, EQ_KERNEL_SMOOTH, nodes * (3 * sizeof(T) + sizeof(TI)) + cnodes * sizeof(T),
-                         k_level_up<T, TI, true>, gf, s, (const FusedTile*)L.d_fused, nf, (const LevelProb*)L.d_probs,
+                         k_level_up<T, TI, kFusedTileH0, true>, gf, s, (const FusedTile*)L.d_fused, nf, (const LevelProb*)L.d_probs,
                          (const T*)final_buffer(1), b0, (const T*)L.xb, out, omega, st, in, b->partial);
         } else if (ns > 0 && fused_level(l)) {
             Level& C = b->levels[l + 1];
@@ -3302,12 +3317,12 @@ struct VCycle {
             const double nodes = L.nodes_smooth, cnodes = C.nodes_active;
             // b read, x written; the coarse b written
             launch_tiles(b, EQ_KERNEL_TRANSFER_COARSE, nodes * 2 * sizeof(T) + cnodes * sizeof(T),
-                         k_level_down<T, false>, gf, s, (const FusedTile*)L.d_fused, nf, (const LevelProb*)L.d_probs,
+                         k_level_down<T, kFusedTileHc, false>, gf, s, (const FusedTile*)L.d_fused, nf, (const LevelProb*)L.d_probs,
                          (const T*)L.b, (T*)L.xb, (T*)C.b, omega, st, (const T*)nullptr);
             run(l + 1);
             // x and b read, x written; the coarse correction read
             launch_tiles(b, EQ_KERNEL_SMOOTH_COARSE, nodes * 3 * sizeof(T) + cnodes * sizeof(T),
-                         k_level_up<T, T, false>, gf, s, (const FusedTile*)L.d_fused, nf, (const LevelProb*)L.d_probs,
+                         k_level_up<T, T, kFusedTileHc, false>, gf, s, (const FusedTile*)L.d_fused, nf, (const LevelProb*)L.d_probs,
                          (const T*)final_buffer(l + 1), (const T*)L.b, (const T*)L.xb, (T*)L.xa, omega, st,
                          (const T*)nullptr, (double*)nullptr);
         } else if (ns > 0) {

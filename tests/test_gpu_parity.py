@@ -738,3 +738,48 @@ def test_affine_coupling_matches_oracle(eqf, oracle):
         for k in range(len(descs)):
             ref, _ = oracle.step(probs[k], u[k], "lu")
             assert _rel(fb.download(k), ref) <= 1e-9, k
+
+
+@pytest.mark.parametrize("solver", ["gmg", "mixed", "ir"])
+def test_fused_levels_match_separate_launches(eqf, oracle, solver):
+    """The fused level kernels (a level's way down and way up in one launch
+    each, level 0 from the Krylov update's first sweep with r.z in its own
+    partial slots) against the separate launches (mg_level_fused = 0) and the
+    oracle's LU.  One batch holds a trap whose whole V-cycle runs in one
+    workgroup (its r.z partials stay in the level's own slots), nested and
+    regridded hierarchies, odd sizes, and Dirichlet, Robin and Neumann walls,
+    so the fused kernels meet interior tiles, wall tiles, walls with and
+    without Dirichlet rows, and both transfer forms.  Two steps: fields
+    within 1e-9 of the oracle either way, iteration counts within one."""
+    kind = {"gmg": eqf.GMG_CG, "mixed": eqf.MIXED_GMG_CG, "ir": eqf.IR_GMG_CG}[solver]
+    shapes = [(40, 20, 2.5, 1.25), (300, 150, 18.75, 9.375), (203, 97, 12.6875, 6.0625), (131, 67, 8.1875, 4.1875)]
+    wallsets = [{oracle.TOP: (oracle.DIRICHLET, 0.0, 0, 0), oracle.LEFT: (oracle.ROBIN, 0, 1.5, 0.2)},
+                {w: (oracle.DIRICHLET, 0.25, 0, 0) for w in range(4)}]
+    descs, probs, u0 = [], [], []
+    rng = np.random.default_rng(41)
+    for t, (nx, ny, W, H) in enumerate(shapes):
+        walls = wallsets[t % 2]
+        for D in (300.0, 50.0):
+            descs.append(eqf.field_desc(nx, ny, W, H, 0.1, D, 0.01, 0.0, walls))
+            probs.append(oracle.problem(nx, ny, W, H, 0.1, D, 0.01, 0.0, walls=walls))
+            u0.append(rng.random((nx + 1) * (ny + 1)))
+    out, its = {}, {}
+    for fused in (1, 0):
+        with eqf.FieldBatch(descs, nspecies=2, solver=eqf.solver_desc(kind, mg_level_fused=fused)) as fb:
+            for k in range(len(descs)):
+                fb.upload(k, u0[k])
+            u, n = [], []
+            for step in range(2):
+                print(f"fused levels {solver} fused={fused} step {step}", flush=True)
+                st = fb.step()
+                assert st.nconverged == len(descs), (fused, step)
+                u.append([fb.download(k) for k in range(len(descs))])
+                n.append(list(fb.problem_stats()[0]))
+            out[fused], its[fused] = u, n
+    ref = [x.copy() for x in u0]
+    for step in range(2):
+        for k in range(len(descs)):
+            ref[k], _ = oracle.step(probs[k], ref[k], "lu")
+            for fused in (1, 0):
+                assert _rel(out[fused][step][k], ref[k]) <= 1e-9, (fused, step, k)
+        assert all(abs(a - b) <= 1 for a, b in zip(its[1][step], its[0][step])), (step, its[1][step], its[0][step])
