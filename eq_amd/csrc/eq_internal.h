@@ -71,7 +71,12 @@ struct CgState {
 // then each coarser level down to the coarsest) start at rec in the level's
 // bottom record table; nl coarser levels follow the bottom one
 struct BottomRef {
-    int32_t rec, nl, prob, pad_;
+    int32_t rec, nl, prob;   // first record of the chain, levels under the bottom level, global problem
+    int32_t nx, ny, pitch;   // the bottom level (right-hand side and correction in the level pool)
+    int32_t lds_total;       // LDS elements of the chain's level arrays
+    int32_t nz;              // nodes of the coarsest level
+    int64_t origin;          // the bottom level's node (0,0) in the pool
+    int64_t dense_off;       // the coarsest level's inverse in the dense pool
 };
 
 // one coarsest-level inverse rebuilt on the device (cell-driven tensor)

@@ -1352,7 +1352,7 @@ __device__ __forceinline__ T first_sweep_at(const LevelProb& P, int i, int j, in
 // pre-smoothing sweeps 1 and 2 (sweep 1 from x = 0); bcopy as in k_smooth
 template <int R, typename T, typename TB, bool TENS>
 __global__ __launch_bounds__(kBlock) void k_presmooth2(LevelView L, const TB* __restrict__ b, T* __restrict__ xout,
-                                                       T* __restrict__ bcopy, T omega, const CgState* st)
+                                                       T* __restrict__ bcopy, T omega, T omega2, const CgState* st)
 {
     NodeCtx c;
     if (!decode<R>(L, c)) return;
@@ -1388,7 +1388,7 @@ __global__ __launch_bounds__(kBlock) void k_presmooth2(LevelView L, const TB* __
         for (int r = 0; r < R; ++r) {
             const int64_t k = c.idx + (int64_t)r * P.pitch;
             const Row<T> rc = row_q(r + 1);
-            const T wd = omega * plain_dinv<T, TENS>(P, rc);
+            const T wd = omega2 * plain_dinv<T, TENS>(P, rc);
             const T bv = (T)B.m[r + 1];
             xout[k] = x1m[r + 1] + wd * (bv - row_apply<TENS>(rc, x1m[r + 1], x1w[r + 1], x1e[r + 1], x1m[r], x1m[r + 2],
                                                                x1w[r], x1e[r + 2]));
@@ -1413,7 +1413,7 @@ __global__ __launch_bounds__(kBlock) void k_presmooth2(LevelView L, const TB* __
         const T xsw = first_sweep_at<T, TB, TENS>(P, i - 1, j - 1, k - pt - 1, b, omega);
         const T xne = first_sweep_at<T, TB, TENS>(P, i + 1, j + 1, k + pt + 1, b, omega);
         const Row<T> rc = row_coefs<T, TENS>(P, i, j, k);
-        const T wd = omega * row_dinv<TENS>(P, i, j, rc);
+        const T wd = omega2 * row_dinv<TENS>(P, i, j, rc);
         const T bv = (T)b[k];
         xout[k] = xc + wd * (bv - row_apply<TENS>(rc, xc, xw, xe, xs, xn, xsw, xne));
         if (bcopy) bcopy[k] = bv;
@@ -1855,76 +1855,92 @@ __device__ __forceinline__ void lds_residual_plain(const LevelProb& P, const Lds
 template <typename T, bool TENS>
 __global__ __launch_bounds__(kBottomThreads) void k_bottom(const BottomRef* __restrict__ refs,
                                                            const LevelProb* __restrict__ recs, const T* __restrict__ bin,
-                                                           T* __restrict__ out, T omega, int pre, int post,
+                                                           T* __restrict__ out, T omega, T omega2, int pre, int post,
                                                            const T* __restrict__ dense, const CgState* st)
 {
     extern __shared__ __align__(16) unsigned char eq_smem[];
     __shared__ int fin[kMaxLev];
     __shared__ T red[kBottomThreads]; // coarsest-level partial products
-    // the ring rows of the level being smoothed share red's storage (red is
-    // used only by the coarsest solve, between barriers)
-    static_assert(RingTable<T>::kWords <= kBottomThreads, "ring table fits the partials' array");
-    const RingTable<T> rt{red};
+    // the ring rows of a level share red's storage (red is used only by the
+    // coarsest solve, between barriers): two tables, level k's at rtab(k),
+    // so the next level's is filled while the current one is read
+    static_assert(2 * RingTable<T>::kWords <= kBottomThreads, "ring tables fit the partials' array");
+    auto rtab = [&](int k) { return RingTable<T>{red + (k & 1) * RingTable<T>::kWords}; };
     const BottomRef ref = refs[blockIdx.x];
-    if (st[ref.prob].done) return;
     const int nl = ref.nl;
     T* sm = reinterpret_cast<T*>(eq_smem);
-    // The coarsest level's dense inverse (below): this thread's first kPre
-    // entries are loaded now, with the right-hand side, and used after the
-    // down-sweep, so the solve there waits on no load from memory.
+    // Every load of the set-up waits on ref alone, and all are issued before
+    // the first use: the coarsest level's dense inverse (this thread's first
+    // kPre entries, used after the down-sweep, so the solve there waits on no
+    // load from memory), the bottom level's right-hand side (kLoad nodes per
+    // thread), the chain's level records and the problem's state.
     constexpr int kPre = (sizeof(T) == 4 && !TENS) ? 32 : 0;
     T dpre[kPre > 0 ? kPre : 1];
     {
-        const LevelProb& Pz = recs[ref.rec + nl];
-        const int n = (Pz.nx + 1) * (Pz.ny + 1), nch = max(1, kBottomThreads / n);
+        const int n = ref.nz, nch = max(1, kBottomThreads / n);
         const int t = (int)threadIdx.x, ch = t / n, k = t - ch * n;
-        const T* D = dense + Pz.dense_off;
+        const T* D = dense + ref.dense_off;
 #pragma unroll
         for (int u = 0; u < kPre; ++u) {
             const int m = ch + u * nch;
             dpre[u] = (t < nch * n && m < n) ? D[(int64_t)m * n + k] : (T)0;
         }
     }
-    const LevelProb* lrec;
-    {
-        // the bottom level's record from memory (uniform registers): its
-        // sizes place the arrays and the records
-        const LevelProb P0 = recs[ref.rec];
-        lrec = reinterpret_cast<const LevelProb*>(eq_smem + bottom_rec_offset<T>(P0));
-        const LdsLevel<T> S = lds_level(sm, P0);
-        const uint32_t* __restrict__ ws = reinterpret_cast<const uint32_t*>(recs + ref.rec);
-        const int nwords = (nl + 1) * kRecWords;
-        uint32_t wv[3];
+    constexpr int kLoad = 12;
+    const int rw = ref.nx + 1, nb = rw * (ref.ny + 1);
+    const float rrw = 1.0f / (float)rw;
+    const T* __restrict__ bs = bin + ref.origin;
+    T bv[kLoad];
+    int bi[kLoad], bj[kLoad];
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            const int t = (int)threadIdx.x + q * kBottomThreads;
-            wv[q] = t < nwords ? ws[t] : 0u;
+    for (int q = 0; q < kLoad; ++q) {
+        const int m = q * kBottomThreads + (int)threadIdx.x;
+        bv[q] = (T)0;
+        bi[q] = -1;
+        bj[q] = 0;
+        if (m < nb) {
+            const int i = div_floor_pos(m, rw, rrw), j = m - i * rw;
+            bv[q] = bs[(int64_t)i * ref.pitch + j];
+            bi[q] = i;
+            bj[q] = j;
         }
-        for (int m = threadIdx.x; m < P0.lds_total; m += kBottomThreads) sm[m] = (T)0;
+    }
+    const uint32_t* __restrict__ ws = reinterpret_cast<const uint32_t*>(recs + ref.rec);
+    const int nwords = (nl + 1) * kRecWords;
+    uint32_t wv[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const int t = (int)threadIdx.x + q * kBottomThreads;
+        wv[q] = t < nwords ? ws[t] : 0u;
+    }
+    const LevelProb P0 = recs[ref.rec];
+    if (st[ref.prob].done) return;
+    const LevelProb* lrec = reinterpret_cast<const LevelProb*>(eq_smem + bottom_rec_offset<T>(P0));
+    {
+        const LdsLevel<T> S = lds_level(sm, P0);
+        for (int m = threadIdx.x; m < ref.lds_total; m += kBottomThreads) sm[m] = (T)0;
+        const bool plain0 = !(TENS && (P0.aniso || P0.vel));
+        if (plain0) rtab(0).template fill<TENS>(P0);
         __syncthreads();
-        // right-hand side by batches of kLoad nodes per thread, each batch's
-        // loads ahead of its stores
-        constexpr int kLoad = 8;
-        const int rw = P0.nx + 1, n = rw * (P0.ny + 1);
-        const float rrw = 1.0f / (float)rw;
-        const T* __restrict__ bs = bin + P0.origin;
-        for (int m0 = 0; m0 < n; m0 += kLoad * kBottomThreads) {
-            T v[kLoad];
-            int at[kLoad];
+        // b, and on a plain level the first sweep from zero (omega b / diag
+        // on free nodes) with it
+        const T wdi0 = omega * interior_dinv<T>(P0);
+        auto put = [&](int i, int j, T v) {
+            const int a = S.ati(i, j);
+            S.b[a] = v;
+            if (!plain0) return;
+            if (interior(P0, i, j))
+                S.x0[a] = wdi0 * v;
+            else if (!is_dirichlet(P0, i, j))
+                S.x0[a] = (omega * rtab(0).dinv()[wall_code(P0, i, j)]) * v;
+        };
 #pragma unroll
-            for (int q = 0; q < kLoad; ++q) {
-                const int m = m0 + q * kBottomThreads + (int)threadIdx.x;
-                v[q] = (T)0;
-                at[q] = -1;
-                if (m < n) {
-                    const int i = div_floor_pos(m, rw, rrw), j = m - i * rw;
-                    v[q] = bs[(int64_t)i * P0.pitch + j];
-                    at[q] = (int)S.at(i, j);
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < kLoad; ++q)
-                if (at[q] >= 0) S.b[at[q]] = v[q];
+        for (int q = 0; q < kLoad; ++q)
+            if (bi[q] >= 0) put(bi[q], bj[q], bv[q]);
+        // levels of more nodes than one batch holds
+        for (int m = kLoad * kBottomThreads + (int)threadIdx.x; m < nb; m += kBottomThreads) {
+            const int i = div_floor_pos(m, rw, rrw), j = m - i * rw;
+            put(i, j, bs[(int64_t)i * ref.pitch + j]);
         }
         uint32_t* wd = reinterpret_cast<uint32_t*>(eq_smem + bottom_rec_offset<T>(P0));
 #pragma unroll
@@ -1938,26 +1954,30 @@ __global__ __launch_bounds__(kBottomThreads) void k_bottom(const BottomRef* __re
         const LevelProb P = uniform_rec(lrec + k);
         const LevelProb Pc = uniform_rec(lrec + k + 1);
         const LdsLevel<T> S = lds_level(sm, P), C = lds_level(sm, Pc);
+        // a plain level's ring rows and first sweep from zero were written by
+        // the phase that formed its b (the load, or the restriction above)
         const bool plain = !(TENS && (P.aniso || P.vel));
-        if (plain) {
-            rt.template fill<TENS>(P);
-            __syncthreads();
-        }
-        auto sweep = [&](int from, int to, bool zero_init) {
+        const RingTable<T> rt = rtab(k);
+        // pre-sweep s is damped by omega (s even) or omega2 (s odd)
+        auto sweep = [&](int from, int to, bool zero_init, T w) {
             if (plain)
-                lds_sweep_plain<T, TENS>(P, S, rt, from, to, omega, zero_init);
+                lds_sweep_plain<T, TENS>(P, S, rt, from, to, w, zero_init);
             else
-                lds_sweep<T, TENS>(P, S, from, to, omega, zero_init);
+                lds_sweep<T, TENS>(P, S, from, to, w, zero_init);
         };
-        sweep(0, 0, true);
+        if (!plain) sweep(0, 0, true, omega);
         int cur = 0;
         for (int s = 1; s < pre; ++s) {
-            sweep(cur, cur ^ 1, false);
+            sweep(cur, cur ^ 1, false, (s & 1) ? omega2 : omega);
             cur ^= 1;
         }
         if (threadIdx.x == 0) fin[k] = cur;
         // residual into the free buffer, then restrict it into the coarser b
+        // (with the coarser level's first sweep when it is plain and smoothed)
         T* r = S.x(cur ^ 1);
+        const bool cplain = !Pc.coarsest && !(TENS && (Pc.aniso || Pc.vel));
+        const RingTable<T> rtc = rtab(k + 1);
+        if (cplain) rtc.template fill<TENS>(Pc); // read after the barrier below
         if (plain) {
             lds_residual_plain<T, TENS>(P, S, rt, S.x(cur), r);
         } else {
@@ -1986,10 +2006,21 @@ __global__ __launch_bounds__(kBottomThreads) void k_bottom(const BottomRef* __re
         {
             // interior coarse nodes are never Dirichlet
             const int64_t f0 = S.at(0, 0);
+            const T wdic = omega * interior_dinv<T>(Pc);
             for_nodes_split(
-                Pc, [&](int i, int j) { C.b[C.ati(i, j)] = restrict_val<T, false>(Pc, P, i, j, r, f0, S.pitch); },
+                Pc,
                 [&](int i, int j) {
-                    if (!is_dirichlet(Pc, i, j)) C.b[C.ati(i, j)] = restrict_val<T, false>(Pc, P, i, j, r, f0, S.pitch);
+                    const int a = C.ati(i, j);
+                    const T v = restrict_val<T, false>(Pc, P, i, j, r, f0, S.pitch);
+                    C.b[a] = v;
+                    if (cplain) C.x0[a] = wdic * v;
+                },
+                [&](int i, int j) {
+                    if (is_dirichlet(Pc, i, j)) return;
+                    const int a = C.ati(i, j);
+                    const T v = restrict_val<T, false>(Pc, P, i, j, r, f0, S.pitch);
+                    C.b[a] = v;
+                    if (cplain) C.x0[a] = (omega * rtc.dinv()[wall_code(Pc, i, j)]) * v;
                 });
         }
         __syncthreads();
@@ -2054,24 +2085,66 @@ __global__ __launch_bounds__(kBottomThreads) void k_bottom(const BottomRef* __re
         const LdsLevel<T> S = lds_level(sm, P), C = lds_level(sm, Pc);
         int cur = fin[k];
         const bool plain = !(TENS && (P.aniso || P.vel));
+        const RingTable<T> rt = rtab(k);
         if (plain) rt.template fill<TENS>(P); // read after the barrier below
         {
             // interior fine nodes are never Dirichlet
             T* __restrict__ xf = S.x(cur);
             const T* __restrict__ xcs = C.x(from);
             const int64_t c0 = C.at(0, 0);
-            for_nodes_split(
-                P, [&](int i, int j) { xf[S.ati(i, j)] += prolong_val<T>(P, Pc, i, j, xcs, c0, C.pitch); },
-                [&](int i, int j) {
-                    if (!is_dirichlet(P, i, j)) xf[S.ati(i, j)] += prolong_val<T>(P, Pc, i, j, xcs, c0, C.pitch);
+            if (P.nx == 2 * Pc.nx && P.ny == 2 * Pc.ny) {
+                // nested: the interior in strips (strip rows i0 .. i0+3 start
+                // odd, so they read coarse rows (i0-1)/2 .. (i0-1)/2 + 2 at
+                // columns j/2 and j/2 + 1, all loaded first), prolong_val's
+                // values; the ring node by node
+                const BotSplit sp = bot_split(P);
+                for_strips(P, S, sp, [&](int i0, int j, int a0) {
+                    const int R0 = (i0 - 1) >> 1, jc = j >> 1, jo = j & 1;
+                    const int ca = C.ati(R0, jc);
+                    T cv[3][2], xv[kBotRows];
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) {
+                        const int ar = ca + min(q, Pc.ny - R0) * C.pitch;
+                        cv[q][0] = xcs[ar];
+                        cv[q][1] = xcs[ar + 1];
+                    }
+#pragma unroll
+                    for (int r = 0; r < kBotRows; ++r) xv[r] = xf[a0 + (min(i0 + r, P.ny) - i0) * S.pitch];
+#pragma unroll
+                    for (int r = 0; r < kBotRows; ++r) {
+                        if (i0 + r >= P.ny) continue;
+                        // fine row i0 + r: coarse row R0 + (r + 1) / 2, odd when r is even
+                        const int q = (r + 1) >> 1;
+                        const T x00 = cv[q][0];
+                        T add;
+                        if ((r & 1) == 0) { // odd fine row: the vertical or diagonal coarse edge
+                            const T xo = jo ? cv[q + 1][1] : cv[q + 1][0];
+                            add = (T)0.5 * (x00 + xo);
+                        } else { // even fine row: the coincident node or the horizontal edge
+                            add = jo ? (T)0.5 * (x00 + cv[q][1]) : x00;
+                        }
+                        xf[a0 + r * S.pitch] = xv[r] + add;
+                    }
                 });
+                for_ring_nodes(P, S, sp, [&](int i, int j, int a) {
+                    if (!is_dirichlet(P, i, j)) xf[a] += prolong_val<T>(P, Pc, i, j, xcs, c0, C.pitch);
+                });
+            } else {
+                for_nodes_split(
+                    P, [&](int i, int j) { xf[S.ati(i, j)] += prolong_val<T>(P, Pc, i, j, xcs, c0, C.pitch); },
+                    [&](int i, int j) {
+                        if (!is_dirichlet(P, i, j)) xf[S.ati(i, j)] += prolong_val<T>(P, Pc, i, j, xcs, c0, C.pitch);
+                    });
+            }
         }
         __syncthreads();
         for (int s = 0; s < post; ++s) {
+            // the pre-sweeps' dampings in reverse order
+            const T w = ((post - 1 - s) & 1) ? omega2 : omega;
             if (plain)
-                lds_sweep_plain<T, TENS>(P, S, rt, cur, cur ^ 1, omega, false);
+                lds_sweep_plain<T, TENS>(P, S, rt, cur, cur ^ 1, w, false);
             else
-                lds_sweep<T, TENS>(P, S, cur, cur ^ 1, omega, false);
+                lds_sweep<T, TENS>(P, S, cur, cur ^ 1, w, false);
             cur ^= 1;
         }
         from = cur;
@@ -2189,7 +2262,7 @@ __device__ __forceinline__ void region_strips(const T* __restrict__ src, int ra,
 template <typename T, int TH, bool X1>
 __global__ __launch_bounds__(kBlock) void k_level_down(const FusedTile* __restrict__ tiles, int ntiles,
                                                        const LevelProb* __restrict__ probs, const T* __restrict__ b,
-                                                       T* __restrict__ xout, T* __restrict__ bc, T omega,
+                                                       T* __restrict__ xout, T* __restrict__ bc, T omega, T omega2,
                                                        const CgState* st, const T* __restrict__ x1)
 {
     EQ_FUSED_DIMS(TH)
@@ -2223,7 +2296,7 @@ __global__ __launch_bounds__(kBlock) void k_level_down(const FusedTile* __restri
         // every node of the region interior (none on a wall, none outside the
         // grid): the level's shared row, no per-node tests, rows in strips
         const Row<T> rin = interior_row<T>(P);
-        const T wdi = omega * interior_dinv<T>(P);
+        const T wdi = omega * interior_dinv<T>(P), wdi2 = omega2 * interior_dinv<T>(P);
 #pragma unroll
         for (int q = 0; q < kFRows; ++q) {
             const int i = ilo - 2 + w + q * kWaves;
@@ -2238,7 +2311,7 @@ __global__ __launch_bounds__(kBlock) void k_level_down(const FusedTile* __restri
             region_strips<(kFH + 5 + kWaves - 1) / kWaves>(
                 s1, ilo - 1 - ri, ihi + 1 - ri, lane, [&](int, int r, T c, T vw, T ve, T vs, T vn, T vsw, T vne) {
                     const int a = r * kFRW + lane, i = r + ri;
-                    const T x2 = c + wdi * (sb[a] - row_apply<false>(rin, c, vw, ve, vs, vn, vsw, vne));
+                    const T x2 = c + wdi2 * (sb[a] - row_apply<false>(rin, c, vw, ve, vs, vn, vsw, vne));
                     s2[a] = x2;
                     if (i >= i0 && i < i0 + kFH) {
                         T* __restrict__ row = xout + (G.origin + (int64_t)i * G.pitch);
@@ -2278,7 +2351,7 @@ __global__ __launch_bounds__(kBlock) void k_level_down(const FusedTile* __restri
             if (in && !is_dirichlet(G, i, j)) {
                 T wd;
                 const T xc = s1[a];
-                const T ax = fused_apply(P, rt, i, j, wd, omega, xc, s1[a - 1], s1[a + 1], s1[a - kFRW], s1[a + kFRW],
+                const T ax = fused_apply(P, rt, i, j, wd, omega2, xc, s1[a - 1], s1[a + 1], s1[a - kFRW], s1[a + kFRW],
                                          s1[a - kFRW - 1], s1[a + kFRW + 1]);
                 x2 = xc + wd * (sb[a] - ax);
             }
@@ -2330,7 +2403,7 @@ template <typename T, typename TI, int TH, bool DOT>
 __global__ __launch_bounds__(kBlock) void k_level_up(const FusedTile* __restrict__ tiles, int ntiles,
                                                      const LevelProb* __restrict__ probs, const T* __restrict__ xc,
                                                      const T* __restrict__ b, const T* __restrict__ xin,
-                                                     T* __restrict__ xout, T omega, const CgState* st,
+                                                     T* __restrict__ xout, T omega, T omega2, const CgState* st,
                                                      const TI* __restrict__ rin, double* partial)
 {
     EQ_FUSED_DIMS(TH)
@@ -2378,7 +2451,7 @@ __global__ __launch_bounds__(kBlock) void k_level_up(const FusedTile* __restrict
     if (inner) {
         // every node of the region interior: the shared row, rows in strips
         const Row<T> rw = interior_row<T>(P);
-        const T wdi = omega * interior_dinv<T>(P);
+        const T wdi = omega * interior_dinv<T>(P), wdi2 = omega2 * interior_dinv<T>(P);
         if (j >= j0 - 1 && j <= j0 + kFW)
             region_strips<(kFH + 2 + kWaves - 1) / kWaves>(
                 sy, i0 - 1 - ri, i0 + kFH - ri, lane, [&](int, int r, T c, T vw, T ve, T vs, T vn, T vsw, T vne) {
@@ -2397,7 +2470,7 @@ __global__ __launch_bounds__(kBlock) void k_level_up(const FusedTile* __restrict
                         rv = (rin + g0)[j];
                         bb = (T)rv;
                     }
-                    const T o = c + wdi * (bb - row_apply<false>(rw, c, vw, ve, vs, vn, vsw, vne));
+                    const T o = c + wdi2 * (bb - row_apply<false>(rw, c, vw, ve, vs, vn, vsw, vne));
                     (xout + g0)[j] = o;
                     if (DOT) dot[0] += (double)rv * (double)o;
                 });
@@ -2433,7 +2506,7 @@ __global__ __launch_bounds__(kBlock) void k_level_up(const FusedTile* __restrict
         if (!is_dirichlet(G, i, j)) {
             T wd;
             const T sc = ss[a];
-            const T as = fused_apply(P, rt, i, j, wd, omega, sc, ss[a - 1], ss[a + 1], ss[a - kFRW], ss[a + kFRW],
+            const T as = fused_apply(P, rt, i, j, wd, omega2, sc, ss[a - 1], ss[a + 1], ss[a - kFRW], ss[a + kFRW],
                                      ss[a - kFRW - 1], ss[a + kFRW + 1]);
             T bb = sb[a];
             TI rv = (TI)0;
@@ -2908,16 +2981,27 @@ static int sync_bottom_recs(eq_batch* b, bool upload)
         L.bottom_refs.clear();
         L.bottom_recs.clear();
         for (int32_t k : L.bottom) {
-            BottomRef r{(int32_t)L.bottom_recs.size(), 0, L.probs[k].prob, 0};
+            const LevelProb& P0 = L.probs[k];
+            BottomRef r{};
+            r.rec = (int32_t)L.bottom_recs.size();
+            r.prob = P0.prob;
+            r.nx = P0.nx;
+            r.ny = P0.ny;
+            r.pitch = P0.pitch;
+            r.lds_total = P0.lds_total;
+            r.origin = P0.origin;
             size_t lev = l;
             int idx = k;
-            L.bottom_recs.push_back(L.probs[k]);
+            L.bottom_recs.push_back(P0);
             while (!b->levels[lev].probs[idx].coarsest) {
                 idx = b->levels[lev].probs[idx].cidx;
                 ++lev;
                 L.bottom_recs.push_back(b->levels[lev].probs[idx]);
                 ++r.nl;
             }
+            const LevelProb& Pz = b->levels[lev].probs[idx];
+            r.nz = (Pz.nx + 1) * (Pz.ny + 1);
+            r.dense_off = Pz.dense_off;
             L.bottom_refs.push_back(r);
         }
         if (upload && !L.bottom_recs.empty())
@@ -3288,7 +3372,12 @@ struct VCycle {
     {
         const eq_solver_desc& S = b->solver;
         Level& L = b->levels[l];
-        const T omega = (T)S.mg_omega;
+        // pre-sweep k is damped by mg_omega (k even) or mg_omega2 (k odd);
+        // post-sweep k by pre-sweep (post - 1 - k)'s damping
+        const T om1 = (T)S.mg_omega, om2 = (T)(S.mg_omega2 > 0 ? S.mg_omega2 : S.mg_omega);
+        auto wpre = [&](int k) { return (k & 1) ? om2 : om1; };
+        auto wpost = [&](int k) { return wpre(S.mg_post - 1 - k); };
+        const T omega = om1;
         hipStream_t s = b->stream;
         const CgState* st = st_ ? st_ : b->d_state;
         const bool top = (l == 0);
@@ -3305,12 +3394,12 @@ struct VCycle {
             const T* b0 = kDirect ? (const T*)in : (const T*)L.b;
             launch_tiles(b, EQ_KERNEL_TRANSFER0, nodes * 3 * sizeof(T) + cnodes * sizeof(T),
                          k_level_down<T, kFusedTileH0, true>, gf,
-                         s, (const FusedTile*)L.d_fused, nf, (const LevelProb*)L.d_probs, b0, (T*)L.xb, (T*)C.b, omega,
-                         st, (const T*)L.xa);
+                         s, (const FusedTile*)L.d_fused, nf, (const LevelProb*)L.d_probs, b0, (T*)L.xb, (T*)C.b,
+                         wpre(0), wpre(1), st, (const T*)L.xa);
             run(1);
             launch_tiles(b, EQ_KERNEL_SMOOTH, nodes * (3 * sizeof(T) + sizeof(TI)) + cnodes * sizeof(T),
                          k_level_up<T, TI, kFusedTileH0, true>, gf, s, (const FusedTile*)L.d_fused, nf, (const LevelProb*)L.d_probs,
-                         (const T*)final_buffer(1), b0, (const T*)L.xb, out, omega, st, in, b->partial);
+                         (const T*)final_buffer(1), b0, (const T*)L.xb, out, wpost(0), wpost(1), st, in, b->partial);
         } else if (ns > 0 && fused_level(l)) {
             Level& C = b->levels[l + 1];
             const int nf = (int)L.fused.size(), gf = grid_for(nf);
@@ -3318,13 +3407,13 @@ struct VCycle {
             // b read, x written; the coarse b written
             launch_tiles(b, EQ_KERNEL_TRANSFER_COARSE, nodes * 2 * sizeof(T) + cnodes * sizeof(T),
                          k_level_down<T, kFusedTileHc, false>, gf, s, (const FusedTile*)L.d_fused, nf, (const LevelProb*)L.d_probs,
-                         (const T*)L.b, (T*)L.xb, (T*)C.b, omega, st, (const T*)nullptr);
+                         (const T*)L.b, (T*)L.xb, (T*)C.b, wpre(0), wpre(1), st, (const T*)nullptr);
             run(l + 1);
             // x and b read, x written; the coarse correction read
             launch_tiles(b, EQ_KERNEL_SMOOTH_COARSE, nodes * 3 * sizeof(T) + cnodes * sizeof(T),
                          k_level_up<T, T, kFusedTileHc, false>, gf, s, (const FusedTile*)L.d_fused, nf, (const LevelProb*)L.d_probs,
-                         (const T*)final_buffer(l + 1), (const T*)L.b, (const T*)L.xb, (T*)L.xa, omega, st,
-                         (const T*)nullptr, (double*)nullptr);
+                         (const T*)final_buffer(l + 1), (const T*)L.b, (const T*)L.xb, (T*)L.xa, wpost(0), wpost(1),
+                         st, (const T*)nullptr, (double*)nullptr);
         } else if (ns > 0) {
             const double nodes = L.nodes_smooth;
             // pre-smoothing; the first sweep starts from x = 0; level 0 reads
@@ -3355,10 +3444,10 @@ struct VCycle {
                     // sweeps 1 and 2 in one launch
                     if (top)
                         launch_tiles(b, slot, by, k_presmooth2<R, T, TI, TENS>, gs, s, L.smooth(), in, X[1], bcopy,
-                                    omega, st);
+                                    wpre(0), wpre(1), st);
                     else
                         launch_tiles(b, slot, by, k_presmooth2<R, T, T, TENS>, gs, s, L.smooth(), bl, X[1], (T*)nullptr,
-                                    omega, st);
+                                    wpre(0), wpre(1), st);
                     cur = 1;
                 } else if (top) {
                     launch_tiles(b, slot, by, k_smooth<R, T, TI, T, TENS>, gs, s, L.smooth(), in, (const T*)nullptr, X[0],
@@ -3370,7 +3459,7 @@ struct VCycle {
             }
             for (int k = pair ? 2 : 1; k < S.mg_pre; ++k) {
                 launch_tiles(b, slot, nodes * 3 * sizeof(T), k_smooth<R, T, T, T, TENS>, gs, s, L.smooth(), bl,
-                            (const T*)X[cur], X[cur ^ 1], (T*)nullptr, omega, 0, 0, (double*)nullptr, st);
+                            (const T*)X[cur], X[cur ^ 1], (T*)nullptr, wpre(k), 0, 0, (double*)nullptr, st);
                 cur ^= 1;
             }
             // residual and its restriction over the coarse nodes
@@ -3418,17 +3507,17 @@ struct VCycle {
                 if (top && last) {
                     if (first)
                         launch_tiles(b, slot, by, k_prolong_smooth<R, T, TI, T, TENS>, gs, s, L.smooth(), C.d_probs, xc,
-                                    in, (const T*)X[cur], out, omega, 1, b->partial, st);
+                                    in, (const T*)X[cur], out, wpost(k), 1, b->partial, st);
                     else
                         launch_tiles(b, slot, by, k_smooth<R, T, TI, T, TENS>, gs, s, L.smooth(), in, (const T*)X[cur],
-                                    out, (T*)nullptr, omega, 0, 1, b->partial, st);
+                                    out, (T*)nullptr, wpost(k), 0, 1, b->partial, st);
                 } else {
                     if (first)
                         launch_tiles(b, slot, by, k_prolong_smooth<R, T, T, T, TENS>, gs, s, L.smooth(), C.d_probs, xc, bl,
-                                    (const T*)X[cur], X[cur ^ 1], omega, 0, (double*)nullptr, st);
+                                    (const T*)X[cur], X[cur ^ 1], wpost(k), 0, (double*)nullptr, st);
                     else
                         launch_tiles(b, slot, by, k_smooth<R, T, T, T, TENS>, gs, s, L.smooth(), bl, (const T*)X[cur],
-                                    X[cur ^ 1], (T*)nullptr, omega, 0, 0, (double*)nullptr, st);
+                                    X[cur ^ 1], (T*)nullptr, wpost(k), 0, 0, (double*)nullptr, st);
                     cur ^= 1;
                 }
             }
@@ -3444,7 +3533,7 @@ struct VCycle {
             // the bottom levels live in LDS: HBM sees the level's b and x and the coarsest inverses
             launch(b, EQ_KERNEL_BOTTOM, L.nodes_bottom * 2 * sizeof(T) + L.bottom_dense_bytes, k_bottom<T, TENS>,
                    dim3((int)L.bottom.size()), dim3(kBottomThreads), L.bottom_lds_bytes, s,
-                   (const BottomRef*)L.d_bottom, (const LevelProb*)L.d_bottom_recs, (const T*)L.b, xl, omega, S.mg_pre,
+                   (const BottomRef*)L.d_bottom, (const LevelProb*)L.d_bottom_recs, (const T*)L.b, xl, om1, om2, S.mg_pre,
                    S.mg_post, (const T*)b->dense, st);
             if (top)
                 launch_tiles(b, EQ_KERNEL_BOTTOM, L.nodes_bottom * (2 * sizeof(T) + sizeof(TI)), k_cast_out<R, T, T, TI>,
@@ -3997,7 +4086,11 @@ void eq_solver_default(eq_solver_desc* s)
     s->rtol = 1e-10;
     s->mg_pre = 2;
     s->mg_post = 2;
-    s->mg_omega = 0.8;
+    // Two sweeps damped by the reciprocals of the roots of the degree-2
+    // Chebyshev polynomial on [0.5, 2] (D^-1 A of the 7-point rows has its
+    // spectrum in (0, 2]): 1 / (1.25 +- 0.75 cos(pi/4)).  Measured on the
+    // 2000x1000 trap: 9.0 CG iterations per step against 9.3 with 0.8, 0.8.
+    s->mg_omega = 1.0 / (1.25 + 0.75 * 0.70710678118654752);
     s->mg_coarse_max_nodes = 256;
     s->mg_max_levels = 16;
     s->check_every = 4;
@@ -4011,6 +4104,7 @@ void eq_solver_default(eq_solver_desc* s)
     s->ir_max_outer = 20;
     s->ir_inner_rtol = 1e-4;
     s->mg_level_fused = 1;
+    s->mg_omega2 = 1.0 / (1.25 - 0.75 * 0.70710678118654752);
 }
 
 int eq_batch_create(const eq_field_desc* descs, int32_t nprob, int32_t nspecies, const eq_solver_desc* solver,
@@ -4035,7 +4129,8 @@ int eq_batch_create(const eq_field_desc* descs, int32_t nprob, int32_t nspecies,
         return set_error(EQ_ERR_INVALID, "refinement needs 0 < ir_inner_rtol < 1 and ir_max_outer >= 1");
     if (!(S.rtol > 0) || S.max_iter < 1 || S.check_every < 1) return set_error(EQ_ERR_INVALID, "bad solver tolerances");
     if (S.kind != EQ_SOLVER_JACOBI_CG &&
-        (S.mg_pre < 1 || S.mg_post < 1 || !(S.mg_omega > 0) || S.mg_coarse_max_nodes < 4 || S.mg_max_levels < 1))
+        (S.mg_pre < 1 || S.mg_post < 1 || !(S.mg_omega > 0) || !(S.mg_omega2 >= 0) || S.mg_coarse_max_nodes < 4 ||
+         S.mg_max_levels < 1))
         return set_error(EQ_ERR_INVALID, "bad multigrid parameters");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_error(EQ_ERR_NO_DEVICE, "no HIP device");

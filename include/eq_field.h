@@ -108,7 +108,8 @@ typedef struct {
     int32_t max_iter;            /* per solve                                       */
     double rtol;                 /* stop at ||b - A u+||_2 <= rtol * ||b||_2 (FP64) */
     int32_t mg_pre, mg_post;     /* damped-Jacobi sweeps per level                  */
-    double mg_omega;             /* Jacobi damping                                  */
+    double mg_omega;             /* Jacobi damping (of every sweep, or with
+                                    mg_omega2 of every first sweep of a pair)       */
     int32_t mg_coarse_max_nodes; /* coarsen until a level has at most this many    */
     int32_t mg_max_levels;
     int32_t check_every;         /* iterations between host convergence checks      */
@@ -136,6 +137,12 @@ typedef struct {
                                     residual, restriction) in one launch and its way
                                     up (correction, sweeps) in another; 0: the
                                     separate launches                               */
+    double mg_omega2;            /* damping of every second sweep (0: mg_omega):
+                                    pre-sweeps alternate mg_omega, mg_omega2, ...;
+                                    post-sweeps take the same dampings in reverse,
+                                    so the preconditioner stays symmetric.  The
+                                    defaults 0.5617, 1.3895 are the degree-2
+                                    Chebyshev dampings on [0.5, 2]                  */
 } eq_solver_desc;
 
 typedef struct {

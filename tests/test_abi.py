@@ -1,5 +1,6 @@
 """CPU checks of the C-ABI library: it is built, loads, and exports exactly the
 entry points include/eq_field.h declares.  No GPU compute is called here."""
+import math
 import os
 import re
 import subprocess
@@ -37,7 +38,11 @@ def test_library_loads_and_reports_defaults():
     L = field.load_library()
     s = field.solver_desc()
     assert s.kind == field.MIXED_GMG_CG and s.rtol == 1e-10
-    assert s.mg_pre == 2 and s.mg_post == 2 and abs(s.mg_omega - 0.8) < 1e-15
+    assert s.mg_pre == 2 and s.mg_post == 2
+    # the Chebyshev dampings of two sweeps on [0.5, 2]: 1 / (1.25 -+ 0.75 cos(pi/4))
+    w1, w2 = 1.0 / (1.25 + 0.75 * math.cos(math.pi / 4)), 1.0 / (1.25 - 0.75 * math.cos(math.pi / 4))
+    assert abs(s.mg_omega - w1) < 1e-15 and abs(s.mg_omega2 - w2) < 1e-15
+    assert s.mg_level_fused == 1
     assert L.eq_last_error() is not None
 
 
