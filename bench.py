@@ -27,9 +27,9 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# the kernel symbol of each family whose counter run (profiles/traffic.json)
-# stands for the family's traffic
-KERNEL_SYMBOLS = {0: "k_ap_dir", 1: "k_smooth", 2: "k_update", 3: "k_tile_couple", 7: "k_bottom", 8: "k_finish"}
+# profiles/traffic.json: per workload ("config1", "config3", "config4") and
+# kernel family id, the HBM bytes per launch that rocprofv3's FETCH_SIZE and
+# WRITE_SIZE counters give for the family's launches
 
 
 def log(msg):
@@ -244,7 +244,7 @@ def family_profile(F, fb, step_fn, nsteps):
     return fams, eager
 
 
-def roofline_from(fams, step_bytes, step_s, nsteps_profiled, eager_s):
+def roofline_from(fams, step_bytes, step_s, nsteps_profiled, eager_s, workload="config1"):
     """Roofline record of the family with the largest summed kernel time, with
     the whole step's fraction (all launches' algorithmic bytes over the timed
     step) beside it."""
@@ -253,7 +253,7 @@ def roofline_from(fams, step_bytes, step_s, nsteps_profiled, eager_s):
     ach = f["achieved_gbs"]
     step_ach = step_bytes / step_s / 1e9
     return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-            "traffic": load_traffic(KERNEL_SYMBOLS.get(dom, "")),
+            "traffic": load_traffic(workload, dom),
             "kernel": f["family"], "family_id": dom, "share_of_kernel_time": f["share"],
             "launches_per_step": f["launches_per_step"], "avg_us": f["avg_us"],
             "bytes_per_launch": f["bytes_per_step"] / f["launches_per_step"],
@@ -264,11 +264,11 @@ def roofline_from(fams, step_bytes, step_s, nsteps_profiled, eager_s):
                       f"wall time / peak"}
 
 
-def load_traffic(symbol):
+def load_traffic(workload, family):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
-            return json.load(f).get(symbol)
+            return json.load(f).get(workload, {}).get(str(family))
     except (OSError, ValueError):
         return None
 
@@ -441,7 +441,7 @@ def config3_batched(F, W, nsteps, warmup=2, ntraps=512, sigma=0.05, prof=2):
     dt = time.perf_counter() - t0
     step_bytes = fb.traffic(reset=True)[0] / nsteps
     fams, eager = family_profile(F, fb, lambda k: step(total - prof + k), prof) if prof else ({}, 0.0)
-    roof = roofline_from(fams, step_bytes, dt / nsteps, prof, eager) if fams else None
+    roof = roofline_from(fams, step_bytes, dt / nsteps, prof, eager, "config3") if fams else None
     res = {"workload": f"{ntraps} uneven traps (seed 3), h=0.1 um, 2 species, {int(off[-1])} rods",
            "value": nodes * nsteps / dt, "unit": "DOF-timesteps/s", "steps": nsteps,
            "ms_per_step": dt / nsteps * 1e3, "dofs": nodes, "problems": len(descs),
@@ -500,7 +500,7 @@ def config4_advection(F, W, nsteps, warmup=2, prof=1):
     dt = time.perf_counter() - t0
     step_bytes = fb.traffic(reset=True)[0] / nsteps
     fams, eager = family_profile(F, fb, lambda k: step(), prof) if prof else ({}, 0.0)
-    roof = roofline_from(fams, step_bytes, dt / nsteps, prof, eager) if fams else None
+    roof = roofline_from(fams, step_bytes, dt / nsteps, prof, eager, "config4") if fams else None
     res = {"workload": "one 204.8x102.4 um trap, 8192x4096 P1 mesh (33 566 721 nodes/species), 4 species, "
                        "parabolic v_x peak 50 um/s, 100000 static rods (seed 4), mixed GMG-BiCGStab",
            "value": nodes * nsteps / dt, "unit": "DOF-timesteps/s", "steps": nsteps, "warmup": warmup,
@@ -575,7 +575,7 @@ def main():
     trap = W.config1()
     nsteps = args.warmup + args.steps
     log(f"bench: building {nsteps} steps of cell positions for {trap.cells.shape[0]} cells")
-    need = max(nsteps + min(10, args.steps), args.variant_steps + 1, 0 if args.no_variants else args.c2_steps)
+    need = max(nsteps + min(10, args.steps), args.variant_steps + 3, 0 if args.no_variants else args.c2_steps)
     cells_all = make_cells(trap, need)
     cells_dev = torch.from_numpy(cells_all).to("cuda", dtype=torch.float64).contiguous()
     q_dev = torch.from_numpy(trap.rates * 0.1).to("cuda", dtype=torch.float64).contiguous()
@@ -656,15 +656,17 @@ def main():
                 continue
             r = Run(F, trap, kinds[name], cells_dev, q_dev, sampled_dev, max_iter=20000,
                     check_every=(64 if name == "jacobi" else 4))
-            r.step(0)
+            # three untimed steps: the first captures the iteration graphs
+            for k in range(3):
+                r.step(k)
             torch.cuda.synchronize()
             t1 = time.perf_counter()
-            for k in range(1, 1 + args.variant_steps):
+            for k in range(3, 3 + args.variant_steps):
                 r.step(k)
             torch.cuda.synchronize()
             dtv = time.perf_counter() - t1
-            variants[name] = {"value": nodes * args.variant_steps / dtv, "steps": args.variant_steps,
-                              "iterations_max": int(max(r.iters[1:])), "relres_max": float(max(r.relres[1:]))}
+            variants[name] = {"value": nodes * args.variant_steps / dtv, "steps": args.variant_steps, "warmup": 3,
+                              "iterations_max": int(max(r.iters[3:])), "relres_max": float(max(r.relres[3:]))}
             if name == "jacobi":
                 variants[name]["first_step_iterations"] = [int(v) for v in r.fb.problem_stats()[0]]
                 jac0 = Run(F, trap, kinds[name], cells_dev, q_dev, sampled_dev, max_iter=20000, check_every=64)

@@ -31,12 +31,16 @@ KERNEL_FINISH, KERNEL_SOLVE_ONCE, KERNEL_BICG_VEC, KERNEL_COUPLE_SETUP, KERNEL_B
 KERNEL_COUNT = 13
 KERNEL_FAMILIES = {
     KERNEL_STENCIL64: "Krylov operator apply (k_ap_dir, k_bicg_apply)",
-    KERNEL_SMOOTH: "finest-level Jacobi sweeps (k_smooth, k_presmooth2, k_prolong_smooth on level 0)",
+    KERNEL_SMOOTH: "finest-level way up: correction and sweeps (k_level_up; k_smooth, k_presmooth2, "
+                   "k_prolong_smooth on level 0)",
     KERNEL_CG_UPDATE: "Krylov x, r update + dots (k_update, k_bicg_x)",
     KERNEL_COUPLE: "cell sample/deposit per tile (k_tile_couple)",
-    KERNEL_SMOOTH_COARSE: "coarse-level Jacobi sweeps (k_smooth, k_presmooth2, k_prolong_smooth)",
-    KERNEL_TRANSFER0: "finest-level residual and transfers (k_residual, k_restrict, k_residual_restrict, k_prolong)",
-    KERNEL_TRANSFER_COARSE: "coarse-level residual and transfers",
+    KERNEL_SMOOTH_COARSE: "coarse-level way up: correction and sweeps (k_level_up; k_smooth, k_presmooth2, "
+                          "k_prolong_smooth)",
+    KERNEL_TRANSFER0: "finest-level way down: sweep, residual, restriction (k_level_down; k_residual, k_restrict, "
+                      "k_residual_restrict, k_prolong on level 0)",
+    KERNEL_TRANSFER_COARSE: "coarse-level way down: sweeps, residual, restriction (k_level_down; k_residual, "
+                            "k_restrict, k_residual_restrict)",
     KERNEL_BOTTOM: "LDS bottom V-cycle (k_bottom, k_cast_in, k_cast_out)",
     KERNEL_FINISH: "per-problem reduction finish (k_finish)",
     KERNEL_SOLVE_ONCE: "once per solve (k_rhs, k_init, k_true_residual, k_keep, first sweep, k_copy)",
