@@ -167,6 +167,7 @@ struct eq_batch {
     std::vector<std::vector<std::vector<double>>> vel_host; // per problem, per level: row velocities
     std::vector<void*> vel_bufs;       // device copies of the row velocities: [(prob * levels cap + level) * 2 + {0: V-cycle precision, 1: FP64 on level 0}], allocated on first use and reused
     int64_t lds_budget = 64 * 1024;    // LDS bytes the bottom V-cycle may use per workgroup
+    int64_t lds_budget_max = 64 * 1024; // ... at most (a coarsest level above lds_budget takes this)
     std::vector<eqk::Level> levels;
     // level 0 FP64 Krylov vectors (pooled over problems)
     double* u = nullptr;   // field / iterate x

@@ -33,6 +33,8 @@ namespace eqk {
 
 constexpr int kMaxLev = 16;          // multigrid levels per problem
 constexpr int64_t kBottomLdsMax = 150 * 1024; // cap on the LDS bytes of the levels one workgroup runs
+constexpr int64_t kBottomLdsSmall = 40 * 1024; // ... for batches of fewer than kBottomSmallBatch problems
+constexpr int kBottomSmallBatch = 64;
 constexpr int kBottomThreads = 1024;
 constexpr int kFinishThreads = 1024;
 constexpr double kFuseMaxNodes = 512.0 * 1024.0; // levels below this many nodes use the fused multigrid kernels
@@ -3075,12 +3077,18 @@ static int build_hierarchy(eq_batch* b)
             // bottom level: the first level from which every coarser level's
             // three LDS arrays (x0, x1, b with a zero frame) and its level
             // record fit in the LDS budget
+            // (a coarsest level too large for the batch's cap takes the device's)
+            auto level_foot = [&](int k) {
+                return 3 * (int64_t)(dims[p][k].first + 3) * (dims[p][k].second + 3) * (int64_t)sizeof(T) +
+                       (int64_t)sizeof(LevelProb);
+            };
+            const int nlp = (int)dims[p].size();
+            const int64_t budget = 16 + level_foot(nlp - 1) > b->lds_budget ? b->lds_budget_max : b->lds_budget;
             int64_t foot = 16; // alignment of the records after the arrays
-            int l = (int)dims[p].size();
+            int l = nlp;
             while (l > 0) {
-                const int64_t f = 3 * (int64_t)(dims[p][l - 1].first + 3) * (dims[p][l - 1].second + 3) * (int64_t)sizeof(T) +
-                                  (int64_t)sizeof(LevelProb);
-                if (foot + f > b->lds_budget) break;
+                const int64_t f = level_foot(l - 1);
+                if (foot + f > budget) break;
                 foot += f;
                 --l;
             }
@@ -4097,7 +4105,7 @@ void eq_solver_default(eq_solver_desc* s)
     s->use_graphs = 1;
     s->mg_fuse_max_nodes = (int32_t)kFuseMaxNodes;
     s->mg_fuse_pre_max_nodes = (int32_t)kFuseMaxNodes;
-    s->mg_bottom_lds_bytes = (int32_t)kBottomLdsMax;
+    s->mg_bottom_lds_bytes = 0; // by batch size (eq_batch_create)
     s->tile4_min_nodes = (int32_t)kWideRowsMinNodes;
     s->tile8_min_nodes = (int32_t)kWideRows8MinNodes;
     s->extrapolate = 0; // measured: no fewer iterations on the 2000x1000 workload (10 either way)
@@ -4143,8 +4151,16 @@ int eq_batch_create(const eq_field_desc* descs, int32_t nprob, int32_t nspecies,
             lds = 64 * 1024;
         // leave room for the kernel's static shared memory (tables and the
         // coarsest-level partials)
-        const int64_t cap = S.mg_bottom_lds_bytes > 0 ? (int64_t)S.mg_bottom_lds_bytes : kBottomLdsMax;
-        b->lds_budget = std::min<int64_t>(cap, (int64_t)lds - 1024 - kBottomThreads * (int64_t)sizeof(double));
+        // 0: by the batch's size.  The bottom solve runs one workgroup per
+        // problem: a batch of few problems leaves almost every CU idle during
+        // it, so its levels go to the grid kernels sooner (measured: 2000x1000
+        // trap, 2 problems, 40 KiB 1.5 % faster per step than 150 KiB; the
+        // 512-trap batch, 1024 problems, 0.6 % slower)
+        const int64_t cap = S.mg_bottom_lds_bytes > 0 ? (int64_t)S.mg_bottom_lds_bytes
+                            : nprob < kBottomSmallBatch ? kBottomLdsSmall
+                                                        : kBottomLdsMax;
+        b->lds_budget_max = std::min<int64_t>(kBottomLdsMax, (int64_t)lds - 1024 - kBottomThreads * (int64_t)sizeof(double));
+        b->lds_budget = std::min<int64_t>(cap, b->lds_budget_max);
     }
     b->nprob = nprob;
     b->nspecies = nspecies;

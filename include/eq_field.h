@@ -119,7 +119,9 @@ typedef struct {
     int32_t mg_fuse_pre_max_nodes; /* the same for the two pre-sweeps in one launch;
                                       <= 0: mg_fuse_max_nodes                       */
     int32_t mg_bottom_lds_bytes; /* LDS one workgroup may use for the levels it runs
-                                    (the bottom of the V-cycle); <= 0: 153600        */
+                                    (the bottom of the V-cycle); <= 0 (default): 40960
+                                    for batches of fewer than 64 problems, else
+                                    153600                                          */
     int32_t mg_fuse_pre_top;     /* 1: the finest level's first two sweeps run in one
                                     launch (pays off on large fields)               */
     int32_t tile4_min_nodes;     /* coarse levels with at least this many nodes (all
