@@ -18,8 +18,10 @@
 //   5. k_tile_bounds  first/last pair of every tile
 //   6. k_tile_couple  one block per tile, four rows of one column per
 //                     thread, all species in registers: for each of the
-//                     tile's cells in order, the covered nodes' sum (block
-//                     reduction, one partial per pair) and then the deposit
+//                     tile's cells in order, the covered nodes' sum (one
+//                     wave sum per wavefront, summed over the wavefronts
+//                     once per chunk of cells: one partial per pair) and
+//                     then the deposit
 //   7. k_cell_sample  per cell: its pair partials in tile order / |S_c|
 #include <hip/hip_runtime.h>
 #include <rocprim/device/device_radix_sort.hpp>
@@ -331,8 +333,11 @@ __device__ __forceinline__ double wave_sum64(double v)
 }
 
 // one block per tile of the call's trap range
-constexpr int kChunk = 128; // cells staged in LDS at a time
+constexpr int kChunk = 64; // cells staged in LDS at a time
 
+// P species held in registers per pass (2 covers one or two species, 4 more);
+// TENS: a trap of the call sets its diffusion tensor from the cells
+template <int P, bool TENS>
 __global__ __launch_bounds__(kCThreads) void k_tile_couple(
     const TrapGrid* __restrict__ traps, int t0, int ntr, uint32_t tile0, const uint32_t* __restrict__ start,
     const uint32_t* __restrict__ end, const uint32_t* __restrict__ sorted_pairs, const uint32_t* __restrict__ pair_cell,
@@ -340,13 +345,13 @@ __global__ __launch_bounds__(kCThreads) void k_tile_couple(
     double* __restrict__ partial, double npm, unsigned flags, double* const* __restrict__ tens64,
     float* const* __restrict__ tens32, int64_t tstride)
 {
-    __shared__ double red[2][kCThreads / 64][kSpeciesPass]; // double-buffered wave sums
+    __shared__ double red[kChunk][kCThreads / 64][P]; // per cell of the chunk: each wave's sums
     __shared__ CellGeo sg[kChunk];      // cx, cy replaced by the inverse translation
     __shared__ uint32_t sp[kChunk];
-    __shared__ double sq[kChunk][kSpeciesPass]; // q_c / |S_c|
+    __shared__ double sq[kChunk][P]; // q_c / |S_c|
     const uint32_t tl = blockIdx.x;
     const uint32_t e0 = start[tl], e1 = end[tl];
-    const bool want_tens = (flags & EQ_COUPLE_TENSOR) != 0;
+    const bool want_tens = TENS && (flags & EQ_COUPLE_TENSOR) != 0;
     if (e0 >= e1 && !want_tens) return;
     // the tile's trap: traps are numbered in tile order
     int lo = t0, hi = t0 + ntr - 1;
@@ -376,21 +381,20 @@ __global__ __launch_bounds__(kCThreads) void k_tile_couple(
         t22[r] = 1.0;
         t12[r] = 0.0;
     }
-    int buf = 0;
-    for (int s0 = 0; s0 < ns && e0 < e1; s0 += kSpeciesPass) {
-        const int nsp = min(kSpeciesPass, ns - s0);
-        double u[kSpeciesPass][kCRows];
+    for (int s0 = 0; s0 < ns && e0 < e1; s0 += P) {
+        const int nsp = min(P, ns - s0);
+        double u[P][kCRows];
         bool in[kCRows];
 #pragma unroll
         for (int r = 0; r < kCRows; ++r) {
             in[r] = (ib + r < G.nodes_h) && (j < G.nodes_w);
 #pragma unroll
-            for (int s = 0; s < kSpeciesPass; ++s)
+            for (int s = 0; s < P; ++s)
                 u[s][r] = (s < nsp && in[r]) ? fields[G.prob0 + s0 + s][(int64_t)(ib + r) * G.pitch + j] : 0.0;
         }
         for (uint32_t cb = e0; cb < e1; cb += kChunk) {
             const int n = (int)min((uint32_t)kChunk, e1 - cb);
-            __syncthreads(); // the previous chunk's entries are no longer read
+            __syncthreads(); // the previous chunk's entries and wave sums are no longer read
             if ((int)threadIdx.x < n) {
                 const uint32_t p = sorted_pairs[cb + threadIdx.x];
                 const uint32_t c = pair_cell[p];
@@ -402,7 +406,7 @@ __global__ __launch_bounds__(kCThreads) void k_tile_couple(
                 sp[threadIdx.x] = p;
                 const double cntd = (double)(g.cnt > 0 ? g.cnt : 1);
 #pragma unroll
-                for (int s = 0; s < kSpeciesPass; ++s)
+                for (int s = 0; s < P; ++s)
                     sq[threadIdx.x][s] = (s < nsp && dep) ? q[(int64_t)c * ns + s0 + s] / cntd : 0.0;
             }
             __syncthreads();
@@ -411,6 +415,14 @@ __global__ __launch_bounds__(kCThreads) void k_tile_couple(
                 const double cs = g.cs, sn = g.sn, itx = g.cx, ity = g.cy, o = g.o, rad = g.rad;
                 const bool fallback = g.cnt == 0;
                 const bool colin = j >= g.j1 && j <= g.j2;
+                // a wavefront whose four rows miss the cell's window adds nothing
+                // (uniform over the wavefront)
+                if (__ballot(colin && ib <= g.i2 && ib + kCRows - 1 >= g.i1) == 0ull) {
+                    if (lane == 0)
+#pragma unroll
+                        for (int s = 0; s < P; ++s) red[k][w][s] = 0.0;
+                    continue;
+                }
                 const double csX = cs * X, snX = (-sn) * X;
                 bool cov[kCRows];
                 bool anyc = false;
@@ -438,7 +450,7 @@ __global__ __launch_bounds__(kCThreads) void k_tile_couple(
                 }
                 const bool wave_any = __ballot(anyc) != 0ull; // uniform over the wavefront
 #pragma unroll
-                for (int s = 0; s < kSpeciesPass; ++s) {
+                for (int s = 0; s < P; ++s) {
                     if (s >= nsp) break;
                     double sum = 0.0;
                     const double d = sq[k][s];
@@ -450,25 +462,25 @@ __global__ __launch_bounds__(kCThreads) void k_tile_couple(
                         }
                     }
                     if (wave_any) sum = wave_sum64(sum);
-                    if (lane == 0) red[buf][w][s] = wave_any ? sum : 0.0;
+                    if (lane == 0) red[k][w][s] = wave_any ? sum : 0.0;
                 }
-                __syncthreads();
-                // per-pair partial sums, waves in order; the other buffer takes
-                // the next cell, so no second barrier is needed
-                if ((int)threadIdx.x < nsp) {
-                    const int s = threadIdx.x;
-                    double acc = 0.0;
-                    for (int q4 = 0; q4 < kCThreads / 64; ++q4) acc += red[buf][q4][s];
-                    partial[(int64_t)sp[k] * ns + s0 + s] = acc;
-                }
-                buf ^= 1;
+            }
+            // the waves walk the chunk's cells independently (each owns its
+            // own nodes); the per-pair partial sums, waves in order, follow
+            // once every wave is through the chunk
+            __syncthreads();
+            for (int t = threadIdx.x; t < n * nsp; t += kCThreads) {
+                const int k = t / nsp, s = t - k * nsp;
+                double acc = 0.0;
+                for (int q4 = 0; q4 < kCThreads / 64; ++q4) acc += red[k][q4][s];
+                partial[(int64_t)sp[k] * ns + s0 + s] = acc;
             }
         }
         if (dep) {
 #pragma unroll
             for (int r = 0; r < kCRows; ++r)
 #pragma unroll
-                for (int s = 0; s < kSpeciesPass; ++s)
+                for (int s = 0; s < P; ++s)
                     if (s < nsp && in[r]) fields[G.prob0 + s0 + s][(int64_t)(ib + r) * G.pitch + j] = u[s][r];
         }
     }
@@ -783,7 +795,9 @@ static int couple_impl(eq_batch* b, int32_t t0, int32_t ntr, const int64_t* cell
         for (int t : b->cell_tensor_traps) tens = tens || (t >= t0 && t < t0 + ntr);
     const unsigned kflags = tens ? flags : (flags & ~EQ_COUPLE_TENSOR);
     const double tb = tens ? nodes * ns * 3.0 * (sizeof(double) + (b->mixed ? sizeof(float) : 0)) : 0.0;
-    launch(b, EQ_KERNEL_COUPLE, cb + tb, k_tile_couple, dim3(ntiles), dim3(kCThreads), 0, s, traps, t0, ntr, tile0,
+    auto kern = ns <= 2 ? (tens ? k_tile_couple<2, true> : k_tile_couple<2, false>)
+                        : (tens ? k_tile_couple<kSpeciesPass, true> : k_tile_couple<kSpeciesPass, false>);
+    launch(b, EQ_KERNEL_COUPLE, cb + tb, kern, dim3(ntiles), dim3(kCThreads), 0, s, traps, t0, ntr, tile0,
            (const uint32_t*)t_start, (const uint32_t*)t_end, (const uint32_t*)vals2, (const uint32_t*)pair_cell,
            (const CellGeo*)geo, (double* const*)b->d_field_ptrs, ns, d_q, partial, npm, kflags,
            (double* const*)b->d_tens64, (float* const*)b->d_tens32, (int64_t)b->levels[0].pool);
