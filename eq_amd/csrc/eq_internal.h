@@ -202,9 +202,13 @@ struct eq_batch {
     hipGraphExec_t g_ir2 = nullptr;
     eqk::Acct g_ir2_acct;
     int32_t last_iters = 0;
-    // captured launch sequences (created on first use): two CG iterations
-    // (the search-direction buffers swap back after two), and the set-up
-    hipGraphExec_t g_iter2 = nullptr;
+    // captured launch sequences (created on first use): the set-up, and CG
+    // iterations: g_cg[d][n][t] runs n iterations (1 <= n <= kCgGraphMax)
+    // from search direction p (d = 0) or p2 (d = 1), then (t = 1) the
+    // convergence confirmation
+    static constexpr int kCgGraphMax = 16;
+    hipGraphExec_t g_cg[2][kCgGraphMax + 1][2] = {};
+    eqk::Acct g_cg_acct[2][kCgGraphMax + 1][2];
     hipGraphExec_t g_init = nullptr;
     std::vector<int32_t> iters;
     std::vector<double> relres;
@@ -235,7 +239,7 @@ struct eq_batch {
     // at every replay)
     eqk::Acct acct;
     eqk::Acct* acct_to = &acct;
-    eqk::Acct g_init_acct, g_iter2_acct, g_bicg_acct;
+    eqk::Acct g_init_acct, g_bicg_acct;
     // tensors set from the cells' angles (allocated on first use): per problem,
     // node (0,0) of its finest-level tensor in FP64 and (FP32 V-cycle) FP32;
     // the coarsest inverses to rebuild after each such coupling

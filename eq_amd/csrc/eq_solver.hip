@@ -244,7 +244,10 @@ enum FinishOp {
     // BiCGStab
     kFinBInit = 6, kFinBAlpha = 7, kFinBOmega = 8, kFinBUpdate = 9, kFinBRestart = 10,
     // iterative refinement: the inner solve's start, the outer residual check
-    kFinIrInit = 11, kFinIrOuter = 12
+    kFinIrInit = 11, kFinIrOuter = 12,
+    // CG confirmation on the device: kFinTrue for the problems that report
+    // convergence only; kFinRz for the problems it restarted only
+    kFinTrueDone = 13, kFinRzRestart = 14
 };
 
 template <typename T>
@@ -598,15 +601,22 @@ __global__ __launch_bounds__(kBlock) void k_update(LevelView L, TV* __restrict__
     tile_partial<3>(v, partial, c.gt);
 }
 
-// true residual r = b - A x (replaces the recurrence residual); final stop test
+// true residual r = b - A x (replaces the recurrence residual); final stop test.
+// done_only: the problems that report convergence only (the others are still
+// iterating and keep their recurrence residual).  keep (optional): x is also
+// kept as the next solve's start (the confirmation that passes is the last
+// read of the final x), and the problem marked as having one.
 template <int R0, bool TENS>
 __global__ __launch_bounds__(kBlock) void k_true_residual(LevelView L, const double* __restrict__ x,
                                                           const double* __restrict__ b, double* __restrict__ r,
-                                                          double* partial)
+                                                          double* partial, CgState* st, int done_only,
+                                                          double* __restrict__ keep)
 {
     NodeCtx c;
     if (!decode<R0>(L, c)) return;
+    if (done_only && !st[c.prob].done) return;
     const LevelProb P = *c.P;
+    if (keep && c.gt == P.tile_begin && threadIdx.x == 0) st[c.prob].have_prev = 1;
     double v[1] = {0.0};
     if (c.full) {
         // interior tile: every load first
@@ -623,18 +633,24 @@ __global__ __launch_bounds__(kBlock) void k_true_residual(LevelView L, const dou
             const Row<double> rc = row_coefs<double, TENS>(P, c.i + w, c.j, k);
             const double rv = bl[w] - X.template apply<TENS>(rc, w);
             r[k] = rv;
+            if (keep) keep[k] = X.m[w + 1];
             v[0] += rv * rv;
         });
         tile_partial<1>(v, partial, c.gt);
         return;
     }
     each_row<R0>(c, [&](const NodeCtx& n) {
-        if (!n.in || is_dirichlet(P, n.i, n.j)) return;
+        if (!n.in) return;
+        if (is_dirichlet(P, n.i, n.j)) {
+            if (keep) keep[n.idx] = x[n.idx];
+            return;
+        }
         double xc, xw, xe, xs, xn, xsw, xne;
         load7(x, n.idx, P.pitch, xc, xw, xe, xs, xn, xsw, xne);
         const Row<double> rc = row_coefs<double, TENS>(P, n.i, n.j, n.idx);
         const double rv = b[n.idx] - row_apply<TENS>(rc, xc, xw, xe, xs, xn, xsw, xne);
         r[n.idx] = rv;
+        if (keep) keep[n.idx] = xc;
         v[0] += rv * rv;
     });
     tile_partial<1>(v, partial, c.gt);
@@ -720,12 +736,13 @@ __global__ __launch_bounds__(kBlock) void k_ir_apply(LevelView L, double* __rest
 
 template <int R0, bool TENS>
 __global__ __launch_bounds__(kBlock) void k_jacobi_z(LevelView L, const double* __restrict__ r,
-                                                     double* __restrict__ z, double* partial, const CgState* st)
+                                                     double* __restrict__ z, double* partial, const CgState* st,
+                                                     int restart_only)
 {
     NodeCtx c;
     if (!decode<R0>(L, c)) return;
     const LevelProb P = *c.P;
-    if (st[c.prob].done) return;
+    if (st[c.prob].done || (restart_only && !st[c.prob].restart)) return;
     double v[1] = {0.0};
     each_row<R0>(c, [&](const NodeCtx& n) {
         if (!n.in) return;
@@ -773,6 +790,7 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(const LevelProb* __re
     if ((op == kFinAp || op == kFinUpdate || op == kFinRz || op == kFinBAlpha || op == kFinBOmega ||
          op == kFinBUpdate || op == kFinIrInit) && s.done)
         return;
+    if ((op == kFinTrueDone && !s.done) || (op == kFinRzRestart && (s.done || !s.restart))) return;
     block_sum<3, kFinishThreads>(v);
     if (threadIdx.x != 0) return;
     switch (op) {
@@ -808,6 +826,7 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(const LevelProb* __re
         s.done = (v[0] <= s.tol2 * s.bb || s.iter >= s.max_iter) ? 1 : 0;
         break;
     case kFinRz:
+    case kFinRzRestart:
         if (s.iter == 0 || s.restart) {
             s.beta = 0.0;
             s.restart = 0;
@@ -818,6 +837,7 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(const LevelProb* __re
         s.rz = v[0];
         break;
     case kFinTrue:
+    case kFinTrueDone:
         s.rr_true = v[0];
         s.rr = v[0];
         s.done = (v[0] <= s.tol2 * s.bb || s.iter >= s.max_iter) ? 1 : 0;
@@ -1140,7 +1160,8 @@ __device__ __forceinline__ T prolong_val(const LevelProb& Pf, const LevelProb& P
 
 // Damped Jacobi sweep over the launched tiles.  bcopy (optional) keeps b in
 // the smoother precision; dot (last level-0 sweep) adds the partial r.z with
-// b the FP64 residual and x_out the preconditioned residual.
+// b the FP64 residual and x_out the preconditioned residual.  zero_init 2:
+// from x = 0 for the problems a confirmation has just restarted only.
 template <int R, typename T, typename TB, typename TO, bool TENS>
 __global__ __launch_bounds__(kBlock) void k_smooth(LevelView L, const TB* __restrict__ b, const T* __restrict__ xin,
                                                    TO* __restrict__ xout, T* __restrict__ bcopy, T omega,
@@ -1149,7 +1170,7 @@ __global__ __launch_bounds__(kBlock) void k_smooth(LevelView L, const TB* __rest
     NodeCtx c;
     if (!decode<R>(L, c)) return;
     const LevelProb P = *c.P;
-    if (st[c.prob].done) return;
+    if (st[c.prob].done || (zero_init == 2 && !st[c.prob].restart)) return;
     double v[1] = {0.0};
     if (R > 1 && c.full && !(TENS && P.aniso)) {
         // interior tile without a tensor field: the same arithmetic as
@@ -3552,6 +3573,22 @@ struct VCycle {
 
 // ------------------------------------------------------------ solve driver
 
+// Captured launch sequences hold the batch's buffers, cycle shape and kernel
+// specialisations: a change of any of them drops them all (recaptured on use).
+static void drop_graphs(eq_batch* b)
+{
+    for (auto& d : b->g_cg)
+        for (auto& n : d)
+            for (hipGraphExec_t& g : n) {
+                if (g) hipGraphExecDestroy(g);
+                g = nullptr;
+            }
+    for (hipGraphExec_t* g : {&b->g_init, &b->g_bicg, &b->g_ir2}) {
+        if (*g) hipGraphExecDestroy(*g);
+        *g = nullptr;
+    }
+}
+
 static int read_states(eq_batch* b)
 {
     EQ_HIP(hipMemcpyAsync(b->h_state, b->d_state, sizeof(CgState) * b->nprob, hipMemcpyDeviceToHost, b->stream));
@@ -3584,12 +3621,13 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
     T* bcopy = (!jacobi && b->mixed) ? (T*)L0.b : nullptr;
     const T omega = (T)S.mg_omega;
     vc.pre1 = !jacobi;
-    auto first_sweep = [&]() {
+    // zero_init 2: for the problems a confirmation restarted only
+    auto first_sweep = [&](int zero_init) {
         const int ns = (int)L0.tiles_smooth.size();
         if (jacobi || ns == 0) return;
         launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, L0.nodes_smooth * (sizeof(double) + sizeof(T) * (bcopy ? 2 : 1)),
                      k_smooth<R0, T, double, T, TENS>, grid_for(ns), s, L0.smooth(), (const double*)b->r,
-                     (const T*)nullptr, x1, bcopy, omega, 1, 0, (double*)nullptr, (const CgState*)b->d_state);
+                     (const T*)nullptr, x1, bcopy, omega, zero_init, 0, (double*)nullptr, (const CgState*)b->d_state);
     };
     // one CG iteration from search direction pold to pnew
     auto iteration = [&](double* pold, double* pnew) {
@@ -3610,9 +3648,33 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
                     (const CgState*)b->d_state, jacobi ? 1 : 0, x1, bcopy, omega);
         finish(kFinUpdate);
     };
-    auto two_iterations = [&]() {
-        iteration(b->p, b->p2);
-        iteration(b->p2, b->p);
+    // Without extrapolation the confirmation that passes keeps the solution
+    // as the next solve's start; with it, k_keep shifts the two kept ones
+    // once the solve is over.
+    double* keep = S.extrapolate ? nullptr : b->xprev;
+    // The confirmation after every chunk of iterations: problems that report
+    // convergence get their true residual (the others are left to iterate);
+    // those above rtol restart CG from it and form their first sweep (or
+    // Jacobi z) again.  Repeating it on a confirmed problem changes nothing.
+    auto confirm = [&]() {
+        launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, fine_nodes * (keep ? 32.0 : 24.0), k_true_residual<R0, TENS>, g0, s, A,
+                     (const double*)b->u, (const double*)b->rhs, b->r, b->partial, b->d_state, 1, keep);
+        finish(kFinTrueDone);
+        first_sweep(2);
+        if (jacobi) {
+            launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, fine_nodes * 16.0, k_jacobi_z<R0, TENS>, g0, s, A,
+                         (const double*)b->r, b->z, b->partial, (const CgState*)b->d_state, 1);
+            finish(kFinRzRestart);
+        }
+    };
+    // n iterations from search direction p (dir 0) or p2 (dir 1): the two
+    // buffers swap roles every iteration
+    auto iterations = [&](int dir, int n, bool tail) {
+        for (int k = 0; k < n; ++k) {
+            const bool odd = ((dir + k) & 1) != 0;
+            iteration(odd ? b->p2 : b->p, odd ? b->p : b->p2);
+        }
+        if (tail) confirm();
     };
     auto setup = [&]() {
         launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, fine_nodes * 16.0, k_rhs<R0>, g0, s, A, (const double*)b->u, b->rhs,
@@ -3622,7 +3684,7 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
                      b->u, (const double*)b->xprev, (const double*)b->xprev2, (const double*)b->rhs, b->r, b->z,
                      b->partial, (const CgState*)b->d_state, jacobi ? 1 : 0);
         finish(kFinInit);
-        first_sweep();
+        first_sweep(1);
     };
     // Launch sequences are replayed from hipGraphs unless per-launch event
     // timing is on (events need eager launches).
@@ -3649,9 +3711,7 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
     };
     int rc;
     if (graphs) {
-        if ((rc = capture(&b->g_init, &b->g_init_acct, setup)) ||
-            (rc = capture(&b->g_iter2, &b->g_iter2_acct, two_iterations)))
-            return rc;
+        if ((rc = capture(&b->g_init, &b->g_init_acct, setup))) return rc;
         EQ_HIP(hipGraphLaunch(b->g_init, s));
         replayed(b, b->g_init_acct);
     } else {
@@ -3659,41 +3719,38 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
     }
     EQ_HIP(hipGetLastError());
 
+    // Each host check follows a chunk of iterations and the confirmation; the
+    // first chunk is the last solve's iteration count, so a solve that needs
+    // as many again takes one graph launch and one read of the states.
+    constexpr int kMaxG = eq_batch::kCgGraphMax;
     int chunk = std::max(1, b->last_iters > 0 ? b->last_iters : S.check_every);
-    for (int guard = 0; guard < 1000000; ++guard) {
-        // iterations go in pairs, so p holds the current direction between chunks
-        for (int k = 0; k < chunk; k += 2) {
+    int dir = 0;
+    bool all = false;
+    for (int guard = 0; guard < 1000000 && !all; ++guard) {
+        for (int left = chunk; left > 0;) {
+            const int n = std::min(left, kMaxG);
+            const bool tail = (n == left);
             if (graphs) {
-                EQ_HIP(hipGraphLaunch(b->g_iter2, s));
-                replayed(b, b->g_iter2_acct);
+                hipGraphExec_t* g = &b->g_cg[dir][n][tail ? 1 : 0];
+                Acct* ac = &b->g_cg_acct[dir][n][tail ? 1 : 0];
+                if ((rc = capture(g, ac, [&]() { iterations(dir, n, tail); }))) return rc;
+                EQ_HIP(hipGraphLaunch(*g, s));
+                replayed(b, *ac);
             } else {
-                two_iterations();
+                iterations(dir, n, tail);
             }
+            dir = (dir + n) & 1;
+            left -= n;
         }
         EQ_HIP(hipGetLastError());
         if ((rc = read_states(b))) return rc;
-        bool all = true;
+        all = true;
         for (int p = 0; p < b->nprob; ++p) all = all && b->h_state[p].done;
-        if (all) {
-            // confirm on the true residual; problems above rtol restart CG from it
-            launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, fine_nodes * 24.0, k_true_residual<R0, TENS>, g0, s, A,
-                         (const double*)b->u, (const double*)b->rhs, b->r, b->partial);
-            finish(kFinTrue);
-            first_sweep();
-            if ((rc = read_states(b))) return rc;
-            all = true;
-            for (int p = 0; p < b->nprob; ++p) all = all && b->h_state[p].done;
-            if (all) break;
-            if (jacobi) {
-                launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, fine_nodes * 16.0, k_jacobi_z<R0, TENS>, g0, s, A,
-                             (const double*)b->r, b->z, b->partial, (const CgState*)b->d_state);
-                finish(kFinRz);
-            }
-        }
         chunk = std::max(1, S.check_every);
     }
-    launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, fine_nodes * (S.extrapolate ? 32.0 : 16.0), k_keep<R0>, g0, s, A,
-                 (const double*)b->u, b->xprev, b->xprev2, b->d_state);
+    if (!keep)
+        launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, fine_nodes * 32.0, k_keep<R0>, g0, s, A, (const double*)b->u, b->xprev,
+                     b->xprev2, b->d_state);
     EQ_HIP(hipGetLastError());
     prof_collect(b);
     int32_t itmax = 0, nconv = 0;
@@ -3827,7 +3884,7 @@ static int solve_bicgstab(eq_batch* b, eq_step_stats* stats)
         if (all) {
             // confirm on the true residual; the others restart from r^ = r
             launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, n0 * 24.0, k_true_residual<R0, TENS>, g0, s, A, (const double*)b->u,
-                         (const double*)b->rhs, b->r, b->partial);
+                         (const double*)b->rhs, b->r, b->partial, st, 0, (double*)nullptr);
             finish(kFinTrue);
             launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, n0 * 16.0, k_copy<R0>, g0, s, A, (const double*)b->r, b->bs_rhat,
                          (const CgState*)st);
@@ -4003,7 +4060,7 @@ static int solve_ir(eq_batch* b, eq_step_stats* stats)
         launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, n0 * (16.0 + sizeof(T)), k_ir_apply<R0>, g0, s, A, b->u,
                      (const T*)b->ir_d, (const double*)b->ir_scale, (const CgState*)so);
         launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, n0 * 24.0, k_true_residual<R0, TENS>, g0, s, A, (const double*)b->u,
-                     (const double*)b->rhs, b->r, b->partial);
+                     (const double*)b->rhs, b->r, b->partial, so, 0, (double*)nullptr);
         finish(so, kFinIrOuter);
         if ((rc = read_both())) return rc;
     }
@@ -4246,8 +4303,7 @@ void eq_batch_destroy(eq_batch* b)
     if (!b) return;
     hipSetDevice(b->device);
     if (b->stream) hipStreamSynchronize(b->stream);
-    if (b->g_iter2) hipGraphExecDestroy(b->g_iter2);
-    if (b->g_init) hipGraphExecDestroy(b->g_init);
+    drop_graphs(b);
     for (Level& L : b->levels) {
         hipFree(L.d_probs);
         hipFree(L.d_tiles_all);
@@ -4513,14 +4569,7 @@ static void refresh_general(eq_batch* b)
     for (const LevelProb& P : b->levels[0].probs) any = any || P.aniso || P.vel;
     if (any != b->any_tensor) {
         b->any_tensor = any;
-        if (b->g_iter2) hipGraphExecDestroy(b->g_iter2);
-        if (b->g_init) hipGraphExecDestroy(b->g_init);
-        if (b->g_bicg) hipGraphExecDestroy(b->g_bicg);
-        if (b->g_ir2) hipGraphExecDestroy(b->g_ir2);
-        b->g_iter2 = nullptr;
-        b->g_init = nullptr;
-        b->g_bicg = nullptr;
-        b->g_ir2 = nullptr;
+        drop_graphs(b);
     }
 }
 
@@ -4825,16 +4874,6 @@ int eq_batch_autotune(eq_batch* b, int32_t repeats, eq_solver_desc* chosen)
         EQ_HIP(hipStreamSynchronize(s));
         return EQ_OK;
     };
-    auto drop_graphs = [&]() {
-        if (b->g_iter2) hipGraphExecDestroy(b->g_iter2);
-        if (b->g_init) hipGraphExecDestroy(b->g_init);
-        if (b->g_bicg) hipGraphExecDestroy(b->g_bicg);
-        if (b->g_ir2) hipGraphExecDestroy(b->g_ir2);
-        b->g_iter2 = nullptr;
-        b->g_init = nullptr;
-        b->g_bicg = nullptr;
-        b->g_ir2 = nullptr;
-    };
     const eq_solver_desc orig = b->solver;
     const int32_t orig_last = b->last_iters;
     int rc = copy_state(true);
@@ -4857,7 +4896,7 @@ int eq_batch_autotune(eq_batch* b, int32_t repeats, eq_solver_desc* chosen)
         b->solver.mg_post = kTuneShapes[shape][1];
         b->solver.mg_fuse_max_nodes = fuse;
         b->solver.mg_fuse_pre_top = top;
-        drop_graphs();
+        drop_graphs(b);
         double t = 0.0;
         bool ok = true;
         for (int r = 0; r <= reps && rc == EQ_OK; ++r) { // r = 0 captures the graphs, untimed
@@ -4890,7 +4929,7 @@ int eq_batch_autotune(eq_batch* b, int32_t repeats, eq_solver_desc* chosen)
         b->solver.mg_fuse_pre_top = best_top;
     }
     b->last_iters = orig_last;
-    drop_graphs();
+    drop_graphs(b);
     hipFree(save);
     hipFree(save_st);
     if (rc != EQ_OK) return rc;
