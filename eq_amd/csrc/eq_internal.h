@@ -203,12 +203,12 @@ struct eq_batch {
     eqk::Acct g_ir2_acct;
     int32_t last_iters = 0;
     // captured launch sequences (created on first use): the set-up, and CG
-    // iterations: g_cg[d][n][t] runs n iterations (1 <= n <= kCgGraphMax)
-    // from search direction p (d = 0) or p2 (d = 1), then (t = 1) the
-    // convergence confirmation
+    // iterations: g_cg[f][d][n][t] runs the solve's set-up (f = 1), then n
+    // iterations (1 <= n <= kCgGraphMax) from search direction p (d = 0) or
+    // p2 (d = 1), then (t = 1) the convergence confirmation
     static constexpr int kCgGraphMax = 16;
-    hipGraphExec_t g_cg[2][kCgGraphMax + 1][2] = {};
-    eqk::Acct g_cg_acct[2][kCgGraphMax + 1][2];
+    hipGraphExec_t g_cg[2][2][kCgGraphMax + 1][2] = {};
+    eqk::Acct g_cg_acct[2][2][kCgGraphMax + 1][2];
     hipGraphExec_t g_init = nullptr;
     std::vector<int32_t> iters;
     std::vector<double> relres;

@@ -760,6 +760,53 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_z(LevelView L, const double* 
 // Per-problem totals of the tile partials, summed in tile order
 // (deterministic), and the Krylov scalar updates that follow from them.
 // One block per problem; problems the producer skipped (done) are skipped.
+// slots OFF.. OFF+NV-1 of v += the problem's tile partials 0..NV-1 (ntiles
+// tiles from base), in a fixed order: every total is deterministic
+template <int NV, int OFF, int NT>
+__device__ __forceinline__ void sum_tile_partials(const double* base, int ntiles, double* v)
+{
+    int q = threadIdx.x;
+    for (; q + 3 * NT < ntiles; q += 4 * NT) {
+        const double* p0 = base + (int64_t)q * 4;
+        const double* p1 = p0 + 4 * NT;
+        const double* p2 = p1 + 4 * NT;
+        const double* p3 = p2 + 4 * NT;
+#pragma unroll
+        for (int k = 0; k < NV; ++k) v[OFF + k] += (p0[k] + p1[k]) + (p2[k] + p3[k]);
+    }
+    for (; q < ntiles; q += NT) {
+        const double* p0 = base + (int64_t)q * 4;
+#pragma unroll
+        for (int k = 0; k < NV; ++k) v[OFF + k] += p0[k];
+    }
+}
+
+// CG scalars after k_update: v[0] = r.r, v[1] = r.z (Jacobi), v[2] = r_new.z_old
+__device__ __forceinline__ void fin_update(CgState& s, const double* v, int jacobi)
+{
+    s.iter += 1;
+    s.rr = v[0];
+    s.rzold = v[2];
+    if (jacobi) {
+        s.beta = v[1] / s.rz;
+        s.rz = v[1];
+    }
+    s.done = (v[0] <= s.tol2 * s.bb || s.iter >= s.max_iter) ? 1 : 0;
+}
+
+// CG scalars after the preconditioner: rz = r.z, beta (Polak-Ribiere when flexible)
+__device__ __forceinline__ void fin_rz(CgState& s, double rz)
+{
+    if (s.iter == 0 || s.restart) {
+        s.beta = 0.0;
+        s.restart = 0;
+    } else {
+        const double num = s.flexible ? (rz - s.rzold) : rz;
+        s.beta = num / s.rz;
+    }
+    s.rz = rz;
+}
+
 __global__ __launch_bounds__(kFinishThreads) void k_finish(const LevelProb* __restrict__ probs,
                                                            const double* __restrict__ partial, CgState* st, int op,
                                                            int jacobi)
@@ -770,23 +817,7 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(const LevelProb* __re
     // and the state's are in flight together (a finished problem's partials
     // are read and dropped)
     double v[3] = {0.0, 0.0, 0.0};
-    const double* base = partial + (int64_t)P.tile_begin * 4;
-    int q = threadIdx.x;
-    for (; q + 3 * kFinishThreads < P.ntiles; q += 4 * kFinishThreads) {
-        const double* p0 = base + (int64_t)q * 4;
-        const double* p1 = p0 + 4 * kFinishThreads;
-        const double* p2 = p1 + 4 * kFinishThreads;
-        const double* p3 = p2 + 4 * kFinishThreads;
-        v[0] += (p0[0] + p1[0]) + (p2[0] + p3[0]);
-        v[1] += (p0[1] + p1[1]) + (p2[1] + p3[1]);
-        v[2] += (p0[2] + p1[2]) + (p2[2] + p3[2]);
-    }
-    for (; q < P.ntiles; q += kFinishThreads) {
-        const double* p0 = base + (int64_t)q * 4;
-        v[0] += p0[0];
-        v[1] += p0[1];
-        v[2] += p0[2];
-    }
+    sum_tile_partials<3, 0, kFinishThreads>(partial + (int64_t)P.tile_begin * 4, P.ntiles, v);
     if ((op == kFinAp || op == kFinUpdate || op == kFinRz || op == kFinBAlpha || op == kFinBOmega ||
          op == kFinBUpdate || op == kFinIrInit) && s.done)
         return;
@@ -816,25 +847,11 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(const LevelProb* __re
         }
         break;
     case kFinUpdate:
-        s.iter += 1;
-        s.rr = v[0];
-        s.rzold = v[2];
-        if (jacobi) {
-            s.beta = v[1] / s.rz;
-            s.rz = v[1];
-        }
-        s.done = (v[0] <= s.tol2 * s.bb || s.iter >= s.max_iter) ? 1 : 0;
+        fin_update(s, v, jacobi);
         break;
     case kFinRz:
     case kFinRzRestart:
-        if (s.iter == 0 || s.restart) {
-            s.beta = 0.0;
-            s.restart = 0;
-        } else {
-            const double num = s.flexible ? (v[0] - s.rzold) : v[0];
-            s.beta = num / s.rz;
-        }
-        s.rz = v[0];
+        fin_rz(s, v[0]);
         break;
     case kFinTrue:
     case kFinTrueDone:
@@ -904,6 +921,29 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(const LevelProb* __re
         break;
     }
     }
+}
+
+// kFinUpdate for the last k_update (its partials in the tile slots of the
+// level-0 table upd) and then, for the problems still iterating, kFinRz for
+// the V-cycle that followed it (r.z in the slots of table rz, which must not
+// overlap the update's): one launch where the two take one each.  The
+// V-cycle ran before the update's stop test; a problem that stopped there
+// drops its r.z, as kFinRz does for a finished problem.
+__global__ __launch_bounds__(kFinishThreads) void k_finish_update_rz(const LevelProb* __restrict__ rz,
+                                                                     const LevelProb* __restrict__ upd,
+                                                                     const double* __restrict__ partial, CgState* st)
+{
+    const LevelProb& P = upd[blockIdx.x];
+    const LevelProb& Q = rz[blockIdx.x];
+    CgState& s = st[P.prob];
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    sum_tile_partials<3, 0, kFinishThreads>(partial + (int64_t)P.tile_begin * 4, P.ntiles, v);
+    sum_tile_partials<1, 3, kFinishThreads>(partial + (int64_t)Q.tile_begin * 4, Q.ntiles, v);
+    if (s.done) return;
+    block_sum<4, kFinishThreads>(v);
+    if (threadIdx.x != 0) return;
+    fin_update(s, v, 0);
+    if (!s.done) fin_rz(s, v[3]);
 }
 
 // ------------------------------------------------------------ BiCGStab kernels (FP64 vectors)
@@ -3577,12 +3617,13 @@ struct VCycle {
 // specialisations: a change of any of them drops them all (recaptured on use).
 static void drop_graphs(eq_batch* b)
 {
-    for (auto& d : b->g_cg)
-        for (auto& n : d)
-            for (hipGraphExec_t& g : n) {
-                if (g) hipGraphExecDestroy(g);
-                g = nullptr;
-            }
+    for (auto& f : b->g_cg)
+        for (auto& d : f)
+            for (auto& n : d)
+                for (hipGraphExec_t& g : n) {
+                    if (g) hipGraphExecDestroy(g);
+                    g = nullptr;
+                }
     for (hipGraphExec_t* g : {&b->g_init, &b->g_bicg, &b->g_ir2}) {
         if (*g) hipGraphExecDestroy(*g);
         *g = nullptr;
@@ -3629,12 +3670,22 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
                      k_smooth<R0, T, double, T, TENS>, grid_for(ns), s, L0.smooth(), (const double*)b->r,
                      (const T*)nullptr, x1, bcopy, omega, zero_init, 0, (double*)nullptr, (const CgState*)b->d_state);
     };
-    // one CG iteration from search direction pold to pnew
-    auto iteration = [&](double* pold, double* pnew) {
+    // The finest level's r.z has slots of its own when that level is fused:
+    // the update's scalars can then wait for the finish after the next
+    // V-cycle (merged), and a chunk of n iterations takes 2n + 1 finishes.
+    const bool merge = !jacobi && vc.fused_level(0);
+    // one CG iteration from search direction pold to pnew; merged: the
+    // previous update's finish is still to do
+    auto iteration = [&](double* pold, double* pnew, bool merged) {
         if (!jacobi) {
             vc.run0();
-            launch(b, EQ_KERNEL_FINISH, fin_bytes, k_finish, dim3(b->nprob), dim3(kFinishThreads), 0, s, vc.rz_table(),
-                   (const double*)b->partial, b->d_state, (int)kFinRz, 0);
+            if (merged)
+                launch(b, EQ_KERNEL_FINISH, fin_bytes + (double)L0.fused.size() * sizeof(double), k_finish_update_rz,
+                       dim3(b->nprob), dim3(kFinishThreads), 0, s, vc.rz_table(), (const LevelProb*)L0.d_probs,
+                       (const double*)b->partial, b->d_state);
+            else
+                launch(b, EQ_KERNEL_FINISH, fin_bytes, k_finish, dim3(b->nprob), dim3(kFinishThreads), 0, s,
+                       vc.rz_table(), (const double*)b->partial, b->d_state, (int)kFinRz, 0);
         }
         // reads z, p_old; writes p_new, q (FP64)
         launch_tiles(b, EQ_KERNEL_STENCIL64, fine_nodes * (3 * sizeof(T) + sizeof(double)), k_ap_dir<R0, T, TENS>, g0, s, A,
@@ -3646,7 +3697,7 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
                     fine_nodes * (5 * sizeof(double) + sizeof(T) + (jacobi ? sizeof(T) : b->mixed ? 3 * sizeof(T) : sizeof(T))),
                     k_update<R0, T, TENS>, g0, s, A, b->u, b->r, (T*)b->z, (const T*)pnew, (const double*)b->q, b->partial,
                     (const CgState*)b->d_state, jacobi ? 1 : 0, x1, bcopy, omega);
-        finish(kFinUpdate);
+        if (!merge) finish(kFinUpdate);
     };
     // Without extrapolation the confirmation that passes keeps the solution
     // as the next solve's start; with it, k_keep shifts the two kept ones
@@ -3672,8 +3723,9 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
     auto iterations = [&](int dir, int n, bool tail) {
         for (int k = 0; k < n; ++k) {
             const bool odd = ((dir + k) & 1) != 0;
-            iteration(odd ? b->p2 : b->p, odd ? b->p : b->p2);
+            iteration(odd ? b->p2 : b->p, odd ? b->p : b->p2, merge && k > 0);
         }
+        if (merge) finish(kFinUpdate);
         if (tail) confirm();
     };
     auto setup = [&]() {
@@ -3710,18 +3762,10 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
         return EQ_OK;
     };
     int rc;
-    if (graphs) {
-        if ((rc = capture(&b->g_init, &b->g_init_acct, setup))) return rc;
-        EQ_HIP(hipGraphLaunch(b->g_init, s));
-        replayed(b, b->g_init_acct);
-    } else {
-        setup();
-    }
-    EQ_HIP(hipGetLastError());
-
-    // Each host check follows a chunk of iterations and the confirmation; the
-    // first chunk is the last solve's iteration count, so a solve that needs
-    // as many again takes one graph launch and one read of the states.
+    // The set-up, each chunk of iterations and the confirmation after it go
+    // out together; the first chunk is the last solve's iteration count, so
+    // a solve that needs as many again takes one graph launch and one read
+    // of the states.
     constexpr int kMaxG = eq_batch::kCgGraphMax;
     int chunk = std::max(1, b->last_iters > 0 ? b->last_iters : S.check_every);
     int dir = 0;
@@ -3730,14 +3774,19 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
         for (int left = chunk; left > 0;) {
             const int n = std::min(left, kMaxG);
             const bool tail = (n == left);
+            const int first = (guard == 0 && left == chunk) ? 1 : 0;
+            auto body = [&]() {
+                if (first) setup();
+                iterations(dir, n, tail);
+            };
             if (graphs) {
-                hipGraphExec_t* g = &b->g_cg[dir][n][tail ? 1 : 0];
-                Acct* ac = &b->g_cg_acct[dir][n][tail ? 1 : 0];
-                if ((rc = capture(g, ac, [&]() { iterations(dir, n, tail); }))) return rc;
+                hipGraphExec_t* g = &b->g_cg[first][dir][n][tail ? 1 : 0];
+                Acct* ac = &b->g_cg_acct[first][dir][n][tail ? 1 : 0];
+                if ((rc = capture(g, ac, body))) return rc;
                 EQ_HIP(hipGraphLaunch(*g, s));
                 replayed(b, *ac);
             } else {
-                iterations(dir, n, tail);
+                body();
             }
             dir = (dir + n) & 1;
             left -= n;
