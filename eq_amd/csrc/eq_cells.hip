@@ -80,7 +80,9 @@ __device__ __forceinline__ int32_t to_index(double v, double npm)
 // (-o,r),(o,r),(o,-r),(-o,-r).  With the box's axis-aligned edges the four
 // cpvcross(edge, p - vertex) < 0 tests are exactly -o < lx < o and
 // -r < ly < r (products with a zero edge component are +-0, and the sign of
-// a rounded difference is the sign of the exact difference).
+// a rounded difference is the sign of the exact difference), that is
+// |lx| < o and |ly| < r: one compare each (false for NaN, and for o <= 0 or
+// r <= 0 as the two-sided tests are).
 __device__ __forceinline__ double inv_tx(double cx, double cy, double cs, double sn) { return (-sn) * cy - cs * cx; }
 __device__ __forceinline__ double inv_ty(double cx, double cy, double cs, double sn) { return sn * cx - cs * cy; }
 
@@ -91,7 +93,7 @@ __device__ __forceinline__ bool inside_box(double csX, double snX, double snY, d
 {
     const double lx = csX + snY + itx;
     const double ly = snX + csY + ity;
-    return (lx - o < 0.0) && (ly - rad < 0.0) && (ly + rad > 0.0) && (lx + o > 0.0);
+    return (fabs(lx) < o) && (fabs(ly) < rad);
 }
 
 __device__ __forceinline__ bool inside_cell(double cx, double cy, double cs, double sn, double o, double rad,
