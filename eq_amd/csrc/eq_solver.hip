@@ -2320,11 +2320,12 @@ __device__ __forceinline__ void region_strips(const T* __restrict__ src, int ra,
         if (r0 + k <= rb) f(k, r0 + k, mv[k + 1], wv[k + 1], ev[k + 1], mv[k], mv[k + 2], wv[k], ev[k + 2]);
 }
 
-// X1: sweep 1 is already in x1 (the finest level, where the Krylov update
-// forms it); otherwise it is formed here from b.  TH: tile rows.
-template <typename T, int TH, bool X1>
+// X1: sweep 1 is already in x1 (formed by the kernel that formed b);
+// otherwise it is formed here from b, read in type TB and rounded to T (the
+// finest level reads the Krylov residual itself).  TH: tile rows.
+template <typename T, int TH, bool X1, typename TB = T>
 __global__ __launch_bounds__(kBlock) void k_level_down(const FusedTile* __restrict__ tiles, int ntiles,
-                                                       const LevelProb* __restrict__ probs, const T* __restrict__ b,
+                                                       const LevelProb* __restrict__ probs, const TB* __restrict__ b,
                                                        T* __restrict__ xout, T* __restrict__ bc, T omega, T omega2,
                                                        const CgState* st, const T* __restrict__ x1)
 {
@@ -2348,7 +2349,7 @@ __global__ __launch_bounds__(kBlock) void k_level_down(const FusedTile* __restri
         bv[q] = xv[q] = (T)0;
         if (i <= ihi + 2 && j >= jlo - 2 && j <= jhi + 2 && jin && i >= 0 && i <= G.ny) {
             const int64_t g = G.origin + (int64_t)i * G.pitch + j;
-            bv[q] = b[g];
+            bv[q] = (T)b[g];
             if (X1) xv[q] = x1[g];
         }
     }
@@ -2461,11 +2462,12 @@ __global__ __launch_bounds__(kBlock) void k_level_down(const FusedTile* __restri
 }
 
 // DOT (the finest level): the last sweep's b is r (type TI) rounded to T, as
-// in k_smooth, and the tile adds r.x_out to its partial sum in slot gt
-template <typename T, typename TI, int TH, bool DOT>
+// in k_smooth, and the tile adds r.x_out to its partial sum in slot gt.  b is
+// read in type TB and rounded to T (the finest level: r itself).
+template <typename T, typename TI, int TH, bool DOT, typename TB = T>
 __global__ __launch_bounds__(kBlock) void k_level_up(const FusedTile* __restrict__ tiles, int ntiles,
                                                      const LevelProb* __restrict__ probs, const T* __restrict__ xc,
-                                                     const T* __restrict__ b, const T* __restrict__ xin,
+                                                     const TB* __restrict__ b, const T* __restrict__ xin,
                                                      T* __restrict__ xout, T omega, T omega2, const CgState* st,
                                                      const TI* __restrict__ rin, double* partial)
 {
@@ -2490,7 +2492,7 @@ __global__ __launch_bounds__(kBlock) void k_level_up(const FusedTile* __restrict
         if (jq && i <= i0 + kFH + 1 && i >= 0 && i <= G.ny) {
             const int64_t g = G.origin + (int64_t)i * G.pitch + j;
             xv[q] = xin[g];
-            bv[q] = b[g];
+            bv[q] = (T)b[g];
             if (!is_dirichlet(G, i, j)) cv[q] = prolong_val<T>(G, Gc, i, j, xc, Gc.origin, Gc.pitch);
         }
     }
@@ -3380,6 +3382,7 @@ struct VCycle {
                                 // it is already in the smoother's precision)
     T* out = nullptr;           // level-0 result
     bool pre1 = false;          // level 0's first sweep from zero is already in xa (and its b in L.b)
+    bool fuse_top = false;      // level 0 may run fused (then it forms its first sweep from `in` itself)
     const CgState* st_ = nullptr; // Krylov state whose `done` the launches test (default: the batch's)
     // an input in the smoother's precision is read directly; an FP64 one is
     // copied to L.b by the first sweep when the smoother runs in FP32
@@ -3411,12 +3414,12 @@ struct VCycle {
     // restricted residual into the next level's b) and up in another
     // (k_level_up: xb plus the correction, post-sweeps into xa; level 0: into
     // `out`, with the partial r.z in the fused tiles' slots).  Level 0 does
-    // so when its first sweep is already in xa (pre1).
+    // so when its solver allows it (fuse_top), reading `in` itself.
     bool fused_level(int l) const
     {
         const eq_solver_desc& S = b->solver;
         if (TENS || S.mg_level_fused == 0 || S.mg_pre != 2 || S.mg_post != 2 || b->levels[l].fused.empty()) return false;
-        return l > 0 || pre1;
+        return l > 0 || fuse_top;
     }
     // the level table whose tile ranges locate the partial r.z of the last run0()
     const LevelProb* rz_table() const
@@ -3458,17 +3461,16 @@ struct VCycle {
             Level& C = b->levels[1];
             const int nf = (int)L.fused.size(), gf = grid_for(nf);
             const double nodes = L.nodes_smooth, cnodes = C.nodes_active;
-            // the smoother's b: the residual itself in the smoother's precision,
-            // or the FP32 copy the Krylov update wrote with the first sweep
-            const T* b0 = kDirect ? (const T*)in : (const T*)L.b;
-            launch_tiles(b, EQ_KERNEL_TRANSFER0, nodes * 3 * sizeof(T) + cnodes * sizeof(T),
-                         k_level_down<T, kFusedTileH0, true>, gf,
-                         s, (const FusedTile*)L.d_fused, nf, (const LevelProb*)L.d_probs, b0, (T*)L.xb, (T*)C.b,
-                         wpre(0), wpre(1), st, (const T*)L.xa);
+            // b: the Krylov residual `in`, rounded to T as it is read (the
+            // first sweep from zero formed from it here)
+            launch_tiles(b, EQ_KERNEL_TRANSFER0, nodes * (sizeof(TI) + sizeof(T)) + cnodes * sizeof(T),
+                         k_level_down<T, kFusedTileH0, false, TI>, gf, s, (const FusedTile*)L.d_fused, nf,
+                         (const LevelProb*)L.d_probs, in, (T*)L.xb, (T*)C.b, wpre(0), wpre(1), st, (const T*)nullptr);
             run(1);
-            launch_tiles(b, EQ_KERNEL_SMOOTH, nodes * (3 * sizeof(T) + sizeof(TI)) + cnodes * sizeof(T),
-                         k_level_up<T, TI, kFusedTileH0, true>, gf, s, (const FusedTile*)L.d_fused, nf, (const LevelProb*)L.d_probs,
-                         (const T*)final_buffer(1), b0, (const T*)L.xb, out, wpost(0), wpost(1), st, in, b->partial);
+            launch_tiles(b, EQ_KERNEL_SMOOTH, nodes * (2 * sizeof(T) + sizeof(TI)) + cnodes * sizeof(T),
+                         k_level_up<T, TI, kFusedTileH0, true, TI>, gf, s, (const FusedTile*)L.d_fused, nf,
+                         (const LevelProb*)L.d_probs, (const T*)final_buffer(1), in, (const T*)L.xb, out, wpost(0),
+                         wpost(1), st, in, b->partial);
         } else if (ns > 0 && fused_level(l)) {
             Level& C = b->levels[l + 1];
             const int nf = (int)L.fused.size(), gf = grid_for(nf);
@@ -3654,18 +3656,20 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
         launch(b, EQ_KERNEL_FINISH, fin_bytes, k_finish, dim3(b->nprob), dim3(kFinishThreads), 0, s,
                (const LevelProb*)L0.d_probs, (const double*)b->partial, b->d_state, op, jacobi ? 1 : 0);
     };
-    // The V-cycle's first level-0 sweep (x = omega D^-1 r from x = 0) is
+    // A fused finest level reads the residual and forms the V-cycle's first
+    // sweep (x = omega D^-1 r from x = 0) itself.  Otherwise that sweep is
     // pointwise in the new residual: k_update forms it with r, and after
-    // k_init or the true residual a sweep launch does.  The V-cycle then
+    // k_init or the true residual a sweep launch does; the V-cycle then
     // starts from its second sweep.
-    T* x1 = jacobi ? nullptr : (T*)L0.xa;
-    T* bcopy = (!jacobi && b->mixed) ? (T*)L0.b : nullptr;
+    vc.fuse_top = !jacobi;
+    vc.pre1 = !jacobi && !vc.fused_level(0);
+    T* x1 = vc.pre1 ? (T*)L0.xa : nullptr;
+    T* bcopy = (vc.pre1 && b->mixed) ? (T*)L0.b : nullptr;
     const T omega = (T)S.mg_omega;
-    vc.pre1 = !jacobi;
     // zero_init 2: for the problems a confirmation restarted only
     auto first_sweep = [&](int zero_init) {
         const int ns = (int)L0.tiles_smooth.size();
-        if (jacobi || ns == 0) return;
+        if (!vc.pre1 || ns == 0) return;
         launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, L0.nodes_smooth * (sizeof(double) + sizeof(T) * (bcopy ? 2 : 1)),
                      k_smooth<R0, T, double, T, TENS>, grid_for(ns), s, L0.smooth(), (const double*)b->r,
                      (const T*)nullptr, x1, bcopy, omega, zero_init, 0, (double*)nullptr, (const CgState*)b->d_state);
@@ -3692,9 +3696,10 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
                     (const T*)b->z, (const T*)pold, (T*)pnew, b->q, b->partial, (const CgState*)b->d_state);
         finish(kFinAp);
         // reads x, r, q (FP64) and p (+ z when flexible); writes x, r (+ z for
-        // Jacobi; + the first sweep, and the FP32 residual when mixed, otherwise)
+        // Jacobi; + the first sweep, and the FP32 residual when mixed, when the
+        // finest level is not fused)
         launch_tiles(b, EQ_KERNEL_CG_UPDATE,
-                    fine_nodes * (5 * sizeof(double) + sizeof(T) + (jacobi ? sizeof(T) : b->mixed ? 3 * sizeof(T) : sizeof(T))),
+                    fine_nodes * (5 * sizeof(double) + 2 * sizeof(T) + (x1 ? sizeof(T) : 0) + (bcopy ? sizeof(T) : 0)),
                     k_update<R0, T, TENS>, g0, s, A, b->u, b->r, (T*)b->z, (const T*)pnew, (const double*)b->q, b->partial,
                     (const CgState*)b->d_state, jacobi ? 1 : 0, x1, bcopy, omega);
         if (!merge) finish(kFinUpdate);
@@ -3998,13 +4003,16 @@ static int solve_ir(eq_batch* b, eq_step_stats* stats)
         launch(b, EQ_KERNEL_FINISH, fin_bytes, k_finish, dim3(b->nprob), dim3(kFinishThreads), 0, s,
                (const LevelProb*)L0.d_probs, (const double*)b->partial, st, op, 0);
     };
-    T* x1 = (T*)L0.xa;
     const T omega = (T)S.mg_omega;
     VCycle<T, TENS, T> vc(b);
     vc.in = b->ir_r;
     vc.out = (T*)b->z;
     vc.st_ = si;
-    vc.pre1 = true;
+    // the fused finest level forms its first sweep itself; otherwise the
+    // inner update (and the launch after k_ir_begin) forms it
+    vc.fuse_top = true;
+    vc.pre1 = !vc.fused_level(0);
+    T* x1 = vc.pre1 ? (T*)L0.xa : nullptr;
     // one inner iteration from direction pold to pnew (FP32 throughout)
     auto iteration = [&](T* pold, T* pnew) {
         vc.run0();
@@ -4013,7 +4021,7 @@ static int solve_ir(eq_batch* b, eq_step_stats* stats)
         launch_tiles(b, EQ_KERNEL_STENCIL64, n0 * 4.0 * sizeof(T), k_ap_dir<R0, T, TENS, T>, g0, s, A, (const T*)b->z,
                      (const T*)pold, pnew, (T*)b->q, b->partial, (const CgState*)si);
         finish(si, kFinAp);
-        launch_tiles(b, EQ_KERNEL_CG_UPDATE, n0 * 7.0 * sizeof(T), k_update<R0, T, TENS, T>, g0, s, A, b->ir_d, b->ir_r,
+        launch_tiles(b, EQ_KERNEL_CG_UPDATE, n0 * (x1 ? 7.0 : 6.0) * sizeof(T), k_update<R0, T, TENS, T>, g0, s, A, b->ir_d, b->ir_r,
                      (T*)b->z, (const T*)pnew, (const T*)b->q, b->partial, (const CgState*)si, 0, x1, (T*)nullptr, omega);
         finish(si, kFinUpdate);
     };
@@ -4078,7 +4086,7 @@ static int solve_ir(eq_batch* b, eq_step_stats* stats)
                      (const double*)b->r, b->ir_r, b->ir_d, b->ir_scale, (const CgState*)so, si, b->partial,
                      S.ir_inner_rtol);
         finish(si, kFinIrInit);
-        if (ns > 0)
+        if (ns > 0 && x1)
             launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, L0.nodes_smooth * 2.0 * sizeof(T), k_smooth<R0, T, T, T, TENS>,
                          grid_for(ns), s, L0.smooth(), (const T*)b->ir_r, (const T*)nullptr, x1, (T*)nullptr, omega, 1, 0,
                          (double*)nullptr, (const CgState*)si);
