@@ -1170,6 +1170,22 @@ __device__ __forceinline__ T restrict_val(const LevelProb& Pc, const LevelProb& 
     return restrict_gen<T, FETCH>(Pc, Pf, I, J, fpitch, [&](int, int, int64_t k) { return rf[forg + k]; });
 }
 
+// prolong_val's nested case with the four corners of the coarse cell loaded
+// unconditionally (indices clamped into the coarse grid) and the one needed
+// selected: the same value, with no load under a branch
+template <typename T>
+__device__ __forceinline__ T prolong_nested(int i, int j, int nxc, int nyc, const T* __restrict__ xc, int64_t corg,
+                                            int cpitch)
+{
+    const int I = i >> 1, J = j >> 1, a = i & 1, bb = j & 1;
+    const int I1 = min(I + 1, nyc), J1 = min(J + 1, nxc);
+    const T* r0 = xc + corg + (int64_t)I * cpitch;
+    const T* r1 = xc + corg + (int64_t)I1 * cpitch;
+    const T x00 = r0[J], x01 = r0[J1], x10 = r1[J], x11 = r1[J1];
+    const T xo = a ? (bb ? x11 : x10) : (bb ? x01 : x00);
+    return (a | bb) ? (T)0.5 * (x00 + xo) : x00;
+}
+
 // (P x_c) at fine node (i,j): the four corners of the coarse cell holding it;
 // coarse node (I,J) lives at xc[corg + I*cpitch + J]
 template <typename T>
@@ -2341,17 +2357,31 @@ __global__ __launch_bounds__(kBlock) void k_level_down(const FusedTile* __restri
     const int lane = (int)(threadIdx.x & 63), w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int j = rj + lane;
     const bool jin = j >= 0 && j <= G.nx;
-    // b (and x1) over the support and two nodes around it, every load first
-    T bv[kFRows], xv[kFRows];
+    // b (and x1) over the support and two nodes around it, every load first:
+    // each lane loads from its node clamped into that region (a load under a
+    // branch would wait for its value at the branch's end, one row at a time)
+    // and keeps zero where it is outside; b is rounded to T where it is staged
+    TB bv[kFRows];
+    T xv[kFRows];
+    uint32_t okm = 0;
+    {
+        const int ia = max(ilo - 2, 0), ib = min(ihi + 2, G.ny);
+        const int jc = min(max(j, max(jlo - 2, 0)), min(jhi + 2, G.nx));
+        const bool jok = j >= jlo - 2 && j <= jhi + 2 && jin;
 #pragma unroll
-    for (int q = 0; q < kFRows; ++q) {
-        const int i = ilo - 2 + w + q * kWaves;
-        bv[q] = xv[q] = (T)0;
-        if (i <= ihi + 2 && j >= jlo - 2 && j <= jhi + 2 && jin && i >= 0 && i <= G.ny) {
-            const int64_t g = G.origin + (int64_t)i * G.pitch + j;
-            bv[q] = (T)b[g];
-            if (X1) xv[q] = x1[g];
+        for (int q = 0; q < kFRows; ++q) {
+            const int i = ilo - 2 + w + q * kWaves;
+            const int64_t g = G.origin + (int64_t)min(max(i, ia), ib) * G.pitch + jc;
+            bv[q] = b[g];
+            xv[q] = X1 ? x1[g] : (T)0;
+            if (jok && i <= ihi + 2 && i >= 0 && i <= G.ny) okm |= 1u << q;
         }
+#pragma unroll
+        for (int q = 0; q < kFRows; ++q)
+            if (!((okm >> q) & 1u)) {
+                bv[q] = (TB)0;
+                xv[q] = (T)0;
+            }
     }
     const LevelProb P = probs[f.pi];
     if (st[f.prob].done) return;
@@ -2366,8 +2396,9 @@ __global__ __launch_bounds__(kBlock) void k_level_down(const FusedTile* __restri
             const int i = ilo - 2 + w + q * kWaves;
             if (i > ihi + 2 || j < jlo - 2 || j > jhi + 2) continue;
             const int a = (i - ri) * kFRW + lane;
-            sb[a] = bv[q];
-            s1[a] = X1 ? xv[q] : wdi * bv[q];
+            const T bq = (T)bv[q];
+            sb[a] = bq;
+            s1[a] = X1 ? xv[q] : wdi * bq;
         }
         __syncthreads();
         const bool jt = j >= j0 && j < j0 + kFW; // a column of the tile
@@ -2400,8 +2431,9 @@ __global__ __launch_bounds__(kBlock) void k_level_down(const FusedTile* __restri
             if (i > ihi + 2 || j < jlo - 2 || j > jhi + 2) continue;
             const int a = (i - ri) * kFRW + lane;
             const bool in = jin && i >= 0 && i <= G.ny;
-            sb[a] = bv[q];
-            s1[a] = (in && !is_dirichlet(G, i, j)) ? (X1 ? xv[q] : fused_wd(P, rt, i, j, omega) * bv[q]) : (T)0;
+            const T bq = (T)bv[q];
+            sb[a] = bq;
+            s1[a] = (in && !is_dirichlet(G, i, j)) ? (X1 ? xv[q] : fused_wd(P, rt, i, j, omega) * bq) : (T)0;
         }
         __syncthreads();
         // sweep 2 one node around the support; the tile's own nodes go to xout
@@ -2484,16 +2516,43 @@ __global__ __launch_bounds__(kBlock) void k_level_up(const FusedTile* __restrict
     const bool jq = j >= j0 - 2 && j <= j0 + kFW + 1 && j >= 0 && j <= G.nx;
     // x + P x_c two nodes around the tile (x alone on Dirichlet nodes), and b
     constexpr int kUpRows = (kFH + 4 + kWaves - 1) / kWaves;
-    T xv[kUpRows], bv[kUpRows], cv[kUpRows];
+    // Every lane loads from its node clamped into that region, with no branch
+    // around the loads (a load under a branch waits for its value at the
+    // branch's end, one row at a time), and keeps zero outside it; b is
+    // rounded to T where it is staged.
+    T xv[kUpRows], cv[kUpRows];
+    TB bv[kUpRows];
+    {
+        const int ia = max(i0 - 2, 0), ib = min(i0 + kFH + 1, G.ny);
+        const int jc = min(max(j, max(j0 - 2, 0)), min(j0 + kFW + 1, G.nx));
+        uint32_t okm = 0, dm = 0;
+        // the rows' loads in one straight run (the transfer's kind is uniform)
+        auto rows = [&](auto&& pval) {
 #pragma unroll
-    for (int q = 0; q < kUpRows; ++q) {
-        const int i = i0 - 2 + w + q * kWaves;
-        xv[q] = bv[q] = cv[q] = (T)0;
-        if (jq && i <= i0 + kFH + 1 && i >= 0 && i <= G.ny) {
-            const int64_t g = G.origin + (int64_t)i * G.pitch + j;
-            xv[q] = xin[g];
-            bv[q] = (T)b[g];
-            if (!is_dirichlet(G, i, j)) cv[q] = prolong_val<T>(G, Gc, i, j, xc, Gc.origin, Gc.pitch);
+            for (int q = 0; q < kUpRows; ++q) {
+                const int i = i0 - 2 + w + q * kWaves;
+                const int ic = min(max(i, ia), ib);
+                const int64_t g = G.origin + (int64_t)ic * G.pitch + jc;
+                xv[q] = xin[g];
+                bv[q] = b[g];
+                cv[q] = pval(ic, jc);
+                if (jq && i <= i0 + kFH + 1 && i >= 0 && i <= G.ny) {
+                    okm |= 1u << q;
+                    if (is_dirichlet(G, i, j)) dm |= 1u << q;
+                }
+            }
+        };
+        if (G.nx == 2 * Gc.nx && G.ny == 2 * Gc.ny)
+            rows([&](int ic, int jc2) { return prolong_nested<T>(ic, jc2, Gc.nx, Gc.ny, xc, Gc.origin, Gc.pitch); });
+        else
+            rows([&](int ic, int jc2) { return prolong_val<T>(G, Gc, ic, jc2, xc, Gc.origin, Gc.pitch); });
+#pragma unroll
+        for (int q = 0; q < kUpRows; ++q) {
+            if (!((okm >> q) & 1u)) {
+                xv[q] = (T)0;
+                bv[q] = (TB)0;
+            }
+            if (!((okm >> q) & 1u) || ((dm >> q) & 1u)) cv[q] = (T)0;
         }
     }
     const bool inner = (f.inner & 2) != 0;
@@ -2509,7 +2568,7 @@ __global__ __launch_bounds__(kBlock) void k_level_up(const FusedTile* __restrict
         if (i > i0 + kFH + 1 || j < j0 - 2 || j > j0 + kFW + 1) continue;
         const int a = (i - ri) * kFRW + lane;
         sy[a] = xv[q] + cv[q];
-        sb[a] = bv[q];
+        sb[a] = (T)bv[q];
     }
     __syncthreads();
     double dot[1] = {0.0};
