@@ -774,10 +774,28 @@ __device__ __forceinline__ void sum_tile_partials(const double* base, int ntiles
 #pragma unroll
         for (int k = 0; k < NV; ++k) v[OFF + k] += (p0[k] + p1[k]) + (p2[k] + p3[k]);
     }
-    for (; q < ntiles; q += NT) {
+    // at most three more per thread: loaded together (indices clamped to the
+    // last tile), added in the same order as one at a time
+    if (q < ntiles) {
+        const int last = ntiles - 1;
         const double* p0 = base + (int64_t)q * 4;
+        const double* p1 = base + (int64_t)min(q + NT, last) * 4;
+        const double* p2 = base + (int64_t)min(q + 2 * NT, last) * 4;
+        double a0[NV], a1[NV], a2[NV];
 #pragma unroll
-        for (int k = 0; k < NV; ++k) v[OFF + k] += p0[k];
+        for (int k = 0; k < NV; ++k) {
+            a0[k] = p0[k];
+            a1[k] = p1[k];
+            a2[k] = p2[k];
+        }
+#pragma unroll
+        for (int k = 0; k < NV; ++k) v[OFF + k] += a0[k];
+        if (q + NT < ntiles)
+#pragma unroll
+            for (int k = 0; k < NV; ++k) v[OFF + k] += a1[k];
+        if (q + 2 * NT < ntiles)
+#pragma unroll
+            for (int k = 0; k < NV; ++k) v[OFF + k] += a2[k];
     }
 }
 
@@ -1394,13 +1412,20 @@ __global__ __launch_bounds__(kBlock) void k_prolong(LevelView Lf, const LevelPro
     if (st[Pf.prob].done) return;
     const LevelProb Pc = coarse[Pf.cidx];
     if (R > 1 && c.full) {
-        // interior tile: the coarse values and the fine x first, then the R rows
+        // interior tile: the coarse values and the fine x first, then the R
+        // rows (the transfer's kind picked once, so no load sits under a branch)
         T add[R], xv[R];
+        auto rows = [&](auto&& pval) {
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            add[r] = prolong_val<T>(Pf, Pc, c.i + r, c.j, xc, Pc.origin, Pc.pitch);
-            xv[r] = xf[c.idx + (int64_t)r * Pf.pitch];
-        }
+            for (int r = 0; r < R; ++r) {
+                add[r] = pval(c.i + r);
+                xv[r] = xf[c.idx + (int64_t)r * Pf.pitch];
+            }
+        };
+        if (Pf.nx == 2 * Pc.nx && Pf.ny == 2 * Pc.ny)
+            rows([&](int i) { return prolong_nested<T>(i, c.j, Pc.nx, Pc.ny, xc, Pc.origin, Pc.pitch); });
+        else
+            rows([&](int i) { return prolong_val<T>(Pf, Pc, i, c.j, xc, Pc.origin, Pc.pitch); });
 #pragma unroll
         for (int r = 0; r < R; ++r) xf[c.idx + (int64_t)r * Pf.pitch] = xv[r] + add[r];
         return;
