@@ -293,9 +293,11 @@ __global__ __launch_bounds__(kScanBlock) void k_pair_offsets(CellGeo* __restrict
 }
 
 // pair p of cell c: key = global tile id, value = p; the cell's tiles row by row
+// (pairs at or past cap are not written: the call's pair buffers hold cap)
 __global__ __launch_bounds__(256) void k_pairs(const CellGeo* __restrict__ geo, int64_t nc,
                                                const TrapGrid* __restrict__ traps, uint32_t* __restrict__ keys,
-                                               uint32_t* __restrict__ vals, uint32_t* __restrict__ pair_cell)
+                                               uint32_t* __restrict__ vals, uint32_t* __restrict__ pair_cell,
+                                               int64_t cap)
 {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
@@ -304,18 +306,38 @@ __global__ __launch_bounds__(256) void k_pairs(const CellGeo* __restrict__ geo, 
     for (int k = 0; k < g.npairs; ++k) {
         const int ty = g.ty1 + k / g.ntx, tx = g.tx1 + k % g.ntx;
         const int64_t p = g.poff + k;
+        if (p >= cap) break;
         keys[p] = (uint32_t)(G.tile_base + ty * G.tiles_x + tx);
         vals[p] = (uint32_t)p;
         pair_cell[p] = (uint32_t)c;
     }
 }
 
+// A call sized before its pair count is known sorts cap slots: the slots past
+// the count take a key above every tile id (the sort is over the low bits of
+// the keys, and every tile id is below 2^bits - 1), so they sort last; the
+// count is also written to host memory (mapped) for the next call's sizing.
+__global__ __launch_bounds__(256) void k_pad_pairs(uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                                   const int64_t* __restrict__ total, int64_t cap,
+                                                   int64_t* __restrict__ host_total)
+{
+    const int64_t n = *total;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && host_total) *host_total = n;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x + n; k < cap; k += (int64_t)gridDim.x * blockDim.x) {
+        keys[k] = 0xffffffffu;
+        vals[k] = (uint32_t)k;
+    }
+}
+
 // [start, end) of every tile's run in the sorted pairs (tiles without pairs
-// keep the zeros they were cleared to)
+// keep the zeros they were cleared to); total (optional): the pairs are the
+// first min(*total, n) of the n sorted slots
 __global__ __launch_bounds__(256) void k_tile_bounds(const uint32_t* __restrict__ keys, int64_t n, uint32_t tile0,
-                                                     uint32_t* __restrict__ start, uint32_t* __restrict__ end)
+                                                     uint32_t* __restrict__ start, uint32_t* __restrict__ end,
+                                                     const int64_t* __restrict__ total)
 {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (total) n = min(n, *total);
     if (k >= n) return;
     const uint32_t key = keys[k];
     if (k == 0 || keys[k - 1] != key) start[key - tile0] = (uint32_t)k;
@@ -345,12 +367,14 @@ __global__ __launch_bounds__(kCThreads) void k_tile_couple(
     const uint32_t* __restrict__ end, const uint32_t* __restrict__ sorted_pairs, const uint32_t* __restrict__ pair_cell,
     const CellGeo* __restrict__ geo, double* const* __restrict__ fields, int ns, const double* __restrict__ q,
     double* __restrict__ partial, double npm, unsigned flags, double* const* __restrict__ tens64,
-    float* const* __restrict__ tens32, int64_t tstride)
+    float* const* __restrict__ tens32, int64_t tstride, const int64_t* __restrict__ total, int64_t cap)
 {
     __shared__ double red[kChunk][kCThreads / 64][P]; // per cell of the chunk: each wave's sums
     __shared__ CellGeo sg[kChunk];      // cx, cy replaced by the inverse translation
     __shared__ uint32_t sp[kChunk];
     __shared__ double sq[kChunk][P]; // q_c / |S_c|
+    // more pairs than the call's buffers hold: k_couple_affine does the call
+    if (total && *total > cap) return;
     const uint32_t tl = blockIdx.x;
     const uint32_t e0 = start[tl], e1 = end[tl];
     const bool want_tens = TENS && (flags & EQ_COUPLE_TENSOR) != 0;
@@ -509,12 +533,14 @@ __global__ __launch_bounds__(kCThreads) void k_tile_couple(
 }
 
 // sampled[c,s] = (sum of the cell's pair partials in tile order) / |S_c|
+// (more pairs than the call's buffers hold: k_couple_affine does the call)
 __global__ __launch_bounds__(256) void k_cell_sample(const CellGeo* __restrict__ geo, int64_t nc, int ns,
                                                      const double* __restrict__ partial, double* __restrict__ sampled,
-                                                     int32_t* __restrict__ counts)
+                                                     int32_t* __restrict__ counts, const int64_t* __restrict__ total,
+                                                     int64_t cap)
 {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nc) return;
+    if (c >= nc || (total && *total > cap)) return;
     const CellGeo& g = geo[c];
     const int cnt = g.cnt > 0 ? g.cnt : 1;
     if (counts) counts[c] = cnt;
@@ -536,6 +562,10 @@ static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 // cell's), forms e = sum/|S_c| and q, and the covered nodes take q/|S_c|
 // (src/abm/eQabm.cpp:326-359, 384-405).  A barrier separates each cell's
 // reads from its writes and its writes from the next cell's reads.
+// With qb null the amount is qa itself (qa null: nothing deposited): the
+// tile path's result, by the same node tests and the same per-node order of
+// additions.  total (optional): run only when *total > cap, as the tile
+// path's stand-in for a call whose pairs outgrew its buffers.
 constexpr int kAThreads = 1024;
 __global__ __launch_bounds__(kAThreads) void k_couple_affine(const TrapGrid* __restrict__ traps, int t0,
                                                              const int64_t* __restrict__ off,
@@ -543,10 +573,11 @@ __global__ __launch_bounds__(kAThreads) void k_couple_affine(const TrapGrid* __r
                                                              double* const* __restrict__ fields, int ns,
                                                              const double* __restrict__ qa, const double* __restrict__ qb,
                                                              double* __restrict__ sampled, int32_t* __restrict__ counts,
-                                                             double npm)
+                                                             double npm, const int64_t* __restrict__ total, int64_t cap)
 {
     __shared__ double red[kAThreads / 64][kSpeciesPass];
     __shared__ double dep[kSpeciesPass];
+    if (total && *total <= cap) return;
     const int t = blockIdx.x;
     const TrapGrid G = traps[t0 + t];
     const int64_t c0 = off[t], c1 = off[t + 1];
@@ -584,10 +615,13 @@ __global__ __launch_bounds__(kAThreads) void k_couple_affine(const TrapGrid* __r
                 for (int q4 = 0; q4 < kAThreads / 64; ++q4) tot += red[q4][s];
                 const double e = tot / (double)cnt;
                 if (sampled) sampled[c * ns + s0 + s] = e;
-                const double amount = qa[c * ns + s0 + s] + qb[c * ns + s0 + s] * e;
-                dep[s] = amount / (double)cnt;
+                if (qa) {
+                    const double amount = qb ? qa[c * ns + s0 + s] + qb[c * ns + s0 + s] * e : qa[c * ns + s0 + s];
+                    dep[s] = amount / (double)cnt;
+                }
             }
             __syncthreads();
+            if (!qa) continue; // nothing deposited (no barrier skipped: qa is uniform)
             for (int k = threadIdx.x; k < nwin; k += kAThreads) {
                 const int i = g.i1 + k / wj, j = g.j1 + k % wj;
                 const bool in = g.cnt == 0 ||
@@ -678,6 +712,18 @@ static int ensure_traps(eq_batch* b)
     return EQ_OK;
 }
 
+// a call's cell offsets (relative to its first cell) to the device, through
+// the pinned staging once the previous upload from it has run
+static int upload_cell_off(eq_batch* b, const int64_t* cell_off, int ntr)
+{
+    if (b->cell_off_ev) EQ_HIP(hipEventSynchronize(b->cell_off_ev));
+    else EQ_HIP(hipEventCreateWithFlags(&b->cell_off_ev, hipEventDisableTiming));
+    for (int k = 0; k <= ntr; ++k) b->h_cell_off[k] = cell_off[k] - cell_off[0];
+    EQ_HIP(hipMemcpyAsync(b->d_cell_off, b->h_cell_off, sizeof(int64_t) * (ntr + 1), hipMemcpyHostToDevice, b->stream));
+    EQ_HIP(hipEventRecord(b->cell_off_ev, b->stream));
+    return EQ_OK;
+}
+
 // cells of traps t0 .. t0+ntr-1, concatenated; cell_off[k] (host) = first cell
 // of trap t0+k, cell_off[ntr] = total
 static int couple_impl(eq_batch* b, int32_t t0, int32_t ntr, const int64_t* cell_off, const eq_cell* d_cells,
@@ -700,8 +746,7 @@ static int couple_impl(eq_batch* b, int32_t t0, int32_t ntr, const int64_t* cell
     const uint32_t tile0 = (uint32_t)b->cell_tile_base[t0];
     const uint32_t ntiles = (uint32_t)(b->cell_tile_base[t0 + ntr] - b->cell_tile_base[t0]);
     // cell offsets relative to the first cell of the call
-    for (int k = 0; k <= ntr; ++k) b->h_cell_off[k] = cell_off[k] - cell_off[0];
-    EQ_HIP(hipMemcpyAsync(b->d_cell_off, b->h_cell_off, sizeof(int64_t) * (ntr + 1), hipMemcpyHostToDevice, s));
+    if ((rc = upload_cell_off(b, cell_off, ntr))) return rc;
     // scratch: geo | total | tile start | tile end | (pairs: keys, vals, keys', vals', pair_cell, partial, sort temp)
     const size_t geo_bytes = align256(sizeof(CellGeo) * nc);
     const size_t tb_bytes = align256(sizeof(uint32_t) * ntiles);
@@ -736,11 +781,33 @@ static int couple_impl(eq_batch* b, int32_t t0, int32_t ntr, const int64_t* cell
         launch(b, kid, 2.0 * gb + 8.0 * nblk, k_pair_offsets, dim3((unsigned)nblk), dim3(kScanBlock), 0, s, geo,
                (int64_t)nc, (const int64_t*)d_blk);
     };
+    // the tensor is written only where a trap of the call has one set from its cells
+    bool tens = false;
+    if (flags & EQ_COUPLE_TENSOR)
+        for (int t : b->cell_tensor_traps) tens = tens || (t >= t0 && t < t0 + ntr);
+    // The pair buffers are sized before the call's pair count is known (no
+    // host wait inside the call) from the counts of earlier calls: the last
+    // one read back once its call has run, the buffers kept at 1.5x the
+    // largest.  A call with more pairs than that is done by k_couple_affine
+    // in list order (same deposits), and the next call gets room for it.  A
+    // first call, a tensor call and the fixed scratch read the count first.
+    if (b->pair_pending && hipEventQuery(b->pair_ev) == hipSuccess) {
+        b->pair_pending = false;
+        const int64_t last = *b->h_pair_total;
+        if (last > b->pair_cap - b->pair_cap / 4) b->pair_cap = last + last / 2 + 4096;
+    }
+    const bool sized = !fixed && !tens && b->pair_cap > 0;
     geometry();
     int64_t total = 0;
-    EQ_HIP(hipMemcpyAsync(&total, d_total, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-    EQ_HIP(hipStreamSynchronize(s));
+    if (sized) {
+        total = b->pair_cap;
+    } else {
+        EQ_HIP(hipMemcpyAsync(&total, d_total, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        EQ_HIP(hipStreamSynchronize(s));
+    }
     if (total >= ((int64_t)1 << 32)) return set_error(EQ_ERR_INVALID, "too many cell/tile pairs");
+    if (!fixed && !sized) b->pair_cap = std::max(b->pair_cap, total + total / 2 + 4096);
+    // slots the pair buffers hold (sized: the kept capacity; else the count)
     const size_t pb = align256(sizeof(uint32_t) * (size_t)total);
     size_t sort_bytes = 0;
     EQ_HIP(rocprim::radix_sort_pairs(nullptr, sort_bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
@@ -769,7 +836,19 @@ static int couple_impl(eq_batch* b, int32_t t0, int32_t ntr, const int64_t* cell
     double* partial = (double*)(pbase + 5 * pb);
     void* sort_tmp = pbase + 5 * pb + align256(sizeof(double) * (size_t)total * ns);
     launch(b, kid, gb + 12.0 * total, k_pairs, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s, (const CellGeo*)geo,
-           (int64_t)nc, traps, keys, vals, pair_cell);
+           (int64_t)nc, traps, keys, vals, pair_cell, (int64_t)total);
+    // sized: the unused slots sort last; the count goes to the host for later calls
+    const int64_t* gate = sized ? (const int64_t*)d_total : nullptr;
+    if (sized) {
+        if (!b->h_pair_total &&
+            hipHostMalloc((void**)&b->h_pair_total, sizeof(int64_t), hipHostMallocMapped) != hipSuccess)
+            return set_error(EQ_ERR_NOMEM, "pair count staging");
+        if (!b->pair_ev) EQ_HIP(hipEventCreateWithFlags(&b->pair_ev, hipEventDisableTiming));
+        int64_t* host_total = nullptr;
+        EQ_HIP(hipHostGetDevicePointer((void**)&host_total, b->h_pair_total, 0));
+        launch(b, kid, 8.0 * total, k_pad_pairs, dim3((unsigned)std::min<int64_t>((total + 255) / 256, 1024)), dim3(256),
+               0, s, keys, vals, (const int64_t*)d_total, (int64_t)total, host_total);
+    }
     // key bits: global tile ids up to the last tile of the batch
     unsigned bits = 1;
     while (bits < 32 && ((uint64_t)b->cell_tile_base[b->ntraps] >> bits) != 0) ++bits;
@@ -780,7 +859,7 @@ static int couple_impl(eq_batch* b, int32_t t0, int32_t ntr, const int64_t* cell
     launch(b, kid, (double)(2 * tb_bytes), k_zero_u32, dim3((unsigned)((2 * tb_bytes / 4 + 255) / 256)), dim3(256), 0,
            s, t_start, (int64_t)(2 * tb_bytes / 4));
     launch(b, kid, 4.0 * total + 8.0 * ntiles, k_tile_bounds, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
-           (const uint32_t*)keys2, (int64_t)total, tile0, t_start, t_end);
+           (const uint32_t*)keys2, (int64_t)total, tile0, t_start, t_end, gate);
     // algorithmic bytes: every node of the call's traps read (and written when
     // depositing) once per species, the pairs, the cell records and amounts,
     // and one partial per pair and species
@@ -791,10 +870,6 @@ static int couple_impl(eq_batch* b, int32_t t0, int32_t ntr, const int64_t* cell
     }
     const double cb = nodes * ns * sizeof(double) * ((flags & EQ_COUPLE_DEPOSIT) ? 2 : 1) + 8.0 * total + gb +
                       8.0 * ns * nc + 8.0 * ns * total + 8.0 * ntiles;
-    // the tensor is written only where a trap of the call has one set from its cells
-    bool tens = false;
-    if (flags & EQ_COUPLE_TENSOR)
-        for (int t : b->cell_tensor_traps) tens = tens || (t >= t0 && t < t0 + ntr);
     const unsigned kflags = tens ? flags : (flags & ~EQ_COUPLE_TENSOR);
     const double tb = tens ? nodes * ns * 3.0 * (sizeof(double) + (b->mixed ? sizeof(float) : 0)) : 0.0;
     auto kern = ns <= 2 ? (tens ? k_tile_couple<2, true> : k_tile_couple<2, false>)
@@ -802,11 +877,20 @@ static int couple_impl(eq_batch* b, int32_t t0, int32_t ntr, const int64_t* cell
     launch(b, EQ_KERNEL_COUPLE, cb + tb, kern, dim3(ntiles), dim3(kCThreads), 0, s, traps, t0, ntr, tile0,
            (const uint32_t*)t_start, (const uint32_t*)t_end, (const uint32_t*)vals2, (const uint32_t*)pair_cell,
            (const CellGeo*)geo, (double* const*)b->d_field_ptrs, ns, d_q, partial, npm, kflags,
-           (double* const*)b->d_tens64, (float* const*)b->d_tens32, (int64_t)b->levels[0].pool);
+           (double* const*)b->d_tens64, (float* const*)b->d_tens32, (int64_t)b->levels[0].pool, gate, (int64_t)total);
     if (d_sampled || d_counts)
         launch(b, kid, gb + 8.0 * ns * total + 12.0 * ns * nc, k_cell_sample, dim3((unsigned)((nc + 255) / 256)),
                dim3(256), 0, s, (const CellGeo*)geo, (int64_t)nc, ns, (const double*)partial,
-               (flags & EQ_COUPLE_SAMPLE) ? d_sampled : nullptr, d_counts);
+               (flags & EQ_COUPLE_SAMPLE) ? d_sampled : nullptr, d_counts, gate, (int64_t)total);
+    if (sized) {
+        // the call's stand-in when its pairs outgrew the buffers (exits at once otherwise)
+        launch(b, EQ_KERNEL_COUPLE, 0.0, k_couple_affine, dim3((unsigned)ntr), dim3(kAThreads), 0, s, traps, (int)t0,
+               (const int64_t*)b->d_cell_off, (const CellGeo*)geo, (double* const*)b->d_field_ptrs, ns,
+               (flags & EQ_COUPLE_DEPOSIT) ? d_q : (const double*)nullptr, (const double*)nullptr,
+               (flags & EQ_COUPLE_SAMPLE) ? d_sampled : nullptr, d_counts, npm, (const int64_t*)d_total, (int64_t)total);
+        EQ_HIP(hipEventRecord(b->pair_ev, s));
+        b->pair_pending = true;
+    }
     EQ_HIP(hipGetLastError());
     if (tens) return refresh_cell_tensors(b);
     return EQ_OK;
@@ -888,8 +972,7 @@ int eq_batch_couple_affine_dev(eq_batch* b, int32_t trap0, int32_t ntraps, const
     int rc = ensure_traps(b);
     if (rc) return rc;
     hipStream_t s = b->stream;
-    for (int k = 0; k <= ntraps; ++k) b->h_cell_off[k] = cell_offsets[k] - cell_offsets[0];
-    EQ_HIP(hipMemcpyAsync(b->d_cell_off, b->h_cell_off, sizeof(int64_t) * (ntraps + 1), hipMemcpyHostToDevice, s));
+    if ((rc = upload_cell_off(b, cell_offsets, ntraps))) return rc;
     const size_t geo_bytes = align256(sizeof(CellGeo) * nc);
     if ((rc = ensure_scratch(b, geo_bytes))) return rc;
     CellGeo* geo = (CellGeo*)b->cell_scratch;
@@ -901,7 +984,8 @@ int eq_batch_couple_affine_dev(eq_batch* b, int32_t trap0, int32_t ntraps, const
     // every covered node read and written once per species, the cells' records and amounts
     launch(b, EQ_KERNEL_COUPLE, (double)sizeof(CellGeo) * nc + 24.0 * ns * nc, k_couple_affine, dim3((unsigned)ntraps),
            dim3(kAThreads), 0, s, traps, (int)trap0, (const int64_t*)b->d_cell_off, (const CellGeo*)geo,
-           (double* const*)b->d_field_ptrs, ns, d_a, d_b, d_sampled, d_counts, npm);
+           (double* const*)b->d_field_ptrs, ns, d_a, d_b, d_sampled, d_counts, npm, (const int64_t*)nullptr,
+           (int64_t)0);
     EQ_HIP(hipGetLastError());
     return EQ_OK;
 }

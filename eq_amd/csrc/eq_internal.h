@@ -219,6 +219,15 @@ struct eq_batch {
     void* d_cell_traps = nullptr;    // coupling tile grid of every trap (device)
     std::vector<int32_t> cell_tile_base; // first coupling tile of every trap, and the total
     int64_t* h_cell_off = nullptr;   // pinned staging of a call's cell offsets
+    // pair buffers of the tile coupling, sized before a call's pair count is
+    // known: pair_cap slots (0: size the next call from its count, read back);
+    // the last call's count arrives in h_pair_total (pinned, written by a
+    // kernel) once pair_ev has completed
+    int64_t pair_cap = 0;
+    int64_t* h_pair_total = nullptr;
+    hipEvent_t pair_ev = nullptr;
+    bool pair_pending = false;
+    hipEvent_t cell_off_ev = nullptr; // the last upload of h_cell_off (reused only once it has run)
     int64_t* d_cell_off = nullptr;
     double* flux_out = nullptr;      // device scalar
     double* h_flux = nullptr;        // pinned

@@ -4500,6 +4500,9 @@ void eq_batch_destroy(eq_batch* b)
     hipFree(b->d_cell_traps);
     hipFree(b->d_cell_off);
     if (b->h_cell_off) hipHostFree(b->h_cell_off);
+    if (b->h_pair_total) hipHostFree(b->h_pair_total);
+    if (b->pair_ev) hipEventDestroy(b->pair_ev);
+    if (b->cell_off_ev) hipEventDestroy(b->cell_off_ev);
     if (b->h_state) hipHostFree(b->h_state);
     if (b->h_flux) hipHostFree(b->h_flux);
     for (int k = 0; k < EQ_KERNEL_COUNT_; ++k) {

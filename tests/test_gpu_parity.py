@@ -313,6 +313,32 @@ def test_couple_traps_batched_matches_per_trap_and_oracle(eqf, oracle):
             assert np.array_equal(got1[2 * t + s], ref[s]), (t, s)
 
 
+def test_couple_past_the_kept_pair_capacity(eqf, oracle):
+    """The tile coupling sizes its pair buffers from earlier calls: a call with
+    far more pairs than those (run by the list-order kernel instead) and the
+    call after it (buffers grown) both match the sequential loop, fields bit
+    for bit."""
+    from eq_amd import workloads as Wl
+    nx, ny, npm, W, H = 400, 200, 20.0, 20.0, 10.0
+    rng = Wl.SplitMix64(31)
+    grid = oracle.grid(nx + 1, ny + 1, npm, W, H)
+    u = [np.random.default_rng(2).random((nx + 1) * (ny + 1)), np.random.default_rng(3).random((nx + 1) * (ny + 1))]
+    d = [eqf.field_desc(nx, ny, W, H, 0.1, 100.0) for _ in range(2)]
+    with eqf.FieldBatch(d, nspecies=2, solver=eqf.solver_desc(eqf.JACOBI_CG)) as fb:
+        for s in range(2):
+            fb.upload(s, u[s])
+        for nc in (4, 3000, 2500):  # a few pairs, then ~24 000 (past the kept room), then room for them
+            cells, rates = Wl.seeded_cells(rng, nc, W, H, 2)
+            sampled, counts = fb.couple(0, cells, npm, rates)
+            got = [fb.download(s) for s in range(2)]
+            ref, s_ref, c_ref = oracle.couple(grid, cells, u, rates)
+            assert np.array_equal(counts, c_ref), nc
+            assert np.allclose(sampled, s_ref, rtol=2e-13, atol=1e-300), nc
+            for s in range(2):
+                assert np.array_equal(got[s], ref[s]), (nc, s)
+            u = ref
+
+
 def test_autotune_restores_state_and_keeps_parity(eqf, oracle):
     """The autotuner's trial steps leave no trace: fields unchanged, and the
     next step (with the chosen cycle shape) still matches the oracle."""
