@@ -67,6 +67,18 @@ struct CgState {
     double rho, omega, ts, tt;
 };
 
+// One group of the batch's problems as the CG launches see it: global
+// problems [p0, p1); level-0 problem records [pi0, pi0 + npi) (the finish
+// tables); the level-0 tiles; per level, the fused tiles [f0, f0 + nf) and
+// the bottom workgroups [r0, r0 + nr).  Every range is contiguous: the level
+// tables are built problem by problem.
+struct GroupView {
+    int32_t p0 = 0, p1 = 0, pi0 = 0, npi = 0;
+    int32_t t0 = 0, nt = 0;
+    std::vector<int32_t> f0, nf, r0, nr;
+    double nodes = 0.0;
+};
+
 // one bottom V-cycle workgroup: its problem's level records (bottom level,
 // then each coarser level down to the coarsest) start at rec in the level's
 // bottom record table; nl coarser levels follow the bottom one
@@ -203,12 +215,22 @@ struct eq_batch {
     eqk::Acct g_ir2_acct;
     int32_t last_iters = 0;
     // captured launch sequences (created on first use): the set-up, and CG
-    // iterations: g_cg[f][d][n][t] runs the solve's set-up (f = 1), then n
-    // iterations (1 <= n <= kCgGraphMax) from search direction p (d = 0) or
-    // p2 (d = 1), then (t = 1) the convergence confirmation
+    // iterations: g_cg[g][f][d][n][t] runs, for problem group g, the solve's
+    // set-up (f = 1), then n iterations (1 <= n <= kCgGraphMax) from search
+    // direction p (d = 0) or p2 (d = 1), then (t = 1) the convergence
+    // confirmation
     static constexpr int kCgGraphMax = 16;
-    hipGraphExec_t g_cg[2][2][kCgGraphMax + 1][2] = {};
-    eqk::Acct g_cg_acct[2][2][kCgGraphMax + 1][2];
+    hipGraphExec_t g_cg[2][2][2][kCgGraphMax + 1][2] = {};
+    eqk::Acct g_cg_acct[2][2][2][kCgGraphMax + 1][2];
+    // CG problem groups solved concurrently (solver option `streams`): the
+    // views of each (built with the hierarchy; empty when the batch cannot be
+    // split), the second group's stream, and the events that order the two
+    // streams around a solve
+    std::vector<eqk::GroupView> groups;
+    bool groups_built = false;
+    hipStream_t stream2 = nullptr;
+    hipEvent_t grp_ev[2] = {};
+    int32_t last_iters_g[2] = {0, 0};
     hipGraphExec_t g_init = nullptr;
     std::vector<int32_t> iters;
     std::vector<double> relres;

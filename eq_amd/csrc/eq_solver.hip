@@ -3467,6 +3467,8 @@ struct VCycle {
     T* out = nullptr;           // level-0 result
     bool pre1 = false;          // level 0's first sweep from zero is already in xa (and its b in L.b)
     bool fuse_top = false;      // level 0 may run fused (then it forms its first sweep from `in` itself)
+    const GroupView* grp = nullptr; // a problem group: its fused tiles and bottom workgroups only (every level fused)
+    hipStream_t strm = nullptr;     // the stream the launches go to (default: the batch's)
     const CgState* st_ = nullptr; // Krylov state whose `done` the launches test (default: the batch's)
     // an input in the smoother's precision is read directly; an FP64 one is
     // copied to L.b by the first sweep when the smoother runs in FP32
@@ -3509,7 +3511,7 @@ struct VCycle {
     const LevelProb* rz_table() const
     {
         const Level& L0 = b->levels[0];
-        return fused_level(0) ? L0.d_probs_fin : L0.d_probs;
+        return (fused_level(0) ? L0.d_probs_fin : L0.d_probs) + (grp ? grp->pi0 : 0);
     }
 
     void run(int l)
@@ -3534,41 +3536,46 @@ struct VCycle {
         auto wpre = [&](int k) { return (k & 1) ? om2 : om1; };
         auto wpost = [&](int k) { return wpre(S.mg_post - 1 - k); };
         const T omega = om1;
-        hipStream_t s = b->stream;
+        hipStream_t s = strm ? strm : b->stream;
         const CgState* st = st_ ? st_ : b->d_state;
         const bool top = (l == 0);
         const int ns = (int)L.tiles_smooth.size();
         const int gs = grid_for(ns);
+        // the fused tiles this cycle launches (a group's, or all)
+        const FusedTile* ft = (const FusedTile*)L.d_fused + (grp ? grp->f0[l] : 0);
+        const int nf = grp ? grp->nf[l] : (int)L.fused.size(), gf = grid_for(nf);
         T* X[2] = {(T*)L.xa, (T*)L.xb};
         int cur = 0;
         if (ns > 0 && top && fused_level(0)) {
             Level& C = b->levels[1];
-            const int nf = (int)L.fused.size(), gf = grid_for(nf);
             const double nodes = L.nodes_smooth, cnodes = C.nodes_active;
             // b: the Krylov residual `in`, rounded to T as it is read (the
             // first sweep from zero formed from it here)
             launch_tiles(b, EQ_KERNEL_TRANSFER0, nodes * (sizeof(TI) + sizeof(T)) + cnodes * sizeof(T),
-                         k_level_down<T, kFusedTileH0, false, TI>, gf, s, (const FusedTile*)L.d_fused, nf,
+                         k_level_down<T, kFusedTileH0, false, TI>, gf, s, ft, nf,
                          (const LevelProb*)L.d_probs, in, (T*)L.xb, (T*)C.b, wpre(0), wpre(1), st, (const T*)nullptr);
             run(1);
             launch_tiles(b, EQ_KERNEL_SMOOTH, nodes * (2 * sizeof(T) + sizeof(TI)) + cnodes * sizeof(T),
-                         k_level_up<T, TI, kFusedTileH0, true, TI>, gf, s, (const FusedTile*)L.d_fused, nf,
+                         k_level_up<T, TI, kFusedTileH0, true, TI>, gf, s, ft, nf,
                          (const LevelProb*)L.d_probs, (const T*)final_buffer(1), in, (const T*)L.xb, out, wpost(0),
                          wpost(1), st, in, b->partial);
         } else if (ns > 0 && fused_level(l)) {
-            Level& C = b->levels[l + 1];
-            const int nf = (int)L.fused.size(), gf = grid_for(nf);
-            const double nodes = L.nodes_smooth, cnodes = C.nodes_active;
-            // b read, x written; the coarse b written
-            launch_tiles(b, EQ_KERNEL_TRANSFER_COARSE, nodes * 2 * sizeof(T) + cnodes * sizeof(T),
-                         k_level_down<T, kFusedTileHc, false>, gf, s, (const FusedTile*)L.d_fused, nf, (const LevelProb*)L.d_probs,
-                         (const T*)L.b, (T*)L.xb, (T*)C.b, wpre(0), wpre(1), st, (const T*)nullptr);
-            run(l + 1);
-            // x and b read, x written; the coarse correction read
-            launch_tiles(b, EQ_KERNEL_SMOOTH_COARSE, nodes * 3 * sizeof(T) + cnodes * sizeof(T),
-                         k_level_up<T, T, kFusedTileHc, false>, gf, s, (const FusedTile*)L.d_fused, nf, (const LevelProb*)L.d_probs,
-                         (const T*)final_buffer(l + 1), (const T*)L.b, (const T*)L.xb, (T*)L.xa, wpost(0), wpost(1),
-                         st, (const T*)nullptr, (double*)nullptr);
+            // (a group with no problem smoothed here has none below either;
+            // its problems whose bottom is this level follow)
+            if (nf > 0) {
+                Level& C = b->levels[l + 1];
+                const double nodes = L.nodes_smooth, cnodes = C.nodes_active;
+                // b read, x written; the coarse b written
+                launch_tiles(b, EQ_KERNEL_TRANSFER_COARSE, nodes * 2 * sizeof(T) + cnodes * sizeof(T),
+                             k_level_down<T, kFusedTileHc, false>, gf, s, ft, nf, (const LevelProb*)L.d_probs,
+                             (const T*)L.b, (T*)L.xb, (T*)C.b, wpre(0), wpre(1), st, (const T*)nullptr);
+                run(l + 1);
+                // x and b read, x written; the coarse correction read
+                launch_tiles(b, EQ_KERNEL_SMOOTH_COARSE, nodes * 3 * sizeof(T) + cnodes * sizeof(T),
+                             k_level_up<T, T, kFusedTileHc, false>, gf, s, ft, nf, (const LevelProb*)L.d_probs,
+                             (const T*)final_buffer(l + 1), (const T*)L.b, (const T*)L.xb, (T*)L.xa, wpost(0),
+                             wpost(1), st, (const T*)nullptr, (double*)nullptr);
+            }
         } else if (ns > 0) {
             const double nodes = L.nodes_smooth;
             // pre-smoothing; the first sweep starts from x = 0; level 0 reads
@@ -3677,7 +3684,8 @@ struct VCycle {
                 }
             }
         }
-        if (!L.bottom.empty()) {
+        const int r0 = grp ? grp->r0[l] : 0, nr = grp ? grp->nr[l] : (int)L.bottom.size();
+        if (nr > 0) {
             // the bottom solve leaves the level's correction in xl; on level 0
             // it is copied to `out` with the partial (in, out)
             T* xl = final_buffer(l);
@@ -3687,8 +3695,8 @@ struct VCycle {
                              grid_for(nb), s, L.bottom_tiles(), in, (T*)L.b, st);
             // the bottom levels live in LDS: HBM sees the level's b and x and the coarsest inverses
             launch(b, EQ_KERNEL_BOTTOM, L.nodes_bottom * 2 * sizeof(T) + L.bottom_dense_bytes, k_bottom<T, TENS>,
-                   dim3((int)L.bottom.size()), dim3(kBottomThreads), L.bottom_lds_bytes, s,
-                   (const BottomRef*)L.d_bottom, (const LevelProb*)L.d_bottom_recs, (const T*)L.b, xl, om1, om2, S.mg_pre,
+                   dim3(nr), dim3(kBottomThreads), L.bottom_lds_bytes, s,
+                   (const BottomRef*)L.d_bottom + r0, (const LevelProb*)L.d_bottom_recs, (const T*)L.b, xl, om1, om2, S.mg_pre,
                    S.mg_post, (const T*)b->dense, st);
             if (top)
                 launch_tiles(b, EQ_KERNEL_BOTTOM, L.nodes_bottom * (2 * sizeof(T) + sizeof(TI)), k_cast_out<R, T, T, TI>,
@@ -3703,17 +3711,77 @@ struct VCycle {
 // specialisations: a change of any of them drops them all (recaptured on use).
 static void drop_graphs(eq_batch* b)
 {
-    for (auto& f : b->g_cg)
-        for (auto& d : f)
-            for (auto& n : d)
-                for (hipGraphExec_t& g : n) {
-                    if (g) hipGraphExecDestroy(g);
-                    g = nullptr;
-                }
+    for (auto& gr : b->g_cg)
+        for (auto& f : gr)
+            for (auto& d : f)
+                for (auto& n : d)
+                    for (hipGraphExec_t& g : n) {
+                        if (g) hipGraphExecDestroy(g);
+                        g = nullptr;
+                    }
     for (hipGraphExec_t* g : {&b->g_init, &b->g_bicg, &b->g_ir2}) {
         if (*g) hipGraphExecDestroy(*g);
         *g = nullptr;
     }
+}
+
+// The CG problem groups: problems [0, P/2) and [P/2, P), each with the ranges
+// of every level table its launches read.  No groups (a batch of one
+// problem, or a table whose entries are not ordered by group) leaves the
+// batch one group.
+static void build_groups(eq_batch* b)
+{
+    b->groups.clear();
+    b->groups_built = true;
+    const int np = b->nprob, split = np / 2;
+    if (np < 2) return;
+    auto grp_of = [&](int prob) { return prob < split ? 0 : 1; };
+    // a table's two runs, or false when its entries are not ordered by group
+    auto ranges = [&](int n, auto&& group_at, int32_t* begin, int32_t* count) -> bool {
+        begin[0] = 0;
+        begin[1] = n;
+        count[0] = count[1] = 0;
+        int prev = 0;
+        for (int k = 0; k < n; ++k) {
+            const int g = group_at(k);
+            if (g < prev) return false;
+            if (count[g] == 0) begin[g] = k;
+            ++count[g];
+            prev = g;
+        }
+        return true;
+    };
+    std::vector<GroupView> G(2);
+    G[0].p1 = G[1].p0 = split;
+    G[1].p1 = np;
+    int32_t bg[2], ct[2];
+    const Level& L0 = b->levels[0];
+    if (!ranges((int)L0.probs.size(), [&](int k) { return grp_of(L0.probs[k].prob); }, bg, ct)) return;
+    for (int g = 0; g < 2; ++g) {
+        G[g].pi0 = bg[g];
+        G[g].npi = ct[g];
+    }
+    if (!ranges((int)L0.tiles_all.size(), [&](int k) { return grp_of(L0.tiles_all[k].prob); }, bg, ct)) return;
+    for (int g = 0; g < 2; ++g) {
+        G[g].t0 = bg[g];
+        G[g].nt = ct[g];
+    }
+    for (const LevelProb& P : L0.probs) G[grp_of(P.prob)].nodes += (double)(P.nx + 1) * (P.ny + 1);
+    for (const Level& L : b->levels) {
+        if (!ranges((int)L.fused.size(), [&](int k) { return grp_of(L.fused[k].prob); }, bg, ct)) return;
+        for (int g = 0; g < 2; ++g) {
+            G[g].f0.push_back(bg[g]);
+            G[g].nf.push_back(ct[g]);
+        }
+        if (!ranges((int)L.bottom.size(), [&](int k) { return grp_of(L.probs[L.bottom[k]].prob); }, bg, ct)) return;
+        for (int g = 0; g < 2; ++g) {
+            G[g].r0.push_back(bg[g]);
+            G[g].nr.push_back(ct[g]);
+        }
+    }
+    for (int g = 0; g < 2; ++g)
+        if (G[g].npi == 0 || G[g].nt == 0) return;
+    b->groups = G;
 }
 
 static int read_states(eq_batch* b)
@@ -3729,31 +3797,77 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
     const auto t0 = std::chrono::steady_clock::now();
     const eq_solver_desc& S = b->solver;
     Level& L0 = b->levels[0];
-    const LevelView A = L0.all();
-    const int g0 = grid_for(A.ntiles);
-    hipStream_t s = b->stream;
     const bool jacobi = (S.kind == EQ_SOLVER_JACOBI_CG);
-    const double fine_nodes = L0.nodes_all;
-    VCycle<T, TENS> vc(b);
-    const double fin_bytes = (double)L0.tiles_all.size() * 3 * sizeof(double) + 2.0 * b->nprob * sizeof(CgState);
-    auto finish = [&](int op) {
-        launch(b, EQ_KERNEL_FINISH, fin_bytes, k_finish, dim3(b->nprob), dim3(kFinishThreads), 0, s,
-               (const LevelProb*)L0.d_probs, (const double*)b->partial, b->d_state, op, jacobi ? 1 : 0);
-    };
+    // Launch sequences are replayed from hipGraphs unless per-launch event
+    // timing is on (events need eager launches).
+    bool profiling = false;
+    for (int k = 0; k < EQ_KERNEL_COUNT_; ++k) profiling = profiling || b->prof[k].enabled;
+    const bool graphs = S.use_graphs && !profiling;
     // A fused finest level reads the residual and forms the V-cycle's first
     // sweep (x = omega D^-1 r from x = 0) itself.  Otherwise that sweep is
     // pointwise in the new residual: k_update forms it with r, and after
     // k_init or the true residual a sweep launch does; the V-cycle then
     // starts from its second sweep.
-    vc.fuse_top = !jacobi;
-    vc.pre1 = !jacobi && !vc.fused_level(0);
-    T* x1 = vc.pre1 ? (T*)L0.xa : nullptr;
-    T* bcopy = (vc.pre1 && b->mixed) ? (T*)L0.b : nullptr;
+    VCycle<T, TENS> vcs[2] = {VCycle<T, TENS>(b), VCycle<T, TENS>(b)};
+    for (auto& v : vcs) {
+        v.fuse_top = !jacobi;
+        v.pre1 = !jacobi && !v.fused_level(0);
+    }
+    const bool pre1 = vcs[0].pre1;
+    T* x1 = pre1 ? (T*)L0.xa : nullptr;
+    T* bcopy = (pre1 && b->mixed) ? (T*)L0.b : nullptr;
     const T omega = (T)S.mg_omega;
+    // Two problem groups, each on its own stream and stopping at its own
+    // iteration count: one group's latency-bound launches (coarse levels,
+    // bottom solve, finishes) overlap the other's bandwidth-bound ones.  Only
+    // where every level runs on the fused kernels, whose launches take a
+    // group's tiles; the per-problem arithmetic is the same either way.
+    if (!b->groups_built) build_groups(b);
+    bool split = S.streams == 2 && graphs && !jacobi && !b->groups.empty() && vcs[0].fused_level(0) &&
+                 L0.bottom.empty();
+    for (size_t l = 0; split && l < b->levels.size(); ++l)
+        if (!b->levels[l].tiles_smooth.empty() && !vcs[0].fused_level((int)l)) split = false;
+    const int ng = split ? 2 : 1;
+    if (split && !b->stream2) {
+        EQ_HIP(hipStreamCreateWithFlags(&b->stream2, hipStreamNonBlocking));
+        for (hipEvent_t& e : b->grp_ev) EQ_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    if (split)
+        for (int g = 0; g < 2; ++g) {
+            vcs[g].grp = &b->groups[g];
+            vcs[g].strm = g ? b->stream2 : b->stream;
+        }
+    // the group the lambdas below launch for (the whole batch when unsplit)
+    hipStream_t s = b->stream;
+    LevelView A = L0.all();
+    int g0 = grid_for(A.ntiles);
+    double fine_nodes = L0.nodes_all;
+    int fin_n = b->nprob;                  // finish blocks: the group's level-0 records
+    const LevelProb* fin_tab = L0.d_probs; // from here
+    int gp0 = 0, gp1 = b->nprob;           // the group's problems
+    VCycle<T, TENS>* vc = &vcs[0];
+    auto use_group = [&](int g) {
+        vc = &vcs[g];
+        if (ng == 1) return;
+        const GroupView& G = b->groups[g];
+        s = g ? b->stream2 : b->stream;
+        A = LevelView{L0.d_probs, L0.d_tiles_all + G.t0, G.nt};
+        g0 = grid_for(A.ntiles);
+        fine_nodes = G.nodes;
+        fin_n = G.npi;
+        fin_tab = L0.d_probs + G.pi0;
+        gp0 = G.p0;
+        gp1 = G.p1;
+    };
+    auto fin_bytes = [&]() { return (double)A.ntiles * 3 * sizeof(double) + 2.0 * fin_n * sizeof(CgState); };
+    auto finish = [&](int op) {
+        launch(b, EQ_KERNEL_FINISH, fin_bytes(), k_finish, dim3(fin_n), dim3(kFinishThreads), 0, s, fin_tab,
+               (const double*)b->partial, b->d_state, op, jacobi ? 1 : 0);
+    };
     // zero_init 2: for the problems a confirmation restarted only
     auto first_sweep = [&](int zero_init) {
         const int ns = (int)L0.tiles_smooth.size();
-        if (!vc.pre1 || ns == 0) return;
+        if (!pre1 || ns == 0) return;
         launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, L0.nodes_smooth * (sizeof(double) + sizeof(T) * (bcopy ? 2 : 1)),
                      k_smooth<R0, T, double, T, TENS>, grid_for(ns), s, L0.smooth(), (const double*)b->r,
                      (const T*)nullptr, x1, bcopy, omega, zero_init, 0, (double*)nullptr, (const CgState*)b->d_state);
@@ -3761,19 +3875,19 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
     // The finest level's r.z has slots of its own when that level is fused:
     // the update's scalars can then wait for the finish after the next
     // V-cycle (merged), and a chunk of n iterations takes 2n + 1 finishes.
-    const bool merge = !jacobi && vc.fused_level(0);
+    const bool merge = !jacobi && vcs[0].fused_level(0);
     // one CG iteration from search direction pold to pnew; merged: the
     // previous update's finish is still to do
     auto iteration = [&](double* pold, double* pnew, bool merged) {
         if (!jacobi) {
-            vc.run0();
+            vc->run0();
             if (merged)
-                launch(b, EQ_KERNEL_FINISH, fin_bytes + (double)L0.fused.size() * sizeof(double), k_finish_update_rz,
-                       dim3(b->nprob), dim3(kFinishThreads), 0, s, vc.rz_table(), (const LevelProb*)L0.d_probs,
+                launch(b, EQ_KERNEL_FINISH, fin_bytes() + (double)L0.fused.size() * sizeof(double) / ng,
+                       k_finish_update_rz, dim3(fin_n), dim3(kFinishThreads), 0, s, vc->rz_table(), fin_tab,
                        (const double*)b->partial, b->d_state);
             else
-                launch(b, EQ_KERNEL_FINISH, fin_bytes, k_finish, dim3(b->nprob), dim3(kFinishThreads), 0, s,
-                       vc.rz_table(), (const double*)b->partial, b->d_state, (int)kFinRz, 0);
+                launch(b, EQ_KERNEL_FINISH, fin_bytes(), k_finish, dim3(fin_n), dim3(kFinishThreads), 0, s,
+                       vc->rz_table(), (const double*)b->partial, b->d_state, (int)kFinRz, 0);
         }
         // reads z, p_old; writes p_new, q (FP64)
         launch_tiles(b, EQ_KERNEL_STENCIL64, fine_nodes * (3 * sizeof(T) + sizeof(double)), k_ap_dir<R0, T, TENS>, g0, s, A,
@@ -3827,11 +3941,6 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
         finish(kFinInit);
         first_sweep(1);
     };
-    // Launch sequences are replayed from hipGraphs unless per-launch event
-    // timing is on (events need eager launches).
-    bool profiling = false;
-    for (int k = 0; k < EQ_KERNEL_COUNT_; ++k) profiling = profiling || b->prof[k].enabled;
-    const bool graphs = S.use_graphs && !profiling;
     auto capture = [&](hipGraphExec_t* exec, Acct* acct, auto&& body) -> int {
         if (*exec) return EQ_OK;
         hipGraph_t g = nullptr;
@@ -3852,43 +3961,74 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
     };
     int rc;
     // The set-up, each chunk of iterations and the confirmation after it go
-    // out together; the first chunk is the last solve's iteration count, so
-    // a solve that needs as many again takes one graph launch and one read
-    // of the states.
+    // out together, then the group's states to the host; the first chunk is
+    // the group's last iteration count, so a solve that needs as many again
+    // takes one graph launch and one read of the states per group.  Every
+    // group's launches are queued before any group's states are waited for.
     constexpr int kMaxG = eq_batch::kCgGraphMax;
-    int chunk = std::max(1, b->last_iters > 0 ? b->last_iters : S.check_every);
-    int dir = 0;
-    bool all = false;
-    for (int guard = 0; guard < 1000000 && !all; ++guard) {
-        for (int left = chunk; left > 0;) {
-            const int n = std::min(left, kMaxG);
-            const bool tail = (n == left);
-            const int first = (guard == 0 && left == chunk) ? 1 : 0;
-            auto body = [&]() {
-                if (first) setup();
-                iterations(dir, n, tail);
-            };
-            if (graphs) {
-                hipGraphExec_t* g = &b->g_cg[first][dir][n][tail ? 1 : 0];
-                Acct* ac = &b->g_cg_acct[first][dir][n][tail ? 1 : 0];
-                if ((rc = capture(g, ac, body))) return rc;
-                EQ_HIP(hipGraphLaunch(*g, s));
-                replayed(b, *ac);
-            } else {
-                body();
+    int chunk[2] = {1, 1}, dir[2] = {0, 0};
+    bool gdone[2] = {false, ng < 2};
+    for (int g = 0; g < ng; ++g) {
+        const int last = ng == 2 ? b->last_iters_g[g] : b->last_iters;
+        chunk[g] = std::max(1, last > 0 ? last : S.check_every);
+    }
+    if (ng == 2) {
+        // the second group starts behind everything already on the batch's stream
+        EQ_HIP(hipEventRecord(b->grp_ev[0], b->stream));
+        EQ_HIP(hipStreamWaitEvent(b->stream2, b->grp_ev[0], 0));
+    }
+    for (int guard = 0; guard < 1000000 && !(gdone[0] && gdone[1]); ++guard) {
+        for (int g = 0; g < ng; ++g) {
+            if (gdone[g]) continue;
+            use_group(g);
+            for (int left = chunk[g]; left > 0;) {
+                const int n = std::min(left, kMaxG);
+                const bool tail = (n == left);
+                const int first = (guard == 0 && left == chunk[g]) ? 1 : 0;
+                const int d = dir[g];
+                auto body = [&]() {
+                    if (first) setup();
+                    iterations(d, n, tail);
+                };
+                if (graphs) {
+                    hipGraphExec_t* ge = &b->g_cg[g][first][d][n][tail ? 1 : 0];
+                    Acct* ac = &b->g_cg_acct[g][first][d][n][tail ? 1 : 0];
+                    if ((rc = capture(ge, ac, body))) return rc;
+                    EQ_HIP(hipGraphLaunch(*ge, s));
+                    replayed(b, *ac);
+                } else {
+                    body();
+                }
+                dir[g] = (d + n) & 1;
+                left -= n;
             }
-            dir = (dir + n) & 1;
-            left -= n;
+            EQ_HIP(hipMemcpyAsync(b->h_state + gp0, b->d_state + gp0, sizeof(CgState) * (gp1 - gp0),
+                                  hipMemcpyDeviceToHost, s));
         }
         EQ_HIP(hipGetLastError());
-        if ((rc = read_states(b))) return rc;
-        all = true;
-        for (int p = 0; p < b->nprob; ++p) all = all && b->h_state[p].done;
-        chunk = std::max(1, S.check_every);
+        for (int g = 0; g < ng; ++g) {
+            if (gdone[g]) continue;
+            use_group(g);
+            EQ_HIP(hipStreamSynchronize(s));
+            bool all = true;
+            for (int p = gp0; p < gp1; ++p) all = all && b->h_state[p].done;
+            gdone[g] = all;
+            chunk[g] = std::max(1, S.check_every);
+        }
+    }
+    if (ng == 2) {
+        // the batch's stream goes on behind the second group's launches
+        EQ_HIP(hipEventRecord(b->grp_ev[1], b->stream2));
+        EQ_HIP(hipStreamWaitEvent(b->stream, b->grp_ev[1], 0));
+        for (int g = 0; g < 2; ++g) {
+            int32_t m = 0;
+            for (int p = b->groups[g].p0; p < b->groups[g].p1; ++p) m = std::max(m, b->h_state[p].iter);
+            b->last_iters_g[g] = m;
+        }
     }
     if (!keep)
-        launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, fine_nodes * 32.0, k_keep<R0>, g0, s, A, (const double*)b->u, b->xprev,
-                     b->xprev2, b->d_state);
+        launch_tiles(b, EQ_KERNEL_SOLVE_ONCE, L0.nodes_all * 32.0, k_keep<R0>, grid_for((int)L0.tiles_all.size()),
+                     b->stream, L0.all(), (const double*)b->u, b->xprev, b->xprev2, b->d_state);
     EQ_HIP(hipGetLastError());
     prof_collect(b);
     int32_t itmax = 0, nconv = 0;
@@ -4311,6 +4451,7 @@ void eq_solver_default(eq_solver_desc* s)
     s->ir_inner_rtol = 1e-4;
     s->mg_level_fused = 1;
     s->mg_omega2 = 1.0 / (1.25 - 0.75 * 0.70710678118654752);
+    s->streams = 2;
 }
 
 int eq_batch_create(const eq_field_desc* descs, int32_t nprob, int32_t nspecies, const eq_solver_desc* solver,
@@ -4334,6 +4475,7 @@ int eq_batch_create(const eq_field_desc* descs, int32_t nprob, int32_t nspecies,
     if (S.kind == EQ_SOLVER_IR_GMG_CG && (!(S.ir_inner_rtol > 0 && S.ir_inner_rtol < 1) || S.ir_max_outer < 1))
         return set_error(EQ_ERR_INVALID, "refinement needs 0 < ir_inner_rtol < 1 and ir_max_outer >= 1");
     if (!(S.rtol > 0) || S.max_iter < 1 || S.check_every < 1) return set_error(EQ_ERR_INVALID, "bad solver tolerances");
+    if (S.streams < 1 || S.streams > 2) return set_error(EQ_ERR_INVALID, "streams must be 1 or 2");
     if (S.kind != EQ_SOLVER_JACOBI_CG &&
         (S.mg_pre < 1 || S.mg_post < 1 || !(S.mg_omega > 0) || !(S.mg_omega2 >= 0) || S.mg_coarse_max_nodes < 4 ||
          S.mg_max_levels < 1))
@@ -4509,6 +4651,9 @@ void eq_batch_destroy(eq_batch* b)
         for (auto& e : b->prof[k].pool) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
         for (auto& e : b->prof[k].pending) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
     }
+    for (hipEvent_t e : b->grp_ev)
+        if (e) hipEventDestroy(e);
+    if (b->stream2) hipStreamDestroy(b->stream2);
     if (b->stream) hipStreamDestroy(b->stream);
     delete b;
 }
@@ -5020,6 +5165,7 @@ int eq_batch_autotune(eq_batch* b, int32_t repeats, eq_solver_desc* chosen)
     };
     const eq_solver_desc orig = b->solver;
     const int32_t orig_last = b->last_iters;
+    const int32_t orig_last_g[2] = {b->last_iters_g[0], b->last_iters_g[1]};
     int rc = copy_state(true);
     double best_t = 1e300;
     int best = -1, best_fuse = orig.mg_fuse_max_nodes;
@@ -5045,6 +5191,8 @@ int eq_batch_autotune(eq_batch* b, int32_t repeats, eq_solver_desc* chosen)
         bool ok = true;
         for (int r = 0; r <= reps && rc == EQ_OK; ++r) { // r = 0 captures the graphs, untimed
             b->last_iters = orig_last;
+            b->last_iters_g[0] = orig_last_g[0];
+            b->last_iters_g[1] = orig_last_g[1];
             eq_step_stats st{};
             const auto t0 = std::chrono::steady_clock::now();
             const int src = run_solve(b, &st);
@@ -5073,6 +5221,8 @@ int eq_batch_autotune(eq_batch* b, int32_t repeats, eq_solver_desc* chosen)
         b->solver.mg_fuse_pre_top = best_top;
     }
     b->last_iters = orig_last;
+    b->last_iters_g[0] = orig_last_g[0];
+    b->last_iters_g[1] = orig_last_g[1];
     drop_graphs(b);
     hipFree(save);
     hipFree(save_st);

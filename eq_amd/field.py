@@ -109,7 +109,8 @@ class SolverDesc(C.Structure):
                 ("mg_bottom_lds_bytes", C.c_int32), ("mg_fuse_pre_top", C.c_int32),
                 ("tile4_min_nodes", C.c_int32), ("tile8_min_nodes", C.c_int32),
                 ("extrapolate", C.c_int32), ("ir_max_outer", C.c_int32), ("ir_inner_rtol", C.c_double),
-                ("mg_level_fused", C.c_int32), ("mg_omega2", C.c_double)]
+                ("mg_level_fused", C.c_int32), ("mg_omega2", C.c_double),
+                ("streams", C.c_int32)]
 
 
 class StepStats(C.Structure):

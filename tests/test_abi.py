@@ -43,6 +43,7 @@ def test_library_loads_and_reports_defaults():
     w1, w2 = 1.0 / (1.25 + 0.75 * math.cos(math.pi / 4)), 1.0 / (1.25 - 0.75 * math.cos(math.pi / 4))
     assert abs(s.mg_omega - w1) < 1e-15 and abs(s.mg_omega2 - w2) < 1e-15
     assert s.mg_level_fused == 1
+    assert s.streams == 2
     assert L.eq_last_error() is not None
 
 

@@ -766,6 +766,47 @@ def test_affine_coupling_matches_oracle(eqf, oracle):
             assert _rel(fb.download(k), ref) <= 1e-9, k
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("solver", ["gmg", "mixed"])
+@pytest.mark.parametrize("traps", [1, 3])
+def test_two_problem_groups_match_one(eqf, oracle, solver, traps):
+    """The CG kinds solve a batch as two problem groups on their own streams
+    (solver option streams = 2, the default) or as one (streams = 1): every
+    problem's field bit for bit the same, and its iteration count, over
+    three steps of species with different diffusivities (so the groups stop
+    at different counts); the oracle's LU within 1e-9."""
+    kind = {"gmg": eqf.GMG_CG, "mixed": eqf.MIXED_GMG_CG}[solver]
+    shapes = [(300, 150, 18.75, 9.375), (203, 97, 12.6875, 6.0625), (131, 67, 8.1875, 4.1875)][:traps]
+    walls = {oracle.TOP: (oracle.DIRICHLET, 0.0, 0, 0), oracle.LEFT: (oracle.ROBIN, 0, 1.5, 0.2)}
+    descs, probs, u0 = [], [], []
+    rng = np.random.default_rng(43)
+    for (nx, ny, W, H) in shapes:
+        for D in (400.0, 30.0):
+            descs.append(eqf.field_desc(nx, ny, W, H, 0.1, D, 0.01, 0.0, walls))
+            probs.append(oracle.problem(nx, ny, W, H, 0.1, D, 0.01, 0.0, walls=walls))
+            u0.append(rng.random((nx + 1) * (ny + 1)))
+    out, its = {}, {}
+    for streams in (2, 1):
+        with eqf.FieldBatch(descs, nspecies=2, solver=eqf.solver_desc(kind, streams=streams)) as fb:
+            for k in range(len(descs)):
+                fb.upload(k, u0[k])
+            u, n = [], []
+            for step in range(3):
+                print(f"problem groups {solver} traps={traps} streams={streams} step {step}", flush=True)
+                st = fb.step()
+                assert st.nconverged == len(descs), (streams, step)
+                u.append([fb.download(k) for k in range(len(descs))])
+                n.append(list(fb.problem_stats()[0]))
+            out[streams], its[streams] = u, n
+    ref = [x.copy() for x in u0]
+    for step in range(3):
+        assert its[2][step] == its[1][step], (step, its[2][step], its[1][step])
+        for k in range(len(descs)):
+            assert np.array_equal(out[2][step][k], out[1][step][k]), (step, k)
+            ref[k], _ = oracle.step(probs[k], ref[k], "lu")
+            assert _rel(out[2][step][k], ref[k]) <= 1e-9, (step, k)
+
+
 @pytest.mark.parametrize("solver", ["gmg", "mixed", "ir"])
 def test_fused_levels_match_separate_launches(eqf, oracle, solver):
     """The fused level kernels (a level's way down and way up in one launch
