@@ -77,6 +77,9 @@ struct GroupView {
     int32_t t0 = 0, nt = 0;
     std::vector<int32_t> f0, nf, r0, nr;
     double nodes = 0.0;
+    // per level, the group's smoothed, active and bottom nodes and its share
+    // of the coarsest inverses' bytes (the launches' algorithmic bytes)
+    std::vector<double> nsm, nact, nbot, dense;
 };
 
 // one bottom V-cycle workgroup: its problem's level records (bottom level,

@@ -3541,14 +3541,19 @@ struct VCycle {
         const bool top = (l == 0);
         const int ns = (int)L.tiles_smooth.size();
         const int gs = grid_for(ns);
-        // the fused tiles this cycle launches (a group's, or all)
+        // the fused tiles this cycle launches (a group's, or all), and the
+        // nodes its launches count: a group's, or the level's
         const FusedTile* ft = (const FusedTile*)L.d_fused + (grp ? grp->f0[l] : 0);
         const int nf = grp ? grp->nf[l] : (int)L.fused.size(), gf = grid_for(nf);
+        const double lv_smooth = grp ? grp->nsm[l] : L.nodes_smooth;
+        const double lv_bottom = grp ? grp->nbot[l] : L.nodes_bottom;
+        const double lv_dense = grp ? grp->dense[l] : L.bottom_dense_bytes;
+        auto next_active = [&]() { return grp ? grp->nact[l + 1] : b->levels[l + 1].nodes_active; };
         T* X[2] = {(T*)L.xa, (T*)L.xb};
         int cur = 0;
         if (ns > 0 && top && fused_level(0)) {
             Level& C = b->levels[1];
-            const double nodes = L.nodes_smooth, cnodes = C.nodes_active;
+            const double nodes = lv_smooth, cnodes = next_active();
             // b: the Krylov residual `in`, rounded to T as it is read (the
             // first sweep from zero formed from it here)
             launch_tiles(b, EQ_KERNEL_TRANSFER0, nodes * (sizeof(TI) + sizeof(T)) + cnodes * sizeof(T),
@@ -3564,7 +3569,7 @@ struct VCycle {
             // its problems whose bottom is this level follow)
             if (nf > 0) {
                 Level& C = b->levels[l + 1];
-                const double nodes = L.nodes_smooth, cnodes = C.nodes_active;
+                const double nodes = lv_smooth, cnodes = next_active();
                 // b read, x written; the coarse b written
                 launch_tiles(b, EQ_KERNEL_TRANSFER_COARSE, nodes * 2 * sizeof(T) + cnodes * sizeof(T),
                              k_level_down<T, kFusedTileHc, false>, gf, s, ft, nf, (const LevelProb*)L.d_probs,
@@ -3694,7 +3699,7 @@ struct VCycle {
                 launch_tiles(b, EQ_KERNEL_BOTTOM, L.nodes_bottom * (sizeof(TI) + sizeof(T)), k_cast_in<R, T, TI>,
                              grid_for(nb), s, L.bottom_tiles(), in, (T*)L.b, st);
             // the bottom levels live in LDS: HBM sees the level's b and x and the coarsest inverses
-            launch(b, EQ_KERNEL_BOTTOM, L.nodes_bottom * 2 * sizeof(T) + L.bottom_dense_bytes, k_bottom<T, TENS>,
+            launch(b, EQ_KERNEL_BOTTOM, lv_bottom * 2 * sizeof(T) + lv_dense, k_bottom<T, TENS>,
                    dim3(nr), dim3(kBottomThreads), L.bottom_lds_bytes, s,
                    (const BottomRef*)L.d_bottom + r0, (const LevelProb*)L.d_bottom_recs, (const T*)L.b, xl, om1, om2, S.mg_pre,
                    S.mg_post, (const T*)b->dense, st);
@@ -3768,6 +3773,19 @@ static void build_groups(eq_batch* b)
     }
     for (const LevelProb& P : L0.probs) G[grp_of(P.prob)].nodes += (double)(P.nx + 1) * (P.ny + 1);
     for (const Level& L : b->levels) {
+        double nsm[2] = {0, 0}, nact[2] = {0, 0}, nbot[2] = {0, 0};
+        for (const LevelProb& P : L.probs) {
+            const double n = (double)(P.nx + 1) * (P.ny + 1);
+            const int g = grp_of(P.prob);
+            if (!P.bottom && !P.below) nsm[g] += n;
+            if (!P.below) nact[g] += n;
+            if (P.bottom && !P.below) nbot[g] += n;
+        }
+        for (int g = 0; g < 2; ++g) {
+            G[g].nsm.push_back(nsm[g]);
+            G[g].nact.push_back(nact[g]);
+            G[g].nbot.push_back(nbot[g]);
+        }
         if (!ranges((int)L.fused.size(), [&](int k) { return grp_of(L.fused[k].prob); }, bg, ct)) return;
         for (int g = 0; g < 2; ++g) {
             G[g].f0.push_back(bg[g]);
@@ -3777,6 +3795,7 @@ static void build_groups(eq_batch* b)
         for (int g = 0; g < 2; ++g) {
             G[g].r0.push_back(bg[g]);
             G[g].nr.push_back(ct[g]);
+            G[g].dense.push_back(L.bottom.empty() ? 0.0 : L.bottom_dense_bytes * ct[g] / (double)L.bottom.size());
         }
     }
     for (int g = 0; g < 2; ++g)
