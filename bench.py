@@ -32,6 +32,14 @@ HBM_PEAK_GBS = 8000.0
 # WRITE_SIZE counters give for the family's launches
 
 
+# solver options every batch of the run takes (bench flags)
+SOLVER_OPTS = {}
+
+
+def solver_desc(F, kind, **kw):
+    return F.solver_desc(kind, **{**SOLVER_OPTS, **kw})
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -56,7 +64,7 @@ class Run:
         self.trap = trap
         descs = [F.field_desc(trap.nx, trap.ny, trap.width, trap.height, dt, D, g, 0.0, W.TOP_DIRICHLET)
                  for D, g in zip(trap.D, trap.gamma)]
-        self.fb = F.FieldBatch(descs, nspecies=len(descs), solver=F.solver_desc(solver, **solver_kw))
+        self.fb = F.FieldBatch(descs, nspecies=len(descs), solver=solver_desc(F, solver, **solver_kw))
         for s, u in enumerate(trap.u0):
             self.fb.upload(s, u)
         self.cells = cells_dev
@@ -197,7 +205,7 @@ def gpu_config0(F, W, nsteps=100):
     descs = [F.field_desc(t.nx, t.ny, t.width, t.height, 0.1, D, g, 0.0, W.TOP_DIRICHLET)
              for D, g in zip(t.D, t.gamma)]
     q = t.rates * 0.1
-    with F.FieldBatch(descs, nspecies=2, solver=F.solver_desc(F.MIXED_GMG_CG)) as fb:
+    with F.FieldBatch(descs, nspecies=2, solver=solver_desc(F, F.MIXED_GMG_CG)) as fb:
         for s_ in range(2):
             fb.upload(s_, t.u0[s_])
         fb.couple(0, t.cells, t.npm, q)
@@ -415,7 +423,7 @@ def config3_batched(F, W, nsteps, warmup=2, ntraps=512, sigma=0.05, prof=2):
     cells_d = torch.from_numpy(walk).to("cuda").contiguous()
     sampled = torch.empty((cells0.shape[0], 2), dtype=torch.float64, device="cuda")
     t_create = time.perf_counter()
-    fb = F.FieldBatch(descs, nspecies=2, solver=F.solver_desc(F.MIXED_GMG_CG))
+    fb = F.FieldBatch(descs, nspecies=2, solver=solver_desc(F, F.MIXED_GMG_CG))
     t_create = time.perf_counter() - t_create
     stride = cells0.shape[0] * 5 * 8
 
@@ -472,7 +480,7 @@ def config4_advection(F, W, nsteps, warmup=2, prof=1):
     q = torch.from_numpy(trap.rates * 0.1).to("cuda").contiguous()
     sampled = torch.empty((nc, len(descs)), dtype=torch.float64, device="cuda")
     t_create = time.perf_counter()
-    fb = F.FieldBatch(descs, nspecies=len(descs), solver=F.solver_desc(F.MIXED_GMG_BICGSTAB))
+    fb = F.FieldBatch(descs, nspecies=len(descs), solver=solver_desc(F, F.MIXED_GMG_BICGSTAB))
     for s in range(len(descs)):
         fb.set_velocity(s, vx)
     t_create = time.perf_counter() - t_create
@@ -540,10 +548,15 @@ def main():
                     help="timed steps of config 4 (8192x4096 trap, 4 species, advection), part of the variants")
     ap.add_argument("--only", choices=["c3", "c4"], default=None,
                     help="run only the config-3 batch or the config-4 trap (for profiling) and print its line")
+    ap.add_argument("--streams", type=int, default=None, choices=[1, 2],
+                    help="solver option streams (problem groups of a CG solve; default: the library's, 2); the "
+                         "counter runs take 1, so each counted launch covers the whole batch as in the family timing")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 256 MiB read + 256 MiB write elementwise kernel (torch.mul) three times first, "
                          "as the known-size reference for HBM byte counters")
     args = ap.parse_args()
+    if args.streams is not None:
+        SOLVER_OPTS["streams"] = args.streams
     if args.gpus != 1:
         print(json.dumps({"error": "this field step runs on one GPU; multi-GPU runs are not part of this build",
                           "n_gpus": args.gpus}))
