@@ -625,6 +625,10 @@ def main():
             log(f"bench: {f['family']}: {f['ms_per_step']:.3f} ms/step ({f['share']:.1%}), "
                 f"{f['achieved_gbs']:.0f} GB/s")
     fields_final = [run.fb.download(s) for s in range(2)]
+    # each workload below runs on its own batch: this one's streams go now (a
+    # CG solve overlaps its two problem groups only while their streams sit on
+    # different hardware queues, and the device has few)
+    run.fb.close()
     log(f"bench: {value:.4e} DOF-timesteps/s, {elapsed / args.steps * 1e3:.3f} ms/step, "
         f"iterations max {max(iters)}, relres max {max(relres):.2e}")
 

@@ -219,7 +219,7 @@ struct eq_batch {
     int32_t last_iters = 0;
     // captured launch sequences (created on first use): the set-up, and CG
     // iterations: g_cg[g][f][d][n][t] runs, for problem group g, the solve's
-    // set-up (f = 1), then n iterations (1 <= n <= kCgGraphMax) from search
+    // set-up (f = 1), then n iterations (n a power of two up to kCgGraphMax) from search
     // direction p (d = 0) or p2 (d = 1), then (t = 1) the convergence
     // confirmation
     static constexpr int kCgGraphMax = 16;

@@ -3982,8 +3982,8 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
     // The set-up, each chunk of iterations and the confirmation after it go
     // out together, then the group's states to the host; the first chunk is
     // the group's last iteration count, so a solve that needs as many again
-    // takes one graph launch and one read of the states per group.  Every
-    // group's launches are queued before any group's states are waited for.
+    // reads the states once per group.  Every group's launches are queued
+    // before any group's states are waited for.
     constexpr int kMaxG = eq_batch::kCgGraphMax;
     int chunk[2] = {1, 1}, dir[2] = {0, 0};
     bool gdone[2] = {false, ng < 2};
@@ -4001,7 +4001,10 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
             if (gdone[g]) continue;
             use_group(g);
             for (int left = chunk[g]; left > 0;) {
-                const int n = std::min(left, kMaxG);
+                // graphs of 16, 8, 4, 2 and 1 iterations: the few a solve can
+                // need are captured once, whatever its iteration count
+                int n = 1;
+                while (2 * n <= std::min(left, kMaxG)) n *= 2;
                 const bool tail = (n == left);
                 const int first = (guard == 0 && left == chunk[g]) ? 1 : 0;
                 const int d = dir[g];
