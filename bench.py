@@ -548,7 +548,7 @@ def main():
                     help="timed steps of config 4 (8192x4096 trap, 4 species, advection), part of the variants")
     ap.add_argument("--only", choices=["c3", "c4"], default=None,
                     help="run only the config-3 batch or the config-4 trap (for profiling) and print its line")
-    ap.add_argument("--streams", type=int, default=None, choices=[1, 2],
+    ap.add_argument("--streams", type=int, default=None, choices=[1, 2, 3, 4],
                     help="solver option streams (problem groups of a CG solve; default: the library's, 2); the "
                          "counter runs take 1, so each counted launch covers the whole batch as in the family timing")
     ap.add_argument("--copy-probe", action="store_true",

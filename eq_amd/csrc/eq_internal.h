@@ -223,17 +223,19 @@ struct eq_batch {
     // direction p (d = 0) or p2 (d = 1), then (t = 1) the convergence
     // confirmation
     static constexpr int kCgGraphMax = 16;
-    hipGraphExec_t g_cg[2][2][2][kCgGraphMax + 1][2] = {};
-    eqk::Acct g_cg_acct[2][2][2][kCgGraphMax + 1][2];
+    static constexpr int kMaxGroups = 4;
+    hipGraphExec_t g_cg[kMaxGroups][2][2][kCgGraphMax + 1][2] = {};
+    eqk::Acct g_cg_acct[kMaxGroups][2][2][kCgGraphMax + 1][2];
     // CG problem groups solved concurrently (solver option `streams`): the
-    // views of each (built with the hierarchy; empty when the batch cannot be
-    // split), the second group's stream, and the events that order the two
-    // streams around a solve
+    // views of each (built for groups_for groups on first use; empty when the
+    // batch cannot be split), the streams of groups 1.. (group 0 runs on the
+    // batch's stream), and the events that order the streams around a solve
     std::vector<eqk::GroupView> groups;
-    bool groups_built = false;
-    hipStream_t stream2 = nullptr;
-    hipEvent_t grp_ev[2] = {};
-    int32_t last_iters_g[2] = {0, 0};
+    int32_t groups_for = 0;
+    hipStream_t gstream[kMaxGroups] = {};
+    hipEvent_t grp_start = nullptr;
+    hipEvent_t grp_done[kMaxGroups] = {};
+    int32_t last_iters_g[kMaxGroups] = {};
     hipGraphExec_t g_init = nullptr;
     std::vector<int32_t> iters;
     std::vector<double> relres;

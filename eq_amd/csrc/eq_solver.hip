@@ -3730,75 +3730,80 @@ static void drop_graphs(eq_batch* b)
     }
 }
 
-// The CG problem groups: problems [0, P/2) and [P/2, P), each with the ranges
-// of every level table its launches read.  No groups (a batch of one
-// problem, or a table whose entries are not ordered by group) leaves the
-// batch one group.
-static void build_groups(eq_batch* b)
+// The CG problem groups: n runs of consecutive problems (group k: problems
+// [k P / n, (k+1) P / n)), each with the ranges of every level table its
+// launches read.  No groups (fewer than two problems, or a table whose entries
+// are not ordered by group) leaves the batch one group.
+static void build_groups(eq_batch* b, int n)
 {
     b->groups.clear();
-    b->groups_built = true;
-    const int np = b->nprob, split = np / 2;
-    if (np < 2) return;
-    auto grp_of = [&](int prob) { return prob < split ? 0 : 1; };
-    // a table's two runs, or false when its entries are not ordered by group
-    auto ranges = [&](int n, auto&& group_at, int32_t* begin, int32_t* count) -> bool {
-        begin[0] = 0;
-        begin[1] = n;
-        count[0] = count[1] = 0;
+    b->groups_for = n;
+    const int np = b->nprob;
+    n = std::min(n, np);
+    if (n < 2) return;
+    std::vector<int> gof(np);
+    for (int g = 0; g < n; ++g)
+        for (int p = (int)((int64_t)g * np / n); p < (int)((int64_t)(g + 1) * np / n); ++p) gof[p] = g;
+    auto grp_of = [&](int prob) { return gof[prob]; };
+    // a table's runs per group, or false when its entries are not ordered by group
+    std::vector<int32_t> bg(n), ct(n);
+    auto ranges = [&](int m, auto&& group_at) -> bool {
+        std::fill(bg.begin(), bg.end(), 0);
+        std::fill(ct.begin(), ct.end(), 0);
         int prev = 0;
-        for (int k = 0; k < n; ++k) {
+        for (int k = 0; k < m; ++k) {
             const int g = group_at(k);
             if (g < prev) return false;
-            if (count[g] == 0) begin[g] = k;
-            ++count[g];
+            if (ct[g] == 0) bg[g] = k;
+            ++ct[g];
             prev = g;
         }
         return true;
     };
-    std::vector<GroupView> G(2);
-    G[0].p1 = G[1].p0 = split;
-    G[1].p1 = np;
-    int32_t bg[2], ct[2];
+    std::vector<GroupView> G(n);
+    for (int g = 0; g < n; ++g) {
+        G[g].p0 = (int32_t)((int64_t)g * np / n);
+        G[g].p1 = (int32_t)((int64_t)(g + 1) * np / n);
+    }
     const Level& L0 = b->levels[0];
-    if (!ranges((int)L0.probs.size(), [&](int k) { return grp_of(L0.probs[k].prob); }, bg, ct)) return;
-    for (int g = 0; g < 2; ++g) {
+    if (!ranges((int)L0.probs.size(), [&](int k) { return grp_of(L0.probs[k].prob); })) return;
+    for (int g = 0; g < n; ++g) {
         G[g].pi0 = bg[g];
         G[g].npi = ct[g];
     }
-    if (!ranges((int)L0.tiles_all.size(), [&](int k) { return grp_of(L0.tiles_all[k].prob); }, bg, ct)) return;
-    for (int g = 0; g < 2; ++g) {
+    if (!ranges((int)L0.tiles_all.size(), [&](int k) { return grp_of(L0.tiles_all[k].prob); })) return;
+    for (int g = 0; g < n; ++g) {
         G[g].t0 = bg[g];
         G[g].nt = ct[g];
     }
     for (const LevelProb& P : L0.probs) G[grp_of(P.prob)].nodes += (double)(P.nx + 1) * (P.ny + 1);
     for (const Level& L : b->levels) {
-        double nsm[2] = {0, 0}, nact[2] = {0, 0}, nbot[2] = {0, 0};
+        std::vector<double> nsm(n, 0.0), nact(n, 0.0), nbot(n, 0.0);
         for (const LevelProb& P : L.probs) {
-            const double n = (double)(P.nx + 1) * (P.ny + 1);
+            const double v = (double)(P.nx + 1) * (P.ny + 1);
             const int g = grp_of(P.prob);
-            if (!P.bottom && !P.below) nsm[g] += n;
-            if (!P.below) nact[g] += n;
-            if (P.bottom && !P.below) nbot[g] += n;
+            if (!P.bottom && !P.below) nsm[g] += v;
+            if (!P.below) nact[g] += v;
+            if (P.bottom && !P.below) nbot[g] += v;
         }
-        for (int g = 0; g < 2; ++g) {
+        for (int g = 0; g < n; ++g) {
             G[g].nsm.push_back(nsm[g]);
             G[g].nact.push_back(nact[g]);
             G[g].nbot.push_back(nbot[g]);
         }
-        if (!ranges((int)L.fused.size(), [&](int k) { return grp_of(L.fused[k].prob); }, bg, ct)) return;
-        for (int g = 0; g < 2; ++g) {
+        if (!ranges((int)L.fused.size(), [&](int k) { return grp_of(L.fused[k].prob); })) return;
+        for (int g = 0; g < n; ++g) {
             G[g].f0.push_back(bg[g]);
             G[g].nf.push_back(ct[g]);
         }
-        if (!ranges((int)L.bottom.size(), [&](int k) { return grp_of(L.probs[L.bottom[k]].prob); }, bg, ct)) return;
-        for (int g = 0; g < 2; ++g) {
+        if (!ranges((int)L.bottom.size(), [&](int k) { return grp_of(L.probs[L.bottom[k]].prob); })) return;
+        for (int g = 0; g < n; ++g) {
             G[g].r0.push_back(bg[g]);
             G[g].nr.push_back(ct[g]);
             G[g].dense.push_back(L.bottom.empty() ? 0.0 : L.bottom_dense_bytes * ct[g] / (double)L.bottom.size());
         }
     }
-    for (int g = 0; g < 2; ++g)
+    for (int g = 0; g < n; ++g)
         if (G[g].npi == 0 || G[g].nt == 0) return;
     b->groups = G;
 }
@@ -3827,7 +3832,8 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
     // pointwise in the new residual: k_update forms it with r, and after
     // k_init or the true residual a sweep launch does; the V-cycle then
     // starts from its second sweep.
-    VCycle<T, TENS> vcs[2] = {VCycle<T, TENS>(b), VCycle<T, TENS>(b)};
+    constexpr int kMG = eq_batch::kMaxGroups;
+    VCycle<T, TENS> vcs[kMG] = {VCycle<T, TENS>(b), VCycle<T, TENS>(b), VCycle<T, TENS>(b), VCycle<T, TENS>(b)};
     for (auto& v : vcs) {
         v.fuse_top = !jacobi;
         v.pre1 = !jacobi && !v.fused_level(0);
@@ -3836,25 +3842,29 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
     T* x1 = pre1 ? (T*)L0.xa : nullptr;
     T* bcopy = (pre1 && b->mixed) ? (T*)L0.b : nullptr;
     const T omega = (T)S.mg_omega;
-    // Two problem groups, each on its own stream and stopping at its own
-    // iteration count: one group's latency-bound launches (coarse levels,
-    // bottom solve, finishes) overlap the other's bandwidth-bound ones.  Only
-    // where every level runs on the fused kernels, whose launches take a
-    // group's tiles; the per-problem arithmetic is the same either way.
-    if (!b->groups_built) build_groups(b);
-    bool split = S.streams == 2 && graphs && !jacobi && !b->groups.empty() && vcs[0].fused_level(0) &&
+    // Problem groups (solver option streams), each on its own stream and
+    // stopping at its own iteration count: one group's latency-bound launches
+    // (coarse levels, bottom solve, finishes) overlap another's bandwidth-
+    // bound ones.  Only where every level runs on the fused kernels, whose
+    // launches take a group's tiles; the per-problem arithmetic is the same.
+    if (b->groups_for != S.streams) build_groups(b, S.streams);
+    bool split = S.streams >= 2 && graphs && !jacobi && !b->groups.empty() && vcs[0].fused_level(0) &&
                  L0.bottom.empty();
     for (size_t l = 0; split && l < b->levels.size(); ++l)
         if (!b->levels[l].tiles_smooth.empty() && !vcs[0].fused_level((int)l)) split = false;
-    const int ng = split ? 2 : 1;
-    if (split && !b->stream2) {
-        EQ_HIP(hipStreamCreateWithFlags(&b->stream2, hipStreamNonBlocking));
-        for (hipEvent_t& e : b->grp_ev) EQ_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    const int ng = split ? (int)b->groups.size() : 1;
+    if (split) {
+        if (!b->grp_start) EQ_HIP(hipEventCreateWithFlags(&b->grp_start, hipEventDisableTiming));
+        for (int g = 1; g < ng; ++g) {
+            if (!b->gstream[g]) EQ_HIP(hipStreamCreateWithFlags(&b->gstream[g], hipStreamNonBlocking));
+            if (!b->grp_done[g]) EQ_HIP(hipEventCreateWithFlags(&b->grp_done[g], hipEventDisableTiming));
+        }
     }
+    auto gstream = [&](int g) { return g ? b->gstream[g] : b->stream; };
     if (split)
-        for (int g = 0; g < 2; ++g) {
+        for (int g = 0; g < ng; ++g) {
             vcs[g].grp = &b->groups[g];
-            vcs[g].strm = g ? b->stream2 : b->stream;
+            vcs[g].strm = gstream(g);
         }
     // the group the lambdas below launch for (the whole batch when unsplit)
     hipStream_t s = b->stream;
@@ -3869,7 +3879,7 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
         vc = &vcs[g];
         if (ng == 1) return;
         const GroupView& G = b->groups[g];
-        s = g ? b->stream2 : b->stream;
+        s = gstream(g);
         A = LevelView{L0.d_probs, L0.d_tiles_all + G.t0, G.nt};
         g0 = grid_for(A.ntiles);
         fine_nodes = G.nodes;
@@ -3985,18 +3995,25 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
     // reads the states once per group.  Every group's launches are queued
     // before any group's states are waited for.
     constexpr int kMaxG = eq_batch::kCgGraphMax;
-    int chunk[2] = {1, 1}, dir[2] = {0, 0};
-    bool gdone[2] = {false, ng < 2};
-    for (int g = 0; g < ng; ++g) {
-        const int last = ng == 2 ? b->last_iters_g[g] : b->last_iters;
+    int chunk[kMG], dir[kMG];
+    bool gdone[kMG];
+    for (int g = 0; g < kMG; ++g) {
+        const int last = ng > 1 ? b->last_iters_g[g] : b->last_iters;
         chunk[g] = std::max(1, last > 0 ? last : S.check_every);
+        dir[g] = 0;
+        gdone[g] = g >= ng;
     }
-    if (ng == 2) {
-        // the second group starts behind everything already on the batch's stream
-        EQ_HIP(hipEventRecord(b->grp_ev[0], b->stream));
-        EQ_HIP(hipStreamWaitEvent(b->stream2, b->grp_ev[0], 0));
+    if (ng > 1) {
+        // the other groups start behind everything already on the batch's stream
+        EQ_HIP(hipEventRecord(b->grp_start, b->stream));
+        for (int g = 1; g < ng; ++g) EQ_HIP(hipStreamWaitEvent(b->gstream[g], b->grp_start, 0));
     }
-    for (int guard = 0; guard < 1000000 && !(gdone[0] && gdone[1]); ++guard) {
+    auto all_done = [&]() {
+        for (int g = 0; g < ng; ++g)
+            if (!gdone[g]) return false;
+        return true;
+    };
+    for (int guard = 0; guard < 1000000 && !all_done(); ++guard) {
         for (int g = 0; g < ng; ++g) {
             if (gdone[g]) continue;
             use_group(g);
@@ -4038,11 +4055,13 @@ static int solve_all(eq_batch* b, eq_step_stats* stats)
             chunk[g] = std::max(1, S.check_every);
         }
     }
-    if (ng == 2) {
-        // the batch's stream goes on behind the second group's launches
-        EQ_HIP(hipEventRecord(b->grp_ev[1], b->stream2));
-        EQ_HIP(hipStreamWaitEvent(b->stream, b->grp_ev[1], 0));
-        for (int g = 0; g < 2; ++g) {
+    if (ng > 1) {
+        // the batch's stream goes on behind the other groups' launches
+        for (int g = 1; g < ng; ++g) {
+            EQ_HIP(hipEventRecord(b->grp_done[g], b->gstream[g]));
+            EQ_HIP(hipStreamWaitEvent(b->stream, b->grp_done[g], 0));
+        }
+        for (int g = 0; g < ng; ++g) {
             int32_t m = 0;
             for (int p = b->groups[g].p0; p < b->groups[g].p1; ++p) m = std::max(m, b->h_state[p].iter);
             b->last_iters_g[g] = m;
@@ -4497,7 +4516,7 @@ int eq_batch_create(const eq_field_desc* descs, int32_t nprob, int32_t nspecies,
     if (S.kind == EQ_SOLVER_IR_GMG_CG && (!(S.ir_inner_rtol > 0 && S.ir_inner_rtol < 1) || S.ir_max_outer < 1))
         return set_error(EQ_ERR_INVALID, "refinement needs 0 < ir_inner_rtol < 1 and ir_max_outer >= 1");
     if (!(S.rtol > 0) || S.max_iter < 1 || S.check_every < 1) return set_error(EQ_ERR_INVALID, "bad solver tolerances");
-    if (S.streams < 1 || S.streams > 2) return set_error(EQ_ERR_INVALID, "streams must be 1 or 2");
+    if (S.streams < 1 || S.streams > eq_batch::kMaxGroups) return set_error(EQ_ERR_INVALID, "streams must be 1 to 4");
     if (S.kind != EQ_SOLVER_JACOBI_CG &&
         (S.mg_pre < 1 || S.mg_post < 1 || !(S.mg_omega > 0) || !(S.mg_omega2 >= 0) || S.mg_coarse_max_nodes < 4 ||
          S.mg_max_levels < 1))
@@ -4673,9 +4692,11 @@ void eq_batch_destroy(eq_batch* b)
         for (auto& e : b->prof[k].pool) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
         for (auto& e : b->prof[k].pending) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
     }
-    for (hipEvent_t e : b->grp_ev)
+    if (b->grp_start) hipEventDestroy(b->grp_start);
+    for (hipEvent_t e : b->grp_done)
         if (e) hipEventDestroy(e);
-    if (b->stream2) hipStreamDestroy(b->stream2);
+    for (hipStream_t gs : b->gstream)
+        if (gs) hipStreamDestroy(gs);
     if (b->stream) hipStreamDestroy(b->stream);
     delete b;
 }
@@ -5187,7 +5208,8 @@ int eq_batch_autotune(eq_batch* b, int32_t repeats, eq_solver_desc* chosen)
     };
     const eq_solver_desc orig = b->solver;
     const int32_t orig_last = b->last_iters;
-    const int32_t orig_last_g[2] = {b->last_iters_g[0], b->last_iters_g[1]};
+    int32_t orig_last_g[eq_batch::kMaxGroups];
+    std::copy(b->last_iters_g, b->last_iters_g + eq_batch::kMaxGroups, orig_last_g);
     int rc = copy_state(true);
     double best_t = 1e300;
     int best = -1, best_fuse = orig.mg_fuse_max_nodes;
@@ -5213,8 +5235,7 @@ int eq_batch_autotune(eq_batch* b, int32_t repeats, eq_solver_desc* chosen)
         bool ok = true;
         for (int r = 0; r <= reps && rc == EQ_OK; ++r) { // r = 0 captures the graphs, untimed
             b->last_iters = orig_last;
-            b->last_iters_g[0] = orig_last_g[0];
-            b->last_iters_g[1] = orig_last_g[1];
+            std::copy(orig_last_g, orig_last_g + eq_batch::kMaxGroups, b->last_iters_g);
             eq_step_stats st{};
             const auto t0 = std::chrono::steady_clock::now();
             const int src = run_solve(b, &st);
@@ -5243,8 +5264,7 @@ int eq_batch_autotune(eq_batch* b, int32_t repeats, eq_solver_desc* chosen)
         b->solver.mg_fuse_pre_top = best_top;
     }
     b->last_iters = orig_last;
-    b->last_iters_g[0] = orig_last_g[0];
-    b->last_iters_g[1] = orig_last_g[1];
+    std::copy(orig_last_g, orig_last_g + eq_batch::kMaxGroups, b->last_iters_g);
     drop_graphs(b);
     hipFree(save);
     hipFree(save_st);

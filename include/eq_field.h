@@ -146,9 +146,10 @@ typedef struct {
                                     defaults 0.5617, 1.3895 are the degree-2
                                     Chebyshev dampings on [0.5, 2]                  */
     int32_t streams;             /* CG kinds, every level fused: the batch's problems
-                                    solved as this many groups (1 or 2, default 2),
-                                    each on its own stream, each stopping at its own
-                                    iteration count (same per-problem results)     */
+                                    solved as this many groups (1 to 4, default 2;
+                                    at most one per problem), each on its own stream,
+                                    each stopping at its own iteration count (same
+                                    per-problem results)                            */
 } eq_solver_desc;
 
 typedef struct {

@@ -768,11 +768,11 @@ def test_affine_coupling_matches_oracle(eqf, oracle):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("solver", ["gmg", "mixed"])
-@pytest.mark.parametrize("traps", [1, 3])
-def test_two_problem_groups_match_one(eqf, oracle, solver, traps):
-    """The CG kinds solve a batch as two problem groups on their own streams
-    (solver option streams = 2, the default) or as one (streams = 1): every
-    problem's field bit for bit the same, and its iteration count, over
+@pytest.mark.parametrize("traps,streams", [(1, 2), (3, 2), (3, 3)])
+def test_two_problem_groups_match_one(eqf, oracle, solver, traps, streams):
+    """The CG kinds solve a batch as problem groups on their own streams
+    (solver option streams = 2, the default, or 3) or as one (streams = 1):
+    every problem's field bit for bit the same, and its iteration count, over
     three steps of species with different diffusivities (so the groups stop
     at different counts); the oracle's LU within 1e-9."""
     kind = {"gmg": eqf.GMG_CG, "mixed": eqf.MIXED_GMG_CG}[solver]
@@ -786,25 +786,25 @@ def test_two_problem_groups_match_one(eqf, oracle, solver, traps):
             probs.append(oracle.problem(nx, ny, W, H, 0.1, D, 0.01, 0.0, walls=walls))
             u0.append(rng.random((nx + 1) * (ny + 1)))
     out, its = {}, {}
-    for streams in (2, 1):
-        with eqf.FieldBatch(descs, nspecies=2, solver=eqf.solver_desc(kind, streams=streams)) as fb:
+    for ns in (streams, 1):
+        with eqf.FieldBatch(descs, nspecies=2, solver=eqf.solver_desc(kind, streams=ns)) as fb:
             for k in range(len(descs)):
                 fb.upload(k, u0[k])
             u, n = [], []
             for step in range(3):
-                print(f"problem groups {solver} traps={traps} streams={streams} step {step}", flush=True)
+                print(f"problem groups {solver} traps={traps} streams={ns} step {step}", flush=True)
                 st = fb.step()
-                assert st.nconverged == len(descs), (streams, step)
+                assert st.nconverged == len(descs), (ns, step)
                 u.append([fb.download(k) for k in range(len(descs))])
                 n.append(list(fb.problem_stats()[0]))
-            out[streams], its[streams] = u, n
+            out[ns], its[ns] = u, n
     ref = [x.copy() for x in u0]
     for step in range(3):
-        assert its[2][step] == its[1][step], (step, its[2][step], its[1][step])
+        assert its[streams][step] == its[1][step], (step, its[streams][step], its[1][step])
         for k in range(len(descs)):
-            assert np.array_equal(out[2][step][k], out[1][step][k]), (step, k)
+            assert np.array_equal(out[streams][step][k], out[1][step][k]), (step, k)
             ref[k], _ = oracle.step(probs[k], ref[k], "lu")
-            assert _rel(out[2][step][k], ref[k]) <= 1e-9, (step, k)
+            assert _rel(out[streams][step][k], ref[k]) <= 1e-9, (step, k)
 
 
 @pytest.mark.parametrize("solver", ["gmg", "mixed", "ir"])
